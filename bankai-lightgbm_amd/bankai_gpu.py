@@ -1,0 +1,382 @@
+"""ctypes binding of libbankai_gpu.so: the host-side mirror of the reference's
+tree-learner seat.
+
+`Learner` mirrors the method set `Trainer` uses on `SerialTreeLearner`
+(crates/lgbm-core/src/treelearner/serial.rs:279-2043): construction from a
+binned dataset and a config, `train(gradients, hessians)` returning a tree,
+`add_prediction_to_score`, the rows of a leaf, and `reset_constant_hessian`.
+`Dataset` mirrors `Dataset::from_dense` (crates/lgbm-core/src/io/dataset.rs:904).
+Errors surface as `BankaiError` carrying the library's last message, as the
+reference's C API reports them through LGBM_GetLastError.
+
+The library is required: importing this module does not load it, but every
+call fails loudly when libbankai_gpu.so is missing. There is no CPU fallback.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbankai_gpu.so")
+
+FORM_PLAIN_F64 = 0
+FORM_F32_COMPENSATED = 1
+FORM_QUANTIZED = 2
+
+
+class BankaiError(RuntimeError):
+    pass
+
+
+class DatasetView(C.Structure):
+    _fields_ = [
+        ("num_data", C.c_int64), ("num_groups", C.c_int32), ("num_features", C.c_int32),
+        ("num_total_features", C.c_int32),
+        ("group_bins", C.POINTER(C.POINTER(C.c_uint8))),
+        ("group_num_total_bin", C.POINTER(C.c_int32)),
+        ("group_feature_start", C.POINTER(C.c_int32)),
+        ("group_feature_cnt", C.POINTER(C.c_int32)),
+        ("group_bin_offsets", C.POINTER(C.c_int32)),
+        ("feature_num_bin", C.POINTER(C.c_int32)),
+        ("feature_missing_type", C.POINTER(C.c_int32)),
+        ("feature_default_bin", C.POINTER(C.c_int32)),
+        ("feature_most_freq_bin", C.POINTER(C.c_int32)),
+        ("feature_bin_type", C.POINTER(C.c_int32)),
+        ("feature_real_index", C.POINTER(C.c_int32)),
+        ("feature_penalty", C.POINTER(C.c_double)),
+        ("feature_bin_upper_bound", C.POINTER(C.POINTER(C.c_double))),
+        ("feature_bin_2_categorical", C.POINTER(C.POINTER(C.c_int32))),
+    ]
+
+
+class LearnerParamsC(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int32), ("max_depth", C.c_int32), ("min_data_in_leaf", C.c_int32),
+        ("min_sum_hessian_in_leaf", C.c_double), ("lambda_l1", C.c_double), ("lambda_l2", C.c_double),
+        ("max_delta_step", C.c_double), ("path_smooth", C.c_double), ("min_gain_to_split", C.c_double),
+        ("cat_smooth", C.c_double), ("cat_l2", C.c_double), ("max_cat_threshold", C.c_int32),
+        ("max_cat_to_onehot", C.c_int32), ("min_data_per_group", C.c_int32), ("extra_trees", C.c_int32),
+        ("histogram_pool_size", C.c_double), ("use_quantized_grad", C.c_int32),
+        ("num_grad_quant_bins", C.c_int32), ("stochastic_rounding", C.c_int32),
+        ("quant_train_renew_leaf", C.c_int32), ("seed", C.c_int32), ("num_threads", C.c_int32),
+        ("form", C.c_int32), ("batched", C.c_int32),
+    ]
+
+
+class TreeOutC(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int32), ("num_cat", C.c_int32),
+        ("split_feature_inner", C.POINTER(C.c_int32)), ("split_feature", C.POINTER(C.c_int32)),
+        ("threshold_in_bin", C.POINTER(C.c_uint32)), ("threshold", C.POINTER(C.c_double)),
+        ("decision_type", C.POINTER(C.c_int8)), ("left_child", C.POINTER(C.c_int32)),
+        ("right_child", C.POINTER(C.c_int32)), ("split_gain", C.POINTER(C.c_float)),
+        ("internal_value", C.POINTER(C.c_double)), ("internal_weight", C.POINTER(C.c_double)),
+        ("internal_count", C.POINTER(C.c_int32)), ("leaf_value", C.POINTER(C.c_double)),
+        ("leaf_weight", C.POINTER(C.c_double)), ("leaf_count", C.POINTER(C.c_int32)),
+        ("leaf_parent", C.POINTER(C.c_int32)), ("leaf_depth", C.POINTER(C.c_int32)),
+        ("cat_boundaries", C.POINTER(C.c_int32)), ("cat_threshold", C.POINTER(C.c_uint32)),
+        ("cat_threshold_cap", C.c_int32), ("cat_threshold_len", C.c_int32),
+    ]
+
+
+class TrainStatsC(C.Structure):
+    _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
+                ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32)]
+
+
+class DatasetParamsC(C.Structure):
+    _fields_ = [("max_bin", C.c_int32), ("min_data_in_bin", C.c_int32),
+                ("bin_construct_sample_cnt", C.c_int32), ("data_random_seed", C.c_int32),
+                ("use_missing", C.c_int32), ("zero_as_missing", C.c_int32),
+                ("feature_pre_filter", C.c_int32), ("min_data_in_leaf", C.c_int32),
+                ("num_threads", C.c_int32)]
+
+
+# Every entry point the headers declare, with its argument types.
+_P = C.c_void_p
+EXPORTS = {
+    "bk_last_error": ([], C.c_char_p),
+    "bk_learner_create": ([C.POINTER(DatasetView), C.POINTER(LearnerParamsC), C.c_int, C.POINTER(_P)], C.c_int),
+    "bk_learner_train": ([_P, _P, _P, _P, C.c_int64, C.POINTER(TreeOutC)], C.c_int),
+    "bk_learner_train_device": ([_P, _P, _P, C.POINTER(TreeOutC)], C.c_int),
+    "bk_learner_add_prediction_to_score": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
+    "bk_learner_add_prediction_to_score_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
+    "bk_learner_leaf_rows": ([_P, C.c_int32, _P, C.POINTER(C.c_int64)], C.c_int),
+    "bk_learner_reset_constant_hessian": ([_P, C.c_int], C.c_int),
+    "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
+    "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
+    "bk_learner_free": ([_P], C.c_int),
+    "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
+                              C.c_int),
+    "bk_dataset_from_dense_f32": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32,
+                                   C.POINTER(_P)], C.c_int),
+    "bk_dataset_from_dense_f32_reference": ([_P, C.c_int64, C.c_int32, _P, C.POINTER(_P)], C.c_int),
+    "bk_dataset_get_view": ([_P, C.POINTER(DatasetView)], C.c_int),
+    "bk_dataset_feature_info": ([_P, C.c_int32, C.c_char_p, C.c_int32], C.c_int),
+    "bk_dataset_free": ([_P], C.c_int),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libbankai_gpu.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BankaiError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (args, res) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BankaiError(_lib.bk_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+@dataclass
+class DatasetParams:
+    """Dataset-construction config; defaults are the reference's (config/auto.rs:244-251)."""
+    max_bin: int = 255
+    min_data_in_bin: int = 3
+    bin_construct_sample_cnt: int = 200000
+    data_random_seed: int = 1
+    use_missing: int = 1
+    zero_as_missing: int = 0
+    feature_pre_filter: int = 1
+    min_data_in_leaf: int = 20
+    num_threads: int = 0
+
+    def to_c(self):
+        return DatasetParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})
+
+
+def derive_data_random_seed(seed):
+    """Config::set derives data_random_seed from `seed` (config/mod.rs:397-405)."""
+    x = seed & 0xffffffff
+    x = (x * 214013 + 2531011) & 0xffffffff
+    return ((x >> 16) & 0x7fff) % 32767
+
+
+class Dataset:
+    """A binned dataset (Dataset::from_dense). `data` is a row-major (nrow, ncol) array."""
+
+    def __init__(self, data=None, params=None, categorical=(), _handle=None, _keep=None):
+        lib = load_library()
+        self._lib = lib
+        if _handle is not None:
+            self.handle = _handle
+            self._keep = _keep
+        else:
+            params = params or DatasetParams()
+            data = np.ascontiguousarray(data)
+            nrow, ncol = data.shape
+            cats = np.asarray(list(categorical), dtype=np.int32)
+            out = C.c_void_p()
+            p = params.to_c()
+            if data.dtype == np.float32:
+                _check(lib.bk_dataset_from_dense_f32(_ptr(data), nrow, ncol, C.byref(p), _ptr(cats), len(cats),
+                                                     C.byref(out)))
+            else:
+                data = np.ascontiguousarray(data, dtype=np.float64)
+                _check(lib.bk_dataset_from_dense(_ptr(data), nrow, ncol, C.byref(p), _ptr(cats), len(cats),
+                                                 C.byref(out)))
+            self.handle = out
+        self.view = DatasetView()
+        _check(lib.bk_dataset_get_view(self.handle, C.byref(self.view)))
+
+    def bin_like(self, data):
+        """Bins new rows with this dataset's mappers (Dataset::from_dense_reference)."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        out = C.c_void_p()
+        _check(self._lib.bk_dataset_from_dense_f32_reference(_ptr(data), data.shape[0], data.shape[1], self.handle,
+                                                             C.byref(out)))
+        return Dataset(_handle=out)
+
+    @property
+    def num_data(self):
+        return self.view.num_data
+
+    @property
+    def num_features(self):
+        return self.view.num_features
+
+    def feature_infos(self):
+        buf = C.create_string_buffer(1 << 16)
+        out = []
+        for c in range(self.view.num_total_features):
+            _check(self._lib.bk_dataset_feature_info(self.handle, c, buf, len(buf)))
+            out.append(buf.value.decode())
+        return out
+
+    def group_columns(self):
+        """The stored u8 column of every group, as numpy arrays (copies)."""
+        v = self.view
+        return [np.ctypeslib.as_array(v.group_bins[g], shape=(v.num_data,)).copy() for g in range(v.num_groups)]
+
+    def feature_meta(self):
+        v = self.view
+        nf = v.num_features
+        arr = lambda p: np.ctypeslib.as_array(p, shape=(nf,)).copy()  # noqa: E731
+        return {
+            "num_bin": arr(v.feature_num_bin), "missing_type": arr(v.feature_missing_type),
+            "default_bin": arr(v.feature_default_bin), "most_freq_bin": arr(v.feature_most_freq_bin),
+            "bin_type": arr(v.feature_bin_type), "real_index": arr(v.feature_real_index),
+        }
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self._lib is not None:
+            self._lib.bk_dataset_free(self.handle)
+            self.handle = None
+
+
+@dataclass
+class LearnerParams:
+    """Tree-learner config; defaults are the reference's (config/auto.rs:163-251)."""
+    num_leaves: int = 31
+    max_depth: int = -1
+    min_data_in_leaf: int = 20
+    min_sum_hessian_in_leaf: float = 1e-3
+    lambda_l1: float = 0.0
+    lambda_l2: float = 0.0
+    max_delta_step: float = 0.0
+    path_smooth: float = 0.0
+    min_gain_to_split: float = 0.0
+    cat_smooth: float = 10.0
+    cat_l2: float = 10.0
+    max_cat_threshold: int = 32
+    max_cat_to_onehot: int = 4
+    min_data_per_group: int = 100
+    extra_trees: int = 0
+    histogram_pool_size: float = -1.0
+    use_quantized_grad: int = 0
+    num_grad_quant_bins: int = 4
+    stochastic_rounding: int = 1
+    quant_train_renew_leaf: int = 0
+    seed: int = 0
+    num_threads: int = 1
+    form: int = FORM_PLAIN_F64
+    batched: int = 0
+
+    def to_c(self):
+        return LearnerParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})
+
+
+class TreeBuffers:
+    """Host arrays a bk_tree_out points into; `as_dict()` returns trimmed copies."""
+
+    def __init__(self, num_leaves, cat_cap=1 << 16):
+        ni, nl = max(num_leaves - 1, 1), num_leaves
+        self.a = {
+            "split_feature_inner": np.zeros(ni, np.int32), "split_feature": np.zeros(ni, np.int32),
+            "threshold_in_bin": np.zeros(ni, np.uint32), "threshold": np.zeros(ni, np.float64),
+            "decision_type": np.zeros(ni, np.int8), "left_child": np.zeros(ni, np.int32),
+            "right_child": np.zeros(ni, np.int32), "split_gain": np.zeros(ni, np.float32),
+            "internal_value": np.zeros(ni, np.float64), "internal_weight": np.zeros(ni, np.float64),
+            "internal_count": np.zeros(ni, np.int32), "leaf_value": np.zeros(nl, np.float64),
+            "leaf_weight": np.zeros(nl, np.float64), "leaf_count": np.zeros(nl, np.int32),
+            "leaf_parent": np.zeros(nl, np.int32), "leaf_depth": np.zeros(nl, np.int32),
+            "cat_boundaries": np.zeros(nl, np.int32), "cat_threshold": np.zeros(cat_cap, np.uint32),
+        }
+        ctypes_of = {np.int32: C.c_int32, np.uint32: C.c_uint32, np.float64: C.c_double, np.int8: C.c_int8,
+                     np.float32: C.c_float}
+        kw = {k: v.ctypes.data_as(C.POINTER(ctypes_of[v.dtype.type])) for k, v in self.a.items()}
+        self.c = TreeOutC(num_leaves=1, num_cat=0, cat_threshold_cap=cat_cap, cat_threshold_len=0, **kw)
+
+    def as_dict(self):
+        nl = self.c.num_leaves
+        ni = nl - 1
+        d = {"num_leaves": nl, "num_cat": self.c.num_cat}
+        for k, v in self.a.items():
+            if k in ("leaf_value", "leaf_weight", "leaf_count", "leaf_parent", "leaf_depth"):
+                d[k] = v[:nl].copy()
+            elif k == "cat_boundaries":
+                d[k] = v[:self.c.num_cat + 1].copy()
+            elif k == "cat_threshold":
+                d[k] = v[:self.c.cat_threshold_len].copy()
+            else:
+                d[k] = v[:ni].copy()
+        return d
+
+    def set_leaf_values(self, values):
+        self.a["leaf_value"][: len(values)] = values
+
+
+class Learner:
+    """The GPU tree learner behind the SerialTreeLearner seat."""
+
+    def __init__(self, dataset, params=None, is_constant_hessian=False):
+        self._lib = load_library()
+        self.params = params or LearnerParams()
+        self.dataset = dataset
+        p = self.params.to_c()
+        out = C.c_void_p()
+        _check(self._lib.bk_learner_create(C.byref(dataset.view), C.byref(p), int(is_constant_hessian),
+                                           C.byref(out)))
+        self.handle = out
+        self.tree = TreeBuffers(self.params.num_leaves)
+
+    def train(self, gradients, hessians):
+        """SerialTreeLearner::train on host arrays; returns the tree as a dict."""
+        g = np.ascontiguousarray(gradients, dtype=np.float32)
+        h = np.ascontiguousarray(hessians, dtype=np.float32)
+        if g.shape[0] != self.dataset.num_data or h.shape[0] != self.dataset.num_data:
+            raise BankaiError("gradients and hessians must have num_data entries")
+        _check(self._lib.bk_learner_train(self.handle, _ptr(g), _ptr(h), None, 0, C.byref(self.tree.c)))
+        return self.tree.as_dict()
+
+    def train_device(self, d_gradients_ptr, d_hessians_ptr):
+        """train() on device arrays given as raw pointers (e.g. tensor.data_ptr())."""
+        _check(self._lib.bk_learner_train_device(self.handle, C.c_void_p(d_gradients_ptr),
+                                                 C.c_void_p(d_hessians_ptr), C.byref(self.tree.c)))
+        return self.tree.as_dict()
+
+    def add_prediction_to_score(self, leaf_values, score):
+        """score[row] += leaf_values[leaf(row)] for the last tree (host float64 array, in place)."""
+        self.tree.set_leaf_values(leaf_values)
+        _check(self._lib.bk_learner_add_prediction_to_score(self.handle, C.byref(self.tree.c), _ptr(score)))
+
+    def add_prediction_to_score_device(self, leaf_values, d_score_ptr):
+        self.tree.set_leaf_values(leaf_values)
+        _check(self._lib.bk_learner_add_prediction_to_score_device(self.handle, C.byref(self.tree.c),
+                                                                   C.c_void_p(d_score_ptr)))
+
+    def binary_gradients_device(self, d_score_ptr, d_label_ptr, sigmoid, d_g_ptr, d_h_ptr):
+        _check(self._lib.bk_learner_binary_gradients_device(self.handle, C.c_void_p(d_score_ptr),
+                                                            C.c_void_p(d_label_ptr), float(sigmoid),
+                                                            C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
+
+    def leaf_rows(self, leaf):
+        out = np.zeros(self.dataset.num_data, np.int32)
+        n = C.c_int64()
+        _check(self._lib.bk_learner_leaf_rows(self.handle, leaf, _ptr(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def reset_constant_hessian(self, value):
+        _check(self._lib.bk_learner_reset_constant_hessian(self.handle, int(value)))
+
+    def last_stats(self):
+        s = TrainStatsC()
+        _check(self._lib.bk_learner_last_stats(self.handle, C.byref(s)))
+        return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
+                "speculative_waste": s.speculative_waste}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.bk_learner_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,333 @@
+"""Boosting loop around a tree learner, and the model text writer.
+
+`GBDT` restates the reference's `Trainer::train_one_iter_with` for the plain
+GBDT case (crates/lgbm-core/src/boosting/train.rs:723-839): boost-from-average
+init scores (615-642), objective gradients, one tree per class, shrinkage,
+score update and `add_bias` on the first iteration. The learner is any object
+with the `train(g, h)` / `add_prediction_to_score(values, score)` methods: the
+GPU `bankai_gpu.Learner` in production.
+
+Objectives are the closed-form host ones of crates/lgbm-core/src/objective/
+(binary.rs:71-124, regression.rs:127-142,240-330 for L2, multiclass.rs:68-105).
+
+`model_to_string` writes the model text of crates/lgbm-core/src/boosting/model_text.rs:88-138
+and crates/lgbm-core/src/tree.rs:520-601 (tree blocks; no feature importances or
+parameter section).
+"""
+import numpy as np
+
+K_EPSILON = float(np.float32(1e-15))
+K_ZERO_THRESHOLD = float(np.float32(1e-35))
+
+
+def _round0(v):
+    v = np.asarray(v, dtype=np.float64)
+    return np.where((v >= -K_ZERO_THRESHOLD) & (v <= K_ZERO_THRESHOLD), 0.0, v)
+
+
+class Tree:
+    """Tree::apply_shrinkage / add_bias / to_model_string over a trained tree dict."""
+
+    def __init__(self, d):
+        self.d = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+        self.shrinkage = 1.0
+
+    @property
+    def num_leaves(self):
+        return self.d["num_leaves"]
+
+    @property
+    def leaf_value(self):
+        return self.d["leaf_value"]
+
+    def apply_shrinkage(self, rate):
+        """tree.rs:437-462"""
+        self.d["leaf_value"] = _round0(self.d["leaf_value"] * rate)
+        self.d["internal_value"] = _round0(self.d["internal_value"] * rate)
+        self.shrinkage *= rate
+
+    def add_bias(self, val):
+        """tree.rs:465-478"""
+        self.d["leaf_value"] = _round0(self.d["leaf_value"] + val)
+        self.d["internal_value"] = _round0(self.d["internal_value"] + val)
+        self.shrinkage = 1.0
+
+    @staticmethod
+    def constant(value, count):
+        d = {"num_leaves": 1, "num_cat": 0, "leaf_value": np.array([value]), "leaf_weight": np.array([0.0]),
+             "leaf_count": np.array([count], np.int32), "internal_value": np.zeros(0), "internal_weight": np.zeros(0),
+             "internal_count": np.zeros(0, np.int32), "split_feature": np.zeros(0, np.int32),
+             "split_gain": np.zeros(0, np.float32), "threshold": np.zeros(0), "decision_type": np.zeros(0, np.int8),
+             "left_child": np.zeros(0, np.int32), "right_child": np.zeros(0, np.int32),
+             "cat_boundaries": np.zeros(1, np.int32), "cat_threshold": np.zeros(0, np.uint32)}
+        return Tree(d)
+
+    def to_model_string(self):
+        d = self.d
+        n = d["num_leaves"]
+        ni = n - 1
+        g6 = lambda v: "%g" % v  # noqa: E731
+        g17 = lambda v: "%.17g" % v  # noqa: E731
+        join = lambda a, f=str: " ".join(f(x) for x in a)  # noqa: E731
+        lines = [f"num_leaves={n}", f"num_cat={d['num_cat']}",
+                 "split_feature=" + join(d["split_feature"][:ni], lambda x: str(int(x))),
+                 "split_gain=" + join(d["split_gain"][:ni], lambda x: g6(float(x))),
+                 "threshold=" + join(d["threshold"][:ni], lambda x: g17(float(x))),
+                 "decision_type=" + join(d["decision_type"][:ni], lambda x: str(int(x))),
+                 "left_child=" + join(d["left_child"][:ni], lambda x: str(int(x))),
+                 "right_child=" + join(d["right_child"][:ni], lambda x: str(int(x))),
+                 "leaf_value=" + join(d["leaf_value"][:n], lambda x: g17(float(x))),
+                 "leaf_weight=" + join(d["leaf_weight"][:n], lambda x: g17(float(x))),
+                 "leaf_count=" + join(d["leaf_count"][:n], lambda x: str(int(x))),
+                 "internal_value=" + join(d["internal_value"][:ni], lambda x: g6(float(x))),
+                 "internal_weight=" + join(d["internal_weight"][:ni], lambda x: g6(float(x))),
+                 "internal_count=" + join(d["internal_count"][:ni], lambda x: str(int(x)))]
+        if d["num_cat"] > 0:
+            lines.append("cat_boundaries=" + join(d["cat_boundaries"][: d["num_cat"] + 1], lambda x: str(int(x))))
+            lines.append("cat_threshold=" + join(d["cat_threshold"], lambda x: str(int(x))))
+        lines.append("is_linear=0")
+        lines.append("shrinkage=" + g6(self.shrinkage))
+        return "\n".join(lines) + "\n\n"
+
+
+# ---------------------------------------------------------------------------
+# Objectives
+# ---------------------------------------------------------------------------
+class BinaryObjective:
+    """objective/binary.rs"""
+
+    def __init__(self, label, weights=None, sigmoid=1.0, scale_pos_weight=1.0, is_unbalance=False):
+        self.sigmoid = float(sigmoid)
+        self.is_pos = np.asarray(label) > 0
+        self.weights = None if weights is None else np.asarray(weights, np.float32)
+        cnt_pos = int(self.is_pos.sum())
+        cnt_neg = len(self.is_pos) - cnt_pos
+        self.need_train = cnt_pos != 0 and cnt_neg != 0
+        lw = [1.0, 1.0]
+        if is_unbalance and cnt_pos > 0 and cnt_neg > 0:
+            if cnt_pos > cnt_neg:
+                lw[0] = cnt_pos / cnt_neg
+            else:
+                lw[1] = cnt_neg / cnt_pos
+        lw[1] *= scale_pos_weight
+        self.label_weights = np.array(lw)
+        self.num_class = 1
+        self.const_hess = False
+
+    def name(self):
+        return "binary sigmoid:%g" % self.sigmoid
+
+    def gradients(self, score):
+        s = self.sigmoid
+        lab = np.where(self.is_pos, 1.0, -1.0)
+        lw = self.label_weights[self.is_pos.astype(np.int64)]
+        response = -lab * s / (1.0 + np.exp(lab * s * score))
+        ar = np.abs(response)
+        if self.weights is None:
+            return (response * lw).astype(np.float32), (ar * (s - ar) * lw).astype(np.float32)
+        w = self.weights.astype(np.float64)
+        return (response * lw * w).astype(np.float32), (ar * (s - ar) * lw * w).astype(np.float32)
+
+    def boost_from_score(self, k=0):
+        if self.weights is None:
+            suml = float(np.sum(self.is_pos.astype(np.float64)))
+            sumw = float(len(self.is_pos))
+        else:
+            suml = 0.0
+            sumw = 0.0
+            for p, w in zip(self.is_pos, self.weights):
+                suml += float(np.float32(float(p)) * w)
+                sumw += float(w)
+        pavg = max(min(suml / sumw, 1.0 - K_EPSILON), K_EPSILON)
+        return float(np.log(pavg / (1.0 - pavg)) / self.sigmoid)
+
+
+class RegressionL2Objective:
+    """objective/regression.rs (L2)"""
+
+    def __init__(self, label, weights=None):
+        self.label = np.asarray(label, np.float32)
+        self.weights = None if weights is None else np.asarray(weights, np.float32)
+        self.num_class = 1
+        self.const_hess = self.weights is None
+
+    def name(self):
+        return "regression"
+
+    def gradients(self, score):
+        if self.weights is None:
+            return (score - self.label.astype(np.float64)).astype(np.float32), np.ones(len(self.label), np.float32)
+        g = (score - self.label.astype(np.float64)).astype(np.float32) * self.weights
+        return g.astype(np.float32), self.weights.copy()
+
+    def boost_from_score(self, k=0):
+        if self.weights is None:
+            s = 0.0
+            for v in self.label:
+                s += float(v)
+            return s / len(self.label)
+        suml = sumw = 0.0
+        for v, w in zip(self.label, self.weights):
+            suml = float(v) * float(w) + suml
+            sumw += float(w)
+        return suml / sumw
+
+
+class MulticlassObjective:
+    """objective/multiclass.rs (softmax)"""
+
+    def __init__(self, label, num_class, weights=None):
+        self.label = np.asarray(label).astype(np.int64)
+        self.num_class = int(num_class)
+        self.weights = None if weights is None else np.asarray(weights, np.float32)
+        self.factor = self.num_class / (self.num_class - 1.0)
+        n = len(self.label)
+        if self.weights is None:
+            cnt = np.bincount(self.label, minlength=self.num_class).astype(np.float64)
+            self.init_probs = cnt / n
+        else:
+            probs = np.zeros(self.num_class)
+            for lab, w in zip(self.label, self.weights):
+                probs[lab] += float(w)
+            self.init_probs = probs / float(np.sum(self.weights.astype(np.float64)))
+        self.const_hess = False
+
+    def name(self):
+        return "multiclass num_class:%d" % self.num_class
+
+    def gradients(self, score):
+        n = len(self.label)
+        s = score.reshape(self.num_class, n).T
+        m = s.max(axis=1, keepdims=True)
+        e = np.exp(s - m)
+        p = e / e.sum(axis=1, keepdims=True)
+        onehot = np.zeros_like(p)
+        onehot[np.arange(n), self.label] = 1.0
+        g = p - onehot
+        h = self.factor * p * (1.0 - p)
+        if self.weights is not None:
+            w = self.weights.astype(np.float64)[:, None]
+            g = g * w
+            h = h * w
+        return g.T.reshape(-1).astype(np.float32), h.T.reshape(-1).astype(np.float32)
+
+    def boost_from_score(self, k):
+        return float(np.log(max(K_EPSILON, self.init_probs[k])))
+
+    def class_need_train(self, k):
+        p = abs(self.init_probs[k])
+        return not (p <= K_EPSILON or p >= 1.0 - K_EPSILON)
+
+
+# ---------------------------------------------------------------------------
+# The boosting loop
+# ---------------------------------------------------------------------------
+class GBDT:
+    """Trainer::train_one_iter_with (train.rs:723-839) for plain GBDT."""
+
+    def __init__(self, learners, objective, num_data, learning_rate=0.1, boost_from_average=True):
+        # one learner per class tree of an iteration may share state: the reference
+        # uses a single learner; `learners` is a list of length 1 or num_class
+        self.learners = learners
+        self.obj = objective
+        self.n = num_data
+        self.lr = learning_rate
+        self.boost_from_average = boost_from_average
+        self.ntpi = objective.num_class
+        self.score = np.zeros(self.n * self.ntpi)
+        self.models = []
+        self.iter = 0
+
+    def _learner(self, k):
+        return self.learners[k if len(self.learners) > 1 else 0]
+
+    def _automatic_init(self, k):
+        if not self.models and self.boost_from_average:
+            init = self.obj.boost_from_score(k)
+            if abs(init) > K_EPSILON:
+                return init
+        return 0.0
+
+    def train_one_iter(self):
+        n = self.n
+        inits = []
+        for k in range(self.ntpi):
+            init = self._automatic_init(k)
+            if init != 0.0:
+                self.score[k * n:(k + 1) * n] += init
+            inits.append(init)
+        g, h = self.obj.gradients(self.score)
+        should_continue = False
+        for k in range(self.ntpi):
+            off = k * n
+            need = getattr(self.obj, "class_need_train", lambda _k: True)(k)
+            if need:
+                d = self._learner(k).train(g[off:off + n], h[off:off + n])
+                tree = Tree(d)
+            else:
+                tree = Tree.constant(0.0, n)
+            if tree.num_leaves > 1:
+                should_continue = True
+                tree.apply_shrinkage(self.lr)
+                view = self.score[off:off + n]
+                self._learner(k).add_prediction_to_score(tree.leaf_value, view)
+                self.score[off:off + n] = view
+                if abs(inits[k]) > K_EPSILON:
+                    tree.add_bias(inits[k])
+            elif len(self.models) < self.ntpi:
+                if not self.boost_from_average:
+                    inits[k] = self.obj.boost_from_score(k)
+                    self.score[off:off + n] += inits[k]
+                tree = Tree.constant(inits[k], n)
+            else:
+                tree = Tree.constant(0.0, n)
+            self.models.append(tree)
+        self.iter += 1
+        if not should_continue:
+            if len(self.models) > self.ntpi:
+                del self.models[-self.ntpi:]
+            return True
+        return False
+
+    def train(self, num_iterations):
+        for _ in range(num_iterations):
+            if self.train_one_iter():
+                break
+        return self
+
+
+def model_to_string(models, num_class, ntpi, objective_name, feature_names, feature_infos):
+    """Gbdt::save_model_to_string (model_text.rs:88-138), trees only."""
+    head = ["tree", "version=v4", f"num_class={num_class}", f"num_tree_per_iteration={ntpi}", "label_index=0",
+            f"max_feature_idx={len(feature_names) - 1}", f"objective={objective_name}",
+            "feature_names=" + " ".join(feature_names), "feature_infos=" + " ".join(feature_infos)]
+    blocks = [f"Tree={i}\n{t.to_model_string()}" for i, t in enumerate(models)]
+    head.append("tree_sizes=" + " ".join(str(len(b.encode())) for b in blocks))
+    return "\n".join(head) + "\n\n" + "".join(blocks) + "end of trees\n"
+
+
+def parse_model_trees(text):
+    """Tree blocks of a model text file as dicts of strings (for comparisons)."""
+    trees = []
+    cur = None
+    for line in text.splitlines():
+        if line.startswith("Tree="):
+            cur = {}
+            trees.append(cur)
+        elif line.startswith("end of trees"):
+            break
+        elif cur is not None and "=" in line:
+            k, v = line.split("=", 1)
+            cur[k] = v
+    return trees
+
+
+def parse_model_header(text):
+    out = {}
+    for line in text.splitlines():
+        if line.startswith("Tree="):
+            break
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out[k] = v
+    return out

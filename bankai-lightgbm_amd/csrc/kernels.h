@@ -1,0 +1,208 @@
+// kernels.h — data shared between the host learner (learner.cpp, built by g++)
+// and the gfx950 kernels (kernels.hip, built by hipcc): per-feature metadata, the
+// split routing rule, the per-feature search result, and the launchers.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define BK_HD __host__ __device__
+#else
+#define BK_HD
+#endif
+
+namespace bk {
+
+enum Form { FORM_F64 = 0, FORM_F32C = 1, FORM_QUANT = 2 };
+
+constexpr int kMaxBins = 256;         // group bins per column (u8 storage)
+constexpr int kStreamThreads = 512;   // threads of the row-streaming kernels
+constexpr int kRowsPerThread = 4;     // rows per thread per tile (one dword of u8 bins)
+constexpr int kTileRows = kStreamThreads * kRowsPerThread;
+constexpr int kMaxRowBlocks = 512;    // row blocks per histogram build (partials)
+constexpr int kLdsBytes = 128 * 1024; // LDS budget per workgroup on gfx950
+
+// Per inner feature, as the kernels read it.
+struct FeatDev {
+  int num_bin, missing_type, default_bin, mfb, offset, bin_type;
+  int group;       // group index (= column)
+  int hist_off;    // first bin of the feature slice in a leaf histogram
+  int hist_len;    // num_bin - offset
+  int bin_base;    // bin_offsets[sub] inside the group
+  int minb, maxb, use_min_bin;  // group_view (data_partition.rs:25-64)
+  int real;
+  double penalty;
+};
+
+// SplitRule (data_partition.rs:233-410), dense u8 columns.
+struct RuleDev {
+  int is_cat, use_min_bin, multi_bin;
+  uint32_t minb, maxb, th, t_zero_bin, cat_offset;
+  int miss_is_zero, miss_is_na, mfb_is_zero, mfb_is_na;
+  int default_left_route, missing_default_left, max_bin_left;
+  uint32_t cat_bits[8];
+};
+
+BK_HD inline bool rule_bit(const RuleDev& r, uint32_t pos) {
+  uint32_t i1 = pos / 32u;
+  if (i1 >= 8u) return false;
+  return (r.cat_bits[i1] >> (pos % 32u)) & 1u;
+}
+
+// route_bin (data_partition.rs:366-410) for one stored group value.
+BK_HD inline bool route_left(const RuleDev& r, uint32_t bin) {
+  if (r.is_cat) {
+    if ((r.use_min_bin && (bin < r.minb || bin > r.maxb)) || (!r.use_min_bin && bin == 0u)) return r.default_left_route;
+    return rule_bit(r, bin - r.minb + r.cat_offset);
+  }
+  if (r.multi_bin) {
+    if ((r.miss_is_zero && !r.mfb_is_zero && bin == r.t_zero_bin) || (r.miss_is_na && !r.mfb_is_na && bin == r.maxb))
+      return r.missing_default_left;
+    if ((r.use_min_bin && (bin < r.minb || bin > r.maxb)) || (!r.use_min_bin && bin == 0u)) {
+      if ((r.miss_is_na && r.mfb_is_na) || (r.miss_is_zero && r.mfb_is_zero)) return r.missing_default_left;
+      return r.default_left_route;
+    }
+    return !(bin > r.th);
+  }
+  if (r.miss_is_zero && !r.mfb_is_zero && bin == r.t_zero_bin) return r.missing_default_left;
+  if (bin != r.maxb) {
+    if ((r.miss_is_na && r.mfb_is_na) || (r.miss_is_zero && r.mfb_is_zero)) return r.missing_default_left;
+    return r.default_left_route;
+  }
+  if (r.miss_is_na && !r.mfb_is_na) return r.missing_default_left;
+  return r.max_bin_left;
+}
+
+// SplitRule::numerical (data_partition.rs:299-363).
+BK_HD inline RuleDev make_rule_numerical(const FeatDev& f, uint32_t threshold, bool default_left) {
+  RuleDev r{};
+  r.is_cat = 0;
+  r.use_min_bin = f.use_min_bin;
+  r.minb = (uint32_t)f.minb;
+  r.maxb = (uint32_t)f.maxb;
+  uint32_t default_bin = (uint32_t)f.default_bin, mfb = (uint32_t)f.mfb;
+  if (f.missing_type == 1) {
+    r.miss_is_zero = 1;
+    r.mfb_is_zero = default_bin == mfb;
+  } else if (f.missing_type == 2) {
+    r.miss_is_na = 1;
+    r.mfb_is_na = f.use_min_bin ? (r.maxb == mfb + r.minb && mfb > 0u) : (r.maxb == mfb + 1u && mfb > 0u);
+  }
+  uint32_t th = threshold + r.minb;
+  uint32_t tz = r.minb + default_bin;
+  if (mfb == 0u) {
+    th -= 1u;
+    tz -= 1u;
+  }
+  r.th = th;
+  r.t_zero_bin = tz;
+  r.default_left_route = mfb <= threshold;
+  r.missing_default_left = (r.miss_is_zero || r.miss_is_na) && default_left;
+  r.max_bin_left = r.maxb <= th;
+  r.multi_bin = r.minb < r.maxb;
+  return r;
+}
+
+// SplitRule::categorical (data_partition.rs:266-296); bits = bitset of left bins.
+BK_HD inline RuleDev make_rule_categorical(const FeatDev& f, const uint32_t* bits) {
+  RuleDev r{};
+  r.is_cat = 1;
+  r.use_min_bin = f.use_min_bin;
+  r.minb = (uint32_t)f.minb;
+  r.maxb = (uint32_t)f.maxb;
+  r.cat_offset = f.mfb == 0 ? 1u : 0u;
+  for (int i = 0; i < 8; ++i) r.cat_bits[i] = bits[i];
+  r.default_left_route = f.mfb > 0 && rule_bit(r, (uint32_t)f.mfb);
+  return r;
+}
+
+// Stored group value of real bin b of feature f (fill_dense, dataset.rs:676-684);
+// the most frequent bin is not stored and reads as 0.
+BK_HD inline uint32_t stored_value(const FeatDev& f, int b) {
+  if (b == f.mfb) return 0u;
+  return (uint32_t)(b + f.bin_base - (f.mfb == 0 ? 1 : 0));
+}
+
+// One feature's best split for one leaf (SplitInfo, split_info.rs:9-31) plus the
+// exact row counts the split routes left and right (from the count channel).
+struct SplitOut {
+  double gain, left_output, right_output;
+  double left_sum_gradient, left_sum_hessian, right_sum_gradient, right_sum_hessian;
+  long long left_int_sum, right_int_sum;
+  int threshold, left_count, right_count, exact_left, exact_right;
+  int default_left, splittable, num_cat, feature, pad;
+  uint32_t cat_bits[8];
+};
+
+// Scalars of one leaf handed to the search.
+struct LeafArgs {
+  int leaf, slot, num_data;     // num_data = exact rows of the leaf
+  double sum_g, sum_h, parent_output;
+  long long int_sum;
+  int from_partials;            // 1: reduce the partial buffer; 0: subtract (larger leaf)
+  int part_set;                 // which partial buffer (0/1)
+  int nblk;                     // row blocks that wrote partials
+};
+
+struct ScanParams {
+  double l1, l2, max_delta_step, path_smooth, min_sum_hessian, min_gain_to_split;
+  double cat_smooth, cat_l2;
+  int min_data_in_leaf, max_cat_threshold, max_cat_to_onehot, min_data_per_group;
+  double grad_scale, hess_scale, h0;  // quantized scales; h0 = hessians[0] for a constant hessian
+  int const_hess;
+};
+
+// Device buffers of the learner that kernels read.
+struct DevState {
+  int num_data, npad, num_groups, num_features, total_bins, form;
+  const uint8_t* cols[3];        // 0 = original order, 1/2 = partition buffers (group g at g*npad)
+  int32_t* idx[3];               // idx[0] unused (identity)
+  void* gh[3];                   // row payload: int16 packed (quantized) or float2 (g, h)
+  const FeatDev* feats;
+  const int* group_bin_start;    // first histogram bin of each group
+  double* pool;                  // slots x total_bins x 2 (double g,h or int64 g,h)
+  int* pool_cnt;                 // slots x total_bins
+  unsigned long long* part_q[2]; // quantized partials: kMaxRowBlocks x total_bins
+  double* part_g[2];
+  double* part_h[2];
+  unsigned* part_c[2];
+  int* blk_cnt;                  // per row block left counts of a partition
+  int* err_flag;
+};
+
+// Segment of rows [begin, begin+count) in buffer buf.
+struct Segment {
+  int buf, begin, count;
+};
+
+// Row-block decomposition of one histogram build or partition.
+struct BlockPlan {
+  int nblk, chunk;  // chunk: rows per block, multiple of 4
+};
+
+// Launchers (kernels.hip). All run on `stream` (a hipStream_t passed as void*).
+void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int* d_group_used, int part_set,
+                 int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
+                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
+                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
+                 const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream);
+void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
+void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
+                       double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
+                       long long* d_sums3, void* stream);
+void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream);
+void launch_renew_sums(const DevState& s, const Segment* d_segs, int num_leaves, const float* g, const float* h,
+                       double* d_sums, void* stream);
+void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
+                      double* score, int max_seg, void* stream);
+void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream);
+int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream);
+void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
+                    void* stream);
+void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
+                             void* stream);
+
+}  // namespace bk

@@ -1,0 +1,1508 @@
+// kernels.hip — gfx950 kernels of the histogram tree learner.
+//
+// Data layout in HBM (one learner):
+//   cols[0]        original column-major u8 bins, group g at g*npad (read only)
+//   cols[1..2]     ping-pong copies in partition order: every leaf's rows are a
+//                  contiguous segment of one buffer, so histogram and partition
+//                  passes stream whole cache lines (no per-row gathers)
+//   idx[1..2]      original row id of each partition position (buffer 0: identity)
+//   gh[0..2]       row payload in the same order: int16 packed int8 pair
+//                  (quantized) or float2 (gradient, hessian)
+//   pool           one histogram per leaf slot: (g, h) per bin, plus an exact
+//                  per-bin row count (pool_cnt) used to size partitions in advance
+//
+// All f64 arithmetic that must match the reference bit for bit is written
+// without contraction (the build passes -ffp-contract=off), matching the
+// reference's unfused madd on x86_64 (crates/lgbm-core/src/utils/common.rs:207-216).
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.h"
+
+#define BK_HIP(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " in " #x); \
+  } while (0)
+
+namespace bk {
+
+namespace {
+
+constexpr double kEps = (double)1e-15f;
+constexpr double kMinScore = -__builtin_huge_val();
+
+// ---------------------------------------------------------------------------
+// Packed quantized LDS accumulator: one 64-bit add per (row, group).
+//   bits 43..63: gradient sum (signed), 22..42: hessian sum, 0..21: row count.
+// Exact while a flush covers fewer than 2^20 / max|q| rows (host enforces).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long qpack(int16_t p) {
+  long long g = (long long)(int8_t)(p >> 8);
+  unsigned long long h = (unsigned long long)(uint8_t)(p & 0xff);
+  return ((unsigned long long)g << 43) + (h << 22) + 1ull;
+}
+__device__ __forceinline__ void qunpack(unsigned long long s, long long* g, long long* h, long long* c) {
+  *c = (long long)(s & ((1ull << 22) - 1ull));
+  *h = (long long)((s >> 22) & ((1ull << 21) - 1ull));
+  *g = ((long long)s) >> 43;
+}
+
+__device__ __forceinline__ int sat_i32_dev(double x) {
+  if (x != x) return 0;
+  if (x >= 2147483647.0) return 2147483647;
+  if (x <= -2147483648.0) return (int)(-2147483647 - 1);
+  return (int)x;
+}
+__device__ __forceinline__ int8_t sat_i8_dev(double x) {
+  if (x != x) return 0;
+  if (x >= 127.0) return 127;
+  if (x <= -128.0) return -128;
+  return (int8_t)(int)x;
+}
+
+// Block-wide exclusive scan of two ints over kStreamThreads threads.
+__device__ __forceinline__ void block_scan2(int a, int b, int* pa, int* pb, int* ta, int* tb, int* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int xa = a, xb = b;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int ya = __shfl_up(xa, o, 64), yb = __shfl_up(xb, o, 64);
+    if (lane >= o) {
+      xa += ya;
+      xb += yb;
+    }
+  }
+  if (lane == 63) {
+    scratch[wave] = xa;
+    scratch[16 + wave] = xb;
+  }
+  __syncthreads();
+  int oa = 0, ob = 0, sa = 0, sb = 0;
+  const int nw = kStreamThreads / 64;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wave) {
+      oa += scratch[w];
+      ob += scratch[16 + w];
+    }
+    sa += scratch[w];
+    sb += scratch[16 + w];
+  }
+  *pa = oa + xa - a;
+  *pb = ob + xb - b;
+  *ta = sa;
+  *tb = sb;
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Histogram accumulators in LDS for one group tile.
+// ---------------------------------------------------------------------------
+template <int FORM>
+struct LdsHist;
+
+template <>
+struct LdsHist<FORM_QUANT> {
+  unsigned long long* q;
+  static constexpr int kBytesPerBin = 8;
+  __device__ void bind(unsigned char* smem, int nb) { q = (unsigned long long*)smem; }
+  __device__ void zero(int nb) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) q[i] = 0ull;
+  }
+  __device__ __forceinline__ void add(int b, unsigned long long v) { atomicAdd(&q[b], v); }
+  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
+    unsigned long long* dst = s.part_q[set] + (size_t)blk * s.total_bins + bin_lo;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[i] = first ? q[i] : dst[i] + q[i];
+  }
+};
+
+template <>
+struct LdsHist<FORM_F64> {
+  double* g;
+  double* h;
+  unsigned* c;
+  static constexpr int kBytesPerBin = 20;
+  __device__ void bind(unsigned char* smem, int nb) {
+    g = (double*)smem;
+    h = g + nb;
+    c = (unsigned*)(h + nb);
+  }
+  __device__ void zero(int nb) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      g[i] = 0.0;
+      h[i] = 0.0;
+      c[i] = 0u;
+    }
+  }
+  __device__ __forceinline__ void add(int b, float gg, float hh) {
+    atomicAdd(&g[b], (double)gg);
+    atomicAdd(&h[b], (double)hh);
+    atomicAdd(&c[b], 1u);
+  }
+  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
+    size_t o = (size_t)blk * s.total_bins + bin_lo;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      s.part_g[set][o + i] = first ? g[i] : s.part_g[set][o + i] + g[i];
+      s.part_h[set][o + i] = first ? h[i] : s.part_h[set][o + i] + h[i];
+      s.part_c[set][o + i] = first ? c[i] : s.part_c[set][o + i] + c[i];
+    }
+  }
+};
+
+// f32 sums with a two-sum error term per bin: each atomic returns the old value,
+// so the rounding error of that exact addition is recovered and accumulated.
+template <>
+struct LdsHist<FORM_F32C> {
+  float* g;
+  float* h;
+  float* eg;
+  float* eh;
+  unsigned* c;
+  static constexpr int kBytesPerBin = 20;
+  __device__ void bind(unsigned char* smem, int nb) {
+    g = (float*)smem;
+    h = g + nb;
+    eg = h + nb;
+    eh = eg + nb;
+    c = (unsigned*)(eh + nb);
+  }
+  __device__ void zero(int nb) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      g[i] = 0.0f;
+      h[i] = 0.0f;
+      eg[i] = 0.0f;
+      eh[i] = 0.0f;
+      c[i] = 0u;
+    }
+  }
+  __device__ __forceinline__ static float two_sum_err(float a, float b) {
+    float s = a + b;
+    float bb = s - a;
+    return (a - (s - bb)) + (b - bb);
+  }
+  __device__ __forceinline__ void add(int b, float gg, float hh) {
+    float og = atomicAdd(&g[b], gg);
+    float oh = atomicAdd(&h[b], hh);
+    float e1 = two_sum_err(og, gg), e2 = two_sum_err(oh, hh);
+    if (e1 != 0.0f) atomicAdd(&eg[b], e1);
+    if (e2 != 0.0f) atomicAdd(&eh[b], e2);
+    atomicAdd(&c[b], 1u);
+  }
+  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
+    size_t o = (size_t)blk * s.total_bins + bin_lo;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      double vg = (double)g[i] + (double)eg[i];
+      double vh = (double)h[i] + (double)eh[i];
+      s.part_g[set][o + i] = first ? vg : s.part_g[set][o + i] + vg;
+      s.part_h[set][o + i] = first ? vh : s.part_h[set][o + i] + vh;
+      s.part_c[set][o + i] = first ? c[i] : s.part_c[set][o + i] + c[i];
+    }
+  }
+};
+
+// Adds row k (0..3) of a 4-row dword group to the accumulator.
+template <int FORM>
+struct RowPayload;
+template <>
+struct RowPayload<FORM_QUANT> {
+  unsigned long long v[4];
+  __device__ __forceinline__ void load(const void* gh, long long d) {
+    uint64_t w = ((const uint64_t*)gh)[d];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = qpack((int16_t)(w >> (16 * k)));
+  }
+  __device__ __forceinline__ void add(LdsHist<FORM_QUANT>& L, int k, int b) const { L.add(b, v[k]); }
+};
+template <int FORM>
+struct RowPayload {
+  float g[4], h[4];
+  __device__ __forceinline__ void load(const void* gh, long long d) {
+    const float4* p = (const float4*)gh;
+    float4 a = p[2 * d], b = p[2 * d + 1];
+    g[0] = a.x;
+    h[0] = a.y;
+    g[1] = a.z;
+    h[1] = a.w;
+    g[2] = b.x;
+    h[2] = b.y;
+    g[3] = b.z;
+    h[3] = b.w;
+  }
+  __device__ __forceinline__ void add(LdsHist<FORM>& L, int k, int b) const { L.add(b, g[k], h[k]); }
+};
+
+// ---------------------------------------------------------------------------
+// Histogram of one segment (root build, or a leaf whose sibling cannot be
+// subtracted). Grid: (row blocks, group tiles).
+// ---------------------------------------------------------------------------
+template <int FORM>
+__global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, Segment seg, BlockPlan bp,
+                                                         const int* __restrict__ group_used, int set,
+                                                         const int* __restrict__ tile_first, int rows_per_flush) {
+  extern __shared__ unsigned char smem[];
+  const int blk = blockIdx.x, tile = blockIdx.y;
+  const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
+  const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
+  LdsHist<FORM> L;
+  L.bind(smem, nb);
+  const long long seg_end = (long long)seg.begin + seg.count;
+  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
+  const long long ce = min(cb + (long long)bp.chunk, seg_end);
+  const uint8_t* cols = s.cols[seg.buf];
+  const void* gh = s.gh[seg.buf];
+  bool first = true;
+  for (long long fs = cb; fs < ce || first; fs += rows_per_flush) {
+    const long long fe = min(fs + (long long)rows_per_flush, ce);
+    L.zero(nb);
+    __syncthreads();
+    const long long d0 = fs >> 2, d1 = (fe + 3) >> 2;
+    for (long long d = d0 + threadIdx.x; d < d1; d += blockDim.x) {
+      const long long r0 = d << 2;
+      int valid = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) valid |= (r0 + k >= fs && r0 + k < fe) ? (1 << k) : 0;
+      RowPayload<FORM> P;
+      P.load(gh, d);
+      for (int g = g0; g < g1; ++g) {
+        if (!group_used[g]) continue;
+        const uint32_t w = ((const uint32_t*)(cols + (size_t)g * s.npad))[d];
+        const int base = s.group_bin_start[g] - bin_lo;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (valid & (1 << k)) P.add(L, k, base + (int)((w >> (8 * k)) & 0xffu));
+      }
+    }
+    __syncthreads();
+    L.flush(s, set, blk, bin_lo, nb, first);
+    __syncthreads();
+    first = false;
+    if (fe >= ce) break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable partition of one segment, pass 1: left rows per row block.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, Segment seg, BlockPlan bp, int split_group,
+                                                               RuleDev rule) {
+  __shared__ int red[kStreamThreads / 64];
+  const int blk = blockIdx.x;
+  const long long seg_end = (long long)seg.begin + seg.count;
+  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
+  const long long ce = min(cb + (long long)bp.chunk, seg_end);
+  const uint32_t* col = (const uint32_t*)(s.cols[seg.buf] + (size_t)split_group * s.npad);
+  int cnt = 0;
+  for (long long d = (cb >> 2) + threadIdx.x; d < ((ce + 3) >> 2); d += blockDim.x) {
+    const uint32_t w = col[d];
+    const long long r0 = d << 2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (r0 + k >= cb && r0 + k < ce && route_left(rule, (w >> (8 * k)) & 0xffu)) ++cnt;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kStreamThreads / 64; ++w) t += red[w];
+    s.blk_cnt[blk] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable partition, pass 2: scatter every column, the row ids and the payload
+// into the destination buffer, and accumulate the histogram of the child on
+// `hist_side` (0 left, 1 right, -1 none) from the rows already in registers.
+// Grid: (row blocks, group tiles); tile 0 also moves row ids and payload.
+// ---------------------------------------------------------------------------
+template <int FORM>
+__global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Segment seg, int dst_buf, int split_group,
+                                                                 RuleDev rule, int left_total, BlockPlan bp,
+                                                                 int hist_side, const int* __restrict__ group_used,
+                                                                 int set, const int* __restrict__ tile_first,
+                                                                 int rows_per_flush) {
+  extern __shared__ unsigned char smem[];
+  __shared__ int scratch[32];
+  __shared__ int red[kStreamThreads / 64];
+  const int blk = blockIdx.x, tile = blockIdx.y;
+  const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
+  const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
+  LdsHist<FORM> L;
+  L.bind(smem, nb);
+  // left rows of the row blocks before this one
+  int lb = 0;
+  for (int i = threadIdx.x; i < blk; i += blockDim.x) lb += s.blk_cnt[i];
+  for (int o = 32; o > 0; o >>= 1) lb += __shfl_down(lb, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lb;
+  __syncthreads();
+  int left_before = 0;
+  for (int w = 0; w < kStreamThreads / 64; ++w) left_before += red[w];
+  __syncthreads();
+
+  const long long seg_end = (long long)seg.begin + seg.count;
+  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
+  const long long ce = min(cb + (long long)bp.chunk, seg_end);
+  long long lpos = (long long)seg.begin + left_before;
+  long long rpos = (long long)seg.begin + left_total + ((cb - seg.begin) - left_before);
+  const uint8_t* scols = s.cols[seg.buf];
+  uint8_t* dcols = (uint8_t*)s.cols[dst_buf];
+  const uint32_t* scol = (const uint32_t*)(scols + (size_t)split_group * s.npad);
+  const bool do_hist = hist_side >= 0;
+  if (do_hist) L.zero(nb);
+  __syncthreads();
+  bool first = true;
+  long long since_flush = 0;
+  int my_left_total = 0;
+  for (long long t0 = (cb >> 2); t0 < ((ce + 3) >> 2); t0 += blockDim.x) {
+    const long long d = t0 + threadIdx.x;
+    const long long r0 = d << 2;
+    int valid = 0, lmask = 0;
+    uint32_t sw = 0;
+    if (d < ((ce + 3) >> 2)) {
+      sw = scol[d];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (r0 + k >= cb && r0 + k < ce) {
+          valid |= 1 << k;
+          if (route_left(rule, (sw >> (8 * k)) & 0xffu)) lmask |= 1 << k;
+        }
+      }
+    }
+    const int rmask = valid & ~lmask;
+    const int nl = __popc(lmask), nr = __popc(rmask);
+    int pl, pr, tl, tr;
+    block_scan2(nl, nr, &pl, &pr, &tl, &tr, scratch);
+    long long pos[4];
+    {
+      int il = 0, ir = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (lmask & (1 << k)) pos[k] = lpos + pl + il++;
+        else if (rmask & (1 << k)) pos[k] = rpos + pr + ir++;
+        else pos[k] = -1;
+      }
+    }
+    if (valid) {
+      for (int g = g0; g < g1; ++g) {
+        const uint32_t w = ((const uint32_t*)(scols + (size_t)g * s.npad))[d];
+        uint8_t* dc = dcols + (size_t)g * s.npad;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (valid & (1 << k)) dc[pos[k]] = (uint8_t)((w >> (8 * k)) & 0xffu);
+      }
+      if (tile == 0) {
+        int32_t* didx = s.idx[dst_buf];
+        if (seg.buf == 0) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (valid & (1 << k)) didx[pos[k]] = (int32_t)(r0 + k);
+        } else {
+          const int32_t* sidx = s.idx[seg.buf];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (valid & (1 << k)) didx[pos[k]] = sidx[r0 + k];
+        }
+        if (FORM == FORM_QUANT) {
+          const int16_t* sp = (const int16_t*)s.gh[seg.buf];
+          int16_t* dp = (int16_t*)s.gh[dst_buf];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (valid & (1 << k)) dp[pos[k]] = sp[r0 + k];
+        } else {
+          const float2* sp = (const float2*)s.gh[seg.buf];
+          float2* dp = (float2*)s.gh[dst_buf];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (valid & (1 << k)) dp[pos[k]] = sp[r0 + k];
+        }
+      }
+      if (do_hist) {
+        const int hmask = hist_side == 0 ? lmask : rmask;
+        if (hmask) {
+          RowPayload<FORM> P;
+          P.load(s.gh[seg.buf], d);
+          for (int g = g0; g < g1; ++g) {
+            if (!group_used[g]) continue;
+            const uint32_t w = ((const uint32_t*)(scols + (size_t)g * s.npad))[d];
+            const int base = s.group_bin_start[g] - bin_lo;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              if (hmask & (1 << k)) P.add(L, k, base + (int)((w >> (8 * k)) & 0xffu));
+          }
+        }
+      }
+    }
+    lpos += tl;
+    rpos += tr;
+    my_left_total += tl;
+    since_flush += (long long)blockDim.x * 4;
+    if (do_hist && since_flush >= rows_per_flush) {
+      __syncthreads();
+      L.flush(s, set, blk, bin_lo, nb, first);
+      __syncthreads();
+      L.zero(nb);
+      __syncthreads();
+      first = false;
+      since_flush = 0;
+    }
+  }
+  if (do_hist) {
+    __syncthreads();
+    L.flush(s, set, blk, bin_lo, nb, first);
+  }
+  if (threadIdx.x == 0 && tile == 0 && blk == bp.nblk - 1 && left_before + my_left_total != left_total)
+    atomicOr(s.err_flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Split search (feature_histogram.rs:219-1886), one feature per block.
+// ---------------------------------------------------------------------------
+struct Flags {
+  bool l1, mo, sm;
+};
+
+__device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
+__device__ __forceinline__ double thr_l1(double s, double l1) {
+  double r = fmax(fabs(s) - l1, 0.0);
+  return sgn(s) * r;
+}
+__device__ __forceinline__ int round_int(double x) { return sat_i32_dev(x + 0.5); }
+
+__device__ double leaf_output(double g, double h, double l1, double l2, double mds, double smooth, int n, double parent,
+                              Flags f) {
+  double ret = f.l1 ? -thr_l1(g, l1) / (h + l2) : -g / (h + l2);
+  if (f.mo && mds > 0.0 && fabs(ret) > mds) ret = sgn(ret) * mds;
+  if (f.sm) {
+    double nd = (double)n;
+    ret = ret * (nd / smooth) / (nd / smooth + 1.0) + parent / (nd / smooth + 1.0);
+  }
+  return ret;
+}
+__device__ double gain_given_output(double g, double h, double l1, double l2, double out, bool use_l1) {
+  double gg = use_l1 ? thr_l1(g, l1) : g;
+  double a = 2.0 * gg;
+  double prod = a * out;
+  double c = (h + l2) * out * out;
+  return -(prod + c);
+}
+__device__ double leaf_gain(double g, double h, double l1, double l2, double mds, double smooth, int n, double parent,
+                            Flags f) {
+  if (!f.mo && !f.sm) {
+    if (f.l1) {
+      double sg = thr_l1(g, l1);
+      return (sg * sg) / (h + l2);
+    }
+    return (g * g) / (h + l2);
+  }
+  double out = leaf_output(g, h, l1, l2, mds, smooth, n, parent, f);
+  return gain_given_output(g, h, l1, l2, out, f.l1);
+}
+__device__ __forceinline__ double split_gain_plain(double gl, double hl, double gr, double hr, double l2) {
+  double a = gl * gl / (hl + l2);
+  double b = gr * gr / (hr + l2);
+  return a + b;
+}
+
+struct Cand {
+  bool splittable;
+  double gain;
+  int threshold;
+  bool default_left;
+  double lg, lh;   // f64 forms: best left sums (lh includes kEpsilon); int forms unused
+  long long ilg, ilh;
+  int lc;
+};
+
+// One direction of FindBestThresholdSequentially (f64) or ...Int (quantized).
+template <bool INT>
+__device__ void seq_scan(const double* G, const double* H, const long long* IG, const long long* IH, const FeatDev& m,
+                         const ScanParams& p, Flags fl, double sum_g, double sum_h, long long tg, long long th,
+                         int num_data, double mgs, double parent, bool rev, bool skip_default, bool na_missing,
+                         SplitOut& out) {
+  const int offset = m.offset;
+  bool splittable = false;
+  double best_gain = kMinScore, best_slg = NAN, best_slh = NAN;
+  long long best_lg = 0, best_lh = 0;
+  int best_lc = 0;
+  unsigned best_th = (unsigned)m.num_bin;
+  const double cnt_factor = INT ? (double)num_data / (double)th : (double)num_data / sum_h;
+  auto gains = [&](double slg, double slh, double srg, double srh, int lc, int rc) -> double {
+    if (INT) {
+      slh += kEps;
+      srh += kEps;
+    }
+    if (!fl.l1 && !fl.mo && !fl.sm) return split_gain_plain(slg, slh, srg, srh, p.l2);
+    double a = leaf_gain(slg, slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
+    double b = leaf_gain(srg, srh, p.l1, p.l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
+    return a + b;
+  };
+  if (rev) {
+    double srg = 0.0, srh = kEps;
+    long long rg = 0, rh = 0;
+    int rc = 0;
+    const int t_end = 1 - offset;
+    for (int t = m.num_bin - 1 - offset - (na_missing ? 1 : 0); t >= t_end; --t) {
+      if (skip_default && (t + offset) == m.default_bin) continue;
+      double slh, srg_v, srh_v, slg;
+      int lc;
+      if (INT) {
+        rg += IG[t];
+        rh += IH[t];
+        rc = round_int((double)rh * cnt_factor);
+        srh_v = (double)rh * p.hess_scale;
+        if (rc < p.min_data_in_leaf || srh_v < p.min_sum_hessian) continue;
+        lc = num_data - rc;
+        if (lc < p.min_data_in_leaf) break;
+        long long lh = th - rh;
+        slh = (double)lh * p.hess_scale;
+        if (slh < p.min_sum_hessian) break;
+        long long lg = tg - rg;
+        slg = (double)lg * p.grad_scale;
+        srg_v = (double)rg * p.grad_scale;
+        double cur = gains(slg, slh, srg_v, srh_v, lc, rc);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_lg = lg;
+          best_lh = lh;
+          best_th = (unsigned)(t - 1 + offset);
+          best_gain = cur;
+        }
+      } else {
+        double g = G[t], h = H[t];
+        int cnt = round_int(h * cnt_factor);
+        srg += g;
+        srh += h;
+        rc += cnt;
+        if (rc < p.min_data_in_leaf || srh < p.min_sum_hessian) continue;
+        lc = num_data - rc;
+        if (lc < p.min_data_in_leaf) break;
+        slh = sum_h - srh;
+        if (slh < p.min_sum_hessian) break;
+        slg = sum_g - srg;
+        double cur = gains(slg, slh, srg, srh, lc, rc);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_lc = lc;
+          best_slg = slg;
+          best_slh = slh;
+          best_th = (unsigned)(t - 1 + offset);
+          best_gain = cur;
+        }
+      }
+    }
+  } else {
+    double slg = 0.0, slh = kEps;
+    long long lg = 0, lh = 0;
+    int lc = 0;
+    int t = 0;
+    const int t_end = m.num_bin - 2 - offset;
+    if (na_missing && offset == 1) {
+      if (INT) {
+        lg = tg;
+        lh = th;
+        for (int i = 0; i < m.num_bin - offset; ++i) {
+          lg -= IG[i];
+          lh -= IH[i];
+        }
+      } else {
+        slg = sum_g;
+        slh = sum_h - kEps;
+        lc = num_data;
+        for (int i = 0; i < m.num_bin - offset; ++i) {
+          double g = G[i], h = H[i];
+          int cnt = round_int(h * cnt_factor);
+          slg -= g;
+          slh -= h;
+          lc -= cnt;
+        }
+      }
+      t = -1;
+    }
+    for (; t <= t_end; ++t) {
+      if (skip_default && (t + offset) == m.default_bin) continue;
+      if (INT) {
+        if (t >= 0) {
+          lg += IG[t];
+          lh += IH[t];
+        }
+        int lcc = round_int((double)lh * cnt_factor);
+        double slh_v = (double)lh * p.hess_scale;
+        if (lcc < p.min_data_in_leaf || slh_v < p.min_sum_hessian) continue;
+        int rc = num_data - lcc;
+        if (rc < p.min_data_in_leaf) break;
+        long long rh = th - lh;
+        double srh = (double)rh * p.hess_scale;
+        if (srh < p.min_sum_hessian) break;
+        long long rg = tg - lg;
+        double slg_v = (double)lg * p.grad_scale;
+        double srg = (double)rg * p.grad_scale;
+        double cur = gains(slg_v, slh_v, srg, srh, lcc, rc);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_lg = lg;
+          best_lh = lh;
+          best_th = (unsigned)(t + offset);
+          best_gain = cur;
+        }
+      } else {
+        if (t >= 0) {
+          slg += G[t];
+          slh += H[t];
+          lc += round_int(H[t] * cnt_factor);
+        }
+        if (lc < p.min_data_in_leaf || slh < p.min_sum_hessian) continue;
+        int rc = num_data - lc;
+        if (rc < p.min_data_in_leaf) break;
+        double srh = sum_h - slh;
+        if (srh < p.min_sum_hessian) break;
+        double srg = sum_g - slg;
+        double cur = gains(slg, slh, srg, srh, lc, rc);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_lc = lc;
+          best_slg = slg;
+          best_slh = slh;
+          best_th = (unsigned)(t + offset);
+          best_gain = cur;
+        }
+      }
+    }
+  }
+  if (splittable) out.splittable = 1;
+  if (splittable && best_gain > out.gain + mgs) {
+    out.threshold = (int)best_th;
+    if (INT) {
+      long long rg = tg - best_lg, rh = th - best_lh;
+      double slg = (double)best_lg * p.grad_scale, slh = (double)best_lh * p.hess_scale;
+      double srg = (double)rg * p.grad_scale, srh = (double)rh * p.hess_scale;
+      int lc = round_int((double)best_lh * cnt_factor);
+      int rc = round_int((double)rh * cnt_factor);
+      out.left_output = leaf_output(slg, slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
+      out.left_count = lc;
+      out.left_sum_gradient = slg;
+      out.left_sum_hessian = slh;
+      out.right_output = leaf_output(srg, srh, p.l1, p.l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
+      out.right_count = rc;
+      out.right_sum_gradient = srg;
+      out.right_sum_hessian = srh;
+      out.left_int_sum = (long long)(((unsigned long long)best_lg << 32) | ((unsigned long long)best_lh & 0xffffffffull));
+      out.right_int_sum = (long long)(((unsigned long long)rg << 32) | ((unsigned long long)rh & 0xffffffffull));
+    } else {
+      out.left_output =
+          leaf_output(best_slg, best_slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, best_lc, parent, fl);
+      out.left_count = best_lc;
+      out.left_sum_gradient = best_slg;
+      out.left_sum_hessian = best_slh - kEps;
+      out.right_output = leaf_output(sum_g - best_slg, sum_h - best_slh, p.l1, p.l2, p.max_delta_step, p.path_smooth,
+                                     num_data - best_lc, parent, fl);
+      out.right_count = num_data - best_lc;
+      out.right_sum_gradient = sum_g - best_slg;
+      out.right_sum_hessian = sum_h - best_slh - kEps;
+    }
+    out.gain = best_gain - mgs;
+    out.default_left = rev ? 1 : 0;
+  }
+}
+
+template <bool INT>
+__device__ void search_numerical(const double* G, const double* H, const long long* IG, const long long* IH,
+                                 const FeatDev& m, const ScanParams& p, double sum_g_in, double sum_h_in, long long tg,
+                                 long long th, int num_data, double parent, SplitOut& out) {
+  out.default_left = 1;
+  out.gain = kMinScore;
+  Flags fl{p.l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > kEps};
+  double sum_g, sum_h;
+  if (INT) {
+    sum_g = (double)tg * p.grad_scale;
+    sum_h = (double)th * p.hess_scale;
+  } else {
+    sum_g = sum_g_in;
+    sum_h = sum_h_in + 2.0 * kEps;
+  }
+  double gain_shift = leaf_gain(sum_g, sum_h, p.l1, p.l2, p.max_delta_step, p.path_smooth, num_data, parent, fl);
+  double mgs = gain_shift + p.min_gain_to_split;
+  if (m.num_bin > 2 && m.missing_type != 0) {
+    if (m.missing_type == 1) {
+      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, true, false, out);
+      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, false, true, false, out);
+    } else {
+      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, false, true, out);
+      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, false, false, true, out);
+    }
+  } else {
+    seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, false, false, out);
+    if (m.missing_type == 2) out.default_left = 0;
+  }
+  out.gain *= m.penalty;
+}
+
+__device__ double leaf_gain_cat(double g, double h, double l1, double l2, double mds, double smooth, int n,
+                                double parent, Flags f) {
+  return leaf_gain(g, h, l1, l2, mds, smooth, n, parent, f);
+}
+
+// Categorical search (feature_histogram.rs:1328-1597, 1606-1886). `sorted` holds
+// the eligible bins in stable ctr order (computed by the wave beforehand).
+template <bool INT>
+__device__ void search_categorical(const double* G, const double* H, const long long* IG, const long long* IH,
+                                   const FeatDev& m, const ScanParams& p, double sum_g_in, double sum_h_in,
+                                   long long tg, long long th, int num_data, double parent, const int* sorted,
+                                   int used_bin, SplitOut& out) {
+  double sum_g, sum_h;
+  if (INT) {
+    sum_g = (double)tg * p.grad_scale;
+    sum_h = (double)th * p.hess_scale;
+  } else {
+    sum_g = sum_g_in;
+    sum_h = sum_h_in + 2.0 * kEps;
+  }
+  out.default_left = 0;
+  out.gain = kMinScore;
+  Flags fl{p.l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > kEps};
+  double gain_shift;
+  if (fl.sm) gain_shift = gain_given_output(sum_g, sum_h, p.l1, p.l2, parent, fl.l1);
+  else gain_shift = leaf_gain(sum_g, sum_h, p.l1, p.l2, p.max_delta_step, 0.0, num_data, 0.0, Flags{fl.l1, fl.mo, false});
+  const double mgs = gain_shift + p.min_gain_to_split;
+  const int offset = m.offset;
+  const int bin_start = 1 - offset, bin_end = m.num_bin - offset;
+  const double cnt_factor = INT ? (double)num_data / (double)th : (double)num_data / sum_h;
+  const bool onehot = m.num_bin <= p.max_cat_to_onehot;
+  double l2 = p.l2;
+  bool splittable = false;
+  double best_gain = kMinScore, best_slg = 0.0, best_slh = 0.0;
+  long long best_lg = 0, best_lh = 0;
+  int best_threshold = -1, best_dir = 1, best_lc = 0;
+  auto gains = [&](double slg, double slh, double srg, double srh, int lc, int rc) -> double {
+    double a = leaf_gain_cat(slg, slh, p.l1, l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
+    double b = leaf_gain_cat(srg, srh, p.l1, l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
+    return a + b;
+  };
+  if (onehot) {
+    for (int t = bin_start; t < bin_end; ++t) {
+      if (INT) {
+        long long ig = IG[t], ih = IH[t];
+        double h = (double)ih * p.hess_scale;
+        int cnt = round_int((double)ih * cnt_factor);
+        if (cnt < p.min_data_in_leaf || h < p.min_sum_hessian) continue;
+        int oc = num_data - cnt;
+        if (oc < p.min_data_in_leaf) continue;
+        long long ioh = th - ih;
+        double soh = (double)ioh * p.hess_scale;
+        if (soh < p.min_sum_hessian) continue;
+        double g = (double)ig * p.grad_scale;
+        double sog = (double)(tg - ig) * p.grad_scale;
+        double cur = gains(sog, soh, g, h, oc, cnt);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_threshold = t;
+          best_lg = ig;
+          best_lh = ih;
+          best_gain = cur;
+        }
+      } else {
+        double g = G[t], h = H[t];
+        int cnt = round_int(h * cnt_factor);
+        if (cnt < p.min_data_in_leaf || h < p.min_sum_hessian) continue;
+        int oc = num_data - cnt;
+        if (oc < p.min_data_in_leaf) continue;
+        double soh = sum_h - h - kEps;
+        if (soh < p.min_sum_hessian) continue;
+        double sog = sum_g - g;
+        double cur = gains(sog, soh, g, h + kEps, oc, cnt);
+        if (cur <= mgs) continue;
+        splittable = true;
+        if (cur > best_gain) {
+          best_threshold = t;
+          best_slg = g;
+          best_slh = h + kEps;
+          best_lc = cnt;
+          best_gain = cur;
+        }
+      }
+    }
+  } else {
+    l2 += p.cat_l2;
+    const int max_num_cat = min(p.max_cat_threshold, (used_bin + 1) / 2);
+    for (int di = 0; di < 2; ++di) {
+      const int dir = di == 0 ? 1 : -1;
+      int pos = di == 0 ? 0 : used_bin - 1;
+      int cnt_cur_group = 0;
+      double slg = 0.0, slh = kEps;
+      long long lg = 0, lh = 0;
+      int lc = 0;
+      for (int i = 0; i < used_bin && i < max_num_cat; ++i) {
+        const int t = sorted[pos];
+        pos += dir;
+        if (INT) {
+          long long ig = IG[t], ih = IH[t];
+          int cnt = round_int((double)ih * cnt_factor);
+          lg += ig;
+          lh += ih;
+          lc += cnt;
+          cnt_cur_group += cnt;
+          double slh_v = (double)lh * p.hess_scale;
+          if (lc < p.min_data_in_leaf || slh_v < p.min_sum_hessian) continue;
+          int rc = num_data - lc;
+          if (rc < p.min_data_in_leaf || rc < p.min_data_per_group) break;
+          long long rh = th - lh;
+          double srh = (double)rh * p.hess_scale;
+          if (srh < p.min_sum_hessian) break;
+          if (cnt_cur_group < p.min_data_per_group) continue;
+          cnt_cur_group = 0;
+          double slg_v = (double)lg * p.grad_scale;
+          double srg = (double)(tg - lg) * p.grad_scale;
+          double cur = gains(slg_v, slh_v, srg, srh, lc, rc);
+          if (cur > mgs) {
+            splittable = true;
+            if (cur > best_gain) {
+              best_lg = lg;
+              best_lh = lh;
+              best_threshold = i;
+              best_gain = cur;
+              best_dir = dir;
+            }
+          }
+        } else {
+          double g = G[t], h = H[t];
+          int cnt = round_int(h * cnt_factor);
+          slg += g;
+          slh += h;
+          lc += cnt;
+          cnt_cur_group += cnt;
+          if (lc < p.min_data_in_leaf || slh < p.min_sum_hessian) continue;
+          int rc = num_data - lc;
+          if (rc < p.min_data_in_leaf || rc < p.min_data_per_group) break;
+          double srh = sum_h - slh;
+          if (srh < p.min_sum_hessian) break;
+          if (cnt_cur_group < p.min_data_per_group) continue;
+          cnt_cur_group = 0;
+          double srg = sum_g - slg;
+          double cur = gains(slg, slh, srg, srh, lc, rc);
+          if (cur > mgs) {
+            splittable = true;
+            if (cur > best_gain) {
+              best_lc = lc;
+              best_slg = slg;
+              best_slh = slh;
+              best_threshold = i;
+              best_gain = cur;
+              best_dir = dir;
+            }
+          }
+        }
+      }
+    }
+  }
+  out.splittable = splittable ? 1 : 0;
+  if (splittable) {
+    if (INT) {
+      double slg = (double)best_lg * p.grad_scale, slh = (double)best_lh * p.hess_scale;
+      long long rg = tg - best_lg, rh = th - best_lh;
+      double srg = (double)rg * p.grad_scale, srh = (double)rh * p.hess_scale;
+      int lc = round_int((double)best_lh * cnt_factor);
+      int rc = round_int((double)rh * cnt_factor);
+      out.left_output = leaf_output(slg, slh, p.l1, l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
+      out.left_count = lc;
+      out.left_sum_gradient = slg;
+      out.left_sum_hessian = slh;
+      out.right_output = leaf_output(srg, srh, p.l1, l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
+      out.right_count = rc;
+      out.right_sum_gradient = srg;
+      out.right_sum_hessian = srh;
+      out.left_int_sum = (long long)(((unsigned long long)best_lg << 32) | ((unsigned long long)best_lh & 0xffffffffull));
+      out.right_int_sum = (long long)(((unsigned long long)rg << 32) | ((unsigned long long)rh & 0xffffffffull));
+    } else {
+      out.left_output = leaf_output(best_slg, best_slh, p.l1, l2, p.max_delta_step, p.path_smooth, best_lc, parent, fl);
+      out.left_count = best_lc;
+      out.left_sum_gradient = best_slg;
+      out.left_sum_hessian = best_slh - kEps;
+      out.right_output = leaf_output(sum_g - best_slg, sum_h - best_slh, p.l1, l2, p.max_delta_step, p.path_smooth,
+                                     num_data - best_lc, parent, fl);
+      out.right_count = num_data - best_lc;
+      out.right_sum_gradient = sum_g - best_slg;
+      out.right_sum_hessian = sum_h - best_slh - kEps;
+    }
+    out.gain = best_gain - mgs;
+    for (int i = 0; i < 8; ++i) out.cat_bits[i] = 0u;
+    if (onehot) {
+      out.num_cat = 1;
+      unsigned b = (unsigned)(best_threshold + offset);
+      out.cat_bits[b / 32] |= 1u << (b % 32);
+    } else {
+      out.num_cat = best_threshold + 1;
+      for (int i = 0; i <= best_threshold; ++i) {
+        int ps = best_dir == 1 ? i : used_bin - 1 - i;
+        unsigned b = (unsigned)(sorted[ps] + offset);
+        out.cat_bits[b / 32] |= 1u << (b % 32);
+      }
+    }
+  }
+  out.gain *= m.penalty;
+}
+
+// One leaf's feature slice in LDS.
+struct SliceLds {
+  double g[kMaxBins], h[kMaxBins];
+  long long ig[kMaxBins], ih[kMaxBins];
+  int c[kMaxBins];
+  int sorted[kMaxBins];
+  int used_bin;
+};
+
+template <int FORM>
+__device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, const LeafArgs& a, const ScanParams& p,
+                                         SliceLds& S) {
+  const int set = a.part_set;
+  for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
+    const int bin = m.hist_off + i;
+    if (FORM == FORM_QUANT) {
+      long long g = 0, h = 0, c = 0;
+      for (int j = 0; j < a.nblk; ++j) {
+        long long gg, hh, cc;
+        qunpack(s.part_q[set][(size_t)j * s.total_bins + bin], &gg, &hh, &cc);
+        g += gg;
+        h += hh;
+        c += cc;
+      }
+      S.ig[i] = g;
+      S.ih[i] = h;
+      S.c[i] = (int)c;
+    } else {
+      double g = 0.0, h = 0.0;
+      unsigned c = 0;
+      for (int j = 0; j < a.nblk; ++j) {
+        g += s.part_g[set][(size_t)j * s.total_bins + bin];
+        h += s.part_h[set][(size_t)j * s.total_bins + bin];
+        c += s.part_c[set][(size_t)j * s.total_bins + bin];
+      }
+      if (p.const_hess) h *= p.h0;
+      S.g[i] = g;
+      S.h[i] = h;
+      S.c[i] = (int)c;
+    }
+  }
+}
+
+// fix_slice / fix_slice_int (serial.rs:2205-2238) plus the count channel.
+template <int FORM>
+__device__ void fix_slice(const FeatDev& m, const LeafArgs& a, SliceLds& S) {
+  if (threadIdx.x != 0 || m.mfb <= 0) return;
+  const int mfb = m.mfb, nb = m.num_bin;
+  if (FORM == FORM_QUANT) {
+    long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
+    S.ig[mfb] = tg;
+    S.ih[mfb] = th;
+    for (int i = 0; i < nb; ++i)
+      if (i != mfb) {
+        S.ig[mfb] -= S.ig[i];
+        S.ih[mfb] -= S.ih[i];
+      }
+  } else {
+    S.g[mfb] = a.sum_g;
+    S.h[mfb] = a.sum_h;
+    for (int i = 0; i < nb; ++i)
+      if (i != mfb) {
+        S.g[mfb] -= S.g[i];
+        S.h[mfb] -= S.h[i];
+      }
+  }
+  int c = a.num_data;
+  for (int i = 0; i < nb; ++i)
+    if (i != mfb) c -= S.c[i];
+  S.c[mfb] = c;
+}
+
+template <int FORM>
+__device__ void store_slice(const DevState& s, const FeatDev& m, int slot, const SliceLds& S) {
+  double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
+  long long* pooli = (long long*)pool;
+  int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
+  for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
+    const int bin = m.hist_off + i;
+    if (FORM == FORM_QUANT) {
+      pooli[2 * bin] = S.ig[i];
+      pooli[2 * bin + 1] = S.ih[i];
+    } else {
+      pool[2 * bin] = S.g[i];
+      pool[2 * bin + 1] = S.h[i];
+    }
+    cnt[bin] = S.c[i];
+  }
+}
+
+// Larger leaf = parent (in place in the larger slot) - fixed smaller, over the slice.
+template <int FORM>
+__device__ void subtract_slice(const DevState& s, const FeatDev& m, int slot, const SliceLds& small, SliceLds& L) {
+  double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
+  long long* pooli = (long long*)pool;
+  int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
+  for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
+    const int bin = m.hist_off + i;
+    if (FORM == FORM_QUANT) {
+      L.ig[i] = pooli[2 * bin] - small.ig[i];
+      L.ih[i] = pooli[2 * bin + 1] - small.ih[i];
+    } else {
+      L.g[i] = pool[2 * bin] - small.g[i];
+      L.h[i] = pool[2 * bin + 1] - small.h[i];
+    }
+    L.c[i] = cnt[bin] - small.c[i];
+  }
+}
+
+// Stable ctr order of the eligible bins (feature_histogram.rs:1478-1486):
+// rank = number of eligible bins with a smaller ctr, or an equal ctr and a lower index.
+template <int FORM>
+__device__ void cat_sort(const FeatDev& m, const ScanParams& p, const LeafArgs& a, SliceLds& S, int* elig) {
+  const int offset = m.offset, bin_start = 1 - offset, bin_end = m.num_bin - offset;
+  double cnt_factor;
+  if (FORM == FORM_QUANT) cnt_factor = (double)a.num_data / (double)(a.int_sum & 0xffffffffLL);
+  else cnt_factor = (double)a.num_data / (a.sum_h + 2.0 * kEps);
+  auto ctr = [&](int i) -> double {
+    if (FORM == FORM_QUANT)
+      return ((double)S.ig[i] * p.grad_scale) / ((double)S.ih[i] * p.hess_scale + p.cat_smooth);
+    return S.g[i] / (S.h[i] + p.cat_smooth);
+  };
+  for (int i = threadIdx.x; i < kMaxBins; i += blockDim.x) {
+    int e = 0;
+    if (i >= bin_start && i < bin_end) {
+      double hv = FORM == FORM_QUANT ? (double)S.ih[i] * cnt_factor : S.h[i] * cnt_factor;
+      e = (double)round_int(hv) >= p.cat_smooth ? 1 : 0;
+    }
+    elig[i] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int i = 0; i < kMaxBins; ++i) n += elig[i];
+    S.used_bin = n;
+  }
+  for (int i = threadIdx.x; i < kMaxBins; i += blockDim.x) {
+    if (!elig[i]) continue;
+    const double ci = ctr(i);
+    int r = 0;
+    for (int j = 0; j < kMaxBins; ++j) {
+      if (!elig[j] || j == i) continue;
+      const double cj = ctr(j);
+      if (cj < ci || (!(ci < cj) && j < i)) ++r;
+    }
+    S.sorted[r] = i;
+  }
+  __syncthreads();
+}
+
+// Exact rows routed left by the chosen split of one leaf (count channel).
+__device__ void exact_counts(const FeatDev& m, const SliceLds& S, int num_data, SplitOut& o) {
+  RuleDev r = m.bin_type == 1 ? make_rule_categorical(m, o.cat_bits)
+                              : make_rule_numerical(m, (uint32_t)o.threshold, o.default_left != 0);
+  long long left = 0, inslice = 0;
+  for (int i = 0; i < m.hist_len; ++i) {
+    inslice += S.c[i];
+    if (route_left(r, stored_value(m, i + m.offset))) left += S.c[i];
+  }
+  if (m.offset == 1 && route_left(r, 0u)) left += (long long)num_data - inslice;
+  o.exact_left = (int)left;
+  o.exact_right = num_data - (int)left;
+}
+
+template <int FORM>
+__device__ void search_leaf(const FeatDev& m, const ScanParams& p, const LeafArgs& a, SliceLds& S, SplitOut& o) {
+  constexpr bool INT = FORM == FORM_QUANT;
+  o = SplitOut{};
+  o.gain = kMinScore;
+  o.default_left = 1;
+  o.feature = m.real;
+  long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
+  if (m.bin_type == 1) {
+    search_categorical<INT>(S.g, S.h, S.ig, S.ih, m, p, a.sum_g, a.sum_h, tg, th, a.num_data, a.parent_output,
+                            S.sorted, S.used_bin, o);
+  } else {
+    search_numerical<INT>(S.g, S.h, S.ig, S.ih, m, p, a.sum_g, a.sum_h, tg, th, a.num_data, a.parent_output, o);
+  }
+  if (o.gain > kMinScore) exact_counts(m, S, a.num_data, o);
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafArgs large, int has_large,
+                                              int use_subtract, ScanParams p, const int* __restrict__ used,
+                                              SplitOut* out) {
+  __shared__ SliceLds S[2];
+  __shared__ int elig[2][kMaxBins];
+  const int f = blockIdx.x;
+  if (!used[f]) return;
+  const FeatDev m = s.feats[f];
+  // smaller leaf: reduce the row-block partials, fix, store
+  load_slice_from_partials<FORM>(s, m, small, p, S[0]);
+  __syncthreads();
+  fix_slice<FORM>(m, small, S[0]);
+  __syncthreads();
+  store_slice<FORM>(s, m, small.slot, S[0]);
+  if (has_large) {
+    if (use_subtract) {
+      subtract_slice<FORM>(s, m, large.slot, S[0], S[1]);
+      __syncthreads();
+    } else {
+      load_slice_from_partials<FORM>(s, m, large, p, S[1]);
+      __syncthreads();
+      fix_slice<FORM>(m, large, S[1]);
+      __syncthreads();
+    }
+    store_slice<FORM>(s, m, large.slot, S[1]);
+  }
+  __syncthreads();
+  // the ctr sort of a categorical feature uses the whole block, one leaf at a time
+  if (m.bin_type == 1) {
+    cat_sort<FORM>(m, p, small, S[0], elig[0]);
+    if (has_large) cat_sort<FORM>(m, p, large, S[1], elig[1]);
+  }
+  __syncthreads();
+  // the two searches run in separate waves
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    if (wave == 0) {
+      search_leaf<FORM>(m, p, small, S[0], out[f]);
+    } else if (has_large) {
+      search_leaf<FORM>(m, p, large, S[1], out[s.num_features + f]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gradient discretization (gradient_discretizer/mod.rs:116-201)
+// ---------------------------------------------------------------------------
+__global__ void k_max_abs(const float* __restrict__ g, const float* __restrict__ h, int n, unsigned* out) {
+  unsigned mg = 0u, mh = 0u;
+  int bad = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gv = g[i], hv = h[i];
+    if (!isfinite(gv) || !isfinite(hv) || hv < 0.0f) bad = 1;
+    mg = max(mg, __float_as_uint(fabsf(gv)));
+    mh = max(mh, __float_as_uint(fabsf(hv)));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mg = max(mg, (unsigned)__shfl_down((int)mg, o, 64));
+    mh = max(mh, (unsigned)__shfl_down((int)mh, o, 64));
+    bad |= __shfl_down(bad, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], mg);
+    atomicMax(&out[1], mh);
+    if (bad) atomicOr(&out[2], 1u);
+  }
+}
+
+__global__ void k_discretize(const float* __restrict__ g, const float* __restrict__ h, const double* __restrict__ gn,
+                             const double* __restrict__ hn, int n, int start, double inv_g, double inv_h,
+                             int const_hess, int stochastic, int16_t* __restrict__ packed, long long* sums) {
+  long long sg = 0, sh = 0, sa = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double gd = (double)g[i];
+    int pos = i + start;
+    if (pos >= n) pos -= n;
+    double noise, hnoise;
+    if (stochastic) {
+      noise = gn[pos];
+      hnoise = hn[pos];
+    } else {
+      noise = 0.5;
+      hnoise = 0.5;
+    }
+    const double sn = gd >= 0.0 ? noise : -noise;
+    const double pg = gd * inv_g;
+    const int8_t qg = sat_i8_dev(pg + sn);
+    int8_t qh = 1;
+    if (!const_hess) {
+      const double ph = (double)h[i] * inv_h;
+      qh = sat_i8_dev(ph + hnoise);
+    }
+    packed[i] = (int16_t)(uint16_t)(((unsigned)(uint8_t)qg << 8) | (unsigned)(uint8_t)qh);
+    sg += qg;
+    sh += (uint8_t)qh;
+    sa += qg < 0 ? -(int)qg : (int)qg;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sg += __shfl_down(sg, o, 64);
+    sh += __shfl_down(sh, o, 64);
+    sa += __shfl_down(sa, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)&sums[0], (unsigned long long)sg);
+    atomicAdd((unsigned long long*)&sums[1], (unsigned long long)sh);
+    atomicAdd((unsigned long long*)&sums[2], (unsigned long long)sa);
+  }
+}
+
+// With a constant hessian the hessian slot counts rows (scaled by hessians[0] after
+// the reduction), as histogram.rs:226-232 and serial.rs:982-988 do.
+__global__ void k_pack_f32(const float* __restrict__ g, const float* __restrict__ h, int n, int const_hess,
+                           float2* __restrict__ gh) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    gh[i] = make_float2(g[i], const_hess ? 1.0f : h[i]);
+}
+
+// renew_int_grad_tree_output (serial.rs:634-655): sequential f64 sums over each
+// leaf's rows in ascending row order. Lane 0 sums gradients, lane 1 hessians.
+__global__ void k_renew_sums(DevState s, const Segment* __restrict__ segs, const float* __restrict__ g,
+                             const float* __restrict__ h, double* out) {
+  const int leaf = blockIdx.x;
+  const Segment seg = segs[leaf];
+  const int lane = threadIdx.x;
+  if (lane > 1) return;
+  const float* src = lane == 0 ? g : h;
+  double acc = 0.0;
+  if (seg.buf == 0) {
+    for (int p = seg.begin; p < seg.begin + seg.count; ++p) acc += (double)src[p];
+  } else {
+    const int32_t* ix = s.idx[seg.buf];
+    for (int p = seg.begin; p < seg.begin + seg.count; ++p) acc += (double)src[ix[p]];
+  }
+  out[2 * leaf + lane] = acc;
+}
+
+__global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const double* __restrict__ vals,
+                            double* score) {
+  const int leaf = blockIdx.y;
+  const Segment seg = segs[leaf];
+  const double v = vals[leaf];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
+    const int pos = seg.begin + p;
+    const int row = seg.buf == 0 ? pos : s.idx[seg.buf][pos];
+    score[row] += v;
+  }
+}
+
+__global__ void k_leaf_rows(DevState s, Segment seg, int32_t* out) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
+    const int pos = seg.begin + p;
+    out[p] = seg.buf == 0 ? pos : s.idx[seg.buf][pos];
+  }
+}
+
+// Binary log-loss gradients (objective/binary.rs:71-90) for a device-side boosting loop.
+__global__ void k_binary_gradients(const double* __restrict__ score, const float* __restrict__ label, int n,
+                                   double sigmoid, float* __restrict__ g, float* __restrict__ h) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double lab = label[i] > 0.0f ? 1.0 : -1.0;
+    const double response = -lab * sigmoid / (1.0 + exp(lab * sigmoid * score[i]));
+    const double ar = fabs(response);
+    g[i] = (float)response;
+    h[i] = (float)(ar * (sigmoid - ar));
+  }
+}
+
+
+// Per-block f64 sums of gradients and hessians (root totals of the float forms).
+__global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ h, int n, double* out) {
+  __shared__ double rg[4], rh[4];
+  double sg = 0.0, sh = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    sg += (double)g[i];
+    sh += (double)h[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sg += __shfl_down(sg, o, 64);
+    sh += __shfl_down(sh, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rg[threadIdx.x >> 6] = sg;
+    rh[threadIdx.x >> 6] = sh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      a += rg[w];
+      b += rh[w];
+    }
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// leaf index of every row, from the leaf segments.
+__global__ void k_leaf_of(DevState s, const Segment* __restrict__ segs, int32_t* leaf_of) {
+  const int leaf = blockIdx.y;
+  const Segment seg = segs[leaf];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
+    const int pos = seg.begin + p;
+    leaf_of[seg.buf == 0 ? pos : s.idx[seg.buf][pos]] = leaf;
+  }
+}
+
+inline int grid_for(long long n, int threads, int cap = 4096) {
+  long long b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+template <int FORM>
+size_t lds_bytes(int nb) {
+  return (size_t)nb * (FORM == FORM_QUANT ? 8 : 20);
+}
+
+template <typename K>
+void allow_lds(K kernel) {
+  static bool done = false;
+  if (!done) {
+    BK_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+    done = true;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int* d_group_used, int part_set,
+                 int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+  if (bp.nblk <= 0) return;
+  dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
+  hipStream_t st = (hipStream_t)stream;
+  switch (s.form) {
+    case FORM_QUANT:
+      allow_lds(k_hist<FORM_QUANT>);
+      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kStreamThreads), lds_bytes<FORM_QUANT>(max_tile_bins), st, s,
+                         seg, bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+      break;
+    case FORM_F64:
+      allow_lds(k_hist<FORM_F64>);
+      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kStreamThreads), lds_bytes<FORM_F64>(max_tile_bins), st, s, seg,
+                         bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+      break;
+    default:
+      allow_lds(k_hist<FORM_F32C>);
+      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kStreamThreads), lds_bytes<FORM_F32C>(max_tile_bins), st, s,
+                         seg, bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+      break;
+  }
+  BK_HIP(hipGetLastError());
+}
+
+void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
+                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
+                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+  if (bp.nblk <= 0) return;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_part_count, dim3((unsigned)bp.nblk), dim3(kStreamThreads), 0, st, s, seg, bp, split_group,
+                     rule);
+  BK_HIP(hipGetLastError());
+  dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
+  const size_t per_bin = s.form == FORM_QUANT ? 8 : 20;
+  size_t lds = hist_side >= 0 ? (size_t)max_tile_bins * per_bin : 0;
+  switch (s.form) {
+    case FORM_QUANT:
+      allow_lds(k_part_scatter<FORM_QUANT>);
+      hipLaunchKernelGGL(k_part_scatter<FORM_QUANT>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf,
+                         split_group, rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first,
+                         rows_per_flush);
+      break;
+    case FORM_F64:
+      allow_lds(k_part_scatter<FORM_F64>);
+      hipLaunchKernelGGL(k_part_scatter<FORM_F64>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf, split_group,
+                         rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first, rows_per_flush);
+      break;
+    default:
+      allow_lds(k_part_scatter<FORM_F32C>);
+      hipLaunchKernelGGL(k_part_scatter<FORM_F32C>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf, split_group,
+                         rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first, rows_per_flush);
+      break;
+  }
+  BK_HIP(hipGetLastError());
+}
+
+void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
+                 const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  LeafArgs lg = large ? *large : LeafArgs{};
+  int has_large = large ? 1 : 0;
+  switch (s.form) {
+    case FORM_QUANT:
+      hipLaunchKernelGGL(k_scan<FORM_QUANT>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg,
+                         has_large, use_subtract, sp, d_used, d_out);
+      break;
+    case FORM_F64:
+      hipLaunchKernelGGL(k_scan<FORM_F64>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg, has_large,
+                         use_subtract, sp, d_used, d_out);
+      break;
+    default:
+      hipLaunchKernelGGL(k_scan<FORM_F32C>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg,
+                         has_large, use_subtract, sp, d_used, d_out);
+      break;
+  }
+  BK_HIP(hipGetLastError());
+}
+
+void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream) {
+  hipLaunchKernelGGL(k_max_abs, dim3(grid_for(n, 256, 2048)), dim3(256), 0, (hipStream_t)stream, g, h, n, d_out4);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
+                       double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
+                       long long* d_sums3, void* stream) {
+  hipLaunchKernelGGL(k_discretize, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
+                     hnoise, n, start, inv_g, inv_h, const_hess, stochastic, packed, d_sums3);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream) {
+  hipLaunchKernelGGL(k_pack_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, g, h, n,
+                     const_hess, (float2*)gh);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_renew_sums(const DevState& s, const Segment* d_segs, int num_leaves, const float* g, const float* h,
+                       double* d_sums, void* stream) {
+  hipLaunchKernelGGL(k_renew_sums, dim3((unsigned)num_leaves), dim3(64), 0, (hipStream_t)stream, s, d_segs, g, h,
+                     d_sums);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
+                      double* score, int max_seg, void* stream) {
+  dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
+  hipLaunchKernelGGL(k_add_score, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, d_leaf_values, score);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream) {
+  if (seg.count <= 0) return;
+  hipLaunchKernelGGL(k_leaf_rows, dim3(grid_for(seg.count, 256, 1024)), dim3(256), 0, (hipStream_t)stream, s, seg,
+                     out);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
+                             void* stream) {
+  hipLaunchKernelGGL(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score,
+                     label, n, sigmoid, g, h);
+  BK_HIP(hipGetLastError());
+}
+
+int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream) {
+  const int nb = grid_for(n, 256, 1024);
+  hipLaunchKernelGGL(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, n, d_partials);
+  BK_HIP(hipGetLastError());
+  return nb;
+}
+
+void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
+                    void* stream) {
+  dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
+  hipLaunchKernelGGL(k_leaf_of, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, leaf_of);
+  BK_HIP(hipGetLastError());
+}
+
+}  // namespace bk

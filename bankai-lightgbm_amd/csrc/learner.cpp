@@ -1,0 +1,1100 @@
+// learner.cpp — host side of the MI355X histogram tree learner and its C ABI.
+//
+// The control flow mirrors SerialTreeLearner (crates/lgbm-core/src/treelearner/serial.rs):
+// histogram pool and LRU (80-162), before_train (658-776), before_find_best_split
+// (779-825), find_best_splits (830-887, 1212-1300, 1322-1566), split (1685-1876) and
+// renew_int_grad_tree_output (634-655). Row data never leaves the GPU: the host
+// keeps O(leaves) bookkeeping (slots, segments, the tree) and reads back one
+// small array of per-feature split candidates per step.
+//
+// Differences of mechanism (same results):
+//   - rows are physically partitioned in HBM, so every leaf is a contiguous segment;
+//   - histograms carry an exact row count per bin, so both children's sizes are
+//     known before their rows move and the smaller child's histogram is built by
+//     the partition pass itself;
+//   - the quantized root sums (serial.rs:711-740) are taken from exact integer sums
+//     when every sequential partial sum is representable in f64 (then the
+//     sequential f64 result equals the exact one), else summed on the host in
+//     row order.
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bankai_gpu.h"
+#include "bk_common.h"
+#include "kernels.h"
+
+#define BK_HIPC(x)                                                                                          \
+  do {                                                                                                      \
+    hipError_t e_ = (x);                                                                                    \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " in " #x); \
+  } while (0)
+
+namespace bk {
+
+std::string& last_error() {
+  static thread_local std::string e;
+  return e;
+}
+
+namespace {
+
+const double kEps = (double)1e-15f;
+const double kZeroThreshold = (double)1e-35f;
+const double kMinScore = -std::numeric_limits<double>::infinity();
+
+// std::mt19937 with libc++'s uniform_int_distribution (gradient_discretizer/mt19937.rs:48-128);
+// the learner draws one rotation offset per tree on the host.
+struct Mt {
+  uint32_t st[624];
+  int i = 624;
+  explicit Mt(uint32_t seed = 5489u) { seed_with(seed); }
+  void seed_with(uint32_t seed) {
+    st[0] = seed;
+    for (int k = 1; k < 624; ++k) st[k] = 1812433253u * (st[k - 1] ^ (st[k - 1] >> 30)) + (uint32_t)k;
+    i = 624;
+  }
+  uint32_t next() {
+    if (i >= 624) {
+      for (int k = 0; k < 624; ++k) {
+        uint32_t y = (st[k] & 0x80000000u) | (st[(k + 1) % 624] & 0x7fffffffu);
+        uint32_t v = st[(k + 397) % 624] ^ (y >> 1);
+        if (y & 1u) v ^= 0x9908b0dfu;
+        st[k] = v;
+      }
+      i = 0;
+    }
+    uint32_t y = st[i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
+  double real01() {
+    double a = (double)next(), b = (double)next();
+    const double rp = 4294967296.0;
+    return (a + b * rp) / (rp * rp);
+  }
+  uint32_t uint_incl(uint32_t n) {
+    uint32_t rp = n + 1u;
+    if (rp == 1u) return 0;
+    if (rp == 0u) return next();
+    uint32_t w = 32u - (uint32_t)__builtin_clz(rp) - 1u;
+    if (rp & (0xffffffffu >> (32u - w))) w += 1u;
+    uint32_t mask = w >= 32u ? 0xffffffffu : ((1u << w) - 1u);
+    for (;;) {
+      uint32_t u = next() & mask;
+      if (u < rp) return u;
+    }
+  }
+};
+
+inline double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
+// CalculateSplittedLeafOutput<L1=true, MAX=true, SMOOTH=false> with parent 0
+// (feature_histogram.rs:76-101), as used for the root and for renewed leaves.
+double leaf_output_plain(double g, double h, double l1, double l2, double mds) {
+  double reg = std::max(std::fabs(g) - l1, 0.0);
+  double ret = -(sgn(g) * reg) / (h + l2);
+  if (mds > 0.0 && std::fabs(ret) > mds) ret = sgn(ret) * mds;
+  return ret;
+}
+double leaf_output_root(const bk_learner_params& p, double g, double h) {
+  double r = leaf_output_plain(g, h, p.lambda_l1, p.lambda_l2, p.max_delta_step);
+  if (r < -std::numeric_limits<double>::max()) return -std::numeric_limits<double>::max();
+  if (r > std::numeric_limits<double>::max()) return std::numeric_limits<double>::max();
+  return r;
+}
+
+// HistogramPool bookkeeping (serial.rs:80-162); the data lives on the device.
+struct PoolMap {
+  std::vector<std::vector<char>> splittable;
+  std::vector<int> leaf_to_slot, slot_to_leaf;
+  size_t next_free = 0;
+  std::vector<uint64_t> last_used;
+  uint64_t clock = 0;
+  void init(size_t nl, size_t cache, size_t nf) {
+    splittable.assign(cache, std::vector<char>(nf, 1));
+    leaf_to_slot.assign(nl, -1);
+    slot_to_leaf.assign(cache, -1);
+    last_used.assign(cache < nl ? cache : 0, 0);
+  }
+  size_t size() const { return splittable.size(); }
+  void reset() {
+    std::fill(leaf_to_slot.begin(), leaf_to_slot.end(), -1);
+    std::fill(slot_to_leaf.begin(), slot_to_leaf.end(), -1);
+    next_free = 0;
+    std::fill(last_used.begin(), last_used.end(), 0);
+    clock = 0;
+  }
+  void touch(size_t s) {
+    if (!last_used.empty()) last_used[s] = ++clock;
+  }
+  std::pair<size_t, bool> get(size_t leaf) {
+    if (leaf_to_slot[leaf] >= 0) {
+      size_t s = (size_t)leaf_to_slot[leaf];
+      touch(s);
+      return {s, true};
+    }
+    size_t s;
+    if (next_free < size()) {
+      s = next_free++;
+    } else {
+      size_t oldest = 0;
+      for (size_t k = 1; k < last_used.size(); ++k)
+        if (last_used[k] < last_used[oldest]) oldest = k;
+      if (slot_to_leaf[oldest] >= 0) leaf_to_slot[(size_t)slot_to_leaf[oldest]] = -1;
+      s = oldest;
+    }
+    touch(s);
+    leaf_to_slot[leaf] = (int)s;
+    slot_to_leaf[s] = (int)leaf;
+    return {s, false};
+  }
+  void move_leaf(size_t src, size_t dst) {
+    int s = leaf_to_slot[src];
+    if (s < 0) return;
+    leaf_to_slot[src] = -1;
+    if (leaf_to_slot[dst] >= 0) slot_to_leaf[(size_t)leaf_to_slot[dst]] = -1;
+    leaf_to_slot[dst] = s;
+    slot_to_leaf[(size_t)s] = (int)dst;
+    touch((size_t)s);
+  }
+};
+
+// SplitInfo on the host (split_info.rs:9-85) with the exact child counts.
+struct HostSplit {
+  SplitOut o{};
+  int inner = -1;
+  HostSplit() {
+    o.gain = kMinScore;
+    o.feature = -1;
+    o.default_left = 1;
+  }
+  bool better_than(const HostSplit& b) const {
+    double x = o.gain, y = b.o.gain;
+    if (x != y) return x > y;
+    int fa = o.feature == -1 ? INT32_MAX : o.feature;
+    int fb = b.o.feature == -1 ? INT32_MAX : b.o.feature;
+    return fa < fb;
+  }
+  void reset() {
+    o.feature = -1;
+    o.gain = kMinScore;
+  }
+};
+
+struct LeafSplits {
+  int leaf = -1, num_data = 0;
+  double sum_g = 0, sum_h = 0, weight = 0;
+  long long int_sum = 0;
+};
+
+// Training fields of Tree (tree.rs:259-445).
+struct Tree {
+  int num_leaves = 1, num_cat = 0;
+  std::vector<int32_t> left, right, sf_inner, sf, leaf_parent, leaf_depth, leaf_count, internal_count, cat_bound;
+  std::vector<uint32_t> th_bin, cat_th;
+  std::vector<double> th, leaf_value, leaf_weight, internal_value, internal_weight;
+  std::vector<float> gain;
+  std::vector<int8_t> dtype;
+  explicit Tree(int ml) {
+    size_t ni = ml > 1 ? (size_t)ml - 1 : 0, nl = (size_t)ml;
+    left.assign(ni, 0);
+    right.assign(ni, 0);
+    sf_inner.assign(ni, 0);
+    sf.assign(ni, 0);
+    gain.assign(ni, 0.0f);
+    th.assign(ni, 0.0);
+    th_bin.assign(ni, 0);
+    dtype.assign(ni, 0);
+    internal_value.assign(ni, 0.0);
+    internal_weight.assign(ni, 0.0);
+    internal_count.assign(ni, 0);
+    leaf_value.assign(nl, 0.0);
+    leaf_weight.assign(nl, 0.0);
+    leaf_count.assign(nl, 0);
+    leaf_parent.assign(nl, 0);
+    if (nl) leaf_parent[0] = -1;
+    leaf_depth.assign(nl, 0);
+    cat_bound.assign(1, 0);
+  }
+  static double round0(double v) { return (v >= -kZeroThreshold && v <= kZeroThreshold) ? 0.0 : v; }
+  void set_leaf_output(int l, double v) { leaf_value[(size_t)l] = round0(v); }
+  int split_common(int leaf, int feature, int real, double lv, double rv, int lc, int rc, double lw, double rw,
+                   float g) {
+    int node = num_leaves - 1, nl = num_leaves;
+    int parent = leaf_parent[(size_t)leaf];
+    if (parent >= 0) {
+      if (left[(size_t)parent] == ~leaf) left[(size_t)parent] = node;
+      else right[(size_t)parent] = node;
+    }
+    sf_inner[(size_t)node] = feature;
+    sf[(size_t)node] = real;
+    gain[(size_t)node] = g;
+    left[(size_t)node] = ~leaf;
+    right[(size_t)node] = ~nl;
+    leaf_parent[(size_t)leaf] = node;
+    leaf_parent[(size_t)nl] = node;
+    internal_weight[(size_t)node] = lw + rw;
+    internal_value[(size_t)node] = leaf_value[(size_t)leaf];
+    internal_count[(size_t)node] = lc + rc;
+    leaf_value[(size_t)leaf] = std::isnan(lv) ? 0.0 : lv;
+    leaf_weight[(size_t)leaf] = lw;
+    leaf_count[(size_t)leaf] = lc;
+    leaf_value[(size_t)nl] = std::isnan(rv) ? 0.0 : rv;
+    leaf_weight[(size_t)nl] = rw;
+    leaf_count[(size_t)nl] = rc;
+    leaf_depth[(size_t)nl] = leaf_depth[(size_t)leaf] + 1;
+    leaf_depth[(size_t)leaf] += 1;
+    return node;
+  }
+};
+
+std::vector<uint32_t> bitset_of(const std::vector<uint32_t>& vals) {
+  std::vector<uint32_t> r;
+  for (uint32_t v : vals) {
+    size_t i1 = v / 32;
+    if (r.size() < i1 + 1) r.resize(i1 + 1, 0);
+    r[i1] |= 1u << (v % 32);
+  }
+  return r;
+}
+
+template <typename T>
+T* dmalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  BK_HIPC(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+
+}  // namespace
+
+struct Learner {
+  bk_learner_params P{};
+  int N = 0, npad = 0, G = 0, F = 0, TB = 0;
+  int form = FORM_F64;
+  bool quant = false, const_hess = false, stochastic = false, renew = false;
+  int bins = 4;
+  std::vector<FeatDev> feats;
+  std::vector<int> group_bin_start;
+  std::vector<std::vector<double>> upper;
+  std::vector<std::vector<int32_t>> bin2cat;
+  // group tiles for LDS
+  std::vector<int> tile_first;
+  int max_tile_bins = 0;
+  int rows_per_flush = 0;
+  size_t cache = 0;
+
+  // device memory
+  hipStream_t stream = nullptr;
+  DevState ds{};
+  uint8_t* d_cols[3] = {nullptr, nullptr, nullptr};
+  int32_t* d_idx[3] = {nullptr, nullptr, nullptr};
+  void* d_gh[3] = {nullptr, nullptr, nullptr};
+  float *d_grad = nullptr, *d_hess = nullptr;
+  double *d_gnoise = nullptr, *d_hnoise = nullptr;
+  FeatDev* d_feats = nullptr;
+  int* d_group_bin_start = nullptr;
+  double* d_pool = nullptr;
+  int* d_pool_cnt = nullptr;
+  unsigned long long* d_part_q[2] = {nullptr, nullptr};
+  double* d_part_g[2] = {nullptr, nullptr};
+  double* d_part_h[2] = {nullptr, nullptr};
+  unsigned* d_part_c[2] = {nullptr, nullptr};
+  int* d_blk_cnt = nullptr;
+  int* d_err = nullptr;
+  int* d_tile_first = nullptr;
+  int* d_used = nullptr;        // per feature
+  int* d_group_used = nullptr;  // per group
+  SplitOut* d_out = nullptr;
+  unsigned* d_max4 = nullptr;
+  long long* d_sums3 = nullptr;
+  double* d_sum_partials = nullptr;
+  Segment* d_segs = nullptr;
+  double* d_renew = nullptr;
+  double* d_leaf_vals = nullptr;
+  int32_t* d_leaf_of = nullptr;
+  // pinned host staging
+  SplitOut* h_out = nullptr;
+
+  // per-tree state (serial.rs:183-245)
+  PoolMap pool;
+  std::vector<HostSplit> best;
+  LeafSplits smaller, larger;
+  size_t smaller_slot = 0;
+  long larger_slot = -1, parent_slot = -1;
+  std::vector<Segment> segs;
+  Mt use_start_eng{0};
+  double grad_scale = 0.0, hess_scale = 0.0;
+  long long last_int_g = 0, last_int_h = 0;
+  double last_root_g = 0.0, last_root_h = 0.0;
+  bk_train_stats stats{};
+  std::vector<int> used_feat, used_group;
+  int last_num_leaves = 1;
+  bool pending = false;
+  bool pending_has_large = false, pending_subtract = false;
+  LeafArgs pend_small{}, pend_large{};
+  double h0 = 1.0;
+
+  ~Learner() { release(); }
+
+  void release() {
+    auto f = [](void* p) {
+      if (p) (void)hipFree(p);
+    };
+    for (int b = 0; b < 3; ++b) {
+      f(d_cols[b]);
+      f(d_idx[b]);
+      f(d_gh[b]);
+    }
+    f(d_grad);
+    f(d_hess);
+    f(d_gnoise);
+    f(d_hnoise);
+    f(d_feats);
+    f(d_group_bin_start);
+    f(d_pool);
+    f(d_pool_cnt);
+    for (int k = 0; k < 2; ++k) {
+      f(d_part_q[k]);
+      f(d_part_g[k]);
+      f(d_part_h[k]);
+      f(d_part_c[k]);
+    }
+    f(d_blk_cnt);
+    f(d_err);
+    f(d_tile_first);
+    f(d_used);
+    f(d_group_used);
+    f(d_out);
+    f(d_max4);
+    f(d_sums3);
+    f(d_sum_partials);
+    f(d_segs);
+    f(d_renew);
+    f(d_leaf_vals);
+    f(d_leaf_of);
+    if (h_out) (void)hipHostFree(h_out);
+    if (stream) (void)hipStreamDestroy(stream);
+    std::memset(d_cols, 0, sizeof d_cols);
+    stream = nullptr;
+    h_out = nullptr;
+  }
+
+  BlockPlan plan(int count) const {
+    BlockPlan bp{0, 0};
+    if (count <= 0) return bp;
+    const int min_chunk = 4 * kTileRows;
+    long long chunk = ((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks;
+    chunk = std::max<long long>(chunk, min_chunk);
+    chunk = (chunk + kTileRows - 1) / kTileRows * kTileRows;
+    bp.chunk = (int)chunk;
+    bp.nblk = (int)(((long long)count + chunk - 1) / chunk);
+    return bp;
+  }
+
+  void create(const bk_dataset_view* v, const bk_learner_params* p, int is_const_hess) {
+    P = *p;
+    if (P.extra_trees) throw std::invalid_argument("extra_trees is not supported by the GPU learner");
+    if (v->num_data <= 0) throw std::invalid_argument("empty dataset");
+    if (v->num_data > (1LL << 30)) throw std::invalid_argument("more than 2^30 rows");
+    if (P.num_leaves < 2) throw std::invalid_argument("num_leaves must be at least 2");
+    N = (int)v->num_data;
+    npad = (N + 255) / 256 * 256;
+    G = v->num_groups;
+    F = v->num_features;
+    const_hess = is_const_hess != 0;
+    quant = P.use_quantized_grad && P.num_grad_quant_bins >= 2 && P.num_grad_quant_bins <= 127;
+    if (P.form == BK_FORM_QUANTIZED && !quant)
+      throw std::invalid_argument("form=quantized needs use_quantized_grad and 2 <= num_grad_quant_bins <= 127");
+    form = quant ? FORM_QUANT : (P.form == BK_FORM_F32_COMPENSATED ? FORM_F32C : FORM_F64);
+    bins = P.num_grad_quant_bins;
+    stochastic = quant && P.stochastic_rounding;
+    renew = quant && P.quant_train_renew_leaf;
+    // groups and features
+    group_bin_start.assign((size_t)G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+      int nb = v->group_num_total_bin[g];
+      if (nb > kMaxBins || nb < 1) throw std::invalid_argument("group bins must be in 1..256 (one byte per row)");
+      group_bin_start[(size_t)g + 1] = group_bin_start[(size_t)g] + nb;
+    }
+    TB = group_bin_start[(size_t)G];
+    feats.assign((size_t)F, FeatDev{});
+    upper.assign((size_t)F, {});
+    bin2cat.assign((size_t)F, {});
+    for (int g = 0; g < G; ++g) {
+      const int fs = v->group_feature_start[g], fc = v->group_feature_cnt[g];
+      const int32_t* bo = v->group_bin_offsets + fs + g;
+      for (int s = 0; s < fc; ++s) {
+        int f = fs + s;
+        FeatDev& m = feats[(size_t)f];
+        m.num_bin = v->feature_num_bin[f];
+        m.missing_type = v->feature_missing_type[f];
+        m.default_bin = v->feature_default_bin[f];
+        m.mfb = v->feature_most_freq_bin[f];
+        m.offset = m.mfb == 0 ? 1 : 0;
+        m.bin_type = v->feature_bin_type[f];
+        m.group = g;
+        m.bin_base = bo[s];
+        m.hist_off = group_bin_start[(size_t)g] + bo[s];
+        m.hist_len = m.num_bin - m.offset;
+        if (fc == 1) {
+          m.minb = 1;
+          m.maxb = bo[1] - 1;
+          m.use_min_bin = 0;
+        } else {
+          m.minb = bo[s];
+          m.maxb = bo[s + 1] - 1;
+          m.use_min_bin = 1;
+        }
+        m.real = v->feature_real_index[f];
+        m.penalty = v->feature_penalty ? v->feature_penalty[f] : 1.0;
+        if (m.num_bin > kMaxBins) throw std::invalid_argument("feature with more than 256 bins");
+        if (m.bin_type == BK_BIN_CATEGORICAL) {
+          if (!v->feature_bin_2_categorical || !v->feature_bin_2_categorical[f])
+            throw std::invalid_argument("categorical feature without bin_2_categorical");
+          bin2cat[(size_t)f].assign(v->feature_bin_2_categorical[f], v->feature_bin_2_categorical[f] + m.num_bin);
+        } else {
+          if (!v->feature_bin_upper_bound || !v->feature_bin_upper_bound[f])
+            throw std::invalid_argument("numerical feature without bin_upper_bound");
+          upper[(size_t)f].assign(v->feature_bin_upper_bound[f], v->feature_bin_upper_bound[f] + m.num_bin);
+        }
+      }
+    }
+    // LDS tiles of whole groups
+    const int per_bin = form == FORM_QUANT ? 8 : 20;
+    const int tile_cap = kLdsBytes / per_bin;
+    tile_first.assign(1, 0);
+    max_tile_bins = 0;
+    int cur = 0;
+    for (int g = 0; g < G; ++g) {
+      int nb = v->group_num_total_bin[g];
+      if (cur + nb > tile_cap) {
+        tile_first.push_back(g);
+        max_tile_bins = std::max(max_tile_bins, cur);
+        cur = 0;
+      }
+      cur += nb;
+    }
+    tile_first.push_back(G);
+    max_tile_bins = std::max(max_tile_bins, cur);
+    // rows one LDS accumulation may hold: the packed quantized fields stay exact
+    if (form == FORM_QUANT) {
+      int maxq = bins + 1;
+      rows_per_flush = std::max(kTileRows, ((1 << 20) - 1) / maxq / kTileRows * kTileRows);
+    } else {
+      rows_per_flush = 1 << 30;
+    }
+    cache = (size_t)P.num_leaves;
+    if (P.histogram_pool_size > 0.0) {
+      size_t bytes = 0;
+      for (auto& m : feats) bytes += (size_t)m.num_bin * 16;
+      size_t c = (size_t)(P.histogram_pool_size * 1024.0 * 1024.0 / (double)bytes);
+      cache = std::min(std::max(c, (size_t)2), (size_t)P.num_leaves);
+    }
+    // device allocation
+    BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (int b = 0; b < 3; ++b) {
+      d_cols[b] = dmalloc<uint8_t>((size_t)G * npad);
+      if (b > 0) d_idx[b] = dmalloc<int32_t>((size_t)npad);
+      d_gh[b] = form == FORM_QUANT ? (void*)dmalloc<int16_t>((size_t)npad) : (void*)dmalloc<float>((size_t)npad * 2);
+    }
+    BK_HIPC(hipMemsetAsync(d_gh[0], 0, (size_t)npad * (form == FORM_QUANT ? 2 : 8), stream));
+    for (int g = 0; g < G; ++g) {
+      BK_HIPC(hipMemcpyAsync(d_cols[0] + (size_t)g * npad, v->group_bins[g], (size_t)N, hipMemcpyHostToDevice, stream));
+      if (npad > N) BK_HIPC(hipMemsetAsync(d_cols[0] + (size_t)g * npad + N, 0, (size_t)(npad - N), stream));
+    }
+    d_grad = dmalloc<float>((size_t)npad);
+    d_hess = dmalloc<float>((size_t)npad);
+    d_feats = dmalloc<FeatDev>((size_t)F);
+    BK_HIPC(hipMemcpyAsync(d_feats, feats.data(), sizeof(FeatDev) * (size_t)F, hipMemcpyHostToDevice, stream));
+    d_group_bin_start = dmalloc<int>((size_t)G + 1);
+    BK_HIPC(hipMemcpyAsync(d_group_bin_start, group_bin_start.data(), sizeof(int) * (size_t)(G + 1),
+                           hipMemcpyHostToDevice, stream));
+    d_pool = dmalloc<double>(cache * (size_t)TB * 2);
+    d_pool_cnt = dmalloc<int>(cache * (size_t)TB);
+    for (int k = 0; k < 2; ++k) {
+      if (form == FORM_QUANT) {
+        d_part_q[k] = dmalloc<unsigned long long>((size_t)kMaxRowBlocks * TB);
+      } else {
+        d_part_g[k] = dmalloc<double>((size_t)kMaxRowBlocks * TB);
+        d_part_h[k] = dmalloc<double>((size_t)kMaxRowBlocks * TB);
+      }
+      if (form != FORM_QUANT) d_part_c[k] = dmalloc<unsigned>((size_t)kMaxRowBlocks * TB);
+    }
+    d_blk_cnt = dmalloc<int>((size_t)kMaxRowBlocks);
+    d_err = dmalloc<int>(1);
+    BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
+    d_tile_first = dmalloc<int>(tile_first.size());
+    BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
+                           stream));
+    d_used = dmalloc<int>((size_t)F);
+    d_group_used = dmalloc<int>((size_t)G);
+    d_out = dmalloc<SplitOut>((size_t)F * 2);
+    d_max4 = dmalloc<unsigned>(4);
+    d_sums3 = dmalloc<long long>(4);
+    d_sum_partials = dmalloc<double>(2048 + 2);
+    d_segs = dmalloc<Segment>((size_t)P.num_leaves);
+    d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
+    d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
+    BK_HIPC(hipHostMalloc((void**)&h_out, sizeof(SplitOut) * (size_t)F * 2, hipHostMallocDefault));
+    if (stochastic) {
+      // init_stochastic (gradient_discretizer/mod.rs:87-106): per-block MT19937 noise
+      std::vector<double> gn((size_t)N), hn((size_t)N);
+      size_t threads = P.num_threads > 0 ? (size_t)P.num_threads : 1;
+      size_t nb = std::min(threads, ((size_t)N + 511) / 512);
+      size_t bs = (size_t)N;
+      if (nb > 1) {
+        size_t s = ((size_t)N + nb - 1) / nb;
+        bs = (s + 31) / 32 * 32;
+      }
+      for (size_t b = 0; b < nb; ++b) {
+        size_t start = b * bs, end = std::min(start + bs, (size_t)N);
+        Mt ge((uint32_t)P.seed + (uint32_t)b), he((uint32_t)P.seed + (uint32_t)b + (uint32_t)threads);
+        for (size_t i = start; i < end; ++i) {
+          gn[i] = ge.real01();
+          hn[i] = he.real01();
+        }
+      }
+      d_gnoise = dmalloc<double>((size_t)N);
+      d_hnoise = dmalloc<double>((size_t)N);
+      BK_HIPC(hipMemcpyAsync(d_gnoise, gn.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice, stream));
+      BK_HIPC(hipMemcpyAsync(d_hnoise, hn.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice, stream));
+      BK_HIPC(hipStreamSynchronize(stream));
+      use_start_eng.seed_with((uint32_t)P.seed);
+    }
+    BK_HIPC(hipStreamSynchronize(stream));
+    // kernel view
+    ds.num_data = N;
+    ds.npad = npad;
+    ds.num_groups = G;
+    ds.num_features = F;
+    ds.total_bins = TB;
+    ds.form = form;
+    for (int b = 0; b < 3; ++b) {
+      ds.cols[b] = d_cols[b];
+      ds.idx[b] = d_idx[b];
+      ds.gh[b] = d_gh[b];
+    }
+    ds.feats = d_feats;
+    ds.group_bin_start = d_group_bin_start;
+    ds.pool = d_pool;
+    ds.pool_cnt = d_pool_cnt;
+    for (int k = 0; k < 2; ++k) {
+      ds.part_q[k] = d_part_q[k];
+      ds.part_g[k] = d_part_g[k];
+      ds.part_h[k] = d_part_h[k];
+      ds.part_c[k] = d_part_c[k];
+    }
+    ds.blk_cnt = d_blk_cnt;
+    ds.err_flag = d_err;
+    pool.init((size_t)P.num_leaves, cache, (size_t)F);
+    best.assign((size_t)P.num_leaves, HostSplit());
+    segs.assign((size_t)P.num_leaves, Segment{0, 0, 0});
+    used_feat.assign((size_t)F, 1);
+    used_group.assign((size_t)G, 1);
+  }
+
+  ScanParams scan_params() const {
+    ScanParams sp{};
+    sp.l1 = P.lambda_l1;
+    sp.l2 = P.lambda_l2;
+    sp.max_delta_step = P.max_delta_step;
+    sp.path_smooth = P.path_smooth;
+    sp.min_sum_hessian = P.min_sum_hessian_in_leaf;
+    sp.min_gain_to_split = P.min_gain_to_split;
+    sp.cat_smooth = P.cat_smooth;
+    sp.cat_l2 = P.cat_l2;
+    sp.min_data_in_leaf = P.min_data_in_leaf;
+    sp.max_cat_threshold = P.max_cat_threshold;
+    sp.max_cat_to_onehot = P.max_cat_to_onehot;
+    sp.min_data_per_group = P.min_data_per_group;
+    sp.grad_scale = grad_scale;
+    sp.hess_scale = hess_scale;
+    sp.h0 = h0;
+    sp.const_hess = const_hess ? 1 : 0;
+    return sp;
+  }
+
+  // Discretization and root sums (serial.rs:658-776, gradient_discretizer/mod.rs:134-201).
+  void before_train(const float* g, const float* h) {
+    pool.reset();
+    for (auto& s : best) s.reset();
+    std::fill(segs.begin(), segs.end(), Segment{0, 0, 0});
+    segs[0] = Segment{0, 0, N};
+    stats = bk_train_stats{};
+    double sg = 0.0, sh = 0.0;
+    long long isum = 0;
+    if (quant) {
+      BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
+      launch_max_abs(g, h, N, d_max4, stream);
+      unsigned mx[4];
+      BK_HIPC(hipMemcpyAsync(mx, d_max4, sizeof mx, hipMemcpyDeviceToHost, stream));
+      BK_HIPC(hipStreamSynchronize(stream));
+      float max_g, max_h;
+      std::memcpy(&max_g, &mx[0], 4);
+      std::memcpy(&max_h, &mx[1], 4);
+      bool ok = mx[2] == 0u && max_g != 0.0f && (const_hess || max_h != 0.0f);
+      if (ok) {
+        grad_scale = (double)max_g / (double)(bins / 2);
+        hess_scale = const_hess ? (double)max_h : (double)max_h / (double)bins;
+        double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
+        int start = stochastic ? (int)use_start_eng.uint_incl((uint32_t)N) : 0;
+        BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
+        launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0, stochastic ? 1 : 0,
+                          (int16_t*)d_gh[0], d_sums3, stream);
+        long long sums[4];
+        BK_HIPC(hipMemcpyAsync(sums, d_sums3, sizeof sums, hipMemcpyDeviceToHost, stream));
+        BK_HIPC(hipStreamSynchronize(stream));
+        last_int_g = sums[0];
+        last_int_h = sums[1];
+        // exact when every sequential partial sum fits the f64 significand
+        auto exact = [](long long sum_abs, double scale) {
+          if (scale == 0.0) return true;
+          int e;
+          double m = std::frexp(scale, &e);
+          uint64_t mant = (uint64_t)std::ldexp(m, 53);
+          while (mant && !(mant & 1ull)) mant >>= 1;
+          unsigned __int128 prod = (unsigned __int128)(uint64_t)sum_abs * mant;
+          return prod < ((unsigned __int128)1 << 53);
+        };
+        if (exact(sums[2], grad_scale) && exact(sums[1], hess_scale)) {
+          last_root_g = (double)sums[0] * grad_scale;
+          last_root_h = (double)sums[1] * hess_scale;
+        } else {
+          std::vector<int16_t> pk((size_t)N);
+          BK_HIPC(hipMemcpyAsync(pk.data(), d_gh[0], sizeof(int16_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+          BK_HIPC(hipStreamSynchronize(stream));
+          double a = 0.0, b = 0.0;
+          for (int i = 0; i < N; ++i) {
+            a += (double)(int8_t)(pk[(size_t)i] >> 8) * grad_scale;
+            b += (double)(uint8_t)(pk[(size_t)i] & 0xff) * hess_scale;
+          }
+          last_root_g = a;
+          last_root_h = b;
+        }
+      }
+      // on refusal the previous discretization and its sums are kept (serial.rs:692-703)
+      sg = last_root_g;
+      sh = last_root_h;
+      isum = (long long)(((unsigned long long)last_int_g << 32) + (unsigned long long)last_int_h);
+    } else {
+      launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh[0], stream);
+      int nb = launch_sum_gh(g, h, N, d_sum_partials, stream);
+      std::vector<double> part((size_t)nb * 2);
+      float hh0 = 0.0f;
+      BK_HIPC(hipMemcpyAsync(part.data(), d_sum_partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
+      BK_HIPC(hipMemcpyAsync(&hh0, h, sizeof(float), hipMemcpyDeviceToHost, stream));
+      BK_HIPC(hipStreamSynchronize(stream));
+      for (int b = 0; b < nb; ++b) {
+        sg += part[(size_t)2 * b];
+        sh += part[(size_t)2 * b + 1];
+      }
+      h0 = (double)hh0;
+    }
+    double carried = smaller.weight;
+    smaller = LeafSplits{0, N, sg, sh, carried, isum};
+    larger = LeafSplits{};
+  }
+
+  // before_find_best_split (serial.rs:779-825)
+  bool before_find_best_split(const Tree& t, int left_leaf, int right_leaf) {
+    if (P.max_depth > 0 && t.leaf_depth[(size_t)left_leaf] >= P.max_depth) {
+      best[(size_t)left_leaf].o.gain = kMinScore;
+      if (right_leaf >= 0) best[(size_t)right_leaf].o.gain = kMinScore;
+      return false;
+    }
+    int nl = left_leaf >= 0 ? segs[(size_t)left_leaf].count : 0;
+    int nr = right_leaf >= 0 ? segs[(size_t)right_leaf].count : 0;
+    if (nr < P.min_data_in_leaf * 2 && nl < P.min_data_in_leaf * 2) {
+      best[(size_t)left_leaf].o.gain = kMinScore;
+      if (right_leaf >= 0) best[(size_t)right_leaf].o.gain = kMinScore;
+      return false;
+    }
+    parent_slot = -1;
+    if (right_leaf < 0) {
+      smaller_slot = pool.get((size_t)left_leaf).first;
+      larger_slot = -1;
+    } else if (nl < nr) {
+      auto lg = pool.get((size_t)left_leaf);
+      if (lg.second) parent_slot = (long)lg.first;
+      larger_slot = (long)lg.first;
+      pool.move_leaf((size_t)left_leaf, (size_t)right_leaf);
+      smaller_slot = pool.get((size_t)left_leaf).first;
+    } else {
+      auto lg = pool.get((size_t)left_leaf);
+      if (lg.second) parent_slot = (long)lg.first;
+      larger_slot = (long)lg.first;
+      smaller_slot = pool.get((size_t)right_leaf).first;
+    }
+    return true;
+  }
+
+  double parent_output(const Tree& t, const LeafSplits& ls) const {
+    if (t.num_leaves == 1) return leaf_output_root(P, ls.sum_g, ls.sum_h);
+    return ls.weight;
+  }
+
+  // Feature usage from the parent's splittable flags (serial.rs:838-852).
+  void compute_used() {
+    bool changed = false;
+    std::vector<int> uf((size_t)F, 1);
+    for (int f = 0; f < F; ++f) {
+      if (parent_slot >= 0 && !pool.splittable[(size_t)parent_slot][(size_t)f]) {
+        pool.splittable[smaller_slot][(size_t)f] = 0;
+        uf[(size_t)f] = 0;
+      }
+    }
+    std::vector<int> ug((size_t)G, 0);
+    for (int f = 0; f < F; ++f)
+      if (uf[(size_t)f]) ug[(size_t)feats[(size_t)f].group] = 1;
+    if (uf != used_feat) {
+      used_feat = uf;
+      changed = true;
+    }
+    if (ug != used_group) used_group = ug, changed = true;
+    (void)changed;
+    BK_HIPC(hipMemcpyAsync(d_used, used_feat.data(), sizeof(int) * (size_t)F, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_group_used, used_group.data(), sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
+  }
+
+  LeafArgs leaf_args(const Tree& t, const LeafSplits& ls, size_t slot, int part_set, int nblk) const {
+    LeafArgs a{};
+    a.leaf = ls.leaf;
+    a.slot = (int)slot;
+    a.num_data = ls.num_data;
+    a.sum_g = ls.sum_g;
+    a.sum_h = ls.sum_h;
+    a.parent_output = parent_output(t, ls);
+    a.int_sum = ls.int_sum;
+    a.from_partials = 1;
+    a.part_set = part_set;
+    a.nblk = nblk;
+    return a;
+  }
+
+  // Runs the pending search (fix/subtract/scan), reads back the per-feature
+  // candidates and folds them (serial.rs:1500-1566).
+  void run_search() {
+    if (!pending) return;
+    pending = false;
+    ScanParams sp = scan_params();
+    launch_scan(ds, pend_small, pending_has_large ? &pend_large : nullptr, pending_subtract ? 1 : 0, sp, d_used, d_out,
+                stream);
+    BK_HIPC(hipMemcpyAsync(h_out, d_out, sizeof(SplitOut) * (size_t)F * 2, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
+    HostSplit sb, lb;
+    for (int f = 0; f < F; ++f) {
+      if (!used_feat[(size_t)f]) continue;
+      HostSplit c;
+      c.o = h_out[f];
+      c.inner = f;
+      pool.splittable[smaller_slot][(size_t)f] = c.o.splittable ? 1 : 0;
+      if (c.better_than(sb)) sb = c;
+      if (pending_has_large) {
+        HostSplit d;
+        d.o = h_out[F + f];
+        d.inner = f;
+        pool.splittable[(size_t)larger_slot][(size_t)f] = d.o.splittable ? 1 : 0;
+        if (d.better_than(lb)) lb = d;
+      }
+    }
+    best[(size_t)smaller.leaf] = sb;
+    if (larger.leaf >= 0) best[(size_t)larger.leaf] = lb;
+  }
+
+  // find_best_splits for the root (serial.rs:830-887): build the root histogram.
+  void prepare_root_search(const Tree& t) {
+    compute_used();
+    BlockPlan bp = plan(N);
+    launch_hist(ds, segs[0], bp, d_group_used, 0, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
+                rows_per_flush, stream);
+    stats.hist_rows += N;
+    pend_small = leaf_args(t, smaller, smaller_slot, 0, bp.nblk);
+    pending_has_large = false;
+    pending_subtract = false;
+    pending = true;
+  }
+
+  // split (serial.rs:1685-1876) plus the next step's before_find_best_split, with
+  // the smaller child's histogram built by the partition pass.
+  std::pair<int, int> split(Tree& t, int best_leaf) {
+    HostSplit info = best[(size_t)best_leaf];
+    const int inner = info.inner;
+    const FeatDev& m = feats[(size_t)inner];
+    const float gain = (float)(info.o.gain + P.min_gain_to_split);
+    const int lc = info.o.exact_left, rc = info.o.exact_right;
+    RuleDev rule;
+    if (m.bin_type == BK_BIN_CATEGORICAL) {
+      std::vector<uint32_t> bins_left, real;
+      for (uint32_t b = 0; b < 256; ++b)
+        if ((info.o.cat_bits[b / 32] >> (b % 32)) & 1u) bins_left.push_back(b);
+      for (uint32_t b : bins_left) real.push_back((uint32_t)bin2cat[(size_t)inner][b]);
+      std::vector<uint32_t> bits_real = bitset_of(real);
+      rule = make_rule_categorical(m, info.o.cat_bits);
+      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
+                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
+      t.dtype[(size_t)node] = (int8_t)(1 | (m.missing_type << 2));
+      t.th_bin[(size_t)node] = (uint32_t)t.num_cat;
+      t.th[(size_t)node] = (double)t.num_cat;
+      t.num_cat += 1;
+      t.cat_bound.push_back(t.cat_bound.back() + (int)bits_real.size());
+      t.cat_th.insert(t.cat_th.end(), bits_real.begin(), bits_real.end());
+      t.num_leaves += 1;
+    } else {
+      rule = make_rule_numerical(m, (uint32_t)info.o.threshold, info.o.default_left != 0);
+      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
+                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
+      int8_t dt = 0;
+      if (info.o.default_left) dt |= 2;
+      dt = (int8_t)((dt & 3) | (m.missing_type << 2));
+      t.dtype[(size_t)node] = dt;
+      t.th_bin[(size_t)node] = (uint32_t)info.o.threshold;
+      t.th[(size_t)node] = upper[(size_t)inner][(size_t)info.o.threshold];
+      t.num_leaves += 1;
+    }
+    const int right = t.num_leaves - 1;
+    LeafSplits l{best_leaf, lc, info.o.left_sum_gradient, info.o.left_sum_hessian, info.o.left_output,
+                 info.o.left_int_sum};
+    LeafSplits r{right, rc, info.o.right_sum_gradient, info.o.right_sum_hessian, info.o.right_output,
+                 info.o.right_int_sum};
+    if (lc < rc) {
+      smaller = l;
+      larger = r;
+    } else {
+      smaller = r;
+      larger = l;
+    }
+    best[(size_t)best_leaf] = info;
+    // the rows move now; both children's sizes are already known
+    const Segment ps = segs[(size_t)best_leaf];
+    const int dst = ps.buf == 1 ? 2 : 1;
+    segs[(size_t)best_leaf] = Segment{dst, ps.begin, lc};
+    segs[(size_t)right] = Segment{dst, ps.begin + lc, rc};
+    const bool search = before_find_best_split(t, best_leaf, right);
+    BlockPlan bp = plan(ps.count);
+    int hist_side = -1;
+    if (search) {
+      compute_used();
+      hist_side = (lc < rc) ? 0 : 1;
+    }
+    launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins, d_tile_first,
+                     (int)tile_first.size() - 1, rows_per_flush, stream);
+    stats.part_rows += ps.count;
+    if (search) {
+      stats.hist_rows += smaller.num_data;
+      const bool use_subtract = parent_slot >= 0;
+      pend_small = leaf_args(t, smaller, smaller_slot, 0, bp.nblk);
+      pending_has_large = larger.leaf >= 0 && larger_slot >= 0;
+      pending_subtract = use_subtract;
+      if (pending_has_large) {
+        pend_large = leaf_args(t, larger, (size_t)larger_slot, 1, 0);
+        if (!use_subtract) {
+          const Segment lseg = segs[(size_t)larger.leaf];
+          BlockPlan lbp = plan(lseg.count);
+          launch_hist(ds, lseg, lbp, d_group_used, 1, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
+                      rows_per_flush, stream);
+          stats.hist_rows += lseg.count;
+          pend_large.nblk = lbp.nblk;
+        }
+      }
+      pending = true;
+    }
+    stats.splits += 1;
+    return {best_leaf, right};
+  }
+
+  // grow_tree (serial.rs:578-628)
+  Tree train(const float* g, const float* h) {
+    before_train(g, h);
+    Tree t(P.num_leaves);
+    t.set_leaf_output(0, leaf_output_root(P, smaller.sum_g, smaller.sum_h));
+    if (before_find_best_split(t, 0, -1)) prepare_root_search(t);
+    for (int step = 0; step < P.num_leaves - 1; ++step) {
+      run_search();
+      size_t bl = 0;
+      for (size_t i = 1; i < best.size(); ++i)
+        if (best[i].better_than(best[bl])) bl = i;
+      if (best[bl].o.gain <= 0.0) break;
+      split(t, (int)bl);
+    }
+    pending = false;
+    if (renew) {
+      BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
+                             stream));
+      launch_renew_sums(ds, d_segs, t.num_leaves, g, h, d_renew, stream);
+      std::vector<double> sums((size_t)t.num_leaves * 2);
+      BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
+      BK_HIPC(hipStreamSynchronize(stream));
+      for (int leaf = 0; leaf < t.num_leaves; ++leaf)
+        t.set_leaf_output(leaf, leaf_output_plain(sums[(size_t)2 * leaf], sums[(size_t)2 * leaf + 1], P.lambda_l1,
+                                                  P.lambda_l2, P.max_delta_step));
+    }
+    int err = 0;
+    BK_HIPC(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
+    if (err) throw std::runtime_error("partition count mismatch on the device");
+    last_num_leaves = t.num_leaves;
+    return t;
+  }
+
+  void export_tree(const Tree& t, bk_tree_out* o) const {
+    o->num_leaves = t.num_leaves;
+    o->num_cat = t.num_cat;
+    size_t ni = (size_t)(t.num_leaves - 1), nl = (size_t)t.num_leaves;
+    for (size_t i = 0; i < ni; ++i) {
+      o->split_feature_inner[i] = t.sf_inner[i];
+      o->split_feature[i] = t.sf[i];
+      o->threshold_in_bin[i] = t.th_bin[i];
+      o->threshold[i] = t.th[i];
+      o->decision_type[i] = t.dtype[i];
+      o->left_child[i] = t.left[i];
+      o->right_child[i] = t.right[i];
+      o->split_gain[i] = t.gain[i];
+      o->internal_value[i] = t.internal_value[i];
+      o->internal_weight[i] = t.internal_weight[i];
+      o->internal_count[i] = t.internal_count[i];
+    }
+    for (size_t i = 0; i < nl; ++i) {
+      o->leaf_value[i] = t.leaf_value[i];
+      o->leaf_weight[i] = t.leaf_weight[i];
+      o->leaf_count[i] = t.leaf_count[i];
+      o->leaf_parent[i] = t.leaf_parent[i];
+      o->leaf_depth[i] = t.leaf_depth[i];
+    }
+    for (size_t i = 0; i < t.cat_bound.size() && i < nl; ++i) o->cat_boundaries[i] = t.cat_bound[i];
+    if ((int)t.cat_th.size() > o->cat_threshold_cap) throw std::invalid_argument("cat_threshold capacity too small");
+    for (size_t i = 0; i < t.cat_th.size(); ++i) o->cat_threshold[i] = t.cat_th[i];
+    o->cat_threshold_len = (int)t.cat_th.size();
+  }
+
+  int max_segment(int num_leaves) const {
+    int m = 1;
+    for (int l = 0; l < num_leaves; ++l) m = std::max(m, segs[(size_t)l].count);
+    return m;
+  }
+
+  void add_score_device(const bk_tree_out* t, double* d_score) {
+    if (t->num_leaves <= 1) return;
+    BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_leaf_vals, t->leaf_value, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
+                           stream));
+    launch_add_score(ds, d_segs, d_leaf_vals, t->num_leaves, d_score, max_segment(t->num_leaves), stream);
+    BK_HIPC(hipStreamSynchronize(stream));
+  }
+
+  void add_score_host(const bk_tree_out* t, double* score) {
+    if (t->num_leaves <= 1) return;
+    if (!d_leaf_of) d_leaf_of = dmalloc<int32_t>((size_t)N);
+    BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
+    launch_leaf_of(ds, d_segs, t->num_leaves, max_segment(t->num_leaves), d_leaf_of, stream);
+    std::vector<int32_t> lo((size_t)N);
+    BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
+    for (int i = 0; i < N; ++i) score[i] += t->leaf_value[lo[(size_t)i]];
+  }
+
+  void leaf_rows(int leaf, int32_t* out, int64_t* n) {
+    if (leaf < 0 || leaf >= last_num_leaves) throw std::invalid_argument("leaf out of range");
+    const Segment s = segs[(size_t)leaf];
+    int32_t* d = dmalloc<int32_t>((size_t)std::max(s.count, 1));
+    launch_leaf_rows(ds, s, d, stream);
+    BK_HIPC(hipMemcpyAsync(out, d, sizeof(int32_t) * (size_t)s.count, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
+    (void)hipFree(d);
+    *n = s.count;
+  }
+};
+
+}  // namespace bk
+
+struct bk_learner {
+  bk::Learner L;
+};
+
+extern "C" {
+
+const char* bk_last_error(void) { return bk::last_error().c_str(); }
+
+int bk_learner_create(const bk_dataset_view* ds, const bk_learner_params* params, int is_constant_hessian,
+                      bk_learner** out) {
+  return bk::guard([&] {
+    if (!ds || !params || !out) throw std::invalid_argument("null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device available");
+    auto h = std::make_unique<bk_learner>();
+    h->L.create(ds, params, is_constant_hessian);
+    *out = h.release();
+  });
+}
+
+int bk_learner_train(bk_learner* l, const float* g, const float* h, const int32_t* rows, int64_t num_rows,
+                     bk_tree_out* tree) {
+  return bk::guard([&] {
+    if (!l || !g || !h || !tree) throw std::invalid_argument("null argument");
+    if (rows) throw std::invalid_argument("bagged row subsets are not supported by the GPU learner");
+    (void)num_rows;
+    bk::Learner& L = l->L;
+    BK_HIPC(hipMemcpyAsync(L.d_grad, g, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
+    BK_HIPC(hipMemcpyAsync(L.d_hess, h, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
+    bk::Tree t = L.train(L.d_grad, L.d_hess);
+    L.export_tree(t, tree);
+  });
+}
+
+int bk_learner_train_device(bk_learner* l, const float* dg, const float* dh, bk_tree_out* tree) {
+  return bk::guard([&] {
+    if (!l || !dg || !dh || !tree) throw std::invalid_argument("null argument");
+    bk::Tree t = l->L.train(dg, dh);
+    l->L.export_tree(t, tree);
+  });
+}
+
+int bk_learner_add_prediction_to_score(bk_learner* l, const bk_tree_out* tree, double* score) {
+  return bk::guard([&] { l->L.add_score_host(tree, score); });
+}
+
+int bk_learner_add_prediction_to_score_device(bk_learner* l, const bk_tree_out* tree, double* d_score) {
+  return bk::guard([&] { l->L.add_score_device(tree, d_score); });
+}
+
+int bk_learner_leaf_rows(bk_learner* l, int32_t leaf, int32_t* rows_out, int64_t* num_out) {
+  return bk::guard([&] { l->L.leaf_rows(leaf, rows_out, num_out); });
+}
+
+int bk_learner_reset_constant_hessian(bk_learner* l, int is_constant_hessian) {
+  return bk::guard([&] {
+    if (l->L.quant && (is_constant_hessian != 0) != l->L.const_hess)
+      throw std::invalid_argument("the quantized learner fixes the hessian mode at creation");
+    l->L.const_hess = is_constant_hessian != 0;
+  });
+}
+
+int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, const float* d_label, double sigmoid,
+                                       float* d_g, float* d_h) {
+  return bk::guard([&] {
+    bk::launch_binary_gradients(d_score, d_label, l->L.N, sigmoid, d_g, d_h, l->L.stream);
+    BK_HIPC(hipStreamSynchronize(l->L.stream));
+  });
+}
+
+int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
+  return bk::guard([&] { *out = l->L.stats; });
+}
+
+int bk_learner_free(bk_learner* l) {
+  delete l;
+  return 0;
+}
+
+}  // extern "C"

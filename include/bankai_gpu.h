@@ -1,0 +1,199 @@
+/*
+ * bankai_gpu.h — C ABI of the MI355X histogram tree learner.
+ *
+ * The library replaces the seat that `SerialTreeLearner` holds inside the
+ * reference's `Trainer` (crates/lgbm-core/src/boosting/train.rs:483). Each
+ * entry point below names the reference method it stands in for.
+ *
+ * Conventions (mirroring crates/lgbm-capi/src/lib.rs:23-26,63-90,181):
+ *   - every function returns 0 on success and -1 on failure;
+ *   - on failure a thread-local message is kept, read with bk_last_error();
+ *   - no C++ exception crosses this boundary;
+ *   - host arrays passed in are borrowed for the duration of the call only;
+ *     device memory is owned by the handle;
+ *   - one handle is driven by one thread at a time.
+ *
+ * Plain pointers and sizes only: no torch or HIP types appear here.
+ */
+#ifndef BANKAI_GPU_H
+#define BANKAI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Missing-value handling of a feature (crates/lgbm-core/src/meta.rs:10-16). */
+enum { BK_MISSING_NONE = 0, BK_MISSING_ZERO = 1, BK_MISSING_NAN = 2 };
+/* Bin type of a feature (crates/lgbm-core/src/io/bin.rs BinType). */
+enum { BK_BIN_NUMERICAL = 0, BK_BIN_CATEGORICAL = 1 };
+
+/* Learner forms. */
+enum {
+  BK_FORM_PLAIN_F64 = 0,      /* f64 histograms (the plain drop-in learner)   */
+  BK_FORM_F32_COMPENSATED = 1,/* f32 histograms with two-sum compensation     */
+  BK_FORM_QUANTIZED = 2       /* int8 gradients, exact integer histograms     */
+};
+
+/*
+ * Binned dataset, column-major by feature group. Mirrors the layout fields of
+ * `DatasetLayout` / `FeatureGroupLayout` (crates/lgbm-core/src/io/dataset.rs:21-93)
+ * and the per-feature `BinMapper` metadata (crates/lgbm-core/src/io/bin.rs:21-36)
+ * the learner reads. Only dense single-value groups with at most 256 group bins
+ * (one byte per row) are accepted.
+ */
+typedef struct bk_dataset_view {
+  int64_t num_data;
+  int32_t num_groups;
+  int32_t num_features;                 /* inner (used) features              */
+  int32_t num_total_features;           /* real columns of the raw matrix      */
+  const uint8_t* const* group_bins;     /* [num_groups] -> num_data bytes       */
+  const int32_t* group_num_total_bin;   /* [num_groups]                         */
+  const int32_t* group_feature_start;   /* [num_groups]                         */
+  const int32_t* group_feature_cnt;     /* [num_groups]                         */
+  /* bin_offsets of every group, concatenated: group g owns the
+   * group_feature_cnt[g] + 1 entries starting at group_feature_start[g] + g. */
+  const int32_t* group_bin_offsets;
+  /* per inner feature */
+  const int32_t* feature_num_bin;
+  const int32_t* feature_missing_type;
+  const int32_t* feature_default_bin;
+  const int32_t* feature_most_freq_bin;
+  const int32_t* feature_bin_type;
+  const int32_t* feature_real_index;
+  const double* feature_penalty;        /* NULL = all 1.0                       */
+  /* Numerical: num_bin upper bounds (BinMapper::bin_upper_bound).
+   * Categorical: NULL (bin_2_categorical is used instead). */
+  const double* const* feature_bin_upper_bound;
+  const int32_t* const* feature_bin_2_categorical;
+} bk_dataset_view;
+
+/*
+ * Learner parameters: the `Config` fields the tree learner reads
+ * (SplitParams / CatParams at crates/lgbm-core/src/treelearner/feature_histogram.rs:32-56,1146-1164
+ * and the quantized options at crates/lgbm-core/src/config/auto.rs:85-88).
+ */
+typedef struct bk_learner_params {
+  int32_t num_leaves;
+  int32_t max_depth;
+  int32_t min_data_in_leaf;
+  double min_sum_hessian_in_leaf;
+  double lambda_l1;
+  double lambda_l2;
+  double max_delta_step;
+  double path_smooth;
+  double min_gain_to_split;
+  double cat_smooth;
+  double cat_l2;
+  int32_t max_cat_threshold;
+  int32_t max_cat_to_onehot;
+  int32_t min_data_per_group;
+  int32_t extra_trees;            /* must be 0 (refused otherwise)              */
+  double histogram_pool_size;     /* MB; <= 0 keeps one slot per leaf           */
+  int32_t use_quantized_grad;
+  int32_t num_grad_quant_bins;
+  int32_t stochastic_rounding;
+  int32_t quant_train_renew_leaf;
+  int32_t seed;                   /* Config::seed, seeds the rounding noise      */
+  int32_t num_threads;            /* fixes the rounding-noise block layout       */
+  int32_t form;                   /* BK_FORM_*                                   */
+  int32_t batched;                /* 1 = speculative depth-batched growth        */
+} bk_learner_params;
+
+/*
+ * One trained tree: the training-time fields of `Tree` (crates/lgbm-core/src/tree.rs:18-49).
+ * The caller allocates every array with capacity num_leaves_max (leaf arrays) or
+ * num_leaves_max - 1 (node arrays); cat_threshold with cat_threshold_cap words.
+ */
+typedef struct bk_tree_out {
+  int32_t num_leaves;
+  int32_t num_cat;
+  int32_t* split_feature_inner;
+  int32_t* split_feature;
+  uint32_t* threshold_in_bin;
+  double* threshold;
+  int8_t* decision_type;
+  int32_t* left_child;
+  int32_t* right_child;
+  float* split_gain;
+  double* internal_value;
+  double* internal_weight;
+  int32_t* internal_count;
+  double* leaf_value;
+  double* leaf_weight;
+  int32_t* leaf_count;
+  int32_t* leaf_parent;
+  int32_t* leaf_depth;
+  int32_t* cat_boundaries;        /* num_leaves_max entries                      */
+  uint32_t* cat_threshold;
+  int32_t cat_threshold_cap;
+  int32_t cat_threshold_len;
+} bk_tree_out;
+
+typedef struct bk_learner bk_learner;
+
+/* Thread-local message of the last failure (crates/lgbm-capi/src/lib.rs:181 LGBM_GetLastError). */
+const char* bk_last_error(void);
+
+/* SerialTreeLearner::new (crates/lgbm-core/src/treelearner/serial.rs:284).
+ * Uploads the bin columns once; they stay resident in HBM. */
+int bk_learner_create(const bk_dataset_view* ds, const bk_learner_params* params,
+                      int is_constant_hessian, bk_learner** out);
+
+/* SerialTreeLearner::train (crates/lgbm-core/src/treelearner/serial.rs:569) with host
+ * gradient/hessian arrays of num_data entries. rows: ascending bagged subset or NULL. */
+int bk_learner_train(bk_learner* learner, const float* gradients, const float* hessians,
+                     const int32_t* rows, int64_t num_rows, bk_tree_out* tree);
+
+/* Same as bk_learner_train with gradients and hessians already in device memory
+ * (num_data floats each). Keeps the boosting loop on the GPU. */
+int bk_learner_train_device(bk_learner* learner, const float* d_gradients,
+                            const float* d_hessians, bk_tree_out* tree);
+
+/* SerialTreeLearner::add_prediction_to_score (crates/lgbm-core/src/treelearner/serial.rs:1974):
+ * score[row] += leaf_value[leaf(row)] for the last trained tree, host scores. */
+int bk_learner_add_prediction_to_score(bk_learner* learner, const bk_tree_out* tree,
+                                       double* score);
+
+/* The same on a device score array of num_data doubles. */
+int bk_learner_add_prediction_to_score_device(bk_learner* learner, const bk_tree_out* tree,
+                                              double* d_score);
+
+/* DataPartition::indices_on_leaf (crates/lgbm-core/src/treelearner/data_partition.rs:109-112),
+ * read through the public `partition` field by Trainer::renew_tree_output
+ * (crates/lgbm-core/src/boosting/train.rs:1198-1211). Copies the rows of `leaf`,
+ * ascending, into rows_out (capacity num_data); *num_out receives the count. */
+int bk_learner_leaf_rows(bk_learner* learner, int32_t leaf, int32_t* rows_out,
+                         int64_t* num_out);
+
+/* SerialTreeLearner::reset_constant_hessian (crates/lgbm-core/src/treelearner/serial.rs:279). */
+int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessian);
+
+/* Binary log-loss gradients and hessians (crates/lgbm-core/src/objective/binary.rs:71-90,
+ * unweighted, label_weights 1) from a device score array, written to device arrays,
+ * on the learner's stream. A device-side boosting loop uses it; it has no FFI
+ * counterpart in the reference, whose objective runs on the host. */
+int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
+                                       double sigmoid, float* d_gradients, float* d_hessians);
+
+/* Per-tree counters of the last train call: rows read by histogram builds
+ * (sum of R_b), rows moved by partitions (sum of Q_s), splits made, rounds. */
+typedef struct bk_train_stats {
+  int64_t hist_rows;
+  int64_t part_rows;
+  int32_t splits;
+  int32_t rounds;
+  int32_t speculative_waste;
+  int32_t reserved;
+} bk_train_stats;
+int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
+
+/* Frees the handle and its device memory. NULL is accepted. */
+int bk_learner_free(bk_learner* learner);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BANKAI_GPU_H */
