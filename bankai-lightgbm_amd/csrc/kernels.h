@@ -21,7 +21,7 @@ constexpr int kStreamThreads = 512;   // threads of the row-streaming kernels
 constexpr int kRowsPerThread = 4;     // rows per thread per tile (one dword of u8 bins)
 constexpr int kTileRows = kStreamThreads * kRowsPerThread;
 constexpr int kMaxRowBlocks = 512;    // row blocks per histogram build (partials)
-constexpr int kLdsBytes = 128 * 1024; // LDS budget per workgroup on gfx950
+constexpr int kLdsBytes = 120 * 1024; // dynamic LDS budget per workgroup on gfx950 (160 KB per CU)
 
 // Per inner feature, as the kernels read it.
 struct FeatDev {

@@ -388,6 +388,15 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
         else pos[k] = -1;
       }
     }
+    // every destination must stay inside the segment; a count that disagrees
+    // with the rows is reported instead of written past the segment
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if ((valid & (1 << k)) && (pos[k] < (long long)seg.begin || pos[k] >= seg_end)) {
+        valid &= ~(1 << k);
+        atomicOr(s.err_flag, 2);
+      }
+    }
     if (valid) {
       for (int g = g0; g < g1; ++g) {
         const uint32_t w = ((const uint32_t*)(scols + (size_t)g * s.npad))[d];

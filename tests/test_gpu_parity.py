@@ -1,0 +1,166 @@
+"""Parity of the HIP learner (through the C ABI) with the CPU oracle.
+
+Bars (from the target specification of this path):
+  - quantized form: identical trees, bit for bit, and identical model text, for
+    the same seed, gradients and num_threads;
+  - float forms (plain f64, f32 compensated): same split features and thresholds,
+    leaf values within 1e-6 relative; the upstream golden models are reproduced in
+    structure by the plain form.
+Every test prints its case before the first GPU call.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import bankai_gpu as bg
+import boosting as bst
+import oracle_binding as ob
+import synth
+
+pytestmark = pytest.mark.gpu
+
+TREE_KEYS = ("split_feature", "split_feature_inner", "threshold_in_bin", "threshold", "decision_type", "left_child",
+             "right_child", "split_gain", "internal_value", "internal_weight", "internal_count", "leaf_value",
+             "leaf_weight", "leaf_count", "leaf_parent", "leaf_depth", "cat_boundaries", "cat_threshold")
+
+
+def announce(msg):
+    print(f"\n[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def assert_trees_bitwise(a, b, ctx=""):
+    assert a["num_leaves"] == b["num_leaves"], (ctx, a["num_leaves"], b["num_leaves"])
+    assert a["num_cat"] == b["num_cat"], ctx
+    for k in TREE_KEYS:
+        assert a[k].tobytes() == b[k].tobytes(), (ctx, k, a[k][:8], b[k][:8])
+
+
+def assert_trees_close(a, b, ctx="", rtol=1e-6):
+    assert a["num_leaves"] == b["num_leaves"], (ctx, a["num_leaves"], b["num_leaves"])
+    for k in ("split_feature", "threshold_in_bin", "decision_type", "left_child", "right_child", "leaf_count",
+              "internal_count", "cat_threshold"):
+        assert np.array_equal(a[k], b[k]), (ctx, k)
+    for k in ("leaf_value", "leaf_weight", "internal_value", "internal_weight"):
+        assert np.allclose(a[k], b[k], rtol=rtol, atol=1e-12), (ctx, k, np.max(np.abs(a[k] - b[k])))
+
+
+@pytest.fixture(scope="module")
+def binary_10k():
+    x, y = synth.binary_28(10_000, seed=0)
+    return bg.Dataset(x, bg.DatasetParams(num_threads=8)), y
+
+
+def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise):
+    """Boosts the oracle; the GPU learner trains each tree on the oracle's gradients."""
+    o = ob.OracleLearner(ds, params, obj.const_hess)
+    gpu = bg.Learner(ds, params, obj.const_hess)
+    score = np.zeros(ds.num_data)
+    init = obj.boost_from_score(0)
+    score += init
+    trees = []
+    for it in range(iters):
+        g, h = obj.gradients(score)
+        to = o.train(g, h)
+        tg = gpu.train(g, h)
+        compare(tg, to, f"iter {it}")
+        for leaf in range(to["num_leaves"]):
+            assert np.array_equal(gpu.leaf_rows(leaf), o.leaf_rows(leaf)), (it, leaf)
+        vals = to["leaf_value"] * lr
+        s_gpu = score.copy()
+        gpu.add_prediction_to_score(vals, s_gpu)
+        o.add_prediction_to_score(vals, score)
+        assert np.array_equal(s_gpu, score)
+        trees.append((tg, to))
+    return trees
+
+
+@pytest.mark.parametrize("bins,stochastic,renew", [(4, 1, 1), (4, 0, 0), (16, 1, 0), (2, 1, 1), (127, 1, 1)])
+def test_quantized_bit_exact(binary_10k, bins, stochastic, renew):
+    announce(f"quantized 10k x 28 bins={bins} stochastic={stochastic} renew={renew}")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=bins, stochastic_rounding=stochastic,
+                         quant_train_renew_leaf=renew, seed=7, num_threads=4, form=bg.FORM_QUANTIZED)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=6)
+
+
+def test_quantized_model_text_identical(binary_10k):
+    announce("quantized model text, 20 iterations")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=3, num_threads=8, form=bg.FORM_QUANTIZED)
+    texts = []
+    for make in (lambda: ob.OracleLearner(ds, p, False), lambda: bg.Learner(ds, p, False)):
+        obj = bst.BinaryObjective(y)
+        gb = bst.GBDT([make()], obj, ds.num_data, learning_rate=0.1).train(20)
+        texts.append(bst.model_to_string(gb.models, 1, 1, obj.name(), [f"Column_{i}" for i in range(28)],
+                                         ds.feature_infos()))
+    assert texts[0] == texts[1]
+
+
+def test_quantized_constant_hessian_regression():
+    announce("quantized regression (constant hessian) 7000 x 28")
+    x, _ = synth.binary_28(7000, seed=5)
+    yv = x[:, 0] * 2 - x[:, 3] + np.random.default_rng(5).normal(size=7000)
+    ds = bg.Dataset(x)
+    p = bg.LearnerParams(num_leaves=15, use_quantized_grad=1, num_grad_quant_bins=8, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=1, num_threads=2, form=bg.FORM_QUANTIZED)
+    lockstep(ds, bst.RegressionL2Objective(yv), p, iters=4)
+
+
+@pytest.mark.parametrize("form", [bg.FORM_PLAIN_F64, bg.FORM_F32_COMPENSATED])
+def test_float_forms_match_oracle(binary_10k, form):
+    announce(f"float form {form} 10k x 28")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=31, form=form)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=5, compare=assert_trees_close)
+
+
+@pytest.mark.parametrize("n,leaves,min_data", [(25, 4, 2), (63, 8, 3), (1001, 31, 20), (4097, 63, 5), (9999, 255, 1)])
+def test_ragged_sizes_quantized(n, leaves, min_data):
+    announce(f"ragged n={n} leaves={leaves} min_data={min_data}")
+    x, y = synth.binary_28(n, seed=n)
+    ds = bg.Dataset(x, bg.DatasetParams(min_data_in_leaf=min_data))
+    p = bg.LearnerParams(num_leaves=leaves, min_data_in_leaf=min_data, use_quantized_grad=1, num_grad_quant_bins=4,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=2, form=bg.FORM_QUANTIZED)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=3)
+
+
+def test_max_depth_and_pool_eviction(binary_10k):
+    announce("max_depth=4 and a 2-slot histogram pool")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=31, max_depth=4, use_quantized_grad=1, num_grad_quant_bins=4, seed=9,
+                         form=bg.FORM_QUANTIZED)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=3)
+    bytes_per_leaf = sum(int(b) for b in ds.feature_meta()["num_bin"]) * 16
+    p2 = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=4, seed=9, form=bg.FORM_QUANTIZED,
+                          histogram_pool_size=3 * bytes_per_leaf / (1024.0 * 1024.0))
+    lockstep(ds, bst.BinaryObjective(y), p2, iters=3)
+
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "train")
+
+
+@pytest.mark.parametrize("case", sorted(os.listdir(GOLDEN)) if os.path.isdir(GOLDEN) else [])
+def test_golden_models_on_gpu(case):
+    announce(f"golden {case} plain f64 on the GPU")
+    import test_golden_train as tg
+    ds, gbdt, model, objective = tg.train_case(case, lambda ds, lp, ch: bg.Learner(ds, lp, ch))
+    tg.compare_models(ds, gbdt, model, objective)
+
+
+@pytest.mark.parametrize("case", ["cat_default", "cat_nan", "l2_nan", "l2_zero_as_missing", "multiclass"])
+def test_golden_datasets_quantized_bit_exact(case):
+    announce(f"golden dataset {case} quantized, GPU vs oracle")
+    import test_golden_train as tg
+    spec, p, x, y, w, model = tg.build_case(case)
+    ds = bg.Dataset(x, tg.dataset_params(p), categorical=spec["categorical"])
+    obj = tg.make_objective(p, y, w)
+    lp = tg.learner_params(p)
+    lp.use_quantized_grad, lp.num_grad_quant_bins, lp.seed, lp.form = 1, 4, 7, bg.FORM_QUANTIZED
+    lp.quant_train_renew_leaf = 1
+    if obj.num_class > 1:
+        obj = bst.BinaryObjective((y == 1).astype(float))
+    lockstep(ds, obj, lp, iters=4)
