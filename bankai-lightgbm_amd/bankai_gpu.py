@@ -86,6 +86,13 @@ class TrainStatsC(C.Structure):
                 ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32)]
 
 
+class KernelStatsC(C.Structure):
+    _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("algo_bytes", C.c_double), ("moved_bytes", C.c_double)]
+
+
+KERNEL_KINDS = ("hist", "partition_count", "partition_scatter", "search", "discretize", "renew", "score_gradients")
+
+
 class DatasetParamsC(C.Structure):
     _fields_ = [("max_bin", C.c_int32), ("min_data_in_bin", C.c_int32),
                 ("bin_construct_sample_cnt", C.c_int32), ("data_random_seed", C.c_int32),
@@ -107,6 +114,8 @@ EXPORTS = {
     "bk_learner_reset_constant_hessian": ([_P, C.c_int], C.c_int),
     "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
     "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
+    "bk_learner_profile": ([_P, C.c_int], C.c_int),
+    "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
                               C.c_int),
@@ -372,6 +381,18 @@ class Learner:
         _check(self._lib.bk_learner_last_stats(self.handle, C.byref(s)))
         return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
                 "speculative_waste": s.speculative_waste}
+
+    def profile(self, enable=True):
+        """Turns per-kernel HIP-event timing on (and resets the counters)."""
+        _check(self._lib.bk_learner_profile(self.handle, int(enable)))
+
+    def kernel_stats(self):
+        out = {}
+        for k, name in enumerate(KERNEL_KINDS):
+            s = KernelStatsC()
+            _check(self._lib.bk_learner_kernel_stats(self.handle, k, C.byref(s)))
+            out[name] = {"ms": s.ms, "launches": s.launches, "algo_bytes": s.algo_bytes, "moved_bytes": s.moved_bytes}
+        return out
 
     def close(self):
         if getattr(self, "handle", None):

@@ -187,6 +187,12 @@ void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int*
 void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
                       int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
                       int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+void launch_partition_count(const DevState& s, Segment seg, int split_group, const RuleDev& rule,
+                            const BlockPlan& bp, void* stream);
+void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
+                              int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
+                              int part_set, int max_tile_bins, const int* d_tile_first, int ntiles,
+                              int rows_per_flush, void* stream);
 void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
                  const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);

@@ -1399,14 +1399,20 @@ void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int*
   BK_HIP(hipGetLastError());
 }
 
-void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
-                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
-                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+void launch_partition_count(const DevState& s, Segment seg, int split_group, const RuleDev& rule,
+                            const BlockPlan& bp, void* stream) {
+  if (bp.nblk <= 0) return;
+  hipLaunchKernelGGL(k_part_count, dim3((unsigned)bp.nblk), dim3(kStreamThreads), 0, (hipStream_t)stream, s, seg, bp,
+                     split_group, rule);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
+                              int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
+                              int part_set, int max_tile_bins, const int* d_tile_first, int ntiles,
+                              int rows_per_flush, void* stream) {
   if (bp.nblk <= 0) return;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_part_count, dim3((unsigned)bp.nblk), dim3(kStreamThreads), 0, st, s, seg, bp, split_group,
-                     rule);
-  BK_HIP(hipGetLastError());
   dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
   const size_t per_bin = s.form == FORM_QUANT ? 8 : 20;
   size_t lds = hist_side >= 0 ? (size_t)max_tile_bins * per_bin : 0;
@@ -1429,6 +1435,14 @@ void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_gro
       break;
   }
   BK_HIP(hipGetLastError());
+}
+
+void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
+                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
+                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+  launch_partition_count(s, seg, split_group, rule, bp, stream);
+  launch_partition_scatter(s, seg, dst_buf, split_group, rule, left_total, bp, hist_side, d_group_used, part_set,
+                           max_tile_bins, d_tile_first, ntiles, rows_per_flush, stream);
 }
 
 void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,

@@ -277,6 +277,58 @@ T* dmalloc(size_t n) {
   return (T*)p;
 }
 
+// Event pairs around launches on the learner's stream, resolved at the end of a tree.
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Rec {
+    int kind;
+    size_t ev;
+    double algo, moved;
+  };
+  std::vector<Rec> pending;
+  double ms[8] = {0};
+  int64_t launches[8] = {0};
+  double algo[8] = {0}, moved[8] = {0};
+  hipEvent_t ev(size_t i) {
+    while (pool.size() <= i) {
+      hipEvent_t e;
+      BK_HIPC(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[i];
+  }
+  size_t begin(hipStream_t s) {
+    size_t i = used;
+    used += 2;
+    BK_HIPC(hipEventRecord(ev(i), s));
+    return i;
+  }
+  void end(hipStream_t s, size_t i, int kind, double a, double m) {
+    BK_HIPC(hipEventRecord(ev(i + 1), s));
+    pending.push_back(Rec{kind, i, a, m});
+  }
+  void resolve() {
+    for (const Rec& r : pending) {
+      float t = 0.0f;
+      BK_HIPC(hipEventElapsedTime(&t, pool[r.ev], pool[r.ev + 1]));
+      ms[r.kind] += t;
+      launches[r.kind] += 1;
+      algo[r.kind] += r.algo;
+      moved[r.kind] += r.moved;
+    }
+    pending.clear();
+    used = 0;
+  }
+  void reset() {
+    for (int k = 0; k < 8; ++k) ms[k] = 0, launches[k] = 0, algo[k] = 0, moved[k] = 0;
+  }
+  ~Prof() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 }  // namespace
 
 struct Learner {
@@ -345,6 +397,24 @@ struct Learner {
   bool pending_has_large = false, pending_subtract = false;
   LeafArgs pend_small{}, pend_large{};
   double h0 = 1.0;
+  Prof prof;
+  int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
+  int used_group_count() const {
+    int c = 0;
+    for (int u : used_group) c += u;
+    return c;
+  }
+  // wraps a launch in events when profiling
+  template <typename F>
+  void timed(int kind, double algo, double moved, F&& launch) {
+    if (!prof.on) {
+      launch();
+      return;
+    }
+    size_t i = prof.begin(stream);
+    launch();
+    prof.end(stream, i, kind, algo, moved);
+  }
 
   ~Learner() { release(); }
 
@@ -651,8 +721,11 @@ struct Learner {
         double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
         int start = stochastic ? (int)use_start_eng.uint_incl((uint32_t)N) : 0;
         BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
-        launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0, stochastic ? 1 : 0,
-                          (int16_t*)d_gh[0], d_sums3, stream);
+        const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
+        timed(4, db, db, [&] {
+          launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0,
+                            stochastic ? 1 : 0, (int16_t*)d_gh[0], d_sums3, stream);
+        });
         long long sums[4];
         BK_HIPC(hipMemcpyAsync(sums, d_sums3, sizeof sums, hipMemcpyDeviceToHost, stream));
         BK_HIPC(hipStreamSynchronize(stream));
@@ -789,8 +862,10 @@ struct Learner {
     if (!pending) return;
     pending = false;
     ScanParams sp = scan_params();
-    launch_scan(ds, pend_small, pending_has_large ? &pend_large : nullptr, pending_subtract ? 1 : 0, sp, d_used, d_out,
-                stream);
+    timed(3, 0.0, 0.0, [&] {
+      launch_scan(ds, pend_small, pending_has_large ? &pend_large : nullptr, pending_subtract ? 1 : 0, sp, d_used,
+                  d_out, stream);
+    });
     BK_HIPC(hipMemcpyAsync(h_out, d_out, sizeof(SplitOut) * (size_t)F * 2, hipMemcpyDeviceToHost, stream));
     BK_HIPC(hipStreamSynchronize(stream));
     HostSplit sb, lb;
@@ -817,8 +892,11 @@ struct Learner {
   void prepare_root_search(const Tree& t) {
     compute_used();
     BlockPlan bp = plan(N);
-    launch_hist(ds, segs[0], bp, d_group_used, 0, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
-                rows_per_flush, stream);
+    const double hb = (double)N * (used_group_count() + payload_bytes());
+    timed(0, hb, hb, [&] {
+      launch_hist(ds, segs[0], bp, d_group_used, 0, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
+                  rows_per_flush, stream);
+    });
     stats.hist_rows += N;
     pend_small = leaf_args(t, smaller, smaller_slot, 0, bp.nblk);
     pending_has_large = false;
@@ -888,8 +966,26 @@ struct Learner {
       compute_used();
       hist_side = (lc < rc) ? 0 : 1;
     }
-    launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins, d_tile_first,
-                     (int)tile_first.size() - 1, rows_per_flush, stream);
+    {
+      const double q = (double)ps.count;
+      const double rec = (double)(G + payload_bytes() + 4);
+      const double hist_rows = search ? (double)std::min(lc, rc) : 0.0;
+      const double algo = q * 9.0 + hist_rows * (used_group_count() + payload_bytes());
+      const double moved = q * (2.0 * rec + 1.0);
+      if (prof.on) {
+        // the count pass and the scatter pass are timed separately
+        size_t i0 = prof.begin(stream);
+        launch_partition_count(ds, ps, m.group, rule, bp, stream);
+        prof.end(stream, i0, 1, q * 1.0, q * 1.0);
+        size_t i1 = prof.begin(stream);
+        launch_partition_scatter(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins,
+                                 d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+        prof.end(stream, i1, 2, algo, moved);
+      } else {
+        launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins, d_tile_first,
+                         (int)tile_first.size() - 1, rows_per_flush, stream);
+      }
+    }
     stats.part_rows += ps.count;
     if (search) {
       stats.hist_rows += smaller.num_data;
@@ -932,7 +1028,8 @@ struct Learner {
     if (renew) {
       BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
                              stream));
-      launch_renew_sums(ds, d_segs, t.num_leaves, g, h, d_renew, stream);
+      timed(5, (double)N * 12.0, (double)N * 12.0,
+            [&] { launch_renew_sums(ds, d_segs, t.num_leaves, g, h, d_renew, stream); });
       std::vector<double> sums((size_t)t.num_leaves * 2);
       BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
       BK_HIPC(hipStreamSynchronize(stream));
@@ -944,6 +1041,7 @@ struct Learner {
     BK_HIPC(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, stream));
     BK_HIPC(hipStreamSynchronize(stream));
     if (err) throw std::runtime_error("partition count mismatch on the device");
+    if (prof.on) prof.resolve();
     last_num_leaves = t.num_leaves;
     return t;
   }
@@ -989,8 +1087,11 @@ struct Learner {
     BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
     BK_HIPC(hipMemcpyAsync(d_leaf_vals, t->leaf_value, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
                            stream));
-    launch_add_score(ds, d_segs, d_leaf_vals, t->num_leaves, d_score, max_segment(t->num_leaves), stream);
+    timed(6, (double)N * 20.0, (double)N * 20.0, [&] {
+      launch_add_score(ds, d_segs, d_leaf_vals, t->num_leaves, d_score, max_segment(t->num_leaves), stream);
+    });
     BK_HIPC(hipStreamSynchronize(stream));
+    if (prof.on) prof.resolve();
   }
 
   void add_score_host(const bk_tree_out* t, double* score) {
@@ -1083,13 +1184,33 @@ int bk_learner_reset_constant_hessian(bk_learner* l, int is_constant_hessian) {
 int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, const float* d_label, double sigmoid,
                                        float* d_g, float* d_h) {
   return bk::guard([&] {
-    bk::launch_binary_gradients(d_score, d_label, l->L.N, sigmoid, d_g, d_h, l->L.stream);
-    BK_HIPC(hipStreamSynchronize(l->L.stream));
+    bk::Learner& L = l->L;
+    L.timed(6, (double)L.N * 20.0, (double)L.N * 20.0,
+            [&] { bk::launch_binary_gradients(d_score, d_label, L.N, sigmoid, d_g, d_h, L.stream); });
+    BK_HIPC(hipStreamSynchronize(L.stream));
+    if (L.prof.on) L.prof.resolve();
   });
 }
 
 int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
   return bk::guard([&] { *out = l->L.stats; });
+}
+
+int bk_learner_profile(bk_learner* l, int enable) {
+  return bk::guard([&] {
+    l->L.prof.on = enable != 0;
+    l->L.prof.reset();
+  });
+}
+
+int bk_learner_kernel_stats(bk_learner* l, int kind, bk_kernel_stats* out) {
+  return bk::guard([&] {
+    if (kind < 0 || kind >= 8) throw std::invalid_argument("kernel kind out of range");
+    out->ms = l->L.prof.ms[kind];
+    out->launches = l->L.prof.launches[kind];
+    out->algo_bytes = l->L.prof.algo[kind];
+    out->moved_bytes = l->L.prof.moved[kind];
+  });
 }
 
 int bk_learner_free(bk_learner* l) {

@@ -1,0 +1,241 @@
+"""Benchmark: boosting iterations per second of the MI355X learner on the
+10.5M x 28 binary workload (255 leaves, max_bin 255, quantized int8 gradients
+with 4 bins, stochastic rounding, leaf renewal), next to the plain f64 GPU
+learner and the CPU oracle timed on this host.
+
+One step = one boosting iteration with every input resident in HBM: binary
+log-loss gradients on the device, one tree grown by the learner, the score
+update. Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bankai-lightgbm_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def predict_raw(trees, x):
+    """Sum of tree outputs for raw features (numerical splits without missing values)."""
+    import numpy as np
+
+    out = np.zeros(x.shape[0])
+    rows = np.arange(x.shape[0])
+    for t in trees:
+        if t["num_leaves"] <= 1:
+            out += t["leaf_value"][0]
+            continue
+        node = np.zeros(x.shape[0], np.int64)
+        active = np.ones(x.shape[0], bool)
+        leaf = np.zeros(x.shape[0], np.int64)
+        while active.any():
+            idx = rows[active]
+            n = node[idx]
+            f = t["split_feature"][n]
+            go_left = x[idx, f] <= t["threshold"][n]
+            child = np.where(go_left, t["left_child"][n], t["right_child"][n])
+            done = child < 0
+            leaf[idx[done]] = ~child[done]
+            node[idx[~done]] = child[~done]
+            active[idx[done]] = False
+        out += t["leaf_value"][leaf]
+    return out
+
+
+def auc(y, s):
+    import numpy as np
+
+    order = np.argsort(s, kind="mergesort")
+    ys = y[order]
+    ranks = np.empty(len(s))
+    ranks[order] = np.arange(1, len(s) + 1)
+    npos = ys.sum()
+    nneg = len(ys) - npos
+    return float((ranks[y > 0].sum() - npos * (npos + 1) / 2) / (npos * nneg))
+
+
+def run_gpu(ds, y, params, steps, warmup, lr, profile):
+    import numpy as np
+    import torch
+
+    import bankai_gpu as bg
+    import boosting as bst
+
+    obj = bst.BinaryObjective(y)
+    init = obj.boost_from_score(0)
+    learner = bg.Learner(ds, params, False)
+    n = ds.num_data
+    dev = torch.device("cuda:0")
+    score = torch.full((n,), init, dtype=torch.float64, device=dev)
+    label = torch.as_tensor(y.astype(np.float32), device=dev)
+    g = torch.empty(n, dtype=torch.float32, device=dev)
+    h = torch.empty(n, dtype=torch.float32, device=dev)
+    trees = []
+
+    def step(record):
+        learner.binary_gradients_device(score.data_ptr(), label.data_ptr(), 1.0, g.data_ptr(), h.data_ptr())
+        t = learner.train_device(g.data_ptr(), h.data_ptr())
+        tree = bst.Tree(t)
+        tree.apply_shrinkage(lr)
+        learner.add_prediction_to_score_device(tree.leaf_value, score.data_ptr())
+        if record:
+            trees.append(tree)
+
+    for i in range(warmup):
+        step(True)
+    torch.cuda.synchronize()
+    if profile:
+        learner.profile(True)
+    stats = {"hist_rows": 0, "part_rows": 0, "splits": 0}
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(True)
+        s = learner.last_stats()
+        for k in stats:
+            stats[k] += s[k]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kstats = learner.kernel_stats() if profile else None
+    out_trees = []
+    for k, tr in enumerate(trees):
+        d = dict(tr.d)
+        if k == 0 and abs(init) > bst.K_EPSILON:
+            tr2 = bst.Tree(d)
+            tr2.add_bias(init)
+            d = tr2.d
+        out_trees.append(d)
+    learner.close()
+    return elapsed, stats, kstats, out_trees
+
+
+def cpu_baseline(ds, y, params, budget_s=25.0):
+    """Times the oracle (C++ restatement of the reference CPU learner) on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import boosting as bst
+    import oracle_binding as ob
+
+    threads = min(os.cpu_count() or 1, 16)
+    params.num_threads = threads
+    obj = bst.BinaryObjective(y)
+    score = np.full(ds.num_data, obj.boost_from_score(0))
+    learner = ob.OracleLearner(ds, params, False)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        g, h = obj.gradients(score)
+        t = learner.train(g, h)
+        learner.add_prediction_to_score(t["leaf_value"] * 0.1, score)
+        done += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or done >= 5:
+            break
+    return done / el, threads, done, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=10_500_000)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--holdout", type=int, default=1_050_000)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import bankai_gpu as bg
+    import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    seed = 0
+    t0 = time.perf_counter()
+    intercept = synth.solve_intercept(seed)
+    x, y = synth.binary_28(args.rows, seed=seed, intercept=intercept)
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=min(os.cpu_count() or 1, 16)))
+    log(f"data {args.rows} x 28, positive rate {y.mean():.4f}, binned in {time.perf_counter() - t0:.1f}s")
+    lr = 0.1
+    qparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                               stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
+                               form=bg.FORM_QUANTIZED)
+    elapsed, stats, kstats, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=True)
+    its = args.steps / elapsed
+    log(f"quantized: {its:.2f} it/s ({1000 * elapsed / args.steps:.1f} ms/iter)")
+
+    result = {
+        "metric": "boost iters/sec @10.5M x 28, 255 leaves (quantized int8 GPU learner)",
+        "value": its,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": f"binary logloss {args.rows} x 28 dense N(0,1) f32, 53% positive, num_leaves 255, "
+                               "max_bin 255, min_data_in_leaf 20, lr 0.1, use_quantized_grad, 4 bins, stochastic "
+                               "rounding, quant_train_renew_leaf", "rows": args.rows, "features": 28,
+                   "num_leaves": 255, "max_bin": 255, "form": "quantized"},
+    }
+    # dominant kernel by time and its roofline
+    dom = max(kstats, key=lambda k: kstats[k]["ms"])
+    ks = kstats[dom]
+    achieved = ks["algo_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
+    result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                          "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / max(ks["launches"], 1),
+                          "moved_GBs": ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0}
+    total_ms = sum(v["ms"] for v in kstats.values())
+    result["kernels"] = {k: {"ms_per_iter": v["ms"] / args.steps, "launches_per_iter": v["launches"] / args.steps,
+                             "algo_GBs": v["algo_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
+                         for k, v in kstats.items()}
+    result["kernel_ms_share"] = total_ms / (1000.0 * elapsed)
+    result["rows_per_iter"] = {k: v / args.steps for k, v in stats.items()}
+
+    hx, hy = synth.binary_28(args.holdout, seed=seed, intercept=intercept, stream=3)
+    q_auc = auc(hy, predict_raw(qtrees, hx))
+    result["holdout_auc"] = q_auc
+    if not args.no_plain:
+        pparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64)
+        pel, _, _, ptrees = run_gpu(ds, y, pparams, args.plain_steps, 1, lr, profile=False)
+        p_its = args.plain_steps / pel
+        log(f"plain f64: {p_its:.2f} it/s")
+        result["plain_f64_iters_per_sec"] = p_its
+        result["speedup_vs_plain_f64"] = its / p_its
+        # AUC at equal tree counts
+        k = len(ptrees)
+        result["holdout_auc_at_equal_trees"] = {"trees": k, "quantized": auc(hy, predict_raw(qtrees[:k], hx)),
+                                                "plain_f64": auc(hy, predict_raw(ptrees, hx))}
+    if not args.no_cpu_baseline:
+        cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                                   stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)
+        v, cores, done, el = cpu_baseline(ds, y, cparams)
+        result["cpu_baseline"] = {"value": v, "unit": "iter/s", "cores": cores, "kind": "port",
+                                  "sample": f"{done} boosting iterations of the same workload (quantized, full "
+                                            f"{args.rows} rows) in {el:.1f}s"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -189,6 +189,22 @@ typedef struct bk_train_stats {
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
+/* Per-kernel timing with HIP events recorded on the learner's stream around
+ * every launch (off by default). Kinds: 0 segment histogram, 1 partition count,
+ * 2 partition scatter with the child histogram, 3 split search, 4 discretizer,
+ * 5 leaf renewal, 6 score update and gradients. algo_bytes follows the per-row
+ * byte model of the histogram (rows x (features + payload)) and of the partition
+ * (rows x 9: row id read, one bin, row id write); moved_bytes counts what the
+ * kernel actually reads and writes. No reference counterpart (measurement only). */
+typedef struct bk_kernel_stats {
+  double ms;
+  int64_t launches;
+  double algo_bytes;
+  double moved_bytes;
+} bk_kernel_stats;
+int bk_learner_profile(bk_learner* learner, int enable);
+int bk_learner_kernel_stats(bk_learner* learner, int kind, bk_kernel_stats* out);
+
 /* Frees the handle and its device memory. NULL is accepted. */
 int bk_learner_free(bk_learner* learner);
 
