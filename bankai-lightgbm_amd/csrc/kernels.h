@@ -22,6 +22,7 @@ constexpr int kRowsPerThread = 4;     // rows per thread per tile (one dword of 
 constexpr int kTileRows = kStreamThreads * kRowsPerThread;
 constexpr int kMaxRowBlocks = 512;    // row blocks per histogram build (partials)
 constexpr int kLdsBytes = 120 * 1024; // dynamic LDS budget per workgroup on gfx950 (160 KB per CU)
+constexpr int kTileGroups = 4;       // columns per group tile of the partition / histogram blocks
 
 // Per inner feature, as the kernels read it.
 struct FeatDev {
@@ -186,13 +187,14 @@ void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int*
                  int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
 void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
                       int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
-                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+                      int max_tile_bins, int max_tile_groups, const int* d_tile_first, int ntiles, int rows_per_flush,
+                      void* stream);
 void launch_partition_count(const DevState& s, Segment seg, int split_group, const RuleDev& rule,
                             const BlockPlan& bp, void* stream);
 void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
                               int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
-                              int part_set, int max_tile_bins, const int* d_tile_first, int ntiles,
-                              int rows_per_flush, void* stream);
+                              int part_set, int max_tile_bins, int max_tile_groups, const int* d_tile_first,
+                              int ntiles, int rows_per_flush, void* stream);
 void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
                  const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
@@ -200,8 +202,8 @@ void launch_discretize(const float* g, const float* h, const double* gnoise, con
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream);
 void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream);
-void launch_renew_sums(const DevState& s, const Segment* d_segs, int num_leaves, const float* g, const float* h,
-                       double* d_sums, void* stream);
+void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
+                       const float* h, double* d_sums, void* stream);
 void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
                       double* score, int max_seg, void* stream);
 void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream);

@@ -320,6 +320,8 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, Segme
 // into the destination buffer, and accumulate the histogram of the child on
 // `hist_side` (0 left, 1 right, -1 none) from the rows already in registers.
 // Grid: (row blocks, group tiles); tile 0 also moves row ids and payload.
+// Each tile of rows is permuted in LDS first (left rows, then right rows, both
+// in row order), so the writes are two contiguous runs per column.
 // ---------------------------------------------------------------------------
 template <int FORM>
 __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Segment seg, int dst_buf, int split_group,
@@ -330,11 +332,19 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   extern __shared__ unsigned char smem[];
   __shared__ int scratch[32];
   __shared__ int red[kStreamThreads / 64];
+  constexpr int kPay = FORM == FORM_QUANT ? 2 : 8;
   const int blk = blockIdx.x, tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
+  const int ng = g1 - g0;
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
+  const bool do_hist = hist_side >= 0;
+  // LDS: [histogram (nb bins)] [staged bytes: ng x kTileRows] [row ids] [payload]
   LdsHist<FORM> L;
   L.bind(smem, nb);
+  const size_t hist_bytes = do_hist ? (((size_t)nb * LdsHist<FORM>::kBytesPerBin + 15) & ~(size_t)15) : 0;
+  uint8_t* stage = smem + hist_bytes;
+  int32_t* stage_idx = (int32_t*)(stage + (size_t)ng * kTileRows);
+  unsigned char* stage_pay = (unsigned char*)(stage_idx + kTileRows);
   // left rows of the row blocks before this one
   int lb = 0;
   for (int i = threadIdx.x; i < blk; i += blockDim.x) lb += s.blk_cnt[i];
@@ -353,7 +363,6 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   const uint8_t* scols = s.cols[seg.buf];
   uint8_t* dcols = (uint8_t*)s.cols[dst_buf];
   const uint32_t* scol = (const uint32_t*)(scols + (size_t)split_group * s.npad);
-  const bool do_hist = hist_side >= 0;
   if (do_hist) L.zero(nb);
   __syncthreads();
   bool first = true;
@@ -363,9 +372,8 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
     const long long d = t0 + threadIdx.x;
     const long long r0 = d << 2;
     int valid = 0, lmask = 0;
-    uint32_t sw = 0;
     if (d < ((ce + 3) >> 2)) {
-      sw = scol[d];
+      const uint32_t sw = scol[d];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (r0 + k >= cb && r0 + k < ce) {
@@ -378,57 +386,46 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
     const int nl = __popc(lmask), nr = __popc(rmask);
     int pl, pr, tl, tr;
     block_scan2(nl, nr, &pl, &pr, &tl, &tr, scratch);
-    long long pos[4];
+    // tile-local destinations: left rows at [0, tl), right rows at [tl, tl + tr)
+    int lp[4];
     {
       int il = 0, ir = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (lmask & (1 << k)) pos[k] = lpos + pl + il++;
-        else if (rmask & (1 << k)) pos[k] = rpos + pr + ir++;
-        else pos[k] = -1;
-      }
-    }
-    // every destination must stay inside the segment; a count that disagrees
-    // with the rows is reported instead of written past the segment
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if ((valid & (1 << k)) && (pos[k] < (long long)seg.begin || pos[k] >= seg_end)) {
-        valid &= ~(1 << k);
-        atomicOr(s.err_flag, 2);
+        if (lmask & (1 << k)) lp[k] = pl + il++;
+        else if (rmask & (1 << k)) lp[k] = tl + pr + ir++;
+        else lp[k] = -1;
       }
     }
     if (valid) {
-      for (int g = g0; g < g1; ++g) {
-        const uint32_t w = ((const uint32_t*)(scols + (size_t)g * s.npad))[d];
-        uint8_t* dc = dcols + (size_t)g * s.npad;
+      for (int gi = 0; gi < ng; ++gi) {
+        const uint32_t w = ((const uint32_t*)(scols + (size_t)(g0 + gi) * s.npad))[d];
+        uint8_t* st = stage + (size_t)gi * kTileRows;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          if (valid & (1 << k)) dc[pos[k]] = (uint8_t)((w >> (8 * k)) & 0xffu);
+          if (valid & (1 << k)) st[lp[k]] = (uint8_t)((w >> (8 * k)) & 0xffu);
       }
       if (tile == 0) {
-        int32_t* didx = s.idx[dst_buf];
         if (seg.buf == 0) {
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) didx[pos[k]] = (int32_t)(r0 + k);
+            if (valid & (1 << k)) stage_idx[lp[k]] = (int32_t)(r0 + k);
         } else {
           const int32_t* sidx = s.idx[seg.buf];
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) didx[pos[k]] = sidx[r0 + k];
+            if (valid & (1 << k)) stage_idx[lp[k]] = sidx[r0 + k];
         }
         if (FORM == FORM_QUANT) {
           const int16_t* sp = (const int16_t*)s.gh[seg.buf];
-          int16_t* dp = (int16_t*)s.gh[dst_buf];
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) dp[pos[k]] = sp[r0 + k];
+            if (valid & (1 << k)) ((int16_t*)stage_pay)[lp[k]] = sp[r0 + k];
         } else {
           const float2* sp = (const float2*)s.gh[seg.buf];
-          float2* dp = (float2*)s.gh[dst_buf];
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) dp[pos[k]] = sp[r0 + k];
+            if (valid & (1 << k)) ((float2*)stage_pay)[lp[k]] = sp[r0 + k];
         }
       }
       if (do_hist) {
@@ -447,6 +444,22 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
         }
       }
     }
+    __syncthreads();
+    // write the two runs; consecutive threads write consecutive destinations
+    const int tn = tl + tr;
+    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+      const long long dst = i < tl ? lpos + i : rpos + (i - tl);
+      if (dst < (long long)seg.begin || dst >= seg_end) {
+        atomicOr(s.err_flag, 2);
+        continue;
+      }
+      for (int gi = 0; gi < ng; ++gi) dcols[(size_t)(g0 + gi) * s.npad + dst] = stage[(size_t)gi * kTileRows + i];
+      if (tile == 0) {
+        s.idx[dst_buf][dst] = stage_idx[i];
+        if (FORM == FORM_QUANT) ((int16_t*)s.gh[dst_buf])[dst] = ((const int16_t*)stage_pay)[i];
+        else ((float2*)s.gh[dst_buf])[dst] = ((const float2*)stage_pay)[i];
+      }
+    }
     lpos += tl;
     rpos += tr;
     my_left_total += tl;
@@ -456,10 +469,10 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
       L.flush(s, set, blk, bin_lo, nb, first);
       __syncthreads();
       L.zero(nb);
-      __syncthreads();
       first = false;
       since_flush = 0;
     }
+    __syncthreads();
   }
   if (do_hist) {
     __syncthreads();
@@ -467,6 +480,7 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   }
   if (threadIdx.x == 0 && tile == 0 && blk == bp.nblk - 1 && left_before + my_left_total != left_total)
     atomicOr(s.err_flag, 1);
+  (void)kPay;
 }
 
 // ---------------------------------------------------------------------------
@@ -518,101 +532,51 @@ __device__ __forceinline__ double split_gain_plain(double gl, double hl, double 
   return a + b;
 }
 
-struct Cand {
-  bool splittable;
-  double gain;
-  int threshold;
-  bool default_left;
-  double lg, lh;   // f64 forms: best left sums (lh includes kEpsilon); int forms unused
-  long long ilg, ilh;
-  int lc;
+// Prefix state of one scan direction at every bin, in LDS: the sums the
+// sequential loop of FindBestThresholdSequentially(Int) holds when it evaluates
+// bin t (feature_histogram.rs:363-663, 839-1142). Index t + 1 (t may be -1).
+struct ScanPrefix {
+  double g[kMaxBins + 1], h[kMaxBins + 1];
+  long long ig[kMaxBins + 1], ih[kMaxBins + 1];
+  int c[kMaxBins + 1];
+  unsigned char active[kMaxBins + 1];
 };
 
-// One direction of FindBestThresholdSequentially (f64) or ...Int (quantized).
+// Lane 0 runs the accumulation in the reference's order, so f64 prefix sums
+// round exactly as the sequential scan does; integer sums are exact anyway.
 template <bool INT>
-__device__ void seq_scan(const double* G, const double* H, const long long* IG, const long long* IH, const FeatDev& m,
-                         const ScanParams& p, Flags fl, double sum_g, double sum_h, long long tg, long long th,
-                         int num_data, double mgs, double parent, bool rev, bool skip_default, bool na_missing,
-                         SplitOut& out) {
+__device__ void build_prefix(const double* G, const double* H, const long long* IG, const long long* IH,
+                             const FeatDev& m, double sum_g, double sum_h, long long tg, long long th, int num_data,
+                             double cnt_factor, bool rev, bool skip_default, bool na_missing, ScanPrefix& X) {
   const int offset = m.offset;
-  bool splittable = false;
-  double best_gain = kMinScore, best_slg = NAN, best_slh = NAN;
-  long long best_lg = 0, best_lh = 0;
-  int best_lc = 0;
-  unsigned best_th = (unsigned)m.num_bin;
-  const double cnt_factor = INT ? (double)num_data / (double)th : (double)num_data / sum_h;
-  auto gains = [&](double slg, double slh, double srg, double srh, int lc, int rc) -> double {
-    if (INT) {
-      slh += kEps;
-      srh += kEps;
-    }
-    if (!fl.l1 && !fl.mo && !fl.sm) return split_gain_plain(slg, slh, srg, srh, p.l2);
-    double a = leaf_gain(slg, slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
-    double b = leaf_gain(srg, srh, p.l1, p.l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
-    return a + b;
-  };
+  for (int i = 0; i <= kMaxBins; ++i) X.active[i] = 0;
   if (rev) {
     double srg = 0.0, srh = kEps;
     long long rg = 0, rh = 0;
     int rc = 0;
-    const int t_end = 1 - offset;
-    for (int t = m.num_bin - 1 - offset - (na_missing ? 1 : 0); t >= t_end; --t) {
+    for (int t = m.num_bin - 1 - offset - (na_missing ? 1 : 0); t >= 1 - offset; --t) {
       if (skip_default && (t + offset) == m.default_bin) continue;
-      double slh, srg_v, srh_v, slg;
-      int lc;
       if (INT) {
         rg += IG[t];
         rh += IH[t];
-        rc = round_int((double)rh * cnt_factor);
-        srh_v = (double)rh * p.hess_scale;
-        if (rc < p.min_data_in_leaf || srh_v < p.min_sum_hessian) continue;
-        lc = num_data - rc;
-        if (lc < p.min_data_in_leaf) break;
-        long long lh = th - rh;
-        slh = (double)lh * p.hess_scale;
-        if (slh < p.min_sum_hessian) break;
-        long long lg = tg - rg;
-        slg = (double)lg * p.grad_scale;
-        srg_v = (double)rg * p.grad_scale;
-        double cur = gains(slg, slh, srg_v, srh_v, lc, rc);
-        if (cur <= mgs) continue;
-        splittable = true;
-        if (cur > best_gain) {
-          best_lg = lg;
-          best_lh = lh;
-          best_th = (unsigned)(t - 1 + offset);
-          best_gain = cur;
-        }
+        X.ig[t + 1] = rg;
+        X.ih[t + 1] = rh;
       } else {
-        double g = G[t], h = H[t];
-        int cnt = round_int(h * cnt_factor);
+        const double g = G[t], h = H[t];
         srg += g;
         srh += h;
-        rc += cnt;
-        if (rc < p.min_data_in_leaf || srh < p.min_sum_hessian) continue;
-        lc = num_data - rc;
-        if (lc < p.min_data_in_leaf) break;
-        slh = sum_h - srh;
-        if (slh < p.min_sum_hessian) break;
-        slg = sum_g - srg;
-        double cur = gains(slg, slh, srg, srh, lc, rc);
-        if (cur <= mgs) continue;
-        splittable = true;
-        if (cur > best_gain) {
-          best_lc = lc;
-          best_slg = slg;
-          best_slh = slh;
-          best_th = (unsigned)(t - 1 + offset);
-          best_gain = cur;
-        }
+        rc += round_int(h * cnt_factor);
+        X.g[t + 1] = srg;
+        X.h[t + 1] = srh;
+        X.c[t + 1] = rc;
       }
+      X.active[t + 1] = 1;
     }
   } else {
     double slg = 0.0, slh = kEps;
     long long lg = 0, lh = 0;
     int lc = 0;
     int t = 0;
-    const int t_end = m.num_bin - 2 - offset;
     if (na_missing && offset == 1) {
       if (INT) {
         lg = tg;
@@ -626,76 +590,182 @@ __device__ void seq_scan(const double* G, const double* H, const long long* IG, 
         slh = sum_h - kEps;
         lc = num_data;
         for (int i = 0; i < m.num_bin - offset; ++i) {
-          double g = G[i], h = H[i];
-          int cnt = round_int(h * cnt_factor);
+          const double g = G[i], h = H[i];
           slg -= g;
           slh -= h;
-          lc -= cnt;
+          lc -= round_int(h * cnt_factor);
         }
       }
       t = -1;
     }
-    for (; t <= t_end; ++t) {
+    for (; t <= m.num_bin - 2 - offset; ++t) {
       if (skip_default && (t + offset) == m.default_bin) continue;
-      if (INT) {
-        if (t >= 0) {
+      if (t >= 0) {
+        if (INT) {
           lg += IG[t];
           lh += IH[t];
-        }
-        int lcc = round_int((double)lh * cnt_factor);
-        double slh_v = (double)lh * p.hess_scale;
-        if (lcc < p.min_data_in_leaf || slh_v < p.min_sum_hessian) continue;
-        int rc = num_data - lcc;
-        if (rc < p.min_data_in_leaf) break;
-        long long rh = th - lh;
-        double srh = (double)rh * p.hess_scale;
-        if (srh < p.min_sum_hessian) break;
-        long long rg = tg - lg;
-        double slg_v = (double)lg * p.grad_scale;
-        double srg = (double)rg * p.grad_scale;
-        double cur = gains(slg_v, slh_v, srg, srh, lcc, rc);
-        if (cur <= mgs) continue;
-        splittable = true;
-        if (cur > best_gain) {
-          best_lg = lg;
-          best_lh = lh;
-          best_th = (unsigned)(t + offset);
-          best_gain = cur;
-        }
-      } else {
-        if (t >= 0) {
+        } else {
           slg += G[t];
           slh += H[t];
           lc += round_int(H[t] * cnt_factor);
         }
-        if (lc < p.min_data_in_leaf || slh < p.min_sum_hessian) continue;
-        int rc = num_data - lc;
-        if (rc < p.min_data_in_leaf) break;
-        double srh = sum_h - slh;
-        if (srh < p.min_sum_hessian) break;
-        double srg = sum_g - slg;
-        double cur = gains(slg, slh, srg, srh, lc, rc);
-        if (cur <= mgs) continue;
-        splittable = true;
-        if (cur > best_gain) {
-          best_lc = lc;
-          best_slg = slg;
-          best_slh = slh;
-          best_th = (unsigned)(t + offset);
-          best_gain = cur;
-        }
       }
+      if (INT) {
+        X.ig[t + 1] = lg;
+        X.ih[t + 1] = lh;
+      } else {
+        X.g[t + 1] = slg;
+        X.h[t + 1] = slh;
+        X.c[t + 1] = lc;
+      }
+      X.active[t + 1] = 1;
     }
   }
-  if (splittable) out.splittable = 1;
-  if (splittable && best_gain > out.gain + mgs) {
-    out.threshold = (int)best_th;
+}
+
+struct Best {
+  double gain;
+  int pos;  // index t + 1
+};
+
+// earlier in scan order wins ties: higher t when reversing, lower t otherwise
+__device__ __forceinline__ bool better(const Best& a, const Best& b, bool rev) {
+  if (a.pos < 0) return false;
+  if (b.pos < 0) return true;
+  if (a.gain != b.gain) return a.gain > b.gain;
+  return rev ? a.pos > b.pos : a.pos < b.pos;
+}
+
+// One direction evaluated by the 64 lanes of a wave from the prefix state.
+// Returns in `out` what the sequential scan would leave there.
+template <bool INT>
+__device__ void eval_direction(const FeatDev& m, const ScanParams& p, Flags fl, double sum_g, double sum_h,
+                               long long tg, long long th, int num_data, double mgs, double parent,
+                               double cnt_factor, bool rev, const ScanPrefix& X, SplitOut& out) {
+  const int lane = threadIdx.x & 63;
+  const int offset = m.offset;
+  // break position: the first bin in scan order where the side constraints fail
+  int brk = rev ? -1 : kMaxBins + 2;
+  Best best{kMinScore, -1};
+  int any = 0;
+  double g_pos[5];
+  int valid_pos[5];
+  for (int j = 0; j < 5; ++j) {
+    const int pos = lane + 64 * j;
+    valid_pos[j] = 0;
+    g_pos[j] = kMinScore;
+    if (pos > kMaxBins || !X.active[pos]) continue;
+    double slg, slh, srg, srh;
+    int lc, rc;
+    bool c1, c23;
     if (INT) {
-      long long rg = tg - best_lg, rh = th - best_lh;
-      double slg = (double)best_lg * p.grad_scale, slh = (double)best_lh * p.hess_scale;
-      double srg = (double)rg * p.grad_scale, srh = (double)rh * p.hess_scale;
-      int lc = round_int((double)best_lh * cnt_factor);
-      int rc = round_int((double)rh * cnt_factor);
+      if (rev) {
+        const long long rg = X.ig[pos], rh = X.ih[pos];
+        rc = round_int((double)rh * cnt_factor);
+        srh = (double)rh * p.hess_scale;
+        c1 = rc < p.min_data_in_leaf || srh < p.min_sum_hessian;
+        lc = num_data - rc;
+        const long long lh = th - rh;
+        slh = (double)lh * p.hess_scale;
+        c23 = lc < p.min_data_in_leaf || slh < p.min_sum_hessian;
+        slg = (double)(tg - rg) * p.grad_scale;
+        srg = (double)rg * p.grad_scale;
+      } else {
+        const long long lg = X.ig[pos], lh = X.ih[pos];
+        lc = round_int((double)lh * cnt_factor);
+        slh = (double)lh * p.hess_scale;
+        c1 = lc < p.min_data_in_leaf || slh < p.min_sum_hessian;
+        rc = num_data - lc;
+        const long long rh = th - lh;
+        srh = (double)rh * p.hess_scale;
+        c23 = rc < p.min_data_in_leaf || srh < p.min_sum_hessian;
+        slg = (double)lg * p.grad_scale;
+        srg = (double)(tg - lg) * p.grad_scale;
+      }
+    } else {
+      if (rev) {
+        srg = X.g[pos];
+        srh = X.h[pos];
+        rc = X.c[pos];
+        c1 = rc < p.min_data_in_leaf || srh < p.min_sum_hessian;
+        lc = num_data - rc;
+        slh = sum_h - srh;
+        c23 = lc < p.min_data_in_leaf || slh < p.min_sum_hessian;
+        slg = sum_g - srg;
+      } else {
+        slg = X.g[pos];
+        slh = X.h[pos];
+        lc = X.c[pos];
+        c1 = lc < p.min_data_in_leaf || slh < p.min_sum_hessian;
+        rc = num_data - lc;
+        srh = sum_h - slh;
+        c23 = rc < p.min_data_in_leaf || srh < p.min_sum_hessian;
+        srg = sum_g - slg;
+      }
+    }
+    if (c1) continue;
+    if (c23) {
+      brk = rev ? max(brk, pos) : min(brk, pos);
+      continue;
+    }
+    double gain;
+    {
+      double a_slh = slh, a_srh = srh;
+      if (INT) {
+        a_slh += kEps;
+        a_srh += kEps;
+      }
+      if (!fl.l1 && !fl.mo && !fl.sm) {
+        gain = split_gain_plain(slg, a_slh, srg, a_srh, p.l2);
+      } else {
+        double a = leaf_gain(slg, a_slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
+        double b = leaf_gain(srg, a_srh, p.l1, p.l2, p.max_delta_step, p.path_smooth, rc, parent, fl);
+        gain = a + b;
+      }
+    }
+    g_pos[j] = gain;
+    valid_pos[j] = 1;
+  }
+  // reduce the break position over the wave
+  for (int o = 32; o > 0; o >>= 1) {
+    const int other = __shfl_xor(brk, o, 64);
+    brk = rev ? max(brk, other) : min(brk, other);
+  }
+  for (int j = 0; j < 5; ++j) {
+    const int pos = lane + 64 * j;
+    if (!valid_pos[j]) continue;
+    if (rev ? pos <= brk : pos >= brk) continue;
+    const double gain = g_pos[j];
+    if (gain <= mgs) continue;  // NaN gains are not skipped: they count as splittable
+    any = 1;
+    Best c{gain, pos};
+    if (gain == gain && better(c, best, rev)) best = c;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    Best other;
+    other.gain = __shfl_xor(best.gain, o, 64);
+    other.pos = __shfl_xor(best.pos, o, 64);
+    if (better(other, best, rev)) best = other;
+    any |= __shfl_xor(any, o, 64);
+  }
+  if (any) out.splittable = 1;
+  if (any && best.pos >= 0 && best.gain > out.gain + mgs) {
+    const int pos = best.pos, t = pos - 1;
+    out.threshold = rev ? t - 1 + offset : t + offset;
+    if (INT) {
+      long long blg, blh;
+      if (rev) {
+        blg = tg - X.ig[pos];
+        blh = th - X.ih[pos];
+      } else {
+        blg = X.ig[pos];
+        blh = X.ih[pos];
+      }
+      const long long rg = tg - blg, rh = th - blh;
+      const double slg = (double)blg * p.grad_scale, slh = (double)blh * p.hess_scale;
+      const double srg = (double)rg * p.grad_scale, srh = (double)rh * p.hess_scale;
+      const int lc = round_int((double)blh * cnt_factor);
+      const int rc = round_int((double)rh * cnt_factor);
       out.left_output = leaf_output(slg, slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, lc, parent, fl);
       out.left_count = lc;
       out.left_sum_gradient = slg;
@@ -704,55 +774,33 @@ __device__ void seq_scan(const double* G, const double* H, const long long* IG, 
       out.right_count = rc;
       out.right_sum_gradient = srg;
       out.right_sum_hessian = srh;
-      out.left_int_sum = (long long)(((unsigned long long)best_lg << 32) | ((unsigned long long)best_lh & 0xffffffffull));
+      out.left_int_sum = (long long)(((unsigned long long)blg << 32) | ((unsigned long long)blh & 0xffffffffull));
       out.right_int_sum = (long long)(((unsigned long long)rg << 32) | ((unsigned long long)rh & 0xffffffffull));
     } else {
-      out.left_output =
-          leaf_output(best_slg, best_slh, p.l1, p.l2, p.max_delta_step, p.path_smooth, best_lc, parent, fl);
-      out.left_count = best_lc;
-      out.left_sum_gradient = best_slg;
-      out.left_sum_hessian = best_slh - kEps;
-      out.right_output = leaf_output(sum_g - best_slg, sum_h - best_slh, p.l1, p.l2, p.max_delta_step, p.path_smooth,
-                                     num_data - best_lc, parent, fl);
-      out.right_count = num_data - best_lc;
-      out.right_sum_gradient = sum_g - best_slg;
-      out.right_sum_hessian = sum_h - best_slh - kEps;
+      double bslg, bslh;
+      int blc;
+      if (rev) {
+        bslg = sum_g - X.g[pos];
+        bslh = sum_h - X.h[pos];
+        blc = num_data - X.c[pos];
+      } else {
+        bslg = X.g[pos];
+        bslh = X.h[pos];
+        blc = X.c[pos];
+      }
+      out.left_output = leaf_output(bslg, bslh, p.l1, p.l2, p.max_delta_step, p.path_smooth, blc, parent, fl);
+      out.left_count = blc;
+      out.left_sum_gradient = bslg;
+      out.left_sum_hessian = bslh - kEps;
+      out.right_output = leaf_output(sum_g - bslg, sum_h - bslh, p.l1, p.l2, p.max_delta_step, p.path_smooth,
+                                     num_data - blc, parent, fl);
+      out.right_count = num_data - blc;
+      out.right_sum_gradient = sum_g - bslg;
+      out.right_sum_hessian = sum_h - bslh - kEps;
     }
-    out.gain = best_gain - mgs;
+    out.gain = best.gain - mgs;
     out.default_left = rev ? 1 : 0;
   }
-}
-
-template <bool INT>
-__device__ void search_numerical(const double* G, const double* H, const long long* IG, const long long* IH,
-                                 const FeatDev& m, const ScanParams& p, double sum_g_in, double sum_h_in, long long tg,
-                                 long long th, int num_data, double parent, SplitOut& out) {
-  out.default_left = 1;
-  out.gain = kMinScore;
-  Flags fl{p.l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > kEps};
-  double sum_g, sum_h;
-  if (INT) {
-    sum_g = (double)tg * p.grad_scale;
-    sum_h = (double)th * p.hess_scale;
-  } else {
-    sum_g = sum_g_in;
-    sum_h = sum_h_in + 2.0 * kEps;
-  }
-  double gain_shift = leaf_gain(sum_g, sum_h, p.l1, p.l2, p.max_delta_step, p.path_smooth, num_data, parent, fl);
-  double mgs = gain_shift + p.min_gain_to_split;
-  if (m.num_bin > 2 && m.missing_type != 0) {
-    if (m.missing_type == 1) {
-      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, true, false, out);
-      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, false, true, false, out);
-    } else {
-      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, false, true, out);
-      seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, false, false, true, out);
-    }
-  } else {
-    seq_scan<INT>(G, H, IG, IH, m, p, fl, sum_g, sum_h, tg, th, num_data, mgs, parent, true, false, false, out);
-    if (m.missing_type == 2) out.default_left = 0;
-  }
-  out.gain *= m.penalty;
 }
 
 __device__ double leaf_gain_cat(double g, double h, double l1, double l2, double mds, double smooth, int n,
@@ -1110,35 +1158,75 @@ __device__ void cat_sort(const FeatDev& m, const ScanParams& p, const LeafArgs& 
   __syncthreads();
 }
 
-// Exact rows routed left by the chosen split of one leaf (count channel).
+// Exact rows routed left by the chosen split of one leaf (count channel), one wave.
 __device__ void exact_counts(const FeatDev& m, const SliceLds& S, int num_data, SplitOut& o) {
+  const int lane = threadIdx.x & 63;
   RuleDev r = m.bin_type == 1 ? make_rule_categorical(m, o.cat_bits)
                               : make_rule_numerical(m, (uint32_t)o.threshold, o.default_left != 0);
   long long left = 0, inslice = 0;
-  for (int i = 0; i < m.hist_len; ++i) {
+  for (int i = lane; i < m.hist_len; i += 64) {
     inslice += S.c[i];
     if (route_left(r, stored_value(m, i + m.offset))) left += S.c[i];
+  }
+  for (int o2 = 32; o2 > 0; o2 >>= 1) {
+    left += __shfl_xor(left, o2, 64);
+    inslice += __shfl_xor(inslice, o2, 64);
   }
   if (m.offset == 1 && route_left(r, 0u)) left += (long long)num_data - inslice;
   o.exact_left = (int)left;
   o.exact_right = num_data - (int)left;
 }
 
+// Numerical search of one leaf by one wave; every wave of the block calls it
+// with the same feature (uniform barriers), `enabled` off for an absent leaf.
 template <int FORM>
-__device__ void search_leaf(const FeatDev& m, const ScanParams& p, const LeafArgs& a, SliceLds& S, SplitOut& o) {
+__device__ void search_numerical_wave(const FeatDev& m, const ScanParams& p, const LeafArgs& a, const SliceLds& S,
+                                      ScanPrefix& X, bool enabled, SplitOut& out) {
   constexpr bool INT = FORM == FORM_QUANT;
-  o = SplitOut{};
-  o.gain = kMinScore;
-  o.default_left = 1;
-  o.feature = m.real;
-  long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
-  if (m.bin_type == 1) {
-    search_categorical<INT>(S.g, S.h, S.ig, S.ih, m, p, a.sum_g, a.sum_h, tg, th, a.num_data, a.parent_output,
-                            S.sorted, S.used_bin, o);
+  const int lane = threadIdx.x & 63;
+  Flags fl{p.l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > kEps};
+  const long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
+  double sum_g, sum_h;
+  if (INT) {
+    sum_g = (double)tg * p.grad_scale;
+    sum_h = (double)th * p.hess_scale;
   } else {
-    search_numerical<INT>(S.g, S.h, S.ig, S.ih, m, p, a.sum_g, a.sum_h, tg, th, a.num_data, a.parent_output, o);
+    sum_g = a.sum_g;
+    sum_h = a.sum_h + 2.0 * kEps;
   }
-  if (o.gain > kMinScore) exact_counts(m, S, a.num_data, o);
+  const double cnt_factor = INT ? (double)a.num_data / (double)th : (double)a.num_data / sum_h;
+  const double gain_shift =
+      leaf_gain(sum_g, sum_h, p.l1, p.l2, p.max_delta_step, p.path_smooth, a.num_data, a.parent_output, fl);
+  const double mgs = gain_shift + p.min_gain_to_split;
+  out.default_left = 1;
+  out.gain = kMinScore;
+  bool dirs_rev[2], dirs_skip[2], dirs_na[2];
+  int nd;
+  if (m.num_bin > 2 && m.missing_type != 0) {
+    nd = 2;
+    const bool zero = m.missing_type == 1;
+    dirs_rev[0] = true;
+    dirs_rev[1] = false;
+    dirs_skip[0] = dirs_skip[1] = zero;
+    dirs_na[0] = dirs_na[1] = !zero;
+  } else {
+    nd = 1;
+    dirs_rev[0] = true;
+    dirs_skip[0] = false;
+    dirs_na[0] = false;
+  }
+  for (int d = 0; d < nd; ++d) {
+    if (enabled && lane == 0)
+      build_prefix<INT>(S.g, S.h, S.ig, S.ih, m, sum_g, sum_h, tg, th, a.num_data, cnt_factor, dirs_rev[d],
+                        dirs_skip[d], dirs_na[d], X);
+    __syncthreads();
+    if (enabled)
+      eval_direction<INT>(m, p, fl, sum_g, sum_h, tg, th, a.num_data, mgs, a.parent_output, cnt_factor, dirs_rev[d],
+                          X, out);
+    __syncthreads();
+  }
+  if (nd == 1 && m.missing_type == 2) out.default_left = 0;
+  out.gain *= m.penalty;
 }
 
 template <int FORM>
@@ -1146,6 +1234,7 @@ __global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafAr
                                               int use_subtract, ScanParams p, const int* __restrict__ used,
                                               SplitOut* out) {
   __shared__ SliceLds S[2];
+  __shared__ ScanPrefix X[2];
   __shared__ int elig[2][kMaxBins];
   const int f = blockIdx.x;
   if (!used[f]) return;
@@ -1169,21 +1258,33 @@ __global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafAr
     store_slice<FORM>(s, m, large.slot, S[1]);
   }
   __syncthreads();
-  // the ctr sort of a categorical feature uses the whole block, one leaf at a time
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool enabled = wave == 0 || has_large;
+  const LeafArgs& a = wave == 0 ? small : large;
+  SplitOut o = SplitOut{};
+  o.gain = kMinScore;
+  o.default_left = 1;
+  o.feature = m.real;
+  const long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
   if (m.bin_type == 1) {
+    // the ctr sort uses the whole block, one leaf at a time; the scan runs on lane 0
     cat_sort<FORM>(m, p, small, S[0], elig[0]);
     if (has_large) cat_sort<FORM>(m, p, large, S[1], elig[1]);
+    __syncthreads();
+    if (enabled && lane == 0)
+      search_categorical<FORM == FORM_QUANT>(S[wave].g, S[wave].h, S[wave].ig, S[wave].ih, m, p, a.sum_g, a.sum_h, tg,
+                                             th, a.num_data, a.parent_output, S[wave].sorted, S[wave].used_bin, o);
+    // lane 0's result to the whole wave
+    o.gain = __shfl(o.gain, 0, 64);
+    o.num_cat = __shfl(o.num_cat, 0, 64);
+    for (int i = 0; i < 8; ++i) o.cat_bits[i] = (uint32_t)__shfl((int)o.cat_bits[i], 0, 64);
+  } else {
+    search_numerical_wave<FORM>(m, p, a, S[wave], X[wave], enabled, o);
   }
-  __syncthreads();
-  // the two searches run in separate waves
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    if (wave == 0) {
-      search_leaf<FORM>(m, p, small, S[0], out[f]);
-    } else if (has_large) {
-      search_leaf<FORM>(m, p, large, S[1], out[s.num_features + f]);
-    }
-  }
+  if (!enabled) return;
+  o.feature = m.real;
+  if (o.gain > kMinScore) exact_counts(m, S[wave], a.num_data, o);
+  if (lane == 0) out[(wave == 0 ? 0 : s.num_features) + f] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,22 +1361,48 @@ __global__ void k_pack_f32(const float* __restrict__ g, const float* __restrict_
 }
 
 // renew_int_grad_tree_output (serial.rs:634-655): sequential f64 sums over each
-// leaf's rows in ascending row order. Lane 0 sums gradients, lane 1 hessians.
-__global__ void k_renew_sums(DevState s, const Segment* __restrict__ segs, const float* __restrict__ g,
-                             const float* __restrict__ h, double* out) {
-  const int leaf = blockIdx.x;
+// leaf's rows in ascending row order. The order of additions is the reference's,
+// so the chain itself stays on one lane: wave 0 sums gradients, wave 1 hessians.
+// The other lanes gather the next 256 rows into LDS while lane 0 adds the
+// current ones, so the chain is limited by the f64 add latency, not by memory.
+constexpr int kRenewChunk = 256;
+__global__ void __launch_bounds__(128) k_renew_sums(DevState s, const Segment* __restrict__ segs,
+                                                    const int* __restrict__ order, const float* __restrict__ g,
+                                                    const float* __restrict__ h, double* out) {
+  __shared__ float buf[2][2][kRenewChunk];
+  const int leaf = order[blockIdx.x];
   const Segment seg = segs[leaf];
-  const int lane = threadIdx.x;
-  if (lane > 1) return;
-  const float* src = lane == 0 ? g : h;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float* src = wave == 0 ? g : h;
+  const int32_t* ix = seg.buf == 0 ? nullptr : s.idx[seg.buf];
+  const int nchunk = (seg.count + kRenewChunk - 1) / kRenewChunk;
+  float r[kRenewChunk / 64];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int j = 0; j < kRenewChunk / 64; ++j) {
+      const int p = c * kRenewChunk + lane + 64 * j;
+      float v = 0.0f;
+      if (p < seg.count) {
+        const int pos = seg.begin + p;
+        v = src[ix ? ix[pos] : pos];
+      }
+      r[j] = v;
+    }
+  };
   double acc = 0.0;
-  if (seg.buf == 0) {
-    for (int p = seg.begin; p < seg.begin + seg.count; ++p) acc += (double)src[p];
-  } else {
-    const int32_t* ix = s.idx[seg.buf];
-    for (int p = seg.begin; p < seg.begin + seg.count; ++p) acc += (double)src[ix[p]];
+  if (nchunk > 0) fetch(0);
+  for (int c = 0; c < nchunk; ++c) {
+    float* cur = buf[wave][c & 1];
+#pragma unroll
+    for (int j = 0; j < kRenewChunk / 64; ++j) cur[lane + 64 * j] = r[j];
+    if (c + 1 < nchunk) fetch(c + 1);
+    __syncthreads();
+    if (lane == 0) {
+      const int n = min(kRenewChunk, seg.count - c * kRenewChunk);
+      for (int i = 0; i < n; ++i) acc += (double)cur[i];
+    }
   }
-  out[2 * leaf + lane] = acc;
+  if (lane == 0) out[2 * leaf + wave] = acc;
 }
 
 __global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const double* __restrict__ vals,
@@ -1409,13 +1536,14 @@ void launch_partition_count(const DevState& s, Segment seg, int split_group, con
 
 void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
                               int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
-                              int part_set, int max_tile_bins, const int* d_tile_first, int ntiles,
-                              int rows_per_flush, void* stream) {
+                              int part_set, int max_tile_bins, int max_tile_groups, const int* d_tile_first,
+                              int ntiles, int rows_per_flush, void* stream) {
   if (bp.nblk <= 0) return;
   hipStream_t st = (hipStream_t)stream;
   dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
   const size_t per_bin = s.form == FORM_QUANT ? 8 : 20;
-  size_t lds = hist_side >= 0 ? (size_t)max_tile_bins * per_bin : 0;
+  size_t lds = hist_side >= 0 ? (((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15) : 0;
+  lds += (size_t)max_tile_groups * kTileRows + (size_t)kTileRows * (4 + (s.form == FORM_QUANT ? 2 : 8));
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_part_scatter<FORM_QUANT>);
@@ -1439,10 +1567,11 @@ void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int s
 
 void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
                       int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
-                      int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+                      int max_tile_bins, int max_tile_groups, const int* d_tile_first, int ntiles, int rows_per_flush,
+                      void* stream) {
   launch_partition_count(s, seg, split_group, rule, bp, stream);
   launch_partition_scatter(s, seg, dst_buf, split_group, rule, left_total, bp, hist_side, d_group_used, part_set,
-                           max_tile_bins, d_tile_first, ntiles, rows_per_flush, stream);
+                           max_tile_bins, max_tile_groups, d_tile_first, ntiles, rows_per_flush, stream);
 }
 
 void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
@@ -1486,10 +1615,10 @@ void launch_pack_f32(const float* g, const float* h, int n, int const_hess, floa
   BK_HIP(hipGetLastError());
 }
 
-void launch_renew_sums(const DevState& s, const Segment* d_segs, int num_leaves, const float* g, const float* h,
-                       double* d_sums, void* stream) {
-  hipLaunchKernelGGL(k_renew_sums, dim3((unsigned)num_leaves), dim3(64), 0, (hipStream_t)stream, s, d_segs, g, h,
-                     d_sums);
+void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
+                       const float* h, double* d_sums, void* stream) {
+  hipLaunchKernelGGL(k_renew_sums, dim3((unsigned)num_leaves), dim3(128), 0, (hipStream_t)stream, s, d_segs, d_order, g,
+                     h, d_sums);
   BK_HIP(hipGetLastError());
 }
 

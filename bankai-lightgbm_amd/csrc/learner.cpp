@@ -344,6 +344,10 @@ struct Learner {
   // group tiles for LDS
   std::vector<int> tile_first;
   int max_tile_bins = 0;
+  int max_tile_groups = 0;
+  std::vector<int> htile_first;
+  int max_htile_bins = 0;
+  int* d_htile_first = nullptr;
   int rows_per_flush = 0;
   size_t cache = 0;
 
@@ -374,6 +378,8 @@ struct Learner {
   double* d_sum_partials = nullptr;
   Segment* d_segs = nullptr;
   double* d_renew = nullptr;
+  int* d_order = nullptr;
+  std::vector<int> renew_order;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
   // pinned host staging
@@ -444,6 +450,7 @@ struct Learner {
     f(d_blk_cnt);
     f(d_err);
     f(d_tile_first);
+    f(d_htile_first);
     f(d_used);
     f(d_group_used);
     f(d_out);
@@ -452,6 +459,7 @@ struct Learner {
     f(d_sum_partials);
     f(d_segs);
     f(d_renew);
+    f(d_order);
     f(d_leaf_vals);
     f(d_leaf_of);
     if (h_out) (void)hipHostFree(h_out);
@@ -464,7 +472,7 @@ struct Learner {
   BlockPlan plan(int count) const {
     BlockPlan bp{0, 0};
     if (count <= 0) return bp;
-    const int min_chunk = 4 * kTileRows;
+    const int min_chunk = kTileRows;
     long long chunk = ((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks;
     chunk = std::max<long long>(chunk, min_chunk);
     chunk = (chunk + kTileRows - 1) / kTileRows * kTileRows;
@@ -547,17 +555,37 @@ struct Learner {
     tile_first.assign(1, 0);
     max_tile_bins = 0;
     int cur = 0;
+    int cur_groups = 0;
+    for (int g = 0; g < G; ++g) {
+      int nb = v->group_num_total_bin[g];
+      if (cur + nb > tile_cap || cur_groups == kTileGroups) {
+        tile_first.push_back(g);
+        max_tile_bins = std::max(max_tile_bins, cur);
+        cur = 0;
+        cur_groups = 0;
+      }
+      cur += nb;
+      cur_groups += 1;
+    }
+    tile_first.push_back(G);
+    max_tile_bins = std::max(max_tile_bins, cur);
+    for (size_t t = 0; t + 1 < tile_first.size(); ++t)
+      max_tile_groups = std::max(max_tile_groups, tile_first[t + 1] - tile_first[t]);
+    // segment histograms stream every column once: tiles as large as LDS allows
+    htile_first.assign(1, 0);
+    max_htile_bins = 0;
+    cur = 0;
     for (int g = 0; g < G; ++g) {
       int nb = v->group_num_total_bin[g];
       if (cur + nb > tile_cap) {
-        tile_first.push_back(g);
-        max_tile_bins = std::max(max_tile_bins, cur);
+        htile_first.push_back(g);
+        max_htile_bins = std::max(max_htile_bins, cur);
         cur = 0;
       }
       cur += nb;
     }
-    tile_first.push_back(G);
-    max_tile_bins = std::max(max_tile_bins, cur);
+    htile_first.push_back(G);
+    max_htile_bins = std::max(max_htile_bins, cur);
     // rows one LDS accumulation may hold: the packed quantized fields stay exact
     if (form == FORM_QUANT) {
       int maxq = bins + 1;
@@ -608,6 +636,9 @@ struct Learner {
     d_tile_first = dmalloc<int>(tile_first.size());
     BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
                            stream));
+    d_htile_first = dmalloc<int>(htile_first.size());
+    BK_HIPC(hipMemcpyAsync(d_htile_first, htile_first.data(), sizeof(int) * htile_first.size(), hipMemcpyHostToDevice,
+                           stream));
     d_used = dmalloc<int>((size_t)F);
     d_group_used = dmalloc<int>((size_t)G);
     d_out = dmalloc<SplitOut>((size_t)F * 2);
@@ -616,6 +647,7 @@ struct Learner {
     d_sum_partials = dmalloc<double>(2048 + 2);
     d_segs = dmalloc<Segment>((size_t)P.num_leaves);
     d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
+    d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
     BK_HIPC(hipHostMalloc((void**)&h_out, sizeof(SplitOut) * (size_t)F * 2, hipHostMallocDefault));
     if (stochastic) {
@@ -894,7 +926,7 @@ struct Learner {
     BlockPlan bp = plan(N);
     const double hb = (double)N * (used_group_count() + payload_bytes());
     timed(0, hb, hb, [&] {
-      launch_hist(ds, segs[0], bp, d_group_used, 0, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
+      launch_hist(ds, segs[0], bp, d_group_used, 0, max_htile_bins, d_htile_first, (int)htile_first.size() - 1,
                   rows_per_flush, stream);
     });
     stats.hist_rows += N;
@@ -979,11 +1011,11 @@ struct Learner {
         prof.end(stream, i0, 1, q * 1.0, q * 1.0);
         size_t i1 = prof.begin(stream);
         launch_partition_scatter(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins,
-                                 d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+                                 max_tile_groups, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
         prof.end(stream, i1, 2, algo, moved);
       } else {
-        launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins, d_tile_first,
-                         (int)tile_first.size() - 1, rows_per_flush, stream);
+        launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins,
+                         max_tile_groups, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
       }
     }
     stats.part_rows += ps.count;
@@ -998,7 +1030,7 @@ struct Learner {
         if (!use_subtract) {
           const Segment lseg = segs[(size_t)larger.leaf];
           BlockPlan lbp = plan(lseg.count);
-          launch_hist(ds, lseg, lbp, d_group_used, 1, max_tile_bins, d_tile_first, (int)tile_first.size() - 1,
+          launch_hist(ds, lseg, lbp, d_group_used, 1, max_htile_bins, d_htile_first, (int)htile_first.size() - 1,
                       rows_per_flush, stream);
           stats.hist_rows += lseg.count;
           pend_large.nblk = lbp.nblk;
@@ -1028,8 +1060,15 @@ struct Learner {
     if (renew) {
       BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
                              stream));
+      // largest leaves first: the longest chains start on the first blocks dispatched
+      renew_order.resize((size_t)t.num_leaves);
+      for (int i = 0; i < t.num_leaves; ++i) renew_order[(size_t)i] = i;
+      std::stable_sort(renew_order.begin(), renew_order.end(),
+                       [&](int a, int b) { return segs[(size_t)a].count > segs[(size_t)b].count; });
+      BK_HIPC(hipMemcpyAsync(d_order, renew_order.data(), sizeof(int) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
+                             stream));
       timed(5, (double)N * 12.0, (double)N * 12.0,
-            [&] { launch_renew_sums(ds, d_segs, t.num_leaves, g, h, d_renew, stream); });
+            [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, d_renew, stream); });
       std::vector<double> sums((size_t)t.num_leaves * 2);
       BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
       BK_HIPC(hipStreamSynchronize(stream));
