@@ -22,6 +22,7 @@ constexpr int kRowsPerThread = 4;     // rows per thread per tile (one dword of 
 constexpr int kTileRows = kStreamThreads * kRowsPerThread;
 constexpr int kMaxRowBlocks = 512;    // row blocks per histogram build (partials)
 constexpr int kLdsBytes = 120 * 1024; // dynamic LDS budget per workgroup on gfx950 (160 KB per CU)
+constexpr int kScanThreads = 256;    // threads of the split-search blocks
 constexpr int kTileGroups = 4;       // columns per group tile of the partition / histogram blocks
 
 // Per inner feature, as the kernels read it.

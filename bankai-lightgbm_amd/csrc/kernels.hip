@@ -542,6 +542,112 @@ struct ScanPrefix {
   unsigned char active[kMaxBins + 1];
 };
 
+// Integer prefix state by a wave scan (exact in any order). Lane l owns bins
+// 4l..4l+3; the reverse direction is a suffix scan, the forward one a prefix
+// scan starting from the NA-as-missing initial sums at t = -1.
+__device__ void build_prefix_int_wave(const long long* IG, const long long* IH, const FeatDev& m, long long tg,
+                                      long long th, bool rev, bool skip_default, bool na_missing, ScanPrefix& X) {
+  const int lane = threadIdx.x & 63;
+  const int offset = m.offset;
+  int lo, hi;  // active bins [lo, hi] of this direction
+  if (rev) {
+    hi = m.num_bin - 1 - offset - (na_missing ? 1 : 0);
+    lo = 1 - offset;
+  } else {
+    lo = 0;
+    hi = m.num_bin - 2 - offset;
+  }
+  long long vg[4], vh[4];
+  bool act[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = 4 * lane + k;
+    act[k] = t >= lo && t <= hi && !(skip_default && (t + offset) == m.default_bin);
+    vg[k] = act[k] ? IG[t] : 0;
+    vh[k] = act[k] ? IH[t] : 0;
+  }
+  if (rev) {
+    long long sg[4], sh[4];
+    sg[3] = vg[3];
+    sh[3] = vh[3];
+#pragma unroll
+    for (int k = 2; k >= 0; --k) {
+      sg[k] = sg[k + 1] + vg[k];
+      sh[k] = sh[k + 1] + vh[k];
+    }
+    // exclusive suffix over the lanes above
+    long long eg = sg[0], eh = sh[0];
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long yg = __shfl_down(eg, o, 64), yh = __shfl_down(eh, o, 64);
+      if (lane + o < 64) {
+        eg += yg;
+        eh += yh;
+      }
+    }
+    eg -= sg[0];
+    eh -= sh[0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = 4 * lane + k;
+      X.ig[t + 1] = sg[k] + eg;
+      X.ih[t + 1] = sh[k] + eh;
+      X.active[t + 1] = act[k];
+    }
+    if (lane == 0) X.active[0] = 0;
+  } else {
+    long long ig0 = 0, ih0 = 0;
+    bool start_na = na_missing && offset == 1;
+    if (start_na) {
+      // tg - sum of every bin of the slice
+      long long a = 0, b = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int t = 4 * lane + k;
+        if (t < m.num_bin - offset) {
+          a += IG[t];
+          b += IH[t];
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+      }
+      ig0 = tg - a;
+      ih0 = th - b;
+    }
+    long long pg[4], ph[4];
+    pg[0] = vg[0];
+    ph[0] = vh[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+      pg[k] = pg[k - 1] + vg[k];
+      ph[k] = ph[k - 1] + vh[k];
+    }
+    long long eg = pg[3], eh = ph[3];
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long yg = __shfl_up(eg, o, 64), yh = __shfl_up(eh, o, 64);
+      if (lane >= o) {
+        eg += yg;
+        eh += yh;
+      }
+    }
+    eg -= pg[3];
+    eh -= ph[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = 4 * lane + k;
+      X.ig[t + 1] = ig0 + pg[k] + eg;
+      X.ih[t + 1] = ih0 + ph[k] + eh;
+      X.active[t + 1] = act[k];
+    }
+    if (lane == 0) {
+      X.ig[0] = ig0;
+      X.ih[0] = ih0;
+      X.active[0] = start_na;
+    }
+  }
+}
+
 // Lane 0 runs the accumulation in the reference's order, so f64 prefix sums
 // round exactly as the sequential scan does; integer sums are exact anyway.
 template <bool INT>
@@ -1021,27 +1127,43 @@ template <int FORM>
 __device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, const LeafArgs& a, const ScanParams& p,
                                          SliceLds& S) {
   const int set = a.part_set;
+  const size_t stride = (size_t)s.total_bins;
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
-      long long g = 0, h = 0, c = 0;
-      for (int j = 0; j < a.nblk; ++j) {
-        long long gg, hh, cc;
-        qunpack(s.part_q[set][(size_t)j * s.total_bins + bin], &gg, &hh, &cc);
-        g += gg;
-        h += hh;
-        c += cc;
+      const unsigned long long* src = s.part_q[set] + bin;
+      long long g[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+      int j = 0;
+      for (; j + 4 <= a.nblk; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          long long gg, hh, cc;
+          qunpack(src[(size_t)(j + u) * stride], &gg, &hh, &cc);
+          g[u] += gg;
+          h[u] += hh;
+          c[u] += cc;
+        }
       }
-      S.ig[i] = g;
-      S.ih[i] = h;
-      S.c[i] = (int)c;
+      for (; j < a.nblk; ++j) {
+        long long gg, hh, cc;
+        qunpack(src[(size_t)j * stride], &gg, &hh, &cc);
+        g[0] += gg;
+        h[0] += hh;
+        c[0] += cc;
+      }
+      S.ig[i] = (g[0] + g[1]) + (g[2] + g[3]);
+      S.ih[i] = (h[0] + h[1]) + (h[2] + h[3]);
+      S.c[i] = (int)((c[0] + c[1]) + (c[2] + c[3]));
     } else {
+      const double* sg = s.part_g[set] + bin;
+      const double* sh = s.part_h[set] + bin;
+      const unsigned* sc = s.part_c[set] + bin;
       double g = 0.0, h = 0.0;
       unsigned c = 0;
       for (int j = 0; j < a.nblk; ++j) {
-        g += s.part_g[set][(size_t)j * s.total_bins + bin];
-        h += s.part_h[set][(size_t)j * s.total_bins + bin];
-        c += s.part_c[set][(size_t)j * s.total_bins + bin];
+        g += sg[(size_t)j * stride];
+        h += sh[(size_t)j * stride];
+        c += sc[(size_t)j * stride];
       }
       if (p.const_hess) h *= p.h0;
       S.g[i] = g;
@@ -1052,32 +1174,56 @@ __device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, co
 }
 
 // fix_slice / fix_slice_int (serial.rs:2205-2238) plus the count channel.
+// Integer sums are exact in any order, so they are reduced by the block; the
+// f64 form keeps the reference's sequential subtraction order on one lane.
+// Must be called by the whole block (it synchronizes).
 template <int FORM>
-__device__ void fix_slice(const FeatDev& m, const LeafArgs& a, SliceLds& S) {
-  if (threadIdx.x != 0 || m.mfb <= 0) return;
+__device__ void fix_slice(const FeatDev& m, const LeafArgs& a, SliceLds& S, long long* red) {
   const int mfb = m.mfb, nb = m.num_bin;
-  if (FORM == FORM_QUANT) {
-    long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
-    S.ig[mfb] = tg;
-    S.ih[mfb] = th;
-    for (int i = 0; i < nb; ++i)
-      if (i != mfb) {
-        S.ig[mfb] -= S.ig[i];
-        S.ih[mfb] -= S.ih[i];
-      }
-  } else {
-    S.g[mfb] = a.sum_g;
-    S.h[mfb] = a.sum_h;
-    for (int i = 0; i < nb; ++i)
-      if (i != mfb) {
-        S.g[mfb] -= S.g[i];
-        S.h[mfb] -= S.h[i];
-      }
+  if (mfb <= 0) return;  // uniform over the block
+  long long sg = 0, sh = 0, sc = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    if (i == mfb) continue;
+    if (FORM == FORM_QUANT) {
+      sg += S.ig[i];
+      sh += S.ih[i];
+    }
+    sc += S.c[i];
   }
-  int c = a.num_data;
-  for (int i = 0; i < nb; ++i)
-    if (i != mfb) c -= S.c[i];
-  S.c[mfb] = c;
+  for (int o = 32; o > 0; o >>= 1) {
+    sg += __shfl_xor(sg, o, 64);
+    sh += __shfl_xor(sh, o, 64);
+    sc += __shfl_xor(sc, o, 64);
+  }
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * wave] = sg;
+    red[3 * wave + 1] = sh;
+    red[3 * wave + 2] = sc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long tg = 0, th = 0, tc = 0;
+    for (int w = 0; w < nw; ++w) {
+      tg += red[3 * w];
+      th += red[3 * w + 1];
+      tc += red[3 * w + 2];
+    }
+    if (FORM == FORM_QUANT) {
+      S.ig[mfb] = (a.int_sum >> 32) - tg;
+      S.ih[mfb] = (a.int_sum & 0xffffffffLL) - th;
+    } else {
+      S.g[mfb] = a.sum_g;
+      S.h[mfb] = a.sum_h;
+      for (int i = 0; i < nb; ++i)
+        if (i != mfb) {
+          S.g[mfb] -= S.g[i];
+          S.h[mfb] -= S.h[i];
+        }
+    }
+    S.c[mfb] = (int)((long long)a.num_data - tc);
+  }
+  __syncthreads();
 }
 
 template <int FORM>
@@ -1216,9 +1362,12 @@ __device__ void search_numerical_wave(const FeatDev& m, const ScanParams& p, con
     dirs_na[0] = false;
   }
   for (int d = 0; d < nd; ++d) {
-    if (enabled && lane == 0)
+    if (INT) {
+      if (enabled) build_prefix_int_wave(S.ig, S.ih, m, tg, th, dirs_rev[d], dirs_skip[d], dirs_na[d], X);
+    } else if (enabled && lane == 0) {
       build_prefix<INT>(S.g, S.h, S.ig, S.ih, m, sum_g, sum_h, tg, th, a.num_data, cnt_factor, dirs_rev[d],
                         dirs_skip[d], dirs_na[d], X);
+    }
     __syncthreads();
     if (enabled)
       eval_direction<INT>(m, p, fl, sum_g, sum_h, tg, th, a.num_data, mgs, a.parent_output, cnt_factor, dirs_rev[d],
@@ -1230,7 +1379,7 @@ __device__ void search_numerical_wave(const FeatDev& m, const ScanParams& p, con
 }
 
 template <int FORM>
-__global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafArgs large, int has_large,
+__global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, LeafArgs small, LeafArgs large, int has_large,
                                               int use_subtract, ScanParams p, const int* __restrict__ used,
                                               SplitOut* out) {
   __shared__ SliceLds S[2];
@@ -1240,10 +1389,10 @@ __global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafAr
   if (!used[f]) return;
   const FeatDev m = s.feats[f];
   // smaller leaf: reduce the row-block partials, fix, store
+  __shared__ long long red[3 * (kScanThreads / 64)];
   load_slice_from_partials<FORM>(s, m, small, p, S[0]);
   __syncthreads();
-  fix_slice<FORM>(m, small, S[0]);
-  __syncthreads();
+  fix_slice<FORM>(m, small, S[0], red);
   store_slice<FORM>(s, m, small.slot, S[0]);
   if (has_large) {
     if (use_subtract) {
@@ -1252,15 +1401,17 @@ __global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafAr
     } else {
       load_slice_from_partials<FORM>(s, m, large, p, S[1]);
       __syncthreads();
-      fix_slice<FORM>(m, large, S[1]);
-      __syncthreads();
+      fix_slice<FORM>(m, large, S[1], red);
     }
     store_slice<FORM>(s, m, large.slot, S[1]);
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const bool enabled = wave == 0 || has_large;
-  const LeafArgs& a = wave == 0 ? small : large;
+  // waves 0 and 1 search the smaller and the larger leaf; the others only help
+  // with the reductions above and take part in the barriers below
+  const bool enabled = wave == 0 || (wave == 1 && has_large);
+  const int wl = wave < 2 ? wave : 0;
+  const LeafArgs& a = wl == 0 ? small : large;
   SplitOut o = SplitOut{};
   o.gain = kMinScore;
   o.default_left = 1;
@@ -1272,19 +1423,19 @@ __global__ void __launch_bounds__(128) k_scan(DevState s, LeafArgs small, LeafAr
     if (has_large) cat_sort<FORM>(m, p, large, S[1], elig[1]);
     __syncthreads();
     if (enabled && lane == 0)
-      search_categorical<FORM == FORM_QUANT>(S[wave].g, S[wave].h, S[wave].ig, S[wave].ih, m, p, a.sum_g, a.sum_h, tg,
-                                             th, a.num_data, a.parent_output, S[wave].sorted, S[wave].used_bin, o);
+      search_categorical<FORM == FORM_QUANT>(S[wl].g, S[wl].h, S[wl].ig, S[wl].ih, m, p, a.sum_g, a.sum_h, tg, th,
+                                             a.num_data, a.parent_output, S[wl].sorted, S[wl].used_bin, o);
     // lane 0's result to the whole wave
     o.gain = __shfl(o.gain, 0, 64);
     o.num_cat = __shfl(o.num_cat, 0, 64);
     for (int i = 0; i < 8; ++i) o.cat_bits[i] = (uint32_t)__shfl((int)o.cat_bits[i], 0, 64);
   } else {
-    search_numerical_wave<FORM>(m, p, a, S[wave], X[wave], enabled, o);
+    search_numerical_wave<FORM>(m, p, a, S[wl], X[wl], enabled, o);
   }
   if (!enabled) return;
   o.feature = m.real;
-  if (o.gain > kMinScore) exact_counts(m, S[wave], a.num_data, o);
-  if (lane == 0) out[(wave == 0 ? 0 : s.num_features) + f] = o;
+  if (o.gain > kMinScore) exact_counts(m, S[wl], a.num_data, o);
+  if (lane == 0) out[(wl == 0 ? 0 : s.num_features) + f] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -1581,15 +1732,15 @@ void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large
   int has_large = large ? 1 : 0;
   switch (s.form) {
     case FORM_QUANT:
-      hipLaunchKernelGGL(k_scan<FORM_QUANT>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg,
+      hipLaunchKernelGGL(k_scan<FORM_QUANT>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg,
                          has_large, use_subtract, sp, d_used, d_out);
       break;
     case FORM_F64:
-      hipLaunchKernelGGL(k_scan<FORM_F64>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg, has_large,
+      hipLaunchKernelGGL(k_scan<FORM_F64>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg, has_large,
                          use_subtract, sp, d_used, d_out);
       break;
     default:
-      hipLaunchKernelGGL(k_scan<FORM_F32C>, dim3((unsigned)s.num_features), dim3(128), 0, st, s, small, lg,
+      hipLaunchKernelGGL(k_scan<FORM_F32C>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg,
                          has_large, use_subtract, sp, d_used, d_out);
       break;
   }

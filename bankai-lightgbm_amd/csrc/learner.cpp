@@ -472,7 +472,7 @@ struct Learner {
   BlockPlan plan(int count) const {
     BlockPlan bp{0, 0};
     if (count <= 0) return bp;
-    const int min_chunk = kTileRows;
+    const int min_chunk = 4 * kTileRows;
     long long chunk = ((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks;
     chunk = std::max<long long>(chunk, min_chunk);
     chunk = (chunk + kTileRows - 1) / kTileRows * kTileRows;
