@@ -142,9 +142,8 @@ struct LeafArgs {
   int leaf, slot, num_data;     // num_data = exact rows of the leaf
   double sum_g, sum_h, parent_output;
   long long int_sum;
-  int from_partials;            // 1: reduce the partial buffer; 0: subtract (larger leaf)
-  int part_set;                 // which partial buffer (0/1)
-  int nblk;                     // row blocks that wrote partials
+  int part_first;               // first partial histogram (task index) of its build
+  int nblk;                     // partial histograms to reduce
 };
 
 struct ScanParams {
@@ -165,11 +164,11 @@ struct DevState {
   const int* group_bin_start;    // first histogram bin of each group
   double* pool;                  // slots x total_bins x 2 (double g,h or int64 g,h)
   int* pool_cnt;                 // slots x total_bins
-  unsigned long long* part_q[2]; // quantized partials: kMaxRowBlocks x total_bins
-  double* part_g[2];
-  double* part_h[2];
-  unsigned* part_c[2];
-  int* blk_cnt;                  // per row block left counts of a partition
+  unsigned long long* part_q;    // quantized partial histograms: one per task, total_bins each
+  double* part_g;                // float forms: gradient, hessian and count planes
+  double* part_h;
+  unsigned* part_c;
+  int* blk_cnt;                  // left rows of every partition task
   int* err_flag;
 };
 
@@ -178,26 +177,45 @@ struct Segment {
   int buf, begin, count;
 };
 
-// Row-block decomposition of one histogram build or partition.
-struct BlockPlan {
-  int nblk, chunk;  // chunk: rows per block, multiple of 4
+// One row block of one item of a launch.
+struct Task {
+  int item, blk;
+};
+
+// A histogram build over a whole segment, split in ntasks row blocks of `chunk` rows.
+struct HistItem {
+  Segment seg;
+  int first_task, ntasks, chunk, pad;
+};
+
+// One leaf split: its rows move from seg (buffer seg.buf) to dst_buf, left rows
+// first; the child on hist_side (0 left, 1 right, -1 none) gets a histogram.
+struct PartItem {
+  Segment seg;
+  int dst_buf, split_group, left_total, hist_side;
+  int first_task, ntasks, chunk, pad;
+  RuleDev rule;
+};
+
+// One search launch item: the smaller child (partials) and the larger one
+// (subtraction from the parent slot, or its own partials).
+struct ScanItem {
+  LeafArgs small, large;
+  int has_large, use_subtract, used_off, out_off;
 };
 
 // Launchers (kernels.hip). All run on `stream` (a hipStream_t passed as void*).
-void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int* d_group_used, int part_set,
-                 int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
-void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
-                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
-                      int max_tile_bins, int max_tile_groups, const int* d_tile_first, int ntiles, int rows_per_flush,
-                      void* stream);
-void launch_partition_count(const DevState& s, Segment seg, int split_group, const RuleDev& rule,
-                            const BlockPlan& bp, void* stream);
-void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
-                              int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
-                              int part_set, int max_tile_bins, int max_tile_groups, const int* d_tile_first,
-                              int ntiles, int rows_per_flush, void* stream);
-void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
-                 const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream);
+// Tasks `d_tasks[0..ntasks)` write partial histograms task_base .. task_base+ntasks-1.
+void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
+                 const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
+                 void* stream);
+void launch_partition_count(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                            void* stream);
+void launch_partition_scatter(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                              bool any_hist, const int* d_group_used, int max_tile_bins, int max_tile_groups,
+                              const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
+                 SplitOut* d_out, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,

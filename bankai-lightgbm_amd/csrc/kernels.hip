@@ -116,8 +116,8 @@ struct LdsHist<FORM_QUANT> {
     for (int i = threadIdx.x; i < nb; i += blockDim.x) q[i] = 0ull;
   }
   __device__ __forceinline__ void add(int b, unsigned long long v) { atomicAdd(&q[b], v); }
-  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
-    unsigned long long* dst = s.part_q[set] + (size_t)blk * s.total_bins + bin_lo;
+  __device__ void flush(const DevState& s, int blk, int bin_lo, int nb, bool first) {
+    unsigned long long* dst = s.part_q + (size_t)blk * s.total_bins + bin_lo;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[i] = first ? q[i] : dst[i] + q[i];
   }
 };
@@ -145,12 +145,12 @@ struct LdsHist<FORM_F64> {
     atomicAdd(&h[b], (double)hh);
     atomicAdd(&c[b], 1u);
   }
-  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
+  __device__ void flush(const DevState& s, int blk, int bin_lo, int nb, bool first) {
     size_t o = (size_t)blk * s.total_bins + bin_lo;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      s.part_g[set][o + i] = first ? g[i] : s.part_g[set][o + i] + g[i];
-      s.part_h[set][o + i] = first ? h[i] : s.part_h[set][o + i] + h[i];
-      s.part_c[set][o + i] = first ? c[i] : s.part_c[set][o + i] + c[i];
+      s.part_g[o + i] = first ? g[i] : s.part_g[o + i] + g[i];
+      s.part_h[o + i] = first ? h[i] : s.part_h[o + i] + h[i];
+      s.part_c[o + i] = first ? c[i] : s.part_c[o + i] + c[i];
     }
   }
 };
@@ -194,14 +194,14 @@ struct LdsHist<FORM_F32C> {
     if (e2 != 0.0f) atomicAdd(&eh[b], e2);
     atomicAdd(&c[b], 1u);
   }
-  __device__ void flush(const DevState& s, int set, int blk, int bin_lo, int nb, bool first) {
+  __device__ void flush(const DevState& s, int blk, int bin_lo, int nb, bool first) {
     size_t o = (size_t)blk * s.total_bins + bin_lo;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
       double vg = (double)g[i] + (double)eg[i];
       double vh = (double)h[i] + (double)eh[i];
-      s.part_g[set][o + i] = first ? vg : s.part_g[set][o + i] + vg;
-      s.part_h[set][o + i] = first ? vh : s.part_h[set][o + i] + vh;
-      s.part_c[set][o + i] = first ? c[i] : s.part_c[set][o + i] + c[i];
+      s.part_g[o + i] = first ? vg : s.part_g[o + i] + vg;
+      s.part_h[o + i] = first ? vh : s.part_h[o + i] + vh;
+      s.part_c[o + i] = first ? c[i] : s.part_c[o + i] + c[i];
     }
   }
 };
@@ -238,22 +238,28 @@ struct RowPayload {
 };
 
 // ---------------------------------------------------------------------------
-// Histogram of one segment (root build, or a leaf whose sibling cannot be
-// subtracted). Grid: (row blocks, group tiles).
+// Histogram of whole segments (the root, or a leaf whose sibling cannot be
+// subtracted). Grid: (tasks, group tiles); a task is one row block of one item,
+// and writes partial histogram `task_base + blockIdx.x`.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, Segment seg, BlockPlan bp,
-                                                         const int* __restrict__ group_used, int set,
-                                                         const int* __restrict__ tile_first, int rows_per_flush) {
+__global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, const HistItem* __restrict__ items,
+                                                         const Task* __restrict__ tasks,
+                                                         const int* __restrict__ group_used,
+                                                         const int* __restrict__ tile_first, int rows_per_flush,
+                                                         int task_base) {
   extern __shared__ unsigned char smem[];
-  const int blk = blockIdx.x, tile = blockIdx.y;
+  const Task task = tasks[blockIdx.x];
+  const HistItem it = items[task.item];
+  const Segment seg = it.seg;
+  const int tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
   LdsHist<FORM> L;
   L.bind(smem, nb);
   const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
-  const long long ce = min(cb + (long long)bp.chunk, seg_end);
+  const long long cb = (long long)seg.begin + (long long)task.blk * it.chunk;
+  const long long ce = min(cb + (long long)it.chunk, seg_end);
   const uint8_t* cols = s.cols[seg.buf];
   const void* gh = s.gh[seg.buf];
   bool first = true;
@@ -279,7 +285,7 @@ __global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, Segment seg
       }
     }
     __syncthreads();
-    L.flush(s, set, blk, bin_lo, nb, first);
+    L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
     __syncthreads();
     first = false;
     if (fe >= ce) break;
@@ -287,16 +293,19 @@ __global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, Segment seg
 }
 
 // ---------------------------------------------------------------------------
-// Stable partition of one segment, pass 1: left rows per row block.
+// Stable partition, pass 1: left rows of every task (row block of an item).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, Segment seg, BlockPlan bp, int split_group,
-                                                               RuleDev rule) {
+__global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, const PartItem* __restrict__ items,
+                                                               const Task* __restrict__ tasks) {
   __shared__ int red[kStreamThreads / 64];
-  const int blk = blockIdx.x;
+  const Task task = tasks[blockIdx.x];
+  const PartItem& it = items[task.item];
+  const Segment seg = it.seg;
+  const RuleDev rule = it.rule;
   const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
-  const long long ce = min(cb + (long long)bp.chunk, seg_end);
-  const uint32_t* col = (const uint32_t*)(s.cols[seg.buf] + (size_t)split_group * s.npad);
+  const long long cb = (long long)seg.begin + (long long)task.blk * it.chunk;
+  const long long ce = min(cb + (long long)it.chunk, seg_end);
+  const uint32_t* col = (const uint32_t*)(s.cols[seg.buf] + (size_t)it.split_group * s.npad);
   int cnt = 0;
   for (long long d = (cb >> 2) + threadIdx.x; d < ((ce + 3) >> 2); d += blockDim.x) {
     const uint32_t w = col[d];
@@ -311,7 +320,7 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, Segme
   if (threadIdx.x == 0) {
     int t = 0;
     for (int w = 0; w < kStreamThreads / 64; ++w) t += red[w];
-    s.blk_cnt[blk] = t;
+    s.blk_cnt[blockIdx.x] = t;
   }
 }
 
@@ -324,16 +333,22 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, Segme
 // in row order), so the writes are two contiguous runs per column.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Segment seg, int dst_buf, int split_group,
-                                                                 RuleDev rule, int left_total, BlockPlan bp,
-                                                                 int hist_side, const int* __restrict__ group_used,
-                                                                 int set, const int* __restrict__ tile_first,
+__global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, const PartItem* __restrict__ items,
+                                                                 const Task* __restrict__ tasks,
+                                                                 const int* __restrict__ group_used,
+                                                                 const int* __restrict__ tile_first,
                                                                  int rows_per_flush) {
   extern __shared__ unsigned char smem[];
   __shared__ int scratch[32];
   __shared__ int red[kStreamThreads / 64];
   constexpr int kPay = FORM == FORM_QUANT ? 2 : 8;
-  const int blk = blockIdx.x, tile = blockIdx.y;
+  const Task task = tasks[blockIdx.x];
+  const PartItem& it = items[task.item];
+  const Segment seg = it.seg;
+  const RuleDev rule = it.rule;
+  const int dst_buf = it.dst_buf, split_group = it.split_group, left_total = it.left_total;
+  const int hist_side = it.hist_side;
+  const int blk = task.blk, tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
   const int ng = g1 - g0;
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
@@ -345,9 +360,9 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   uint8_t* stage = smem + hist_bytes;
   int32_t* stage_idx = (int32_t*)(stage + (size_t)ng * kTileRows);
   unsigned char* stage_pay = (unsigned char*)(stage_idx + kTileRows);
-  // left rows of the row blocks before this one
+  // left rows of the row blocks of this item before this one
   int lb = 0;
-  for (int i = threadIdx.x; i < blk; i += blockDim.x) lb += s.blk_cnt[i];
+  for (int i = threadIdx.x; i < blk; i += blockDim.x) lb += s.blk_cnt[it.first_task + i];
   for (int o = 32; o > 0; o >>= 1) lb += __shfl_down(lb, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lb;
   __syncthreads();
@@ -356,8 +371,8 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   __syncthreads();
 
   const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)blk * bp.chunk;
-  const long long ce = min(cb + (long long)bp.chunk, seg_end);
+  const long long cb = (long long)seg.begin + (long long)blk * it.chunk;
+  const long long ce = min(cb + (long long)it.chunk, seg_end);
   long long lpos = (long long)seg.begin + left_before;
   long long rpos = (long long)seg.begin + left_total + ((cb - seg.begin) - left_before);
   const uint8_t* scols = s.cols[seg.buf];
@@ -466,7 +481,7 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
     since_flush += (long long)blockDim.x * 4;
     if (do_hist && since_flush >= rows_per_flush) {
       __syncthreads();
-      L.flush(s, set, blk, bin_lo, nb, first);
+      L.flush(s, blockIdx.x, bin_lo, nb, first);
       __syncthreads();
       L.zero(nb);
       first = false;
@@ -476,9 +491,9 @@ __global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, Seg
   }
   if (do_hist) {
     __syncthreads();
-    L.flush(s, set, blk, bin_lo, nb, first);
+    L.flush(s, blockIdx.x, bin_lo, nb, first);
   }
-  if (threadIdx.x == 0 && tile == 0 && blk == bp.nblk - 1 && left_before + my_left_total != left_total)
+  if (threadIdx.x == 0 && tile == 0 && blk == it.ntasks - 1 && left_before + my_left_total != left_total)
     atomicOr(s.err_flag, 1);
   (void)kPay;
 }
@@ -1126,12 +1141,12 @@ struct SliceLds {
 template <int FORM>
 __device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, const LeafArgs& a, const ScanParams& p,
                                          SliceLds& S) {
-  const int set = a.part_set;
   const size_t stride = (size_t)s.total_bins;
+  const size_t base = (size_t)a.part_first * stride;
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
-      const unsigned long long* src = s.part_q[set] + bin;
+      const unsigned long long* src = s.part_q + base + bin;
       long long g[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
       int j = 0;
       for (; j + 4 <= a.nblk; j += 4) {
@@ -1155,9 +1170,9 @@ __device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, co
       S.ih[i] = (h[0] + h[1]) + (h[2] + h[3]);
       S.c[i] = (int)((c[0] + c[1]) + (c[2] + c[3]));
     } else {
-      const double* sg = s.part_g[set] + bin;
-      const double* sh = s.part_h[set] + bin;
-      const unsigned* sc = s.part_c[set] + bin;
+      const double* sg = s.part_g + base + bin;
+      const double* sh = s.part_h + base + bin;
+      const unsigned* sc = s.part_c + base + bin;
       double g = 0.0, h = 0.0;
       unsigned c = 0;
       for (int j = 0; j < a.nblk; ++j) {
@@ -1379,13 +1394,17 @@ __device__ void search_numerical_wave(const FeatDev& m, const ScanParams& p, con
 }
 
 template <int FORM>
-__global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, LeafArgs small, LeafArgs large, int has_large,
-                                              int use_subtract, ScanParams p, const int* __restrict__ used,
-                                              SplitOut* out) {
+__global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanItem* __restrict__ items, ScanParams p,
+                                                       const int* __restrict__ used_all, SplitOut* out_all) {
   __shared__ SliceLds S[2];
   __shared__ ScanPrefix X[2];
   __shared__ int elig[2][kMaxBins];
   const int f = blockIdx.x;
+  const ScanItem& item = items[blockIdx.y];
+  const int* used = used_all + item.used_off;
+  SplitOut* out = out_all + item.out_off;
+  const LeafArgs small = item.small, large = item.large;
+  const int has_large = item.has_large, use_subtract = item.use_subtract;
   if (!used[f]) return;
   const FeatDev m = s.feats[f];
   // smaller leaf: reduce the row-block partials, fix, store
@@ -1652,96 +1671,83 @@ void allow_lds(K kernel) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-void launch_hist(const DevState& s, Segment seg, const BlockPlan& bp, const int* d_group_used, int part_set,
-                 int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
-  if (bp.nblk <= 0) return;
-  dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
+void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
+                 const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
+                 void* stream) {
+  if (ntasks <= 0) return;
+  dim3 grid((unsigned)ntasks, (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
       hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kStreamThreads), lds_bytes<FORM_QUANT>(max_tile_bins), st, s,
-                         seg, bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
-      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kStreamThreads), lds_bytes<FORM_F64>(max_tile_bins), st, s, seg,
-                         bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kStreamThreads), lds_bytes<FORM_F64>(max_tile_bins), st, s,
+                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
       hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kStreamThreads), lds_bytes<FORM_F32C>(max_tile_bins), st, s,
-                         seg, bp, d_group_used, part_set, d_tile_first, rows_per_flush);
+                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
       break;
   }
   BK_HIP(hipGetLastError());
 }
 
-void launch_partition_count(const DevState& s, Segment seg, int split_group, const RuleDev& rule,
-                            const BlockPlan& bp, void* stream) {
-  if (bp.nblk <= 0) return;
-  hipLaunchKernelGGL(k_part_count, dim3((unsigned)bp.nblk), dim3(kStreamThreads), 0, (hipStream_t)stream, s, seg, bp,
-                     split_group, rule);
+void launch_partition_count(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                            void* stream) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_part_count, dim3((unsigned)ntasks), dim3(kStreamThreads), 0, (hipStream_t)stream, s, d_items,
+                     d_tasks);
   BK_HIP(hipGetLastError());
 }
 
-void launch_partition_scatter(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
-                              int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used,
-                              int part_set, int max_tile_bins, int max_tile_groups, const int* d_tile_first,
-                              int ntiles, int rows_per_flush, void* stream) {
-  if (bp.nblk <= 0) return;
+void launch_partition_scatter(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                              bool any_hist, const int* d_group_used, int max_tile_bins, int max_tile_groups,
+                              const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+  if (ntasks <= 0) return;
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid((unsigned)bp.nblk, (unsigned)ntiles);
+  dim3 grid((unsigned)ntasks, (unsigned)ntiles);
   const size_t per_bin = s.form == FORM_QUANT ? 8 : 20;
-  size_t lds = hist_side >= 0 ? (((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15) : 0;
+  size_t lds = any_hist ? (((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15) : 0;
   lds += (size_t)max_tile_groups * kTileRows + (size_t)kTileRows * (4 + (s.form == FORM_QUANT ? 2 : 8));
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_part_scatter<FORM_QUANT>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_QUANT>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf,
-                         split_group, rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first,
-                         rows_per_flush);
+      hipLaunchKernelGGL(k_part_scatter<FORM_QUANT>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
+                         d_group_used, d_tile_first, rows_per_flush);
       break;
     case FORM_F64:
       allow_lds(k_part_scatter<FORM_F64>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_F64>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf, split_group,
-                         rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first, rows_per_flush);
+      hipLaunchKernelGGL(k_part_scatter<FORM_F64>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
+                         d_group_used, d_tile_first, rows_per_flush);
       break;
     default:
       allow_lds(k_part_scatter<FORM_F32C>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_F32C>, grid, dim3(kStreamThreads), lds, st, s, seg, dst_buf, split_group,
-                         rule, left_total, bp, hist_side, d_group_used, part_set, d_tile_first, rows_per_flush);
+      hipLaunchKernelGGL(k_part_scatter<FORM_F32C>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
+                         d_group_used, d_tile_first, rows_per_flush);
       break;
   }
   BK_HIP(hipGetLastError());
 }
 
-void launch_partition(const DevState& s, Segment seg, int dst_buf, int split_group, const RuleDev& rule,
-                      int left_total, const BlockPlan& bp, int hist_side, const int* d_group_used, int part_set,
-                      int max_tile_bins, int max_tile_groups, const int* d_tile_first, int ntiles, int rows_per_flush,
-                      void* stream) {
-  launch_partition_count(s, seg, split_group, rule, bp, stream);
-  launch_partition_scatter(s, seg, dst_buf, split_group, rule, left_total, bp, hist_side, d_group_used, part_set,
-                           max_tile_bins, max_tile_groups, d_tile_first, ntiles, rows_per_flush, stream);
-}
-
-void launch_scan(const DevState& s, const LeafArgs& small, const LeafArgs* large, int use_subtract,
-                 const ScanParams& sp, const int* d_used, SplitOut* d_out, void* stream) {
+void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
+                 SplitOut* d_out, void* stream) {
+  if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
-  LeafArgs lg = large ? *large : LeafArgs{};
-  int has_large = large ? 1 : 0;
+  dim3 grid((unsigned)s.num_features, (unsigned)nitems);
   switch (s.form) {
     case FORM_QUANT:
-      hipLaunchKernelGGL(k_scan<FORM_QUANT>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg,
-                         has_large, use_subtract, sp, d_used, d_out);
+      hipLaunchKernelGGL(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
     case FORM_F64:
-      hipLaunchKernelGGL(k_scan<FORM_F64>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg, has_large,
-                         use_subtract, sp, d_used, d_out);
+      hipLaunchKernelGGL(k_scan<FORM_F64>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
     default:
-      hipLaunchKernelGGL(k_scan<FORM_F32C>, dim3((unsigned)s.num_features), dim3(kScanThreads), 0, st, s, small, lg,
-                         has_large, use_subtract, sp, d_used, d_out);
+      hipLaunchKernelGGL(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
   }
   BK_HIP(hipGetLastError());

@@ -331,25 +331,37 @@ struct Prof {
 
 }  // namespace
 
+// A leaf of the growing tree in the batched form: its rows, sums, best split,
+// and (after a speculative expansion) its two evaluated children.
+struct Node {
+  int depth = 0;
+  Segment seg{0, 0, 0};
+  LeafSplits ls;
+  HostSplit best;
+  int slot = -1;  // pool slot of its histogram
+  int child[2] = {-1, -1};
+  bool expanded = false;
+  std::vector<char> splittable;
+};
+
 struct Learner {
   bk_learner_params P{};
   int N = 0, npad = 0, G = 0, F = 0, TB = 0;
   int form = FORM_F64;
   bool quant = false, const_hess = false, stochastic = false, renew = false;
+  bool batched = false;
   int bins = 4;
   std::vector<FeatDev> feats;
   std::vector<int> group_bin_start;
   std::vector<std::vector<double>> upper;
   std::vector<std::vector<int32_t>> bin2cat;
-  // group tiles for LDS
-  std::vector<int> tile_first;
-  int max_tile_bins = 0;
-  int max_tile_groups = 0;
-  std::vector<int> htile_first;
-  int max_htile_bins = 0;
-  int* d_htile_first = nullptr;
+  // group tiles for LDS: small ones for the partition, large ones for segment histograms
+  std::vector<int> tile_first, htile_first;
+  int max_tile_bins = 0, max_tile_groups = 0, max_htile_bins = 0;
   int rows_per_flush = 0;
-  size_t cache = 0;
+  size_t cache = 0;       // histogram slots of the reference pool
+  size_t num_slots = 0;   // device slots (batched form: one per evaluated leaf)
+  int max_items = 0, max_tasks = 0;
 
   // device memory
   hipStream_t stream = nullptr;
@@ -363,27 +375,40 @@ struct Learner {
   int* d_group_bin_start = nullptr;
   double* d_pool = nullptr;
   int* d_pool_cnt = nullptr;
-  unsigned long long* d_part_q[2] = {nullptr, nullptr};
-  double* d_part_g[2] = {nullptr, nullptr};
-  double* d_part_h[2] = {nullptr, nullptr};
-  unsigned* d_part_c[2] = {nullptr, nullptr};
+  unsigned long long* d_part_q = nullptr;
+  double* d_part_g = nullptr;
+  double* d_part_h = nullptr;
+  unsigned* d_part_c = nullptr;
   int* d_blk_cnt = nullptr;
   int* d_err = nullptr;
   int* d_tile_first = nullptr;
-  int* d_used = nullptr;        // per feature
+  int* d_htile_first = nullptr;
+  int* d_used = nullptr;        // per item x feature
   int* d_group_used = nullptr;  // per group
   SplitOut* d_out = nullptr;
+  HistItem* d_hitems = nullptr;
+  PartItem* d_pitems = nullptr;
+  ScanItem* d_sitems = nullptr;
+  Task* d_tasks = nullptr;      // partition tasks, then histogram tasks
   unsigned* d_max4 = nullptr;
   long long* d_sums3 = nullptr;
   double* d_sum_partials = nullptr;
   Segment* d_segs = nullptr;
   double* d_renew = nullptr;
   int* d_order = nullptr;
-  std::vector<int> renew_order;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
   // pinned host staging
   SplitOut* h_out = nullptr;
+  HistItem* h_hitems = nullptr;
+  PartItem* h_pitems = nullptr;
+  ScanItem* h_sitems = nullptr;
+  Task* h_tasks = nullptr;
+  int* h_used = nullptr;
+  int* h_group_used = nullptr;
+  Segment* h_segs = nullptr;
+  int* h_order = nullptr;
+  double* h_vals = nullptr;
 
   // per-tree state (serial.rs:183-245)
   PoolMap pool;
@@ -391,28 +416,27 @@ struct Learner {
   LeafSplits smaller, larger;
   size_t smaller_slot = 0;
   long larger_slot = -1, parent_slot = -1;
-  std::vector<Segment> segs;
+  std::vector<Segment> segs;  // segment of every leaf of the last tree
   Mt use_start_eng{0};
   double grad_scale = 0.0, hess_scale = 0.0;
   long long last_int_g = 0, last_int_h = 0;
   double last_root_g = 0.0, last_root_h = 0.0;
   bk_train_stats stats{};
-  std::vector<int> used_feat, used_group;
+  std::vector<int> used_feat;
   int last_num_leaves = 1;
-  bool pending = false;
-  bool pending_has_large = false, pending_subtract = false;
-  LeafArgs pend_small{}, pend_large{};
   double h0 = 1.0;
   Prof prof;
+  // The staging arrays are copied asynchronously: the host rewrites them only
+  // after the last copy out of them has completed.
+  hipEvent_t staged_ev = nullptr;
+  bool staged_pending = false;
+  // batched form
+  std::vector<Node> nodes;
+  std::vector<int> leaf_node;
+
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
-  int used_group_count() const {
-    int c = 0;
-    for (int u : used_group) c += u;
-    return c;
-  }
-  // wraps a launch in events when profiling
-  template <typename F>
-  void timed(int kind, double algo, double moved, F&& launch) {
+  template <typename Fn>
+  void timed(int kind, double algo, double moved, Fn&& launch) {
     if (!prof.on) {
       launch();
       return;
@@ -428,57 +452,43 @@ struct Learner {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
+    auto fh = [](void* p) {
+      if (p) (void)hipHostFree(p);
+    };
     for (int b = 0; b < 3; ++b) {
       f(d_cols[b]);
       f(d_idx[b]);
       f(d_gh[b]);
     }
-    f(d_grad);
-    f(d_hess);
-    f(d_gnoise);
-    f(d_hnoise);
-    f(d_feats);
-    f(d_group_bin_start);
-    f(d_pool);
-    f(d_pool_cnt);
-    for (int k = 0; k < 2; ++k) {
-      f(d_part_q[k]);
-      f(d_part_g[k]);
-      f(d_part_h[k]);
-      f(d_part_c[k]);
-    }
-    f(d_blk_cnt);
-    f(d_err);
-    f(d_tile_first);
-    f(d_htile_first);
-    f(d_used);
-    f(d_group_used);
-    f(d_out);
-    f(d_max4);
-    f(d_sums3);
-    f(d_sum_partials);
-    f(d_segs);
-    f(d_renew);
-    f(d_order);
-    f(d_leaf_vals);
-    f(d_leaf_of);
-    if (h_out) (void)hipHostFree(h_out);
+    for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
+                    (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
+                    (void*)d_part_h, (void*)d_part_c, (void*)d_blk_cnt, (void*)d_err, (void*)d_tile_first,
+                    (void*)d_htile_first, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_hitems,
+                    (void*)d_pitems, (void*)d_sitems, (void*)d_tasks, (void*)d_max4, (void*)d_sums3,
+                    (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
+                    (void*)d_leaf_of})
+      f(p);
+    for (void* p : {(void*)h_out, (void*)h_hitems, (void*)h_pitems, (void*)h_sitems, (void*)h_tasks, (void*)h_used,
+                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals})
+      fh(p);
+    if (staged_ev) (void)hipEventDestroy(staged_ev);
+    staged_ev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
-    std::memset(d_cols, 0, sizeof d_cols);
     stream = nullptr;
-    h_out = nullptr;
   }
 
-  BlockPlan plan(int count) const {
-    BlockPlan bp{0, 0};
-    if (count <= 0) return bp;
+  // Row blocks of one item: few large blocks for a lone item, fixed-size blocks in a batch.
+  int chunk_for(int count, bool alone) const {
     const int min_chunk = 4 * kTileRows;
-    long long chunk = ((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks;
-    chunk = std::max<long long>(chunk, min_chunk);
+    long long chunk = min_chunk;
+    if (alone) chunk = std::max<long long>(((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks, min_chunk);
     chunk = (chunk + kTileRows - 1) / kTileRows * kTileRows;
-    bp.chunk = (int)chunk;
-    bp.nblk = (int)(((long long)count + chunk - 1) / chunk);
-    return bp;
+    return (int)chunk;
+  }
+
+  template <typename T>
+  static void pinned(T** p, size_t n) {
+    BK_HIPC(hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault));
   }
 
   void create(const bk_dataset_view* v, const bk_learner_params* p, int is_const_hess) {
@@ -552,40 +562,30 @@ struct Learner {
     // LDS tiles of whole groups
     const int per_bin = form == FORM_QUANT ? 8 : 20;
     const int tile_cap = kLdsBytes / per_bin;
-    tile_first.assign(1, 0);
-    max_tile_bins = 0;
-    int cur = 0;
-    int cur_groups = 0;
-    for (int g = 0; g < G; ++g) {
-      int nb = v->group_num_total_bin[g];
-      if (cur + nb > tile_cap || cur_groups == kTileGroups) {
-        tile_first.push_back(g);
-        max_tile_bins = std::max(max_tile_bins, cur);
-        cur = 0;
-        cur_groups = 0;
+    auto make_tiles = [&](int max_groups, std::vector<int>& first, int& max_bins, int& max_groups_out) {
+      first.assign(1, 0);
+      max_bins = 0;
+      max_groups_out = 0;
+      int cur = 0, cur_groups = 0;
+      for (int g = 0; g < G; ++g) {
+        int nb = v->group_num_total_bin[g];
+        if (cur + nb > tile_cap || cur_groups == max_groups) {
+          first.push_back(g);
+          max_bins = std::max(max_bins, cur);
+          max_groups_out = std::max(max_groups_out, cur_groups);
+          cur = 0;
+          cur_groups = 0;
+        }
+        cur += nb;
+        cur_groups += 1;
       }
-      cur += nb;
-      cur_groups += 1;
-    }
-    tile_first.push_back(G);
-    max_tile_bins = std::max(max_tile_bins, cur);
-    for (size_t t = 0; t + 1 < tile_first.size(); ++t)
-      max_tile_groups = std::max(max_tile_groups, tile_first[t + 1] - tile_first[t]);
-    // segment histograms stream every column once: tiles as large as LDS allows
-    htile_first.assign(1, 0);
-    max_htile_bins = 0;
-    cur = 0;
-    for (int g = 0; g < G; ++g) {
-      int nb = v->group_num_total_bin[g];
-      if (cur + nb > tile_cap) {
-        htile_first.push_back(g);
-        max_htile_bins = std::max(max_htile_bins, cur);
-        cur = 0;
-      }
-      cur += nb;
-    }
-    htile_first.push_back(G);
-    max_htile_bins = std::max(max_htile_bins, cur);
+      first.push_back(G);
+      max_bins = std::max(max_bins, cur);
+      max_groups_out = std::max(max_groups_out, cur_groups);
+    };
+    int unused_groups = 0;
+    make_tiles(kTileGroups, tile_first, max_tile_bins, max_tile_groups);
+    make_tiles(1 << 30, htile_first, max_htile_bins, unused_groups);
     // rows one LDS accumulation may hold: the packed quantized fields stay exact
     if (form == FORM_QUANT) {
       int maxq = bins + 1;
@@ -600,8 +600,14 @@ struct Learner {
       size_t c = (size_t)(P.histogram_pool_size * 1024.0 * 1024.0 / (double)bytes);
       cache = std::min(std::max(c, (size_t)2), (size_t)P.num_leaves);
     }
+    // the batched form keeps every evaluated leaf's histogram, so it needs an unbounded pool
+    batched = P.batched != 0 && cache == (size_t)P.num_leaves;
+    num_slots = batched ? (size_t)P.num_leaves * 2 + 2 : cache;
+    max_items = P.num_leaves + 1;
+    max_tasks = (N + 4 * kTileRows - 1) / (4 * kTileRows) + 2 * max_items + kMaxRowBlocks + 8;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming));
     for (int b = 0; b < 3; ++b) {
       d_cols[b] = dmalloc<uint8_t>((size_t)G * npad);
       if (b > 0) d_idx[b] = dmalloc<int32_t>((size_t)npad);
@@ -619,18 +625,16 @@ struct Learner {
     d_group_bin_start = dmalloc<int>((size_t)G + 1);
     BK_HIPC(hipMemcpyAsync(d_group_bin_start, group_bin_start.data(), sizeof(int) * (size_t)(G + 1),
                            hipMemcpyHostToDevice, stream));
-    d_pool = dmalloc<double>(cache * (size_t)TB * 2);
-    d_pool_cnt = dmalloc<int>(cache * (size_t)TB);
-    for (int k = 0; k < 2; ++k) {
-      if (form == FORM_QUANT) {
-        d_part_q[k] = dmalloc<unsigned long long>((size_t)kMaxRowBlocks * TB);
-      } else {
-        d_part_g[k] = dmalloc<double>((size_t)kMaxRowBlocks * TB);
-        d_part_h[k] = dmalloc<double>((size_t)kMaxRowBlocks * TB);
-      }
-      if (form != FORM_QUANT) d_part_c[k] = dmalloc<unsigned>((size_t)kMaxRowBlocks * TB);
+    d_pool = dmalloc<double>(num_slots * (size_t)TB * 2);
+    d_pool_cnt = dmalloc<int>(num_slots * (size_t)TB);
+    if (form == FORM_QUANT) {
+      d_part_q = dmalloc<unsigned long long>((size_t)max_tasks * TB);
+    } else {
+      d_part_g = dmalloc<double>((size_t)max_tasks * TB);
+      d_part_h = dmalloc<double>((size_t)max_tasks * TB);
+      d_part_c = dmalloc<unsigned>((size_t)max_tasks * TB);
     }
-    d_blk_cnt = dmalloc<int>((size_t)kMaxRowBlocks);
+    d_blk_cnt = dmalloc<int>((size_t)max_tasks);
     d_err = dmalloc<int>(1);
     BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
     d_tile_first = dmalloc<int>(tile_first.size());
@@ -639,9 +643,13 @@ struct Learner {
     d_htile_first = dmalloc<int>(htile_first.size());
     BK_HIPC(hipMemcpyAsync(d_htile_first, htile_first.data(), sizeof(int) * htile_first.size(), hipMemcpyHostToDevice,
                            stream));
-    d_used = dmalloc<int>((size_t)F);
+    d_used = dmalloc<int>((size_t)F * max_items);
     d_group_used = dmalloc<int>((size_t)G);
-    d_out = dmalloc<SplitOut>((size_t)F * 2);
+    d_out = dmalloc<SplitOut>((size_t)F * 2 * max_items);
+    d_hitems = dmalloc<HistItem>((size_t)max_items);
+    d_pitems = dmalloc<PartItem>((size_t)max_items);
+    d_sitems = dmalloc<ScanItem>((size_t)max_items);
+    d_tasks = dmalloc<Task>((size_t)max_tasks);
     d_max4 = dmalloc<unsigned>(4);
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
@@ -649,7 +657,16 @@ struct Learner {
     d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
     d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
-    BK_HIPC(hipHostMalloc((void**)&h_out, sizeof(SplitOut) * (size_t)F * 2, hipHostMallocDefault));
+    pinned(&h_out, (size_t)F * 2 * max_items);
+    pinned(&h_hitems, (size_t)max_items);
+    pinned(&h_pitems, (size_t)max_items);
+    pinned(&h_sitems, (size_t)max_items);
+    pinned(&h_tasks, (size_t)max_tasks);
+    pinned(&h_used, (size_t)F * max_items);
+    pinned(&h_group_used, (size_t)G);
+    pinned(&h_segs, (size_t)P.num_leaves);
+    pinned(&h_order, (size_t)P.num_leaves);
+    pinned(&h_vals, (size_t)P.num_leaves);
     if (stochastic) {
       // init_stochastic (gradient_discretizer/mod.rs:87-106): per-block MT19937 noise
       std::vector<double> gn((size_t)N), hn((size_t)N);
@@ -692,19 +709,19 @@ struct Learner {
     ds.group_bin_start = d_group_bin_start;
     ds.pool = d_pool;
     ds.pool_cnt = d_pool_cnt;
-    for (int k = 0; k < 2; ++k) {
-      ds.part_q[k] = d_part_q[k];
-      ds.part_g[k] = d_part_g[k];
-      ds.part_h[k] = d_part_h[k];
-      ds.part_c[k] = d_part_c[k];
-    }
+    ds.part_q = d_part_q;
+    ds.part_g = d_part_g;
+    ds.part_h = d_part_h;
+    ds.part_c = d_part_c;
     ds.blk_cnt = d_blk_cnt;
     ds.err_flag = d_err;
     pool.init((size_t)P.num_leaves, cache, (size_t)F);
     best.assign((size_t)P.num_leaves, HostSplit());
     segs.assign((size_t)P.num_leaves, Segment{0, 0, 0});
     used_feat.assign((size_t)F, 1);
-    used_group.assign((size_t)G, 1);
+    for (int g = 0; g < G; ++g) h_group_used[g] = 1;
+    BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
   }
 
   ScanParams scan_params() const {
@@ -729,7 +746,7 @@ struct Learner {
   }
 
   // Discretization and root sums (serial.rs:658-776, gradient_discretizer/mod.rs:134-201).
-  void before_train(const float* g, const float* h) {
+  LeafSplits before_train(const float* g, const float* h) {
     pool.reset();
     for (auto& s : best) s.reset();
     std::fill(segs.begin(), segs.end(), Segment{0, 0, 0});
@@ -808,10 +825,162 @@ struct Learner {
       h0 = (double)hh0;
     }
     double carried = smaller.weight;
-    smaller = LeafSplits{0, N, sg, sh, carried, isum};
+    LeafSplits root{0, N, sg, sh, carried, isum};
+    smaller = root;
     larger = LeafSplits{};
+    return root;
   }
 
+  double parent_output(const Tree& t, const LeafSplits& ls) const {
+    if (t.num_leaves == 1) return leaf_output_root(P, ls.sum_g, ls.sum_h);
+    return ls.weight;
+  }
+
+  LeafArgs leaf_args(const Tree& t, const LeafSplits& ls, int slot) const {
+    return leaf_args_with(ls, slot, parent_output(t, ls));
+  }
+
+  static LeafArgs leaf_args_with(const LeafSplits& ls, int slot, double parent_out) {
+    LeafArgs a{};
+    a.leaf = ls.leaf;
+    a.slot = slot;
+    a.num_data = ls.num_data;
+    a.sum_g = ls.sum_g;
+    a.sum_h = ls.sum_h;
+    a.parent_output = parent_out;
+    a.int_sum = ls.int_sum;
+    return a;
+  }
+
+  // ---- launch helpers over staged item arrays -------------------------------
+  void staged() {
+    BK_HIPC(hipEventRecord(staged_ev, stream));
+    staged_pending = true;
+  }
+
+  void stage_wait() {
+    if (!staged_pending) return;
+    BK_HIPC(hipEventSynchronize(staged_ev));
+    staged_pending = false;
+  }
+
+  int add_tasks(int item, int count, int chunk, int& ntasks) {
+    const int first = ntasks;
+    const int n = count > 0 ? (count + chunk - 1) / chunk : 0;
+    if (ntasks + n > max_tasks) throw std::runtime_error("task buffer too small");
+    for (int b = 0; b < n; ++b) h_tasks[ntasks++] = Task{item, b};
+    return first;
+  }
+
+  void run_hist(int nitems, int first_task, int ntasks, double rows) {
+    if (ntasks <= first_task) return;
+    BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_tasks + first_task, h_tasks + first_task, sizeof(Task) * (size_t)(ntasks - first_task),
+                           hipMemcpyHostToDevice, stream));
+    const double hb = rows * (G + payload_bytes());
+    timed(0, hb, hb, [&] {
+      launch_hist(ds, d_hitems, d_tasks + first_task, ntasks - first_task, first_task, d_group_used, max_htile_bins,
+                  d_htile_first, (int)htile_first.size() - 1, rows_per_flush, stream);
+    });
+    staged();
+    stats.hist_rows += (int64_t)rows;
+  }
+
+  void run_partition(int nitems, int ntasks, double part_rows, double hist_rows, bool any_hist) {
+    if (ntasks <= 0) return;
+    BK_HIPC(hipMemcpyAsync(d_pitems, h_pitems, sizeof(PartItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_tasks, h_tasks, sizeof(Task) * (size_t)ntasks, hipMemcpyHostToDevice, stream));
+    const double rec = (double)(G + payload_bytes() + 4);
+    timed(1, part_rows, part_rows, [&] { launch_partition_count(ds, d_pitems, d_tasks, ntasks, stream); });
+    const double algo = part_rows * 9.0 + hist_rows * (G + payload_bytes());
+    const double moved = part_rows * (2.0 * rec + 1.0);
+    timed(2, algo, moved, [&] {
+      launch_partition_scatter(ds, d_pitems, d_tasks, ntasks, any_hist, d_group_used, max_tile_bins, max_tile_groups,
+                               d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+    });
+    staged();
+    stats.part_rows += (int64_t)part_rows;
+    stats.hist_rows += (int64_t)hist_rows;
+  }
+
+  // launches the search of the staged items and copies the candidates back
+  void run_scan(int nitems) {
+    if (nitems <= 0) return;
+    ScanParams sp = scan_params();
+    BK_HIPC(hipMemcpyAsync(d_sitems, h_sitems, sizeof(ScanItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_used, h_used, sizeof(int) * (size_t)F * nitems, hipMemcpyHostToDevice, stream));
+    timed(3, 0.0, 0.0, [&] { launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream); });
+    BK_HIPC(hipMemcpyAsync(h_out, d_out, sizeof(SplitOut) * (size_t)F * 2 * nitems, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipStreamSynchronize(stream));
+    staged_pending = false;
+    stats.rounds += 1;
+  }
+
+  // folds the candidates of one leaf in feature order (serial.rs:1500-1566)
+  HostSplit fold(int out_off, const int* used, std::vector<char>& splittable) const {
+    HostSplit b;
+    for (int f = 0; f < F; ++f) {
+      if (!used[f]) continue;
+      HostSplit c;
+      c.o = h_out[out_off + f];
+      c.inner = f;
+      splittable[(size_t)f] = c.o.splittable ? 1 : 0;
+      if (c.better_than(b)) b = c;
+    }
+    return b;
+  }
+
+  // The rule and the tree update of one committed split (split, serial.rs:1685-1876).
+  // Returns the right leaf index.
+  int commit_split(Tree& t, int best_leaf, const HostSplit& info) {
+    const int inner = info.inner;
+    const FeatDev& m = feats[(size_t)inner];
+    const float gain = (float)(info.o.gain + P.min_gain_to_split);
+    const int lc = info.o.exact_left, rc = info.o.exact_right;
+    if (m.bin_type == BK_BIN_CATEGORICAL) {
+      std::vector<uint32_t> real;
+      for (uint32_t b = 0; b < 256; ++b)
+        if ((info.o.cat_bits[b / 32] >> (b % 32)) & 1u) real.push_back((uint32_t)bin2cat[(size_t)inner][b]);
+      std::vector<uint32_t> bits_real = bitset_of(real);
+      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
+                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
+      t.dtype[(size_t)node] = (int8_t)(1 | (m.missing_type << 2));
+      t.th_bin[(size_t)node] = (uint32_t)t.num_cat;
+      t.th[(size_t)node] = (double)t.num_cat;
+      t.num_cat += 1;
+      t.cat_bound.push_back(t.cat_bound.back() + (int)bits_real.size());
+      t.cat_th.insert(t.cat_th.end(), bits_real.begin(), bits_real.end());
+    } else {
+      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
+                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
+      int8_t dt = 0;
+      if (info.o.default_left) dt |= 2;
+      dt = (int8_t)((dt & 3) | (m.missing_type << 2));
+      t.dtype[(size_t)node] = dt;
+      t.th_bin[(size_t)node] = (uint32_t)info.o.threshold;
+      t.th[(size_t)node] = upper[(size_t)inner][(size_t)info.o.threshold];
+    }
+    t.num_leaves += 1;
+    stats.splits += 1;
+    return t.num_leaves - 1;
+  }
+
+  RuleDev rule_of(const HostSplit& info) const {
+    const FeatDev& m = feats[(size_t)info.inner];
+    if (m.bin_type == BK_BIN_CATEGORICAL) return make_rule_categorical(m, info.o.cat_bits);
+    return make_rule_numerical(m, (uint32_t)info.o.threshold, info.o.default_left != 0);
+  }
+
+  // before_find_best_split's data rules (serial.rs:780-799) for two fresh children
+  bool children_searchable(int child_depth, int nl, int nr) const {
+    if (P.max_depth > 0 && child_depth >= P.max_depth) return false;
+    if (nr < P.min_data_in_leaf * 2 && nl < P.min_data_in_leaf * 2) return false;
+    return true;
+  }
+
+  // ===========================================================================
+  // Sequential form: one split per step, in the reference's order
+  // ===========================================================================
   // before_find_best_split (serial.rs:779-825)
   bool before_find_best_split(const Tree& t, int left_leaf, int right_leaf) {
     if (P.max_depth > 0 && t.leaf_depth[(size_t)left_leaf] >= P.max_depth) {
@@ -845,228 +1014,303 @@ struct Learner {
     return true;
   }
 
-  double parent_output(const Tree& t, const LeafSplits& ls) const {
-    if (t.num_leaves == 1) return leaf_output_root(P, ls.sum_g, ls.sum_h);
-    return ls.weight;
-  }
-
-  // Feature usage from the parent's splittable flags (serial.rs:838-852).
+  // is_feature_used from the parent's splittable flags (serial.rs:838-852)
   void compute_used() {
-    bool changed = false;
-    std::vector<int> uf((size_t)F, 1);
+    stage_wait();
     for (int f = 0; f < F; ++f) {
+      used_feat[(size_t)f] = 1;
       if (parent_slot >= 0 && !pool.splittable[(size_t)parent_slot][(size_t)f]) {
         pool.splittable[smaller_slot][(size_t)f] = 0;
-        uf[(size_t)f] = 0;
+        used_feat[(size_t)f] = 0;
       }
+      h_used[f] = used_feat[(size_t)f];
     }
-    std::vector<int> ug((size_t)G, 0);
-    for (int f = 0; f < F; ++f)
-      if (uf[(size_t)f]) ug[(size_t)feats[(size_t)f].group] = 1;
-    if (uf != used_feat) {
-      used_feat = uf;
-      changed = true;
-    }
-    if (ug != used_group) used_group = ug, changed = true;
-    (void)changed;
-    BK_HIPC(hipMemcpyAsync(d_used, used_feat.data(), sizeof(int) * (size_t)F, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_group_used, used_group.data(), sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
   }
 
-  LeafArgs leaf_args(const Tree& t, const LeafSplits& ls, size_t slot, int part_set, int nblk) const {
-    LeafArgs a{};
-    a.leaf = ls.leaf;
-    a.slot = (int)slot;
-    a.num_data = ls.num_data;
-    a.sum_g = ls.sum_g;
-    a.sum_h = ls.sum_h;
-    a.parent_output = parent_output(t, ls);
-    a.int_sum = ls.int_sum;
-    a.from_partials = 1;
-    a.part_set = part_set;
-    a.nblk = nblk;
-    return a;
-  }
-
-  // Runs the pending search (fix/subtract/scan), reads back the per-feature
-  // candidates and folds them (serial.rs:1500-1566).
-  void run_search() {
-    if (!pending) return;
-    pending = false;
-    ScanParams sp = scan_params();
-    timed(3, 0.0, 0.0, [&] {
-      launch_scan(ds, pend_small, pending_has_large ? &pend_large : nullptr, pending_subtract ? 1 : 0, sp, d_used,
-                  d_out, stream);
-    });
-    BK_HIPC(hipMemcpyAsync(h_out, d_out, sizeof(SplitOut) * (size_t)F * 2, hipMemcpyDeviceToHost, stream));
-    BK_HIPC(hipStreamSynchronize(stream));
-    HostSplit sb, lb;
-    for (int f = 0; f < F; ++f) {
-      if (!used_feat[(size_t)f]) continue;
-      HostSplit c;
-      c.o = h_out[f];
-      c.inner = f;
-      pool.splittable[smaller_slot][(size_t)f] = c.o.splittable ? 1 : 0;
-      if (c.better_than(sb)) sb = c;
-      if (pending_has_large) {
-        HostSplit d;
-        d.o = h_out[F + f];
-        d.inner = f;
-        pool.splittable[(size_t)larger_slot][(size_t)f] = d.o.splittable ? 1 : 0;
-        if (d.better_than(lb)) lb = d;
-      }
-    }
+  void fold_sequential(bool has_large) {
+    HostSplit sb = fold(0, h_used, pool.splittable[smaller_slot]);
     best[(size_t)smaller.leaf] = sb;
-    if (larger.leaf >= 0) best[(size_t)larger.leaf] = lb;
+    if (larger.leaf >= 0)
+      best[(size_t)larger.leaf] = has_large ? fold(F, h_used, pool.splittable[(size_t)larger_slot]) : HostSplit();
   }
 
-  // find_best_splits for the root (serial.rs:830-887): build the root histogram.
-  void prepare_root_search(const Tree& t) {
-    compute_used();
-    BlockPlan bp = plan(N);
-    const double hb = (double)N * (used_group_count() + payload_bytes());
-    timed(0, hb, hb, [&] {
-      launch_hist(ds, segs[0], bp, d_group_used, 0, max_htile_bins, d_htile_first, (int)htile_first.size() - 1,
-                  rows_per_flush, stream);
-    });
-    stats.hist_rows += N;
-    pend_small = leaf_args(t, smaller, smaller_slot, 0, bp.nblk);
-    pending_has_large = false;
-    pending_subtract = false;
-    pending = true;
-  }
-
-  // split (serial.rs:1685-1876) plus the next step's before_find_best_split, with
-  // the smaller child's histogram built by the partition pass.
-  std::pair<int, int> split(Tree& t, int best_leaf) {
-    HostSplit info = best[(size_t)best_leaf];
-    const int inner = info.inner;
-    const FeatDev& m = feats[(size_t)inner];
-    const float gain = (float)(info.o.gain + P.min_gain_to_split);
-    const int lc = info.o.exact_left, rc = info.o.exact_right;
-    RuleDev rule;
-    if (m.bin_type == BK_BIN_CATEGORICAL) {
-      std::vector<uint32_t> bins_left, real;
-      for (uint32_t b = 0; b < 256; ++b)
-        if ((info.o.cat_bits[b / 32] >> (b % 32)) & 1u) bins_left.push_back(b);
-      for (uint32_t b : bins_left) real.push_back((uint32_t)bin2cat[(size_t)inner][b]);
-      std::vector<uint32_t> bits_real = bitset_of(real);
-      rule = make_rule_categorical(m, info.o.cat_bits);
-      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
-                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
-      t.dtype[(size_t)node] = (int8_t)(1 | (m.missing_type << 2));
-      t.th_bin[(size_t)node] = (uint32_t)t.num_cat;
-      t.th[(size_t)node] = (double)t.num_cat;
-      t.num_cat += 1;
-      t.cat_bound.push_back(t.cat_bound.back() + (int)bits_real.size());
-      t.cat_th.insert(t.cat_th.end(), bits_real.begin(), bits_real.end());
-      t.num_leaves += 1;
-    } else {
-      rule = make_rule_numerical(m, (uint32_t)info.o.threshold, info.o.default_left != 0);
-      int node = t.split_common(best_leaf, inner, info.o.feature, info.o.left_output, info.o.right_output, lc, rc,
-                                info.o.left_sum_hessian, info.o.right_sum_hessian, gain);
-      int8_t dt = 0;
-      if (info.o.default_left) dt |= 2;
-      dt = (int8_t)((dt & 3) | (m.missing_type << 2));
-      t.dtype[(size_t)node] = dt;
-      t.th_bin[(size_t)node] = (uint32_t)info.o.threshold;
-      t.th[(size_t)node] = upper[(size_t)inner][(size_t)info.o.threshold];
-      t.num_leaves += 1;
-    }
-    const int right = t.num_leaves - 1;
-    LeafSplits l{best_leaf, lc, info.o.left_sum_gradient, info.o.left_sum_hessian, info.o.left_output,
-                 info.o.left_int_sum};
-    LeafSplits r{right, rc, info.o.right_sum_gradient, info.o.right_sum_hessian, info.o.right_output,
-                 info.o.right_int_sum};
-    if (lc < rc) {
-      smaller = l;
-      larger = r;
-    } else {
-      smaller = r;
-      larger = l;
-    }
-    best[(size_t)best_leaf] = info;
-    // the rows move now; both children's sizes are already known
-    const Segment ps = segs[(size_t)best_leaf];
-    const int dst = ps.buf == 1 ? 2 : 1;
-    segs[(size_t)best_leaf] = Segment{dst, ps.begin, lc};
-    segs[(size_t)right] = Segment{dst, ps.begin + lc, rc};
-    const bool search = before_find_best_split(t, best_leaf, right);
-    BlockPlan bp = plan(ps.count);
-    int hist_side = -1;
-    if (search) {
-      compute_used();
-      hist_side = (lc < rc) ? 0 : 1;
-    }
-    {
-      const double q = (double)ps.count;
-      const double rec = (double)(G + payload_bytes() + 4);
-      const double hist_rows = search ? (double)std::min(lc, rc) : 0.0;
-      const double algo = q * 9.0 + hist_rows * (used_group_count() + payload_bytes());
-      const double moved = q * (2.0 * rec + 1.0);
-      if (prof.on) {
-        // the count pass and the scatter pass are timed separately
-        size_t i0 = prof.begin(stream);
-        launch_partition_count(ds, ps, m.group, rule, bp, stream);
-        prof.end(stream, i0, 1, q * 1.0, q * 1.0);
-        size_t i1 = prof.begin(stream);
-        launch_partition_scatter(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins,
-                                 max_tile_groups, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
-        prof.end(stream, i1, 2, algo, moved);
-      } else {
-        launch_partition(ds, ps, dst, m.group, rule, lc, bp, hist_side, d_group_used, 0, max_tile_bins,
-                         max_tile_groups, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
-      }
-    }
-    stats.part_rows += ps.count;
-    if (search) {
-      stats.hist_rows += smaller.num_data;
-      const bool use_subtract = parent_slot >= 0;
-      pend_small = leaf_args(t, smaller, smaller_slot, 0, bp.nblk);
-      pending_has_large = larger.leaf >= 0 && larger_slot >= 0;
-      pending_subtract = use_subtract;
-      if (pending_has_large) {
-        pend_large = leaf_args(t, larger, (size_t)larger_slot, 1, 0);
-        if (!use_subtract) {
-          const Segment lseg = segs[(size_t)larger.leaf];
-          BlockPlan lbp = plan(lseg.count);
-          launch_hist(ds, lseg, lbp, d_group_used, 1, max_htile_bins, d_htile_first, (int)htile_first.size() - 1,
-                      rows_per_flush, stream);
-          stats.hist_rows += lseg.count;
-          pend_large.nblk = lbp.nblk;
-        }
-      }
-      pending = true;
-    }
-    stats.splits += 1;
-    return {best_leaf, right};
-  }
-
-  // grow_tree (serial.rs:578-628)
-  Tree train(const float* g, const float* h) {
+  Tree train_sequential(const float* g, const float* h) {
     before_train(g, h);
     Tree t(P.num_leaves);
     t.set_leaf_output(0, leaf_output_root(P, smaller.sum_g, smaller.sum_h));
-    if (before_find_best_split(t, 0, -1)) prepare_root_search(t);
+    if (before_find_best_split(t, 0, -1)) {
+      compute_used();
+      int nt = 0;
+      h_hitems[0] = HistItem{segs[0], 0, 0, chunk_for(N, true), 0};
+      h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
+      h_hitems[0].ntasks = nt;
+      run_hist(1, 0, nt, N);
+      ScanItem& si = h_sitems[0];
+      si = ScanItem{};
+      si.small = leaf_args(t, smaller, (int)smaller_slot);
+      si.small.part_first = 0;
+      si.small.nblk = nt;
+      run_scan(1);
+      fold_sequential(false);
+    }
     for (int step = 0; step < P.num_leaves - 1; ++step) {
-      run_search();
       size_t bl = 0;
       for (size_t i = 1; i < best.size(); ++i)
         if (best[i].better_than(best[bl])) bl = i;
       if (best[bl].o.gain <= 0.0) break;
-      split(t, (int)bl);
+      const HostSplit info = best[bl];
+      const int best_leaf = (int)bl;
+      const int right = commit_split(t, best_leaf, info);
+      const int lc = info.o.exact_left, rc = info.o.exact_right;
+      LeafSplits l{best_leaf, lc, info.o.left_sum_gradient, info.o.left_sum_hessian, info.o.left_output,
+                   info.o.left_int_sum};
+      LeafSplits r{right, rc, info.o.right_sum_gradient, info.o.right_sum_hessian, info.o.right_output,
+                   info.o.right_int_sum};
+      smaller = lc < rc ? l : r;
+      larger = lc < rc ? r : l;
+      const Segment ps = segs[(size_t)best_leaf];
+      const int dst = ps.buf == 1 ? 2 : 1;
+      segs[(size_t)best_leaf] = Segment{dst, ps.begin, lc};
+      segs[(size_t)right] = Segment{dst, ps.begin + lc, rc};
+      // the next step's before_find_best_split runs now: its pool moves happen
+      // before any other pool access, as in the reference
+      const bool search = step + 1 < P.num_leaves - 1 && before_find_best_split(t, best_leaf, right);
+      if (search) compute_used();
+      stage_wait();
+      int nt = 0;
+      PartItem& pi = h_pitems[0];
+      pi = PartItem{};
+      pi.seg = ps;
+      pi.dst_buf = dst;
+      pi.split_group = feats[(size_t)info.inner].group;
+      pi.left_total = lc;
+      pi.hist_side = search ? (lc < rc ? 0 : 1) : -1;
+      pi.chunk = chunk_for(ps.count, true);
+      pi.rule = rule_of(info);
+      pi.first_task = add_tasks(0, ps.count, pi.chunk, nt);
+      pi.ntasks = nt;
+      run_partition(1, nt, ps.count, search ? std::min(lc, rc) : 0, search);
+      if (search) {
+        const bool use_subtract = parent_slot >= 0;
+        ScanItem& si = h_sitems[0];
+        si = ScanItem{};
+        si.small = leaf_args(t, smaller, (int)smaller_slot);
+        si.small.part_first = 0;
+        si.small.nblk = nt;
+        si.has_large = larger.leaf >= 0 && larger_slot >= 0;
+        si.use_subtract = use_subtract;
+        if (si.has_large) {
+          si.large = leaf_args(t, larger, (int)larger_slot);
+          if (!use_subtract) {
+            // the parent's histogram was evicted: build the larger child too
+            const Segment lseg = segs[(size_t)larger.leaf];
+            int ht = nt;
+            h_hitems[0] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
+            h_hitems[0].first_task = add_tasks(0, lseg.count, h_hitems[0].chunk, ht);
+            h_hitems[0].ntasks = ht - nt;
+            run_hist(1, nt, ht, lseg.count);
+            si.large.part_first = nt;
+            si.large.nblk = ht - nt;
+          }
+        }
+        run_scan(1);
+        fold_sequential(si.has_large != 0);
+      }
     }
-    pending = false;
+    return t;
+  }
+
+  // ===========================================================================
+  // Batched form: every round expands the top frontier leaves in one launch of
+  // each kernel; commits replay the reference's best-first order exactly.
+  // ===========================================================================
+  Tree train_batched(const float* g, const float* h) {
+    LeafSplits root = before_train(g, h);
+    Tree t(P.num_leaves);
+    t.set_leaf_output(0, leaf_output_root(P, root.sum_g, root.sum_h));
+    nodes.clear();
+    nodes.reserve((size_t)P.num_leaves * 4 + 4);
+    Node r0;
+    r0.seg = Segment{0, 0, N};
+    r0.ls = root;
+    r0.slot = 0;
+    r0.splittable.assign((size_t)F, 1);
+    nodes.push_back(r0);
+    leaf_node.assign((size_t)P.num_leaves, -1);
+    leaf_node[0] = 0;
+    int next_slot = 1;
+    // the root search (serial.rs:591-604 with the root's before_find_best_split)
+    if (N >= P.min_data_in_leaf * 2) {
+      for (int f = 0; f < F; ++f) h_used[f] = 1;
+      int nt = 0;
+      h_hitems[0] = HistItem{nodes[0].seg, 0, 0, chunk_for(N, true), 0};
+      h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
+      h_hitems[0].ntasks = nt;
+      run_hist(1, 0, nt, N);
+      ScanItem& si = h_sitems[0];
+      si = ScanItem{};
+      si.small = leaf_args(t, root, 0);
+      si.small.part_first = 0;
+      si.small.nblk = nt;
+      si.used_off = 0;
+      si.out_off = 0;
+      run_scan(1);
+      nodes[0].best = fold(0, h_used, nodes[0].splittable);
+    } else {
+      nodes[0].best.o.gain = kMinScore;
+    }
+    best[0] = nodes[0].best;
+    std::vector<int> order;
+    bool done = false;
+    while (!done) {
+      // replay commits in the reference's order while the needed children are known
+      for (;;) {
+        if (t.num_leaves >= P.num_leaves) {
+          done = true;
+          break;
+        }
+        size_t bl = 0;
+        for (size_t i = 1; i < best.size(); ++i)
+          if (best[i].better_than(best[bl])) bl = i;
+        if (best[bl].o.gain <= 0.0) {
+          done = true;
+          break;
+        }
+        const int x = leaf_node[bl];
+        if (!nodes[(size_t)x].expanded) break;
+        const HostSplit info = best[bl];
+        const int right = commit_split(t, (int)bl, info);
+        const int cl = nodes[(size_t)x].child[0], cr = nodes[(size_t)x].child[1];
+        nodes[(size_t)cl].ls.leaf = (int)bl;
+        nodes[(size_t)cr].ls.leaf = right;
+        leaf_node[bl] = cl;
+        leaf_node[(size_t)right] = cr;
+        best[bl] = nodes[(size_t)cl].best;
+        best[(size_t)right] = nodes[(size_t)cr].best;
+      }
+      if (done) break;
+      // expand the best unexpanded frontier leaves, at most the remaining budget
+      stage_wait();
+      order.clear();
+      for (int l = 0; l < t.num_leaves; ++l) {
+        const Node& nd = nodes[(size_t)leaf_node[(size_t)l]];
+        if (!nd.expanded && nd.best.o.gain > 0.0) order.push_back(l);
+      }
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return best[(size_t)a].better_than(best[(size_t)b]); });
+      const size_t budget = (size_t)(P.num_leaves - t.num_leaves);
+      if (order.size() > budget) order.resize(budget);
+      int ni = 0, nt = 0, ns = 0;
+      double part_rows = 0.0, hist_rows = 0.0;
+      bool any_hist = false;
+      for (int l : order) {
+        const int x = leaf_node[(size_t)l];
+        Node& X = nodes[(size_t)x];
+        const HostSplit& info = X.best;
+        const int lc = info.o.exact_left, rc = info.o.exact_right;
+        const int dst = X.seg.buf == 1 ? 2 : 1;
+        Node cl, cr;
+        cl.depth = cr.depth = X.depth + 1;
+        cl.seg = Segment{dst, X.seg.begin, lc};
+        cr.seg = Segment{dst, X.seg.begin + lc, rc};
+        cl.ls = LeafSplits{-1, lc, info.o.left_sum_gradient, info.o.left_sum_hessian, info.o.left_output,
+                           info.o.left_int_sum};
+        cr.ls = LeafSplits{-1, rc, info.o.right_sum_gradient, info.o.right_sum_hessian, info.o.right_output,
+                           info.o.right_int_sum};
+        cl.best.o.gain = cr.best.o.gain = kMinScore;
+        const bool search = children_searchable(X.depth + 1, lc, rc);
+        const bool left_small = lc < rc;
+        PartItem& pi = h_pitems[ni];
+        pi = PartItem{};
+        pi.seg = X.seg;
+        pi.dst_buf = dst;
+        pi.split_group = feats[(size_t)info.inner].group;
+        pi.left_total = lc;
+        pi.hist_side = search ? (left_small ? 0 : 1) : -1;
+        pi.chunk = chunk_for(X.seg.count, false);
+        pi.rule = rule_of(info);
+        pi.first_task = add_tasks(ni, X.seg.count, pi.chunk, nt);
+        pi.ntasks = nt - pi.first_task;
+        part_rows += X.seg.count;
+        if (search) {
+          any_hist = true;
+          hist_rows += std::min(lc, rc);
+          // the smaller child gets a new slot; the larger one reuses the parent's
+          Node& sm = left_small ? cl : cr;
+          Node& lg = left_small ? cr : cl;
+          if ((size_t)next_slot >= num_slots) throw std::runtime_error("histogram slots exhausted");
+          sm.slot = next_slot++;
+          lg.slot = X.slot;
+          sm.splittable = X.splittable;
+          lg.splittable = X.splittable;
+          ScanItem& si = h_sitems[ns];
+          si = ScanItem{};
+          // a child's parent output is its own output from the split (never the root's formula)
+          si.small = leaf_args_with(sm.ls, sm.slot, sm.ls.weight);
+          si.small.part_first = pi.first_task;
+          si.small.nblk = pi.ntasks;
+          si.large = leaf_args_with(lg.ls, lg.slot, lg.ls.weight);
+          si.has_large = 1;
+          si.use_subtract = 1;
+          si.used_off = ns * F;
+          si.out_off = ns * 2 * F;
+          for (int f = 0; f < F; ++f) h_used[ns * F + f] = X.splittable[(size_t)f] ? 1 : 0;
+          ++ns;
+        }
+        X.child[0] = (int)nodes.size();
+        X.child[1] = (int)nodes.size() + 1;
+        X.expanded = true;
+        nodes.push_back(cl);
+        nodes.push_back(cr);
+        ++ni;
+      }
+      if (ni == 0) break;
+      run_partition(ni, nt, part_rows, hist_rows, any_hist);
+      if (ns > 0) run_scan(ns);
+      // fold the children's candidates
+      int k = 0;
+      for (int l : order) {
+        Node& X = nodes[(size_t)leaf_node[(size_t)l]];
+        const HostSplit& info = X.best;
+        const int lc = info.o.exact_left, rc = info.o.exact_right;
+        if (!children_searchable(X.depth + 1, lc, rc)) continue;
+        const bool left_small = lc < rc;
+        Node& sm = nodes[(size_t)X.child[left_small ? 0 : 1]];
+        Node& lg = nodes[(size_t)X.child[left_small ? 1 : 0]];
+        std::vector<char> sp_s(sm.splittable), sp_l(lg.splittable);
+        const int* used = h_used + k * F;
+        sm.best = fold(k * 2 * F, used, sp_s);
+        lg.best = fold(k * 2 * F + F, used, sp_l);
+        for (int f = 0; f < F; ++f) {
+          if (!used[f]) {
+            sp_s[(size_t)f] = 0;
+            sp_l[(size_t)f] = 0;
+          }
+        }
+        sm.splittable = sp_s;
+        lg.splittable = sp_l;
+        ++k;
+      }
+    }
+    for (int l = 0; l < t.num_leaves; ++l) segs[(size_t)l] = nodes[(size_t)leaf_node[(size_t)l]].seg;
+    stats.speculative_waste = 0;
+    for (const Node& nd : nodes)
+      if (nd.expanded) ++stats.speculative_waste;
+    stats.speculative_waste -= (t.num_leaves - 1);
+    return t;
+  }
+
+  // grow_tree (serial.rs:578-628) followed by renew_int_grad_tree_output (634-655)
+  Tree train(const float* g, const float* h) {
+    Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
     if (renew) {
-      BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
-                             stream));
+      for (int i = 0; i < t.num_leaves; ++i) h_segs[i] = segs[(size_t)i];
+      BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice, stream));
       // largest leaves first: the longest chains start on the first blocks dispatched
-      renew_order.resize((size_t)t.num_leaves);
-      for (int i = 0; i < t.num_leaves; ++i) renew_order[(size_t)i] = i;
-      std::stable_sort(renew_order.begin(), renew_order.end(),
+      for (int i = 0; i < t.num_leaves; ++i) h_order[i] = i;
+      std::stable_sort(h_order, h_order + t.num_leaves,
                        [&](int a, int b) { return segs[(size_t)a].count > segs[(size_t)b].count; });
-      BK_HIPC(hipMemcpyAsync(d_order, renew_order.data(), sizeof(int) * (size_t)t.num_leaves, hipMemcpyHostToDevice,
-                             stream));
+      BK_HIPC(hipMemcpyAsync(d_order, h_order, sizeof(int) * (size_t)t.num_leaves, hipMemcpyHostToDevice, stream));
       timed(5, (double)N * 12.0, (double)N * 12.0,
             [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, d_renew, stream); });
       std::vector<double> sums((size_t)t.num_leaves * 2);
@@ -1123,8 +1367,12 @@ struct Learner {
 
   void add_score_device(const bk_tree_out* t, double* d_score) {
     if (t->num_leaves <= 1) return;
-    BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_leaf_vals, t->leaf_value, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
+    for (int i = 0; i < t->num_leaves; ++i) {
+      h_segs[i] = segs[(size_t)i];
+      h_vals[i] = t->leaf_value[i];
+    }
+    BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_leaf_vals, h_vals, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
                            stream));
     timed(6, (double)N * 20.0, (double)N * 20.0, [&] {
       launch_add_score(ds, d_segs, d_leaf_vals, t->num_leaves, d_score, max_segment(t->num_leaves), stream);
@@ -1136,7 +1384,8 @@ struct Learner {
   void add_score_host(const bk_tree_out* t, double* score) {
     if (t->num_leaves <= 1) return;
     if (!d_leaf_of) d_leaf_of = dmalloc<int32_t>((size_t)N);
-    BK_HIPC(hipMemcpyAsync(d_segs, segs.data(), sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
+    for (int i = 0; i < t->num_leaves; ++i) h_segs[i] = segs[(size_t)i];
+    BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
     launch_leaf_of(ds, d_segs, t->num_leaves, max_segment(t->num_leaves), d_leaf_of, stream);
     std::vector<int32_t> lo((size_t)N);
     BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));

@@ -15,15 +15,16 @@ import synth
 pytestmark = pytest.mark.gpu
 
 
-def test_fullsize_quantized_trees_bit_exact():
-    print("\n[case] full size 10.5M x 28 quantized, 2 trees, GPU vs oracle", flush=True)
+@pytest.mark.parametrize("batched", [1, 0])
+def test_fullsize_quantized_trees_bit_exact(batched):
+    print(f"\n[case] full size 10.5M x 28 quantized batched={batched}, 2 trees, GPU vs oracle", flush=True)
     n = 10_500_000
     x, y = synth.binary_28(n, seed=0, intercept=synth.solve_intercept(0))
     ds = bg.Dataset(x, bg.DatasetParams(num_threads=16))
     del x
     p = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                          stochastic_rounding=1, quant_train_renew_leaf=1, seed=0, num_threads=16,
-                         form=bg.FORM_QUANTIZED)
+                         form=bg.FORM_QUANTIZED, batched=batched)
     obj = bst.BinaryObjective(y)
     score = np.full(n, obj.boost_from_score(0))
     gpu = bg.Learner(ds, p)

@@ -77,13 +77,26 @@ def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise):
     return trees
 
 
+@pytest.mark.parametrize("batched", [0, 1])
 @pytest.mark.parametrize("bins,stochastic,renew", [(4, 1, 1), (4, 0, 0), (16, 1, 0), (2, 1, 1), (127, 1, 1)])
-def test_quantized_bit_exact(binary_10k, bins, stochastic, renew):
-    announce(f"quantized 10k x 28 bins={bins} stochastic={stochastic} renew={renew}")
+def test_quantized_bit_exact(binary_10k, bins, stochastic, renew, batched):
+    announce(f"quantized 10k x 28 bins={bins} stochastic={stochastic} renew={renew} batched={batched}")
     ds, y = binary_10k
     p = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=bins, stochastic_rounding=stochastic,
-                         quant_train_renew_leaf=renew, seed=7, num_threads=4, form=bg.FORM_QUANTIZED)
+                         quant_train_renew_leaf=renew, seed=7, num_threads=4, form=bg.FORM_QUANTIZED, batched=batched)
     lockstep(ds, bst.BinaryObjective(y), p, iters=6)
+
+
+@pytest.mark.parametrize("leaves", [2, 3, 63, 255])
+def test_batched_quantized_leaf_budgets(binary_10k, leaves):
+    """Speculative expansion must never change the tree: small and large leaf budgets,
+    where the last round expands more leaves than can be committed."""
+    announce(f"batched quantized 10k x 28 num_leaves={leaves}")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=leaves, min_data_in_leaf=5, use_quantized_grad=1, num_grad_quant_bins=4,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=4, num_threads=4,
+                         form=bg.FORM_QUANTIZED, batched=1)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=3)
 
 
 def test_quantized_model_text_identical(binary_10k):
@@ -110,21 +123,24 @@ def test_quantized_constant_hessian_regression():
     lockstep(ds, bst.RegressionL2Objective(yv), p, iters=4)
 
 
+@pytest.mark.parametrize("batched", [0, 1])
 @pytest.mark.parametrize("form", [bg.FORM_PLAIN_F64, bg.FORM_F32_COMPENSATED])
-def test_float_forms_match_oracle(binary_10k, form):
-    announce(f"float form {form} 10k x 28")
+def test_float_forms_match_oracle(binary_10k, form, batched):
+    announce(f"float form {form} 10k x 28 batched={batched}")
     ds, y = binary_10k
-    p = bg.LearnerParams(num_leaves=31, form=form)
+    p = bg.LearnerParams(num_leaves=31, form=form, batched=batched)
     lockstep(ds, bst.BinaryObjective(y), p, iters=5, compare=assert_trees_close)
 
 
+@pytest.mark.parametrize("batched", [0, 1])
 @pytest.mark.parametrize("n,leaves,min_data", [(25, 4, 2), (63, 8, 3), (1001, 31, 20), (4097, 63, 5), (9999, 255, 1)])
-def test_ragged_sizes_quantized(n, leaves, min_data):
-    announce(f"ragged n={n} leaves={leaves} min_data={min_data}")
+def test_ragged_sizes_quantized(n, leaves, min_data, batched):
+    announce(f"ragged n={n} leaves={leaves} min_data={min_data} batched={batched}")
     x, y = synth.binary_28(n, seed=n)
     ds = bg.Dataset(x, bg.DatasetParams(min_data_in_leaf=min_data))
     p = bg.LearnerParams(num_leaves=leaves, min_data_in_leaf=min_data, use_quantized_grad=1, num_grad_quant_bins=4,
-                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=2, form=bg.FORM_QUANTIZED)
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=2, form=bg.FORM_QUANTIZED,
+                         batched=batched)
     lockstep(ds, bst.BinaryObjective(y), p, iters=3)
 
 
@@ -134,6 +150,9 @@ def test_max_depth_and_pool_eviction(binary_10k):
     p = bg.LearnerParams(num_leaves=31, max_depth=4, use_quantized_grad=1, num_grad_quant_bins=4, seed=9,
                          form=bg.FORM_QUANTIZED)
     lockstep(ds, bst.BinaryObjective(y), p, iters=3)
+    p.batched = 1
+    lockstep(ds, bst.BinaryObjective(y), p, iters=3)
+    p.batched = 0
     bytes_per_leaf = sum(int(b) for b in ds.feature_meta()["num_bin"]) * 16
     p2 = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=4, seed=9, form=bg.FORM_QUANTIZED,
                           histogram_pool_size=3 * bytes_per_leaf / (1024.0 * 1024.0))
@@ -151,9 +170,10 @@ def test_golden_models_on_gpu(case):
     tg.compare_models(ds, gbdt, model, objective)
 
 
+@pytest.mark.parametrize("batched", [0, 1])
 @pytest.mark.parametrize("case", ["cat_default", "cat_nan", "l2_nan", "l2_zero_as_missing", "multiclass"])
-def test_golden_datasets_quantized_bit_exact(case):
-    announce(f"golden dataset {case} quantized, GPU vs oracle")
+def test_golden_datasets_quantized_bit_exact(case, batched):
+    announce(f"golden dataset {case} quantized batched={batched}, GPU vs oracle")
     import test_golden_train as tg
     spec, p, x, y, w, model = tg.build_case(case)
     ds = bg.Dataset(x, tg.dataset_params(p), categorical=spec["categorical"])
@@ -161,6 +181,7 @@ def test_golden_datasets_quantized_bit_exact(case):
     lp = tg.learner_params(p)
     lp.use_quantized_grad, lp.num_grad_quant_bins, lp.seed, lp.form = 1, 4, 7, bg.FORM_QUANTIZED
     lp.quant_train_renew_leaf = 1
+    lp.batched = batched
     if obj.num_class > 1:
         obj = bst.BinaryObjective((y == 1).astype(float))
     lockstep(ds, obj, lp, iters=4)
