@@ -221,8 +221,21 @@ void launch_discretize(const float* g, const float* h, const double* gnoise, con
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream);
 void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream);
+// Leaf renewal sums: leaf_of from the sorted nonempty final segments, per-block
+// partials, the certificate per leaf, then the sequential chain for flagged leaves
+// (d_flags == nullptr runs the chain for every leaf).
+void launch_leaf_of_pos(const DevState& s, const Segment* d_sorted, const int* d_sorted_leaf, int nseg,
+                        int32_t* leaf_of, void* stream);
+int renew_parts(int n);
+size_t renew_par_lds(int num_leaves);
+void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float* h, int num_leaves,
+                      double* part_sums, int* part_meta, void* stream);
+void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const double* part_sums, const int* part_meta,
+                      double* d_sums, int* d_flags, void* stream);
 void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
-                       const float* h, double* d_sums, void* stream);
+                       const float* h, const int* d_flags, double* d_sums, void* stream);
+void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
+                           void* stream);
 void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
                       double* score, int max_seg, void* stream);
 void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream);

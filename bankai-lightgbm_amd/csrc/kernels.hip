@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -1530,17 +1532,160 @@ __global__ void k_pack_f32(const float* __restrict__ g, const float* __restrict_
     gh[i] = make_float2(g[i], const_hess ? 1.0f : h[i]);
 }
 
-// renew_int_grad_tree_output (serial.rs:634-655): sequential f64 sums over each
-// leaf's rows in ascending row order. The order of additions is the reference's,
-// so the chain itself stays on one lane: wave 0 sums gradients, wave 1 hessians.
-// The other lanes gather the next 256 rows into LDS while lane 0 adds the
-// current ones, so the chain is limited by the f64 add latency, not by memory.
+// ---------------------------------------------------------------------------
+// renew_int_grad_tree_output (serial.rs:634-655) sums each leaf's f32 gradients
+// and hessians in f64, one addition at a time in ascending row order.
+//
+// Exactness certificate: let E be the exponent of the lowest significand bit
+// over a leaf's nonzero values, so every value and every partial sum is a
+// multiple of 2^E. If n * max|x| < 2^(52+E), every partial sum, in ANY order,
+// is below 2^(53+E) in magnitude and hence exactly representable: no addition
+// rounds, and the parallel sum equals the sequential one bit for bit. Leaves
+// without the certificate (very small values next to large sums, NaN, Inf) are
+// summed by the sequential chain below.
+// ---------------------------------------------------------------------------
+
+// exponent of the lowest significand bit of a float: INT_MAX for zero, INT_MIN for NaN/Inf
+__device__ __forceinline__ int lsb_exponent(float x) {
+  const uint32_t b = __float_as_uint(x) & 0x7fffffffu;
+  if (b == 0u) return INT_MAX;
+  const int be = (int)(b >> 23);
+  if (be == 0xff) return INT_MIN;
+  return be == 0 ? -149 : be - 150;
+}
+
+// Leaf of every row. The position ranges of the final leaves tile [0, N): a
+// position belongs to the last range (sorted by begin) starting at or before it.
+constexpr int kLeafLds = 1024;
+__global__ void __launch_bounds__(256) k_leaf_of_pos(DevState s, const Segment* __restrict__ sorted,
+                                                     const int* __restrict__ sorted_leaf, int nseg,
+                                                     int32_t* __restrict__ leaf_of) {
+  __shared__ int begin_lds[kLeafLds];
+  const bool in_lds = nseg <= kLeafLds;
+  if (in_lds)
+    for (int i = threadIdx.x; i < nseg; i += blockDim.x) begin_lds[i] = sorted[i].begin;
+  __syncthreads();
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < s.num_data; p += gridDim.x * blockDim.x) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      const int b = in_lds ? begin_lds[mid] : sorted[mid].begin;
+      if (b <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    const int buf = sorted[lo].buf;
+    const int row = buf == 0 ? p : s.idx[buf][p];
+    leaf_of[row] = sorted_leaf[lo];
+  }
+}
+
+// Per-block partial sums per leaf, in row order over a grid stride. LDS holds
+// per leaf: sum g, sum h (f64), max|g|, max|h| (float bits), min lsb exponents.
+__global__ void __launch_bounds__(256) k_renew_par(int n, const int32_t* __restrict__ leaf_of,
+                                                   const float* __restrict__ g, const float* __restrict__ h,
+                                                   int num_leaves, double* __restrict__ part_sums,
+                                                   int* __restrict__ part_meta) {
+  extern __shared__ unsigned char smem[];
+  double* acc = (double*)smem;                     // [leaf][2]
+  int* meta = (int*)(acc + 2 * (size_t)num_leaves);  // [leaf][4]: maxabs g, maxabs h, lsb g, lsb h
+  for (int i = threadIdx.x; i < num_leaves; i += blockDim.x) {
+    acc[2 * i] = 0.0;
+    acc[2 * i + 1] = 0.0;
+    meta[4 * i] = 0;
+    meta[4 * i + 1] = 0;
+    meta[4 * i + 2] = INT_MAX;
+    meta[4 * i + 3] = INT_MAX;
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int l = leaf_of[i];
+    if ((unsigned)l >= (unsigned)num_leaves) continue;  // never taken: the leaf ranges tile every row
+    const float x = g[i], y = h[i];
+    atomicAdd(&acc[2 * l], (double)x);
+    atomicAdd(&acc[2 * l + 1], (double)y);
+    // NaN and Inf compare above every finite magnitude as unsigned bits and
+    // also force INT_MIN exponents, so the certificate fails for them
+    const int ax = (int)(__float_as_uint(x) & 0x7fffffffu), ay = (int)(__float_as_uint(y) & 0x7fffffffu);
+    const int ex = lsb_exponent(x), ey = lsb_exponent(y);
+    if (ax > meta[4 * l]) atomicMax(&meta[4 * l], ax);
+    if (ay > meta[4 * l + 1]) atomicMax(&meta[4 * l + 1], ay);
+    if (ex < meta[4 * l + 2]) atomicMin(&meta[4 * l + 2], ex);
+    if (ey < meta[4 * l + 3]) atomicMin(&meta[4 * l + 3], ey);
+  }
+  __syncthreads();
+  double* ps = part_sums + (size_t)blockIdx.x * num_leaves * 2;
+  int* pm = part_meta + (size_t)blockIdx.x * num_leaves * 4;
+  for (int i = threadIdx.x; i < 2 * num_leaves; i += blockDim.x) ps[i] = acc[i];
+  for (int i = threadIdx.x; i < 4 * num_leaves; i += blockDim.x) pm[i] = meta[i];
+}
+
+// One block per leaf: folds the block partials, checks the certificate, writes
+// the certified sums and flags the others for the sequential chain.
+__global__ void __launch_bounds__(256) k_renew_fix(const Segment* __restrict__ segs, int num_leaves, int nparts,
+                                                   const double* __restrict__ part_sums,
+                                                   const int* __restrict__ part_meta, double* __restrict__ out,
+                                                   int* __restrict__ flags) {
+  __shared__ double rs[2][256];
+  __shared__ int rm[4][256];
+  const int leaf = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  int mg = 0, mh = 0, eg = INT_MAX, eh = INT_MAX;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    const size_t o = (size_t)k * num_leaves + leaf;
+    a += part_sums[2 * o];
+    b += part_sums[2 * o + 1];
+    mg = max(mg, part_meta[4 * o]);
+    mh = max(mh, part_meta[4 * o + 1]);
+    eg = min(eg, part_meta[4 * o + 2]);
+    eh = min(eh, part_meta[4 * o + 3]);
+  }
+  rs[0][threadIdx.x] = a;
+  rs[1][threadIdx.x] = b;
+  rm[0][threadIdx.x] = mg;
+  rm[1][threadIdx.x] = mh;
+  rm[2][threadIdx.x] = eg;
+  rm[3][threadIdx.x] = eh;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int t = threadIdx.x, u = threadIdx.x + w;
+      rs[0][t] += rs[0][u];
+      rs[1][t] += rs[1][u];
+      rm[0][t] = max(rm[0][t], rm[0][u]);
+      rm[1][t] = max(rm[1][t], rm[1][u]);
+      rm[2][t] = min(rm[2][t], rm[2][u]);
+      rm[3][t] = min(rm[3][t], rm[3][u]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)segs[leaf].count;
+    auto certified = [&](int maxbits, int e) {
+      if (e == INT_MAX) return true;  // only zeros
+      if (e == INT_MIN || maxbits >= 0x7f800000) return false;
+      const double bound = n * (double)__uint_as_float((uint32_t)maxbits);
+      return bound < ldexp(1.0, 52 + e);
+    };
+    const bool okg = certified(rm[0][0], rm[2][0]), okh = certified(rm[1][0], rm[3][0]);
+    out[2 * leaf] = rs[0][0];
+    out[2 * leaf + 1] = rs[1][0];
+    flags[leaf] = (okg ? 0 : 1) | (okh ? 0 : 2);
+  }
+}
+
+// The sequential chain for flagged leaves: wave 0 sums gradients, wave 1
+// hessians, on lane 0 in ascending row order. The other lanes gather the next
+// 256 rows into LDS while lane 0 adds the current ones, so the chain is limited
+// by the f64 add latency, not by memory.
 constexpr int kRenewChunk = 256;
 __global__ void __launch_bounds__(128) k_renew_sums(DevState s, const Segment* __restrict__ segs,
                                                     const int* __restrict__ order, const float* __restrict__ g,
-                                                    const float* __restrict__ h, double* out) {
+                                                    const float* __restrict__ h, const int* __restrict__ flags,
+                                                    double* out) {
   __shared__ float buf[2][2][kRenewChunk];
   const int leaf = order[blockIdx.x];
+  const int need = flags ? flags[leaf] : 3;
+  if (need == 0) return;  // block-uniform
   const Segment seg = segs[leaf];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float* src = wave == 0 ? g : h;
@@ -1572,7 +1717,15 @@ __global__ void __launch_bounds__(128) k_renew_sums(DevState s, const Segment* _
       for (int i = 0; i < n; ++i) acc += (double)cur[i];
     }
   }
-  if (lane == 0) out[2 * leaf + wave] = acc;
+  if (lane == 0 && (need & (1 << wave))) out[2 * leaf + wave] = acc;
+}
+
+__global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, const double* __restrict__ vals,
+                                 int num_leaves, double* __restrict__ score) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int l = leaf_of[i];
+    if (l >= 0 && l < num_leaves) score[i] += vals[l];
+  }
 }
 
 __global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const double* __restrict__ vals,
@@ -1773,9 +1926,9 @@ void launch_pack_f32(const float* g, const float* h, int n, int const_hess, floa
 }
 
 void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
-                       const float* h, double* d_sums, void* stream) {
+                       const float* h, const int* d_flags, double* d_sums, void* stream) {
   hipLaunchKernelGGL(k_renew_sums, dim3((unsigned)num_leaves), dim3(128), 0, (hipStream_t)stream, s, d_segs, d_order, g,
-                     h, d_sums);
+                     h, d_flags, d_sums);
   BK_HIP(hipGetLastError());
 }
 
@@ -1805,6 +1958,41 @@ int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, voi
   hipLaunchKernelGGL(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, n, d_partials);
   BK_HIP(hipGetLastError());
   return nb;
+}
+
+void launch_leaf_of_pos(const DevState& s, const Segment* d_sorted, const int* d_sorted_leaf, int nseg,
+                        int32_t* leaf_of, void* stream) {
+  hipLaunchKernelGGL(k_leaf_of_pos, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), 0, (hipStream_t)stream, s,
+                     d_sorted, d_sorted_leaf, nseg, leaf_of);
+  BK_HIP(hipGetLastError());
+}
+
+int renew_parts(int n) { return grid_for(n, 256 * 16, 1024); }
+
+size_t renew_par_lds(int num_leaves) { return (size_t)num_leaves * (2 * sizeof(double) + 4 * sizeof(int)); }
+
+void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float* h, int num_leaves,
+                      double* part_sums, int* part_meta, void* stream) {
+  const size_t lds = renew_par_lds(num_leaves);
+  if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_renew_par,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_renew_par, dim3(renew_parts(n)), dim3(256), lds, (hipStream_t)stream, n, leaf_of, g, h,
+                     num_leaves, part_sums, part_meta);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const double* part_sums, const int* part_meta,
+                      double* d_sums, int* d_flags, void* stream) {
+  hipLaunchKernelGGL(k_renew_fix, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, d_segs, num_leaves,
+                     nparts, part_sums, part_meta, d_sums, d_flags);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
+                           void* stream) {
+  hipLaunchKernelGGL(k_add_score_rows, dim3(grid_for(n, 256, 8192)), dim3(256), 0, (hipStream_t)stream, n, leaf_of,
+                     d_leaf_values, num_leaves, score);
+  BK_HIP(hipGetLastError());
 }
 
 void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,

@@ -398,6 +398,15 @@ struct Learner {
   int* d_order = nullptr;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
+  Segment* d_sorted_segs = nullptr;
+  int* d_sorted_leaf = nullptr;
+  double* d_renew_part_sums = nullptr;
+  int* d_renew_part_meta = nullptr;
+  int* d_renew_flags = nullptr;
+  Segment* h_sorted_segs = nullptr;
+  int* h_sorted_leaf = nullptr;
+  bool leaf_of_valid = false;
+  int renew_nparts = 0;
   // pinned host staging
   SplitOut* h_out = nullptr;
   HistItem* h_hitems = nullptr;
@@ -466,10 +475,12 @@ struct Learner {
                     (void*)d_htile_first, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_hitems,
                     (void*)d_pitems, (void*)d_sitems, (void*)d_tasks, (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
-                    (void*)d_leaf_of})
+                    (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
+                    (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
     for (void* p : {(void*)h_out, (void*)h_hitems, (void*)h_pitems, (void*)h_sitems, (void*)h_tasks, (void*)h_used,
-                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals})
+                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
+                    (void*)h_sorted_leaf})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -667,6 +678,20 @@ struct Learner {
     pinned(&h_segs, (size_t)P.num_leaves);
     pinned(&h_order, (size_t)P.num_leaves);
     pinned(&h_vals, (size_t)P.num_leaves);
+    d_leaf_of = dmalloc<int32_t>((size_t)npad);
+    BK_HIPC(hipMemsetAsync(d_leaf_of, 0, sizeof(int32_t) * (size_t)npad, stream));
+    d_sorted_segs = dmalloc<Segment>((size_t)P.num_leaves);
+    d_sorted_leaf = dmalloc<int>((size_t)P.num_leaves);
+    pinned(&h_sorted_segs, (size_t)P.num_leaves);
+    pinned(&h_sorted_leaf, (size_t)P.num_leaves);
+    if (renew) {
+      d_renew_flags = dmalloc<int>((size_t)P.num_leaves);
+      if (renew_par_lds(P.num_leaves) <= (size_t)kLdsBytes) {
+        renew_nparts = renew_parts(N);
+        d_renew_part_sums = dmalloc<double>((size_t)renew_nparts * P.num_leaves * 2);
+        d_renew_part_meta = dmalloc<int>((size_t)renew_nparts * P.num_leaves * 4);
+      }
+    }
     if (stochastic) {
       // init_stochastic (gradient_discretizer/mod.rs:87-106): per-block MT19937 noise
       std::vector<double> gn((size_t)N), hn((size_t)N);
@@ -1301,7 +1326,40 @@ struct Learner {
   }
 
   // grow_tree (serial.rs:578-628) followed by renew_int_grad_tree_output (634-655)
+  // Leaf of every row of the last tree (one scattered write per row).
+  void ensure_leaf_of(int num_leaves) {
+    if (leaf_of_valid) return;
+    stage_wait();
+    int n = 0;
+    for (int l = 0; l < num_leaves; ++l) {
+      if (segs[(size_t)l].count == 0) continue;
+      h_sorted_segs[n] = segs[(size_t)l];
+      h_sorted_leaf[n] = l;
+      ++n;
+    }
+    std::vector<int> perm((size_t)n);
+    for (int i = 0; i < n; ++i) perm[(size_t)i] = i;
+    std::sort(perm.begin(), perm.end(), [&](int a, int b) { return h_sorted_segs[a].begin < h_sorted_segs[b].begin; });
+    std::vector<Segment> ss((size_t)n);
+    std::vector<int> sl((size_t)n);
+    for (int i = 0; i < n; ++i) {
+      ss[(size_t)i] = h_sorted_segs[perm[(size_t)i]];
+      sl[(size_t)i] = h_sorted_leaf[perm[(size_t)i]];
+    }
+    for (int i = 0; i < n; ++i) {
+      h_sorted_segs[i] = ss[(size_t)i];
+      h_sorted_leaf[i] = sl[(size_t)i];
+    }
+    BK_HIPC(hipMemcpyAsync(d_sorted_segs, h_sorted_segs, sizeof(Segment) * (size_t)n, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_sorted_leaf, h_sorted_leaf, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, stream));
+    staged();
+    timed(6, (double)N * 8.0, (double)N * 8.0,
+          [&] { launch_leaf_of_pos(ds, d_sorted_segs, d_sorted_leaf, n, d_leaf_of, stream); });
+    leaf_of_valid = true;
+  }
+
   Tree train(const float* g, const float* h) {
+    leaf_of_valid = false;
     Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
     if (renew) {
       for (int i = 0; i < t.num_leaves; ++i) h_segs[i] = segs[(size_t)i];
@@ -1311,8 +1369,19 @@ struct Learner {
       std::stable_sort(h_order, h_order + t.num_leaves,
                        [&](int a, int b) { return segs[(size_t)a].count > segs[(size_t)b].count; });
       BK_HIPC(hipMemcpyAsync(d_order, h_order, sizeof(int) * (size_t)t.num_leaves, hipMemcpyHostToDevice, stream));
-      timed(5, (double)N * 12.0, (double)N * 12.0,
-            [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, d_renew, stream); });
+      const int* flags = nullptr;
+      if (d_renew_part_sums && t.num_leaves > 1) {
+        // certified parallel sums; the chain runs only for leaves the certificate rejects
+        ensure_leaf_of(t.num_leaves);
+        timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
+          launch_renew_par(N, d_leaf_of, g, h, t.num_leaves, d_renew_part_sums, d_renew_part_meta, stream);
+          launch_renew_fix(d_segs, t.num_leaves, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew,
+                           d_renew_flags, stream);
+        });
+        flags = d_renew_flags;
+      }
+      timed(5, 0.0, 0.0,
+            [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
       std::vector<double> sums((size_t)t.num_leaves * 2);
       BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
       BK_HIPC(hipStreamSynchronize(stream));
@@ -1365,31 +1434,35 @@ struct Learner {
     return m;
   }
 
+  void check_tree_matches(const bk_tree_out* t) const {
+    if (t->num_leaves != last_num_leaves)
+      throw std::invalid_argument("the tree does not match the learner's last tree (num_leaves)");
+  }
+
   void add_score_device(const bk_tree_out* t, double* d_score) {
+    check_tree_matches(t);
     if (t->num_leaves <= 1) return;
-    for (int i = 0; i < t->num_leaves; ++i) {
-      h_segs[i] = segs[(size_t)i];
-      h_vals[i] = t->leaf_value[i];
-    }
-    BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
+    ensure_leaf_of(t->num_leaves);
+    stage_wait();
+    for (int i = 0; i < t->num_leaves; ++i) h_vals[i] = t->leaf_value[i];
     BK_HIPC(hipMemcpyAsync(d_leaf_vals, h_vals, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
                            stream));
-    timed(6, (double)N * 20.0, (double)N * 20.0, [&] {
-      launch_add_score(ds, d_segs, d_leaf_vals, t->num_leaves, d_score, max_segment(t->num_leaves), stream);
-    });
+    staged();
+    timed(6, (double)N * 20.0, (double)N * 20.0,
+          [&] { launch_add_score_rows(N, d_leaf_of, d_leaf_vals, t->num_leaves, d_score, stream); });
     BK_HIPC(hipStreamSynchronize(stream));
+    staged_pending = false;
     if (prof.on) prof.resolve();
   }
 
   void add_score_host(const bk_tree_out* t, double* score) {
+    check_tree_matches(t);
     if (t->num_leaves <= 1) return;
-    if (!d_leaf_of) d_leaf_of = dmalloc<int32_t>((size_t)N);
-    for (int i = 0; i < t->num_leaves; ++i) h_segs[i] = segs[(size_t)i];
-    BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t->num_leaves, hipMemcpyHostToDevice, stream));
-    launch_leaf_of(ds, d_segs, t->num_leaves, max_segment(t->num_leaves), d_leaf_of, stream);
+    ensure_leaf_of(t->num_leaves);
     std::vector<int32_t> lo((size_t)N);
     BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
     BK_HIPC(hipStreamSynchronize(stream));
+    staged_pending = false;
     for (int i = 0; i < N; ++i) score[i] += t->leaf_value[lo[(size_t)i]];
   }
 

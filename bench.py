@@ -10,6 +10,7 @@ update. Prints ONE JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -96,7 +97,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
     torch.cuda.synchronize()
     if profile:
         learner.profile(True)
-    stats = {"hist_rows": 0, "part_rows": 0, "splits": 0}
+    stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0}
     t0 = time.perf_counter()
     for i in range(steps):
         step(True)
@@ -151,6 +152,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=10_500_000)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--sequential-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
@@ -175,7 +177,7 @@ def main():
     lr = 0.1
     qparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
-                               form=bg.FORM_QUANTIZED)
+                               form=bg.FORM_QUANTIZED, batched=1)
     elapsed, stats, kstats, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=True)
     its = args.steps / elapsed
     log(f"quantized: {its:.2f} it/s ({1000 * elapsed / args.steps:.1f} ms/iter)")
@@ -196,7 +198,7 @@ def main():
         "config": {"workload": f"binary logloss {args.rows} x 28 dense N(0,1) f32, 53% positive, num_leaves 255, "
                                "max_bin 255, min_data_in_leaf 20, lr 0.1, use_quantized_grad, 4 bins, stochastic "
                                "rounding, quant_train_renew_leaf", "rows": args.rows, "features": 28,
-                   "num_leaves": 255, "max_bin": 255, "form": "quantized"},
+                   "num_leaves": 255, "max_bin": 255, "form": "quantized", "growth": "batched"},
     }
     # dominant kernel by time and its roofline
     dom = max(kstats, key=lambda k: kstats[k]["ms"])
@@ -216,8 +218,13 @@ def main():
     hx, hy = synth.binary_28(args.holdout, seed=seed, intercept=intercept, stream=3)
     q_auc = auc(hy, predict_raw(qtrees, hx))
     result["holdout_auc"] = q_auc
+    if args.sequential_steps > 0:
+        sparams = dataclasses.replace(qparams, batched=0)
+        sel, _, _, _ = run_gpu(ds, y, sparams, args.sequential_steps, 1, lr, profile=False)
+        result["sequential_quantized_iters_per_sec"] = args.sequential_steps / sel
+        log(f"sequential quantized: {args.sequential_steps / sel:.2f} it/s")
     if not args.no_plain:
-        pparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64)
+        pparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
         pel, _, _, ptrees = run_gpu(ds, y, pparams, args.plain_steps, 1, lr, profile=False)
         p_its = args.plain_steps / pel
         log(f"plain f64: {p_its:.2f} it/s")

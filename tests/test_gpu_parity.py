@@ -185,3 +185,34 @@ def test_golden_datasets_quantized_bit_exact(case, batched):
     if obj.num_class > 1:
         obj = bst.BinaryObjective((y == 1).astype(float))
     lockstep(ds, obj, lp, iters=4)
+
+
+class WideRangeObjective:
+    """Gradients spanning 14 decades: many leaves fail the renewal certificate
+    (n * max|x| < 2^(52+E)) and take the sequential chain, others pass."""
+    const_hess = False
+    num_class = 1
+
+    def __init__(self, n, seed):
+        self.rng = np.random.default_rng(seed)
+        self.n = n
+
+    def boost_from_score(self, k=0):
+        return 0.0
+
+    def gradients(self, score):
+        r = self.rng
+        g = r.normal(size=self.n) * 10.0 ** r.uniform(-12, 2, self.n)
+        h = 10.0 ** r.uniform(-9, 0, self.n)
+        return g.astype(np.float32), h.astype(np.float32)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_renew_certificate_fallback_bit_exact(batched):
+    announce(f"renewal with wide-range gradients (sequential fallback), batched={batched}")
+    x, _ = synth.binary_28(20_000, seed=11)
+    ds = bg.Dataset(x)
+    p = bg.LearnerParams(num_leaves=63, min_data_in_leaf=5, use_quantized_grad=1, num_grad_quant_bins=16,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=5, form=bg.FORM_QUANTIZED,
+                         batched=batched)
+    lockstep(ds, WideRangeObjective(20_000, 3), p, iters=3)
