@@ -83,7 +83,8 @@ class TreeOutC(C.Structure):
 
 class TrainStatsC(C.Structure):
     _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
-                ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32)]
+                ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32),
+                ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64)]
 
 
 class KernelStatsC(C.Structure):
@@ -380,6 +381,7 @@ class Learner:
         s = TrainStatsC()
         _check(self._lib.bk_learner_last_stats(self.handle, C.byref(s)))
         return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
+                "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "speculative_waste": s.speculative_waste}
 
     def profile(self, enable=True):

@@ -897,12 +897,15 @@ struct Learner {
     return first;
   }
 
-  void run_hist(int nitems, int first_task, int ntasks, double rows) {
+  // Algorithmic bytes per row: a root build reads F bins and the payload P; a
+  // leaf build also reads a 4-byte row index. A partition reads one bin (count
+  // pass), reads and writes a 4-byte row index (scatter pass).
+  void run_hist(int nitems, int first_task, int ntasks, double rows, bool root = true) {
     if (ntasks <= first_task) return;
     BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
     BK_HIPC(hipMemcpyAsync(d_tasks + first_task, h_tasks + first_task, sizeof(Task) * (size_t)(ntasks - first_task),
                            hipMemcpyHostToDevice, stream));
-    const double hb = rows * (G + payload_bytes());
+    const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
     timed(0, hb, hb, [&] {
       launch_hist(ds, d_hitems, d_tasks + first_task, ntasks - first_task, first_task, d_group_used, max_htile_bins,
                   d_htile_first, (int)htile_first.size() - 1, rows_per_flush, stream);
@@ -917,7 +920,7 @@ struct Learner {
     BK_HIPC(hipMemcpyAsync(d_tasks, h_tasks, sizeof(Task) * (size_t)ntasks, hipMemcpyHostToDevice, stream));
     const double rec = (double)(G + payload_bytes() + 4);
     timed(1, part_rows, part_rows, [&] { launch_partition_count(ds, d_pitems, d_tasks, ntasks, stream); });
-    const double algo = part_rows * 9.0 + hist_rows * (G + payload_bytes());
+    const double algo = part_rows * 8.0 + hist_rows * (G + payload_bytes() + 4);
     const double moved = part_rows * (2.0 * rec + 1.0);
     timed(2, algo, moved, [&] {
       launch_partition_scatter(ds, d_pitems, d_tasks, ntasks, any_hist, d_group_used, max_tile_bins, max_tile_groups,
@@ -987,6 +990,8 @@ struct Learner {
     }
     t.num_leaves += 1;
     stats.splits += 1;
+    stats.committed_part_rows += (int64_t)lc + rc;
+    if (children_searchable(t.leaf_depth[(size_t)best_leaf], lc, rc)) stats.committed_hist_rows += std::min(lc, rc);
     return t.num_leaves - 1;
   }
 
@@ -1070,6 +1075,7 @@ struct Learner {
       h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
       run_hist(1, 0, nt, N);
+      stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, smaller, (int)smaller_slot);
@@ -1133,7 +1139,8 @@ struct Learner {
             h_hitems[0] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
             h_hitems[0].first_task = add_tasks(0, lseg.count, h_hitems[0].chunk, ht);
             h_hitems[0].ntasks = ht - nt;
-            run_hist(1, nt, ht, lseg.count);
+            run_hist(1, nt, ht, lseg.count, false);
+            stats.committed_hist_rows += lseg.count;
             si.large.part_first = nt;
             si.large.nblk = ht - nt;
           }
@@ -1172,6 +1179,7 @@ struct Learner {
       h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
       run_hist(1, 0, nt, N);
+      stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, root, 0);

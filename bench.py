@@ -97,7 +97,8 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
     torch.cuda.synchronize()
     if profile:
         learner.profile(True)
-    stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0}
+    stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0,
+             "committed_hist_rows": 0, "committed_part_rows": 0}
     t0 = time.perf_counter()
     for i in range(steps):
         step(True)
@@ -117,6 +118,38 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
         out_trees.append(d)
     learner.close()
     return elapsed, stats, kstats, out_trees
+
+
+def device_copy_gbs(mib=1024, reps=10):
+    """Bandwidth of a plain device-to-device copy (torch), the practical HBM ceiling."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * mib * 1024 * 1024 / (ms * 1e-3) / 1e9
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def cpu_baseline(ds, y, params, budget_s=25.0):
@@ -156,6 +189,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md).", file=sys.stderr)
@@ -200,13 +234,17 @@ def main():
                                "rounding, quant_train_renew_leaf", "rows": args.rows, "features": 28,
                    "num_leaves": 255, "max_bin": 255, "form": "quantized", "growth": "batched"},
     }
-    # dominant kernel by time and its roofline
+    # dominant kernel by time and its roofline: algorithmic bytes per launch over
+    # the average launch duration (HIP events on the learner's stream)
     dom = max(kstats, key=lambda k: kstats[k]["ms"])
     ks = kstats[dom]
+    launches = max(ks["launches"], 1)
     achieved = ks["algo_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                          "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / max(ks["launches"], 1),
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(dom), "kernel": dom,
+                          "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / launches,
+                          "algo_bytes_per_launch": ks["algo_bytes"] / launches,
+                          "moved_bytes_per_launch": ks["moved_bytes"] / launches,
                           "moved_GBs": ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0}
     total_ms = sum(v["ms"] for v in kstats.values())
     result["kernels"] = {k: {"ms_per_iter": v["ms"] / args.steps, "launches_per_iter": v["launches"] / args.steps,
@@ -214,10 +252,20 @@ def main():
                          for k, v in kstats.items()}
     result["kernel_ms_share"] = total_ms / (1000.0 * elapsed)
     result["rows_per_iter"] = {k: v / args.steps for k, v in stats.items()}
+    # whole-iteration algorithmic bytes of the reference's work (committed splits only):
+    # hist = sum R_b (F + P + I) with I = 0 for the root; part = sum Q_s * 9;
+    # discretizer = N (8 + 16 + 2); score = N (4 + 16)
+    f_bytes, p_bytes, n = ds.num_features, 2, args.rows
+    hist_b = stats["committed_hist_rows"] * (f_bytes + p_bytes + 4) - args.steps * n * 4
+    part_b = stats["committed_part_rows"] * 9
+    iter_bytes = (hist_b + part_b) / args.steps + n * (26 + 20)
+    result["iteration_algo_bytes"] = iter_bytes
+    result["iteration_algo_GBs"] = iter_bytes / (elapsed / args.steps) / 1e9
+    result["device_copy_GBs"] = device_copy_gbs(args.copy_mib)
 
-    hx, hy = synth.binary_28(args.holdout, seed=seed, intercept=intercept, stream=3)
-    q_auc = auc(hy, predict_raw(qtrees, hx))
-    result["holdout_auc"] = q_auc
+    if args.holdout > 0:
+        hx, hy = synth.binary_28(args.holdout, seed=seed, intercept=intercept, stream=3)
+        result["holdout_auc"] = auc(hy, predict_raw(qtrees, hx))
     if args.sequential_steps > 0:
         sparams = dataclasses.replace(qparams, batched=0)
         sel, _, _, _ = run_gpu(ds, y, sparams, args.sequential_steps, 1, lr, profile=False)
@@ -232,8 +280,10 @@ def main():
         result["speedup_vs_plain_f64"] = its / p_its
         # AUC at equal tree counts
         k = len(ptrees)
-        result["holdout_auc_at_equal_trees"] = {"trees": k, "quantized": auc(hy, predict_raw(qtrees[:k], hx)),
-                                                "plain_f64": auc(hy, predict_raw(ptrees, hx))}
+        if args.holdout > 0:
+            result["holdout_auc_at_equal_trees"] = {"trees": k,
+                                                    "quantized": auc(hy, predict_raw(qtrees[:k], hx)),
+                                                    "plain_f64": auc(hy, predict_raw(ptrees, hx))}
     if not args.no_cpu_baseline:
         cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                    stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)

@@ -177,8 +177,12 @@ int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessi
 int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
                                        double sigmoid, float* d_gradients, float* d_hessians);
 
-/* Per-tree counters of the last train call: rows read by histogram builds
- * (sum of R_b), rows moved by partitions (sum of Q_s), splits made, rounds. */
+/* Per-tree counters of the last train call. hist_rows / part_rows count every
+ * row the kernels processed (the batched form includes speculative expansions
+ * that were never committed); committed_hist_rows / committed_part_rows count
+ * the reference algorithm's own work: sum of R_b over histogram builds (root N
+ * plus each searched smaller child) and sum of Q_s over committed splits.
+ * rounds = search launches; speculative_waste = expansions not committed. */
 typedef struct bk_train_stats {
   int64_t hist_rows;
   int64_t part_rows;
@@ -186,6 +190,8 @@ typedef struct bk_train_stats {
   int32_t rounds;
   int32_t speculative_waste;
   int32_t reserved;
+  int64_t committed_hist_rows;
+  int64_t committed_part_rows;
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
