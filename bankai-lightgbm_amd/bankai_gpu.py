@@ -91,7 +91,7 @@ class KernelStatsC(C.Structure):
     _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("algo_bytes", C.c_double), ("moved_bytes", C.c_double)]
 
 
-KERNEL_KINDS = ("hist", "partition_count", "partition_scatter", "search", "discretize", "renew", "score_gradients")
+KERNEL_KINDS = ("root_hist", "child_hist", "partition", "search", "discretize", "renew", "score_gradients")
 
 
 class DatasetParamsC(C.Structure):

@@ -17,13 +17,14 @@ namespace bk {
 enum Form { FORM_F64 = 0, FORM_F32C = 1, FORM_QUANT = 2 };
 
 constexpr int kMaxBins = 256;         // group bins per column (u8 storage)
-constexpr int kStreamThreads = 512;   // threads of the row-streaming kernels
-constexpr int kRowsPerThread = 4;     // rows per thread per tile (one dword of u8 bins)
-constexpr int kTileRows = kStreamThreads * kRowsPerThread;
+constexpr int kRowThreads = 256;      // threads of the row-streaming kernels (histogram, partition)
+constexpr int kRowTile = 1024;        // rows per tile of those kernels: 4 per thread
+constexpr int kTileRows = 2048;       // granularity of the row blocks (tasks) the host cuts
 constexpr int kMaxRowBlocks = 512;    // row blocks per histogram build (partials)
 constexpr int kLdsBytes = 120 * 1024; // dynamic LDS budget per workgroup on gfx950 (160 KB per CU)
+constexpr int kHistLdsBytes = 72 * 1024;  // LDS histogram budget: two row blocks fit on one CU
+constexpr int kMaxGroups = 1024;      // groups per row record
 constexpr int kScanThreads = 256;    // threads of the split-search blocks
-constexpr int kTileGroups = 4;       // columns per group tile of the partition / histogram blocks
 
 // Per inner feature, as the kernels read it.
 struct FeatDev {
@@ -155,11 +156,17 @@ struct ScanParams {
 };
 
 // Device buffers of the learner that kernels read.
+// Rows are stored as records of `stride` bytes (a multiple of 16): the bin of
+// group g at byte g, and in buffers 1 and 2 the row payload at byte pay_off.
+// Buffer 0 holds the rows in their original order; its payload lives in gh0
+// (rewritten every tree). Buffers 1 and 2 are the partition ping-pong buffers,
+// with the original row id of every position in idx[1] / idx[2].
 struct DevState {
   int num_data, npad, num_groups, num_features, total_bins, form;
-  const uint8_t* cols[3];        // 0 = original order, 1/2 = partition buffers (group g at g*npad)
+  int stride, pay_off;
+  uint8_t* rec[3];
   int32_t* idx[3];               // idx[0] unused (identity)
-  void* gh[3];                   // row payload: int16 packed (quantized) or float2 (g, h)
+  void* gh0;                     // payload of buffer 0: int16 packed (quantized) or float2 (g, h)
   const FeatDev* feats;
   const int* group_bin_start;    // first histogram bin of each group
   double* pool;                  // slots x total_bins x 2 (double g,h or int64 g,h)
@@ -168,7 +175,6 @@ struct DevState {
   double* part_g;                // float forms: gradient, hessian and count planes
   double* part_h;
   unsigned* part_c;
-  int* blk_cnt;                  // left rows of every partition task
   int* err_flag;
 };
 
@@ -189,11 +195,11 @@ struct HistItem {
 };
 
 // One leaf split: its rows move from seg (buffer seg.buf) to dst_buf, left rows
-// first; the child on hist_side (0 left, 1 right, -1 none) gets a histogram.
+// first, in tiles of `chunk` rows (tasks first_task .. first_task + ntasks - 1).
 struct PartItem {
   Segment seg;
-  int dst_buf, split_group, left_total, hist_side;
-  int first_task, ntasks, chunk, pad;
+  int dst_buf, split_group, left_total, pad0;
+  int first_task, ntasks, chunk, pad1;
   RuleDev rule;
 };
 
@@ -205,15 +211,22 @@ struct ScanItem {
 };
 
 // Launchers (kernels.hip). All run on `stream` (a hipStream_t passed as void*).
+// Records of buffer 0 from column-major bins (group g at cols + g * col_stride).
+void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream);
+size_t hist_lds_bytes(int form, int max_tile_bins);
 // Tasks `d_tasks[0..ntasks)` write partial histograms task_base .. task_base+ntasks-1.
 void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
                  const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
                  void* stream);
-void launch_partition_count(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
-                            void* stream);
-void launch_partition_scatter(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
-                              bool any_hist, const int* d_group_used, int max_tile_bins, int max_tile_groups,
-                              const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+// Rows per partition tile for a record stride (2048 16-byte chunks per tile).
+int partition_tile_rows(int stride);
+// Single-pass stable partition of every item: tasks are the items' tiles in
+// order (chunk == partition_tile_rows). d_status holds one word per task,
+// d_ticket a 64-bit counter of tiles taken (ticket_base = its value before this
+// launch), epoch a nonzero tag below 2^30 that differs from the previous launch.
+void launch_partition(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                      unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
+                      unsigned epoch, void* stream);
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
                  SplitOut* d_out, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);

@@ -69,7 +69,8 @@ __device__ __forceinline__ int8_t sat_i8_dev(double x) {
   return (int8_t)(int)x;
 }
 
-// Block-wide exclusive scan of two ints over kStreamThreads threads.
+// Block-wide exclusive scan of two ints over NT threads.
+template <int NT>
 __device__ __forceinline__ void block_scan2(int a, int b, int* pa, int* pb, int* ta, int* tb, int* scratch) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int xa = a, xb = b;
@@ -87,7 +88,8 @@ __device__ __forceinline__ void block_scan2(int a, int b, int* pa, int* pb, int*
   }
   __syncthreads();
   int oa = 0, ob = 0, sa = 0, sb = 0;
-  const int nw = kStreamThreads / 64;
+  constexpr int nw = NT / 64;
+#pragma unroll
   for (int w = 0; w < nw; ++w) {
     if (w < wave) {
       oa += scratch[w];
@@ -101,6 +103,19 @@ __device__ __forceinline__ void block_scan2(int a, int b, int* pa, int* pb, int*
   *ta = sa;
   *tb = sb;
   __syncthreads();
+}
+
+// Block-wide sum over NT threads (result in every thread).
+template <int NT>
+__device__ __forceinline__ int block_sum(int v, int* scratch) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += scratch[w];
+  __syncthreads();
+  return t;
 }
 
 // ---------------------------------------------------------------------------
@@ -208,49 +223,94 @@ struct LdsHist<FORM_F32C> {
   }
 };
 
-// Adds row k (0..3) of a 4-row dword group to the accumulator.
+// ---------------------------------------------------------------------------
+// Row payload: packed int16 (quantized) or float2 (g, h). Buffer 0 keeps it in
+// gh0; buffers 1 and 2 carry it inside the record at pay_off.
+// ---------------------------------------------------------------------------
 template <int FORM>
-struct RowPayload;
-template <>
-struct RowPayload<FORM_QUANT> {
-  unsigned long long v[4];
-  __device__ __forceinline__ void load(const void* gh, long long d) {
-    uint64_t w = ((const uint64_t*)gh)[d];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = qpack((int16_t)(w >> (16 * k)));
-  }
-  __device__ __forceinline__ void add(LdsHist<FORM_QUANT>& L, int k, int b) const { L.add(b, v[k]); }
+struct Pay {
+  using T = float2;
 };
-template <int FORM>
-struct RowPayload {
-  float g[4], h[4];
-  __device__ __forceinline__ void load(const void* gh, long long d) {
-    const float4* p = (const float4*)gh;
-    float4 a = p[2 * d], b = p[2 * d + 1];
-    g[0] = a.x;
-    h[0] = a.y;
-    g[1] = a.z;
-    h[1] = a.w;
-    g[2] = b.x;
-    h[2] = b.y;
-    g[3] = b.z;
-    h[3] = b.w;
-  }
-  __device__ __forceinline__ void add(LdsHist<FORM>& L, int k, int b) const { L.add(b, g[k], h[k]); }
+template <>
+struct Pay<FORM_QUANT> {
+  using T = int16_t;
 };
 
-// ---------------------------------------------------------------------------
-// Histogram of whole segments (the root, or a leaf whose sibling cannot be
-// subtracted). Grid: (tasks, group tiles); a task is one row block of one item,
-// and writes partial histogram `task_base + blockIdx.x`.
-// ---------------------------------------------------------------------------
 template <int FORM>
-__global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, const HistItem* __restrict__ items,
-                                                         const Task* __restrict__ tasks,
-                                                         const int* __restrict__ group_used,
-                                                         const int* __restrict__ tile_first, int rows_per_flush,
-                                                         int task_base) {
-  extern __shared__ unsigned char smem[];
+__device__ __forceinline__ typename Pay<FORM>::T load_payload(const DevState& s, int buf, long long pos) {
+  using T = typename Pay<FORM>::T;
+  if (buf == 0) return ((const T*)s.gh0)[pos];
+  return *(const T*)(s.rec[buf] + (size_t)pos * s.stride + s.pay_off);
+}
+
+__device__ __forceinline__ uint32_t lane_word(const uint4& v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void set_word(uint4& v, int i, uint32_t w) {
+  if (i == 0) v.x = w;
+  else if (i == 1) v.y = w;
+  else if (i == 2) v.z = w;
+  else v.w = w;
+}
+
+// writes the payload into a 16-byte record chunk; `off` is its byte offset in the chunk
+__device__ __forceinline__ void patch_payload(uint4& v, int off, int16_t p) {
+  const int i = off >> 2, sh = (off & 3) * 8;
+  set_word(v, i, (lane_word(v, i) & ~(0xffffu << sh)) | ((uint32_t)(uint16_t)p << sh));
+}
+__device__ __forceinline__ void patch_payload(uint4& v, int off, float2 p) {
+  const int i = off >> 2;
+  set_word(v, i, __float_as_uint(p.x));
+  set_word(v, i + 1, __float_as_uint(p.y));
+}
+
+__device__ __forceinline__ void hist_add(LdsHist<FORM_QUANT>& L, int b, int16_t p) { L.add(b, qpack(p)); }
+template <int FORM>
+__device__ __forceinline__ void hist_add(LdsHist<FORM>& L, int b, float2 p) {
+  L.add(b, p.x, p.y);
+}
+
+template <int FORM>
+__device__ __forceinline__ int hist_bytes_of(int nb) {
+  return (nb * LdsHist<FORM>::kBytesPerBin + 15) & ~15;
+}
+
+// group tile [g0, g1) -> per-group first bin relative to the tile, -1 if skipped
+__device__ __forceinline__ void init_gbs(const DevState& s, int* gbs, int g0, int g1, int bin_lo,
+                                         const int* __restrict__ group_used) {
+  for (int g = threadIdx.x; g < s.num_groups; g += blockDim.x)
+    gbs[g] = (g >= g0 && g < g1 && group_used[g]) ? s.group_bin_start[g] - bin_lo : -1;
+}
+
+// Adds one 16-byte chunk of a row with the chunk's group bin offsets in
+// registers (off[j] < 0: group outside the tile, unused, or padding).
+template <int FORM>
+__device__ __forceinline__ void hist_chunk_regs(LdsHist<FORM>& L, const uint4& v, const int (&off)[16],
+                                                typename Pay<FORM>::T p) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (off[j] >= 0) hist_add(L, off[j] + (int)((lane_word(v, j >> 2) >> (8 * (j & 3))) & 0xffu), p);
+}
+
+// ---------------------------------------------------------------------------
+// Histogram of whole segments: the root, the smaller child of every split (its
+// rows are contiguous in the partition buffer), or a leaf whose sibling cannot
+// be subtracted. Grid: (tasks, group tiles); a task is one row block of one
+// item, and writes partial histogram `task_base + blockIdx.x`. Thread i always
+// reads chunk i % nc of the tile's chunk range, so its 16 bin offsets stay in
+// registers; consecutive threads read consecutive 16-byte chunks.
+// ---------------------------------------------------------------------------
+constexpr int kHistThreads = 512;
+constexpr int kHistRows = 2048;  // rows per payload staging tile
+
+template <int FORM>
+__global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistItem* __restrict__ items,
+                                                       const Task* __restrict__ tasks,
+                                                       const int* __restrict__ group_used,
+                                                       const int* __restrict__ tile_first, int rows_per_flush,
+                                                       int task_base) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using PT = typename Pay<FORM>::T;
   const Task task = tasks[blockIdx.x];
   const HistItem it = items[task.item];
   const Segment seg = it.seg;
@@ -259,245 +319,245 @@ __global__ void __launch_bounds__(kStreamThreads) k_hist(DevState s, const HistI
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
   LdsHist<FORM> L;
   L.bind(smem, nb);
+  int* gbs = (int*)(smem + hist_bytes_of<FORM>(nb));
+  PT* pay = (PT*)(gbs + ((s.num_groups + 3) & ~3));
+  init_gbs(s, gbs, g0, g1, bin_lo, group_used);
+  L.zero(nb);
   const long long seg_end = (long long)seg.begin + seg.count;
   const long long cb = (long long)seg.begin + (long long)task.blk * it.chunk;
   const long long ce = min(cb + (long long)it.chunk, seg_end);
-  const uint8_t* cols = s.cols[seg.buf];
-  const void* gh = s.gh[seg.buf];
+  const int c0 = g0 >> 4, nc = ((g1 - 1) >> 4) + 1 - c0;
+  const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
+  const int rstep = tc / nc;
+  const int my_c = c0 + (int)threadIdx.x % nc;
+  const uint8_t* src = s.rec[seg.buf] + 16 * my_c;
+  __syncthreads();
+  int off[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int g = 16 * my_c + j;
+    off[j] = g < s.num_groups ? gbs[g] : -1;
+  }
   bool first = true;
-  for (long long fs = cb; fs < ce || first; fs += rows_per_flush) {
-    const long long fe = min(fs + (long long)rows_per_flush, ce);
-    L.zero(nb);
+  long long since = 0;
+  for (long long t0 = cb; t0 < ce; t0 += kHistRows) {
+    const int rows = (int)min((long long)kHistRows, ce - t0);
+    for (int r = threadIdx.x; r < rows; r += kHistThreads) pay[r] = load_payload<FORM>(s, seg.buf, t0 + r);
     __syncthreads();
-    const long long d0 = fs >> 2, d1 = (fe + 3) >> 2;
-    for (long long d = d0 + threadIdx.x; d < d1; d += blockDim.x) {
-      const long long r0 = d << 2;
-      int valid = 0;
+    if ((int)threadIdx.x < tc) {
+      const uint8_t* base = src + (size_t)t0 * s.stride;
+      for (int r0 = (int)threadIdx.x / nc; r0 < rows; r0 += 4 * rstep) {
+        uint4 v[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) valid |= (r0 + k >= fs && r0 + k < fe) ? (1 << k) : 0;
-      RowPayload<FORM> P;
-      P.load(gh, d);
-      for (int g = g0; g < g1; ++g) {
-        if (!group_used[g]) continue;
-        const uint32_t w = ((const uint32_t*)(cols + (size_t)g * s.npad))[d];
-        const int base = s.group_bin_start[g] - bin_lo;
+        for (int u = 0; u < 4; ++u) {
+          const int r = r0 + u * rstep;
+          if (r < rows) v[u] = *(const uint4*)(base + (size_t)r * s.stride);
+        }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (valid & (1 << k)) P.add(L, k, base + (int)((w >> (8 * k)) & 0xffu));
-      }
-    }
-    __syncthreads();
-    L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
-    __syncthreads();
-    first = false;
-    if (fe >= ce) break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Stable partition, pass 1: left rows of every task (row block of an item).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kStreamThreads) k_part_count(DevState s, const PartItem* __restrict__ items,
-                                                               const Task* __restrict__ tasks) {
-  __shared__ int red[kStreamThreads / 64];
-  const Task task = tasks[blockIdx.x];
-  const PartItem& it = items[task.item];
-  const Segment seg = it.seg;
-  const RuleDev rule = it.rule;
-  const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)task.blk * it.chunk;
-  const long long ce = min(cb + (long long)it.chunk, seg_end);
-  const uint32_t* col = (const uint32_t*)(s.cols[seg.buf] + (size_t)it.split_group * s.npad);
-  int cnt = 0;
-  for (long long d = (cb >> 2) + threadIdx.x; d < ((ce + 3) >> 2); d += blockDim.x) {
-    const uint32_t w = col[d];
-    const long long r0 = d << 2;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (r0 + k >= cb && r0 + k < ce && route_left(rule, (w >> (8 * k)) & 0xffu)) ++cnt;
-  }
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < kStreamThreads / 64; ++w) t += red[w];
-    s.blk_cnt[blockIdx.x] = t;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Stable partition, pass 2: scatter every column, the row ids and the payload
-// into the destination buffer, and accumulate the histogram of the child on
-// `hist_side` (0 left, 1 right, -1 none) from the rows already in registers.
-// Grid: (row blocks, group tiles); tile 0 also moves row ids and payload.
-// Each tile of rows is permuted in LDS first (left rows, then right rows, both
-// in row order), so the writes are two contiguous runs per column.
-// ---------------------------------------------------------------------------
-template <int FORM>
-__global__ void __launch_bounds__(kStreamThreads) k_part_scatter(DevState s, const PartItem* __restrict__ items,
-                                                                 const Task* __restrict__ tasks,
-                                                                 const int* __restrict__ group_used,
-                                                                 const int* __restrict__ tile_first,
-                                                                 int rows_per_flush) {
-  extern __shared__ unsigned char smem[];
-  __shared__ int scratch[32];
-  __shared__ int red[kStreamThreads / 64];
-  constexpr int kPay = FORM == FORM_QUANT ? 2 : 8;
-  const Task task = tasks[blockIdx.x];
-  const PartItem& it = items[task.item];
-  const Segment seg = it.seg;
-  const RuleDev rule = it.rule;
-  const int dst_buf = it.dst_buf, split_group = it.split_group, left_total = it.left_total;
-  const int hist_side = it.hist_side;
-  const int blk = task.blk, tile = blockIdx.y;
-  const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
-  const int ng = g1 - g0;
-  const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
-  const bool do_hist = hist_side >= 0;
-  // LDS: [histogram (nb bins)] [staged bytes: ng x kTileRows] [row ids] [payload]
-  LdsHist<FORM> L;
-  L.bind(smem, nb);
-  const size_t hist_bytes = do_hist ? (((size_t)nb * LdsHist<FORM>::kBytesPerBin + 15) & ~(size_t)15) : 0;
-  uint8_t* stage = smem + hist_bytes;
-  int32_t* stage_idx = (int32_t*)(stage + (size_t)ng * kTileRows);
-  unsigned char* stage_pay = (unsigned char*)(stage_idx + kTileRows);
-  // left rows of the row blocks of this item before this one
-  int lb = 0;
-  for (int i = threadIdx.x; i < blk; i += blockDim.x) lb += s.blk_cnt[it.first_task + i];
-  for (int o = 32; o > 0; o >>= 1) lb += __shfl_down(lb, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lb;
-  __syncthreads();
-  int left_before = 0;
-  for (int w = 0; w < kStreamThreads / 64; ++w) left_before += red[w];
-  __syncthreads();
-
-  const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)blk * it.chunk;
-  const long long ce = min(cb + (long long)it.chunk, seg_end);
-  long long lpos = (long long)seg.begin + left_before;
-  long long rpos = (long long)seg.begin + left_total + ((cb - seg.begin) - left_before);
-  const uint8_t* scols = s.cols[seg.buf];
-  uint8_t* dcols = (uint8_t*)s.cols[dst_buf];
-  const uint32_t* scol = (const uint32_t*)(scols + (size_t)split_group * s.npad);
-  if (do_hist) L.zero(nb);
-  __syncthreads();
-  bool first = true;
-  long long since_flush = 0;
-  int my_left_total = 0;
-  for (long long t0 = (cb >> 2); t0 < ((ce + 3) >> 2); t0 += blockDim.x) {
-    const long long d = t0 + threadIdx.x;
-    const long long r0 = d << 2;
-    int valid = 0, lmask = 0;
-    if (d < ((ce + 3) >> 2)) {
-      const uint32_t sw = scol[d];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (r0 + k >= cb && r0 + k < ce) {
-          valid |= 1 << k;
-          if (route_left(rule, (sw >> (8 * k)) & 0xffu)) lmask |= 1 << k;
+        for (int u = 0; u < 4; ++u) {
+          const int r = r0 + u * rstep;
+          if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, pay[r]);
         }
       }
     }
-    const int rmask = valid & ~lmask;
-    const int nl = __popc(lmask), nr = __popc(rmask);
-    int pl, pr, tl, tr;
-    block_scan2(nl, nr, &pl, &pr, &tl, &tr, scratch);
-    // tile-local destinations: left rows at [0, tl), right rows at [tl, tl + tr)
-    int lp[4];
-    {
-      int il = 0, ir = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (lmask & (1 << k)) lp[k] = pl + il++;
-        else if (rmask & (1 << k)) lp[k] = tl + pr + ir++;
-        else lp[k] = -1;
-      }
-    }
-    if (valid) {
-      for (int gi = 0; gi < ng; ++gi) {
-        const uint32_t w = ((const uint32_t*)(scols + (size_t)(g0 + gi) * s.npad))[d];
-        uint8_t* st = stage + (size_t)gi * kTileRows;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (valid & (1 << k)) st[lp[k]] = (uint8_t)((w >> (8 * k)) & 0xffu);
-      }
-      if (tile == 0) {
-        if (seg.buf == 0) {
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) stage_idx[lp[k]] = (int32_t)(r0 + k);
-        } else {
-          const int32_t* sidx = s.idx[seg.buf];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) stage_idx[lp[k]] = sidx[r0 + k];
-        }
-        if (FORM == FORM_QUANT) {
-          const int16_t* sp = (const int16_t*)s.gh[seg.buf];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) ((int16_t*)stage_pay)[lp[k]] = sp[r0 + k];
-        } else {
-          const float2* sp = (const float2*)s.gh[seg.buf];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (valid & (1 << k)) ((float2*)stage_pay)[lp[k]] = sp[r0 + k];
-        }
-      }
-      if (do_hist) {
-        const int hmask = hist_side == 0 ? lmask : rmask;
-        if (hmask) {
-          RowPayload<FORM> P;
-          P.load(s.gh[seg.buf], d);
-          for (int g = g0; g < g1; ++g) {
-            if (!group_used[g]) continue;
-            const uint32_t w = ((const uint32_t*)(scols + (size_t)g * s.npad))[d];
-            const int base = s.group_bin_start[g] - bin_lo;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-              if (hmask & (1 << k)) P.add(L, k, base + (int)((w >> (8 * k)) & 0xffu));
-          }
-        }
-      }
-    }
+    since += rows;
     __syncthreads();
-    // write the two runs; consecutive threads write consecutive destinations
-    const int tn = tl + tr;
-    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-      const long long dst = i < tl ? lpos + i : rpos + (i - tl);
-      if (dst < (long long)seg.begin || dst >= seg_end) {
-        atomicOr(s.err_flag, 2);
-        continue;
-      }
-      for (int gi = 0; gi < ng; ++gi) dcols[(size_t)(g0 + gi) * s.npad + dst] = stage[(size_t)gi * kTileRows + i];
-      if (tile == 0) {
-        s.idx[dst_buf][dst] = stage_idx[i];
-        if (FORM == FORM_QUANT) ((int16_t*)s.gh[dst_buf])[dst] = ((const int16_t*)stage_pay)[i];
-        else ((float2*)s.gh[dst_buf])[dst] = ((const float2*)stage_pay)[i];
-      }
-    }
-    lpos += tl;
-    rpos += tr;
-    my_left_total += tl;
-    since_flush += (long long)blockDim.x * 4;
-    if (do_hist && since_flush >= rows_per_flush) {
-      __syncthreads();
-      L.flush(s, blockIdx.x, bin_lo, nb, first);
+    if (since >= rows_per_flush) {
+      L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
       __syncthreads();
       L.zero(nb);
+      __syncthreads();
       first = false;
-      since_flush = 0;
+      since = 0;
     }
-    __syncthreads();
   }
-  if (do_hist) {
-    __syncthreads();
-    L.flush(s, blockIdx.x, bin_lo, nb, first);
+  if (since > 0 || first) L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass stable partition (data_partition.rs:124-231): left rows first,
+// then right rows, both in row order. Every item is cut into tiles of
+// part_tile_rows(stride) rows; blocks take tiles in ticket order (an atomic
+// counter), so a tile only waits on tiles that are already running. Per tile:
+// the records are loaded as 16-byte chunks into registers (consecutive lanes,
+// consecutive chunks), the lanes holding the split group's byte route their
+// rows, a block scan ranks the rows, and the tile's left count is published
+// for the tiles after it while wave 0 looks back over the tiles before it
+// (decoupled look-back) for its left offset. Then the chunks and row ids are
+// written to their destinations: two contiguous runs per tile.
+// status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
+// ---------------------------------------------------------------------------
+constexpr int kPartThreads = 256;
+constexpr int kPartChunks = 8;  // 16-byte chunks per thread per tile
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ unsigned long long status_word(unsigned flag, unsigned epoch, unsigned value) {
+  return ((unsigned long long)flag << 62) | ((unsigned long long)(epoch & 0x3fffffffu) << 32) |
+         (unsigned long long)value;
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
+                                                            const Task* __restrict__ tasks, int ntasks,
+                                                            unsigned long long* status, unsigned long long* ticket,
+                                                            unsigned long long ticket_base, unsigned epoch) {
+  __shared__ int sh_tile, sh_left_before;
+  __shared__ int scratch[32];
+  extern __shared__ __align__(16) unsigned char smem[];
+  using PT = typename Pay<FORM>::T;
+  const int stride = s.stride, nc = stride >> 4;
+  const int R = kPartThreads * kPartChunks / nc;  // rows per tile
+  const int rpt = (R + kPartThreads - 1) / kPartThreads;
+  uint32_t* dstp = (uint32_t*)smem;
+  uint8_t* lflag = (uint8_t*)(dstp + R);
+  if (threadIdx.x == 0) sh_tile = (int)(atomicAdd(ticket, 1ull) - ticket_base);
+  __syncthreads();
+  const int t = sh_tile;
+  if (t < 0 || t >= ntasks) return;  // one ticket per block: never taken
+  const Task task = tasks[t];
+  const PartItem& it = items[task.item];
+  const Segment seg = it.seg;
+  const RuleDev rule = it.rule;
+  const int left_total = it.left_total;
+  const long long seg_end = (long long)seg.begin + seg.count;
+  const long long t0 = (long long)seg.begin + (long long)task.blk * it.chunk;
+  const int rows = (int)min((long long)R, seg_end - t0);
+  const uint8_t* src = s.rec[seg.buf] + (size_t)t0 * stride;
+  uint8_t* dst = s.rec[it.dst_buf];
+  const int total = rows * nc;
+  const int cs = it.split_group >> 4, bs = it.split_group & 15;
+  uint4 v[kPartChunks];
+#pragma unroll
+  for (int k = 0; k < kPartChunks; ++k) {
+    const int q = (int)threadIdx.x + k * kPartThreads;
+    if (q < total) v[k] = *(const uint4*)(src + 16 * (size_t)q);
   }
-  if (threadIdx.x == 0 && tile == 0 && blk == it.ntasks - 1 && left_before + my_left_total != left_total)
-    atomicOr(s.err_flag, 1);
-  (void)kPay;
+#pragma unroll
+  for (int k = 0; k < kPartChunks; ++k) {
+    const int q = (int)threadIdx.x + k * kPartThreads;
+    if (q < total) {
+      const int r = q / nc;
+      if (q - r * nc == cs) lflag[r] = route_left(rule, (lane_word(v[k], bs >> 2) >> (8 * (bs & 3))) & 0xffu) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  int vm = 0, lm = 0;
+  for (int i = 0; i < rpt; ++i) {
+    const int r = (int)threadIdx.x * rpt + i;
+    if (r < rows) {
+      vm |= 1 << i;
+      if (lflag[r]) lm |= 1 << i;
+    }
+  }
+  int pl, pr, tl, tr;
+  block_scan2<kPartThreads>(__popc(lm), __popc(vm & ~lm), &pl, &pr, &tl, &tr, scratch);
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    long long excl = 0;
+    if (task.blk == 0) {
+      if (lane == 0)
+        __hip_atomic_store(&status[t], status_word(2, epoch, (unsigned)tl), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0)
+        __hip_atomic_store(&status[t], status_word(1, epoch, (unsigned)tl), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      int j = t - 1, spins = 0;
+      for (;;) {
+        const int idx = j - lane;
+        unsigned flag = 2u, val = 0u;  // before the item's first tile: nothing to add
+        if (idx >= it.first_task) {
+          const unsigned long long w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          flag = ((unsigned)(w >> 32) & 0x3fffffffu) == (epoch & 0x3fffffffu) ? (unsigned)(w >> 62) : 0u;
+          val = (unsigned)w;
+        }
+        if (__any(flag == 0u)) {
+          if (++spins > kSpinLimit) {  // a predecessor never published: report, do not hang
+            if (lane == 0) atomicOr(s.err_flag, 4);
+            excl = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const unsigned long long inc = __ballot(flag == 2u);
+        if (inc) {
+          const int k = __ffsll((long long)inc) - 1;
+          excl += wave_sum64(lane <= k ? (long long)val : 0ll);
+          break;
+        }
+        excl += wave_sum64((long long)val);
+        j -= 64;
+      }
+      if (lane == 0)
+        __hip_atomic_store(&status[t], status_word(2, epoch, (unsigned)(excl + tl)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) sh_left_before = (int)excl;
+  }
+  __syncthreads();
+  const int left_before = sh_left_before;
+  constexpr uint32_t kSkip = 0xffffffffu;
+  {
+    long long lpos = (long long)seg.begin + left_before + pl;
+    long long rpos = (long long)seg.begin + left_total + ((t0 - seg.begin) - left_before) + pr;
+    for (int i = 0; i < rpt; ++i) {
+      const int r = (int)threadIdx.x * rpt + i;
+      if (!((vm >> i) & 1)) continue;
+      const long long d = ((lm >> i) & 1) ? lpos++ : rpos++;
+      uint32_t u = (uint32_t)d;
+      if (d < (long long)seg.begin || d >= seg_end) {
+        atomicOr(s.err_flag, 2);
+        u = kSkip;
+      }
+      dstp[r] = u;
+    }
+  }
+  __syncthreads();
+  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
+#pragma unroll
+  for (int k = 0; k < kPartChunks; ++k) {
+    const int q = (int)threadIdx.x + k * kPartThreads;
+    if (q < total) {
+      const int r = q / nc, c = q - r * nc;
+      const uint32_t d = dstp[r];
+      if (d == kSkip) continue;
+      if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
+      *(uint4*)(dst + (size_t)d * stride + 16 * c) = v[k];
+    }
+  }
+  int32_t* didx = s.idx[it.dst_buf];
+  const int32_t* sidx = s.idx[seg.buf];
+  for (int r = threadIdx.x; r < rows; r += kPartThreads) {
+    const uint32_t d = dstp[r];
+    if (d != kSkip) didx[d] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
+  }
+  if (threadIdx.x == 0 && task.blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
+}
+
+// Records of buffer 0 from column-major bins: a block stages 64 rows of every
+// group in LDS (coalesced column reads), then writes whole records.
+constexpr int kRecRows = 64;
+__global__ void __launch_bounds__(256) k_to_records(DevState s, const uint8_t* __restrict__ cols, size_t col_stride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const long long row0 = (long long)blockIdx.x * kRecRows;
+  const int rows = (int)min((long long)kRecRows, (long long)s.num_data - row0);
+  const int stride = s.stride;
+  for (int i = threadIdx.x; i < kRecRows * stride / 4; i += blockDim.x) ((uint32_t*)smem)[i] = 0u;
+  __syncthreads();
+  for (int i = threadIdx.x; i < s.num_groups * kRecRows; i += blockDim.x) {
+    const int g = i / kRecRows, r = i - g * kRecRows;
+    if (r < rows) smem[r * stride + g] = cols[(size_t)g * col_stride + row0 + r];
+  }
+  __syncthreads();
+  const int nchunk = stride >> 4;
+  for (int q = threadIdx.x; q < rows * nchunk; q += blockDim.x)
+    *(uint4*)(s.rec[0] + (size_t)row0 * stride + 16 * (size_t)q) = ((const uint4*)smem)[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1462,7 +1522,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
 // ---------------------------------------------------------------------------
 // Gradient discretization (gradient_discretizer/mod.rs:116-201)
 // ---------------------------------------------------------------------------
-__global__ void k_max_abs(const float* __restrict__ g, const float* __restrict__ h, int n, unsigned* out) {
+__global__ void __launch_bounds__(256) k_max_abs(const float* __restrict__ g, const float* __restrict__ h, int n, unsigned* out) {
   unsigned mg = 0u, mh = 0u;
   int bad = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1476,14 +1536,28 @@ __global__ void k_max_abs(const float* __restrict__ g, const float* __restrict__
     mh = max(mh, (unsigned)__shfl_down((int)mh, o, 64));
     bad |= __shfl_down(bad, o, 64);
   }
+  // one atomic per block: same-address atomics from every wave serialize
+  __shared__ unsigned red[3][4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    red[0][w] = mg;
+    red[1][w] = mh;
+    red[2][w] = (unsigned)bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      mg = max(mg, red[0][i]);
+      mh = max(mh, red[1][i]);
+      bad |= (int)red[2][i];
+    }
     atomicMax(&out[0], mg);
     atomicMax(&out[1], mh);
     if (bad) atomicOr(&out[2], 1u);
   }
 }
 
-__global__ void k_discretize(const float* __restrict__ g, const float* __restrict__ h, const double* __restrict__ gn,
+__global__ void __launch_bounds__(256) k_discretize(const float* __restrict__ g, const float* __restrict__ h, const double* __restrict__ gn,
                              const double* __restrict__ hn, int n, int start, double inv_g, double inv_h,
                              int const_hess, int stochastic, int16_t* __restrict__ packed, long long* sums) {
   long long sg = 0, sh = 0, sa = 0;
@@ -1517,7 +1591,20 @@ __global__ void k_discretize(const float* __restrict__ g, const float* __restric
     sh += __shfl_down(sh, o, 64);
     sa += __shfl_down(sa, o, 64);
   }
+  __shared__ long long red[3][4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    red[0][w] = sg;
+    red[1][w] = sh;
+    red[2][w] = sa;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      sg += red[0][i];
+      sh += red[1][i];
+      sa += red[2][i];
+    }
     atomicAdd((unsigned long long*)&sums[0], (unsigned long long)sg);
     atomicAdd((unsigned long long*)&sums[1], (unsigned long long)sh);
     atomicAdd((unsigned long long*)&sums[2], (unsigned long long)sa);
@@ -1805,10 +1892,6 @@ inline int grid_for(long long n, int threads, int cap = 4096) {
   return (int)b;
 }
 
-template <int FORM>
-size_t lds_bytes(int nb) {
-  return (size_t)nb * (FORM == FORM_QUANT ? 8 : 20);
-}
 
 template <typename K>
 void allow_lds(K kernel) {
@@ -1824,66 +1907,69 @@ void allow_lds(K kernel) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+size_t hist_lds_bytes(int form, int max_tile_bins) {
+  const size_t per_bin = form == FORM_QUANT ? 8 : 20;
+  return ((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15;
+}
+
 void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
                  const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
                  void* stream) {
   if (ntasks <= 0) return;
   dim3 grid((unsigned)ntasks, (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
+  const size_t pay = s.form == FORM_QUANT ? 2 : 8;
+  const size_t lds = hist_lds_bytes(s.form, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
+  if (lds > (size_t)kLdsBytes) throw std::runtime_error("histogram tile exceeds the LDS budget");
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
-      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kStreamThreads), lds_bytes<FORM_QUANT>(max_tile_bins), st, s,
-                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+                         d_tile_first, rows_per_flush, task_base);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
-      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kStreamThreads), lds_bytes<FORM_F64>(max_tile_bins), st, s,
-                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+                         d_tile_first, rows_per_flush, task_base);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
-      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kStreamThreads), lds_bytes<FORM_F32C>(max_tile_bins), st, s,
-                         d_items, d_tasks, d_group_used, d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+                         d_tile_first, rows_per_flush, task_base);
       break;
   }
   BK_HIP(hipGetLastError());
 }
 
-void launch_partition_count(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
-                            void* stream) {
-  if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_part_count, dim3((unsigned)ntasks), dim3(kStreamThreads), 0, (hipStream_t)stream, s, d_items,
-                     d_tasks);
-  BK_HIP(hipGetLastError());
-}
+int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
-void launch_partition_scatter(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
-                              bool any_hist, const int* d_group_used, int max_tile_bins, int max_tile_groups,
-                              const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+void launch_partition(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+                      unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
+                      unsigned epoch, void* stream) {
   if (ntasks <= 0) return;
+  const int R = partition_tile_rows(s.stride);
+  if (R < 1) throw std::runtime_error("row record too large for the partition tile");
+  const size_t lds = 4 * (size_t)R + (size_t)R;
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid((unsigned)ntasks, (unsigned)ntiles);
-  const size_t per_bin = s.form == FORM_QUANT ? 8 : 20;
-  size_t lds = any_hist ? (((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15) : 0;
-  lds += (size_t)max_tile_groups * kTileRows + (size_t)kTileRows * (4 + (s.form == FORM_QUANT ? 2 : 8));
   switch (s.form) {
     case FORM_QUANT:
-      allow_lds(k_part_scatter<FORM_QUANT>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_QUANT>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
-                         d_group_used, d_tile_first, rows_per_flush);
-      break;
-    case FORM_F64:
-      allow_lds(k_part_scatter<FORM_F64>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_F64>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
-                         d_group_used, d_tile_first, rows_per_flush);
+      hipLaunchKernelGGL(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
+                         d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
     default:
-      allow_lds(k_part_scatter<FORM_F32C>);
-      hipLaunchKernelGGL(k_part_scatter<FORM_F32C>, grid, dim3(kStreamThreads), lds, st, s, d_items, d_tasks,
-                         d_group_used, d_tile_first, rows_per_flush);
+      hipLaunchKernelGGL(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
+                         d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
   }
+  BK_HIP(hipGetLastError());
+}
+
+void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream) {
+  const size_t lds = (size_t)kRecRows * s.stride;
+  if (lds > (size_t)kLdsBytes) throw std::runtime_error("row record too large");
+  allow_lds(k_to_records);
+  const unsigned blocks = (unsigned)((s.num_data + kRecRows - 1) / kRecRows);
+  hipLaunchKernelGGL(k_to_records, dim3(blocks), dim3(256), lds, (hipStream_t)stream, s, cols, col_stride);
   BK_HIP(hipGetLastError());
 }
 
@@ -1907,14 +1993,15 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
 }
 
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream) {
-  hipLaunchKernelGGL(k_max_abs, dim3(grid_for(n, 256, 2048)), dim3(256), 0, (hipStream_t)stream, g, h, n, d_out4);
+  hipLaunchKernelGGL(k_max_abs, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, n,
+                     d_out4);
   BK_HIP(hipGetLastError());
 }
 
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream) {
-  hipLaunchKernelGGL(k_discretize, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
+  hipLaunchKernelGGL(k_discretize, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
                      hnoise, n, start, inv_g, inv_h, const_hess, stochastic, packed, d_sums3);
   BK_HIP(hipGetLastError());
 }

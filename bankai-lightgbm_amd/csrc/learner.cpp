@@ -356,19 +356,23 @@ struct Learner {
   std::vector<std::vector<double>> upper;
   std::vector<std::vector<int32_t>> bin2cat;
   // group tiles for LDS: small ones for the partition, large ones for segment histograms
-  std::vector<int> tile_first, htile_first;
-  int max_tile_bins = 0, max_tile_groups = 0, max_htile_bins = 0;
+  std::vector<int> tile_first;
+  int max_tile_bins = 0;
+  int stride = 0, pay_off = 0;  // row record layout
   int rows_per_flush = 0;
   size_t cache = 0;       // histogram slots of the reference pool
   size_t num_slots = 0;   // device slots (batched form: one per evaluated leaf)
-  int max_items = 0, max_tasks = 0;
+  int max_items = 0, max_tasks = 0, max_partials = 0, max_part_tiles = 0;
+  int part_tile = 0;                   // rows per partition tile
+  unsigned long long ticket_base = 0;  // partition tiles taken before the next launch
+  unsigned epoch = 0;                  // tag of the last partition launch
 
   // device memory
   hipStream_t stream = nullptr;
   DevState ds{};
-  uint8_t* d_cols[3] = {nullptr, nullptr, nullptr};
+  uint8_t* d_rec[3] = {nullptr, nullptr, nullptr};
   int32_t* d_idx[3] = {nullptr, nullptr, nullptr};
-  void* d_gh[3] = {nullptr, nullptr, nullptr};
+  void* d_gh0 = nullptr;
   float *d_grad = nullptr, *d_hess = nullptr;
   double *d_gnoise = nullptr, *d_hnoise = nullptr;
   FeatDev* d_feats = nullptr;
@@ -379,10 +383,10 @@ struct Learner {
   double* d_part_g = nullptr;
   double* d_part_h = nullptr;
   unsigned* d_part_c = nullptr;
-  int* d_blk_cnt = nullptr;
+  unsigned long long* d_status = nullptr;
+  unsigned long long* d_ticket = nullptr;
   int* d_err = nullptr;
   int* d_tile_first = nullptr;
-  int* d_htile_first = nullptr;
   int* d_used = nullptr;        // per item x feature
   int* d_group_used = nullptr;  // per group
   SplitOut* d_out = nullptr;
@@ -442,6 +446,7 @@ struct Learner {
   // batched form
   std::vector<Node> nodes;
   std::vector<int> leaf_node;
+  std::vector<Segment> hist_segs;
 
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
@@ -465,14 +470,13 @@ struct Learner {
       if (p) (void)hipHostFree(p);
     };
     for (int b = 0; b < 3; ++b) {
-      f(d_cols[b]);
+      f(d_rec[b]);
       f(d_idx[b]);
-      f(d_gh[b]);
     }
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
-                    (void*)d_part_h, (void*)d_part_c, (void*)d_blk_cnt, (void*)d_err, (void*)d_tile_first,
-                    (void*)d_htile_first, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_hitems,
+                    (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
+                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_hitems,
                     (void*)d_pitems, (void*)d_sitems, (void*)d_tasks, (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
@@ -570,33 +574,30 @@ struct Learner {
         }
       }
     }
-    // LDS tiles of whole groups
+    // row records: G bins, the payload at pay_off (aligned to its size), padded to 16 bytes
+    if (G > kMaxGroups) throw std::invalid_argument("more than 1024 feature groups");
+    const int pay_bytes = form == FORM_QUANT ? 2 : 8;
+    pay_off = (G + pay_bytes - 1) / pay_bytes * pay_bytes;
+    stride = (pay_off + pay_bytes + 15) / 16 * 16;
+    // LDS histogram tiles of whole groups
     const int per_bin = form == FORM_QUANT ? 8 : 20;
-    const int tile_cap = kLdsBytes / per_bin;
-    auto make_tiles = [&](int max_groups, std::vector<int>& first, int& max_bins, int& max_groups_out) {
-      first.assign(1, 0);
-      max_bins = 0;
-      max_groups_out = 0;
-      int cur = 0, cur_groups = 0;
+    const int tile_cap = kHistLdsBytes / per_bin;
+    tile_first.assign(1, 0);
+    max_tile_bins = 0;
+    {
+      int cur = 0;
       for (int g = 0; g < G; ++g) {
         int nb = v->group_num_total_bin[g];
-        if (cur + nb > tile_cap || cur_groups == max_groups) {
-          first.push_back(g);
-          max_bins = std::max(max_bins, cur);
-          max_groups_out = std::max(max_groups_out, cur_groups);
+        if (cur + nb > tile_cap) {
+          tile_first.push_back(g);
+          max_tile_bins = std::max(max_tile_bins, cur);
           cur = 0;
-          cur_groups = 0;
         }
         cur += nb;
-        cur_groups += 1;
       }
-      first.push_back(G);
-      max_bins = std::max(max_bins, cur);
-      max_groups_out = std::max(max_groups_out, cur_groups);
-    };
-    int unused_groups = 0;
-    make_tiles(kTileGroups, tile_first, max_tile_bins, max_tile_groups);
-    make_tiles(1 << 30, htile_first, max_htile_bins, unused_groups);
+      tile_first.push_back(G);
+      max_tile_bins = std::max(max_tile_bins, cur);
+    }
     // rows one LDS accumulation may hold: the packed quantized fields stay exact
     if (form == FORM_QUANT) {
       int maxq = bins + 1;
@@ -615,20 +616,24 @@ struct Learner {
     batched = P.batched != 0 && cache == (size_t)P.num_leaves;
     num_slots = batched ? (size_t)P.num_leaves * 2 + 2 : cache;
     max_items = P.num_leaves + 1;
-    max_tasks = (N + 4 * kTileRows - 1) / (4 * kTileRows) + 2 * max_items + kMaxRowBlocks + 8;
+    part_tile = partition_tile_rows(stride);
+    max_part_tiles = N / part_tile + max_items + 8;
+    max_partials = N / (4 * kTileRows) + 2 * max_items + 2 * kMaxRowBlocks + 8;
+    max_tasks = max_part_tiles + max_partials;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming));
     for (int b = 0; b < 3; ++b) {
-      d_cols[b] = dmalloc<uint8_t>((size_t)G * npad);
+      d_rec[b] = dmalloc<uint8_t>((size_t)npad * stride);
       if (b > 0) d_idx[b] = dmalloc<int32_t>((size_t)npad);
-      d_gh[b] = form == FORM_QUANT ? (void*)dmalloc<int16_t>((size_t)npad) : (void*)dmalloc<float>((size_t)npad * 2);
     }
-    BK_HIPC(hipMemsetAsync(d_gh[0], 0, (size_t)npad * (form == FORM_QUANT ? 2 : 8), stream));
-    for (int g = 0; g < G; ++g) {
-      BK_HIPC(hipMemcpyAsync(d_cols[0] + (size_t)g * npad, v->group_bins[g], (size_t)N, hipMemcpyHostToDevice, stream));
-      if (npad > N) BK_HIPC(hipMemsetAsync(d_cols[0] + (size_t)g * npad + N, 0, (size_t)(npad - N), stream));
-    }
+    d_gh0 = form == FORM_QUANT ? (void*)dmalloc<int16_t>((size_t)npad) : (void*)dmalloc<float>((size_t)npad * 2);
+    BK_HIPC(hipMemsetAsync(d_gh0, 0, (size_t)npad * pay_bytes, stream));
+    // the bins arrive column-major; they are transposed into records on the device
+    uint8_t* d_colstage = dmalloc<uint8_t>((size_t)G * npad);
+    std::unique_ptr<void, void (*)(void*)> colstage_guard(d_colstage, [](void* q) { (void)hipFree(q); });
+    for (int g = 0; g < G; ++g)
+      BK_HIPC(hipMemcpyAsync(d_colstage + (size_t)g * npad, v->group_bins[g], (size_t)N, hipMemcpyHostToDevice, stream));
     d_grad = dmalloc<float>((size_t)npad);
     d_hess = dmalloc<float>((size_t)npad);
     d_feats = dmalloc<FeatDev>((size_t)F);
@@ -639,20 +644,20 @@ struct Learner {
     d_pool = dmalloc<double>(num_slots * (size_t)TB * 2);
     d_pool_cnt = dmalloc<int>(num_slots * (size_t)TB);
     if (form == FORM_QUANT) {
-      d_part_q = dmalloc<unsigned long long>((size_t)max_tasks * TB);
+      d_part_q = dmalloc<unsigned long long>((size_t)max_partials * TB);
     } else {
-      d_part_g = dmalloc<double>((size_t)max_tasks * TB);
-      d_part_h = dmalloc<double>((size_t)max_tasks * TB);
-      d_part_c = dmalloc<unsigned>((size_t)max_tasks * TB);
+      d_part_g = dmalloc<double>((size_t)max_partials * TB);
+      d_part_h = dmalloc<double>((size_t)max_partials * TB);
+      d_part_c = dmalloc<unsigned>((size_t)max_partials * TB);
     }
-    d_blk_cnt = dmalloc<int>((size_t)max_tasks);
+    d_status = dmalloc<unsigned long long>((size_t)max_part_tiles);
+    BK_HIPC(hipMemsetAsync(d_status, 0, sizeof(unsigned long long) * (size_t)max_part_tiles, stream));
+    d_ticket = dmalloc<unsigned long long>(1);
+    BK_HIPC(hipMemsetAsync(d_ticket, 0, sizeof(unsigned long long), stream));
     d_err = dmalloc<int>(1);
     BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
     d_tile_first = dmalloc<int>(tile_first.size());
     BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
-                           stream));
-    d_htile_first = dmalloc<int>(htile_first.size());
-    BK_HIPC(hipMemcpyAsync(d_htile_first, htile_first.data(), sizeof(int) * htile_first.size(), hipMemcpyHostToDevice,
                            stream));
     d_used = dmalloc<int>((size_t)F * max_items);
     d_group_used = dmalloc<int>((size_t)G);
@@ -725,11 +730,13 @@ struct Learner {
     ds.num_features = F;
     ds.total_bins = TB;
     ds.form = form;
+    ds.stride = stride;
+    ds.pay_off = pay_off;
     for (int b = 0; b < 3; ++b) {
-      ds.cols[b] = d_cols[b];
+      ds.rec[b] = d_rec[b];
       ds.idx[b] = d_idx[b];
-      ds.gh[b] = d_gh[b];
     }
+    ds.gh0 = d_gh0;
     ds.feats = d_feats;
     ds.group_bin_start = d_group_bin_start;
     ds.pool = d_pool;
@@ -738,8 +745,9 @@ struct Learner {
     ds.part_g = d_part_g;
     ds.part_h = d_part_h;
     ds.part_c = d_part_c;
-    ds.blk_cnt = d_blk_cnt;
     ds.err_flag = d_err;
+    launch_to_records(ds, d_colstage, (size_t)npad, stream);
+    BK_HIPC(hipStreamSynchronize(stream));
     pool.init((size_t)P.num_leaves, cache, (size_t)F);
     best.assign((size_t)P.num_leaves, HostSplit());
     segs.assign((size_t)P.num_leaves, Segment{0, 0, 0});
@@ -798,7 +806,7 @@ struct Learner {
         const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
         timed(4, db, db, [&] {
           launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0,
-                            stochastic ? 1 : 0, (int16_t*)d_gh[0], d_sums3, stream);
+                            stochastic ? 1 : 0, (int16_t*)d_gh0, d_sums3, stream);
         });
         long long sums[4];
         BK_HIPC(hipMemcpyAsync(sums, d_sums3, sizeof sums, hipMemcpyDeviceToHost, stream));
@@ -820,7 +828,7 @@ struct Learner {
           last_root_h = (double)sums[1] * hess_scale;
         } else {
           std::vector<int16_t> pk((size_t)N);
-          BK_HIPC(hipMemcpyAsync(pk.data(), d_gh[0], sizeof(int16_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+          BK_HIPC(hipMemcpyAsync(pk.data(), d_gh0, sizeof(int16_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
           BK_HIPC(hipStreamSynchronize(stream));
           double a = 0.0, b = 0.0;
           for (int i = 0; i < N; ++i) {
@@ -836,7 +844,7 @@ struct Learner {
       sh = last_root_h;
       isum = (long long)(((unsigned long long)last_int_g << 32) + (unsigned long long)last_int_h);
     } else {
-      launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh[0], stream);
+      launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
       int nb = launch_sum_gh(g, h, N, d_sum_partials, stream);
       std::vector<double> part((size_t)nb * 2);
       float hh0 = 0.0f;
@@ -898,37 +906,39 @@ struct Learner {
   }
 
   // Algorithmic bytes per row: a root build reads F bins and the payload P; a
-  // leaf build also reads a 4-byte row index. A partition reads one bin (count
-  // pass), reads and writes a 4-byte row index (scatter pass).
-  void run_hist(int nitems, int first_task, int ntasks, double rows, bool root = true) {
-    if (ntasks <= first_task) return;
+  // leaf build also reads a 4-byte row index; a partition reads one bin and
+  // reads and writes a 4-byte row index.
+  //
+  // Histogram of the staged items over tasks [first, end) of the task array;
+  // task first + j writes partial histogram partial_base + j.
+  void run_hist(int nitems, int first, int end, int partial_base, double rows, bool root) {
+    if (end <= first) return;
+    if (partial_base + (end - first) > max_partials) throw std::runtime_error("partial buffer too small");
     BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_tasks + first_task, h_tasks + first_task, sizeof(Task) * (size_t)(ntasks - first_task),
+    BK_HIPC(hipMemcpyAsync(d_tasks + first, h_tasks + first, sizeof(Task) * (size_t)(end - first),
                            hipMemcpyHostToDevice, stream));
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
-    timed(0, hb, hb, [&] {
-      launch_hist(ds, d_hitems, d_tasks + first_task, ntasks - first_task, first_task, d_group_used, max_htile_bins,
-                  d_htile_first, (int)htile_first.size() - 1, rows_per_flush, stream);
+    timed(root ? 0 : 1, hb, rows * (stride + (root ? payload_bytes() : 0)), [&] {
+      launch_hist(ds, d_hitems, d_tasks + first, end - first, partial_base, d_group_used, max_tile_bins, d_tile_first,
+                  (int)tile_first.size() - 1, rows_per_flush, stream);
     });
     staged();
     stats.hist_rows += (int64_t)rows;
   }
 
-  void run_partition(int nitems, int ntasks, double part_rows, double hist_rows, bool any_hist) {
+  // Partition of the staged items over tasks [0, ntasks) of the task array.
+  void run_partition(int nitems, int ntasks, double part_rows) {
     if (ntasks <= 0) return;
+    if (ntasks > max_part_tiles) throw std::runtime_error("partition status buffer too small");
     BK_HIPC(hipMemcpyAsync(d_pitems, h_pitems, sizeof(PartItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
     BK_HIPC(hipMemcpyAsync(d_tasks, h_tasks, sizeof(Task) * (size_t)ntasks, hipMemcpyHostToDevice, stream));
-    const double rec = (double)(G + payload_bytes() + 4);
-    timed(1, part_rows, part_rows, [&] { launch_partition_count(ds, d_pitems, d_tasks, ntasks, stream); });
-    const double algo = part_rows * 8.0 + hist_rows * (G + payload_bytes() + 4);
-    const double moved = part_rows * (2.0 * rec + 1.0);
-    timed(2, algo, moved, [&] {
-      launch_partition_scatter(ds, d_pitems, d_tasks, ntasks, any_hist, d_group_used, max_tile_bins, max_tile_groups,
-                               d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+    epoch = epoch >= 0x3ffffffeu ? 1u : epoch + 1u;
+    timed(2, part_rows * 9.0, part_rows * (2.0 * stride + 8.0), [&] {
+      launch_partition(ds, d_pitems, d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
     });
+    ticket_base += (unsigned long long)ntasks;
     staged();
     stats.part_rows += (int64_t)part_rows;
-    stats.hist_rows += (int64_t)hist_rows;
   }
 
   // launches the search of the staged items and copies the candidates back
@@ -1074,7 +1084,7 @@ struct Learner {
       h_hitems[0] = HistItem{segs[0], 0, 0, chunk_for(N, true), 0};
       h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
-      run_hist(1, 0, nt, N);
+      run_hist(1, 0, nt, 0, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
@@ -1115,36 +1125,45 @@ struct Learner {
       pi.dst_buf = dst;
       pi.split_group = feats[(size_t)info.inner].group;
       pi.left_total = lc;
-      pi.hist_side = search ? (lc < rc ? 0 : 1) : -1;
-      pi.chunk = chunk_for(ps.count, true);
+      pi.chunk = part_tile;
       pi.rule = rule_of(info);
       pi.first_task = add_tasks(0, ps.count, pi.chunk, nt);
       pi.ntasks = nt;
-      run_partition(1, nt, ps.count, search ? std::min(lc, rc) : 0, search);
+      run_partition(1, nt, ps.count);
       if (search) {
+        // the smaller child's rows are contiguous now: build its histogram, and
+        // the larger child's too when the parent's histogram was evicted
         const bool use_subtract = parent_slot >= 0;
         ScanItem& si = h_sitems[0];
         si = ScanItem{};
         si.small = leaf_args(t, smaller, (int)smaller_slot);
-        si.small.part_first = 0;
-        si.small.nblk = nt;
         si.has_large = larger.leaf >= 0 && larger_slot >= 0;
         si.use_subtract = use_subtract;
+        const Segment sseg = segs[(size_t)smaller.leaf];
+        int ht = nt;
+        h_hitems[0] = HistItem{sseg, 0, 0, chunk_for(sseg.count, true), 0};
+        h_hitems[0].first_task = add_tasks(0, sseg.count, h_hitems[0].chunk, ht);
+        h_hitems[0].ntasks = ht - nt;
+        si.small.part_first = 0;
+        si.small.nblk = ht - nt;
+        int nh = 1;
+        double hrows = sseg.count;
         if (si.has_large) {
           si.large = leaf_args(t, larger, (int)larger_slot);
           if (!use_subtract) {
-            // the parent's histogram was evicted: build the larger child too
             const Segment lseg = segs[(size_t)larger.leaf];
-            int ht = nt;
-            h_hitems[0] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
-            h_hitems[0].first_task = add_tasks(0, lseg.count, h_hitems[0].chunk, ht);
-            h_hitems[0].ntasks = ht - nt;
-            run_hist(1, nt, ht, lseg.count, false);
+            const int lfirst = ht;
+            h_hitems[1] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
+            h_hitems[1].first_task = add_tasks(1, lseg.count, h_hitems[1].chunk, ht);
+            h_hitems[1].ntasks = ht - lfirst;
             stats.committed_hist_rows += lseg.count;
-            si.large.part_first = nt;
-            si.large.nblk = ht - nt;
+            si.large.part_first = lfirst - nt;
+            si.large.nblk = ht - lfirst;
+            nh = 2;
+            hrows += lseg.count;
           }
         }
+        run_hist(nh, nt, ht, 0, hrows, false);
         run_scan(1);
         fold_sequential(si.has_large != 0);
       }
@@ -1178,7 +1197,7 @@ struct Learner {
       h_hitems[0] = HistItem{nodes[0].seg, 0, 0, chunk_for(N, true), 0};
       h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
-      run_hist(1, 0, nt, N);
+      run_hist(1, 0, nt, 0, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
@@ -1235,7 +1254,7 @@ struct Learner {
       if (order.size() > budget) order.resize(budget);
       int ni = 0, nt = 0, ns = 0;
       double part_rows = 0.0, hist_rows = 0.0;
-      bool any_hist = false;
+      hist_segs.clear();
       for (int l : order) {
         const int x = leaf_node[(size_t)l];
         Node& X = nodes[(size_t)x];
@@ -1259,14 +1278,12 @@ struct Learner {
         pi.dst_buf = dst;
         pi.split_group = feats[(size_t)info.inner].group;
         pi.left_total = lc;
-        pi.hist_side = search ? (left_small ? 0 : 1) : -1;
-        pi.chunk = chunk_for(X.seg.count, false);
+        pi.chunk = part_tile;
         pi.rule = rule_of(info);
         pi.first_task = add_tasks(ni, X.seg.count, pi.chunk, nt);
         pi.ntasks = nt - pi.first_task;
         part_rows += X.seg.count;
         if (search) {
-          any_hist = true;
           hist_rows += std::min(lc, rc);
           // the smaller child gets a new slot; the larger one reuses the parent's
           Node& sm = left_small ? cl : cr;
@@ -1280,8 +1297,7 @@ struct Learner {
           si = ScanItem{};
           // a child's parent output is its own output from the split (never the root's formula)
           si.small = leaf_args_with(sm.ls, sm.slot, sm.ls.weight);
-          si.small.part_first = pi.first_task;
-          si.small.nblk = pi.ntasks;
+          hist_segs.push_back(sm.seg);
           si.large = leaf_args_with(lg.ls, lg.slot, lg.ls.weight);
           si.has_large = 1;
           si.use_subtract = 1;
@@ -1298,8 +1314,22 @@ struct Learner {
         ++ni;
       }
       if (ni == 0) break;
-      run_partition(ni, nt, part_rows, hist_rows, any_hist);
-      if (ns > 0) run_scan(ns);
+      run_partition(ni, nt, part_rows);
+      if (ns > 0) {
+        // histograms of the smaller children, now contiguous in their buffers
+        int ht = nt;
+        for (int k2 = 0; k2 < ns; ++k2) {
+          const Segment sg = hist_segs[(size_t)k2];
+          HistItem& hi = h_hitems[k2];
+          hi = HistItem{sg, 0, 0, chunk_for(sg.count, false), 0};
+          hi.first_task = add_tasks(k2, sg.count, hi.chunk, ht);
+          hi.ntasks = ht - hi.first_task;
+          h_sitems[k2].small.part_first = hi.first_task - nt;
+          h_sitems[k2].small.nblk = hi.ntasks;
+        }
+        run_hist(ns, nt, ht, 0, hist_rows, false);
+        run_scan(ns);
+      }
       // fold the children's candidates
       int k = 0;
       for (int l : order) {

@@ -196,12 +196,13 @@ typedef struct bk_train_stats {
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
 /* Per-kernel timing with HIP events recorded on the learner's stream around
- * every launch (off by default). Kinds: 0 segment histogram, 1 partition count,
- * 2 partition scatter with the child histogram, 3 split search, 4 discretizer,
- * 5 leaf renewal, 6 score update and gradients. algo_bytes follows the per-row
- * byte model of the histogram (rows x (features + payload)) and of the partition
- * (rows x 9: row id read, one bin, row id write); moved_bytes counts what the
- * kernel actually reads and writes. No reference counterpart (measurement only). */
+ * every launch (off by default). Kinds: 0 root histogram, 1 child histograms,
+ * 2 partition, 3 split search, 4 discretizer, 5 leaf renewal, 6 score update
+ * and gradients. algo_bytes follows the per-row byte model of the histogram
+ * (rows x (features + payload), plus a 4-byte row index for a child) and of the
+ * partition (rows x 9: one bin, row id read, row id write); moved_bytes counts
+ * what the kernel actually reads and writes. No reference counterpart
+ * (measurement only). */
 typedef struct bk_kernel_stats {
   double ms;
   int64_t launches;
