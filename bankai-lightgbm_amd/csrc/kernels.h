@@ -183,12 +183,9 @@ struct Segment {
   int buf, begin, count;
 };
 
-// One row block of one item of a launch.
-struct Task {
-  int item, blk;
-};
-
-// A histogram build over a whole segment, split in ntasks row blocks of `chunk` rows.
+// A histogram build over a whole segment, split in ntasks row blocks of `chunk`
+// rows. The items of a launch own consecutive block ranges: item i owns blocks
+// first_task .. first_task + ntasks - 1 (first_task ascending, from 0).
 struct HistItem {
   Segment seg;
   int first_task, ntasks, chunk, pad;
@@ -203,6 +200,12 @@ struct PartItem {
   RuleDev rule;
 };
 
+// The folded best split of one leaf: the candidate and its inner feature (-1: none).
+struct FoldOut {
+  SplitOut o;
+  int inner, pad[3];
+};
+
 // One search launch item: the smaller child (partials) and the larger one
 // (subtraction from the parent slot, or its own partials).
 struct ScanItem {
@@ -214,8 +217,8 @@ struct ScanItem {
 // Records of buffer 0 from column-major bins (group g at cols + g * col_stride).
 void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream);
 size_t hist_lds_bytes(int form, int max_tile_bins);
-// Tasks `d_tasks[0..ntasks)` write partial histograms task_base .. task_base+ntasks-1.
-void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
+// Blocks 0 .. ntasks-1 of the items write partial histograms task_base .. task_base+ntasks-1.
+void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int ntasks, int task_base,
                  const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
                  void* stream);
 // Rows per partition tile for a record stride (2048 16-byte chunks per tile).
@@ -224,11 +227,15 @@ int partition_tile_rows(int stride);
 // order (chunk == partition_tile_rows). d_status holds one word per task,
 // d_ticket a 64-bit counter of tiles taken (ticket_base = its value before this
 // launch), epoch a nonzero tag below 2^30 that differs from the previous launch.
-void launch_partition(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks,
                       unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
                       unsigned epoch, void* stream);
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
                  SplitOut* d_out, void* stream);
+// Best candidate of every (item, side) -> d_best[2 * item + side], and every
+// feature's splittable flag -> d_flags[(2 * item + side) * F + f].
+void launch_fold(const DevState& s, const ScanItem* d_items, int nitems, const SplitOut* d_out, const int* d_used,
+                 FoldOut* d_best, uint8_t* d_flags, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,

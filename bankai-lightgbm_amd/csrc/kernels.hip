@@ -282,6 +282,19 @@ __device__ __forceinline__ void init_gbs(const DevState& s, int* gbs, int g0, in
     gbs[g] = (g >= g0 && g < g1 && group_used[g]) ? s.group_bin_start[g] - bin_lo : -1;
 }
 
+// Item owning task t of a launch: items are listed with ascending first_task,
+// item i owning tasks first_task .. first_task + ntasks - 1.
+template <typename Item>
+__device__ __forceinline__ int find_item(const Item* __restrict__ items, int nitems, int t) {
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].first_task <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Adds one 16-byte chunk of a row with the chunk's group bin offsets in
 // registers (off[j] < 0: group outside the tile, unused, or padding).
 template <int FORM>
@@ -304,15 +317,14 @@ constexpr int kHistThreads = 512;
 constexpr int kHistRows = 2048;  // rows per payload staging tile
 
 template <int FORM>
-__global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistItem* __restrict__ items,
-                                                       const Task* __restrict__ tasks,
+__global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistItem* __restrict__ items, int nitems,
                                                        const int* __restrict__ group_used,
                                                        const int* __restrict__ tile_first, int rows_per_flush,
                                                        int task_base) {
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
-  const Task task = tasks[blockIdx.x];
-  const HistItem it = items[task.item];
+  const HistItem it = items[find_item(items, nitems, (int)blockIdx.x)];
+  const int blk = (int)blockIdx.x - it.first_task;
   const Segment seg = it.seg;
   const int tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
@@ -324,7 +336,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   init_gbs(s, gbs, g0, g1, bin_lo, group_used);
   L.zero(nb);
   const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)task.blk * it.chunk;
+  const long long cb = (long long)seg.begin + (long long)blk * it.chunk;
   const long long ce = min(cb + (long long)it.chunk, seg_end);
   const int c0 = g0 >> 4, nc = ((g1 - 1) >> 4) + 1 - c0;
   const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
@@ -403,7 +415,7 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
 
 template <int FORM>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
-                                                            const Task* __restrict__ tasks, int ntasks,
+                                                            int nitems, int ntasks,
                                                             unsigned long long* status, unsigned long long* ticket,
                                                             unsigned long long ticket_base, unsigned epoch) {
   __shared__ int sh_tile, sh_left_before;
@@ -419,13 +431,13 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   __syncthreads();
   const int t = sh_tile;
   if (t < 0 || t >= ntasks) return;  // one ticket per block: never taken
-  const Task task = tasks[t];
-  const PartItem& it = items[task.item];
+  const PartItem& it = items[find_item(items, nitems, t)];
+  const int blk = t - it.first_task;
   const Segment seg = it.seg;
   const RuleDev rule = it.rule;
   const int left_total = it.left_total;
   const long long seg_end = (long long)seg.begin + seg.count;
-  const long long t0 = (long long)seg.begin + (long long)task.blk * it.chunk;
+  const long long t0 = (long long)seg.begin + (long long)blk * it.chunk;
   const int rows = (int)min((long long)R, seg_end - t0);
   const uint8_t* src = s.rec[seg.buf] + (size_t)t0 * stride;
   uint8_t* dst = s.rec[it.dst_buf];
@@ -459,7 +471,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
     long long excl = 0;
-    if (task.blk == 0) {
+    if (blk == 0) {
       if (lane == 0)
         __hip_atomic_store(&status[t], status_word(2, epoch, (unsigned)tl), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -537,7 +549,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     const uint32_t d = dstp[r];
     if (d != kSkip) didx[d] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
   }
-  if (threadIdx.x == 0 && task.blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
+  if (threadIdx.x == 0 && blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
 }
 
 // Records of buffer 0 from column-major bins: a block stages 64 rows of every
@@ -1885,6 +1897,64 @@ __global__ void k_leaf_of(DevState s, const Segment* __restrict__ segs, int32_t*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Per-leaf fold of the per-feature candidates in feature order with
+// better_than (serial.rs:1500-1566, split_info.rs:67-85): the highest gain
+// wins, a tie goes to the lower real feature index (-1 counting as largest),
+// a NaN gain never wins. One wave per (item, side). Also writes every
+// feature's splittable flag (0 for features not searched).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
+                                             const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
+                                             uint8_t* __restrict__ flags) {
+  const int item = blockIdx.x >> 1, side = blockIdx.x & 1;
+  const ScanItem& it = items[item];
+  const int lane = threadIdx.x;
+  uint8_t* fl = flags + (size_t)blockIdx.x * F;
+  FoldOut* dst = best + blockIdx.x;
+  const bool active = side == 0 || it.has_large;
+  const int* used = used_all + it.used_off;
+  const SplitOut* out = out_all + it.out_off + (side ? F : 0);
+  double bg = kMinScore;
+  int bk = INT_MAX, bf = -1;
+  for (int f = lane; f < F; f += 64) {
+    int sp = 0;
+    if (active && used[f]) {
+      const double gain = out[f].gain;
+      const int feat = out[f].feature;
+      sp = out[f].splittable ? 1 : 0;
+      const int key = feat == -1 ? INT_MAX : feat;
+      if (!(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
+        bg = gain;
+        bk = key;
+        bf = f;
+      }
+    }
+    fl[f] = (uint8_t)sp;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double og = __shfl_xor(bg, o, 64);
+    const int ok = __shfl_xor(bk, o, 64), of = __shfl_xor(bf, o, 64);
+    if (of >= 0 && (bf < 0 || og > bg || (og == bg && ok < bk))) {
+      bg = og;
+      bk = ok;
+      bf = of;
+    }
+  }
+  if (lane == 0) {
+    FoldOut r{};
+    if (bf >= 0) {
+      r.o = out[bf];
+    } else {
+      r.o.gain = kMinScore;
+      r.o.feature = -1;
+      r.o.default_left = 1;
+    }
+    r.inner = bf;
+    *dst = r;
+  }
+}
+
 inline int grid_for(long long n, int threads, int cap = 4096) {
   long long b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -1912,7 +1982,7 @@ size_t hist_lds_bytes(int form, int max_tile_bins) {
   return ((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15;
 }
 
-void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks, int ntasks, int task_base,
+void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int ntasks, int task_base,
                  const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
                  void* stream) {
   if (ntasks <= 0) return;
@@ -1924,17 +1994,17 @@ void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
-      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
                          d_tile_first, rows_per_flush, task_base);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
-      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
                          d_tile_first, rows_per_flush, task_base);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
-      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, d_tasks, d_group_used,
+      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
                          d_tile_first, rows_per_flush, task_base);
       break;
   }
@@ -1943,7 +2013,7 @@ void launch_hist(const DevState& s, const HistItem* d_items, const Task* d_tasks
 
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
-void launch_partition(const DevState& s, const PartItem* d_items, const Task* d_tasks, int ntasks,
+void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks,
                       unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
                       unsigned epoch, void* stream) {
   if (ntasks <= 0) return;
@@ -1954,11 +2024,11 @@ void launch_partition(const DevState& s, const PartItem* d_items, const Task* d_
   switch (s.form) {
     case FORM_QUANT:
       hipLaunchKernelGGL(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
-                         d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch);
+                         nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
     default:
       hipLaunchKernelGGL(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
-                         d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch);
+                         nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -1970,6 +2040,14 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
   allow_lds(k_to_records);
   const unsigned blocks = (unsigned)((s.num_data + kRecRows - 1) / kRecRows);
   hipLaunchKernelGGL(k_to_records, dim3(blocks), dim3(256), lds, (hipStream_t)stream, s, cols, col_stride);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_fold(const DevState& s, const ScanItem* d_items, int nitems, const SplitOut* d_out, const int* d_used,
+                 FoldOut* d_best, uint8_t* d_flags, void* stream) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, (hipStream_t)stream, d_items, d_out, d_used,
+                     s.num_features, d_best, d_flags);
   BK_HIP(hipGetLastError());
 }
 

@@ -362,7 +362,7 @@ struct Learner {
   int rows_per_flush = 0;
   size_t cache = 0;       // histogram slots of the reference pool
   size_t num_slots = 0;   // device slots (batched form: one per evaluated leaf)
-  int max_items = 0, max_tasks = 0, max_partials = 0, max_part_tiles = 0;
+  int max_items = 0, max_partials = 0, max_part_tiles = 0;
   int part_tile = 0;                   // rows per partition tile
   unsigned long long ticket_base = 0;  // partition tiles taken before the next launch
   unsigned epoch = 0;                  // tag of the last partition launch
@@ -390,10 +390,11 @@ struct Learner {
   int* d_used = nullptr;        // per item x feature
   int* d_group_used = nullptr;  // per group
   SplitOut* d_out = nullptr;
+  FoldOut* d_fold = nullptr;
+  uint8_t* d_flags = nullptr;
   HistItem* d_hitems = nullptr;
   PartItem* d_pitems = nullptr;
   ScanItem* d_sitems = nullptr;
-  Task* d_tasks = nullptr;      // partition tasks, then histogram tasks
   unsigned* d_max4 = nullptr;
   long long* d_sums3 = nullptr;
   double* d_sum_partials = nullptr;
@@ -412,11 +413,11 @@ struct Learner {
   bool leaf_of_valid = false;
   int renew_nparts = 0;
   // pinned host staging
-  SplitOut* h_out = nullptr;
+  FoldOut* h_fold = nullptr;
+  uint8_t* h_flags = nullptr;
   HistItem* h_hitems = nullptr;
   PartItem* h_pitems = nullptr;
   ScanItem* h_sitems = nullptr;
-  Task* h_tasks = nullptr;
   int* h_used = nullptr;
   int* h_group_used = nullptr;
   Segment* h_segs = nullptr;
@@ -476,13 +477,13 @@ struct Learner {
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
                     (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
-                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_hitems,
-                    (void*)d_pitems, (void*)d_sitems, (void*)d_tasks, (void*)d_max4, (void*)d_sums3,
+                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_fold, (void*)d_flags, (void*)d_hitems,
+                    (void*)d_pitems, (void*)d_sitems, (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
-    for (void* p : {(void*)h_out, (void*)h_hitems, (void*)h_pitems, (void*)h_sitems, (void*)h_tasks, (void*)h_used,
+    for (void* p : {(void*)h_fold, (void*)h_flags, (void*)h_hitems, (void*)h_pitems, (void*)h_sitems, (void*)h_used,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
                     (void*)h_sorted_leaf})
       fh(p);
@@ -619,7 +620,6 @@ struct Learner {
     part_tile = partition_tile_rows(stride);
     max_part_tiles = N / part_tile + max_items + 8;
     max_partials = N / (4 * kTileRows) + 2 * max_items + 2 * kMaxRowBlocks + 8;
-    max_tasks = max_part_tiles + max_partials;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming));
@@ -665,7 +665,6 @@ struct Learner {
     d_hitems = dmalloc<HistItem>((size_t)max_items);
     d_pitems = dmalloc<PartItem>((size_t)max_items);
     d_sitems = dmalloc<ScanItem>((size_t)max_items);
-    d_tasks = dmalloc<Task>((size_t)max_tasks);
     d_max4 = dmalloc<unsigned>(4);
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
@@ -673,11 +672,13 @@ struct Learner {
     d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
     d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
-    pinned(&h_out, (size_t)F * 2 * max_items);
+    d_fold = dmalloc<FoldOut>((size_t)2 * max_items);
+    d_flags = dmalloc<uint8_t>((size_t)2 * max_items * F);
+    pinned(&h_fold, (size_t)2 * max_items);
+    pinned(&h_flags, (size_t)2 * max_items * F);
     pinned(&h_hitems, (size_t)max_items);
     pinned(&h_pitems, (size_t)max_items);
     pinned(&h_sitems, (size_t)max_items);
-    pinned(&h_tasks, (size_t)max_tasks);
     pinned(&h_used, (size_t)F * max_items);
     pinned(&h_group_used, (size_t)G);
     pinned(&h_segs, (size_t)P.num_leaves);
@@ -897,11 +898,10 @@ struct Learner {
     staged_pending = false;
   }
 
-  int add_tasks(int item, int count, int chunk, int& ntasks) {
+  // Reserves the row blocks of one item of a launch: returns its first block.
+  static int add_tasks(int count, int chunk, int& ntasks) {
     const int first = ntasks;
-    const int n = count > 0 ? (count + chunk - 1) / chunk : 0;
-    if (ntasks + n > max_tasks) throw std::runtime_error("task buffer too small");
-    for (int b = 0; b < n; ++b) h_tasks[ntasks++] = Task{item, b};
+    ntasks += count > 0 ? (count + chunk - 1) / chunk : 0;
     return first;
   }
 
@@ -909,32 +909,29 @@ struct Learner {
   // leaf build also reads a 4-byte row index; a partition reads one bin and
   // reads and writes a 4-byte row index.
   //
-  // Histogram of the staged items over tasks [first, end) of the task array;
-  // task first + j writes partial histogram partial_base + j.
-  void run_hist(int nitems, int first, int end, int partial_base, double rows, bool root) {
-    if (end <= first) return;
-    if (partial_base + (end - first) > max_partials) throw std::runtime_error("partial buffer too small");
+  // Histogram of the staged items (blocks 0 .. ntasks-1); block j writes
+  // partial histogram partial_base + j.
+  void run_hist(int nitems, int ntasks, int partial_base, double rows, bool root) {
+    if (ntasks <= 0) return;
+    if (partial_base + ntasks > max_partials) throw std::runtime_error("partial buffer too small");
     BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_tasks + first, h_tasks + first, sizeof(Task) * (size_t)(end - first),
-                           hipMemcpyHostToDevice, stream));
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
     timed(root ? 0 : 1, hb, rows * (stride + (root ? payload_bytes() : 0)), [&] {
-      launch_hist(ds, d_hitems, d_tasks + first, end - first, partial_base, d_group_used, max_tile_bins, d_tile_first,
+      launch_hist(ds, d_hitems, nitems, ntasks, partial_base, d_group_used, max_tile_bins, d_tile_first,
                   (int)tile_first.size() - 1, rows_per_flush, stream);
     });
     staged();
     stats.hist_rows += (int64_t)rows;
   }
 
-  // Partition of the staged items over tasks [0, ntasks) of the task array.
+  // Partition of the staged items (tiles 0 .. ntasks-1).
   void run_partition(int nitems, int ntasks, double part_rows) {
     if (ntasks <= 0) return;
     if (ntasks > max_part_tiles) throw std::runtime_error("partition status buffer too small");
     BK_HIPC(hipMemcpyAsync(d_pitems, h_pitems, sizeof(PartItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_tasks, h_tasks, sizeof(Task) * (size_t)ntasks, hipMemcpyHostToDevice, stream));
     epoch = epoch >= 0x3ffffffeu ? 1u : epoch + 1u;
     timed(2, part_rows * 9.0, part_rows * (2.0 * stride + 8.0), [&] {
-      launch_partition(ds, d_pitems, d_tasks, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
+      launch_partition(ds, d_pitems, nitems, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
     });
     ticket_base += (unsigned long long)ntasks;
     staged();
@@ -947,24 +944,26 @@ struct Learner {
     ScanParams sp = scan_params();
     BK_HIPC(hipMemcpyAsync(d_sitems, h_sitems, sizeof(ScanItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
     BK_HIPC(hipMemcpyAsync(d_used, h_used, sizeof(int) * (size_t)F * nitems, hipMemcpyHostToDevice, stream));
-    timed(3, 0.0, 0.0, [&] { launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream); });
-    BK_HIPC(hipMemcpyAsync(h_out, d_out, sizeof(SplitOut) * (size_t)F * 2 * nitems, hipMemcpyDeviceToHost, stream));
+    timed(3, 0.0, 0.0, [&] {
+      launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream);
+      launch_fold(ds, d_sitems, nitems, d_out, d_used, d_fold, d_flags, stream);
+    });
+    BK_HIPC(hipMemcpyAsync(h_fold, d_fold, sizeof(FoldOut) * (size_t)2 * nitems, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipMemcpyAsync(h_flags, d_flags, (size_t)2 * nitems * F, hipMemcpyDeviceToHost, stream));
     BK_HIPC(hipStreamSynchronize(stream));
     staged_pending = false;
     stats.rounds += 1;
   }
 
-  // folds the candidates of one leaf in feature order (serial.rs:1500-1566)
-  HostSplit fold(int out_off, const int* used, std::vector<char>& splittable) const {
+  // The folded best split of (item, side) k = 2 * item + side and the
+  // splittable flags of its features (serial.rs:1500-1566; folded on the device).
+  HostSplit fold(int k, std::vector<char>& splittable) const {
     HostSplit b;
-    for (int f = 0; f < F; ++f) {
-      if (!used[f]) continue;
-      HostSplit c;
-      c.o = h_out[out_off + f];
-      c.inner = f;
-      splittable[(size_t)f] = c.o.splittable ? 1 : 0;
-      if (c.better_than(b)) b = c;
+    if (h_fold[k].inner >= 0) {
+      b.o = h_fold[k].o;
+      b.inner = h_fold[k].inner;
     }
+    for (int f = 0; f < F; ++f) splittable[(size_t)f] = (char)h_flags[(size_t)k * F + f];
     return b;
   }
 
@@ -1068,10 +1067,10 @@ struct Learner {
   }
 
   void fold_sequential(bool has_large) {
-    HostSplit sb = fold(0, h_used, pool.splittable[smaller_slot]);
+    HostSplit sb = fold(0, pool.splittable[smaller_slot]);
     best[(size_t)smaller.leaf] = sb;
     if (larger.leaf >= 0)
-      best[(size_t)larger.leaf] = has_large ? fold(F, h_used, pool.splittable[(size_t)larger_slot]) : HostSplit();
+      best[(size_t)larger.leaf] = has_large ? fold(1, pool.splittable[(size_t)larger_slot]) : HostSplit();
   }
 
   Tree train_sequential(const float* g, const float* h) {
@@ -1082,9 +1081,9 @@ struct Learner {
       compute_used();
       int nt = 0;
       h_hitems[0] = HistItem{segs[0], 0, 0, chunk_for(N, true), 0};
-      h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
+      h_hitems[0].first_task = add_tasks(N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
-      run_hist(1, 0, nt, 0, N, true);
+      run_hist(1, nt, 0, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
@@ -1127,7 +1126,7 @@ struct Learner {
       pi.left_total = lc;
       pi.chunk = part_tile;
       pi.rule = rule_of(info);
-      pi.first_task = add_tasks(0, ps.count, pi.chunk, nt);
+      pi.first_task = add_tasks(ps.count, pi.chunk, nt);
       pi.ntasks = nt;
       run_partition(1, nt, ps.count);
       if (search) {
@@ -1140,12 +1139,12 @@ struct Learner {
         si.has_large = larger.leaf >= 0 && larger_slot >= 0;
         si.use_subtract = use_subtract;
         const Segment sseg = segs[(size_t)smaller.leaf];
-        int ht = nt;
+        int ht = 0;
         h_hitems[0] = HistItem{sseg, 0, 0, chunk_for(sseg.count, true), 0};
-        h_hitems[0].first_task = add_tasks(0, sseg.count, h_hitems[0].chunk, ht);
-        h_hitems[0].ntasks = ht - nt;
+        h_hitems[0].first_task = add_tasks(sseg.count, h_hitems[0].chunk, ht);
+        h_hitems[0].ntasks = ht;
         si.small.part_first = 0;
-        si.small.nblk = ht - nt;
+        si.small.nblk = ht;
         int nh = 1;
         double hrows = sseg.count;
         if (si.has_large) {
@@ -1154,16 +1153,16 @@ struct Learner {
             const Segment lseg = segs[(size_t)larger.leaf];
             const int lfirst = ht;
             h_hitems[1] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
-            h_hitems[1].first_task = add_tasks(1, lseg.count, h_hitems[1].chunk, ht);
+            h_hitems[1].first_task = add_tasks(lseg.count, h_hitems[1].chunk, ht);
             h_hitems[1].ntasks = ht - lfirst;
             stats.committed_hist_rows += lseg.count;
-            si.large.part_first = lfirst - nt;
+            si.large.part_first = lfirst;
             si.large.nblk = ht - lfirst;
             nh = 2;
             hrows += lseg.count;
           }
         }
-        run_hist(nh, nt, ht, 0, hrows, false);
+        run_hist(nh, ht, 0, hrows, false);
         run_scan(1);
         fold_sequential(si.has_large != 0);
       }
@@ -1195,9 +1194,9 @@ struct Learner {
       for (int f = 0; f < F; ++f) h_used[f] = 1;
       int nt = 0;
       h_hitems[0] = HistItem{nodes[0].seg, 0, 0, chunk_for(N, true), 0};
-      h_hitems[0].first_task = add_tasks(0, N, h_hitems[0].chunk, nt);
+      h_hitems[0].first_task = add_tasks(N, h_hitems[0].chunk, nt);
       h_hitems[0].ntasks = nt;
-      run_hist(1, 0, nt, 0, N, true);
+      run_hist(1, nt, 0, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
@@ -1207,7 +1206,7 @@ struct Learner {
       si.used_off = 0;
       si.out_off = 0;
       run_scan(1);
-      nodes[0].best = fold(0, h_used, nodes[0].splittable);
+      nodes[0].best = fold(0, nodes[0].splittable);
     } else {
       nodes[0].best.o.gain = kMinScore;
     }
@@ -1280,7 +1279,7 @@ struct Learner {
         pi.left_total = lc;
         pi.chunk = part_tile;
         pi.rule = rule_of(info);
-        pi.first_task = add_tasks(ni, X.seg.count, pi.chunk, nt);
+        pi.first_task = add_tasks(X.seg.count, pi.chunk, nt);
         pi.ntasks = nt - pi.first_task;
         part_rows += X.seg.count;
         if (search) {
@@ -1317,17 +1316,17 @@ struct Learner {
       run_partition(ni, nt, part_rows);
       if (ns > 0) {
         // histograms of the smaller children, now contiguous in their buffers
-        int ht = nt;
+        int ht = 0;
         for (int k2 = 0; k2 < ns; ++k2) {
           const Segment sg = hist_segs[(size_t)k2];
           HistItem& hi = h_hitems[k2];
           hi = HistItem{sg, 0, 0, chunk_for(sg.count, false), 0};
-          hi.first_task = add_tasks(k2, sg.count, hi.chunk, ht);
+          hi.first_task = add_tasks(sg.count, hi.chunk, ht);
           hi.ntasks = ht - hi.first_task;
-          h_sitems[k2].small.part_first = hi.first_task - nt;
+          h_sitems[k2].small.part_first = hi.first_task;
           h_sitems[k2].small.nblk = hi.ntasks;
         }
-        run_hist(ns, nt, ht, 0, hist_rows, false);
+        run_hist(ns, ht, 0, hist_rows, false);
         run_scan(ns);
       }
       // fold the children's candidates
@@ -1340,18 +1339,8 @@ struct Learner {
         const bool left_small = lc < rc;
         Node& sm = nodes[(size_t)X.child[left_small ? 0 : 1]];
         Node& lg = nodes[(size_t)X.child[left_small ? 1 : 0]];
-        std::vector<char> sp_s(sm.splittable), sp_l(lg.splittable);
-        const int* used = h_used + k * F;
-        sm.best = fold(k * 2 * F, used, sp_s);
-        lg.best = fold(k * 2 * F + F, used, sp_l);
-        for (int f = 0; f < F; ++f) {
-          if (!used[f]) {
-            sp_s[(size_t)f] = 0;
-            sp_l[(size_t)f] = 0;
-          }
-        }
-        sm.splittable = sp_s;
-        lg.splittable = sp_l;
+        sm.best = fold(2 * k, sm.splittable);
+        lg.best = fold(2 * k + 1, lg.splittable);
         ++k;
       }
     }

@@ -121,18 +121,20 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
 
 
 def device_copy_gbs(mib=1024, reps=10):
-    """Bandwidth of a plain device-to-device copy (torch), the practical HBM ceiling."""
+    """Bandwidth of a plain streaming kernel (read mib MiB, write mib MiB: an
+    elementwise negate), the practical HBM ceiling; its PMC counts also calibrate
+    FETCH_SIZE / WRITE_SIZE."""
     import torch
 
     n = mib * 1024 * 1024 // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    torch.neg(src, out=dst)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        dst.copy_(src)
+        torch.neg(src, out=dst)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
