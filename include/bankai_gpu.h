@@ -137,7 +137,8 @@ typedef struct bk_learner bk_learner;
 const char* bk_last_error(void);
 
 /* SerialTreeLearner::new (crates/lgbm-core/src/treelearner/serial.rs:284).
- * Uploads the bin columns once; they stay resident in HBM. */
+ * Uploads the bin columns once and transposes them into row records that stay
+ * resident in HBM for the learner's lifetime. */
 int bk_learner_create(const bk_dataset_view* ds, const bk_learner_params* params,
                       int is_constant_hessian, bk_learner** out);
 
