@@ -47,10 +47,22 @@ struct RuleDev {
   uint32_t cat_bits[8];
 };
 
+// Word selection with constant indices only: a runtime index into the bitset
+// would move the whole struct into per-lane scratch memory on the GPU.
+BK_HD inline uint32_t bits_word(const uint32_t* bits, uint32_t i) {
+  uint32_t w = 0u;
+  for (uint32_t k = 0; k < 8u; ++k)
+    if (i == k) w = bits[k];
+  return w;
+}
+BK_HD inline void bits_set(uint32_t* bits, uint32_t pos) {
+  for (uint32_t k = 0; k < 8u; ++k)
+    if ((pos >> 5) == k) bits[k] |= 1u << (pos & 31u);
+}
 BK_HD inline bool rule_bit(const RuleDev& r, uint32_t pos) {
   uint32_t i1 = pos / 32u;
   if (i1 >= 8u) return false;
-  return (r.cat_bits[i1] >> (pos % 32u)) & 1u;
+  return (bits_word(r.cat_bits, i1) >> (pos % 32u)) & 1u;
 }
 
 // route_bin (data_partition.rs:366-410) for one stored group value.

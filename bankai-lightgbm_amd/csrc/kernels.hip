@@ -236,11 +236,19 @@ struct Pay<FORM_QUANT> {
   using T = int16_t;
 };
 
+// Buffer pointers by buffer number. Selecting with branches keeps the kernel
+// argument in constant memory; indexing its arrays with a runtime value would
+// copy the whole argument into per-lane scratch.
+__device__ __forceinline__ uint8_t* rec_of(const DevState& s, int b) {
+  return b == 0 ? s.rec[0] : (b == 1 ? s.rec[1] : s.rec[2]);
+}
+__device__ __forceinline__ int32_t* idx_of(const DevState& s, int b) { return b == 1 ? s.idx[1] : s.idx[2]; }
+
 template <int FORM>
 __device__ __forceinline__ typename Pay<FORM>::T load_payload(const DevState& s, int buf, long long pos) {
   using T = typename Pay<FORM>::T;
   if (buf == 0) return ((const T*)s.gh0)[pos];
-  return *(const T*)(s.rec[buf] + (size_t)pos * s.stride + s.pay_off);
+  return *(const T*)(rec_of(s, buf) + (size_t)pos * s.stride + s.pay_off);
 }
 
 __device__ __forceinline__ uint32_t lane_word(const uint4& v, int i) {
@@ -342,7 +350,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
   const int rstep = tc / nc;
   const int my_c = c0 + (int)threadIdx.x % nc;
-  const uint8_t* src = s.rec[seg.buf] + 16 * my_c;
+  const uint8_t* src = rec_of(s, seg.buf) + 16 * my_c;
   __syncthreads();
   int off[16];
 #pragma unroll
@@ -413,6 +421,12 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
   return v;
 }
 
+// LDS of the partition: [destinations R x u32][route flags R bytes][staged records R x stride][staged ids R x i32]
+__host__ __device__ inline size_t stage_offset(int R) { return ((size_t)R * 5 + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t partition_lds(int R, int stride) {
+  return stage_offset(R) + (size_t)R * stride + 4 * (size_t)R;
+}
+
 template <int FORM>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
                                                             int nitems, int ntasks,
@@ -434,13 +448,13 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const PartItem& it = items[find_item(items, nitems, t)];
   const int blk = t - it.first_task;
   const Segment seg = it.seg;
-  const RuleDev rule = it.rule;
+  const RuleDev& rule = it.rule;  // by reference: a local copy of its bitset would live in scratch
   const int left_total = it.left_total;
   const long long seg_end = (long long)seg.begin + seg.count;
   const long long t0 = (long long)seg.begin + (long long)blk * it.chunk;
   const int rows = (int)min((long long)R, seg_end - t0);
-  const uint8_t* src = s.rec[seg.buf] + (size_t)t0 * stride;
-  uint8_t* dst = s.rec[it.dst_buf];
+  const uint8_t* src = rec_of(s, seg.buf) + (size_t)t0 * stride;
+  uint8_t* dst = rec_of(s, it.dst_buf);
   const int total = rows * nc;
   const int cs = it.split_group >> 4, bs = it.split_group & 15;
   uint4 v[kPartChunks];
@@ -514,40 +528,51 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   }
   __syncthreads();
   const int left_before = sh_left_before;
-  constexpr uint32_t kSkip = 0xffffffffu;
+  // tile-local destination of every row: left rows at [0, tl), right rows at
+  // [tl, tl + tr), both in row order
   {
-    long long lpos = (long long)seg.begin + left_before + pl;
-    long long rpos = (long long)seg.begin + left_total + ((t0 - seg.begin) - left_before) + pr;
+    int il = pl, ir = tl + pr;
     for (int i = 0; i < rpt; ++i) {
       const int r = (int)threadIdx.x * rpt + i;
-      if (!((vm >> i) & 1)) continue;
-      const long long d = ((lm >> i) & 1) ? lpos++ : rpos++;
-      uint32_t u = (uint32_t)d;
-      if (d < (long long)seg.begin || d >= seg_end) {
-        atomicOr(s.err_flag, 2);
-        u = kSkip;
-      }
-      dstp[r] = u;
+      if ((vm >> i) & 1) dstp[r] = ((lm >> i) & 1) ? il++ : ir++;
     }
   }
   __syncthreads();
+  // permute the records and row ids into destination order in LDS, so that the
+  // writes below are two contiguous runs covered lane by lane
+  uint8_t* stage = smem + stage_offset(R);
+  int32_t* stage_id = (int32_t*)(stage + (size_t)R * stride);
   const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) {
       const int r = q / nc, c = q - r * nc;
-      const uint32_t d = dstp[r];
-      if (d == kSkip) continue;
       if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
-      *(uint4*)(dst + (size_t)d * stride + 16 * c) = v[k];
+      *(uint4*)(stage + (size_t)dstp[r] * stride + 16 * c) = v[k];
     }
   }
-  int32_t* didx = s.idx[it.dst_buf];
-  const int32_t* sidx = s.idx[seg.buf];
-  for (int r = threadIdx.x; r < rows; r += kPartThreads) {
-    const uint32_t d = dstp[r];
-    if (d != kSkip) didx[d] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
+  {
+    const int32_t* sidx = idx_of(s, seg.buf);
+    for (int r = threadIdx.x; r < rows; r += kPartThreads)
+      stage_id[dstp[r]] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
+  }
+  __syncthreads();
+  const long long lpos = (long long)seg.begin + left_before;
+  const long long rpos = (long long)seg.begin + left_total + ((t0 - seg.begin) - left_before);
+  if (lpos < (long long)seg.begin || lpos + tl > seg_end || rpos < (long long)seg.begin || rpos + tr > seg_end) {
+    if (threadIdx.x == 0) atomicOr(s.err_flag, 2);  // inconsistent counts: write nothing
+  } else {
+    // run 1: chunks [0, tl*nc) to lpos; run 2: the rest to rpos
+    const int lchunks = tl * nc;
+    for (int q = threadIdx.x; q < total; q += kPartThreads) {
+      const uint4 w = *(const uint4*)(stage + 16 * (size_t)q);
+      uint8_t* d = q < lchunks ? dst + (size_t)lpos * stride + 16 * (size_t)q
+                               : dst + (size_t)rpos * stride + 16 * (size_t)(q - lchunks);
+      *(uint4*)d = w;
+    }
+    int32_t* didx = idx_of(s, it.dst_buf);
+    for (int r = threadIdx.x; r < rows; r += kPartThreads) didx[r < tl ? lpos + r : rpos + (r - tl)] = stage_id[r];
   }
   if (threadIdx.x == 0 && blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
 }
@@ -1189,14 +1214,12 @@ __device__ void search_categorical(const double* G, const double* H, const long 
     for (int i = 0; i < 8; ++i) out.cat_bits[i] = 0u;
     if (onehot) {
       out.num_cat = 1;
-      unsigned b = (unsigned)(best_threshold + offset);
-      out.cat_bits[b / 32] |= 1u << (b % 32);
+      bits_set(out.cat_bits, (unsigned)(best_threshold + offset));
     } else {
       out.num_cat = best_threshold + 1;
       for (int i = 0; i <= best_threshold; ++i) {
         int ps = best_dir == 1 ? i : used_bin - 1 - i;
-        unsigned b = (unsigned)(sorted[ps] + offset);
-        out.cat_bits[b / 32] |= 1u << (b % 32);
+        bits_set(out.cat_bits, (unsigned)(sorted[ps] + offset));
       }
     }
   }
@@ -1673,7 +1696,7 @@ __global__ void __launch_bounds__(256) k_leaf_of_pos(DevState s, const Segment* 
       else hi = mid - 1;
     }
     const int buf = sorted[lo].buf;
-    const int row = buf == 0 ? p : s.idx[buf][p];
+    const int row = buf == 0 ? p : idx_of(s, buf)[p];
     leaf_of[row] = sorted_leaf[lo];
   }
 }
@@ -1788,7 +1811,7 @@ __global__ void __launch_bounds__(128) k_renew_sums(DevState s, const Segment* _
   const Segment seg = segs[leaf];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float* src = wave == 0 ? g : h;
-  const int32_t* ix = seg.buf == 0 ? nullptr : s.idx[seg.buf];
+  const int32_t* ix = seg.buf == 0 ? nullptr : idx_of(s, seg.buf);
   const int nchunk = (seg.count + kRenewChunk - 1) / kRenewChunk;
   float r[kRenewChunk / 64];
   auto fetch = [&](int c) {
@@ -1834,7 +1857,7 @@ __global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const 
   const double v = vals[leaf];
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
     const int pos = seg.begin + p;
-    const int row = seg.buf == 0 ? pos : s.idx[seg.buf][pos];
+    const int row = seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos];
     score[row] += v;
   }
 }
@@ -1842,7 +1865,7 @@ __global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const 
 __global__ void k_leaf_rows(DevState s, Segment seg, int32_t* out) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
     const int pos = seg.begin + p;
-    out[p] = seg.buf == 0 ? pos : s.idx[seg.buf][pos];
+    out[p] = seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos];
   }
 }
 
@@ -1893,7 +1916,7 @@ __global__ void k_leaf_of(DevState s, const Segment* __restrict__ segs, int32_t*
   const Segment seg = segs[leaf];
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
     const int pos = seg.begin + p;
-    leaf_of[seg.buf == 0 ? pos : s.idx[seg.buf][pos]] = leaf;
+    leaf_of[seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos]] = leaf;
   }
 }
 
@@ -1942,16 +1965,15 @@ __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items,
     }
   }
   if (lane == 0) {
-    FoldOut r{};
     if (bf >= 0) {
-      r.o = out[bf];
+      dst->o = out[bf];
     } else {
-      r.o.gain = kMinScore;
-      r.o.feature = -1;
-      r.o.default_left = 1;
+      dst->o.gain = kMinScore;
+      dst->o.feature = -1;
+      dst->o.default_left = 1;
+      dst->o.splittable = 0;
     }
-    r.inner = bf;
-    *dst = r;
+    dst->inner = bf;
   }
 }
 
@@ -2019,14 +2041,17 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
   if (ntasks <= 0) return;
   const int R = partition_tile_rows(s.stride);
   if (R < 1) throw std::runtime_error("row record too large for the partition tile");
-  const size_t lds = 4 * (size_t)R + (size_t)R;
+  const size_t lds = partition_lds(R, s.stride);
+  if (lds > (size_t)kLdsBytes) throw std::runtime_error("partition tile exceeds the LDS budget");
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
+      allow_lds(k_partition<FORM_QUANT>);
       hipLaunchKernelGGL(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
                          nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
     default:
+      allow_lds(k_partition<FORM_F64>);
       hipLaunchKernelGGL(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
                          nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
