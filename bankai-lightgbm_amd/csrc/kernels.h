@@ -195,12 +195,12 @@ struct Segment {
   int buf, begin, count;
 };
 
-// A histogram build over a whole segment, split in ntasks row blocks of `chunk`
-// rows. The items of a launch own consecutive block ranges: item i owns blocks
-// first_task .. first_task + ntasks - 1 (first_task ascending, from 0).
+// A histogram build over a whole segment. The rows of the items of a launch are
+// concatenated: item i owns rows row0 .. row0 + seg.count - 1 (row0 ascending,
+// from 0, no empty items).
 struct HistItem {
   Segment seg;
-  int first_task, ntasks, chunk, pad;
+  int row0, pad;
 };
 
 // One leaf split: its rows move from seg (buffer seg.buf) to dst_buf, left rows
@@ -210,6 +210,12 @@ struct PartItem {
   int dst_buf, split_group, left_total, pad0;
   int first_task, ntasks, chunk, pad1;
   RuleDev rule;
+};
+
+// Partial histograms part_first .. part_first + nblk - 1 to be summed into the
+// pool slot `slot` (raw: before fix_histogram).
+struct ReduceItem {
+  int slot, part_first, nblk, pad;
 };
 
 // The folded best split of one leaf: the candidate and its inner feature (-1: none).
@@ -229,10 +235,11 @@ struct ScanItem {
 // Records of buffer 0 from column-major bins (group g at cols + g * col_stride).
 void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream);
 size_t hist_lds_bytes(int form, int max_tile_bins);
-// Blocks 0 .. ntasks-1 of the items write partial histograms task_base .. task_base+ntasks-1.
-void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int ntasks, int task_base,
-                 const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
-                 void* stream);
+// Block b covers concatenated rows [b * rows_per_block, (b + 1) * rows_per_block)
+// and writes one partial histogram per item it touches, from slot d_block_slot[b] on.
+void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
+                 int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
+                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
 // Rows per partition tile for a record stride (2048 16-byte chunks per tile).
 int partition_tile_rows(int stride);
 // Single-pass stable partition of every item: tasks are the items' tiles in
@@ -242,6 +249,10 @@ int partition_tile_rows(int stride);
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks,
                       unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
                       unsigned epoch, void* stream);
+// Sums the partial histograms of every item into its pool slot; the search then
+// reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
+// from its slot.
+void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream);
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
                  SplitOut* d_out, void* stream);
 // Best candidate of every (item, side) -> d_best[2 * item + side], and every

@@ -316,24 +316,30 @@ __device__ __forceinline__ void hist_chunk_regs(LdsHist<FORM>& L, const uint4& v
 // ---------------------------------------------------------------------------
 // Histogram of whole segments: the root, the smaller child of every split (its
 // rows are contiguous in the partition buffer), or a leaf whose sibling cannot
-// be subtracted. Grid: (tasks, group tiles); a task is one row block of one
-// item, and writes partial histogram `task_base + blockIdx.x`. Thread i always
-// reads chunk i % nc of the tile's chunk range, so its 16 bin offsets stay in
-// registers; consecutive threads read consecutive 16-byte chunks.
+// be subtracted. The rows of all items of a launch are concatenated (item i
+// owns rows row0 .. row0 + count - 1) and block b takes rows
+// [b * rows_per_block, (b + 1) * rows_per_block), so a block covers one item
+// or the end of one and the start of the next. It writes one partial
+// histogram per item it touches, to slots block_slot[b], block_slot[b] + 1, ...;
+// an item's partials are therefore consecutive slots. Grid: (blocks, group
+// tiles). Thread i always reads chunk i % nc of the tile's chunk range, so its
+// 16 bin offsets stay in registers; consecutive threads read consecutive
+// 16-byte chunks.
 // ---------------------------------------------------------------------------
 constexpr int kHistThreads = 512;
 constexpr int kHistRows = 2048;  // rows per payload staging tile
 
 template <int FORM>
 __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistItem* __restrict__ items, int nitems,
+                                                       int total_rows, int rows_per_block,
+                                                       const int* __restrict__ block_slot,
                                                        const int* __restrict__ group_used,
-                                                       const int* __restrict__ tile_first, int rows_per_flush,
-                                                       int task_base) {
+                                                       const int* __restrict__ tile_first, int rows_per_flush) {
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
-  const HistItem it = items[find_item(items, nitems, (int)blockIdx.x)];
-  const int blk = (int)blockIdx.x - it.first_task;
-  const Segment seg = it.seg;
+  const long long start = (long long)blockIdx.x * rows_per_block;
+  const long long end = min(start + (long long)rows_per_block, (long long)total_rows);
+  if (start >= end) return;  // block-uniform
   const int tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
@@ -343,14 +349,10 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   PT* pay = (PT*)(gbs + ((s.num_groups + 3) & ~3));
   init_gbs(s, gbs, g0, g1, bin_lo, group_used);
   L.zero(nb);
-  const long long seg_end = (long long)seg.begin + seg.count;
-  const long long cb = (long long)seg.begin + (long long)blk * it.chunk;
-  const long long ce = min(cb + (long long)it.chunk, seg_end);
   const int c0 = g0 >> 4, nc = ((g1 - 1) >> 4) + 1 - c0;
   const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
   const int rstep = tc / nc;
   const int my_c = c0 + (int)threadIdx.x % nc;
-  const uint8_t* src = rec_of(s, seg.buf) + 16 * my_c;
   __syncthreads();
   int off[16];
 #pragma unroll
@@ -358,40 +360,66 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
     const int g = 16 * my_c + j;
     off[j] = g < s.num_groups ? gbs[g] : -1;
   }
-  bool first = true;
-  long long since = 0;
-  for (long long t0 = cb; t0 < ce; t0 += kHistRows) {
-    const int rows = (int)min((long long)kHistRows, ce - t0);
-    for (int r = threadIdx.x; r < rows; r += kHistThreads) pay[r] = load_payload<FORM>(s, seg.buf, t0 + r);
-    __syncthreads();
-    if ((int)threadIdx.x < tc) {
-      const uint8_t* base = src + (size_t)t0 * s.stride;
-      for (int r0 = (int)threadIdx.x / nc; r0 < rows; r0 += 4 * rstep) {
-        uint4 v[4];
+  // last item starting at or before `start`
+  int i = 0;
+  {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].row0 <= start) lo = mid;
+      else hi = mid - 1;
+    }
+    i = lo;
+  }
+  int slot = block_slot[blockIdx.x];
+  for (long long x = start; x < end; ++i, ++slot) {
+    const HistItem it = items[i];
+    const Segment seg = it.seg;
+    const long long iend = min(end, (long long)it.row0 + seg.count);
+    const long long cb = (long long)seg.begin + (x - it.row0);
+    const long long ce = (long long)seg.begin + (iend - it.row0);
+    const uint8_t* src = rec_of(s, seg.buf) + 16 * my_c;
+    bool first = true;
+    long long since = 0;
+    for (long long t0 = cb; t0 < ce; t0 += kHistRows) {
+      const int rows = (int)min((long long)kHistRows, ce - t0);
+      for (int r = threadIdx.x; r < rows; r += kHistThreads) pay[r] = load_payload<FORM>(s, seg.buf, t0 + r);
+      __syncthreads();
+      if ((int)threadIdx.x < tc) {
+        const uint8_t* base = src + (size_t)t0 * s.stride;
+        for (int r0 = (int)threadIdx.x / nc; r0 < rows; r0 += 4 * rstep) {
+          uint4 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int r = r0 + u * rstep;
-          if (r < rows) v[u] = *(const uint4*)(base + (size_t)r * s.stride);
-        }
+          for (int u = 0; u < 4; ++u) {
+            const int r = r0 + u * rstep;
+            if (r < rows) v[u] = *(const uint4*)(base + (size_t)r * s.stride);
+          }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int r = r0 + u * rstep;
-          if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, pay[r]);
+          for (int u = 0; u < 4; ++u) {
+            const int r = r0 + u * rstep;
+            if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, pay[r]);
+          }
         }
       }
+      since += rows;
+      __syncthreads();
+      if (since >= rows_per_flush) {
+        L.flush(s, slot, bin_lo, nb, first);
+        __syncthreads();
+        L.zero(nb);
+        __syncthreads();
+        first = false;
+        since = 0;
+      }
     }
-    since += rows;
-    __syncthreads();
-    if (since >= rows_per_flush) {
-      L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
+    if (since > 0 || first) {
+      L.flush(s, slot, bin_lo, nb, first);
       __syncthreads();
       L.zero(nb);
       __syncthreads();
-      first = false;
-      since = 0;
     }
+    x = iend;
   }
-  if (since > 0 || first) L.flush(s, task_base + (int)blockIdx.x, bin_lo, nb, first);
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,60 +1263,27 @@ struct SliceLds {
   int used_bin;
 };
 
+// Raw histogram slice of one feature from a pool slot (summed by k_reduce).
+// A constant hessian counts rows in the hessian slot; it is scaled by
+// hessians[0] here, after the whole sum (histogram.rs:226-232, serial.rs:982-988).
 template <int FORM>
-__device__ void load_slice_from_partials(const DevState& s, const FeatDev& m, const LeafArgs& a, const ScanParams& p,
-                                         SliceLds& S) {
-  const size_t stride = (size_t)s.total_bins;
-  const size_t base = (size_t)a.part_first * stride;
+__device__ void load_slice_raw(const DevState& s, const FeatDev& m, int slot, const ScanParams& p, SliceLds& S) {
+  const double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
+  const long long* pooli = (const long long*)pool;
+  const int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
-      const unsigned long long* src = s.part_q + base + bin;
-      long long g[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
-      int j = 0;
-      for (; j + 4 <= a.nblk; j += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          long long gg, hh, cc;
-          qunpack(src[(size_t)(j + u) * stride], &gg, &hh, &cc);
-          g[u] += gg;
-          h[u] += hh;
-          c[u] += cc;
-        }
-      }
-      for (; j < a.nblk; ++j) {
-        long long gg, hh, cc;
-        qunpack(src[(size_t)j * stride], &gg, &hh, &cc);
-        g[0] += gg;
-        h[0] += hh;
-        c[0] += cc;
-      }
-      S.ig[i] = (g[0] + g[1]) + (g[2] + g[3]);
-      S.ih[i] = (h[0] + h[1]) + (h[2] + h[3]);
-      S.c[i] = (int)((c[0] + c[1]) + (c[2] + c[3]));
+      S.ig[i] = pooli[2 * bin];
+      S.ih[i] = pooli[2 * bin + 1];
     } else {
-      const double* sg = s.part_g + base + bin;
-      const double* sh = s.part_h + base + bin;
-      const unsigned* sc = s.part_c + base + bin;
-      double g = 0.0, h = 0.0;
-      unsigned c = 0;
-      for (int j = 0; j < a.nblk; ++j) {
-        g += sg[(size_t)j * stride];
-        h += sh[(size_t)j * stride];
-        c += sc[(size_t)j * stride];
-      }
-      if (p.const_hess) h *= p.h0;
-      S.g[i] = g;
-      S.h[i] = h;
-      S.c[i] = (int)c;
+      S.g[i] = pool[2 * bin];
+      S.h[i] = p.const_hess ? pool[2 * bin + 1] * p.h0 : pool[2 * bin + 1];
     }
+    S.c[i] = cnt[bin];
   }
 }
 
-// fix_slice / fix_slice_int (serial.rs:2205-2238) plus the count channel.
-// Integer sums are exact in any order, so they are reduced by the block; the
-// f64 form keeps the reference's sequential subtraction order on one lane.
-// Must be called by the whole block (it synchronizes).
 template <int FORM>
 __device__ void fix_slice(const FeatDev& m, const LeafArgs& a, SliceLds& S, long long* red) {
   const int mfb = m.mfb, nb = m.num_bin;
@@ -1504,9 +1499,9 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
   const int has_large = item.has_large, use_subtract = item.use_subtract;
   if (!used[f]) return;
   const FeatDev m = s.feats[f];
-  // smaller leaf: reduce the row-block partials, fix, store
+  // smaller leaf: raw histogram from its slot, fix, store
   __shared__ long long red[3 * (kScanThreads / 64)];
-  load_slice_from_partials<FORM>(s, m, small, p, S[0]);
+  load_slice_raw<FORM>(s, m, small.slot, p, S[0]);
   __syncthreads();
   fix_slice<FORM>(m, small, S[0], red);
   store_slice<FORM>(s, m, small.slot, S[0]);
@@ -1515,7 +1510,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
       subtract_slice<FORM>(s, m, large.slot, S[0], S[1]);
       __syncthreads();
     } else {
-      load_slice_from_partials<FORM>(s, m, large, p, S[1]);
+      load_slice_raw<FORM>(s, m, large.slot, p, S[1]);
       __syncthreads();
       fix_slice<FORM>(m, large, S[1], red);
     }
@@ -1977,6 +1972,74 @@ __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sum of the partial histograms of every item into its pool slot. A block
+// covers 32 bins of one item with 8 lanes per bin, each lane summing every 8th
+// partial (32 consecutive bins of one partial are one coalesced read); the 8
+// sums are then added in lane order, so float forms reduce in a fixed order.
+// ---------------------------------------------------------------------------
+template <int FORM>
+__global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __restrict__ items) {
+  __shared__ long long ra[8][32], rb[8][32];
+  __shared__ double da[8][32], db[8][32];
+  __shared__ long long rc[8][32];
+  const ReduceItem it = items[blockIdx.y];
+  const int lb = threadIdx.x & 31, sp = threadIdx.x >> 5;
+  const int bin = blockIdx.x * 32 + lb;
+  const size_t stride = (size_t)s.total_bins;
+  long long g = 0, h = 0, c = 0;
+  double fg = 0.0, fh = 0.0;
+  if (bin < s.total_bins) {
+    const size_t base = (size_t)it.part_first * stride + bin;
+    if (FORM == FORM_QUANT) {
+      for (int j = sp; j < it.nblk; j += 8) {
+        long long gg, hh, cc;
+        qunpack(s.part_q[base + (size_t)j * stride], &gg, &hh, &cc);
+        g += gg;
+        h += hh;
+        c += cc;
+      }
+    } else {
+      for (int j = sp; j < it.nblk; j += 8) {
+        fg += s.part_g[base + (size_t)j * stride];
+        fh += s.part_h[base + (size_t)j * stride];
+        c += s.part_c[base + (size_t)j * stride];
+      }
+    }
+  }
+  ra[sp][lb] = g;
+  rb[sp][lb] = h;
+  rc[sp][lb] = c;
+  da[sp][lb] = fg;
+  db[sp][lb] = fh;
+  __syncthreads();
+  if (sp == 0 && bin < s.total_bins) {
+    double* pool = s.pool + ((size_t)it.slot * s.total_bins) * 2;
+    int* cnt = s.pool_cnt + (size_t)it.slot * s.total_bins;
+    long long tc = 0;
+    if (FORM == FORM_QUANT) {
+      long long tg = 0, th = 0;
+      for (int k = 0; k < 8; ++k) {
+        tg += ra[k][lb];
+        th += rb[k][lb];
+        tc += rc[k][lb];
+      }
+      ((long long*)pool)[2 * bin] = tg;
+      ((long long*)pool)[2 * bin + 1] = th;
+    } else {
+      double tg = 0.0, th = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        tg += da[k][lb];
+        th += db[k][lb];
+        tc += rc[k][lb];
+      }
+      pool[2 * bin] = tg;
+      pool[2 * bin + 1] = th;
+    }
+    cnt[bin] = (int)tc;
+  }
+}
+
 inline int grid_for(long long n, int threads, int cap = 4096) {
   long long b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -2004,11 +2067,11 @@ size_t hist_lds_bytes(int form, int max_tile_bins) {
   return ((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15;
 }
 
-void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int ntasks, int task_base,
-                 const int* d_group_used, int max_tile_bins, const int* d_tile_first, int ntiles, int rows_per_flush,
-                 void* stream) {
-  if (ntasks <= 0) return;
-  dim3 grid((unsigned)ntasks, (unsigned)ntiles);
+void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
+                 int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
+                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
+  if (nblocks <= 0 || total_rows <= 0) return;
+  dim3 grid((unsigned)nblocks, (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
   const size_t pay = s.form == FORM_QUANT ? 2 : 8;
   const size_t lds = hist_lds_bytes(s.form, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
@@ -2016,18 +2079,18 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int nta
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
-      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
-                         d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
-      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
-                         d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
-      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, d_group_used,
-                         d_tile_first, rows_per_flush, task_base);
+      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -2065,6 +2128,24 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
   allow_lds(k_to_records);
   const unsigned blocks = (unsigned)((s.num_data + kRecRows - 1) / kRecRows);
   hipLaunchKernelGGL(k_to_records, dim3(blocks), dim3(256), lds, (hipStream_t)stream, s, cols, col_stride);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream) {
+  if (nitems <= 0) return;
+  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)nitems);
+  hipStream_t st = (hipStream_t)stream;
+  switch (s.form) {
+    case FORM_QUANT:
+      hipLaunchKernelGGL(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items);
+      break;
+    case FORM_F64:
+      hipLaunchKernelGGL(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items);
+      break;
+    default:
+      hipLaunchKernelGGL(k_reduce<FORM_F32C>, grid, dim3(256), 0, st, s, d_items);
+      break;
+  }
   BK_HIP(hipGetLastError());
 }
 

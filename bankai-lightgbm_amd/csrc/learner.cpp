@@ -393,8 +393,10 @@ struct Learner {
   FoldOut* d_fold = nullptr;
   uint8_t* d_flags = nullptr;
   HistItem* d_hitems = nullptr;
+  int* d_hblock = nullptr;
   PartItem* d_pitems = nullptr;
   ScanItem* d_sitems = nullptr;
+  ReduceItem* d_ritems = nullptr;
   unsigned* d_max4 = nullptr;
   long long* d_sums3 = nullptr;
   double* d_sum_partials = nullptr;
@@ -416,8 +418,12 @@ struct Learner {
   FoldOut* h_fold = nullptr;
   uint8_t* h_flags = nullptr;
   HistItem* h_hitems = nullptr;
+  int* h_hblock = nullptr;
+  std::vector<Segment> hist_seg;
+  std::vector<int> hist_first, hist_count;
   PartItem* h_pitems = nullptr;
   ScanItem* h_sitems = nullptr;
+  ReduceItem* h_ritems = nullptr;
   int* h_used = nullptr;
   int* h_group_used = nullptr;
   Segment* h_segs = nullptr;
@@ -447,7 +453,6 @@ struct Learner {
   // batched form
   std::vector<Node> nodes;
   std::vector<int> leaf_node;
-  std::vector<Segment> hist_segs;
 
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
@@ -477,13 +482,13 @@ struct Learner {
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
                     (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
-                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_fold, (void*)d_flags, (void*)d_hitems,
-                    (void*)d_pitems, (void*)d_sitems, (void*)d_max4, (void*)d_sums3,
+                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_fold, (void*)d_flags, (void*)d_hitems, (void*)d_hblock,
+                    (void*)d_pitems, (void*)d_sitems, (void*)d_ritems, (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
-    for (void* p : {(void*)h_fold, (void*)h_flags, (void*)h_hitems, (void*)h_pitems, (void*)h_sitems, (void*)h_used,
+    for (void* p : {(void*)h_fold, (void*)h_flags, (void*)h_hitems, (void*)h_hblock, (void*)h_pitems, (void*)h_sitems, (void*)h_ritems, (void*)h_used,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
                     (void*)h_sorted_leaf})
       fh(p);
@@ -491,15 +496,6 @@ struct Learner {
     staged_ev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
-  }
-
-  // Row blocks of one item: few large blocks for a lone item, fixed-size blocks in a batch.
-  int chunk_for(int count, bool alone) const {
-    const int min_chunk = 4 * kTileRows;
-    long long chunk = min_chunk;
-    if (alone) chunk = std::max<long long>(((long long)count + kMaxRowBlocks - 1) / kMaxRowBlocks, min_chunk);
-    chunk = (chunk + kTileRows - 1) / kTileRows * kTileRows;
-    return (int)chunk;
   }
 
   template <typename T>
@@ -580,7 +576,9 @@ struct Learner {
     const int pay_bytes = form == FORM_QUANT ? 2 : 8;
     pay_off = (G + pay_bytes - 1) / pay_bytes * pay_bytes;
     stride = (pay_off + pay_bytes + 15) / 16 * 16;
-    // LDS histogram tiles of whole groups
+    // LDS histogram tiles of whole groups, aligned to the 16-group chunks of a
+    // record (a histogram block reads one chunk per row): smaller tiles mean
+    // more blocks and more resident waves per CU for the same bytes read
     const int per_bin = form == FORM_QUANT ? 8 : 20;
     const int tile_cap = kHistLdsBytes / per_bin;
     tile_first.assign(1, 0);
@@ -589,7 +587,7 @@ struct Learner {
       int cur = 0;
       for (int g = 0; g < G; ++g) {
         int nb = v->group_num_total_bin[g];
-        if (cur + nb > tile_cap) {
+        if (cur > 0 && (cur + nb > tile_cap || g % 16 == 0)) {
           tile_first.push_back(g);
           max_tile_bins = std::max(max_tile_bins, cur);
           cur = 0;
@@ -619,7 +617,7 @@ struct Learner {
     max_items = P.num_leaves + 1;
     part_tile = partition_tile_rows(stride);
     max_part_tiles = N / part_tile + max_items + 8;
-    max_partials = N / (4 * kTileRows) + 2 * max_items + 2 * kMaxRowBlocks + 8;
+    max_partials = kHistMaxBlocks + 2 * max_items + 8;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming));
@@ -663,8 +661,10 @@ struct Learner {
     d_group_used = dmalloc<int>((size_t)G);
     d_out = dmalloc<SplitOut>((size_t)F * 2 * max_items);
     d_hitems = dmalloc<HistItem>((size_t)max_items);
+    d_hblock = dmalloc<int>((size_t)kHistMaxBlocks);
     d_pitems = dmalloc<PartItem>((size_t)max_items);
     d_sitems = dmalloc<ScanItem>((size_t)max_items);
+    d_ritems = dmalloc<ReduceItem>((size_t)2 * max_items);
     d_max4 = dmalloc<unsigned>(4);
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
@@ -677,8 +677,10 @@ struct Learner {
     pinned(&h_fold, (size_t)2 * max_items);
     pinned(&h_flags, (size_t)2 * max_items * F);
     pinned(&h_hitems, (size_t)max_items);
+    pinned(&h_hblock, (size_t)kHistMaxBlocks);
     pinned(&h_pitems, (size_t)max_items);
     pinned(&h_sitems, (size_t)max_items);
+    pinned(&h_ritems, (size_t)2 * max_items);
     pinned(&h_used, (size_t)F * max_items);
     pinned(&h_group_used, (size_t)G);
     pinned(&h_segs, (size_t)P.num_leaves);
@@ -911,13 +913,52 @@ struct Learner {
   //
   // Histogram of the staged items (blocks 0 .. ntasks-1); block j writes
   // partial histogram partial_base + j.
-  void run_hist(int nitems, int ntasks, int partial_base, double rows, bool root) {
-    if (ntasks <= 0) return;
-    if (partial_base + ntasks > max_partials) throw std::runtime_error("partial buffer too small");
-    BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
+  // Histograms of the segments hist_seg[0..n): their rows are concatenated and
+  // cut into equal row ranges, one per block (at least kHistMinRows rows, at most
+  // kHistMaxBlocks blocks); a block writes one partial per segment it touches.
+  // Segment i's partials are hist_first[i] .. hist_first[i] + hist_count[i] - 1.
+  static constexpr int kHistMaxBlocks = 1024;
+  static constexpr int kHistMinRows = 4096;
+  void run_hist(int n, double rows, bool root) {
+    hist_first.assign((size_t)n, 0);
+    hist_count.assign((size_t)n, 0);
+    std::vector<int> item_of;  // launch item -> segment
+    long long total = 0;
+    int ni = 0;
+    for (int i = 0; i < n; ++i) {
+      if (hist_seg[(size_t)i].count <= 0) continue;
+      h_hitems[ni] = HistItem{hist_seg[(size_t)i], (int)total, 0};
+      item_of.push_back(i);
+      total += hist_seg[(size_t)i].count;
+      ++ni;
+    }
+    if (ni == 0) return;
+    int nblocks = (int)std::min<long long>(kHistMaxBlocks, std::max<long long>(1, total / kHistMinRows));
+    const int rpb = (int)((total + nblocks - 1) / nblocks);
+    nblocks = (int)((total + rpb - 1) / rpb);
+    // slots: block b gets one per item it touches
+    std::vector<int> first_item((size_t)nblocks);
+    int slot = 0, it = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      const long long s0 = (long long)b * rpb, e0 = std::min<long long>(s0 + rpb, total);
+      while (h_hitems[it].row0 + h_hitems[it].seg.count <= s0) ++it;
+      int last = it;
+      while (h_hitems[last].row0 + h_hitems[last].seg.count < e0) ++last;
+      first_item[(size_t)b] = it;
+      h_hblock[b] = slot;
+      slot += last - it + 1;
+    }
+    if (slot > max_partials) throw std::runtime_error("partial buffer too small");
+    for (int k = 0; k < ni; ++k) {
+      const int b0 = h_hitems[k].row0 / rpb, b1 = (h_hitems[k].row0 + h_hitems[k].seg.count - 1) / rpb;
+      hist_first[(size_t)item_of[(size_t)k]] = h_hblock[b0] + (k - first_item[(size_t)b0]);
+      hist_count[(size_t)item_of[(size_t)k]] = b1 - b0 + 1;
+    }
+    BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)ni, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_hblock, h_hblock, sizeof(int) * (size_t)nblocks, hipMemcpyHostToDevice, stream));
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
     timed(root ? 0 : 1, hb, rows * (stride + (root ? payload_bytes() : 0)), [&] {
-      launch_hist(ds, d_hitems, nitems, ntasks, partial_base, d_group_used, max_tile_bins, d_tile_first,
+      launch_hist(ds, d_hitems, ni, (int)total, nblocks, rpb, d_hblock, d_group_used, max_tile_bins, d_tile_first,
                   (int)tile_first.size() - 1, rows_per_flush, stream);
     });
     staged();
@@ -942,9 +983,18 @@ struct Learner {
   void run_scan(int nitems) {
     if (nitems <= 0) return;
     ScanParams sp = scan_params();
+    int nr = 0;
+    for (int k = 0; k < nitems; ++k) {
+      const ScanItem& si = h_sitems[k];
+      h_ritems[nr++] = ReduceItem{si.small.slot, si.small.part_first, si.small.nblk, 0};
+      if (si.has_large && !si.use_subtract)
+        h_ritems[nr++] = ReduceItem{si.large.slot, si.large.part_first, si.large.nblk, 0};
+    }
     BK_HIPC(hipMemcpyAsync(d_sitems, h_sitems, sizeof(ScanItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
+    BK_HIPC(hipMemcpyAsync(d_ritems, h_ritems, sizeof(ReduceItem) * (size_t)nr, hipMemcpyHostToDevice, stream));
     BK_HIPC(hipMemcpyAsync(d_used, h_used, sizeof(int) * (size_t)F * nitems, hipMemcpyHostToDevice, stream));
     timed(3, 0.0, 0.0, [&] {
+      launch_reduce(ds, d_ritems, nr, stream);
       launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream);
       launch_fold(ds, d_sitems, nitems, d_out, d_used, d_fold, d_flags, stream);
     });
@@ -1079,17 +1129,14 @@ struct Learner {
     t.set_leaf_output(0, leaf_output_root(P, smaller.sum_g, smaller.sum_h));
     if (before_find_best_split(t, 0, -1)) {
       compute_used();
-      int nt = 0;
-      h_hitems[0] = HistItem{segs[0], 0, 0, chunk_for(N, true), 0};
-      h_hitems[0].first_task = add_tasks(N, h_hitems[0].chunk, nt);
-      h_hitems[0].ntasks = nt;
-      run_hist(1, nt, 0, N, true);
+      hist_seg.assign(1, segs[0]);
+      run_hist(1, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, smaller, (int)smaller_slot);
-      si.small.part_first = 0;
-      si.small.nblk = nt;
+      si.small.part_first = hist_first[0];
+      si.small.nblk = hist_count[0];
       run_scan(1);
       fold_sequential(false);
     }
@@ -1139,30 +1186,23 @@ struct Learner {
         si.has_large = larger.leaf >= 0 && larger_slot >= 0;
         si.use_subtract = use_subtract;
         const Segment sseg = segs[(size_t)smaller.leaf];
-        int ht = 0;
-        h_hitems[0] = HistItem{sseg, 0, 0, chunk_for(sseg.count, true), 0};
-        h_hitems[0].first_task = add_tasks(sseg.count, h_hitems[0].chunk, ht);
-        h_hitems[0].ntasks = ht;
-        si.small.part_first = 0;
-        si.small.nblk = ht;
-        int nh = 1;
+        hist_seg.assign(1, sseg);
         double hrows = sseg.count;
-        if (si.has_large) {
-          si.large = leaf_args(t, larger, (int)larger_slot);
-          if (!use_subtract) {
-            const Segment lseg = segs[(size_t)larger.leaf];
-            const int lfirst = ht;
-            h_hitems[1] = HistItem{lseg, 0, 0, chunk_for(lseg.count, true), 0};
-            h_hitems[1].first_task = add_tasks(lseg.count, h_hitems[1].chunk, ht);
-            h_hitems[1].ntasks = ht - lfirst;
-            stats.committed_hist_rows += lseg.count;
-            si.large.part_first = lfirst;
-            si.large.nblk = ht - lfirst;
-            nh = 2;
-            hrows += lseg.count;
-          }
+        const bool rebuild = si.has_large && !use_subtract;
+        if (si.has_large) si.large = leaf_args(t, larger, (int)larger_slot);
+        if (rebuild) {
+          const Segment lseg = segs[(size_t)larger.leaf];
+          hist_seg.push_back(lseg);
+          stats.committed_hist_rows += lseg.count;
+          hrows += lseg.count;
         }
-        run_hist(nh, ht, 0, hrows, false);
+        run_hist((int)hist_seg.size(), hrows, false);
+        si.small.part_first = hist_first[0];
+        si.small.nblk = hist_count[0];
+        if (rebuild) {
+          si.large.part_first = hist_first[1];
+          si.large.nblk = hist_count[1];
+        }
         run_scan(1);
         fold_sequential(si.has_large != 0);
       }
@@ -1192,17 +1232,14 @@ struct Learner {
     // the root search (serial.rs:591-604 with the root's before_find_best_split)
     if (N >= P.min_data_in_leaf * 2) {
       for (int f = 0; f < F; ++f) h_used[f] = 1;
-      int nt = 0;
-      h_hitems[0] = HistItem{nodes[0].seg, 0, 0, chunk_for(N, true), 0};
-      h_hitems[0].first_task = add_tasks(N, h_hitems[0].chunk, nt);
-      h_hitems[0].ntasks = nt;
-      run_hist(1, nt, 0, N, true);
+      hist_seg.assign(1, nodes[0].seg);
+      run_hist(1, N, true);
       stats.committed_hist_rows += N;
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, root, 0);
-      si.small.part_first = 0;
-      si.small.nblk = nt;
+      si.small.part_first = hist_first[0];
+      si.small.nblk = hist_count[0];
       si.used_off = 0;
       si.out_off = 0;
       run_scan(1);
@@ -1253,7 +1290,7 @@ struct Learner {
       if (order.size() > budget) order.resize(budget);
       int ni = 0, nt = 0, ns = 0;
       double part_rows = 0.0, hist_rows = 0.0;
-      hist_segs.clear();
+      hist_seg.clear();
       for (int l : order) {
         const int x = leaf_node[(size_t)l];
         Node& X = nodes[(size_t)x];
@@ -1296,7 +1333,7 @@ struct Learner {
           si = ScanItem{};
           // a child's parent output is its own output from the split (never the root's formula)
           si.small = leaf_args_with(sm.ls, sm.slot, sm.ls.weight);
-          hist_segs.push_back(sm.seg);
+          hist_seg.push_back(sm.seg);
           si.large = leaf_args_with(lg.ls, lg.slot, lg.ls.weight);
           si.has_large = 1;
           si.use_subtract = 1;
@@ -1316,17 +1353,11 @@ struct Learner {
       run_partition(ni, nt, part_rows);
       if (ns > 0) {
         // histograms of the smaller children, now contiguous in their buffers
-        int ht = 0;
+        run_hist(ns, hist_rows, false);
         for (int k2 = 0; k2 < ns; ++k2) {
-          const Segment sg = hist_segs[(size_t)k2];
-          HistItem& hi = h_hitems[k2];
-          hi = HistItem{sg, 0, 0, chunk_for(sg.count, false), 0};
-          hi.first_task = add_tasks(sg.count, hi.chunk, ht);
-          hi.ntasks = ht - hi.first_task;
-          h_sitems[k2].small.part_first = hi.first_task;
-          h_sitems[k2].small.nblk = hi.ntasks;
+          h_sitems[k2].small.part_first = hist_first[(size_t)k2];
+          h_sitems[k2].small.nblk = hist_count[(size_t)k2];
         }
-        run_hist(ns, ht, 0, hist_rows, false);
         run_scan(ns);
       }
       // fold the children's candidates
