@@ -384,9 +384,16 @@ class Learner:
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "speculative_waste": s.speculative_waste}
 
-    def profile(self, enable=True):
-        """Turns per-kernel HIP-event timing on (and resets the counters)."""
-        _check(self._lib.bk_learner_profile(self.handle, int(enable)))
+    def profile(self, kinds=True):
+        """Per-kernel HIP-event timing: True for every kind, False for none, or a
+        list of names from KERNEL_KINDS. Resets the counters."""
+        if kinds is True:
+            mask = (1 << len(KERNEL_KINDS)) - 1
+        elif not kinds:
+            mask = 0
+        else:
+            mask = sum(1 << KERNEL_KINDS.index(k) for k in kinds)
+        _check(self._lib.bk_learner_profile(self.handle, mask))
 
     def kernel_stats(self):
         out = {}

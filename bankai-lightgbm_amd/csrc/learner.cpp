@@ -280,6 +280,7 @@ T* dmalloc(size_t n) {
 // Event pairs around launches on the learner's stream, resolved at the end of a tree.
 struct Prof {
   bool on = false;
+  unsigned mask = 0;  // bit k: time launches of kind k
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Rec {
@@ -294,7 +295,8 @@ struct Prof {
   hipEvent_t ev(size_t i) {
     while (pool.size() <= i) {
       hipEvent_t e;
-      BK_HIPC(hipEventCreate(&e));
+      // timing only: no system-scope fence (a cache writeback per record)
+      BK_HIPC(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       pool.push_back(e);
     }
     return pool[i];
@@ -387,6 +389,20 @@ struct Learner {
   unsigned long long* d_ticket = nullptr;
   int* d_err = nullptr;
   int* d_tile_first = nullptr;
+  // staging arenas: every round's items go up in one copy, results come back in one
+  uint8_t* d_arena = nullptr;
+  uint8_t* h_arena = nullptr;
+  uint8_t* d_back = nullptr;
+  uint8_t* h_back = nullptr;
+  size_t arena_bytes = 0, arena_cur = 0, arena_sent = 0;
+  size_t off_fold = 0;
+  // host-side staging of a round (packed into the arena when uploaded)
+  std::vector<PartItem> v_pitems;
+  std::vector<HistItem> v_hitems;
+  std::vector<int> v_hblock;
+  std::vector<ScanItem> v_sitems;
+  std::vector<ReduceItem> v_ritems;
+  std::vector<int> v_used;
   int* d_used = nullptr;        // per item x feature
   int* d_group_used = nullptr;  // per group
   SplitOut* d_out = nullptr;
@@ -457,7 +473,7 @@ struct Learner {
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
   void timed(int kind, double algo, double moved, Fn&& launch) {
-    if (!prof.on) {
+    if (!((prof.mask >> kind) & 1u)) {
       launch();
       return;
     }
@@ -482,13 +498,13 @@ struct Learner {
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
                     (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
-                    (void*)d_gh0, (void*)d_used, (void*)d_group_used, (void*)d_out, (void*)d_fold, (void*)d_flags, (void*)d_hitems, (void*)d_hblock,
-                    (void*)d_pitems, (void*)d_sitems, (void*)d_ritems, (void*)d_max4, (void*)d_sums3,
+                    (void*)d_gh0, (void*)d_arena, (void*)d_back, (void*)d_group_used, (void*)d_out,
+                    (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
-    for (void* p : {(void*)h_fold, (void*)h_flags, (void*)h_hitems, (void*)h_hblock, (void*)h_pitems, (void*)h_sitems, (void*)h_ritems, (void*)h_used,
+    for (void* p : {(void*)h_arena, (void*)h_back,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
                     (void*)h_sorted_leaf})
       fh(p);
@@ -620,7 +636,8 @@ struct Learner {
     max_partials = kHistMaxBlocks + 2 * max_items + 8;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming));
+    // marks the end of staging transfers, which only read pinned host memory
+    BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming | hipEventDisableSystemFence));
     for (int b = 0; b < 3; ++b) {
       d_rec[b] = dmalloc<uint8_t>((size_t)npad * stride);
       if (b > 0) d_idx[b] = dmalloc<int32_t>((size_t)npad);
@@ -657,14 +674,38 @@ struct Learner {
     d_tile_first = dmalloc<int>(tile_first.size());
     BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
                            stream));
-    d_used = dmalloc<int>((size_t)F * max_items);
     d_group_used = dmalloc<int>((size_t)G);
     d_out = dmalloc<SplitOut>((size_t)F * 2 * max_items);
-    d_hitems = dmalloc<HistItem>((size_t)max_items);
-    d_hblock = dmalloc<int>((size_t)kHistMaxBlocks);
-    d_pitems = dmalloc<PartItem>((size_t)max_items);
-    d_sitems = dmalloc<ScanItem>((size_t)max_items);
-    d_ritems = dmalloc<ReduceItem>((size_t)2 * max_items);
+    {
+      // a round's items are packed into one pinned arena and sent in at most two
+      // transfers; results come back in one
+      v_pitems.resize((size_t)max_items);
+      v_hitems.resize((size_t)max_items);
+      v_hblock.resize((size_t)kHistMaxBlocks);
+      v_sitems.resize((size_t)max_items);
+      v_ritems.resize((size_t)2 * max_items);
+      v_used.resize((size_t)F * max_items);
+      h_pitems = v_pitems.data();
+      h_hitems = v_hitems.data();
+      h_hblock = v_hblock.data();
+      h_sitems = v_sitems.data();
+      h_ritems = v_ritems.data();
+      h_used = v_used.data();
+      auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+      arena_bytes = up(sizeof(PartItem) * (size_t)max_items) + up(sizeof(HistItem) * (size_t)max_items) +
+                    up(sizeof(int) * (size_t)kHistMaxBlocks) + up(sizeof(ScanItem) * (size_t)max_items) +
+                    up(sizeof(ReduceItem) * 2 * (size_t)max_items) + up(sizeof(int) * (size_t)F * max_items);
+      d_arena = dmalloc<uint8_t>(arena_bytes);
+      pinned(&h_arena, arena_bytes);
+      off_fold = up((size_t)2 * max_items * F);
+      const size_t back = off_fold + sizeof(FoldOut) * 2 * (size_t)max_items;
+      d_back = dmalloc<uint8_t>(back);
+      pinned(&h_back, back);
+      d_flags = d_back;
+      h_flags = h_back;
+      d_fold = (FoldOut*)(d_back + off_fold);
+      h_fold = (FoldOut*)(h_back + off_fold);
+    }
     d_max4 = dmalloc<unsigned>(4);
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
@@ -672,16 +713,6 @@ struct Learner {
     d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
     d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
-    d_fold = dmalloc<FoldOut>((size_t)2 * max_items);
-    d_flags = dmalloc<uint8_t>((size_t)2 * max_items * F);
-    pinned(&h_fold, (size_t)2 * max_items);
-    pinned(&h_flags, (size_t)2 * max_items * F);
-    pinned(&h_hitems, (size_t)max_items);
-    pinned(&h_hblock, (size_t)kHistMaxBlocks);
-    pinned(&h_pitems, (size_t)max_items);
-    pinned(&h_sitems, (size_t)max_items);
-    pinned(&h_ritems, (size_t)2 * max_items);
-    pinned(&h_used, (size_t)F * max_items);
     pinned(&h_group_used, (size_t)G);
     pinned(&h_segs, (size_t)P.num_leaves);
     pinned(&h_order, (size_t)P.num_leaves);
@@ -912,35 +943,80 @@ struct Learner {
   // reads and writes a 4-byte row index.
   //
   // Histogram of the staged items (blocks 0 .. ntasks-1); block j writes
-  // partial histogram partial_base + j.
+  // ---- staging: a round's items are written to the host arena, marked, and
+  // copied to the device in one transfer before the round's launches ----------
+  struct HistPlan {
+    int ni = 0, nblocks = 0, rpb = 0;
+    long long total = 0;
+  };
+
+  // starts a round's uploads: the previous round's transfers have completed
+  void begin_uploads() {
+    stage_wait();
+    arena_cur = arena_sent = 0;
+  }
+
+  // packs n elements into the arena; returns where they will be on the device
+  template <typename T>
+  T* put(const T* src, size_t n) {
+    const size_t off = (arena_cur + 255) / 256 * 256;
+    if (off + sizeof(T) * n > arena_bytes) throw std::runtime_error("staging arena too small");
+    std::memcpy(h_arena + off, src, sizeof(T) * n);
+    arena_cur = off + sizeof(T) * n;
+    return (T*)(d_arena + off);
+  }
+
+  // sends what was packed since the last send
+  void send() {
+    if (arena_cur > arena_sent) {
+      BK_HIPC(hipMemcpyAsync(d_arena + arena_sent, h_arena + arena_sent, arena_cur - arena_sent,
+                             hipMemcpyHostToDevice, stream));
+      staged();
+    }
+    arena_sent = arena_cur;
+  }
+
+  void upload_partition(int nitems) {
+    d_pitems = put(h_pitems, (size_t)nitems);
+    send();
+  }
+
+  void upload_search(const HistPlan& hp, int nitems, int nr) {
+    d_hitems = put(h_hitems, (size_t)hp.ni);
+    d_hblock = put(h_hblock, (size_t)hp.nblocks);
+    d_sitems = put(h_sitems, (size_t)nitems);
+    d_ritems = put(h_ritems, (size_t)nr);
+    d_used = put(h_used, (size_t)F * nitems);
+    send();
+  }
+
   // Histograms of the segments hist_seg[0..n): their rows are concatenated and
   // cut into equal row ranges, one per block (at least kHistMinRows rows, at most
   // kHistMaxBlocks blocks); a block writes one partial per segment it touches.
   // Segment i's partials are hist_first[i] .. hist_first[i] + hist_count[i] - 1.
   static constexpr int kHistMaxBlocks = 1024;
   static constexpr int kHistMinRows = 4096;
-  void run_hist(int n, double rows, bool root) {
+  HistPlan plan_hist(int n) {
+    HistPlan hp;
     hist_first.assign((size_t)n, 0);
     hist_count.assign((size_t)n, 0);
     std::vector<int> item_of;  // launch item -> segment
-    long long total = 0;
-    int ni = 0;
     for (int i = 0; i < n; ++i) {
       if (hist_seg[(size_t)i].count <= 0) continue;
-      h_hitems[ni] = HistItem{hist_seg[(size_t)i], (int)total, 0};
+      h_hitems[hp.ni] = HistItem{hist_seg[(size_t)i], (int)hp.total, 0};
       item_of.push_back(i);
-      total += hist_seg[(size_t)i].count;
-      ++ni;
+      hp.total += hist_seg[(size_t)i].count;
+      ++hp.ni;
     }
-    if (ni == 0) return;
-    int nblocks = (int)std::min<long long>(kHistMaxBlocks, std::max<long long>(1, total / kHistMinRows));
-    const int rpb = (int)((total + nblocks - 1) / nblocks);
-    nblocks = (int)((total + rpb - 1) / rpb);
+    if (hp.ni == 0) return hp;
+    int nblocks = (int)std::min<long long>(kHistMaxBlocks, std::max<long long>(1, hp.total / kHistMinRows));
+    hp.rpb = (int)((hp.total + nblocks - 1) / nblocks);
+    hp.nblocks = (int)((hp.total + hp.rpb - 1) / hp.rpb);
     // slots: block b gets one per item it touches
-    std::vector<int> first_item((size_t)nblocks);
+    std::vector<int> first_item((size_t)hp.nblocks);
     int slot = 0, it = 0;
-    for (int b = 0; b < nblocks; ++b) {
-      const long long s0 = (long long)b * rpb, e0 = std::min<long long>(s0 + rpb, total);
+    for (int b = 0; b < hp.nblocks; ++b) {
+      const long long s0 = (long long)b * hp.rpb, e0 = std::min<long long>(s0 + hp.rpb, hp.total);
       while (h_hitems[it].row0 + h_hitems[it].seg.count <= s0) ++it;
       int last = it;
       while (h_hitems[last].row0 + h_hitems[last].seg.count < e0) ++last;
@@ -949,40 +1025,41 @@ struct Learner {
       slot += last - it + 1;
     }
     if (slot > max_partials) throw std::runtime_error("partial buffer too small");
-    for (int k = 0; k < ni; ++k) {
-      const int b0 = h_hitems[k].row0 / rpb, b1 = (h_hitems[k].row0 + h_hitems[k].seg.count - 1) / rpb;
+    for (int k = 0; k < hp.ni; ++k) {
+      const int b0 = h_hitems[k].row0 / hp.rpb, b1 = (h_hitems[k].row0 + h_hitems[k].seg.count - 1) / hp.rpb;
       hist_first[(size_t)item_of[(size_t)k]] = h_hblock[b0] + (k - first_item[(size_t)b0]);
       hist_count[(size_t)item_of[(size_t)k]] = b1 - b0 + 1;
     }
-    BK_HIPC(hipMemcpyAsync(d_hitems, h_hitems, sizeof(HistItem) * (size_t)ni, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_hblock, h_hblock, sizeof(int) * (size_t)nblocks, hipMemcpyHostToDevice, stream));
+    return hp;
+  }
+
+  // Algorithmic bytes per row: a root build reads F bins and the payload P; a
+  // leaf build also reads a 4-byte row index; a partition reads one bin and
+  // reads and writes a 4-byte row index.
+  void run_hist(const HistPlan& hp, double rows, bool root) {
+    if (hp.ni == 0) return;
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
     timed(root ? 0 : 1, hb, rows * (stride + (root ? payload_bytes() : 0)), [&] {
-      launch_hist(ds, d_hitems, ni, (int)total, nblocks, rpb, d_hblock, d_group_used, max_tile_bins, d_tile_first,
-                  (int)tile_first.size() - 1, rows_per_flush, stream);
+      launch_hist(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, d_group_used, max_tile_bins,
+                  d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
     });
-    staged();
     stats.hist_rows += (int64_t)rows;
   }
 
-  // Partition of the staged items (tiles 0 .. ntasks-1).
+  // Partition of the uploaded items (tiles 0 .. ntasks-1).
   void run_partition(int nitems, int ntasks, double part_rows) {
     if (ntasks <= 0) return;
     if (ntasks > max_part_tiles) throw std::runtime_error("partition status buffer too small");
-    BK_HIPC(hipMemcpyAsync(d_pitems, h_pitems, sizeof(PartItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
     epoch = epoch >= 0x3ffffffeu ? 1u : epoch + 1u;
     timed(2, part_rows * 9.0, part_rows * (2.0 * stride + 8.0), [&] {
       launch_partition(ds, d_pitems, nitems, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
     });
     ticket_base += (unsigned long long)ntasks;
-    staged();
     stats.part_rows += (int64_t)part_rows;
   }
 
-  // launches the search of the staged items and copies the candidates back
-  void run_scan(int nitems) {
-    if (nitems <= 0) return;
-    ScanParams sp = scan_params();
+  // Reduce items of the staged search items; returns their number.
+  int plan_scan(int nitems) {
     int nr = 0;
     for (int k = 0; k < nitems; ++k) {
       const ScanItem& si = h_sitems[k];
@@ -990,16 +1067,21 @@ struct Learner {
       if (si.has_large && !si.use_subtract)
         h_ritems[nr++] = ReduceItem{si.large.slot, si.large.part_first, si.large.nblk, 0};
     }
-    BK_HIPC(hipMemcpyAsync(d_sitems, h_sitems, sizeof(ScanItem) * (size_t)nitems, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_ritems, h_ritems, sizeof(ReduceItem) * (size_t)nr, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_used, h_used, sizeof(int) * (size_t)F * nitems, hipMemcpyHostToDevice, stream));
+    return nr;
+  }
+
+  // Sums the partials into the slots, searches, folds, and reads the best
+  // splits and splittable flags back (one transfer).
+  void run_scan(int nitems, int nr) {
+    if (nitems <= 0) return;
+    ScanParams sp = scan_params();
     timed(3, 0.0, 0.0, [&] {
       launch_reduce(ds, d_ritems, nr, stream);
       launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream);
       launch_fold(ds, d_sitems, nitems, d_out, d_used, d_fold, d_flags, stream);
     });
-    BK_HIPC(hipMemcpyAsync(h_fold, d_fold, sizeof(FoldOut) * (size_t)2 * nitems, hipMemcpyDeviceToHost, stream));
-    BK_HIPC(hipMemcpyAsync(h_flags, d_flags, (size_t)2 * nitems * F, hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipMemcpyAsync(h_back, d_back, off_fold + sizeof(FoldOut) * 2 * (size_t)nitems, hipMemcpyDeviceToHost,
+                           stream));
     BK_HIPC(hipStreamSynchronize(stream));
     staged_pending = false;
     stats.rounds += 1;
@@ -1130,14 +1212,18 @@ struct Learner {
     if (before_find_best_split(t, 0, -1)) {
       compute_used();
       hist_seg.assign(1, segs[0]);
-      run_hist(1, N, true);
-      stats.committed_hist_rows += N;
+      const HistPlan hp = plan_hist(1);
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, smaller, (int)smaller_slot);
       si.small.part_first = hist_first[0];
       si.small.nblk = hist_count[0];
-      run_scan(1);
+      const int nr = plan_scan(1);
+      begin_uploads();
+      upload_search(hp, 1, nr);
+      run_hist(hp, N, true);
+      stats.committed_hist_rows += N;
+      run_scan(1, nr);
       fold_sequential(false);
     }
     for (int step = 0; step < P.num_leaves - 1; ++step) {
@@ -1175,10 +1261,13 @@ struct Learner {
       pi.rule = rule_of(info);
       pi.first_task = add_tasks(ps.count, pi.chunk, nt);
       pi.ntasks = nt;
-      run_partition(1, nt, ps.count);
-      if (search) {
-        // the smaller child's rows are contiguous now: build its histogram, and
-        // the larger child's too when the parent's histogram was evicted
+      if (!search) {
+        begin_uploads();
+        upload_partition(1);
+        run_partition(1, nt, ps.count);
+      } else {
+        // after the partition the smaller child's rows are contiguous: build its
+        // histogram, and the larger child's too when the parent's was evicted
         const bool use_subtract = parent_slot >= 0;
         ScanItem& si = h_sitems[0];
         si = ScanItem{};
@@ -1196,14 +1285,20 @@ struct Learner {
           stats.committed_hist_rows += lseg.count;
           hrows += lseg.count;
         }
-        run_hist((int)hist_seg.size(), hrows, false);
+        const HistPlan hp = plan_hist((int)hist_seg.size());
         si.small.part_first = hist_first[0];
         si.small.nblk = hist_count[0];
         if (rebuild) {
           si.large.part_first = hist_first[1];
           si.large.nblk = hist_count[1];
         }
-        run_scan(1);
+        const int nr = plan_scan(1);
+        begin_uploads();
+        upload_partition(1);
+        run_partition(1, nt, ps.count);
+        upload_search(hp, 1, nr);
+        run_hist(hp, hrows, false);
+        run_scan(1, nr);
         fold_sequential(si.has_large != 0);
       }
     }
@@ -1233,8 +1328,7 @@ struct Learner {
     if (N >= P.min_data_in_leaf * 2) {
       for (int f = 0; f < F; ++f) h_used[f] = 1;
       hist_seg.assign(1, nodes[0].seg);
-      run_hist(1, N, true);
-      stats.committed_hist_rows += N;
+      const HistPlan hp = plan_hist(1);
       ScanItem& si = h_sitems[0];
       si = ScanItem{};
       si.small = leaf_args(t, root, 0);
@@ -1242,7 +1336,12 @@ struct Learner {
       si.small.nblk = hist_count[0];
       si.used_off = 0;
       si.out_off = 0;
-      run_scan(1);
+      const int nr = plan_scan(1);
+      begin_uploads();
+      upload_search(hp, 1, nr);
+      run_hist(hp, N, true);
+      stats.committed_hist_rows += N;
+      run_scan(1, nr);
       nodes[0].best = fold(0, nodes[0].splittable);
     } else {
       nodes[0].best.o.gain = kMinScore;
@@ -1350,15 +1449,24 @@ struct Learner {
         ++ni;
       }
       if (ni == 0) break;
-      run_partition(ni, nt, part_rows);
+      HistPlan hp;
+      int nr = 0;
       if (ns > 0) {
-        // histograms of the smaller children, now contiguous in their buffers
-        run_hist(ns, hist_rows, false);
+        // histograms of the smaller children, contiguous after the partition
+        hp = plan_hist(ns);
         for (int k2 = 0; k2 < ns; ++k2) {
           h_sitems[k2].small.part_first = hist_first[(size_t)k2];
           h_sitems[k2].small.nblk = hist_count[(size_t)k2];
         }
-        run_scan(ns);
+        nr = plan_scan(ns);
+      }
+      begin_uploads();
+      upload_partition(ni);
+      run_partition(ni, nt, part_rows);
+      if (ns > 0) {
+        upload_search(hp, ns, nr);
+        run_hist(hp, hist_rows, false);
+        run_scan(ns, nr);
       }
       // fold the children's candidates
       int k = 0;
@@ -1615,9 +1723,10 @@ int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
   return bk::guard([&] { *out = l->L.stats; });
 }
 
-int bk_learner_profile(bk_learner* l, int enable) {
+int bk_learner_profile(bk_learner* l, int kinds) {
   return bk::guard([&] {
-    l->L.prof.on = enable != 0;
+    l->L.prof.mask = (unsigned)kinds;
+    l->L.prof.on = kinds != 0;
     l->L.prof.reset();
   });
 }

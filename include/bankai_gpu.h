@@ -210,7 +210,8 @@ typedef struct bk_kernel_stats {
   double algo_bytes;
   double moved_bytes;
 } bk_kernel_stats;
-int bk_learner_profile(bk_learner* learner, int enable);
+/* kinds: bit k turns timing of kind k on (0 = off); the counters are reset. */
+int bk_learner_profile(bk_learner* learner, int kinds);
 int bk_learner_kernel_stats(bk_learner* learner, int kind, bk_kernel_stats* out);
 
 /* Frees the handle and its device memory. NULL is accepted. */
