@@ -84,7 +84,8 @@ class TreeOutC(C.Structure):
 class TrainStatsC(C.Structure):
     _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
                 ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32),
-                ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64)]
+                ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
+                ("host_ms", C.c_double), ("wait_ms", C.c_double)]
 
 
 class KernelStatsC(C.Structure):
@@ -382,6 +383,7 @@ class Learner:
         _check(self._lib.bk_learner_last_stats(self.handle, C.byref(s)))
         return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
+                "host_ms": s.host_ms, "wait_ms": s.wait_ms,
                 "speculative_waste": s.speculative_waste}
 
     def profile(self, kinds=True):

@@ -253,12 +253,11 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
 // reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
 // from its slot.
 void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream);
+// Split search of every (feature, item), then the fold of the best candidate of
+// every (item, side) into d_best[2 * item + side] and of every feature's
+// splittable flag into d_flags[(2 * item + side) * F + f].
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, void* stream);
-// Best candidate of every (item, side) -> d_best[2 * item + side], and every
-// feature's splittable flag -> d_flags[(2 * item + side) * F + f].
-void launch_fold(const DevState& s, const ScanItem* d_items, int nitems, const SplitOut* d_out, const int* d_used,
-                 FoldOut* d_best, uint8_t* d_flags, void* stream);
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,

@@ -1485,6 +1485,60 @@ __device__ void search_numerical_wave(const FeatDev& m, const ScanParams& p, con
   out.gain *= m.penalty;
 }
 
+// One wave folds the per-feature candidates of (item k, side) in feature order
+// with better_than (serial.rs:1500-1566, split_info.rs:67-85): the highest gain
+// wins, a tie goes to the lower real feature index (-1 counting as largest), a
+// NaN gain never wins. Also writes every feature's splittable flag (0 for
+// features not searched).
+__device__ void fold_side(const ScanItem& it, int k, int side, const SplitOut* out_all,
+                          const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
+                          uint8_t* __restrict__ flags) {
+  const int lane = threadIdx.x & 63;
+  uint8_t* fl = flags + (size_t)(2 * k + side) * F;
+  FoldOut* dst = best + 2 * k + side;
+  const bool active = side == 0 || it.has_large;
+  const int* used = used_all + it.used_off;
+  const SplitOut* out = out_all + it.out_off + (side ? F : 0);
+  double bg = kMinScore;
+  int bk = INT_MAX, bf = -1;
+  for (int f = lane; f < F; f += 64) {
+    int sp = 0;
+    if (active && used[f]) {
+      const double gain = out[f].gain;
+      const int feat = out[f].feature;
+      sp = out[f].splittable ? 1 : 0;
+      const int key = feat == -1 ? INT_MAX : feat;
+      if (!(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
+        bg = gain;
+        bk = key;
+        bf = f;
+      }
+    }
+    fl[f] = (uint8_t)sp;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double og = __shfl_xor(bg, o, 64);
+    const int ok = __shfl_xor(bk, o, 64), of = __shfl_xor(bf, o, 64);
+    if (of >= 0 && (bf < 0 || og > bg || (og == bg && ok < bk))) {
+      bg = og;
+      bk = ok;
+      bf = of;
+    }
+  }
+  constexpr int kWords = (int)(sizeof(SplitOut) / 4);
+  if (bf >= 0) {
+    const uint32_t* src = (const uint32_t*)(out + bf);
+    uint32_t* d = (uint32_t*)&dst->o;
+    for (int i = lane; i < kWords; i += 64) d[i] = src[i];
+  } else if (lane == 0) {
+    dst->o = SplitOut{};
+    dst->o.gain = kMinScore;
+    dst->o.feature = -1;
+    dst->o.default_left = 1;
+  }
+  if (lane == 0) dst->inner = bf;
+}
+
 template <int FORM>
 __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanItem* __restrict__ items, ScanParams p,
                                                        const int* __restrict__ used_all, SplitOut* out_all) {
@@ -1547,6 +1601,15 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
   o.feature = m.real;
   if (o.gain > kMinScore) exact_counts(m, S[wl], a.num_data, o);
   if (lane == 0) out[(wl == 0 ? 0 : s.num_features) + f] = o;
+}
+
+// One wave per (item, side): the fold runs after the search launch, whose
+// candidates it reads (a kernel boundary orders them across the XCDs' L2s).
+__global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
+                                             const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
+                                             uint8_t* __restrict__ flags) {
+  const int k = blockIdx.x >> 1, side = blockIdx.x & 1;
+  fold_side(items[k], k, side, out_all, used_all, F, best, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -1915,64 +1978,6 @@ __global__ void k_leaf_of(DevState s, const Segment* __restrict__ segs, int32_t*
   }
 }
 
-// ---------------------------------------------------------------------------
-// Per-leaf fold of the per-feature candidates in feature order with
-// better_than (serial.rs:1500-1566, split_info.rs:67-85): the highest gain
-// wins, a tie goes to the lower real feature index (-1 counting as largest),
-// a NaN gain never wins. One wave per (item, side). Also writes every
-// feature's splittable flag (0 for features not searched).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
-                                             const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
-                                             uint8_t* __restrict__ flags) {
-  const int item = blockIdx.x >> 1, side = blockIdx.x & 1;
-  const ScanItem& it = items[item];
-  const int lane = threadIdx.x;
-  uint8_t* fl = flags + (size_t)blockIdx.x * F;
-  FoldOut* dst = best + blockIdx.x;
-  const bool active = side == 0 || it.has_large;
-  const int* used = used_all + it.used_off;
-  const SplitOut* out = out_all + it.out_off + (side ? F : 0);
-  double bg = kMinScore;
-  int bk = INT_MAX, bf = -1;
-  for (int f = lane; f < F; f += 64) {
-    int sp = 0;
-    if (active && used[f]) {
-      const double gain = out[f].gain;
-      const int feat = out[f].feature;
-      sp = out[f].splittable ? 1 : 0;
-      const int key = feat == -1 ? INT_MAX : feat;
-      if (!(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
-        bg = gain;
-        bk = key;
-        bf = f;
-      }
-    }
-    fl[f] = (uint8_t)sp;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double og = __shfl_xor(bg, o, 64);
-    const int ok = __shfl_xor(bk, o, 64), of = __shfl_xor(bf, o, 64);
-    if (of >= 0 && (bf < 0 || og > bg || (og == bg && ok < bk))) {
-      bg = og;
-      bk = ok;
-      bf = of;
-    }
-  }
-  if (lane == 0) {
-    if (bf >= 0) {
-      dst->o = out[bf];
-    } else {
-      dst->o.gain = kMinScore;
-      dst->o.feature = -1;
-      dst->o.default_left = 1;
-      dst->o.splittable = 0;
-    }
-    dst->inner = bf;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Sum of the partial histograms of every item into its pool slot. A block
 // covers 32 bins of one item with 8 lanes per bin, each lane summing every 8th
 // partial (32 consecutive bins of one partial are one coalesced read); the 8
@@ -2149,16 +2154,9 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
   BK_HIP(hipGetLastError());
 }
 
-void launch_fold(const DevState& s, const ScanItem* d_items, int nitems, const SplitOut* d_out, const int* d_used,
-                 FoldOut* d_best, uint8_t* d_flags, void* stream) {
-  if (nitems <= 0) return;
-  hipLaunchKernelGGL(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, (hipStream_t)stream, d_items, d_out, d_used,
-                     s.num_features, d_best, d_flags);
-  BK_HIP(hipGetLastError());
-}
 
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, void* stream) {
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream) {
   if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
   dim3 grid((unsigned)s.num_features, (unsigned)nitems);
@@ -2173,6 +2171,9 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
       hipLaunchKernelGGL(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
   }
+  BK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, st, d_items, d_out, d_used, s.num_features,
+                     d_best, d_flags);
   BK_HIP(hipGetLastError());
 }
 

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -482,6 +483,14 @@ struct Learner {
     prof.end(stream, i, kind, algo, moved);
   }
 
+  // host time spent blocked on the device during the current train call
+  double wait_ms = 0.0;
+  void sync() {
+    const auto w0 = std::chrono::steady_clock::now();
+    BK_HIPC(hipStreamSynchronize(stream));
+    wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+  }
+
   ~Learner() { release(); }
 
   void release() {
@@ -753,10 +762,10 @@ struct Learner {
       d_hnoise = dmalloc<double>((size_t)N);
       BK_HIPC(hipMemcpyAsync(d_gnoise, gn.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice, stream));
       BK_HIPC(hipMemcpyAsync(d_hnoise, hn.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice, stream));
-      BK_HIPC(hipStreamSynchronize(stream));
+      sync();
       use_start_eng.seed_with((uint32_t)P.seed);
     }
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     // kernel view
     ds.num_data = N;
     ds.npad = npad;
@@ -781,14 +790,14 @@ struct Learner {
     ds.part_c = d_part_c;
     ds.err_flag = d_err;
     launch_to_records(ds, d_colstage, (size_t)npad, stream);
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     pool.init((size_t)P.num_leaves, cache, (size_t)F);
     best.assign((size_t)P.num_leaves, HostSplit());
     segs.assign((size_t)P.num_leaves, Segment{0, 0, 0});
     used_feat.assign((size_t)F, 1);
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
   }
 
   ScanParams scan_params() const {
@@ -826,7 +835,7 @@ struct Learner {
       launch_max_abs(g, h, N, d_max4, stream);
       unsigned mx[4];
       BK_HIPC(hipMemcpyAsync(mx, d_max4, sizeof mx, hipMemcpyDeviceToHost, stream));
-      BK_HIPC(hipStreamSynchronize(stream));
+      sync();
       float max_g, max_h;
       std::memcpy(&max_g, &mx[0], 4);
       std::memcpy(&max_h, &mx[1], 4);
@@ -844,7 +853,7 @@ struct Learner {
         });
         long long sums[4];
         BK_HIPC(hipMemcpyAsync(sums, d_sums3, sizeof sums, hipMemcpyDeviceToHost, stream));
-        BK_HIPC(hipStreamSynchronize(stream));
+        sync();
         last_int_g = sums[0];
         last_int_h = sums[1];
         // exact when every sequential partial sum fits the f64 significand
@@ -863,7 +872,7 @@ struct Learner {
         } else {
           std::vector<int16_t> pk((size_t)N);
           BK_HIPC(hipMemcpyAsync(pk.data(), d_gh0, sizeof(int16_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
-          BK_HIPC(hipStreamSynchronize(stream));
+          sync();
           double a = 0.0, b = 0.0;
           for (int i = 0; i < N; ++i) {
             a += (double)(int8_t)(pk[(size_t)i] >> 8) * grad_scale;
@@ -884,7 +893,7 @@ struct Learner {
       float hh0 = 0.0f;
       BK_HIPC(hipMemcpyAsync(part.data(), d_sum_partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
       BK_HIPC(hipMemcpyAsync(&hh0, h, sizeof(float), hipMemcpyDeviceToHost, stream));
-      BK_HIPC(hipStreamSynchronize(stream));
+      sync();
       for (int b = 0; b < nb; ++b) {
         sg += part[(size_t)2 * b];
         sh += part[(size_t)2 * b + 1];
@@ -927,7 +936,9 @@ struct Learner {
 
   void stage_wait() {
     if (!staged_pending) return;
+    const auto w0 = std::chrono::steady_clock::now();
     BK_HIPC(hipEventSynchronize(staged_ev));
+    wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     staged_pending = false;
   }
 
@@ -976,17 +987,18 @@ struct Learner {
     arena_sent = arena_cur;
   }
 
-  void upload_partition(int nitems) {
-    d_pitems = put(h_pitems, (size_t)nitems);
-    send();
-  }
-
-  void upload_search(const HistPlan& hp, int nitems, int nr) {
-    d_hitems = put(h_hitems, (size_t)hp.ni);
-    d_hblock = put(h_hblock, (size_t)hp.nblocks);
-    d_sitems = put(h_sitems, (size_t)nitems);
-    d_ritems = put(h_ritems, (size_t)nr);
-    d_used = put(h_used, (size_t)F * nitems);
+  // packs a round's partition items (npart) and search items (nscan, with the
+  // histogram plan and nr reduce items) and sends them in one transfer
+  void upload_round(int npart, const HistPlan* hp, int nscan, int nr) {
+    begin_uploads();
+    if (npart > 0) d_pitems = put(h_pitems, (size_t)npart);
+    if (nscan > 0) {
+      d_hitems = put(h_hitems, (size_t)hp->ni);
+      d_hblock = put(h_hblock, (size_t)hp->nblocks);
+      d_sitems = put(h_sitems, (size_t)nscan);
+      d_ritems = put(h_ritems, (size_t)nr);
+      d_used = put(h_used, (size_t)F * nscan);
+    }
     send();
   }
 
@@ -1077,12 +1089,11 @@ struct Learner {
     ScanParams sp = scan_params();
     timed(3, 0.0, 0.0, [&] {
       launch_reduce(ds, d_ritems, nr, stream);
-      launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, stream);
-      launch_fold(ds, d_sitems, nitems, d_out, d_used, d_fold, d_flags, stream);
+      launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, d_fold, d_flags, stream);
     });
     BK_HIPC(hipMemcpyAsync(h_back, d_back, off_fold + sizeof(FoldOut) * 2 * (size_t)nitems, hipMemcpyDeviceToHost,
                            stream));
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     staged_pending = false;
     stats.rounds += 1;
   }
@@ -1219,8 +1230,7 @@ struct Learner {
       si.small.part_first = hist_first[0];
       si.small.nblk = hist_count[0];
       const int nr = plan_scan(1);
-      begin_uploads();
-      upload_search(hp, 1, nr);
+      upload_round(0, &hp, 1, nr);
       run_hist(hp, N, true);
       stats.committed_hist_rows += N;
       run_scan(1, nr);
@@ -1262,8 +1272,7 @@ struct Learner {
       pi.first_task = add_tasks(ps.count, pi.chunk, nt);
       pi.ntasks = nt;
       if (!search) {
-        begin_uploads();
-        upload_partition(1);
+        upload_round(1, nullptr, 0, 0);
         run_partition(1, nt, ps.count);
       } else {
         // after the partition the smaller child's rows are contiguous: build its
@@ -1293,10 +1302,8 @@ struct Learner {
           si.large.nblk = hist_count[1];
         }
         const int nr = plan_scan(1);
-        begin_uploads();
-        upload_partition(1);
+        upload_round(1, &hp, 1, nr);
         run_partition(1, nt, ps.count);
-        upload_search(hp, 1, nr);
         run_hist(hp, hrows, false);
         run_scan(1, nr);
         fold_sequential(si.has_large != 0);
@@ -1337,8 +1344,7 @@ struct Learner {
       si.used_off = 0;
       si.out_off = 0;
       const int nr = plan_scan(1);
-      begin_uploads();
-      upload_search(hp, 1, nr);
+      upload_round(0, &hp, 1, nr);
       run_hist(hp, N, true);
       stats.committed_hist_rows += N;
       run_scan(1, nr);
@@ -1460,11 +1466,9 @@ struct Learner {
         }
         nr = plan_scan(ns);
       }
-      begin_uploads();
-      upload_partition(ni);
+      upload_round(ni, &hp, ns, nr);
       run_partition(ni, nt, part_rows);
       if (ns > 0) {
-        upload_search(hp, ns, nr);
         run_hist(hp, hist_rows, false);
         run_scan(ns, nr);
       }
@@ -1525,6 +1529,8 @@ struct Learner {
   }
 
   Tree train(const float* g, const float* h) {
+    const auto t0 = std::chrono::steady_clock::now();
+    wait_ms = 0.0;
     leaf_of_valid = false;
     Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
     if (renew) {
@@ -1550,17 +1556,20 @@ struct Learner {
             [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
       std::vector<double> sums((size_t)t.num_leaves * 2);
       BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
-      BK_HIPC(hipStreamSynchronize(stream));
+      sync();
       for (int leaf = 0; leaf < t.num_leaves; ++leaf)
         t.set_leaf_output(leaf, leaf_output_plain(sums[(size_t)2 * leaf], sums[(size_t)2 * leaf + 1], P.lambda_l1,
                                                   P.lambda_l2, P.max_delta_step));
     }
     int err = 0;
     BK_HIPC(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, stream));
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     if (err) throw std::runtime_error("partition count mismatch on the device");
     if (prof.on) prof.resolve();
     last_num_leaves = t.num_leaves;
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats.wait_ms = wait_ms;
+    stats.host_ms = total - wait_ms;
     return t;
   }
 
@@ -1616,7 +1625,7 @@ struct Learner {
     staged();
     timed(6, (double)N * 20.0, (double)N * 20.0,
           [&] { launch_add_score_rows(N, d_leaf_of, d_leaf_vals, t->num_leaves, d_score, stream); });
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     staged_pending = false;
     if (prof.on) prof.resolve();
   }
@@ -1627,7 +1636,7 @@ struct Learner {
     ensure_leaf_of(t->num_leaves);
     std::vector<int32_t> lo((size_t)N);
     BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     staged_pending = false;
     for (int i = 0; i < N; ++i) score[i] += t->leaf_value[lo[(size_t)i]];
   }
@@ -1638,7 +1647,7 @@ struct Learner {
     int32_t* d = dmalloc<int32_t>((size_t)std::max(s.count, 1));
     launch_leaf_rows(ds, s, d, stream);
     BK_HIPC(hipMemcpyAsync(out, d, sizeof(int32_t) * (size_t)s.count, hipMemcpyDeviceToHost, stream));
-    BK_HIPC(hipStreamSynchronize(stream));
+    sync();
     (void)hipFree(d);
     *n = s.count;
   }

@@ -99,7 +99,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
     if profile:
         learner.profile(profile)
     stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0,
-             "committed_hist_rows": 0, "committed_part_rows": 0}
+             "committed_hist_rows": 0, "committed_part_rows": 0, "host_ms": 0.0, "wait_ms": 0.0}
     t0 = time.perf_counter()
     for i in range(steps):
         step(True)

@@ -193,6 +193,8 @@ typedef struct bk_train_stats {
   int32_t reserved;
   int64_t committed_hist_rows;
   int64_t committed_part_rows;
+  double host_ms;  /* wall time of the train call not spent blocked on the device */
+  double wait_ms;  /* wall time blocked on the device (synchronizations)           */
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
