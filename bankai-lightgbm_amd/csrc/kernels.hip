@@ -435,8 +435,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // written to their destinations: two contiguous runs per tile.
 // status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
 // ---------------------------------------------------------------------------
-constexpr int kPartThreads = 256;
-constexpr int kPartChunks = 8;  // 16-byte chunks per thread per tile
+constexpr int kPartThreads = 512;
+constexpr int kPartChunks = 4;  // 16-byte chunks per thread per tile
 constexpr int kSpinLimit = 1 << 22;
 
 __device__ __forceinline__ unsigned long long status_word(unsigned flag, unsigned epoch, unsigned value) {
