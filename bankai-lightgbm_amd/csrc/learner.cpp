@@ -193,6 +193,59 @@ struct HostSplit {
   }
 };
 
+// Argmax over the leaves' best splits with the reference's rule (split_info.rs:88-96):
+// the first maximum under better_than. A tournament tree gives it in O(log L)
+// per changed leaf instead of a scan per split. NaN gains (never better, never
+// worse) break the ordering a tree needs, so with any NaN present the plain
+// scan decides.
+struct ArgMax {
+  const std::vector<HostSplit>* b = nullptr;
+  int n = 0, size = 1, nan_count = 0;
+  std::vector<int> t;
+  std::vector<char> is_nan;
+  bool wins(int i, int j) const {
+    if (j < 0) return i >= 0;
+    if (i < 0) return false;
+    const HostSplit& x = (*b)[(size_t)i];
+    const HostSplit& y = (*b)[(size_t)j];
+    if (x.better_than(y)) return true;
+    if (y.better_than(x)) return false;
+    return i < j;
+  }
+  void init(const std::vector<HostSplit>* best) {
+    b = best;
+    n = (int)best->size();
+    size = 1;
+    while (size < n) size <<= 1;
+    t.assign((size_t)2 * size, -1);
+    is_nan.assign((size_t)n, 0);
+    nan_count = 0;
+    for (int i = 0; i < n; ++i) {
+      t[(size_t)(size + i)] = i;
+      note_nan(i);
+    }
+    for (int k = size - 1; k >= 1; --k) t[(size_t)k] = wins(t[(size_t)2 * k], t[(size_t)2 * k + 1]) ? t[(size_t)2 * k] : t[(size_t)2 * k + 1];
+  }
+  void note_nan(int i) {
+    const double g = (*b)[(size_t)i].o.gain;
+    const char now = g != g ? 1 : 0;
+    nan_count += now - is_nan[(size_t)i];
+    is_nan[(size_t)i] = now;
+  }
+  void update(int i) {
+    note_nan(i);
+    for (int k = (size + i) >> 1; k >= 1; k >>= 1)
+      t[(size_t)k] = wins(t[(size_t)2 * k], t[(size_t)2 * k + 1]) ? t[(size_t)2 * k] : t[(size_t)2 * k + 1];
+  }
+  int top() const {
+    if (nan_count == 0) return t[1];
+    int bl = 0;
+    for (int i = 1; i < n; ++i)
+      if ((*b)[(size_t)i].better_than((*b)[(size_t)bl])) bl = i;
+    return bl;
+  }
+};
+
 struct LeafSplits {
   int leaf = -1, num_data = 0;
   double sum_g = 0, sum_h = 0, weight = 0;
@@ -344,7 +397,6 @@ struct Node {
   int slot = -1;  // pool slot of its histogram
   int child[2] = {-1, -1};
   bool expanded = false;
-  std::vector<char> splittable;
 };
 
 struct Learner {
@@ -450,6 +502,7 @@ struct Learner {
   // per-tree state (serial.rs:183-245)
   PoolMap pool;
   std::vector<HostSplit> best;
+  ArgMax amax;
   LeafSplits smaller, larger;
   size_t smaller_slot = 0;
   long larger_slot = -1, parent_slot = -1;
@@ -470,6 +523,8 @@ struct Learner {
   // batched form
   std::vector<Node> nodes;
   std::vector<int> leaf_node;
+  std::vector<char> node_flags;  // splittable flags of node i at [i * F, (i + 1) * F)
+  char* flags_of(int node) { return node_flags.data() + (size_t)node * F; }
 
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
@@ -825,6 +880,7 @@ struct Learner {
   LeafSplits before_train(const float* g, const float* h) {
     pool.reset();
     for (auto& s : best) s.reset();
+    amax.init(&best);
     std::fill(segs.begin(), segs.end(), Segment{0, 0, 0});
     segs[0] = Segment{0, 0, N};
     stats = bk_train_stats{};
@@ -1098,15 +1154,28 @@ struct Learner {
     stats.rounds += 1;
   }
 
+  void set_best(int leaf, const HostSplit& v) {
+    best[(size_t)leaf] = v;
+    amax.update(leaf);
+  }
+  void set_gain_min(int left_leaf, int right_leaf) {
+    best[(size_t)left_leaf].o.gain = kMinScore;
+    amax.update(left_leaf);
+    if (right_leaf >= 0) {
+      best[(size_t)right_leaf].o.gain = kMinScore;
+      amax.update(right_leaf);
+    }
+  }
+
   // The folded best split of (item, side) k = 2 * item + side and the
   // splittable flags of its features (serial.rs:1500-1566; folded on the device).
-  HostSplit fold(int k, std::vector<char>& splittable) const {
+  HostSplit fold(int k, char* splittable) const {
     HostSplit b;
     if (h_fold[k].inner >= 0) {
       b.o = h_fold[k].o;
       b.inner = h_fold[k].inner;
     }
-    for (int f = 0; f < F; ++f) splittable[(size_t)f] = (char)h_flags[(size_t)k * F + f];
+    for (int f = 0; f < F; ++f) splittable[f] = (char)h_flags[(size_t)k * F + f];
     return b;
   }
 
@@ -1166,15 +1235,13 @@ struct Learner {
   // before_find_best_split (serial.rs:779-825)
   bool before_find_best_split(const Tree& t, int left_leaf, int right_leaf) {
     if (P.max_depth > 0 && t.leaf_depth[(size_t)left_leaf] >= P.max_depth) {
-      best[(size_t)left_leaf].o.gain = kMinScore;
-      if (right_leaf >= 0) best[(size_t)right_leaf].o.gain = kMinScore;
+      set_gain_min(left_leaf, right_leaf);
       return false;
     }
     int nl = left_leaf >= 0 ? segs[(size_t)left_leaf].count : 0;
     int nr = right_leaf >= 0 ? segs[(size_t)right_leaf].count : 0;
     if (nr < P.min_data_in_leaf * 2 && nl < P.min_data_in_leaf * 2) {
-      best[(size_t)left_leaf].o.gain = kMinScore;
-      if (right_leaf >= 0) best[(size_t)right_leaf].o.gain = kMinScore;
+      set_gain_min(left_leaf, right_leaf);
       return false;
     }
     parent_slot = -1;
@@ -1210,10 +1277,10 @@ struct Learner {
   }
 
   void fold_sequential(bool has_large) {
-    HostSplit sb = fold(0, pool.splittable[smaller_slot]);
-    best[(size_t)smaller.leaf] = sb;
+    HostSplit sb = fold(0, pool.splittable[smaller_slot].data());
+    set_best(smaller.leaf, sb);
     if (larger.leaf >= 0)
-      best[(size_t)larger.leaf] = has_large ? fold(1, pool.splittable[(size_t)larger_slot]) : HostSplit();
+      set_best(larger.leaf, has_large ? fold(1, pool.splittable[(size_t)larger_slot].data()) : HostSplit());
   }
 
   Tree train_sequential(const float* g, const float* h) {
@@ -1237,9 +1304,7 @@ struct Learner {
       fold_sequential(false);
     }
     for (int step = 0; step < P.num_leaves - 1; ++step) {
-      size_t bl = 0;
-      for (size_t i = 1; i < best.size(); ++i)
-        if (best[i].better_than(best[bl])) bl = i;
+      const size_t bl = (size_t)amax.top();
       if (best[bl].o.gain <= 0.0) break;
       const HostSplit info = best[bl];
       const int best_leaf = (int)bl;
@@ -1321,12 +1386,14 @@ struct Learner {
     Tree t(P.num_leaves);
     t.set_leaf_output(0, leaf_output_root(P, root.sum_g, root.sum_h));
     nodes.clear();
-    nodes.reserve((size_t)P.num_leaves * 4 + 4);
+    const size_t max_nodes = (size_t)P.num_leaves * 4 + 4;
+    nodes.reserve(max_nodes);
+    if (node_flags.size() < max_nodes * F) node_flags.assign(max_nodes * F, 1);
     Node r0;
     r0.seg = Segment{0, 0, N};
     r0.ls = root;
     r0.slot = 0;
-    r0.splittable.assign((size_t)F, 1);
+    std::fill(node_flags.begin(), node_flags.begin() + F, (char)1);
     nodes.push_back(r0);
     leaf_node.assign((size_t)P.num_leaves, -1);
     leaf_node[0] = 0;
@@ -1348,11 +1415,11 @@ struct Learner {
       run_hist(hp, N, true);
       stats.committed_hist_rows += N;
       run_scan(1, nr);
-      nodes[0].best = fold(0, nodes[0].splittable);
+      nodes[0].best = fold(0, flags_of(0));
     } else {
       nodes[0].best.o.gain = kMinScore;
     }
-    best[0] = nodes[0].best;
+    set_best(0, nodes[0].best);
     std::vector<int> order;
     bool done = false;
     while (!done) {
@@ -1362,9 +1429,7 @@ struct Learner {
           done = true;
           break;
         }
-        size_t bl = 0;
-        for (size_t i = 1; i < best.size(); ++i)
-          if (best[i].better_than(best[bl])) bl = i;
+        const size_t bl = (size_t)amax.top();
         if (best[bl].o.gain <= 0.0) {
           done = true;
           break;
@@ -1378,8 +1443,8 @@ struct Learner {
         nodes[(size_t)cr].ls.leaf = right;
         leaf_node[bl] = cl;
         leaf_node[(size_t)right] = cr;
-        best[bl] = nodes[(size_t)cl].best;
-        best[(size_t)right] = nodes[(size_t)cr].best;
+        set_best((int)bl, nodes[(size_t)cl].best);
+        set_best(right, nodes[(size_t)cr].best);
       }
       if (done) break;
       // expand the best unexpanded frontier leaves, at most the remaining budget
@@ -1432,8 +1497,6 @@ struct Learner {
           if ((size_t)next_slot >= num_slots) throw std::runtime_error("histogram slots exhausted");
           sm.slot = next_slot++;
           lg.slot = X.slot;
-          sm.splittable = X.splittable;
-          lg.splittable = X.splittable;
           ScanItem& si = h_sitems[ns];
           si = ScanItem{};
           // a child's parent output is its own output from the split (never the root's formula)
@@ -1444,9 +1507,11 @@ struct Learner {
           si.use_subtract = 1;
           si.used_off = ns * F;
           si.out_off = ns * 2 * F;
-          for (int f = 0; f < F; ++f) h_used[ns * F + f] = X.splittable[(size_t)f] ? 1 : 0;
+          const char* xf = flags_of(x);
+          for (int f = 0; f < F; ++f) h_used[ns * F + f] = xf[f] ? 1 : 0;
           ++ns;
         }
+        if (nodes.size() + 2 > max_nodes) throw std::runtime_error("node table too small");
         X.child[0] = (int)nodes.size();
         X.child[1] = (int)nodes.size() + 1;
         X.expanded = true;
@@ -1482,8 +1547,8 @@ struct Learner {
         const bool left_small = lc < rc;
         Node& sm = nodes[(size_t)X.child[left_small ? 0 : 1]];
         Node& lg = nodes[(size_t)X.child[left_small ? 1 : 0]];
-        sm.best = fold(2 * k, sm.splittable);
-        lg.best = fold(2 * k + 1, lg.splittable);
+        sm.best = fold(2 * k, flags_of(X.child[left_small ? 0 : 1]));
+        lg.best = fold(2 * k + 1, flags_of(X.child[left_small ? 1 : 0]));
         ++k;
       }
     }
