@@ -449,10 +449,10 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
   return v;
 }
 
-// LDS of the partition: [destinations R x u32][route flags R bytes][staged records R x stride][staged ids R x i32]
-__host__ __device__ inline size_t stage_offset(int R) { return ((size_t)R * 5 + 15) & ~(size_t)15; }
+// LDS of the partition: [destinations R x u32][route flags R bytes]
 __host__ __device__ inline size_t partition_lds(int R, int stride) {
-  return stage_offset(R) + (size_t)R * stride + 4 * (size_t)R;
+  (void)stride;
+  return ((size_t)R * 5 + 15) & ~(size_t)15;
 }
 
 template <int FORM>
@@ -566,41 +566,32 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     }
   }
   __syncthreads();
-  // permute the records and row ids into destination order in LDS, so that the
-  // writes below are two contiguous runs covered lane by lane
-  uint8_t* stage = smem + stage_offset(R);
-  int32_t* stage_id = (int32_t*)(stage + (size_t)R * stride);
-  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
-#pragma unroll
-  for (int k = 0; k < kPartChunks; ++k) {
-    const int q = (int)threadIdx.x + k * kPartThreads;
-    if (q < total) {
-      const int r = q / nc, c = q - r * nc;
-      if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
-      *(uint4*)(stage + (size_t)dstp[r] * stride + 16 * c) = v[k];
-    }
-  }
-  {
-    const int32_t* sidx = idx_of(s, seg.buf);
-    for (int r = threadIdx.x; r < rows; r += kPartThreads)
-      stage_id[dstp[r]] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
-  }
-  __syncthreads();
   const long long lpos = (long long)seg.begin + left_before;
   const long long rpos = (long long)seg.begin + left_total + ((t0 - seg.begin) - left_before);
   if (lpos < (long long)seg.begin || lpos + tl > seg_end || rpos < (long long)seg.begin || rpos + tr > seg_end) {
     if (threadIdx.x == 0) atomicOr(s.err_flag, 2);  // inconsistent counts: write nothing
   } else {
-    // run 1: chunks [0, tl*nc) to lpos; run 2: the rest to rpos
-    const int lchunks = tl * nc;
-    for (int q = threadIdx.x; q < total; q += kPartThreads) {
-      const uint4 w = *(const uint4*)(stage + 16 * (size_t)q);
-      uint8_t* d = q < lchunks ? dst + (size_t)lpos * stride + 16 * (size_t)q
-                               : dst + (size_t)rpos * stride + 16 * (size_t)(q - lchunks);
-      *(uint4*)d = w;
+    // every lane writes its chunks straight to their destination rows; the rows
+    // of a tile land in two contiguous runs, so the lanes of a wave cover
+    // contiguous 16-byte pieces of them
+    const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
+#pragma unroll
+    for (int k = 0; k < kPartChunks; ++k) {
+      const int q = (int)threadIdx.x + k * kPartThreads;
+      if (q < total) {
+        const int r = q / nc, c = q - r * nc;
+        const int l = (int)dstp[r];
+        const long long d = l < tl ? lpos + l : rpos + (l - tl);
+        if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
+        *(uint4*)(dst + (size_t)d * stride + 16 * c) = v[k];
+      }
     }
     int32_t* didx = idx_of(s, it.dst_buf);
-    for (int r = threadIdx.x; r < rows; r += kPartThreads) didx[r < tl ? lpos + r : rpos + (r - tl)] = stage_id[r];
+    const int32_t* sidx = idx_of(s, seg.buf);
+    for (int r = threadIdx.x; r < rows; r += kPartThreads) {
+      const int l = (int)dstp[r];
+      didx[l < tl ? lpos + l : rpos + (l - tl)] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
+    }
   }
   if (threadIdx.x == 0 && blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
 }
