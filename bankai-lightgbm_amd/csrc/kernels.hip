@@ -668,9 +668,17 @@ __device__ __forceinline__ double split_gain_plain(double gl, double hl, double 
 // Prefix state of one scan direction at every bin, in LDS: the sums the
 // sequential loop of FindBestThresholdSequentially(Int) holds when it evaluates
 // bin t (feature_histogram.rs:363-663, 839-1142). Index t + 1 (t may be -1).
+// The float forms use the f64 planes, the quantized form the int64 planes:
+// they share storage (every access is under the form's branch).
 struct ScanPrefix {
-  double g[kMaxBins + 1], h[kMaxBins + 1];
-  long long ig[kMaxBins + 1], ih[kMaxBins + 1];
+  union {
+    double g[kMaxBins + 1];
+    long long ig[kMaxBins + 1];
+  };
+  union {
+    double h[kMaxBins + 1];
+    long long ih[kMaxBins + 1];
+  };
   int c[kMaxBins + 1];
   unsigned char active[kMaxBins + 1];
 };
@@ -1247,8 +1255,14 @@ __device__ void search_categorical(const double* G, const double* H, const long 
 
 // One leaf's feature slice in LDS.
 struct SliceLds {
-  double g[kMaxBins], h[kMaxBins];
-  long long ig[kMaxBins], ih[kMaxBins];
+  union {
+    double g[kMaxBins];
+    long long ig[kMaxBins];
+  };
+  union {
+    double h[kMaxBins];
+    long long ih[kMaxBins];
+  };
   int c[kMaxBins];
   int sorted[kMaxBins];
   int used_bin;
@@ -1567,7 +1581,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
   // with the reductions above and take part in the barriers below
   const bool enabled = wave == 0 || (wave == 1 && has_large);
   const int wl = wave < 2 ? wave : 0;
-  const LeafArgs& a = wl == 0 ? small : large;
+  const LeafArgs a = wl == 0 ? small : large;  // a copy: a reference would put both in scratch
   SplitOut o = SplitOut{};
   o.gain = kMinScore;
   o.default_left = 1;
