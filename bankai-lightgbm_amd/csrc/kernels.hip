@@ -327,6 +327,7 @@ __device__ __forceinline__ void hist_chunk_regs(LdsHist<FORM>& L, const uint4& v
 // 16-byte chunks.
 // ---------------------------------------------------------------------------
 constexpr int kHistThreads = 512;
+constexpr int kXcds = 8;         // accelerator complexes of an MI355X, each with its own L2
 constexpr int kHistRows = 2048;  // rows per payload staging tile
 
 template <int FORM>
@@ -334,13 +335,20 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
                                                        int total_rows, int rows_per_block,
                                                        const int* __restrict__ block_slot,
                                                        const int* __restrict__ group_used,
-                                                       const int* __restrict__ tile_first, int rows_per_flush) {
+                                                       const int* __restrict__ tile_first, int rows_per_flush,
+                                                       int nblocks, int ntiles) {
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
-  const long long start = (long long)blockIdx.x * rows_per_block;
+  // Workgroups go to the XCDs round robin (workgroup i to XCD i % 8). The
+  // group tiles of one row block are given workgroups i, i + 8, ... so they
+  // run on one XCD one after the other and the later tiles read the rows from
+  // that XCD's L2 instead of HBM.
+  const int xcd = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
+  const int rb = (k / ntiles) * kXcds + xcd, tile = k % ntiles;
+  if (rb >= nblocks) return;  // block-uniform
+  const long long start = (long long)rb * rows_per_block;
   const long long end = min(start + (long long)rows_per_block, (long long)total_rows);
   if (start >= end) return;  // block-uniform
-  const int tile = blockIdx.y;
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
   LdsHist<FORM> L;
@@ -371,7 +379,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
     }
     i = lo;
   }
-  int slot = block_slot[blockIdx.x];
+  int slot = block_slot[rb];
   for (long long x = start; x < end; ++i, ++slot) {
     const HistItem it = items[i];
     const Segment seg = it.seg;
@@ -2081,7 +2089,8 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
                  const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
   if (nblocks <= 0 || total_rows <= 0) return;
-  dim3 grid((unsigned)nblocks, (unsigned)ntiles);
+  const unsigned groups8 = (unsigned)((nblocks + kXcds - 1) / kXcds);
+  dim3 grid(groups8 * kXcds * (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
   const size_t pay = s.form == FORM_QUANT ? 2 : 8;
   const size_t lds = hist_lds_bytes(s.form, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
@@ -2090,17 +2099,20 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
       hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
-                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
+                         ntiles);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
       hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
-                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
+                         ntiles);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
       hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
-                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush);
+                         rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
+                         ntiles);
       break;
   }
   BK_HIP(hipGetLastError());
