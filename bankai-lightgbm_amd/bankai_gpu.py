@@ -118,6 +118,7 @@ EXPORTS = {
     "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
     "bk_learner_profile": ([_P, C.c_int], C.c_int),
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
+    "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
                               C.c_int),
@@ -236,6 +237,11 @@ class Dataset:
         """The stored u8 column of every group, as numpy arrays (copies)."""
         v = self.view
         return [np.ctypeslib.as_array(v.group_bins[g], shape=(v.num_data,)).copy() for g in range(v.num_groups)]
+
+    def group_num_total_bin(self):
+        """Bins of every group; the learner's histograms concatenate the groups."""
+        v = self.view
+        return np.ctypeslib.as_array(v.group_num_total_bin, shape=(v.num_groups,)).copy()
 
     def feature_meta(self):
         v = self.view
@@ -374,6 +380,24 @@ class Learner:
         n = C.c_int64()
         _check(self._lib.bk_learner_leaf_rows(self.handle, leaf, _ptr(out), C.byref(n)))
         return out[: n.value].copy()
+
+    def root_histogram(self, gradients, hessians):
+        """Root histogram before fix_histogram: (gh, counts), where gh is a
+        (total_bins, 2) float64 array for the float forms and int64 for the
+        quantized form (exact integer sums of the discretized gradients)."""
+        g = np.ascontiguousarray(gradients, dtype=np.float32)
+        h = np.ascontiguousarray(hessians, dtype=np.float32)
+        if g.shape[0] != self.dataset.num_data or h.shape[0] != self.dataset.num_data:
+            raise BankaiError("gradients and hessians must have num_data entries")
+        tb = int(np.sum(self.dataset.group_num_total_bin()))
+        cnt = np.zeros(tb, np.int32)
+        if self.params.form == FORM_QUANTIZED:
+            gh = np.zeros((tb, 2), np.int64)
+            _check(self._lib.bk_learner_root_histogram(self.handle, _ptr(g), _ptr(h), None, _ptr(gh), _ptr(cnt)))
+        else:
+            gh = np.zeros((tb, 2), np.float64)
+            _check(self._lib.bk_learner_root_histogram(self.handle, _ptr(g), _ptr(h), _ptr(gh), None, _ptr(cnt)))
+        return gh, cnt
 
     def reset_constant_hessian(self, value):
         _check(self._lib.bk_learner_reset_constant_hessian(self.handle, int(value)))

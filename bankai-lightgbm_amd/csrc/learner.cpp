@@ -1706,6 +1706,43 @@ struct Learner {
     for (int i = 0; i < N; ++i) score[i] += t->leaf_value[lo[(size_t)i]];
   }
 
+  // Root histogram over all rows in the learner's form: what construct_histograms
+  // (serial.rs:891-1065) or construct_int_histograms (serial.rs:1074-1164) leave
+  // in the root's slot before fix_histogram, with a constant hessian scaled by
+  // hessians[0] (serial.rs:982-988). It runs the per-tree set-up (discretization
+  // included), so a stochastic learner's noise offset advances as in train.
+  void root_histogram(const float* g, const float* h, double* out_gh, int64_t* out_int, int32_t* out_cnt) {
+    if (quant ? out_gh != nullptr : out_int != nullptr)
+      throw std::invalid_argument(quant ? "the quantized learner returns integer histograms"
+                                        : "float learners return f64 histograms");
+    before_train(g, h);
+    hist_seg.assign(1, segs[0]);
+    const HistPlan hp = plan_hist(1);
+    h_ritems[0] = ReduceItem{0, hist_first[0], hist_count[0], 0};
+    begin_uploads();
+    d_hitems = put(h_hitems, (size_t)hp.ni);
+    d_hblock = put(h_hblock, (size_t)hp.nblocks);
+    d_ritems = put(h_ritems, 1);
+    send();
+    run_hist(hp, N, true);
+    launch_reduce(ds, d_ritems, 1, stream);
+    std::vector<double> gh((size_t)TB * 2);
+    std::vector<int32_t> cnt((size_t)TB);
+    BK_HIPC(hipMemcpyAsync(gh.data(), ds.pool, sizeof(double) * gh.size(), hipMemcpyDeviceToHost, stream));
+    BK_HIPC(hipMemcpyAsync(cnt.data(), ds.pool_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost, stream));
+    sync();
+    staged_pending = false;
+    if (quant) {
+      if (out_int) std::memcpy(out_int, gh.data(), sizeof(int64_t) * gh.size());
+    } else if (out_gh) {
+      for (size_t i = 0; i < (size_t)TB; ++i) {
+        out_gh[2 * i] = gh[2 * i];
+        out_gh[2 * i + 1] = const_hess ? gh[2 * i + 1] * h0 : gh[2 * i + 1];
+      }
+    }
+    if (out_cnt) std::memcpy(out_cnt, cnt.data(), sizeof(int32_t) * cnt.size());
+  }
+
   void leaf_rows(int leaf, int32_t* out, int64_t* n) {
     if (leaf < 0 || leaf >= last_num_leaves) throw std::invalid_argument("leaf out of range");
     const Segment s = segs[(size_t)leaf];
@@ -1772,6 +1809,17 @@ int bk_learner_add_prediction_to_score_device(bk_learner* l, const bk_tree_out* 
 
 int bk_learner_leaf_rows(bk_learner* l, int32_t leaf, int32_t* rows_out, int64_t* num_out) {
   return bk::guard([&] { l->L.leaf_rows(leaf, rows_out, num_out); });
+}
+
+int bk_learner_root_histogram(bk_learner* l, const float* g, const float* h, double* hist_out,
+                              int64_t* int_hist_out, int32_t* count_out) {
+  return bk::guard([&] {
+    if (!l || !g || !h) throw std::invalid_argument("null argument");
+    bk::Learner& L = l->L;
+    BK_HIPC(hipMemcpyAsync(L.d_grad, g, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
+    BK_HIPC(hipMemcpyAsync(L.d_hess, h, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
+    L.root_histogram(L.d_grad, L.d_hess, hist_out, int_hist_out, count_out);
+  });
 }
 
 int bk_learner_reset_constant_hessian(bk_learner* l, int is_constant_hessian) {

@@ -168,6 +168,22 @@ int bk_learner_add_prediction_to_score_device(bk_learner* learner, const bk_tree
 int bk_learner_leaf_rows(bk_learner* learner, int32_t leaf, int32_t* rows_out,
                          int64_t* num_out);
 
+/* Root histogram of host gradients over all rows, as
+ * SerialTreeLearner::construct_histograms (crates/lgbm-core/src/treelearner/serial.rs:891-1065)
+ * leaves it for the root before fix_histogram: per group bin b (bins of all groups
+ * concatenated, total_bins = sum of group_num_total_bin),
+ *   float forms:  hist_out[2b] = sum of g, hist_out[2b+1] = sum of h (a constant
+ *                 hessian counts rows and is scaled by hessians[0], serial.rs:982-988);
+ *   quantized:    int_hist_out[2b], [2b+1] = exact integer sums of the discretized
+ *                 gradients (construct_int_histograms, serial.rs:1074-1164);
+ * count_out[b] = rows in bin b. Each output may be NULL; hist_out must be NULL for the
+ * quantized form and int_hist_out for the float forms. It runs the per-tree set-up
+ * of train (discretization included), so a stochastic learner's noise offset
+ * advances as in a train call. Used to report the per-bin error of the FP32
+ * compensated form against an FP64 histogram. */
+int bk_learner_root_histogram(bk_learner* learner, const float* gradients, const float* hessians,
+                              double* hist_out, int64_t* int_hist_out, int32_t* count_out);
+
 /* SerialTreeLearner::reset_constant_hessian (crates/lgbm-core/src/treelearner/serial.rs:279). */
 int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessian);
 

@@ -182,3 +182,17 @@ def search(hist, num_bin, missing_type=0, default_bin=0, most_freq_bin=0, bin_ty
             "right_count": int(iout[3]), "splittable": bool(iout[4]), "gain": dout[0], "left_output": dout[1],
             "right_output": dout[2], "left_sum_gradient": dout[3], "left_sum_hessian": dout[4],
             "right_sum_gradient": dout[5], "right_sum_hessian": dout[6], "cat": cat[: ncat[0]].copy()}
+
+
+def root_hist_f64(dataset, g, h, const_hess):
+    """The reference's root histogram in FP64 (histogram.rs:117-245 per group, rows in
+    order, serial.rs:982-988 for a constant hessian): (total_bins, 2) float64."""
+    cols = dataset.group_columns()
+    nbins = dataset.group_num_total_bin()
+    parts = []
+    for col, nb in zip(cols, nbins):
+        out = hist_f64(col, None, g, h, const_hess, int(nb)).reshape(-1, 2)
+        if const_hess:
+            out[:, 1] *= float(np.float32(h[0]))
+        parts.append(out)
+    return np.concatenate(parts)

@@ -216,3 +216,15 @@ def test_renew_certificate_fallback_bit_exact(batched):
                          stochastic_rounding=1, quant_train_renew_leaf=1, seed=5, form=bg.FORM_QUANTIZED,
                          batched=batched)
     lockstep(ds, WideRangeObjective(20_000, 3), p, iters=3)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("form", [bg.FORM_PLAIN_F64, bg.FORM_F32_COMPENSATED])
+def test_regression_100_nan_float_forms(form, batched):
+    """L2 regression on 100,000 x 100 (normal, uniform and lognormal columns, 5% NaN
+    in the lognormal ones), 255 leaves: the float forms against the oracle."""
+    announce(f"regression 100k x 100 NaN form={form} batched={batched}")
+    x, y = synth.regression_100(100_000, seed=2)
+    ds = bg.Dataset(x)
+    p = bg.LearnerParams(num_leaves=255, form=form, batched=batched)
+    lockstep(ds, bst.RegressionL2Objective(y), p, iters=3, compare=assert_trees_close)
