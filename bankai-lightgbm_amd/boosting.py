@@ -8,7 +8,8 @@ with the `train(g, h)` / `add_prediction_to_score(values, score)` methods: the
 GPU `bankai_gpu.Learner` in production.
 
 Objectives are the closed-form host ones of crates/lgbm-core/src/objective/
-(binary.rs:71-124, regression.rs:127-142,240-330 for L2, multiclass.rs:68-105).
+(binary.rs:71-124, regression.rs:127-142,240-330 for L2, multiclass.rs:68-105,
+rank.rs:183-361 for lambdarank).
 
 `model_to_string` writes the model text of crates/lgbm-core/src/boosting/model_text.rs:88-138
 and crates/lgbm-core/src/tree.rs:520-601 (tree blocks; no feature importances or
@@ -217,6 +218,150 @@ class MulticlassObjective:
     def class_need_train(self, k):
         p = abs(self.init_probs[k])
         return not (p <= K_EPSILON or p >= 1.0 - K_EPSILON)
+
+
+class LambdarankObjective:
+    """LambdarankNdcg (objective/rank.rs:183-361) without position bias.
+
+    Queries are consecutive row ranges (`group_sizes`). The per-pair arithmetic
+    follows GetGradientsForOneQuery (rank.rs:263-340) in f64 with f32 lambda and
+    hessian accumulators; the pairs of one leading document are evaluated as
+    vectors, and that document's own accumulators are summed in pair order
+    (np.add.accumulate), so every f32 addition happens in the reference's order.
+    exp and log2 are the platform libm's (Python's math module), as in the
+    reference (rank.rs:215-223 table, metric/rank.rs:42-47 discounts)."""
+
+    SIGMOID_BINS = 1024 * 1024
+    MAX_POSITION = 10_000
+
+    def __init__(self, label, group_sizes, weights=None, sigmoid=1.0, norm=True, truncation_level=30,
+                 label_gain=None):
+        import math
+        if sigmoid <= 0.0:
+            raise ValueError(f"Sigmoid param {sigmoid} should be greater than zero")
+        self.label = np.asarray(label, np.float32)
+        self.weights = None if weights is None else np.asarray(weights, np.float32)
+        sizes = np.asarray(group_sizes, np.int64)
+        self.bounds = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        if self.bounds[-1] != len(self.label):
+            raise ValueError("query sizes do not add up to the number of rows")
+        if np.any(sizes > self.MAX_POSITION):
+            raise ValueError(f"a query exceeds the upper limit of {self.MAX_POSITION} rows")
+        # metric/rank.rs:32-40 default gains 2^i - 1
+        self.gain = (np.asarray(label_gain, np.float64) if label_gain
+                     else np.array([0.0] + [float((1 << i) - 1) for i in range(1, 31)]))
+        lab_i = self.label.astype(np.int64)
+        if np.any(np.abs(self.label - lab_i.astype(np.float32)) > K_EPSILON) or np.any(self.label < 0):
+            raise ValueError("ranking labels must be non-negative integers")
+        if np.any(lab_i >= len(self.gain)):
+            raise ValueError("a label is not less than the number of label gains")
+        self.discount = np.array([1.0 / math.log2(2.0 + i) for i in range(self.MAX_POSITION)])
+        self.sigmoid = float(sigmoid)
+        self.norm = bool(norm)
+        self.trunc = int(truncation_level)
+        # cal_max_dcg over the top min(trunc, cnt) labels (metric/rank.rs:92-121), x86 madd
+        k_max = max(self.trunc, 0)
+        inv = np.zeros(len(sizes))
+        for q in range(len(sizes)):
+            lab = np.sort(lab_i[self.bounds[q]:self.bounds[q + 1]])[::-1]
+            cur = 0.0
+            for j in range(min(k_max, len(lab))):
+                cur = self.discount[j] * self.gain[lab[j]] + cur
+            inv[q] = 1.0 / cur if cur > 0.0 else cur
+        self.inverse_max_dcg = inv
+        self.min_in = -50.0 / self.sigmoid / 2.0
+        self.max_in = -self.min_in
+        self.factor = self.SIGMOID_BINS / (self.max_in - self.min_in)
+        self.table = np.array([1.0 / (1.0 + math.exp((i / self.factor + self.min_in) * self.sigmoid))
+                               for i in range(self.SIGMOID_BINS)])
+        self.num_class = 1
+        self.const_hess = False
+
+    def name(self):
+        return "lambdarank"
+
+    def boost_from_score(self, k=0):
+        return 0.0
+
+    def _sigmoid_of(self, d):
+        idx = ((np.clip(d, self.min_in, self.max_in) - self.min_in) * self.factor).astype(np.int64)
+        idx = np.minimum(idx, self.SIGMOID_BINS - 1)
+        v = self.table[idx]
+        v = np.where(d <= self.min_in, self.table[0], v)
+        return np.where(d >= self.max_in, self.table[-1], v)
+
+    def _one_query(self, q, label, score, lam, hes):
+        cnt = len(label)
+        if cnt == 0:
+            return
+        inv_max_dcg = self.inverse_max_dcg[q]
+        order = np.argsort(-score, kind="stable")
+        ls = label[order]
+        ss = score[order]
+        best = ss[0]
+        worst_idx = cnt - 1
+        if worst_idx > 0 and ss[worst_idx] == -np.inf:
+            worst_idx -= 1
+        worst = ss[worst_idx]
+        sum_lambdas = 0.0
+        norm_pairs = self.norm and best != worst
+        eps_norm = float(np.float32(0.01))
+        neg_sig = -self.sigmoid
+        sig2 = self.sigmoid * self.sigmoid
+        for i in range(min(cnt - 1, max(self.trunc, 0))):
+            if ss[i] == -np.inf:
+                continue
+            j = np.arange(i + 1, cnt)
+            j = j[(ss[j] != -np.inf) & (ls[j] != ls[i])]
+            if len(j) == 0:
+                continue
+            i_high = ls[i] > ls[j]
+            high_rank = np.where(i_high, i, j)
+            low_rank = np.where(i_high, j, i)
+            high = order[high_rank]
+            low = order[low_rank]
+            delta_score = score[high] - score[low]
+            dcg_gap = self.gain[label[high].astype(np.int64)] - self.gain[label[low].astype(np.int64)]
+            paired_discount = np.abs(self.discount[high_rank] - self.discount[low_rank])
+            dpn = dcg_gap * paired_discount * inv_max_dcg
+            if norm_pairs:
+                dpn = dpn / (eps_norm + np.abs(delta_score))
+            p_lambda = self._sigmoid_of(delta_score)
+            p_hessian = p_lambda * (1.0 - p_lambda)
+            p_lambda = p_lambda * (neg_sig * dpn)
+            p_hessian = p_hessian * (sig2 * dpn)
+            pl32 = p_lambda.astype(np.float32)
+            ph32 = p_hessian.astype(np.float32)
+            # the other document of each pair is touched once for this i
+            other = order[j]
+            lam[other] = lam[other] + np.where(i_high, -pl32, pl32)
+            hes[other] = hes[other] + ph32
+            # the leading document accumulates in pair order
+            me = order[i]
+            lam[me] = np.add.accumulate(np.concatenate([[lam[me]], np.where(i_high, pl32, -pl32)]))[-1]
+            hes[me] = np.add.accumulate(np.concatenate([[hes[me]], ph32]))[-1]
+            sum_lambdas = np.add.accumulate(np.concatenate([[sum_lambdas], -(2.0 * p_lambda)]))[-1]
+        if self.norm and sum_lambdas > 0.0:
+            import math
+            nf = math.log2(1.0 + sum_lambdas) / sum_lambdas
+            lam[:] = (lam.astype(np.float64) * nf).astype(np.float32)
+            hes[:] = (hes.astype(np.float64) * nf).astype(np.float32)
+
+    def gradients(self, score):
+        n = len(self.label)
+        g = np.zeros(n, np.float32)
+        h = np.zeros(n, np.float32)
+        for q in range(len(self.bounds) - 1):
+            lo, hi = int(self.bounds[q]), int(self.bounds[q + 1])
+            lam = np.zeros(hi - lo, np.float32)
+            hes = np.zeros(hi - lo, np.float32)
+            self._one_query(q, self.label[lo:hi], np.asarray(score[lo:hi], np.float64), lam, hes)
+            if self.weights is not None:
+                lam = lam * self.weights[lo:hi]
+                hes = hes * self.weights[lo:hi]
+            g[lo:hi] = lam
+            h[lo:hi] = hes
+        return g, h
 
 
 # ---------------------------------------------------------------------------

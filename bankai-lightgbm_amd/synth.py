@@ -7,6 +7,11 @@ binary_28: n rows x 28 dense f32 features drawn i.i.d. N(0,1); w ~ N(0, 0.5);
     from its own seed stream.
 regression_100: 60 N(0,1), 30 U(0,1), 10 lognormal(0,1) columns with 5% NaN;
     y = sum of 10 random depth-3 step functions + N(0, 0.1).
+multiclass_50: 40 N(0,1) columns and 10 categorical columns uniform over {0..31};
+    y = argmax over 10 classes of X_dense.W_k + Gumbel noise.
+ranking_136: 68 N(0,1) and 68 U(0,1) columns with 20% exact zeros; query sizes
+    floor(lognormal(4, 1)) clipped to [5, 1000]; relevance 0-4 from a noisy linear
+    score cut at its 50/75/90/97% quantiles (skewed toward 0).
 """
 import numpy as np
 
@@ -78,3 +83,37 @@ def regression_100(n, seed=0):
         y += vals[node - 7]
     y += trng.normal(0.0, 0.1, size=n)
     return x, y
+
+
+MULTICLASS_CATEGORICAL = tuple(range(40, 50))
+
+
+def multiclass_50(n, seed=0, num_class=10):
+    rng = np.random.default_rng([seed, 5])
+    x = np.empty((n, 50), np.float32)
+    x[:, :40] = rng.standard_normal(size=(n, 40), dtype=np.float32)
+    x[:, 40:] = rng.integers(0, 32, size=(n, 10)).astype(np.float32)
+    w = np.random.default_rng([seed, 6]).normal(size=(40, num_class))
+    scores = x[:, :40].astype(np.float64) @ w + rng.gumbel(size=(n, num_class))
+    return x, np.argmax(scores, axis=1).astype(np.float64)
+
+
+def ranking_136(n, seed=0):
+    """Returns x, labels, query sizes (the last query is cut to make n rows)."""
+    rng = np.random.default_rng([seed, 8])
+    sizes = []
+    total = 0
+    while total < n:
+        q = int(np.clip(np.floor(rng.lognormal(4.0, 1.0)), 5, 1000))
+        q = min(q, n - total)
+        sizes.append(q)
+        total += q
+    x = np.empty((n, 136), np.float32)
+    x[:, :68] = rng.standard_normal(size=(n, 68), dtype=np.float32)
+    u = rng.uniform(size=(n, 68)).astype(np.float32)
+    u[rng.uniform(size=(n, 68)) < 0.2] = 0.0
+    x[:, 68:] = u
+    w = np.random.default_rng([seed, 9]).normal(size=136) / np.sqrt(136.0)
+    s = x.astype(np.float64) @ w + rng.normal(0.0, 0.5, size=n)
+    cuts = np.quantile(s, [0.5, 0.75, 0.9, 0.97])
+    return x, np.searchsorted(cuts, s).astype(np.float64), np.asarray(sizes, np.int32)

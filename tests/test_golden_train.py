@@ -1,7 +1,8 @@
 """End-to-end pin of the oracle's f64 learner against upstream LightGBM 4.7.0 models.
 
 Each case under tests/golden/train/ is a copy of the reference's fixture
-tests/fixtures/train/<case>/ (x.npy, y.npy, params.json, model.txt; produced by
+tests/fixtures/train/<case>/ (x.npy, y.npy, params.json, model.txt, plus w.npy for
+weights and group.npy for query sizes; produced by
 scripts/gen_train_fixtures.py with lightgbm==4.7.0, 1 thread, force_col_wise, seed 7).
 The dataset is binned by this repository's host binning (bankai_gpu.Dataset), the
 trees grown by the CPU oracle through the boosting loop of boosting.py, and every
@@ -32,6 +33,8 @@ def build_case(case):
     y = np.load(os.path.join(d, "y.npy"))
     w = np.load(os.path.join(d, "w.npy")) if os.path.exists(os.path.join(d, "w.npy")) else None
     model = open(os.path.join(d, "model.txt")).read()
+    if os.path.exists(os.path.join(d, "group.npy")):
+        spec["group"] = np.load(os.path.join(d, "group.npy"))
     return spec, p, x, y, w, model
 
 
@@ -56,8 +59,12 @@ def learner_params(p):
     return lp
 
 
-def make_objective(p, y, w):
+def make_objective(p, y, w, group=None):
     obj = p["objective"]
+    if obj == "lambdarank":
+        return bst.LambdarankObjective(y, group, w, sigmoid=p.get("sigmoid", 1.0), norm=p.get("lambdarank_norm", True),
+                                       truncation_level=p.get("lambdarank_truncation_level", 30),
+                                       label_gain=p.get("label_gain"))
     if obj == "binary":
         return bst.BinaryObjective(y, w, sigmoid=p.get("sigmoid", 1.0), scale_pos_weight=p.get("scale_pos_weight", 1.0),
                                    is_unbalance=p.get("is_unbalance", False))
@@ -71,7 +78,7 @@ def make_objective(p, y, w):
 def train_case(case, learner_factory):
     spec, p, x, y, w, model = build_case(case)
     ds = bg.Dataset(x, dataset_params(p), categorical=spec["categorical"])
-    objective = make_objective(p, y, w)
+    objective = make_objective(p, y, w, spec.get("group"))
     lp = learner_params(p)
     learners = [learner_factory(ds, lp, objective.const_hess)]
     gbdt = bst.GBDT(learners, objective, ds.num_data, learning_rate=p.get("learning_rate", 0.1),

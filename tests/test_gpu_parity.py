@@ -171,13 +171,13 @@ def test_golden_models_on_gpu(case):
 
 
 @pytest.mark.parametrize("batched", [0, 1])
-@pytest.mark.parametrize("case", ["cat_default", "cat_nan", "l2_nan", "l2_zero_as_missing", "multiclass"])
+@pytest.mark.parametrize("case", ["cat_default", "cat_nan", "l2_nan", "l2_zero_as_missing", "multiclass", "lambdarank"])
 def test_golden_datasets_quantized_bit_exact(case, batched):
     announce(f"golden dataset {case} quantized batched={batched}, GPU vs oracle")
     import test_golden_train as tg
     spec, p, x, y, w, model = tg.build_case(case)
     ds = bg.Dataset(x, tg.dataset_params(p), categorical=spec["categorical"])
-    obj = tg.make_objective(p, y, w)
+    obj = tg.make_objective(p, y, w, spec.get("group"))
     lp = tg.learner_params(p)
     lp.use_quantized_grad, lp.num_grad_quant_bins, lp.seed, lp.form = 1, 4, 7, bg.FORM_QUANTIZED
     lp.quant_train_renew_leaf = 1
@@ -228,3 +228,64 @@ def test_regression_100_nan_float_forms(form, batched):
     ds = bg.Dataset(x)
     p = bg.LearnerParams(num_leaves=255, form=form, batched=batched)
     lockstep(ds, bst.RegressionL2Objective(y), p, iters=3, compare=assert_trees_close)
+
+
+def lockstep_classes(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise):
+    """Multiclass boosting: each side's one learner trains the K class trees of an
+    iteration in class order on the oracle's softmax gradients."""
+    k_classes, n = obj.num_class, ds.num_data
+    o = ob.OracleLearner(ds, params, obj.const_hess)
+    gpu = bg.Learner(ds, params, obj.const_hess)
+    score = np.zeros(n * k_classes)
+    for k in range(k_classes):
+        score[k * n:(k + 1) * n] += obj.boost_from_score(k)
+    for it in range(iters):
+        g, h = obj.gradients(score)
+        for k in range(k_classes):
+            sl = slice(k * n, (k + 1) * n)
+            to = o.train(g[sl], h[sl])
+            tg = gpu.train(g[sl], h[sl])
+            compare(tg, to, f"iter {it} class {k}")
+            vals = to["leaf_value"] * lr
+            want = score[sl].copy()
+            got = want.copy()
+            o.add_prediction_to_score(vals, want)
+            gpu.add_prediction_to_score(vals, got)
+            assert np.array_equal(got, want), (it, k)
+            score[sl] = want
+
+
+def quantized(p, seed):
+    p.use_quantized_grad, p.num_grad_quant_bins, p.stochastic_rounding = 1, 4, 1
+    p.quant_train_renew_leaf, p.seed, p.num_threads = 1, seed, 4
+    return p
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("form", [bg.FORM_QUANTIZED, bg.FORM_PLAIN_F64])
+def test_multiclass_50_categorical(form, batched):
+    """Softmax over 10 classes on 100,000 x 50 (40 normal, 10 categorical columns
+    over 32 categories), max_bin 63, 127 leaves."""
+    announce(f"multiclass 100k x 50 categorical form={form} batched={batched}")
+    x, y = synth.multiclass_50(100_000, seed=1)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63), categorical=synth.MULTICLASS_CATEGORICAL)
+    p = bg.LearnerParams(num_leaves=127, form=form, batched=batched)
+    if form == bg.FORM_QUANTIZED:
+        quantized(p, 3)
+    lockstep_classes(ds, bst.MulticlassObjective(y, 10), p, iters=2,
+                     compare=assert_trees_bitwise if form == bg.FORM_QUANTIZED else assert_trees_close)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("form", [bg.FORM_QUANTIZED, bg.FORM_PLAIN_F64])
+def test_lambdarank_136_uneven_queries(form, batched):
+    """LambdaRank (NDCG) on 30,000 documents x 136 in queries of 5-1000 rows
+    (lognormal sizes), 255 leaves."""
+    announce(f"lambdarank 30k x 136 form={form} batched={batched}")
+    x, y, q = synth.ranking_136(30_000, seed=1)
+    ds = bg.Dataset(x)
+    p = bg.LearnerParams(num_leaves=255, form=form, batched=batched)
+    if form == bg.FORM_QUANTIZED:
+        quantized(p, 5)
+    lockstep(ds, bst.LambdarankObjective(y, q), p, iters=3,
+             compare=assert_trees_bitwise if form == bg.FORM_QUANTIZED else assert_trees_close)
