@@ -119,6 +119,7 @@ EXPORTS = {
     "bk_learner_profile": ([_P, C.c_int], C.c_int),
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
     "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
+    "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
                               C.c_int),
@@ -410,9 +411,10 @@ class Learner:
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
                 "speculative_waste": s.speculative_waste}
 
-    def profile(self, kinds=True):
+    def profile(self, kinds=True, every_trees=1):
         """Per-kernel HIP-event timing: True for every kind, False for none, or a
-        list of names from KERNEL_KINDS. Resets the counters."""
+        list of names from KERNEL_KINDS, on the launches of every `every_trees`-th
+        tree. Resets the counters."""
         if kinds is True:
             mask = (1 << len(KERNEL_KINDS)) - 1
         elif not kinds:
@@ -420,6 +422,7 @@ class Learner:
         else:
             mask = sum(1 << KERNEL_KINDS.index(k) for k in kinds)
         _check(self._lib.bk_learner_profile(self.handle, mask))
+        _check(self._lib.bk_learner_profile_trees(self.handle, int(every_trees)))
 
     def kernel_stats(self):
         out = {}

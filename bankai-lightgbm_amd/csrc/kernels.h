@@ -231,6 +231,14 @@ struct ScanItem {
   int has_large, use_subtract, used_off, out_off;
 };
 
+// Kernel timing. While a source is set (per host thread), every launch below takes
+// a fresh (start, stop) pair of timing events (hipEvent_t as void*) from it and is
+// launched with hipExtLaunchKernel, which stamps the kernel's own begin and end
+// into the pair: no marker packets are queued between kernels. The source
+// returns false to launch without events.
+typedef bool (*EventPairSource)(void* ctx, void** start, void** stop);
+void set_event_source(EventPairSource fn, void* ctx);
+
 // Launchers (kernels.hip). All run on `stream` (a hipStream_t passed as void*).
 // Records of buffer 0 from column-major bins (group g at cols + g * col_stride).
 void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream);
@@ -282,6 +290,10 @@ void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_
                       double* score, int max_seg, void* stream);
 void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream);
 int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream);
+// Copies n bytes with a kernel on `stream`. Either side may be pinned host memory
+// (its device address): small transfers then stay on the compute queue instead
+// of a DMA engine, whose completion the queue would wait for.
+void launch_copy(void* dst, const void* src, size_t n, void* stream);
 void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
                     void* stream);
 void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,

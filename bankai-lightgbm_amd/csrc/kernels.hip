@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include <cfloat>
@@ -24,6 +25,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+
+#include <hip/hip_ext.h>
 
 #include "kernels.h"
 
@@ -34,6 +37,26 @@
   } while (0)
 
 namespace bk {
+
+namespace {
+thread_local EventPairSource t_event_source = nullptr;
+thread_local void* t_event_ctx = nullptr;
+
+template <typename F, typename... Args>
+void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
+  void *start = nullptr, *stop = nullptr;
+  if (t_event_source && t_event_source(t_event_ctx, &start, &stop))
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, st, (hipEvent_t)start, (hipEvent_t)stop, 0u,
+                          args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+}  // namespace
+
+void set_event_source(EventPairSource fn, void* ctx) {
+  t_event_source = fn;
+  t_event_ctx = ctx;
+}
 
 namespace {
 
@@ -444,8 +467,9 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
-constexpr int kPartChunks = 4;  // 16-byte chunks per thread per tile
+constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
 constexpr int kSpinLimit = 1 << 22;
+constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
 
 __device__ __forceinline__ unsigned long long status_word(unsigned flag, unsigned epoch, unsigned value) {
   return ((unsigned long long)flag << 62) | ((unsigned long long)(epoch & 0x3fffffffu) << 32) |
@@ -477,11 +501,29 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const int rpt = (R + kPartThreads - 1) / kPartThreads;
   uint32_t* dstp = (uint32_t*)smem;
   uint8_t* lflag = (uint8_t*)(dstp + R);
+  // the items' first tiles go to LDS while the ticket is taken: the item search
+  // then costs one global load latency instead of one per bisection step
+  __shared__ int sh_first[kPartSearchItems];
+  const bool lds_search = nitems <= kPartSearchItems;
   if (threadIdx.x == 0) sh_tile = (int)(atomicAdd(ticket, 1ull) - ticket_base);
+  if (lds_search)
+    for (int i = threadIdx.x; i < nitems; i += kPartThreads) sh_first[i] = items[i].first_task;
   __syncthreads();
   const int t = sh_tile;
   if (t < 0 || t >= ntasks) return;  // one ticket per block: never taken
-  const PartItem& it = items[find_item(items, nitems, t)];
+  int item;
+  if (lds_search) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sh_first[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    item = lo;
+  } else {
+    item = find_item(items, nitems, t);
+  }
+  const PartItem& it = items[item];
   const int blk = t - it.first_task;
   const Segment seg = it.seg;
   const RuleDev& rule = it.rule;  // by reference: a local copy of its bitset would live in scratch
@@ -2098,19 +2140,19 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_hist<FORM_QUANT>);
-      hipLaunchKernelGGL(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+      launch(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
                          ntiles);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
-      hipLaunchKernelGGL(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+      launch(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
                          ntiles);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
-      hipLaunchKernelGGL(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
+      launch(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
                          ntiles);
       break;
@@ -2132,12 +2174,12 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_partition<FORM_QUANT>);
-      hipLaunchKernelGGL(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
+      launch(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
                          nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
     default:
       allow_lds(k_partition<FORM_F64>);
-      hipLaunchKernelGGL(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
+      launch(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
                          nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
       break;
   }
@@ -2149,7 +2191,7 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("row record too large");
   allow_lds(k_to_records);
   const unsigned blocks = (unsigned)((s.num_data + kRecRows - 1) / kRecRows);
-  hipLaunchKernelGGL(k_to_records, dim3(blocks), dim3(256), lds, (hipStream_t)stream, s, cols, col_stride);
+  launch(k_to_records, dim3(blocks), dim3(256), lds, (hipStream_t)stream, s, cols, col_stride);
   BK_HIP(hipGetLastError());
 }
 
@@ -2159,13 +2201,13 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
-      hipLaunchKernelGGL(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items);
       break;
     case FORM_F64:
-      hipLaunchKernelGGL(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items);
       break;
     default:
-      hipLaunchKernelGGL(k_reduce<FORM_F32C>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_F32C>, grid, dim3(256), 0, st, s, d_items);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -2179,23 +2221,23 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
   dim3 grid((unsigned)s.num_features, (unsigned)nitems);
   switch (s.form) {
     case FORM_QUANT:
-      hipLaunchKernelGGL(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
     case FORM_F64:
-      hipLaunchKernelGGL(k_scan<FORM_F64>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_F64>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
     default:
-      hipLaunchKernelGGL(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
       break;
   }
   BK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, st, d_items, d_out, d_used, s.num_features,
+  launch(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, st, d_items, d_out, d_used, s.num_features,
                      d_best, d_flags);
   BK_HIP(hipGetLastError());
 }
 
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream) {
-  hipLaunchKernelGGL(k_max_abs, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, n,
+  launch(k_max_abs, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, n,
                      d_out4);
   BK_HIP(hipGetLastError());
 }
@@ -2203,20 +2245,20 @@ void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, voi
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream) {
-  hipLaunchKernelGGL(k_discretize, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
+  launch(k_discretize, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
                      hnoise, n, start, inv_g, inv_h, const_hess, stochastic, packed, d_sums3);
   BK_HIP(hipGetLastError());
 }
 
 void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream) {
-  hipLaunchKernelGGL(k_pack_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, g, h, n,
+  launch(k_pack_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, g, h, n,
                      const_hess, (float2*)gh);
   BK_HIP(hipGetLastError());
 }
 
 void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
                        const float* h, const int* d_flags, double* d_sums, void* stream) {
-  hipLaunchKernelGGL(k_renew_sums, dim3((unsigned)num_leaves), dim3(128), 0, (hipStream_t)stream, s, d_segs, d_order, g,
+  launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(128), 0, (hipStream_t)stream, s, d_segs, d_order, g,
                      h, d_flags, d_sums);
   BK_HIP(hipGetLastError());
 }
@@ -2224,34 +2266,56 @@ void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_or
 void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
                       double* score, int max_seg, void* stream) {
   dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
-  hipLaunchKernelGGL(k_add_score, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, d_leaf_values, score);
+  launch(k_add_score, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, d_leaf_values, score);
   BK_HIP(hipGetLastError());
 }
 
 void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream) {
   if (seg.count <= 0) return;
-  hipLaunchKernelGGL(k_leaf_rows, dim3(grid_for(seg.count, 256, 1024)), dim3(256), 0, (hipStream_t)stream, s, seg,
+  launch(k_leaf_rows, dim3(grid_for(seg.count, 256, 1024)), dim3(256), 0, (hipStream_t)stream, s, seg,
                      out);
   BK_HIP(hipGetLastError());
 }
 
 void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
                              void* stream) {
-  hipLaunchKernelGGL(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score,
+  launch(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score,
                      label, n, sigmoid, g, h);
+  BK_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_copy(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, size_t n,
+                                              int vec) {
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t done = 0;
+  if (vec) {
+    const size_t n16 = n / 16;
+    for (size_t i = i0; i < n16; i += step) ((uint4*)dst)[i] = ((const uint4*)src)[i];
+    done = n16 * 16;
+  }
+  for (size_t i = done + i0; i < n; i += step) dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, size_t n, void* stream) {
+  if (n == 0) return;
+  const int vec = ((((uintptr_t)dst) | ((uintptr_t)src)) & 15u) == 0 ? 1 : 0;
+  const size_t units = vec ? (n + 15) / 16 : n;
+  const unsigned grid = (unsigned)std::min<size_t>(64, (units + 255) / 256);
+  launch(k_copy, dim3(grid), dim3(256), 0, (hipStream_t)stream, (uint8_t*)dst, (const uint8_t*)src, n, vec);
   BK_HIP(hipGetLastError());
 }
 
 int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream) {
   const int nb = grid_for(n, 256, 1024);
-  hipLaunchKernelGGL(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, n, d_partials);
+  launch(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, n, d_partials);
   BK_HIP(hipGetLastError());
   return nb;
 }
 
 void launch_leaf_of_pos(const DevState& s, const Segment* d_sorted, const int* d_sorted_leaf, int nseg,
                         int32_t* leaf_of, void* stream) {
-  hipLaunchKernelGGL(k_leaf_of_pos, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), 0, (hipStream_t)stream, s,
+  launch(k_leaf_of_pos, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), 0, (hipStream_t)stream, s,
                      d_sorted, d_sorted_leaf, nseg, leaf_of);
   BK_HIP(hipGetLastError());
 }
@@ -2265,21 +2329,21 @@ void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float
   const size_t lds = renew_par_lds(num_leaves);
   if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_renew_par,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_renew_par, dim3(renew_parts(n)), dim3(256), lds, (hipStream_t)stream, n, leaf_of, g, h,
+  launch(k_renew_par, dim3(renew_parts(n)), dim3(256), lds, (hipStream_t)stream, n, leaf_of, g, h,
                      num_leaves, part_sums, part_meta);
   BK_HIP(hipGetLastError());
 }
 
 void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const double* part_sums, const int* part_meta,
                       double* d_sums, int* d_flags, void* stream) {
-  hipLaunchKernelGGL(k_renew_fix, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, d_segs, num_leaves,
+  launch(k_renew_fix, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, d_segs, num_leaves,
                      nparts, part_sums, part_meta, d_sums, d_flags);
   BK_HIP(hipGetLastError());
 }
 
 void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
                            void* stream) {
-  hipLaunchKernelGGL(k_add_score_rows, dim3(grid_for(n, 256, 8192)), dim3(256), 0, (hipStream_t)stream, n, leaf_of,
+  launch(k_add_score_rows, dim3(grid_for(n, 256, 8192)), dim3(256), 0, (hipStream_t)stream, n, leaf_of,
                      d_leaf_values, num_leaves, score);
   BK_HIP(hipGetLastError());
 }
@@ -2287,7 +2351,7 @@ void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_v
 void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
                     void* stream) {
   dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
-  hipLaunchKernelGGL(k_leaf_of, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, leaf_of);
+  launch(k_leaf_of, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, leaf_of);
   BK_HIP(hipGetLastError());
 }
 

@@ -27,6 +27,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bankai_gpu.h"
@@ -331,15 +332,21 @@ T* dmalloc(size_t n) {
   return (T*)p;
 }
 
-// Event pairs around launches on the learner's stream, resolved at the end of a tree.
+// Per-launch timing on the learner's stream: every kernel launched inside a timed
+// region gets its own pair of events stamped by the launch itself (no marker
+// packets between kernels); the pairs are resolved at the end of a tree.
 struct Prof {
+  static constexpr size_t kResolveAt = 4096;  // events pending before they are read back
   bool on = false;
+  int every = 1;        // time the launches of every `every`-th tree
+  int64_t trees = 0;    // train calls since profiling was set
+  bool tree_on = true;  // the current tree is timed
   unsigned mask = 0;  // bit k: time launches of kind k
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Rec {
     int kind;
-    size_t ev;
+    size_t first, count;  // event pairs first .. first + 2 * count - 1
     double algo, moved;
   };
   std::vector<Rec> pending;
@@ -349,27 +356,33 @@ struct Prof {
   hipEvent_t ev(size_t i) {
     while (pool.size() <= i) {
       hipEvent_t e;
-      // timing only: no system-scope fence (a cache writeback per record)
-      BK_HIPC(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+      BK_HIPC(hipEventCreateWithFlags(&e, hipEventDefault));
       pool.push_back(e);
     }
     return pool[i];
   }
-  size_t begin(hipStream_t s) {
-    size_t i = used;
-    used += 2;
-    BK_HIPC(hipEventRecord(ev(i), s));
-    return i;
+  static bool source(void* ctx, void** start, void** stop) {
+    Prof* p = (Prof*)ctx;
+    const size_t i = p->used;
+    p->used += 2;
+    *start = (void*)p->ev(i);
+    *stop = (void*)p->ev(i + 1);
+    p->pending.back().count += 1;
+    return true;
   }
-  void end(hipStream_t s, size_t i, int kind, double a, double m) {
-    BK_HIPC(hipEventRecord(ev(i + 1), s));
-    pending.push_back(Rec{kind, i, a, m});
+  void arm(int kind, double a, double m) {
+    pending.push_back(Rec{kind, used, 0, a, m});
+    set_event_source(&Prof::source, this);
   }
+  void disarm() { set_event_source(nullptr, nullptr); }
   void resolve() {
     for (const Rec& r : pending) {
-      float t = 0.0f;
-      BK_HIPC(hipEventElapsedTime(&t, pool[r.ev], pool[r.ev + 1]));
-      ms[r.kind] += t;
+      if (r.count == 0) continue;
+      for (size_t j = 0; j < r.count; ++j) {
+        float t = 0.0f;
+        BK_HIPC(hipEventElapsedTime(&t, pool[r.first + 2 * j], pool[r.first + 2 * j + 1]));
+        ms[r.kind] += t;
+      }
       launches[r.kind] += 1;
       algo[r.kind] += r.algo;
       moved[r.kind] += r.moved;
@@ -445,7 +458,7 @@ struct Learner {
   // staging arenas: every round's items go up in one copy, results come back in one
   uint8_t* d_arena = nullptr;
   uint8_t* h_arena = nullptr;
-  uint8_t* d_back = nullptr;
+  uint8_t* d_back = nullptr;  // device address of h_back (not a device allocation)
   uint8_t* h_back = nullptr;
   size_t arena_bytes = 0, arena_cur = 0, arena_sent = 0;
   size_t off_fold = 0;
@@ -470,7 +483,9 @@ struct Learner {
   long long* d_sums3 = nullptr;
   double* d_sum_partials = nullptr;
   Segment* d_segs = nullptr;
-  double* d_renew = nullptr;
+  double* d_renew = nullptr;  // device address of the renewal sums inside h_tail
+  uint8_t* h_tail = nullptr;  // pinned: [error flag, 16 bytes][renewal sums, 2 per leaf]
+  uint8_t* h_misc = nullptr;  // pinned landing area of the per-tree set-up's small reads
   int* d_order = nullptr;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
@@ -526,16 +541,22 @@ struct Learner {
   std::vector<char> node_flags;  // splittable flags of node i at [i * F, (i + 1) * F)
   char* flags_of(int node) { return node_flags.data() + (size_t)node * F; }
 
+  static constexpr size_t kMiscBytes = 16 + sizeof(double) * 2 * (2048 + 2);
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
   void timed(int kind, double algo, double moved, Fn&& launch) {
-    if (!((prof.mask >> kind) & 1u)) {
+    if (!((prof.mask >> kind) & 1u) || !prof.tree_on) {
       launch();
       return;
     }
-    size_t i = prof.begin(stream);
-    launch();
-    prof.end(stream, i, kind, algo, moved);
+    prof.arm(kind, algo, moved);
+    try {
+      launch();
+    } catch (...) {
+      prof.disarm();
+      throw;
+    }
+    prof.disarm();
   }
 
   // host time spent blocked on the device during the current train call
@@ -544,6 +565,7 @@ struct Learner {
     const auto w0 = std::chrono::steady_clock::now();
     BK_HIPC(hipStreamSynchronize(stream));
     wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    staged_pending = false;  // every staged copy has completed
   }
 
   ~Learner() { release(); }
@@ -553,7 +575,10 @@ struct Learner {
       if (p) (void)hipFree(p);
     };
     auto fh = [](void* p) {
-      if (p) (void)hipHostFree(p);
+      if (p) {
+        aliases().erase(p);
+        (void)hipHostFree(p);
+      }
     };
     for (int b = 0; b < 3; ++b) {
       f(d_rec[b]);
@@ -562,13 +587,13 @@ struct Learner {
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
                     (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
-                    (void*)d_gh0, (void*)d_arena, (void*)d_back, (void*)d_group_used, (void*)d_out,
+                    (void*)d_gh0, (void*)d_arena, (void*)d_group_used, (void*)d_out,
                     (void*)d_max4, (void*)d_sums3,
-                    (void*)d_sum_partials, (void*)d_segs, (void*)d_renew, (void*)d_order, (void*)d_leaf_vals,
+                    (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
-    for (void* p : {(void*)h_arena, (void*)h_back,
+    for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
                     (void*)h_sorted_leaf})
       fh(p);
@@ -581,6 +606,18 @@ struct Learner {
   template <typename T>
   static void pinned(T** p, size_t n) {
     BK_HIPC(hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault));
+    aliases()[(void*)*p] = nullptr;
+  }
+  // Device address of a pinned host buffer (looked up once per buffer).
+  static std::unordered_map<void*, void*>& aliases() {
+    static thread_local std::unordered_map<void*, void*> m;
+    return m;
+  }
+  template <typename T>
+  static T* dev_alias(T* h) {
+    void*& d = aliases()[(void*)h];
+    if (!d) BK_HIPC(hipHostGetDevicePointer(&d, (void*)h, 0));
+    return (T*)d;
   }
 
   void create(const bk_dataset_view* v, const bk_learner_params* p, int is_const_hess) {
@@ -763,8 +800,9 @@ struct Learner {
       pinned(&h_arena, arena_bytes);
       off_fold = up((size_t)2 * max_items * F);
       const size_t back = off_fold + sizeof(FoldOut) * 2 * (size_t)max_items;
-      d_back = dmalloc<uint8_t>(back);
+      // the fold writes its results straight into pinned host memory (no copy back)
       pinned(&h_back, back);
+      BK_HIPC(hipHostGetDevicePointer((void**)&d_back, h_back, 0));
       d_flags = d_back;
       h_flags = h_back;
       d_fold = (FoldOut*)(d_back + off_fold);
@@ -774,7 +812,10 @@ struct Learner {
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
     d_segs = dmalloc<Segment>((size_t)P.num_leaves);
-    d_renew = dmalloc<double>((size_t)P.num_leaves * 2);
+    // the renewal kernels write the leaf sums straight into pinned host memory
+    pinned(&h_tail, 16 + sizeof(double) * 2 * (size_t)P.num_leaves);
+    pinned(&h_misc, kMiscBytes);
+    BK_HIPC(hipHostGetDevicePointer((void**)&d_renew, h_tail + 16, 0));
     d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
     pinned(&h_group_used, (size_t)G);
@@ -890,8 +931,9 @@ struct Learner {
       BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
       launch_max_abs(g, h, N, d_max4, stream);
       unsigned mx[4];
-      BK_HIPC(hipMemcpyAsync(mx, d_max4, sizeof mx, hipMemcpyDeviceToHost, stream));
+      launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
       sync();
+      std::memcpy(mx, h_misc, sizeof mx);
       float max_g, max_h;
       std::memcpy(&max_g, &mx[0], 4);
       std::memcpy(&max_h, &mx[1], 4);
@@ -908,8 +950,9 @@ struct Learner {
                             stochastic ? 1 : 0, (int16_t*)d_gh0, d_sums3, stream);
         });
         long long sums[4];
-        BK_HIPC(hipMemcpyAsync(sums, d_sums3, sizeof sums, hipMemcpyDeviceToHost, stream));
+        launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream);
         sync();
+        std::memcpy(sums, h_misc, sizeof sums);
         last_int_g = sums[0];
         last_int_h = sums[1];
         // exact when every sequential partial sum fits the f64 significand
@@ -945,11 +988,14 @@ struct Learner {
     } else {
       launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
       int nb = launch_sum_gh(g, h, N, d_sum_partials, stream);
-      std::vector<double> part((size_t)nb * 2);
-      float hh0 = 0.0f;
-      BK_HIPC(hipMemcpyAsync(part.data(), d_sum_partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
-      BK_HIPC(hipMemcpyAsync(&hh0, h, sizeof(float), hipMemcpyDeviceToHost, stream));
+      if ((size_t)nb * 2 * sizeof(double) + 16 > kMiscBytes) throw std::runtime_error("set-up landing area too small");
+      launch_copy(dev_alias(h_misc) + 16, d_sum_partials, sizeof(double) * 2 * (size_t)nb, stream);
+      launch_copy(dev_alias(h_misc), h, sizeof(float), stream);
       sync();
+      float hh0 = 0.0f;
+      std::memcpy(&hh0, h_misc, sizeof(float));
+      std::vector<double> part((size_t)nb * 2);
+      std::memcpy(part.data(), h_misc + 16, sizeof(double) * part.size());
       for (int b = 0; b < nb; ++b) {
         sg += part[(size_t)2 * b];
         sh += part[(size_t)2 * b + 1];
@@ -1033,19 +1079,21 @@ struct Learner {
     return (T*)(d_arena + off);
   }
 
-  // sends what was packed since the last send
-  void send() {
+  // sends what was packed since the last send; a caller that synchronizes the
+  // stream before the next upload needs no event after the copy (an event is a
+  // marker packet that stalls the queue for several microseconds)
+  void send(bool sync_follows = false) {
     if (arena_cur > arena_sent) {
-      BK_HIPC(hipMemcpyAsync(d_arena + arena_sent, h_arena + arena_sent, arena_cur - arena_sent,
-                             hipMemcpyHostToDevice, stream));
-      staged();
+      const size_t from = arena_sent & ~(size_t)15;  // 16-byte aligned: the bytes before are sent already
+      launch_copy(d_arena + from, dev_alias(h_arena) + from, arena_cur - from, stream);
+      if (!sync_follows) staged();
     }
     arena_sent = arena_cur;
   }
 
   // packs a round's partition items (npart) and search items (nscan, with the
   // histogram plan and nr reduce items) and sends them in one transfer
-  void upload_round(int npart, const HistPlan* hp, int nscan, int nr) {
+  void upload_round(int npart, const HistPlan* hp, int nscan, int nr, bool sync_follows = false) {
     begin_uploads();
     if (npart > 0) d_pitems = put(h_pitems, (size_t)npart);
     if (nscan > 0) {
@@ -1055,7 +1103,7 @@ struct Learner {
       d_ritems = put(h_ritems, (size_t)nr);
       d_used = put(h_used, (size_t)F * nscan);
     }
-    send();
+    send(sync_follows);
   }
 
   // Histograms of the segments hist_seg[0..n): their rows are concatenated and
@@ -1139,7 +1187,7 @@ struct Learner {
   }
 
   // Sums the partials into the slots, searches, folds, and reads the best
-  // splits and splittable flags back (one transfer).
+  // splits and splittable flags back (k_fold writes them to pinned host memory).
   void run_scan(int nitems, int nr) {
     if (nitems <= 0) return;
     ScanParams sp = scan_params();
@@ -1147,10 +1195,7 @@ struct Learner {
       launch_reduce(ds, d_ritems, nr, stream);
       launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, d_fold, d_flags, stream);
     });
-    BK_HIPC(hipMemcpyAsync(h_back, d_back, off_fold + sizeof(FoldOut) * 2 * (size_t)nitems, hipMemcpyDeviceToHost,
-                           stream));
     sync();
-    staged_pending = false;
     stats.rounds += 1;
   }
 
@@ -1297,7 +1342,7 @@ struct Learner {
       si.small.part_first = hist_first[0];
       si.small.nblk = hist_count[0];
       const int nr = plan_scan(1);
-      upload_round(0, &hp, 1, nr);
+      upload_round(0, &hp, 1, nr, true);
       run_hist(hp, N, true);
       stats.committed_hist_rows += N;
       run_scan(1, nr);
@@ -1367,7 +1412,7 @@ struct Learner {
           si.large.nblk = hist_count[1];
         }
         const int nr = plan_scan(1);
-        upload_round(1, &hp, 1, nr);
+        upload_round(1, &hp, 1, nr, true);
         run_partition(1, nt, ps.count);
         run_hist(hp, hrows, false);
         run_scan(1, nr);
@@ -1411,7 +1456,7 @@ struct Learner {
       si.used_off = 0;
       si.out_off = 0;
       const int nr = plan_scan(1);
-      upload_round(0, &hp, 1, nr);
+      upload_round(0, &hp, 1, nr, true);
       run_hist(hp, N, true);
       stats.committed_hist_rows += N;
       run_scan(1, nr);
@@ -1531,7 +1576,7 @@ struct Learner {
         }
         nr = plan_scan(ns);
       }
-      upload_round(ni, &hp, ns, nr);
+      upload_round(ni, &hp, ns, nr, ns > 0);
       run_partition(ni, nt, part_rows);
       if (ns > 0) {
         run_hist(hp, hist_rows, false);
@@ -1585,8 +1630,8 @@ struct Learner {
       h_sorted_segs[i] = ss[(size_t)i];
       h_sorted_leaf[i] = sl[(size_t)i];
     }
-    BK_HIPC(hipMemcpyAsync(d_sorted_segs, h_sorted_segs, sizeof(Segment) * (size_t)n, hipMemcpyHostToDevice, stream));
-    BK_HIPC(hipMemcpyAsync(d_sorted_leaf, h_sorted_leaf, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, stream));
+    launch_copy(d_sorted_segs, dev_alias(h_sorted_segs), sizeof(Segment) * (size_t)n, stream);
+    launch_copy(d_sorted_leaf, dev_alias(h_sorted_leaf), sizeof(int) * (size_t)n, stream);
     staged();
     timed(6, (double)N * 8.0, (double)N * 8.0,
           [&] { launch_leaf_of_pos(ds, d_sorted_segs, d_sorted_leaf, n, d_leaf_of, stream); });
@@ -1597,15 +1642,16 @@ struct Learner {
     const auto t0 = std::chrono::steady_clock::now();
     wait_ms = 0.0;
     leaf_of_valid = false;
+    prof.tree_on = prof.trees++ % prof.every == 0;
     Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
     if (renew) {
       for (int i = 0; i < t.num_leaves; ++i) h_segs[i] = segs[(size_t)i];
-      BK_HIPC(hipMemcpyAsync(d_segs, h_segs, sizeof(Segment) * (size_t)t.num_leaves, hipMemcpyHostToDevice, stream));
+      launch_copy(d_segs, dev_alias(h_segs), sizeof(Segment) * (size_t)t.num_leaves, stream);
       // largest leaves first: the longest chains start on the first blocks dispatched
       for (int i = 0; i < t.num_leaves; ++i) h_order[i] = i;
       std::stable_sort(h_order, h_order + t.num_leaves,
                        [&](int a, int b) { return segs[(size_t)a].count > segs[(size_t)b].count; });
-      BK_HIPC(hipMemcpyAsync(d_order, h_order, sizeof(int) * (size_t)t.num_leaves, hipMemcpyHostToDevice, stream));
+      launch_copy(d_order, dev_alias(h_order), sizeof(int) * (size_t)t.num_leaves, stream);
       const int* flags = nullptr;
       if (d_renew_part_sums && t.num_leaves > 1) {
         // certified parallel sums; the chain runs only for leaves the certificate rejects
@@ -1619,18 +1665,22 @@ struct Learner {
       }
       timed(5, 0.0, 0.0,
             [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
-      std::vector<double> sums((size_t)t.num_leaves * 2);
-      BK_HIPC(hipMemcpyAsync(sums.data(), d_renew, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
-      sync();
+    }
+    // one synchronization for the error flag and the renewal sums
+    launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
+    sync();
+    int err = 0;
+    std::memcpy(&err, h_tail, sizeof(int));
+    if (err) throw std::runtime_error("partition count mismatch on the device");
+    if (renew) {
+      const double* sums = (const double*)(h_tail + 16);
       for (int leaf = 0; leaf < t.num_leaves; ++leaf)
         t.set_leaf_output(leaf, leaf_output_plain(sums[(size_t)2 * leaf], sums[(size_t)2 * leaf + 1], P.lambda_l1,
                                                   P.lambda_l2, P.max_delta_step));
     }
-    int err = 0;
-    BK_HIPC(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, stream));
-    sync();
-    if (err) throw std::runtime_error("partition count mismatch on the device");
-    if (prof.on) prof.resolve();
+    // events are read back when many are pending (the stream is idle here) or
+    // when the statistics are asked for, not per tree
+    if (prof.on && prof.used > Prof::kResolveAt) prof.resolve();
     last_num_leaves = t.num_leaves;
     const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats.wait_ms = wait_ms;
@@ -1685,14 +1735,12 @@ struct Learner {
     ensure_leaf_of(t->num_leaves);
     stage_wait();
     for (int i = 0; i < t->num_leaves; ++i) h_vals[i] = t->leaf_value[i];
-    BK_HIPC(hipMemcpyAsync(d_leaf_vals, h_vals, sizeof(double) * (size_t)t->num_leaves, hipMemcpyHostToDevice,
-                           stream));
+    launch_copy(d_leaf_vals, dev_alias(h_vals), sizeof(double) * (size_t)t->num_leaves, stream);
     staged();
+    // no synchronization: the next use of h_vals waits for this copy (stage_wait),
+    // and the score is read by later work on the same stream
     timed(6, (double)N * 20.0, (double)N * 20.0,
           [&] { launch_add_score_rows(N, d_leaf_of, d_leaf_vals, t->num_leaves, d_score, stream); });
-    sync();
-    staged_pending = false;
-    if (prof.on) prof.resolve();
   }
 
   void add_score_host(const bk_tree_out* t, double* score) {
@@ -1723,7 +1771,7 @@ struct Learner {
     d_hitems = put(h_hitems, (size_t)hp.ni);
     d_hblock = put(h_hblock, (size_t)hp.nblocks);
     d_ritems = put(h_ritems, 1);
-    send();
+    send(true);
     run_hist(hp, N, true);
     launch_reduce(ds, d_ritems, 1, stream);
     std::vector<double> gh((size_t)TB * 2);
@@ -1836,8 +1884,6 @@ int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, con
     bk::Learner& L = l->L;
     L.timed(6, (double)L.N * 20.0, (double)L.N * 20.0,
             [&] { bk::launch_binary_gradients(d_score, d_label, L.N, sigmoid, d_g, d_h, L.stream); });
-    BK_HIPC(hipStreamSynchronize(L.stream));
-    if (L.prof.on) L.prof.resolve();
   });
 }
 
@@ -1850,12 +1896,28 @@ int bk_learner_profile(bk_learner* l, int kinds) {
     l->L.prof.mask = (unsigned)kinds;
     l->L.prof.on = kinds != 0;
     l->L.prof.reset();
+    l->L.prof.trees = 0;
+    l->L.prof.tree_on = true;
+    // create the events now: creating them while launching would add host time
+    if (kinds != 0) l->L.prof.ev(bk::Prof::kResolveAt + 512);
+  });
+}
+
+int bk_learner_profile_trees(bk_learner* l, int every) {
+  return bk::guard([&] {
+    if (every < 1) throw std::invalid_argument("every must be at least 1");
+    l->L.prof.every = every;
+    l->L.prof.trees = 0;
   });
 }
 
 int bk_learner_kernel_stats(bk_learner* l, int kind, bk_kernel_stats* out) {
   return bk::guard([&] {
     if (kind < 0 || kind >= 8) throw std::invalid_argument("kernel kind out of range");
+    if (!l->L.prof.pending.empty()) {
+      l->L.sync();
+      l->L.prof.resolve();
+    }
     out->ms = l->L.prof.ms[kind];
     out->launches = l->L.prof.launches[kind];
     out->algo_bytes = l->L.prof.algo[kind];

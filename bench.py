@@ -65,8 +65,9 @@ def auc(y, s):
     return float((ranks[y > 0].sum() - npos * (npos + 1) / 2) / (npos * nneg))
 
 
-def run_gpu(ds, y, params, steps, warmup, lr, profile):
-    """profile: False, True (every kernel kind) or a list of kind names to time."""
+def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
+    """profile: False, True (every kernel kind) or a list of kind names to time,
+    on the launches of every `every_trees`-th tree."""
     import numpy as np
     import torch
 
@@ -82,6 +83,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
     label = torch.as_tensor(y.astype(np.float32), device=dev)
     g = torch.empty(n, dtype=torch.float32, device=dev)
     h = torch.empty(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()  # the learner's stream reads these tensors
     trees = []
 
     def step(record):
@@ -97,7 +99,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile):
         step(True)
     torch.cuda.synchronize()
     if profile:
-        learner.profile(profile)
+        learner.profile(profile, every_trees)
     stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0,
              "committed_hist_rows": 0, "committed_part_rows": 0, "host_ms": 0.0, "wait_ms": 0.0}
     t0 = time.perf_counter()
@@ -190,6 +192,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--sequential-steps", type=int, default=3)
     ap.add_argument("--breakdown-steps", type=int, default=5)
+    ap.add_argument("--timed-every", type=int, default=4,
+                    help="the measured run times the dominant kernel's launches of every k-th tree (each timed "
+                         "launch costs ~25 us of host time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
@@ -221,7 +226,8 @@ def main():
     _, _, kall, _ = run_gpu(ds, y, qparams, args.breakdown_steps, 1, lr, profile=True)
     dom = max(kall, key=lambda k: kall[k]["ms"])
     # the measured run: only the dominant kernel's launches carry HIP events
-    elapsed, stats, kstats, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=[dom])
+    elapsed, stats, kstats, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=[dom],
+                                             every_trees=args.timed_every)
     its = args.steps / elapsed
     log(f"quantized: {its:.2f} it/s ({1000 * elapsed / args.steps:.1f} ms/iter), dominant kernel {dom}")
 
@@ -250,7 +256,8 @@ def main():
     achieved = ks["algo_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(dom), "kernel": dom,
-                          "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / launches,
+                          "launches": ks["launches"], "timed_trees": f"every {args.timed_every}th tree",
+                          "avg_launch_us": 1000.0 * ks["ms"] / launches,
                           "algo_bytes_per_launch": ks["algo_bytes"] / launches,
                           "moved_bytes_per_launch": ks["moved_bytes"] / launches,
                           "moved_GBs": ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0}
@@ -282,12 +289,19 @@ def main():
         result["sequential_quantized_iters_per_sec"] = args.sequential_steps / sel
         log(f"sequential quantized: {args.sequential_steps / sel:.2f} it/s")
     if not args.no_plain:
-        pparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
+        # the plain drop-in learner: FP64 histograms, one split per step as the
+        # reference grows a tree; and the same form with batched growth
+        pparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=0)
         pel, _, _, ptrees = run_gpu(ds, y, pparams, args.plain_steps, 1, lr, profile=False)
         p_its = args.plain_steps / pel
-        log(f"plain f64: {p_its:.2f} it/s")
+        log(f"plain f64 (drop-in, sequential growth): {p_its:.2f} it/s")
         result["plain_f64_iters_per_sec"] = p_its
         result["speedup_vs_plain_f64"] = its / p_its
+        bparams = dataclasses.replace(pparams, batched=1)
+        bel, _, _, _ = run_gpu(ds, y, bparams, args.plain_steps, 1, lr, profile=False)
+        result["plain_f64_batched_iters_per_sec"] = args.plain_steps / bel
+        result["speedup_vs_plain_f64_batched"] = its / (args.plain_steps / bel)
+        log(f"plain f64 batched: {args.plain_steps / bel:.2f} it/s")
         # AUC at equal tree counts
         k = len(ptrees)
         if args.holdout > 0:

@@ -230,6 +230,11 @@ typedef struct bk_kernel_stats {
 } bk_kernel_stats;
 /* kinds: bit k turns timing of kind k on (0 = off); the counters are reset. */
 int bk_learner_profile(bk_learner* learner, int kinds);
+/* Times only the launches of every `every`-th tree from now on (1, the default:
+ * every tree). Each timed launch costs about 25 microseconds of host time, which
+ * lowers the throughput of a timed run; sampling whole trees keeps the average
+ * over every kind of launch of a tree. */
+int bk_learner_profile_trees(bk_learner* learner, int every);
 int bk_learner_kernel_stats(bk_learner* learner, int kind, bk_kernel_stats* out);
 
 /* Frees the handle and its device memory. NULL is accepted. */
