@@ -274,6 +274,18 @@ __device__ __forceinline__ typename Pay<FORM>::T load_payload(const DevState& s,
   return *(const T*)(rec_of(s, buf) + (size_t)pos * s.stride + s.pay_off);
 }
 
+// Streaming 16-byte accesses: the records a partition moves are read once and
+// written once per launch, so they bypass the caches' normal allocation.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 load_stream(const uint8_t* p) {
+  const u32x4 x = __builtin_nontemporal_load((const u32x4*)p);
+  return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void store_stream(uint8_t* p, uint4 v) {
+  const u32x4 x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, (u32x4*)p);
+}
+
 __device__ __forceinline__ uint32_t lane_word(const uint4& v, int i) {
   return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
@@ -467,7 +479,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
-constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
+constexpr int kPartChunks = 8;  // 16-byte chunks per thread per tile
 constexpr int kSpinLimit = 1 << 22;
 constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
 
@@ -539,7 +551,16 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
-    if (q < total) v[k] = *(const uint4*)(src + 16 * (size_t)q);
+    if (q < total) v[k] = load_stream(src + 16 * (size_t)q);
+  }
+  // row ids of the tile, loaded now so that they arrive with the records (a
+  // load-then-store loop at the end would wait a full memory latency per row)
+  const int32_t* sidx = idx_of(s, seg.buf);
+  int32_t ids[kPartChunks];
+#pragma unroll
+  for (int i = 0; i < kPartChunks; ++i) {
+    const int r = (int)threadIdx.x + i * kPartThreads;
+    if (r < rows) ids[i] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
   }
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
@@ -633,14 +654,17 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
         const int l = (int)dstp[r];
         const long long d = l < tl ? lpos + l : rpos + (l - tl);
         if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
-        *(uint4*)(dst + (size_t)d * stride + 16 * c) = v[k];
+        store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
       }
     }
     int32_t* didx = idx_of(s, it.dst_buf);
-    const int32_t* sidx = idx_of(s, seg.buf);
-    for (int r = threadIdx.x; r < rows; r += kPartThreads) {
-      const int l = (int)dstp[r];
-      didx[l < tl ? lpos + l : rpos + (l - tl)] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
+#pragma unroll
+    for (int i = 0; i < kPartChunks; ++i) {
+      const int r = (int)threadIdx.x + i * kPartThreads;
+      if (r < rows) {
+        const int l = (int)dstp[r];
+        didx[l < tl ? lpos + l : rpos + (l - tl)] = ids[i];
+      }
     }
   }
   if (threadIdx.x == 0 && blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
