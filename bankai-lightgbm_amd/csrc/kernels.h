@@ -177,7 +177,6 @@ struct DevState {
   int num_data, npad, num_groups, num_features, total_bins, form;
   int stride, pay_off;
   uint8_t* rec[3];
-  int32_t* idx[3];               // idx[0] unused (identity)
   void* gh0;                     // payload of buffer 0: int16 packed (quantized) or float2 (g, h)
   const FeatDev* feats;
   const int* group_bin_start;    // first histogram bin of each group
@@ -271,31 +270,42 @@ void launch_discretize(const float* g, const float* h, const double* gnoise, con
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream);
 void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream);
-// Leaf renewal sums: leaf_of from the sorted nonempty final segments, per-block
-// partials, the certificate per leaf, then the sequential chain for flagged leaves
-// (d_flags == nullptr runs the chain for every leaf).
-void launch_leaf_of_pos(const DevState& s, const Segment* d_sorted, const int* d_sorted_leaf, int nseg,
-                        int32_t* leaf_of, void* stream);
+// One internal node of the last tree for the row-to-leaf traversal: the route
+// of every bin value under the rule its split applied in the partition (bit b:
+// bin b goes left, filled with route_left), the group whose bin it reads, and
+// the children (>= 0: an internal node, < 0: ~leaf).
+struct NodeDev {
+  uint32_t left_bins[8];
+  int group, left, right, pad;
+};
+// A committed split as the host hands it over: its rule, group and children.
+struct NodeRule {
+  RuleDev rule;
+  int group, left, right, pad;
+};
+// Leaf of every row: the route bitmaps of the nodes are filled from their rules
+// (one lane per bin value), then each row walks the tree from node 0 over its
+// record in buffer 0 (original row order). num_nodes == 0: every row is in leaf 0.
+void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
+                         int32_t* leaf_of, void* stream);
+// Leaf renewal sums: per-block partials over leaf_of, the certificate per leaf,
+// then the sequential chain for flagged leaves (d_flags == nullptr runs the
+// chain for every leaf).
 int renew_parts(int n);
 size_t renew_par_lds(int num_leaves);
 void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float* h, int num_leaves,
                       double* part_sums, int* part_meta, void* stream);
 void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const double* part_sums, const int* part_meta,
                       double* d_sums, int* d_flags, void* stream);
-void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
+void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int num_leaves, const float* g,
                        const float* h, const int* d_flags, double* d_sums, void* stream);
 void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
                            void* stream);
-void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
-                      double* score, int max_seg, void* stream);
-void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream);
 int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream);
 // Copies n bytes with a kernel on `stream`. Either side may be pinned host memory
 // (its device address): small transfers then stay on the compute queue instead
 // of a DMA engine, whose completion the queue would wait for.
 void launch_copy(void* dst, const void* src, size_t n, void* stream);
-void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
-                    void* stream);
 void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
                              void* stream);
 

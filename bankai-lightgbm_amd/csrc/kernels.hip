@@ -265,7 +265,6 @@ struct Pay<FORM_QUANT> {
 __device__ __forceinline__ uint8_t* rec_of(const DevState& s, int b) {
   return b == 0 ? s.rec[0] : (b == 1 ? s.rec[1] : s.rec[2]);
 }
-__device__ __forceinline__ int32_t* idx_of(const DevState& s, int b) { return b == 1 ? s.idx[1] : s.idx[2]; }
 
 template <int FORM>
 __device__ __forceinline__ typename Pay<FORM>::T load_payload(const DevState& s, int buf, long long pos) {
@@ -475,7 +474,9 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // rows, a block scan ranks the rows, and the tile's left count is published
 // for the tiles after it while wave 0 looks back over the tiles before it
 // (decoupled look-back) for its left offset. Then the chunks and row ids are
-// written to their destinations: two contiguous runs per tile.
+// written to their destinations: two contiguous runs per tile. Rows carry no
+// id: the leaf of every row is found afterwards by walking the tree over the
+// records in their original order.
 // status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
@@ -552,15 +553,6 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) v[k] = load_stream(src + 16 * (size_t)q);
-  }
-  // row ids of the tile, loaded now so that they arrive with the records (a
-  // load-then-store loop at the end would wait a full memory latency per row)
-  const int32_t* sidx = idx_of(s, seg.buf);
-  int32_t ids[kPartChunks];
-#pragma unroll
-  for (int i = 0; i < kPartChunks; ++i) {
-    const int r = (int)threadIdx.x + i * kPartThreads;
-    if (r < rows) ids[i] = seg.buf == 0 ? (int32_t)(t0 + r) : sidx[t0 + r];
   }
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
@@ -655,15 +647,6 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
         const long long d = l < tl ? lpos + l : rpos + (l - tl);
         if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
         store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
-      }
-    }
-    int32_t* didx = idx_of(s, it.dst_buf);
-#pragma unroll
-    for (int i = 0; i < kPartChunks; ++i) {
-      const int r = (int)threadIdx.x + i * kPartThreads;
-      if (r < rows) {
-        const int l = (int)dstp[r];
-        didx[l < tl ? lpos + l : rpos + (l - tl)] = ids[i];
       }
     }
   }
@@ -1813,28 +1796,75 @@ __device__ __forceinline__ int lsb_exponent(float x) {
   return be == 0 ? -149 : be - 150;
 }
 
-// Leaf of every row. The position ranges of the final leaves tile [0, N): a
-// position belongs to the last range (sorted by begin) starting at or before it.
-constexpr int kLeafLds = 1024;
-__global__ void __launch_bounds__(256) k_leaf_of_pos(DevState s, const Segment* __restrict__ sorted,
-                                                     const int* __restrict__ sorted_leaf, int nseg,
-                                                     int32_t* __restrict__ leaf_of) {
-  __shared__ int begin_lds[kLeafLds];
-  const bool in_lds = nseg <= kLeafLds;
-  if (in_lds)
-    for (int i = threadIdx.x; i < nseg; i += blockDim.x) begin_lds[i] = sorted[i].begin;
-  __syncthreads();
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < s.num_data; p += gridDim.x * blockDim.x) {
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      const int b = in_lds ? begin_lds[mid] : sorted[mid].begin;
-      if (b <= p) lo = mid;
-      else hi = mid - 1;
+// Leaf of every row: the row's record in buffer 0 (original order) walks the
+// tree from the root, each internal node sending the row left when the node's
+// route bitmap has the bit of the row's bin in its group, until it reaches a
+// leaf. The nodes sit in LDS; records of up to 64 bytes are held in registers.
+// A path is at most num_nodes long, which bounds every walk.
+__device__ __forceinline__ uint32_t record_byte(const uint4& c0, const uint4& c1, const uint4& c2, const uint4& c3,
+                                                int g) {
+  const int k = g >> 4, w = (g >> 2) & 3;
+  const uint32_t w0 = lane_word(c0, w), w1 = lane_word(c1, w), w2 = lane_word(c2, w), w3 = lane_word(c3, w);
+  const uint32_t x = k == 0 ? w0 : (k == 1 ? w1 : (k == 2 ? w2 : w3));
+  return (x >> (8 * (g & 3))) & 0xffu;
+}
+
+// Route bitmap of node blockIdx.x: lane b evaluates the node's rule on bin b.
+__global__ void __launch_bounds__(256) k_node_bitmaps(const NodeRule* __restrict__ rules, NodeDev* __restrict__ nodes) {
+  const NodeRule& r = rules[blockIdx.x];
+  const uint32_t b = threadIdx.x;
+  const unsigned long long m = __ballot(route_left(r.rule, b));
+  NodeDev& d = nodes[blockIdx.x];
+  if ((threadIdx.x & 63) == 0) {
+    d.left_bins[2 * (threadIdx.x >> 6)] = (uint32_t)m;
+    d.left_bins[2 * (threadIdx.x >> 6) + 1] = (uint32_t)(m >> 32);
+  }
+  if (threadIdx.x == 0) {
+    d.group = r.group;
+    d.left = r.left;
+    d.right = r.right;
+    d.pad = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev* __restrict__ nodes, int num_nodes,
+                                                      int in_lds, int32_t* __restrict__ leaf_of) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const NodeDev* nd = nodes;
+  if (in_lds) {
+    const int words = num_nodes * (int)(sizeof(NodeDev) / 4);
+    for (int i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t*)smem)[i] = ((const uint32_t*)nodes)[i];
+    __syncthreads();
+    nd = (const NodeDev*)smem;
+  }
+  const int stride = s.stride, nc = stride >> 4;
+  for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < s.num_data;
+       row += (long long)gridDim.x * blockDim.x) {
+    const uint8_t* rec = s.rec[0] + (size_t)row * stride;
+    int leaf = 0;
+    if (num_nodes > 0) {
+      const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+      uint4 c0 = z, c1 = z, c2 = z, c3 = z;
+      if (nc <= 4) {
+        c0 = *(const uint4*)rec;
+        if (nc > 1) c1 = *(const uint4*)(rec + 16);
+        if (nc > 2) c2 = *(const uint4*)(rec + 32);
+        if (nc > 3) c3 = *(const uint4*)(rec + 48);
+      }
+      int node = 0;
+      for (int step = 0; step < num_nodes; ++step) {
+        const NodeDev& x = nd[node];
+        const uint32_t bin = nc <= 4 ? record_byte(c0, c1, c2, c3, x.group) : (uint32_t)rec[x.group];
+        // x lives in memory (LDS or global): indexing its bitmap is a plain load
+        const int next = ((x.left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? x.left : x.right;
+        if (next < 0) {
+          leaf = ~next;
+          break;
+        }
+        node = next;
+      }
     }
-    const int buf = sorted[lo].buf;
-    const int row = buf == 0 ? p : idx_of(s, buf)[p];
-    leaf_of[row] = sorted_leaf[lo];
+    leaf_of[row] = leaf;
   }
 }
 
@@ -1932,51 +1962,64 @@ __global__ void __launch_bounds__(256) k_renew_fix(const Segment* __restrict__ s
   }
 }
 
-// The sequential chain for flagged leaves: wave 0 sums gradients, wave 1
-// hessians, on lane 0 in ascending row order. The other lanes gather the next
-// 256 rows into LDS while lane 0 adds the current ones, so the chain is limited
-// by the f64 add latency, not by memory.
-constexpr int kRenewChunk = 256;
-__global__ void __launch_bounds__(128) k_renew_sums(DevState s, const Segment* __restrict__ segs,
+// Sequential f64 sums of one leaf's gradients and hessians in ascending row
+// order (serial.rs:634-655), for the leaves whose certificate failed: the block
+// walks leaf_of in passes of kRenewRows rows; every wave compacts the rows of
+// the leaf among its 1024 rows in row order (ballot ranks) into LDS, and two
+// lanes then add the values one by one (gradients and hessians side by side).
+// The chain is bound by the f64 add latency; the scan of leaf_of is the price of
+// keeping no row ids.
+constexpr int kRenewRows = 4096;
+__global__ void __launch_bounds__(256) k_renew_sums(int n, const int32_t* __restrict__ leaf_of,
                                                     const int* __restrict__ order, const float* __restrict__ g,
                                                     const float* __restrict__ h, const int* __restrict__ flags,
                                                     double* out) {
-  __shared__ float buf[2][2][kRenewChunk];
+  __shared__ float vg[kRenewRows], vh[kRenewRows];
+  __shared__ int wcnt[4];
   const int leaf = order[blockIdx.x];
   const int need = flags ? flags[leaf] : 3;
   if (need == 0) return;  // block-uniform
-  const Segment seg = segs[leaf];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float* src = wave == 0 ? g : h;
-  const int32_t* ix = seg.buf == 0 ? nullptr : idx_of(s, seg.buf);
-  const int nchunk = (seg.count + kRenewChunk - 1) / kRenewChunk;
-  float r[kRenewChunk / 64];
-  auto fetch = [&](int c) {
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double ag = 0.0, ah = 0.0;
+  for (int base = 0; base < n; base += kRenewRows) {
+    unsigned long long m[kRenewRows / 256];
+    float gv[kRenewRows / 256], hv[kRenewRows / 256];
+    int wc = 0;
 #pragma unroll
-    for (int j = 0; j < kRenewChunk / 64; ++j) {
-      const int p = c * kRenewChunk + lane + 64 * j;
-      float v = 0.0f;
-      if (p < seg.count) {
-        const int pos = seg.begin + p;
-        v = src[ix ? ix[pos] : pos];
-      }
-      r[j] = v;
+    for (int j = 0; j < kRenewRows / 256; ++j) {
+      const int r = base + wave * (kRenewRows / 4) + 64 * j + lane;
+      const bool mine = r < n && leaf_of[r] == leaf;
+      m[j] = __ballot(mine);
+      wc += __popcll(m[j]);
+      gv[j] = mine ? g[r] : 0.0f;
+      hv[j] = mine ? h[r] : 0.0f;
     }
-  };
-  double acc = 0.0;
-  if (nchunk > 0) fetch(0);
-  for (int c = 0; c < nchunk; ++c) {
-    float* cur = buf[wave][c & 1];
-#pragma unroll
-    for (int j = 0; j < kRenewChunk / 64; ++j) cur[lane + 64 * j] = r[j];
-    if (c + 1 < nchunk) fetch(c + 1);
+    if (lane == 0) wcnt[wave] = wc;
     __syncthreads();
-    if (lane == 0) {
-      const int n = min(kRenewChunk, seg.count - c * kRenewChunk);
-      for (int i = 0; i < n; ++i) acc += (double)cur[i];
+    int pos = 0, total = 0;
+    for (int w = 0; w < 4; ++w) {
+      if (w < wave) pos += wcnt[w];
+      total += wcnt[w];
     }
+#pragma unroll
+    for (int j = 0; j < kRenewRows / 256; ++j) {
+      if ((m[j] >> lane) & 1ull) {
+        const int at = pos + __popcll(m[j] & below);
+        vg[at] = gv[j];
+        vh[at] = hv[j];
+      }
+      pos += __popcll(m[j]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < total; ++i) ag += (double)vg[i];
+    if (threadIdx.x == 64)
+      for (int i = 0; i < total; ++i) ah += (double)vh[i];
+    __syncthreads();
   }
-  if (lane == 0 && (need & (1 << wave))) out[2 * leaf + wave] = acc;
+  if (threadIdx.x == 0 && (need & 1)) out[2 * leaf] = ag;
+  if (threadIdx.x == 64 && (need & 2)) out[2 * leaf + 1] = ah;
 }
 
 __global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, const double* __restrict__ vals,
@@ -1984,25 +2027,6 @@ __global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, con
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int l = leaf_of[i];
     if (l >= 0 && l < num_leaves) score[i] += vals[l];
-  }
-}
-
-__global__ void k_add_score(DevState s, const Segment* __restrict__ segs, const double* __restrict__ vals,
-                            double* score) {
-  const int leaf = blockIdx.y;
-  const Segment seg = segs[leaf];
-  const double v = vals[leaf];
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
-    const int pos = seg.begin + p;
-    const int row = seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos];
-    score[row] += v;
-  }
-}
-
-__global__ void k_leaf_rows(DevState s, Segment seg, int32_t* out) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
-    const int pos = seg.begin + p;
-    out[p] = seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos];
   }
 }
 
@@ -2047,15 +2071,6 @@ __global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ 
   }
 }
 
-// leaf index of every row, from the leaf segments.
-__global__ void k_leaf_of(DevState s, const Segment* __restrict__ segs, int32_t* leaf_of) {
-  const int leaf = blockIdx.y;
-  const Segment seg = segs[leaf];
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < seg.count; p += gridDim.x * blockDim.x) {
-    const int pos = seg.begin + p;
-    leaf_of[seg.buf == 0 ? pos : idx_of(s, seg.buf)[pos]] = leaf;
-  }
-}
 
 // Sum of the partial histograms of every item into its pool slot. A block
 // covers 32 bins of one item with 8 lanes per bin, each lane summing every 8th
@@ -2280,24 +2295,10 @@ void launch_pack_f32(const float* g, const float* h, int n, int const_hess, floa
   BK_HIP(hipGetLastError());
 }
 
-void launch_renew_sums(const DevState& s, const Segment* d_segs, const int* d_order, int num_leaves, const float* g,
+void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int num_leaves, const float* g,
                        const float* h, const int* d_flags, double* d_sums, void* stream) {
-  launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(128), 0, (hipStream_t)stream, s, d_segs, d_order, g,
-                     h, d_flags, d_sums);
-  BK_HIP(hipGetLastError());
-}
-
-void launch_add_score(const DevState& s, const Segment* d_segs, const double* d_leaf_values, int num_leaves,
-                      double* score, int max_seg, void* stream) {
-  dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
-  launch(k_add_score, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, d_leaf_values, score);
-  BK_HIP(hipGetLastError());
-}
-
-void launch_leaf_rows(const DevState& s, Segment seg, int32_t* out, void* stream) {
-  if (seg.count <= 0) return;
-  launch(k_leaf_rows, dim3(grid_for(seg.count, 256, 1024)), dim3(256), 0, (hipStream_t)stream, s, seg,
-                     out);
+  launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, n, leaf_of, d_order, g, h,
+         d_flags, d_sums);
   BK_HIP(hipGetLastError());
 }
 
@@ -2337,10 +2338,17 @@ int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, voi
   return nb;
 }
 
-void launch_leaf_of_pos(const DevState& s, const Segment* d_sorted, const int* d_sorted_leaf, int nseg,
-                        int32_t* leaf_of, void* stream) {
-  launch(k_leaf_of_pos, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), 0, (hipStream_t)stream, s,
-                     d_sorted, d_sorted_leaf, nseg, leaf_of);
+void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
+                         int32_t* leaf_of, void* stream) {
+  if (num_nodes > 0) {
+    launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes);
+    BK_HIP(hipGetLastError());
+  }
+  const size_t lds = (size_t)num_nodes * sizeof(NodeDev);
+  const int in_lds = lds <= (size_t)kLdsBytes ? 1 : 0;
+  allow_lds(k_leaf_of_tree);
+  launch(k_leaf_of_tree, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), in_lds ? lds : 0, (hipStream_t)stream, s,
+         d_nodes, num_nodes, in_lds, leaf_of);
   BK_HIP(hipGetLastError());
 }
 
@@ -2372,11 +2380,5 @@ void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_v
   BK_HIP(hipGetLastError());
 }
 
-void launch_leaf_of(const DevState& s, const Segment* d_segs, int num_leaves, int max_seg, int32_t* leaf_of,
-                    void* stream) {
-  dim3 grid((unsigned)grid_for(max_seg, 256, 1024), (unsigned)num_leaves);
-  launch(k_leaf_of, grid, dim3(256), 0, (hipStream_t)stream, s, d_segs, leaf_of);
-  BK_HIP(hipGetLastError());
-}
 
 }  // namespace bk

@@ -439,7 +439,6 @@ struct Learner {
   hipStream_t stream = nullptr;
   DevState ds{};
   uint8_t* d_rec[3] = {nullptr, nullptr, nullptr};
-  int32_t* d_idx[3] = {nullptr, nullptr, nullptr};
   void* d_gh0 = nullptr;
   float *d_grad = nullptr, *d_hess = nullptr;
   double *d_gnoise = nullptr, *d_hnoise = nullptr;
@@ -489,13 +488,14 @@ struct Learner {
   int* d_order = nullptr;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
-  Segment* d_sorted_segs = nullptr;
-  int* d_sorted_leaf = nullptr;
+  NodeDev* d_nodes = nullptr;     // internal nodes of the last tree (row-to-leaf walk)
+  NodeRule* d_node_rules = nullptr;
   double* d_renew_part_sums = nullptr;
   int* d_renew_part_meta = nullptr;
   int* d_renew_flags = nullptr;
-  Segment* h_sorted_segs = nullptr;
-  int* h_sorted_leaf = nullptr;
+  NodeRule* h_nodes = nullptr;
+  std::vector<NodeRule> tree_nodes;  // rule and group of every committed split, by internal node
+  std::vector<int> last_left, last_right;  // children of the last tree's internal nodes
   bool leaf_of_valid = false;
   int renew_nparts = 0;
   // pinned host staging
@@ -582,7 +582,6 @@ struct Learner {
     };
     for (int b = 0; b < 3; ++b) {
       f(d_rec[b]);
-      f(d_idx[b]);
     }
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
@@ -590,12 +589,11 @@ struct Learner {
                     (void*)d_gh0, (void*)d_arena, (void*)d_group_used, (void*)d_out,
                     (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
-                    (void*)d_leaf_of, (void*)d_sorted_segs, (void*)d_sorted_leaf, (void*)d_renew_part_sums,
+                    (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc,
-                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_sorted_segs,
-                    (void*)h_sorted_leaf})
+                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -741,7 +739,6 @@ struct Learner {
     BK_HIPC(hipEventCreateWithFlags(&staged_ev, hipEventDisableTiming | hipEventDisableSystemFence));
     for (int b = 0; b < 3; ++b) {
       d_rec[b] = dmalloc<uint8_t>((size_t)npad * stride);
-      if (b > 0) d_idx[b] = dmalloc<int32_t>((size_t)npad);
     }
     d_gh0 = form == FORM_QUANT ? (void*)dmalloc<int16_t>((size_t)npad) : (void*)dmalloc<float>((size_t)npad * 2);
     BK_HIPC(hipMemsetAsync(d_gh0, 0, (size_t)npad * pay_bytes, stream));
@@ -824,10 +821,10 @@ struct Learner {
     pinned(&h_vals, (size_t)P.num_leaves);
     d_leaf_of = dmalloc<int32_t>((size_t)npad);
     BK_HIPC(hipMemsetAsync(d_leaf_of, 0, sizeof(int32_t) * (size_t)npad, stream));
-    d_sorted_segs = dmalloc<Segment>((size_t)P.num_leaves);
-    d_sorted_leaf = dmalloc<int>((size_t)P.num_leaves);
-    pinned(&h_sorted_segs, (size_t)P.num_leaves);
-    pinned(&h_sorted_leaf, (size_t)P.num_leaves);
+    d_nodes = dmalloc<NodeDev>((size_t)P.num_leaves);
+    d_node_rules = dmalloc<NodeRule>((size_t)P.num_leaves);
+    pinned(&h_nodes, (size_t)P.num_leaves);
+    tree_nodes.assign((size_t)P.num_leaves, NodeRule{});
     if (renew) {
       d_renew_flags = dmalloc<int>((size_t)P.num_leaves);
       if (renew_par_lds(P.num_leaves) <= (size_t)kLdsBytes) {
@@ -873,7 +870,6 @@ struct Learner {
     ds.pay_off = pay_off;
     for (int b = 0; b < 3; ++b) {
       ds.rec[b] = d_rec[b];
-      ds.idx[b] = d_idx[b];
     }
     ds.gh0 = d_gh0;
     ds.feats = d_feats;
@@ -1254,6 +1250,9 @@ struct Learner {
       t.th_bin[(size_t)node] = (uint32_t)info.o.threshold;
       t.th[(size_t)node] = upper[(size_t)inner][(size_t)info.o.threshold];
     }
+    NodeRule& nd = tree_nodes[(size_t)(t.num_leaves - 1)];
+    nd.rule = rule_of(info);
+    nd.group = m.group;
     t.num_leaves += 1;
     stats.splits += 1;
     stats.committed_part_rows += (int64_t)lc + rc;
@@ -1605,36 +1604,23 @@ struct Learner {
     return t;
   }
 
-  // grow_tree (serial.rs:578-628) followed by renew_int_grad_tree_output (634-655)
-  // Leaf of every row of the last tree (one scattered write per row).
-  void ensure_leaf_of(int num_leaves) {
+  // Leaf of every row of the last tree: every row walks the tree over its record
+  // in original order (the partition moves no row ids).
+  void ensure_leaf_of() {
     if (leaf_of_valid) return;
     stage_wait();
-    int n = 0;
-    for (int l = 0; l < num_leaves; ++l) {
-      if (segs[(size_t)l].count == 0) continue;
-      h_sorted_segs[n] = segs[(size_t)l];
-      h_sorted_leaf[n] = l;
-      ++n;
+    const int ni = last_num_leaves - 1;
+    for (int k = 0; k < ni; ++k) {
+      h_nodes[k] = tree_nodes[(size_t)k];
+      h_nodes[k].left = last_left[(size_t)k];
+      h_nodes[k].right = last_right[(size_t)k];
     }
-    std::vector<int> perm((size_t)n);
-    for (int i = 0; i < n; ++i) perm[(size_t)i] = i;
-    std::sort(perm.begin(), perm.end(), [&](int a, int b) { return h_sorted_segs[a].begin < h_sorted_segs[b].begin; });
-    std::vector<Segment> ss((size_t)n);
-    std::vector<int> sl((size_t)n);
-    for (int i = 0; i < n; ++i) {
-      ss[(size_t)i] = h_sorted_segs[perm[(size_t)i]];
-      sl[(size_t)i] = h_sorted_leaf[perm[(size_t)i]];
+    if (ni > 0) {
+      launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream);
+      staged();
     }
-    for (int i = 0; i < n; ++i) {
-      h_sorted_segs[i] = ss[(size_t)i];
-      h_sorted_leaf[i] = sl[(size_t)i];
-    }
-    launch_copy(d_sorted_segs, dev_alias(h_sorted_segs), sizeof(Segment) * (size_t)n, stream);
-    launch_copy(d_sorted_leaf, dev_alias(h_sorted_leaf), sizeof(int) * (size_t)n, stream);
-    staged();
-    timed(6, (double)N * 8.0, (double)N * 8.0,
-          [&] { launch_leaf_of_pos(ds, d_sorted_segs, d_sorted_leaf, n, d_leaf_of, stream); });
+    timed(6, (double)N * (stride + 4.0), (double)N * (stride + 4.0),
+          [&] { launch_leaf_of_tree(ds, d_node_rules, d_nodes, ni, d_leaf_of, stream); });
     leaf_of_valid = true;
   }
 
@@ -1644,6 +1630,9 @@ struct Learner {
     leaf_of_valid = false;
     prof.tree_on = prof.trees++ % prof.every == 0;
     Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
+    last_num_leaves = t.num_leaves;
+    last_left.assign(t.left.begin(), t.left.begin() + std::max(t.num_leaves - 1, 0));
+    last_right.assign(t.right.begin(), t.right.begin() + std::max(t.num_leaves - 1, 0));
     if (renew) {
       for (int i = 0; i < t.num_leaves; ++i) h_segs[i] = segs[(size_t)i];
       launch_copy(d_segs, dev_alias(h_segs), sizeof(Segment) * (size_t)t.num_leaves, stream);
@@ -1655,7 +1644,7 @@ struct Learner {
       const int* flags = nullptr;
       if (d_renew_part_sums && t.num_leaves > 1) {
         // certified parallel sums; the chain runs only for leaves the certificate rejects
-        ensure_leaf_of(t.num_leaves);
+        ensure_leaf_of();
         timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
           launch_renew_par(N, d_leaf_of, g, h, t.num_leaves, d_renew_part_sums, d_renew_part_meta, stream);
           launch_renew_fix(d_segs, t.num_leaves, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew,
@@ -1664,7 +1653,7 @@ struct Learner {
         flags = d_renew_flags;
       }
       timed(5, 0.0, 0.0,
-            [&] { launch_renew_sums(ds, d_segs, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
+            [&] { launch_renew_sums(N, d_leaf_of, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
     }
     // one synchronization for the error flag and the renewal sums
     launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
@@ -1681,7 +1670,6 @@ struct Learner {
     // events are read back when many are pending (the stream is idle here) or
     // when the statistics are asked for, not per tree
     if (prof.on && prof.used > Prof::kResolveAt) prof.resolve();
-    last_num_leaves = t.num_leaves;
     const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats.wait_ms = wait_ms;
     stats.host_ms = total - wait_ms;
@@ -1718,12 +1706,6 @@ struct Learner {
     o->cat_threshold_len = (int)t.cat_th.size();
   }
 
-  int max_segment(int num_leaves) const {
-    int m = 1;
-    for (int l = 0; l < num_leaves; ++l) m = std::max(m, segs[(size_t)l].count);
-    return m;
-  }
-
   void check_tree_matches(const bk_tree_out* t) const {
     if (t->num_leaves != last_num_leaves)
       throw std::invalid_argument("the tree does not match the learner's last tree (num_leaves)");
@@ -1732,7 +1714,7 @@ struct Learner {
   void add_score_device(const bk_tree_out* t, double* d_score) {
     check_tree_matches(t);
     if (t->num_leaves <= 1) return;
-    ensure_leaf_of(t->num_leaves);
+    ensure_leaf_of();
     stage_wait();
     for (int i = 0; i < t->num_leaves; ++i) h_vals[i] = t->leaf_value[i];
     launch_copy(d_leaf_vals, dev_alias(h_vals), sizeof(double) * (size_t)t->num_leaves, stream);
@@ -1746,7 +1728,7 @@ struct Learner {
   void add_score_host(const bk_tree_out* t, double* score) {
     check_tree_matches(t);
     if (t->num_leaves <= 1) return;
-    ensure_leaf_of(t->num_leaves);
+    ensure_leaf_of();
     std::vector<int32_t> lo((size_t)N);
     BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
     sync();
@@ -1791,15 +1773,19 @@ struct Learner {
     if (out_cnt) std::memcpy(out_cnt, cnt.data(), sizeof(int32_t) * cnt.size());
   }
 
+  // Rows of a leaf in ascending order (the order of the reference's partition),
+  // read off the row-to-leaf map.
   void leaf_rows(int leaf, int32_t* out, int64_t* n) {
     if (leaf < 0 || leaf >= last_num_leaves) throw std::invalid_argument("leaf out of range");
-    const Segment s = segs[(size_t)leaf];
-    int32_t* d = dmalloc<int32_t>((size_t)std::max(s.count, 1));
-    launch_leaf_rows(ds, s, d, stream);
-    BK_HIPC(hipMemcpyAsync(out, d, sizeof(int32_t) * (size_t)s.count, hipMemcpyDeviceToHost, stream));
+    ensure_leaf_of();
+    std::vector<int32_t> lo((size_t)N);
+    BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
     sync();
-    (void)hipFree(d);
-    *n = s.count;
+    int64_t k = 0;
+    for (int r = 0; r < N; ++r)
+      if (lo[(size_t)r] == leaf) out[k++] = r;
+    if (k != segs[(size_t)leaf].count) throw std::runtime_error("leaf rows disagree with the leaf count");
+    *n = k;
   }
 };
 
