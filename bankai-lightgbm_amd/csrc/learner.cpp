@@ -1047,9 +1047,10 @@ struct Learner {
     return first;
   }
 
-  // Algorithmic bytes per row: a root build reads F bins and the payload P; a
-  // leaf build also reads a 4-byte row index; a partition reads one bin and
-  // reads and writes a 4-byte row index.
+  // Algorithmic bytes per row (the reference's index-based model): a root build
+  // reads F bins and the payload P; a leaf build also reads a 4-byte row index;
+  // a partition reads one bin and reads and writes a 4-byte row index. Moved
+  // bytes are what these kernels touch: whole row records.
   //
   // Histogram of the staged items (blocks 0 .. ntasks-1); block j writes
   // ---- staging: a round's items are written to the host arena, marked, and
@@ -1145,9 +1146,10 @@ struct Learner {
     return hp;
   }
 
-  // Algorithmic bytes per row: a root build reads F bins and the payload P; a
-  // leaf build also reads a 4-byte row index; a partition reads one bin and
-  // reads and writes a 4-byte row index.
+  // Algorithmic bytes per row (the reference's index-based model): a root build
+  // reads F bins and the payload P; a leaf build also reads a 4-byte row index;
+  // a partition reads one bin and reads and writes a 4-byte row index. Moved
+  // bytes are what these kernels touch: whole row records.
   void run_hist(const HistPlan& hp, double rows, bool root) {
     if (hp.ni == 0) return;
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
@@ -1163,7 +1165,7 @@ struct Learner {
     if (ntasks <= 0) return;
     if (ntasks > max_part_tiles) throw std::runtime_error("partition status buffer too small");
     epoch = epoch >= 0x3ffffffeu ? 1u : epoch + 1u;
-    timed(2, part_rows * 9.0, part_rows * (2.0 * stride + 8.0), [&] {
+    timed(2, part_rows * 9.0, part_rows * 2.0 * stride, [&] {
       launch_partition(ds, d_pitems, nitems, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
     });
     ticket_base += (unsigned long long)ntasks;
