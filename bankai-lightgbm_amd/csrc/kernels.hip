@@ -1827,44 +1827,69 @@ __global__ void __launch_bounds__(256) k_node_bitmaps(const NodeRule* __restrict
   }
 }
 
+// Rows of a grid stride walk the tree; two rows per step of each lane, so two
+// independent chains of node loads are in flight.
+template <typename NodePtr>
+__device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num_nodes, int32_t* __restrict__ leaf_of) {
+  const int stride = s.stride, nc = stride >> 4;
+  const long long n = s.num_data, step = 2ll * gridDim.x * blockDim.x;
+  for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < n; r0 += step) {
+    const long long r1 = r0 + (long long)gridDim.x * blockDim.x;
+    const bool has1 = r1 < n;
+    const uint8_t* rec0 = s.rec[0] + (size_t)r0 * stride;
+    const uint8_t* rec1 = s.rec[0] + (size_t)(has1 ? r1 : r0) * stride;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    uint4 a0 = z, a1 = z, a2 = z, a3 = z, b0 = z, b1 = z, b2 = z, b3 = z;
+    if (nc <= 4) {
+      a0 = *(const uint4*)rec0;
+      b0 = *(const uint4*)rec1;
+      if (nc > 1) {
+        a1 = *(const uint4*)(rec0 + 16);
+        b1 = *(const uint4*)(rec1 + 16);
+      }
+      if (nc > 2) {
+        a2 = *(const uint4*)(rec0 + 32);
+        b2 = *(const uint4*)(rec1 + 32);
+      }
+      if (nc > 3) {
+        a3 = *(const uint4*)(rec0 + 48);
+        b3 = *(const uint4*)(rec1 + 48);
+      }
+    }
+    int n0 = 0, n1 = 0;  // current node, or ~leaf once the walk ended
+    for (int k = 0; k < num_nodes && (n0 >= 0 || n1 >= 0); ++k) {
+      if (n0 >= 0) {
+        const int g = nd[n0].group;
+        const uint32_t bin = nc <= 4 ? record_byte(a0, a1, a2, a3, g) : (uint32_t)rec0[g];
+        n0 = ((nd[n0].left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? nd[n0].left : nd[n0].right;
+      }
+      if (n1 >= 0) {
+        const int g = nd[n1].group;
+        const uint32_t bin = nc <= 4 ? record_byte(b0, b1, b2, b3, g) : (uint32_t)rec1[g];
+        n1 = ((nd[n1].left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? nd[n1].left : nd[n1].right;
+      }
+    }
+    leaf_of[r0] = n0 < 0 ? ~n0 : 0;
+    if (has1) leaf_of[r1] = n1 < 0 ? ~n1 : 0;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev* __restrict__ nodes, int num_nodes,
                                                       int in_lds, int32_t* __restrict__ leaf_of) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const NodeDev* nd = nodes;
+  if (num_nodes == 0) {
+    for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < s.num_data;
+         row += (long long)gridDim.x * blockDim.x)
+      leaf_of[row] = 0;
+    return;
+  }
   if (in_lds) {
     const int words = num_nodes * (int)(sizeof(NodeDev) / 4);
     for (int i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t*)smem)[i] = ((const uint32_t*)nodes)[i];
     __syncthreads();
-    nd = (const NodeDev*)smem;
-  }
-  const int stride = s.stride, nc = stride >> 4;
-  for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < s.num_data;
-       row += (long long)gridDim.x * blockDim.x) {
-    const uint8_t* rec = s.rec[0] + (size_t)row * stride;
-    int leaf = 0;
-    if (num_nodes > 0) {
-      const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-      uint4 c0 = z, c1 = z, c2 = z, c3 = z;
-      if (nc <= 4) {
-        c0 = *(const uint4*)rec;
-        if (nc > 1) c1 = *(const uint4*)(rec + 16);
-        if (nc > 2) c2 = *(const uint4*)(rec + 32);
-        if (nc > 3) c3 = *(const uint4*)(rec + 48);
-      }
-      int node = 0;
-      for (int step = 0; step < num_nodes; ++step) {
-        const NodeDev& x = nd[node];
-        const uint32_t bin = nc <= 4 ? record_byte(c0, c1, c2, c3, x.group) : (uint32_t)rec[x.group];
-        // x lives in memory (LDS or global): indexing its bitmap is a plain load
-        const int next = ((x.left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? x.left : x.right;
-        if (next < 0) {
-          leaf = ~next;
-          break;
-        }
-        node = next;
-      }
-    }
-    leaf_of[row] = leaf;
+    walk_rows(s, (const NodeDev*)smem, num_nodes, leaf_of);  // LDS loads
+  } else {
+    walk_rows(s, nodes, num_nodes, leaf_of);
   }
 }
 
