@@ -790,9 +790,14 @@ struct Learner {
       h_ritems = v_ritems.data();
       h_used = v_used.data();
       auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-      arena_bytes = up(sizeof(PartItem) * (size_t)max_items) + up(sizeof(HistItem) * (size_t)max_items) +
-                    up(sizeof(int) * (size_t)kHistMaxBlocks) + up(sizeof(ScanItem) * (size_t)max_items) +
-                    up(sizeof(ReduceItem) * 2 * (size_t)max_items) + up(sizeof(int) * (size_t)F * max_items);
+      // a round's items, or the per-tree renewal upload (segments, order, nodes)
+      const size_t round_bytes = up(sizeof(PartItem) * (size_t)max_items) + up(sizeof(HistItem) * (size_t)max_items) +
+                                 up(sizeof(int) * (size_t)kHistMaxBlocks) + up(sizeof(ScanItem) * (size_t)max_items) +
+                                 up(sizeof(ReduceItem) * 2 * (size_t)max_items) +
+                                 up(sizeof(int) * (size_t)F * max_items);
+      const size_t tree_bytes = up(sizeof(Segment) * (size_t)P.num_leaves) + up(sizeof(int) * (size_t)P.num_leaves) +
+                                up(sizeof(NodeRule) * (size_t)P.num_leaves);
+      arena_bytes = std::max(round_bytes, tree_bytes) + 256;
       d_arena = dmalloc<uint8_t>(arena_bytes);
       pinned(&h_arena, arena_bytes);
       off_fold = up((size_t)2 * max_items * F);
@@ -1608,21 +1613,29 @@ struct Learner {
 
   // Leaf of every row of the last tree: every row walks the tree over its record
   // in original order (the partition moves no row ids).
-  void ensure_leaf_of() {
-    if (leaf_of_valid) return;
-    stage_wait();
-    const int ni = last_num_leaves - 1;
-    for (int k = 0; k < ni; ++k) {
-      h_nodes[k] = tree_nodes[(size_t)k];
-      h_nodes[k].left = last_left[(size_t)k];
-      h_nodes[k].right = last_right[(size_t)k];
+  // internal nodes of the last tree as the walk takes them
+  void fill_nodes(NodeRule* dst) const {
+    for (int k = 0; k + 1 < last_num_leaves; ++k) {
+      dst[k] = tree_nodes[(size_t)k];
+      dst[k].left = last_left[(size_t)k];
+      dst[k].right = last_right[(size_t)k];
     }
-    if (ni > 0) {
+  }
+
+  // `uploaded`: the nodes already on the device (sent with other per-tree data)
+  void ensure_leaf_of(const NodeRule* uploaded = nullptr) {
+    if (leaf_of_valid) return;
+    const int ni = last_num_leaves - 1;
+    const NodeRule* rules = uploaded;
+    if (!rules && ni > 0) {
+      stage_wait();
+      fill_nodes(h_nodes);
       launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream);
       staged();
+      rules = d_node_rules;
     }
     timed(6, (double)N * (stride + 4.0), (double)N * (stride + 4.0),
-          [&] { launch_leaf_of_tree(ds, d_node_rules, d_nodes, ni, d_leaf_of, stream); });
+          [&] { launch_leaf_of_tree(ds, rules, d_nodes, ni, d_leaf_of, stream); });
     leaf_of_valid = true;
   }
 
@@ -1636,26 +1649,31 @@ struct Learner {
     last_left.assign(t.left.begin(), t.left.begin() + std::max(t.num_leaves - 1, 0));
     last_right.assign(t.right.begin(), t.right.begin() + std::max(t.num_leaves - 1, 0));
     if (renew) {
-      for (int i = 0; i < t.num_leaves; ++i) h_segs[i] = segs[(size_t)i];
-      launch_copy(d_segs, dev_alias(h_segs), sizeof(Segment) * (size_t)t.num_leaves, stream);
-      // largest leaves first: the longest chains start on the first blocks dispatched
-      for (int i = 0; i < t.num_leaves; ++i) h_order[i] = i;
-      std::stable_sort(h_order, h_order + t.num_leaves,
+      // one upload (synchronized at the end of the tree): the leaf segments, the
+      // chain order (largest leaves first: the longest chains start on the first
+      // blocks dispatched) and the tree's nodes for the row-to-leaf walk
+      const int nl = t.num_leaves;
+      for (int i = 0; i < nl; ++i) h_order[i] = i;
+      std::stable_sort(h_order, h_order + nl,
                        [&](int a, int b) { return segs[(size_t)a].count > segs[(size_t)b].count; });
-      launch_copy(d_order, dev_alias(h_order), sizeof(int) * (size_t)t.num_leaves, stream);
+      fill_nodes(h_nodes);
+      begin_uploads();
+      const Segment* u_segs = put(segs.data(), (size_t)nl);
+      const int* u_order = put(h_order, (size_t)nl);
+      const NodeRule* u_nodes = put(h_nodes, (size_t)std::max(nl - 1, 1));
+      send(true);
       const int* flags = nullptr;
-      if (d_renew_part_sums && t.num_leaves > 1) {
+      if (d_renew_part_sums && nl > 1) {
         // certified parallel sums; the chain runs only for leaves the certificate rejects
-        ensure_leaf_of();
+        ensure_leaf_of(u_nodes);
         timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
-          launch_renew_par(N, d_leaf_of, g, h, t.num_leaves, d_renew_part_sums, d_renew_part_meta, stream);
-          launch_renew_fix(d_segs, t.num_leaves, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew,
-                           d_renew_flags, stream);
+          launch_renew_par(N, d_leaf_of, g, h, nl, d_renew_part_sums, d_renew_part_meta, stream);
+          launch_renew_fix(u_segs, nl, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew, d_renew_flags,
+                           stream);
         });
         flags = d_renew_flags;
       }
-      timed(5, 0.0, 0.0,
-            [&] { launch_renew_sums(N, d_leaf_of, d_order, t.num_leaves, g, h, flags, d_renew, stream); });
+      timed(5, 0.0, 0.0, [&] { launch_renew_sums(N, d_leaf_of, u_order, nl, g, h, flags, d_renew, stream); });
     }
     // one synchronization for the error flag and the renewal sums
     launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);

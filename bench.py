@@ -86,15 +86,22 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     torch.cuda.synchronize()  # the learner's stream reads these tensors
     trees = []
 
-    def step(record):
+    def gradients():
         learner.binary_gradients_device(score.data_ptr(), label.data_ptr(), 1.0, g.data_ptr(), h.data_ptr())
+
+    def step(record):
+        # one boosting iteration: the tree, the score update (Tree::apply_shrinkage's
+        # values) and the gradients of the next iteration are queued before the
+        # host-side bookkeeping of the tree, which then overlaps the device work
         t = learner.train_device(g.data_ptr(), h.data_ptr())
-        tree = bst.Tree(t)
-        tree.apply_shrinkage(lr)
-        learner.add_prediction_to_score_device(tree.leaf_value, score.data_ptr())
+        learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr())
+        gradients()
         if record:
+            tree = bst.Tree(t)
+            tree.apply_shrinkage(lr)
             trees.append(tree)
 
+    gradients()
     for i in range(warmup):
         step(True)
     torch.cuda.synchronize()
