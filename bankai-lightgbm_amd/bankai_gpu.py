@@ -115,6 +115,7 @@ EXPORTS = {
     "bk_learner_leaf_rows": ([_P, C.c_int32, _P, C.POINTER(C.c_int64)], C.c_int),
     "bk_learner_reset_constant_hessian": ([_P, C.c_int], C.c_int),
     "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
+    "bk_learner_softmax_gradients_device": ([_P, _P, _P, C.c_int, _P, _P], C.c_int),
     "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
     "bk_learner_profile": ([_P, C.c_int], C.c_int),
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
@@ -375,6 +376,11 @@ class Learner:
         _check(self._lib.bk_learner_binary_gradients_device(self.handle, C.c_void_p(d_score_ptr),
                                                             C.c_void_p(d_label_ptr), float(sigmoid),
                                                             C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
+
+    def softmax_gradients_device(self, d_score_ptr, d_label_ptr, num_class, d_g_ptr, d_h_ptr):
+        _check(self._lib.bk_learner_softmax_gradients_device(self.handle, C.c_void_p(d_score_ptr),
+                                                             C.c_void_p(d_label_ptr), int(num_class),
+                                                             C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
 
     def leaf_rows(self, leaf):
         out = np.zeros(self.dataset.num_data, np.int32)

@@ -306,6 +306,8 @@ int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, voi
 // (its device address): small transfers then stay on the compute queue instead
 // of a DMA engine, whose completion the queue would wait for.
 void launch_copy(void* dst, const void* src, size_t n, void* stream);
+void launch_softmax_gradients(const double* score, const float* label, int n, int num_class, float* g, float* h,
+                              void* stream);
 void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
                              void* stream);
 

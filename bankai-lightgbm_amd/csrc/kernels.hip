@@ -2055,6 +2055,27 @@ __global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, con
   }
 }
 
+// Softmax gradients (objective/multiclass.rs:68-90, unweighted) for a device-side
+// boosting loop: score and the outputs hold class k's rows at [k * n, (k + 1) * n).
+// The exponentials are the device's, so the values can differ in the last bit
+// from the host libm's; parity tests compute gradients on the host.
+__global__ void k_softmax_gradients(const double* __restrict__ score, const float* __restrict__ label, int n,
+                                    int num_class, double factor, float* __restrict__ g, float* __restrict__ h) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double wmax = score[i];
+    for (int k = 1; k < num_class; ++k) wmax = fmax(score[(size_t)k * n + i], wmax);
+    double wsum = 0.0;
+    for (int k = 0; k < num_class; ++k) wsum += exp(score[(size_t)k * n + i] - wmax);
+    const int lab = (int)label[i];
+    for (int k = 0; k < num_class; ++k) {
+      const double p = exp(score[(size_t)k * n + i] - wmax) / wsum;
+      const double gi = lab == k ? p - 1.0 : p;
+      g[(size_t)k * n + i] = (float)gi;
+      h[(size_t)k * n + i] = (float)(factor * p * (1.0 - p));
+    }
+  }
+}
+
 // Binary log-loss gradients (objective/binary.rs:71-90) for a device-side boosting loop.
 __global__ void k_binary_gradients(const double* __restrict__ score, const float* __restrict__ label, int n,
                                    double sigmoid, float* __restrict__ g, float* __restrict__ h) {
@@ -2324,6 +2345,14 @@ void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int nu
                        const float* h, const int* d_flags, double* d_sums, void* stream) {
   launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, n, leaf_of, d_order, g, h,
          d_flags, d_sums);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_softmax_gradients(const double* score, const float* label, int n, int num_class, float* g, float* h,
+                              void* stream) {
+  const double factor = (double)num_class / ((double)num_class - 1.0);
+  launch(k_softmax_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score, label, n,
+         num_class, factor, g, h);
   BK_HIP(hipGetLastError());
 }
 

@@ -1893,6 +1893,16 @@ int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, con
   });
 }
 
+int bk_learner_softmax_gradients_device(bk_learner* l, const double* d_score, const float* d_label, int num_class,
+                                        float* d_g, float* d_h) {
+  return bk::guard([&] {
+    if (num_class < 2) throw std::invalid_argument("softmax needs at least two classes");
+    bk::Learner& L = l->L;
+    L.timed(6, (double)L.N * num_class * 16.0 + 4.0 * L.N, (double)L.N * num_class * 16.0 + 4.0 * L.N,
+            [&] { bk::launch_softmax_gradients(d_score, d_label, L.N, num_class, d_g, d_h, L.stream); });
+  });
+}
+
 int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
   return bk::guard([&] { *out = l->L.stats; });
 }

@@ -130,6 +130,52 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     return elapsed, stats, kstats, out_trees
 
 
+def run_multiclass(rows, steps, warmup, lr, params_list, seed=0, num_class=10):
+    """Softmax boosting on the 50-feature multiclass workload (40 normal and 10
+    categorical columns over 32 categories, max_bin 63): gradients on the device,
+    num_class trees per iteration, each tree's score update on the device.
+    Returns iterations per second for every parameter set."""
+    import numpy as np
+    import torch
+
+    import bankai_gpu as bg
+    import boosting as bst
+    import synth
+
+    x, y = synth.multiclass_50(rows, seed=seed, num_class=num_class)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63, num_threads=min(os.cpu_count() or 1, 16)),
+                    categorical=synth.MULTICLASS_CATEGORICAL)
+    obj = bst.MulticlassObjective(y, num_class)
+    n = ds.num_data
+    dev = torch.device("cuda:0")
+    out = []
+    for params in params_list:
+        learner = bg.Learner(ds, params, False)
+        init = np.concatenate([np.full(n, obj.boost_from_score(k)) for k in range(num_class)])
+        score = torch.as_tensor(init, device=dev)
+        label = torch.as_tensor(y.astype(np.float32), device=dev)
+        g = torch.empty(num_class * n, dtype=torch.float32, device=dev)
+        h = torch.empty(num_class * n, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+
+        def iteration():
+            learner.softmax_gradients_device(score.data_ptr(), label.data_ptr(), num_class, g.data_ptr(), h.data_ptr())
+            for k in range(num_class):
+                t = learner.train_device(g.data_ptr() + 4 * k * n, h.data_ptr() + 4 * k * n)
+                learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr() + 8 * k * n)
+
+        for _ in range(warmup):
+            iteration()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            iteration()
+        torch.cuda.synchronize()
+        out.append(steps / (time.perf_counter() - t0))
+        learner.close()
+    return out
+
+
 def device_copy_gbs(mib=1024, reps=10):
     """Bandwidth of a plain streaming kernel (read mib MiB, write mib MiB: an
     elementwise negate), the practical HBM ceiling; its PMC counts also calibrate
@@ -205,6 +251,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
+    ap.add_argument("--multiclass-rows", type=int, default=2_000_000,
+                    help="rows of the 10-class workload measured beside the headline (0 skips it)")
+    ap.add_argument("--multiclass-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -315,6 +364,21 @@ def main():
             result["holdout_auc_at_equal_trees"] = {"trees": k,
                                                     "quantized": auc(hy, predict_raw(qtrees[:k], hx)),
                                                     "plain_f64": auc(hy, predict_raw(ptrees, hx))}
+    if args.multiclass_rows > 0:
+        mq = bg.LearnerParams(num_leaves=127, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                              stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
+                              form=bg.FORM_QUANTIZED, batched=1)
+        mf = bg.LearnerParams(num_leaves=127, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
+        mfs = dataclasses.replace(mf, batched=0)
+        q_its, f_its, fs_its = run_multiclass(args.multiclass_rows, args.multiclass_steps, 1, lr, [mq, mf, mfs])
+        result["other_workloads"] = {
+            f"multiclass_{args.multiclass_rows}x50_10_classes": {
+                "unit": "iter/s (10 trees each)", "num_leaves": 127, "max_bin": 63,
+                "quantized_batched": q_its, "plain_f64_batched": f_its, "plain_f64_drop_in": fs_its,
+                "data": "synthetic: 40 N(0,1) + 10 categorical columns over 32 categories, "
+                        "label = argmax of 10 linear scores + Gumbel noise; softmax gradients on the device"}}
+        log(f"multiclass {args.multiclass_rows} x 50: quantized {q_its:.2f}, f64 batched {f_its:.2f}, "
+            f"f64 drop-in {fs_its:.2f} iter/s")
     if not args.no_cpu_baseline:
         cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                    stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)

@@ -194,6 +194,13 @@ int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessi
 int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
                                        double sigmoid, float* d_gradients, float* d_hessians);
 
+/* Softmax gradients and hessians (crates/lgbm-core/src/objective/multiclass.rs:68-90,
+ * unweighted) for num_class classes from a device score array of num_class * num_data
+ * doubles (class k at [k * num_data, (k + 1) * num_data)), written the same way, on the
+ * learner's stream. A device-side multiclass loop uses it; no FFI counterpart. */
+int bk_learner_softmax_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
+                                        int num_class, float* d_gradients, float* d_hessians);
+
 /* Per-tree counters of the last train call. hist_rows / part_rows count every
  * row the kernels processed (the batched form includes speculative expansions
  * that were never committed); committed_hist_rows / committed_part_rows count

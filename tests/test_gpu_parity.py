@@ -289,3 +289,33 @@ def test_lambdarank_136_uneven_queries(form, batched):
         quantized(p, 5)
     lockstep(ds, bst.LambdarankObjective(y, q), p, iters=3,
              compare=assert_trees_bitwise if form == bg.FORM_QUANTIZED else assert_trees_close)
+
+
+def test_softmax_gradients_device_match_host():
+    """The device softmax gradients (a device-side multiclass loop's objective)
+    against objective/multiclass.rs restated on the host: within 2 f32 ulps
+    (the device exp is not the host libm's)."""
+    announce("softmax gradients on the device, 20k rows x 10 classes")
+    import torch
+    x, y = synth.multiclass_50(20_000, seed=4)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63), categorical=synth.MULTICLASS_CATEGORICAL)
+    learner = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
+    obj = bst.MulticlassObjective(y, 10)
+    n = ds.num_data
+    rng = np.random.default_rng(3)
+    score = rng.normal(0.0, 2.0, size=10 * n)
+    want_g, want_h = obj.gradients(score)
+    dev = torch.device("cuda:0")
+    ts = torch.as_tensor(score, device=dev)
+    tl = torch.as_tensor(y.astype(np.float32), device=dev)
+    tg = torch.empty(10 * n, dtype=torch.float32, device=dev)
+    th = torch.empty(10 * n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    learner.softmax_gradients_device(ts.data_ptr(), tl.data_ptr(), 10, tg.data_ptr(), th.data_ptr())
+    torch.cuda.synchronize()  # the whole device, the learner's stream included
+    learner.close()
+    g, h = tg.cpu().numpy(), th.cpu().numpy()
+    ulp_g = np.spacing(np.abs(want_g).astype(np.float32))
+    ulp_h = np.spacing(np.abs(want_h).astype(np.float32))
+    assert np.all(np.abs(g - want_g) <= 2 * ulp_g)
+    assert np.all(np.abs(h - want_h) <= 2 * ulp_h)
