@@ -116,6 +116,7 @@ EXPORTS = {
     "bk_learner_reset_constant_hessian": ([_P, C.c_int], C.c_int),
     "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
     "bk_learner_softmax_gradients_device": ([_P, _P, _P, C.c_int, _P, _P], C.c_int),
+    "bk_learner_set_tree_slot": ([_P, C.c_int, C.c_int], C.c_int),
     "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
     "bk_learner_profile": ([_P, C.c_int], C.c_int),
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
@@ -376,6 +377,10 @@ class Learner:
         _check(self._lib.bk_learner_binary_gradients_device(self.handle, C.c_void_p(d_score_ptr),
                                                             C.c_void_p(d_label_ptr), float(sigmoid),
                                                             C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
+
+    def set_tree_slot(self, slot, trees_per_iteration):
+        """Train tree `slot` of every `trees_per_iteration` trees of a shared sequence."""
+        _check(self._lib.bk_learner_set_tree_slot(self.handle, int(slot), int(trees_per_iteration)))
 
     def softmax_gradients_device(self, d_score_ptr, d_label_ptr, num_class, d_g_ptr, d_h_ptr):
         _check(self._lib.bk_learner_softmax_gradients_device(self.handle, C.c_void_p(d_score_ptr),

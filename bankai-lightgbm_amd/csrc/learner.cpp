@@ -523,6 +523,7 @@ struct Learner {
   long larger_slot = -1, parent_slot = -1;
   std::vector<Segment> segs;  // segment of every leaf of the last tree
   Mt use_start_eng{0};
+  int tree_slot = 0, tree_period = 1;  // position in a sequence of trees shared by several learners
   double grad_scale = 0.0, hess_scale = 0.0;
   long long last_int_g = 0, last_int_h = 0;
   double last_root_g = 0.0, last_root_h = 0.0;
@@ -943,7 +944,15 @@ struct Learner {
         grad_scale = (double)max_g / (double)(bins / 2);
         hess_scale = const_hess ? (double)max_h : (double)max_h / (double)bins;
         double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
-        int start = stochastic ? (int)use_start_eng.uint_incl((uint32_t)N) : 0;
+        // the rotation of the noise: one draw per tree; a learner that trains tree
+        // tree_slot of every tree_period trees of a shared sequence draws for all of
+        // them and keeps its own (serial.rs:692-703 draws only when discretizing)
+        int start = 0;
+        if (stochastic)
+          for (int j = 0; j < tree_period; ++j) {
+            const int v = (int)use_start_eng.uint_incl((uint32_t)N);
+            if (j == tree_slot) start = v;
+          }
         BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
         const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
         timed(4, db, db, [&] {
@@ -1900,6 +1909,15 @@ int bk_learner_softmax_gradients_device(bk_learner* l, const double* d_score, co
     bk::Learner& L = l->L;
     L.timed(6, (double)L.N * num_class * 16.0 + 4.0 * L.N, (double)L.N * num_class * 16.0 + 4.0 * L.N,
             [&] { bk::launch_softmax_gradients(d_score, d_label, L.N, num_class, d_g, d_h, L.stream); });
+  });
+}
+
+int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
+  return bk::guard([&] {
+    if (trees_per_iteration < 1 || slot < 0 || slot >= trees_per_iteration)
+      throw std::invalid_argument("slot must lie in [0, trees_per_iteration)");
+    l->L.tree_slot = slot;
+    l->L.tree_period = trees_per_iteration;
   });
 }
 

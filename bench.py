@@ -130,11 +130,16 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     return elapsed, stats, kstats, out_trees
 
 
-def run_multiclass(rows, steps, warmup, lr, params_list, seed=0, num_class=10):
+def run_multiclass(rows, steps, warmup, lr, runs, seed=0, num_class=10):
     """Softmax boosting on the 50-feature multiclass workload (40 normal and 10
     categorical columns over 32 categories, max_bin 63): gradients on the device,
     num_class trees per iteration, each tree's score update on the device.
-    Returns iterations per second for every parameter set."""
+    `runs` holds (params, concurrent) pairs: concurrent runs grow the class trees
+    of an iteration at the same time, one learner per class on its own stream and
+    host thread (set_tree_slot keeps the trees those of one learner). Returns
+    iterations per second for every run."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
     import torch
 
@@ -149,8 +154,12 @@ def run_multiclass(rows, steps, warmup, lr, params_list, seed=0, num_class=10):
     n = ds.num_data
     dev = torch.device("cuda:0")
     out = []
-    for params in params_list:
-        learner = bg.Learner(ds, params, False)
+    pool = ThreadPoolExecutor(num_class)
+    for params, concurrent in runs:
+        learners = [bg.Learner(ds, params, False) for _ in range(num_class if concurrent else 1)]
+        if concurrent:
+            for k, lk in enumerate(learners):
+                lk.set_tree_slot(k, num_class)
         init = np.concatenate([np.full(n, obj.boost_from_score(k)) for k in range(num_class)])
         score = torch.as_tensor(init, device=dev)
         label = torch.as_tensor(y.astype(np.float32), device=dev)
@@ -158,11 +167,20 @@ def run_multiclass(rows, steps, warmup, lr, params_list, seed=0, num_class=10):
         h = torch.empty(num_class * n, dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
 
+        def tree(k, learner):
+            t = learner.train_device(g.data_ptr() + 4 * k * n, h.data_ptr() + 4 * k * n)
+            learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr() + 8 * k * n)
+
         def iteration():
-            learner.softmax_gradients_device(score.data_ptr(), label.data_ptr(), num_class, g.data_ptr(), h.data_ptr())
-            for k in range(num_class):
-                t = learner.train_device(g.data_ptr() + 4 * k * n, h.data_ptr() + 4 * k * n)
-                learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr() + 8 * k * n)
+            learners[0].softmax_gradients_device(score.data_ptr(), label.data_ptr(), num_class, g.data_ptr(),
+                                                 h.data_ptr())
+            if concurrent:
+                torch.cuda.synchronize()  # the gradients, for every learner's stream
+                list(pool.map(lambda k: tree(k, learners[k]), range(num_class)))
+                torch.cuda.synchronize()  # every score update, for the next gradients
+            else:
+                for k in range(num_class):
+                    tree(k, learners[0])
 
         for _ in range(warmup):
             iteration()
@@ -172,7 +190,9 @@ def run_multiclass(rows, steps, warmup, lr, params_list, seed=0, num_class=10):
             iteration()
         torch.cuda.synchronize()
         out.append(steps / (time.perf_counter() - t0))
-        learner.close()
+        for lk in learners:
+            lk.close()
+    pool.shutdown()
     return out
 
 
@@ -370,15 +390,17 @@ def main():
                               form=bg.FORM_QUANTIZED, batched=1)
         mf = bg.LearnerParams(num_leaves=127, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
         mfs = dataclasses.replace(mf, batched=0)
-        q_its, f_its, fs_its = run_multiclass(args.multiclass_rows, args.multiclass_steps, 1, lr, [mq, mf, mfs])
+        qc_its, q_its, f_its, fs_its = run_multiclass(args.multiclass_rows, args.multiclass_steps, 1, lr,
+                                                      [(mq, True), (mq, False), (mf, False), (mfs, False)])
         result["other_workloads"] = {
             f"multiclass_{args.multiclass_rows}x50_10_classes": {
                 "unit": "iter/s (10 trees each)", "num_leaves": 127, "max_bin": 63,
-                "quantized_batched": q_its, "plain_f64_batched": f_its, "plain_f64_drop_in": fs_its,
+                "quantized_batched_concurrent_classes": qc_its, "quantized_batched": q_its,
+                "plain_f64_batched": f_its, "plain_f64_drop_in": fs_its,
                 "data": "synthetic: 40 N(0,1) + 10 categorical columns over 32 categories, "
                         "label = argmax of 10 linear scores + Gumbel noise; softmax gradients on the device"}}
-        log(f"multiclass {args.multiclass_rows} x 50: quantized {q_its:.2f}, f64 batched {f_its:.2f}, "
-            f"f64 drop-in {fs_its:.2f} iter/s")
+        log(f"multiclass {args.multiclass_rows} x 50: quantized, classes concurrent {qc_its:.2f}, quantized "
+            f"{q_its:.2f}, f64 batched {f_its:.2f}, f64 drop-in {fs_its:.2f} iter/s")
     if not args.no_cpu_baseline:
         cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                    stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)

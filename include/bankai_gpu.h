@@ -201,6 +201,17 @@ int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_scor
 int bk_learner_softmax_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
                                         int num_class, float* d_gradients, float* d_hessians);
 
+/* Trains tree `slot` of every `trees_per_iteration` trees of a sequence shared by
+ * several learners, e.g. one learner per class of a multiclass iteration, each
+ * on its own stream and host thread, so the class trees grow concurrently. The
+ * learner consumes the stochastic-rounding rotation draws of every tree of the
+ * sequence (GradientDiscretizer, crates/lgbm-core/src/treelearner/gradient_discretizer/mod.rs:172-201)
+ * and keeps its own, so the trees equal those of one learner training the
+ * whole sequence in order, provided no tree of the sequence has its
+ * discretization refused (all-zero or non-finite gradients), which skips a draw.
+ * No reference counterpart: the reference trains the class trees one by one. */
+int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iteration);
+
 /* Per-tree counters of the last train call. hist_rows / part_rows count every
  * row the kernels processed (the batched form includes speculative expansions
  * that were never committed); committed_hist_rows / committed_part_rows count

@@ -319,3 +319,33 @@ def test_softmax_gradients_device_match_host():
     ulp_h = np.spacing(np.abs(want_h).astype(np.float32))
     assert np.all(np.abs(g - want_g) <= 2 * ulp_g)
     assert np.all(np.abs(h - want_h) <= 2 * ulp_h)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_concurrent_class_learners_match_one_learner(batched):
+    """One learner per class, trained concurrently from host threads (each on its
+    own stream, set_tree_slot(k, K)), grows bit for bit the trees of one learner
+    training the classes in order: the stochastic-rounding draws follow the
+    shared sequence."""
+    announce(f"3 class learners in threads vs one learner, batched={batched}")
+    from concurrent.futures import ThreadPoolExecutor
+    x, y = synth.multiclass_50(30_000, seed=6, num_class=3)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63), categorical=synth.MULTICLASS_CATEGORICAL)
+    p = quantized(bg.LearnerParams(num_leaves=31, form=bg.FORM_QUANTIZED, batched=batched), 7)
+    obj = bst.MulticlassObjective(y, 3)
+    n = ds.num_data
+    one = bg.Learner(ds, p, False)
+    many = [bg.Learner(ds, p, False) for _ in range(3)]
+    for k, lk in enumerate(many):
+        lk.set_tree_slot(k, 3)
+    score = np.concatenate([np.full(n, obj.boost_from_score(k)) for k in range(3)])
+    with ThreadPoolExecutor(3) as ex:
+        for it in range(3):
+            g, h = obj.gradients(score)
+            seq = [one.train(g[k * n:(k + 1) * n], h[k * n:(k + 1) * n]) for k in range(3)]
+            conc = list(ex.map(lambda k: many[k].train(g[k * n:(k + 1) * n], h[k * n:(k + 1) * n]), range(3)))
+            for k in range(3):
+                assert_trees_bitwise(conc[k], seq[k], f"iter {it} class {k}")
+                view = score[k * n:(k + 1) * n].copy()
+                many[k].add_prediction_to_score(seq[k]["leaf_value"] * 0.1, view)
+                score[k * n:(k + 1) * n] = view
