@@ -81,6 +81,45 @@ class TreeOutC(C.Structure):
     ]
 
 
+class RankTablesC(C.Structure):
+    _fields_ = [
+        ("query_bounds", C.c_void_p), ("inverse_max_dcg", C.c_void_p), ("label_gain", C.c_void_p),
+        ("discount", C.c_void_p), ("sigmoid_table", C.c_void_p),
+        ("sigmoid", C.c_double), ("min_input", C.c_double), ("max_input", C.c_double), ("table_factor", C.c_double),
+        ("num_queries", C.c_int32), ("num_gain", C.c_int32), ("table_bins", C.c_int32),
+        ("truncation_level", C.c_int32), ("norm", C.c_int32), ("max_query_rows", C.c_int32),
+    ]
+
+
+def rank_tables_on_device(objective, device):
+    """LambdaRank tables of a boosting.LambdarankObjective copied to `device` (torch):
+    returns the RankTablesC and the tensors that back it (keep them alive)."""
+    import torch
+
+    keep = {
+        "bounds": torch.as_tensor(np.asarray(objective.bounds, np.int32), device=device),
+        "inv": torch.as_tensor(np.asarray(objective.inverse_max_dcg, np.float64), device=device),
+        "gain": torch.as_tensor(np.asarray(objective.gain, np.float64), device=device),
+        "discount": torch.as_tensor(np.asarray(objective.discount, np.float64), device=device),
+        "table": torch.as_tensor(np.asarray(objective.table, np.float64), device=device),
+    }
+    t = RankTablesC()
+    t.query_bounds = keep["bounds"].data_ptr()
+    t.inverse_max_dcg = keep["inv"].data_ptr()
+    t.label_gain = keep["gain"].data_ptr()
+    t.discount = keep["discount"].data_ptr()
+    t.sigmoid_table = keep["table"].data_ptr()
+    t.sigmoid, t.min_input, t.max_input = objective.sigmoid, objective.min_in, objective.max_in
+    t.table_factor = objective.factor
+    t.num_queries = len(objective.bounds) - 1
+    t.num_gain = len(objective.gain)
+    t.table_bins = len(objective.table)
+    t.truncation_level = objective.trunc
+    t.norm = int(objective.norm)
+    t.max_query_rows = int(np.max(np.diff(objective.bounds))) if len(objective.bounds) > 1 else 0
+    return t, keep
+
+
 class TrainStatsC(C.Structure):
     _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
                 ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32),
@@ -117,6 +156,7 @@ EXPORTS = {
     "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
     "bk_learner_softmax_gradients_device": ([_P, _P, _P, C.c_int, _P, _P], C.c_int),
     "bk_learner_set_tree_slot": ([_P, C.c_int, C.c_int], C.c_int),
+    "bk_learner_lambdarank_gradients_device": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_last_stats": ([_P, C.POINTER(TrainStatsC)], C.c_int),
     "bk_learner_profile": ([_P, C.c_int], C.c_int),
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
@@ -377,6 +417,12 @@ class Learner:
         _check(self._lib.bk_learner_binary_gradients_device(self.handle, C.c_void_p(d_score_ptr),
                                                             C.c_void_p(d_label_ptr), float(sigmoid),
                                                             C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
+
+    def lambdarank_gradients_device(self, d_score_ptr, d_label_ptr, tables, d_g_ptr, d_h_ptr):
+        """tables: a RankTablesC whose arrays are device pointers."""
+        _check(self._lib.bk_learner_lambdarank_gradients_device(self.handle, C.c_void_p(d_score_ptr),
+                                                                C.c_void_p(d_label_ptr), C.byref(tables),
+                                                                C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
 
     def set_tree_slot(self, slot, trees_per_iteration):
         """Train tree `slot` of every `trees_per_iteration` trees of a shared sequence."""

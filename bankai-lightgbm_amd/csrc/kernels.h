@@ -308,6 +308,19 @@ int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, voi
 void launch_copy(void* dst, const void* src, size_t n, void* stream);
 void launch_softmax_gradients(const double* score, const float* label, int n, int num_class, float* g, float* h,
                               void* stream);
+// LambdaRank tables and switches (objective/rank.rs:183-240).
+struct RankDev {
+  const int* bounds;          // num_queries + 1 row offsets
+  const double* inv_max_dcg;  // per query
+  const double* label_gain;   // per label value
+  const double* discount;     // per rank position
+  const double* sigmoid_table;
+  double sigmoid, min_in, max_in, table_factor;
+  int num_queries, num_gain, table_bins, truncation, norm, pad;
+};
+constexpr int kRankMaxDocs = 2048;  // documents of one query held in LDS by the gradient kernel
+void launch_lambdarank_gradients(const double* score, const float* label, const RankDev& r, float* g, float* h,
+                                 void* stream);
 void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
                              void* stream);
 

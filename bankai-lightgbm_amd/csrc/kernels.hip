@@ -2076,6 +2076,110 @@ __global__ void k_softmax_gradients(const double* __restrict__ score, const floa
   }
 }
 
+// LambdaRank (NDCG) gradients (objective/rank.rs:262-340), one block per query,
+// unweighted, no position bias. The query's documents are ranked by score,
+// descending, ties in document order (a bitonic sort of (score, index) in LDS),
+// as the reference's stable sort does. For every rank i below the truncation
+// level, the lanes take the pairs (i, j > i) with different labels; each pair's
+// lambda and hessian are the reference's (same sigmoid table, gains, discounts
+// and normalisation) and are added to both documents with LDS float atomics. The
+// float sums therefore run in another order than the reference's pair loop;
+// parity tests compute gradients on the host.
+__device__ __forceinline__ bool rank_before(double sa, int ia, double sb, int ib) {
+  // padding (index < 0) sorts last; otherwise score descending, then index
+  if (ia < 0 || ib < 0) return ib < 0 && ia >= 0;
+  return sa > sb || (sa == sb && ia < ib);
+}
+
+__global__ void __launch_bounds__(256) k_lambdarank(const double* __restrict__ score, const float* __restrict__ label,
+                                                    RankDev r, float* __restrict__ g, float* __restrict__ h) {
+  __shared__ double ss[kRankMaxDocs];
+  __shared__ int si[kRankMaxDocs];
+  __shared__ float lam[kRankMaxDocs], hes[kRankMaxDocs];
+  __shared__ double red[4];
+  const int q = blockIdx.x;
+  const int lo = r.bounds[q], cnt = r.bounds[q + 1] - lo;
+  if (cnt <= 0 || cnt > kRankMaxDocs) return;  // block-uniform; larger queries are refused by the host
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    ss[i] = i < cnt ? score[lo + i] : 0.0;
+    si[i] = i < cnt ? i : -1;
+    if (i < cnt) {
+      lam[i] = 0.0f;
+      hes[i] = 0.0f;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const bool swap = up ? rank_before(ss[l], si[l], ss[i], si[i]) : rank_before(ss[i], si[i], ss[l], si[l]);
+          if (swap) {
+            const double ts = ss[i];
+            ss[i] = ss[l];
+            ss[l] = ts;
+            const int ti = si[i];
+            si[i] = si[l];
+            si[l] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const double inv_max_dcg = r.inv_max_dcg[q];
+  const double best = ss[0];
+  int worst_idx = cnt - 1;
+  if (worst_idx > 0 && ss[worst_idx] == -INFINITY) worst_idx -= 1;
+  const double worst = ss[worst_idx];
+  const bool norm_pairs = r.norm && best != worst;
+  const double eps_norm = (double)0.01f;
+  double sum_lambdas = 0.0;
+  const int top = min(cnt - 1, max(r.truncation, 0));
+  for (int i = 0; i < top; ++i) {
+    if (ss[i] == -INFINITY) continue;  // block-uniform
+    const float li = label[lo + si[i]];
+    for (int j = i + 1 + (int)threadIdx.x; j < cnt; j += blockDim.x) {
+      const float lj = label[lo + si[j]];
+      if (ss[j] == -INFINITY || li == lj) continue;
+      const bool i_high = li > lj;
+      const int high_rank = i_high ? i : j, low_rank = i_high ? j : i;
+      const int high = si[high_rank], low = si[low_rank];
+      const double delta_score = ss[high_rank] - ss[low_rank];
+      const double dcg_gap = r.label_gain[(int)label[lo + high]] - r.label_gain[(int)label[lo + low]];
+      const double paired_discount = fabs(r.discount[high_rank] - r.discount[low_rank]);
+      double dpn = dcg_gap * paired_discount * inv_max_dcg;
+      if (norm_pairs) dpn /= eps_norm + fabs(delta_score);
+      double pl;
+      if (delta_score <= r.min_in) pl = r.sigmoid_table[0];
+      else if (delta_score >= r.max_in) pl = r.sigmoid_table[r.table_bins - 1];
+      else pl = r.sigmoid_table[min((int)((delta_score - r.min_in) * r.table_factor), r.table_bins - 1)];
+      double ph = pl * (1.0 - pl);
+      pl = pl * (-r.sigmoid * dpn);
+      ph = ph * (r.sigmoid * r.sigmoid * dpn);
+      atomicAdd(&lam[low], -(float)pl);
+      atomicAdd(&hes[low], (float)ph);
+      atomicAdd(&lam[high], (float)pl);
+      atomicAdd(&hes[high], (float)ph);
+      sum_lambdas -= 2.0 * pl;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) sum_lambdas += __shfl_xor(sum_lambdas, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum_lambdas;
+  __syncthreads();
+  const double total = red[0] + red[1] + red[2] + red[3];
+  const bool scale = r.norm && total > 0.0;
+  const double nf = scale ? log2(1.0 + total) / total : 1.0;
+  for (int d = threadIdx.x; d < cnt; d += blockDim.x) {
+    g[lo + d] = scale ? (float)((double)lam[d] * nf) : lam[d];
+    h[lo + d] = scale ? (float)((double)hes[d] * nf) : hes[d];
+  }
+}
+
 // Binary log-loss gradients (objective/binary.rs:71-90) for a device-side boosting loop.
 __global__ void k_binary_gradients(const double* __restrict__ score, const float* __restrict__ label, int n,
                                    double sigmoid, float* __restrict__ g, float* __restrict__ h) {
@@ -2353,6 +2457,13 @@ void launch_softmax_gradients(const double* score, const float* label, int n, in
   const double factor = (double)num_class / ((double)num_class - 1.0);
   launch(k_softmax_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score, label, n,
          num_class, factor, g, h);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_lambdarank_gradients(const double* score, const float* label, const RankDev& r, float* g, float* h,
+                                 void* stream) {
+  if (r.num_queries <= 0) return;
+  launch(k_lambdarank, dim3((unsigned)r.num_queries), dim3(256), 0, (hipStream_t)stream, score, label, r, g, h);
   BK_HIP(hipGetLastError());
 }
 

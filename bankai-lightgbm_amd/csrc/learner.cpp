@@ -1912,6 +1912,33 @@ int bk_learner_softmax_gradients_device(bk_learner* l, const double* d_score, co
   });
 }
 
+int bk_learner_lambdarank_gradients_device(bk_learner* l, const double* d_score, const float* d_label,
+                                           const bk_rank_tables* t, float* d_g, float* d_h) {
+  return bk::guard([&] {
+    if (!t || !t->query_bounds || !t->inverse_max_dcg || !t->label_gain || !t->discount || !t->sigmoid_table)
+      throw std::invalid_argument("null argument");
+    if (t->max_query_rows > bk::kRankMaxDocs) throw std::invalid_argument("a query exceeds 2048 rows");
+    if (t->table_bins < 1) throw std::invalid_argument("empty sigmoid table");
+    bk::RankDev r{};
+    r.bounds = t->query_bounds;
+    r.inv_max_dcg = t->inverse_max_dcg;
+    r.label_gain = t->label_gain;
+    r.discount = t->discount;
+    r.sigmoid_table = t->sigmoid_table;
+    r.sigmoid = t->sigmoid;
+    r.min_in = t->min_input;
+    r.max_in = t->max_input;
+    r.table_factor = t->table_factor;
+    r.num_queries = t->num_queries;
+    r.num_gain = t->num_gain;
+    r.table_bins = t->table_bins;
+    r.truncation = t->truncation_level;
+    r.norm = t->norm;
+    bk::Learner& L = l->L;
+    L.timed(6, 0.0, 0.0, [&] { bk::launch_lambdarank_gradients(d_score, d_label, r, d_g, d_h, L.stream); });
+  });
+}
+
 int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
   return bk::guard([&] {
     if (trees_per_iteration < 1 || slot < 0 || slot >= trees_per_iteration)

@@ -196,6 +196,52 @@ def run_multiclass(rows, steps, warmup, lr, runs, seed=0, num_class=10):
     return out
 
 
+def run_ranking(rows, steps, warmup, lr, runs, seed=0):
+    """LambdaRank boosting on the 136-feature ranking workload (68 normal and 68
+    uniform columns with 20% exact zeros, lognormal query sizes clipped to
+    [5, 1000], relevance 0-4): gradients on the device (one block per query), one
+    tree and its score update per iteration. Returns iterations per second for
+    every parameter set in `runs`."""
+    import numpy as np
+    import torch
+
+    import bankai_gpu as bg
+    import boosting as bst
+    import synth
+
+    x, y, q = synth.ranking_136(rows, seed=seed)
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=min(os.cpu_count() or 1, 16)))
+    del x
+    obj = bst.LambdarankObjective(y, q)
+    n = ds.num_data
+    dev = torch.device("cuda:0")
+    tables, keep = bg.rank_tables_on_device(obj, dev)
+    out = []
+    for params in runs:
+        learner = bg.Learner(ds, params, False)
+        score = torch.zeros(n, dtype=torch.float64, device=dev)
+        label = torch.as_tensor(y.astype(np.float32), device=dev)
+        g = torch.empty(n, dtype=torch.float32, device=dev)
+        h = torch.empty(n, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+
+        def iteration():
+            learner.lambdarank_gradients_device(score.data_ptr(), label.data_ptr(), tables, g.data_ptr(), h.data_ptr())
+            t = learner.train_device(g.data_ptr(), h.data_ptr())
+            learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr())
+
+        for _ in range(warmup):
+            iteration()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            iteration()
+        torch.cuda.synchronize()
+        out.append(steps / (time.perf_counter() - t0))
+        learner.close()
+    return out, len(q)
+
+
 def device_copy_gbs(mib=1024, reps=10):
     """Bandwidth of a plain streaming kernel (read mib MiB, write mib MiB: an
     elementwise negate), the practical HBM ceiling; its PMC counts also calibrate
@@ -274,6 +320,9 @@ def main():
     ap.add_argument("--multiclass-rows", type=int, default=2_000_000,
                     help="rows of the 10-class workload measured beside the headline (0 skips it)")
     ap.add_argument("--multiclass-steps", type=int, default=3)
+    ap.add_argument("--ranking-rows", type=int, default=5_000_000,
+                    help="documents of the LambdaRank workload measured beside the headline (0 skips it)")
+    ap.add_argument("--ranking-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -401,6 +450,19 @@ def main():
                         "label = argmax of 10 linear scores + Gumbel noise; softmax gradients on the device"}}
         log(f"multiclass {args.multiclass_rows} x 50: quantized, classes concurrent {qc_its:.2f}, quantized "
             f"{q_its:.2f}, f64 batched {f_its:.2f}, f64 drop-in {fs_its:.2f} iter/s")
+    if args.ranking_rows > 0:
+        rq = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                              stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
+                              form=bg.FORM_QUANTIZED, batched=1)
+        rf = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
+        (rq_its, rf_its), nq = run_ranking(args.ranking_rows, args.ranking_steps, 1, lr, [rq, rf])
+        result.setdefault("other_workloads", {})[f"lambdarank_{args.ranking_rows}x136"] = {
+            "unit": "iter/s", "num_leaves": 255, "max_bin": 255, "queries": nq,
+            "quantized_batched": rq_its, "plain_f64_batched": rf_its,
+            "data": "synthetic: 68 N(0,1) + 68 U(0,1) columns with 20% zeros, lognormal(4,1) query sizes in "
+                    "[5, 1000], relevance 0-4; LambdaRank NDCG gradients on the device"}
+        log(f"lambdarank {args.ranking_rows} x 136 ({nq} queries): quantized {rq_its:.2f}, f64 batched "
+            f"{rf_its:.2f} iter/s")
     if not args.no_cpu_baseline:
         cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                    stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)

@@ -201,6 +201,26 @@ int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_scor
 int bk_learner_softmax_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
                                         int num_class, float* d_gradients, float* d_hessians);
 
+/* LambdaRank tables (crates/lgbm-core/src/objective/rank.rs:183-240), all in device
+ * memory: query row offsets, 1 / max DCG per query, label gains, position discounts
+ * and the sigmoid table with its input range. */
+typedef struct bk_rank_tables {
+  const int32_t* query_bounds;     /* num_queries + 1 */
+  const double* inverse_max_dcg;   /* num_queries */
+  const double* label_gain;        /* num_gain */
+  const double* discount;          /* per rank, at least the largest query's rows */
+  const double* sigmoid_table;     /* table_bins */
+  double sigmoid, min_input, max_input, table_factor;
+  int32_t num_queries, num_gain, table_bins, truncation_level, norm, max_query_rows;
+} bk_rank_tables;
+
+/* LambdaRank NDCG gradients and hessians (rank.rs:262-340, unweighted, no position
+ * bias) of every query from a device score array, on the learner's stream. The pair
+ * terms are the reference's; their float sums run in another order. Queries may hold
+ * at most 2048 rows. A device-side ranking loop uses it; no FFI counterpart. */
+int bk_learner_lambdarank_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
+                                           const bk_rank_tables* tables, float* d_gradients, float* d_hessians);
+
 /* Trains tree `slot` of every `trees_per_iteration` trees of a sequence shared by
  * several learners, e.g. one learner per class of a multiclass iteration, each
  * on its own stream and host thread, so the class trees grow concurrently. The

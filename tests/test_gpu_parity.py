@@ -349,3 +349,35 @@ def test_concurrent_class_learners_match_one_learner(batched):
                 view = score[k * n:(k + 1) * n].copy()
                 many[k].add_prediction_to_score(seq[k]["leaf_value"] * 0.1, view)
                 score[k * n:(k + 1) * n] = view
+
+
+def test_lambdarank_gradients_device_match_host():
+    """Device LambdaRank gradients (one block per query) against rank.rs restated on
+    the host, on 20,000 documents in queries of 5-1000 rows: the pair terms are
+    the same, their f32 sums run in another order, so |device - host| must stay
+    within 1e-5 of the largest pair-sum magnitude of the query plus 4 f32 ulps."""
+    announce("lambdarank gradients on the device, 20k documents")
+    import torch
+    x, y, q = synth.ranking_136(20_000, seed=8)
+    ds = bg.Dataset(x[:, :8].copy())
+    obj = bst.LambdarankObjective(y, q)
+    rng = np.random.default_rng(9)
+    score = rng.normal(0.0, 1.0, size=len(y))
+    want_g, want_h = obj.gradients(score)
+    learner = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
+    dev = torch.device("cuda:0")
+    tables, keep = bg.rank_tables_on_device(obj, dev)
+    ts = torch.as_tensor(score, device=dev)
+    tl = torch.as_tensor(y.astype(np.float32), device=dev)
+    tg = torch.zeros(len(y), dtype=torch.float32, device=dev)
+    th = torch.zeros(len(y), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    learner.lambdarank_gradients_device(ts.data_ptr(), tl.data_ptr(), tables, tg.data_ptr(), th.data_ptr())
+    torch.cuda.synchronize()
+    learner.close()
+    g, h = tg.cpu().numpy(), th.cpu().numpy()
+    for lo, hi in zip(obj.bounds[:-1], obj.bounds[1:]):
+        sg = np.max(np.abs(want_g[lo:hi])) + 1e-30
+        sh = np.max(np.abs(want_h[lo:hi])) + 1e-30
+        assert np.all(np.abs(g[lo:hi] - want_g[lo:hi]) <= 1e-5 * sg + 4 * np.spacing(np.abs(want_g[lo:hi])))
+        assert np.all(np.abs(h[lo:hi] - want_h[lo:hi]) <= 1e-5 * sh + 4 * np.spacing(np.abs(want_h[lo:hi])))
