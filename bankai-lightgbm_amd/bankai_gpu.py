@@ -124,7 +124,8 @@ class TrainStatsC(C.Structure):
     _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
                 ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32),
                 ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
-                ("host_ms", C.c_double), ("wait_ms", C.c_double)]
+                ("host_ms", C.c_double), ("wait_ms", C.c_double), ("hist_min_rows", C.c_int32),
+                ("hist_max_blocks", C.c_int32)]
 
 
 class KernelStatsC(C.Structure):
@@ -466,7 +467,8 @@ class Learner:
         return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
-                "speculative_waste": s.speculative_waste}
+                "speculative_waste": s.speculative_waste,
+                "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks}}
 
     def profile(self, kinds=True, every_trees=1):
         """Per-kernel HIP-event timing: True for every kind, False for none, or a

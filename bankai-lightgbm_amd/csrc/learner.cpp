@@ -20,6 +20,9 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -733,7 +736,8 @@ struct Learner {
     max_items = P.num_leaves + 1;
     part_tile = partition_tile_rows(stride);
     max_part_tiles = N / part_tile + max_items + 8;
-    max_partials = kHistMaxBlocks + 2 * max_items + 8;
+    // a build has at most one row block per 2048 rows (the tuner's smallest row count)
+    max_partials = std::min(kHistMaxBlocks, N / 2048 + 1) + 2 * max_items + 8;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // marks the end of staging transfers, which only read pinned host memory
@@ -896,6 +900,66 @@ struct Learner {
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
     sync();
+    autotune_hist();
+  }
+
+  // Autotuner of the histogram tiling (rows per workgroup at least hist_min_rows,
+  // at most hist_max_blocks row blocks): times the histogram kernel on this
+  // dataset's records for every candidate, over all rows (a root build) and over
+  // a fifth of them (a child-sized build), and keeps the candidate with the least
+  // time for one root and three child builds. The payload is whatever buffer 0
+  // holds: the kernel's work does not depend on its values. The choice is kept
+  // per (rows, groups, bins, form) for the process.
+  void autotune_hist() {
+    static std::mutex mu;
+    static std::map<std::array<int, 4>, std::pair<int, int>> picked;
+    const std::array<int, 4> key{N, G, TB, form};
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = picked.find(key);
+    if (it != picked.end()) {
+      hist_min_rows = it->second.first;
+      hist_max_blocks = it->second.second;
+      return;
+    }
+    hipEvent_t e0, e1;
+    BK_HIPC(hipEventCreate(&e0));
+    BK_HIPC(hipEventCreate(&e1));
+    double best_t = 1e300;
+    std::pair<int, int> best_c{hist_min_rows, hist_max_blocks};
+    for (int min_rows : {2048, 4096, 8192})
+      for (int max_blocks : {512, 1024, 2048}) {
+        hist_min_rows = min_rows;
+        hist_max_blocks = max_blocks;
+        double t = 0.0;
+        for (int part = 0; part < 2; ++part) {
+          const int rows = part == 0 ? N : std::max(N / 5, 1);
+          hist_seg.assign(1, Segment{0, 0, rows});
+          const HistPlan hp = plan_hist(1);
+          begin_uploads();
+          d_hitems = put(h_hitems, (size_t)hp.ni);
+          d_hblock = put(h_hblock, (size_t)hp.nblocks);
+          send(true);
+          float ms = 0.0f;
+          for (int rep = 0; rep < 2; ++rep) {  // the first run warms up, the second is timed
+            BK_HIPC(hipEventRecord(e0, stream));
+            launch_hist(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, d_group_used,
+                        max_tile_bins, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+            BK_HIPC(hipEventRecord(e1, stream));
+            sync();
+            BK_HIPC(hipEventElapsedTime(&ms, e0, e1));
+          }
+          t += (part == 0 ? 1.0 : 3.0) * ms;
+        }
+        if (t < best_t) {
+          best_t = t;
+          best_c = {min_rows, max_blocks};
+        }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hist_min_rows = best_c.first;
+    hist_max_blocks = best_c.second;
+    picked[key] = best_c;
   }
 
   ScanParams scan_params() const {
@@ -1118,11 +1182,11 @@ struct Learner {
   }
 
   // Histograms of the segments hist_seg[0..n): their rows are concatenated and
-  // cut into equal row ranges, one per block (at least kHistMinRows rows, at most
-  // kHistMaxBlocks blocks); a block writes one partial per segment it touches.
+  // cut into equal row ranges, one per block (at least hist_min_rows rows, at most
+  // hist_max_blocks blocks); a block writes one partial per segment it touches.
   // Segment i's partials are hist_first[i] .. hist_first[i] + hist_count[i] - 1.
-  static constexpr int kHistMaxBlocks = 1024;
-  static constexpr int kHistMinRows = 4096;
+  static constexpr int kHistMaxBlocks = 2048;  // largest row-block count the tuner may pick
+  int hist_max_blocks = 1024, hist_min_rows = 4096;  // picked by autotune_hist
   HistPlan plan_hist(int n) {
     HistPlan hp;
     hist_first.assign((size_t)n, 0);
@@ -1136,7 +1200,7 @@ struct Learner {
       ++hp.ni;
     }
     if (hp.ni == 0) return hp;
-    int nblocks = (int)std::min<long long>(kHistMaxBlocks, std::max<long long>(1, hp.total / kHistMinRows));
+    int nblocks = (int)std::min<long long>(hist_max_blocks, std::max<long long>(1, hp.total / hist_min_rows));
     hp.rpb = (int)((hp.total + nblocks - 1) / nblocks);
     hp.nblocks = (int)((hp.total + hp.rpb - 1) / hp.rpb);
     // slots: block b gets one per item it touches
@@ -1949,7 +2013,11 @@ int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
 }
 
 int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
-  return bk::guard([&] { *out = l->L.stats; });
+  return bk::guard([&] {
+    *out = l->L.stats;
+    out->hist_min_rows = l->L.hist_min_rows;
+    out->hist_max_blocks = l->L.hist_max_blocks;
+  });
 }
 
 int bk_learner_profile(bk_learner* l, int kinds) {

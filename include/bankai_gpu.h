@@ -249,6 +249,8 @@ typedef struct bk_train_stats {
   int64_t committed_part_rows;
   double host_ms;  /* wall time of the train call not spent blocked on the device */
   double wait_ms;  /* wall time blocked on the device (synchronizations)           */
+  int32_t hist_min_rows;    /* histogram tiling picked by the autotuner at creation: */
+  int32_t hist_max_blocks;  /* rows per workgroup at least / row blocks at most     */
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
