@@ -171,8 +171,9 @@ struct ScanParams {
 // Rows are stored as records of `stride` bytes (a multiple of 16): the bin of
 // group g at byte g, and in buffers 1 and 2 the row payload at byte pay_off.
 // Buffer 0 holds the rows in their original order; its payload lives in gh0
-// (rewritten every tree). Buffers 1 and 2 are the partition ping-pong buffers,
-// with the original row id of every position in idx[1] / idx[2].
+// (rewritten every tree). Buffers 1 and 2 are the partition ping-pong buffers;
+// records carry no row id (the row-to-leaf map comes from a tree walk over
+// buffer 0).
 struct DevState {
   int num_data, npad, num_groups, num_features, total_bins, form;
   int stride, pay_off;

@@ -1,13 +1,15 @@
 // kernels.hip — gfx950 kernels of the histogram tree learner.
 //
 // Data layout in HBM (one learner):
-//   cols[0]        original column-major u8 bins, group g at g*npad (read only)
-//   cols[1..2]     ping-pong copies in partition order: every leaf's rows are a
-//                  contiguous segment of one buffer, so histogram and partition
-//                  passes stream whole cache lines (no per-row gathers)
-//   idx[1..2]      original row id of each partition position (buffer 0: identity)
-//   gh[0..2]       row payload in the same order: int16 packed int8 pair
-//                  (quantized) or float2 (gradient, hessian)
+//   rec[0]         row records in original order: group g's bin at byte g of a
+//                  `stride`-byte record (read only after creation); the payload
+//                  of these rows lives in gh0 (int16 packed int8 pair when
+//                  quantized, float2 (gradient, hessian) otherwise)
+//   rec[1..2]      ping-pong copies in partition order, payload inside the record:
+//                  every leaf's rows are a contiguous segment of one buffer, so
+//                  histogram and partition passes stream whole records (no
+//                  per-row gathers)
+//   leaf_of        leaf of every row of the last tree (tree walk over rec[0])
 //   pool           one histogram per leaf slot: (g, h) per bin, plus an exact
 //                  per-bin row count (pool_cnt) used to size partitions in advance
 //
@@ -473,10 +475,10 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // consecutive chunks), the lanes holding the split group's byte route their
 // rows, a block scan ranks the rows, and the tile's left count is published
 // for the tiles after it while wave 0 looks back over the tiles before it
-// (decoupled look-back) for its left offset. Then the chunks and row ids are
-// written to their destinations: two contiguous runs per tile. Rows carry no
-// id: the leaf of every row is found afterwards by walking the tree over the
-// records in their original order.
+// (decoupled look-back) for its left offset. Then the chunks are written to
+// their destinations: two contiguous runs per tile. Rows carry no id: the leaf
+// of every row is found afterwards by walking the tree over the records in
+// their original order.
 // status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;

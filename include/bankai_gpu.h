@@ -259,8 +259,9 @@ int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
  * 2 partition, 3 split search, 4 discretizer, 5 leaf renewal, 6 score update
  * and gradients. algo_bytes follows the per-row byte model of the histogram
  * (rows x (features + payload), plus a 4-byte row index for a child) and of the
- * partition (rows x 9: one bin, row id read, row id write); moved_bytes counts
- * what the kernel actually reads and writes. No reference counterpart
+ * partition (rows x 9: one bin, row id read, row id write), the reference's
+ * index-based model; moved_bytes counts what the kernel actually reads and
+ * writes (whole row records). No reference counterpart
  * (measurement only). */
 typedef struct bk_kernel_stats {
   double ms;
