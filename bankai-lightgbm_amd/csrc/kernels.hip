@@ -1647,17 +1647,20 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
   o.feature = m.real;
   const long long tg = a.int_sum >> 32, th = a.int_sum & 0xffffffffLL;
   if (m.bin_type == 1) {
-    // the ctr sort uses the whole block, one leaf at a time; the scan runs on lane 0
+    // the ctr sort uses the whole block, one leaf at a time; the scan runs on
+    // lane 0 and leaves its result in LDS (a result in registers handed to the
+    // call would live in per-lane scratch)
+    __shared__ SplitOut cat_out[2];
     cat_sort<FORM>(m, p, small, S[0], elig[0]);
     if (has_large) cat_sort<FORM>(m, p, large, S[1], elig[1]);
+    if (threadIdx.x < 2) cat_out[threadIdx.x] = o;
     __syncthreads();
     if (enabled && lane == 0)
       search_categorical<FORM == FORM_QUANT>(S[wl].g, S[wl].h, S[wl].ig, S[wl].ih, m, p, a.sum_g, a.sum_h, tg, th,
-                                             a.num_data, a.parent_output, S[wl].sorted, S[wl].used_bin, o);
-    // lane 0's result to the whole wave
-    o.gain = __shfl(o.gain, 0, 64);
-    o.num_cat = __shfl(o.num_cat, 0, 64);
-    for (int i = 0; i < 8; ++i) o.cat_bits[i] = (uint32_t)__shfl((int)o.cat_bits[i], 0, 64);
+                                             a.num_data, a.parent_output, S[wl].sorted, S[wl].used_bin,
+                                             cat_out[wl]);
+    __syncthreads();
+    o = cat_out[wl];
   } else {
     search_numerical_wave<FORM>(m, p, a, S[wl], X[wl], enabled, o);
   }
