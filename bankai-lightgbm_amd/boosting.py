@@ -367,13 +367,180 @@ class LambdarankObjective:
 # ---------------------------------------------------------------------------
 # The boosting loop
 # ---------------------------------------------------------------------------
+RAND_BLOCK = 1024  # rows per random generator in the row samplers (sample.rs:15-16)
+
+
+class Random:
+    """The LCG of every LightGBM sampler (utils/random.rs:6-80): x = x*214013 + 2531011
+    mod 2^32; next_float takes 15 bits, (x >> 16) & 0x7FFF, over 32768 in f32."""
+
+    def __init__(self, seed):
+        self.x = int(seed) & 0xFFFFFFFF
+
+    def next_float(self):
+        self.x = (self.x * 214013 + 2531011) & 0xFFFFFFFF
+        return float(np.float32((self.x >> 16) & 0x7FFF) / np.float32(32768.0))
+
+
+_SEED_DEFAULTS = {"data_random_seed": 1, "bagging_seed": 3, "drop_seed": 4, "feature_fraction_seed": 2,
+                  "objective_seed": 5, "extra_seed": 6}
+
+
+def resolve_seed(params, name):
+    """A sampler seed as Config::set resolves it (config/mod.rs:396-406): an explicit value
+    wins; otherwise, when `seed` is given, the seeds are drawn from Random(seed) with
+    next_short(0, 32767) in the order data_random_seed, bagging_seed, drop_seed,
+    feature_fraction_seed, objective_seed, extra_seed; otherwise the default."""
+    if name in params:
+        return int(params[name])
+    if "seed" not in params:
+        return _SEED_DEFAULTS[name]
+    x = int(params["seed"]) & 0xFFFFFFFF
+    for key in _SEED_DEFAULTS:
+        x = (x * 214013 + 2531011) & 0xFFFFFFFF
+        if key == name:
+            return ((x >> 16) & 0x7FFF) % 32767
+    raise KeyError(name)
+
+
+class BlockDraws:
+    """Generators of one row sampler, one per block of 1024 rows seeded seed + block
+    (sample.rs:56-60 block_rands), advanced together: draw() gives one next_float per
+    row in row order, row i from block i // 1024's generator, and the generators keep
+    their state from one call to the next."""
+
+    def __init__(self, seed, n):
+        nb = -(-n // RAND_BLOCK)
+        self.n = n
+        self.x = (np.int64(seed) + np.arange(nb, dtype=np.int64)).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+        self.count = np.minimum(RAND_BLOCK, n - np.arange(nb) * RAND_BLOCK)
+
+    def draw(self):
+        out = np.zeros((len(self.x), RAND_BLOCK), np.float32)
+        for j in range(RAND_BLOCK):
+            live = self.count > j
+            if not live.any():
+                break
+            x = (self.x[live] * np.uint64(214013) + np.uint64(2531011)) & np.uint64(0xFFFFFFFF)
+            self.x[live] = x
+            out[live, j] = ((x >> np.uint64(16)) & np.uint64(0x7FFF)).astype(np.float32) / np.float32(32768.0)
+        return out.reshape(-1)[: self.n]
+
+
+class RowSampler:
+    """SampleStrategy (sample.rs:47-242): all rows, bagging (plain or balanced by the
+    binary label) or GOSS. sample() returns the ascending rows a tree is trained on, or
+    None for all rows, and may rescale the gradients in place (GOSS)."""
+
+    def __init__(self, params, label, num_tree_per_iteration=1):
+        p = params
+        self.n = len(label)
+        self.kind = "full"
+        strategy = p.get("data_sample_strategy", "bagging")
+        seed = resolve_seed(p, "bagging_seed")
+        freq = p.get("bagging_freq", 0)
+        fraction = p.get("bagging_fraction", 1.0)
+        if strategy == "goss":
+            if freq > 0 and fraction != 1.0:
+                raise ValueError("Cannot use bagging in GOSS")
+            self.top_rate = p.get("top_rate", 0.2)
+            self.other_rate = p.get("other_rate", 0.1)
+            if self.top_rate + self.other_rate > 1.0 or self.top_rate <= 0.0 or self.other_rate <= 0.0:
+                raise ValueError("top_rate + other_rate must be <= 1, both positive")
+            self.kind = "goss"
+            self.warmup = int(1.0 / p.get("learning_rate", 0.1))
+            self.rands = [Random(seed + b) for b in range(-(-self.n // RAND_BLOCK))]
+            self.ntpi = num_tree_per_iteration
+            return
+        if strategy != "bagging":
+            raise ValueError(f"data_sample_strategy={strategy}")
+        if p.get("bagging_by_query", False):
+            raise ValueError("bagging_by_query is not supported")
+        positive = np.asarray(label) > 0
+        num_pos = int(positive.sum()) if p.get("objective") == "binary" else 0
+        pos_f, neg_f = p.get("pos_bagging_fraction", 1.0), p.get("neg_bagging_fraction", 1.0)
+        balanced = (pos_f < 1.0 or neg_f < 1.0) and num_pos > 0
+        if not ((fraction < 1.0 or balanced) and freq > 0):
+            return
+        self.kind = "bagging"
+        self.freq = freq
+        self.threshold = np.where(positive, pos_f, neg_f) if balanced else np.full(self.n, fraction)
+        self.draws = BlockDraws(seed, self.n)
+        self.bag = np.zeros(0, np.int32)
+        self.need_rebag = True
+
+    @property
+    def changes_hessians(self):
+        return self.kind == "goss"
+
+    def _bagging_draw(self):
+        # BaggingHelper / BalancedBaggingHelper: one draw per row, kept when below the
+        # row's fraction (sample.rs:203-234), compared in f64
+        r = self.draws.draw().astype(np.float64)
+        return np.flatnonzero(r < self.threshold).astype(np.int32)
+
+    def _goss_draw(self, g, h):
+        # GOSSStrategy::Helper over all rows (sample.rs:244-296)
+        n, ntpi = self.n, self.ntpi
+        if n == 0:
+            return np.zeros(0, np.int32)
+        gk = g.reshape(ntpi, n)
+        hk = h.reshape(ntpi, n)
+        tmp = np.zeros(n, np.float32)
+        for k in range(ntpi):
+            tmp = (tmp + np.abs(gk[k] * hk[k])).astype(np.float32)
+        top_k = max(int(n * self.top_rate), 1)
+        other_k = int(n * self.other_rate)
+        threshold = np.partition(tmp, n - top_k)[n - top_k]  # the top_k-th largest
+        multiply = np.float32(n - top_k) / np.float32(other_k)
+        bag = []
+        big = 0
+        for i in range(n):
+            if tmp[i] >= threshold:
+                bag.append(i)
+                big += 1
+            else:
+                rest_need = other_k - (len(bag) - big)
+                rest_all = (n - i) - (top_k - big)
+                prob = rest_need / rest_all
+                if self.rands[i // RAND_BLOCK].next_float() < prob:
+                    bag.append(i)
+                    for k in range(ntpi):
+                        gk[k, i] = np.float32(gk[k, i] * multiply)
+                        hk[k, i] = np.float32(hk[k, i] * multiply)
+        return np.asarray(bag, np.int32)
+
+    def sample(self, it, g, h):
+        if self.kind == "full":
+            return None
+        if self.kind == "goss":
+            if it < self.warmup:
+                return None
+            bag = self._goss_draw(g, h)
+            return bag if len(bag) < self.n else None
+        if (len(self.bag) < self.n and it % self.freq == 0) or self.need_rebag:
+            self.need_rebag = False
+            self.bag = self._bagging_draw()
+        return self.bag if len(self.bag) < self.n else None
+
+
+def out_of_bag(rows, n):
+    """Rows not in the bag, ascending (train.rs:1282-1291)."""
+    if rows is None:
+        return None
+    mask = np.ones(n, bool)
+    mask[rows] = False
+    return np.flatnonzero(mask).astype(np.int32)
+
+
 class GBDT:
     """Trainer::train_one_iter_with (train.rs:723-839) for plain GBDT."""
 
-    def __init__(self, learners, objective, num_data, learning_rate=0.1, boost_from_average=True):
+    def __init__(self, learners, objective, num_data, learning_rate=0.1, boost_from_average=True, sampler=None):
         # one learner per class tree of an iteration may share state: the reference
         # uses a single learner; `learners` is a list of length 1 or num_class
         self.learners = learners
+        self.sampler = sampler
         self.obj = objective
         self.n = num_data
         self.lr = learning_rate
@@ -402,12 +569,16 @@ class GBDT:
                 self.score[k * n:(k + 1) * n] += init
             inits.append(init)
         g, h = self.obj.gradients(self.score)
+        # the gradients come first and the row sample after (train.rs:746-769)
+        rows = self.sampler.sample(self.iter, g, h) if self.sampler is not None else None
+        oob = out_of_bag(rows, n)
         should_continue = False
         for k in range(self.ntpi):
             off = k * n
             need = getattr(self.obj, "class_need_train", lambda _k: True)(k)
             if need:
-                d = self._learner(k).train(g[off:off + n], h[off:off + n])
+                d = (self._learner(k).train(g[off:off + n], h[off:off + n]) if rows is None else
+                     self._learner(k).train(g[off:off + n], h[off:off + n], rows))
                 tree = Tree(d)
             else:
                 tree = Tree.constant(0.0, n)
@@ -416,6 +587,8 @@ class GBDT:
                 tree.apply_shrinkage(self.lr)
                 view = self.score[off:off + n]
                 self._learner(k).add_prediction_to_score(tree.leaf_value, view)
+                if oob is not None:
+                    self._learner(k).add_out_of_bag_prediction(tree.leaf_value, oob, view)
                 self.score[off:off + n] = view
                 if abs(inits[k]) > K_EPSILON:
                     tree.add_bias(inits[k])

@@ -1207,6 +1207,14 @@ struct Learner {
   double grad_scale = 0.0, hess_scale = 0.0;
   // per-tree stats
   int64_t stat_hist_rows = 0, stat_part_rows = 0;
+  // The last tree's splits in the order they were made (serial.rs:2003 last_splits):
+  // the split leaf, its routing and the new right leaf, replayed on out-of-bag rows.
+  struct SplitRecord {
+    int leaf;
+    SplitRule rule;
+    int right;
+  };
+  std::vector<SplitRecord> splits;
 
   ~Learner() { delete disc; }
 
@@ -1495,6 +1503,7 @@ struct Learner {
       std::vector<uint32_t> bitset_bin = construct_bitset(bins);
       SplitRule rule = rule_categorical(gr, ft, bitset_bin);
       partition_split((size_t)best_leaf, rule, (size_t)next_leaf);
+      splits.push_back(SplitRecord{best_leaf, rule, next_leaf});
       info.left_count = (int)leaf_cnt[(size_t)best_leaf];
       info.right_count = (int)leaf_cnt[(size_t)next_leaf];
       int node = t.split_common(best_leaf, inner, info.feature, info.left_output, info.right_output, info.left_count,
@@ -1511,6 +1520,7 @@ struct Learner {
       double thd = ft.upper[info.threshold];
       SplitRule rule = rule_numerical(gr, ft, info.threshold, info.default_left);
       partition_split((size_t)best_leaf, rule, (size_t)next_leaf);
+      splits.push_back(SplitRecord{best_leaf, rule, next_leaf});
       info.left_count = (int)leaf_cnt[(size_t)best_leaf];
       info.right_count = (int)leaf_cnt[(size_t)next_leaf];
       int node = t.split_common(best_leaf, inner, info.feature, info.left_output, info.right_output, info.left_count,
@@ -1539,15 +1549,24 @@ struct Learner {
     return {best_leaf, right};
   }
 
-  // before_train (serial.rs:658-776) without bagging.
-  void before_train(const float* g, const float* h) {
+  // before_train (serial.rs:658-776). rows: the bagged subset, ascending, or null for
+  // all rows (DataPartition::init_with / init, data_partition.rs:91-107). The
+  // discretizer always sees every row; the root sums run over the bag in order.
+  void before_train(const float* g, const float* h, const int32_t* rows, size_t nrows) {
     pool.reset_map();
     leaf_begin.assign((size_t)num_leaves_cfg, 0);
     leaf_cnt.assign((size_t)num_leaves_cfg, 0);
-    leaf_cnt[0] = num_data;
-    for (size_t i = 0; i < num_data; ++i) indices[i] = (int32_t)i;
+    if (rows) {
+      leaf_cnt[0] = nrows;
+      std::copy(rows, rows + nrows, indices.begin());
+    } else {
+      leaf_cnt[0] = num_data;
+      for (size_t i = 0; i < num_data; ++i) indices[i] = (int32_t)i;
+    }
     for (auto& s : best) s.reset();
+    splits.clear();
     stat_hist_rows = stat_part_rows = 0;
+    const size_t cnt = leaf_cnt[0];
     double sg = 0.0, sh = 0.0;
     int64_t isum = 0;
     if (quant) {
@@ -1557,7 +1576,8 @@ struct Learner {
         hess_scale = disc->hessian_scale;
       }
       const int16_t* pk = disc->packed.data();
-      for (size_t i = 0; i < num_data; ++i) {
+      for (size_t j = 0; j < cnt; ++j) {
+        size_t i = rows ? (size_t)rows[j] : j;
         int16_t p = pk[i];
         int8_t gg = (int8_t)(p >> 8);
         uint8_t hh = (uint8_t)(p & 0xff);
@@ -1566,19 +1586,20 @@ struct Learner {
         isum += (int64_t)(((uint64_t)(int64_t)gg << 32) | (uint64_t)hh);
       }
     } else {
-      for (size_t i = 0; i < num_data; ++i) {
+      for (size_t j = 0; j < cnt; ++j) {
+        size_t i = rows ? (size_t)rows[j] : j;
         sg += (double)g[i];
         sh += (double)h[i];
       }
     }
     double carried = smaller.weight;
-    smaller = LeafSplits{0, (int)num_data, sg, sh, carried, isum};
+    smaller = LeafSplits{0, (int)cnt, sg, sh, carried, isum};
     larger = LeafSplits{};
   }
 
   // grow_tree (serial.rs:578-628) + renew_int_grad_tree_output (634-655)
-  Tree train(const float* g, const float* h) {
-    before_train(g, h);
+  Tree train(const float* g, const float* h, const int32_t* rows, size_t nrows) {
+    before_train(g, h, rows, nrows);
     Tree t(num_leaves_cfg);
     t.set_leaf_output(0, leaf_output_root(sp, smaller.sum_g, smaller.sum_h, smaller.num_data));
     int left_leaf = 0, right_leaf = -1;
@@ -1739,9 +1760,16 @@ int bko_learner_create(const bk_dataset_view* ds, const bk_learner_params* p, in
   }
 }
 
-int bko_learner_train(bko_learner* h, const float* g, const float* hs, bk_tree_out* out) {
+int bko_learner_train(bko_learner* h, const float* g, const float* hs, const int32_t* rows, int64_t nrows,
+                      bk_tree_out* out) {
   try {
-    Tree t = h->L.train(g, hs);
+    if (rows) {
+      if (nrows < 0 || (size_t)nrows > h->L.num_data) throw std::invalid_argument("bag larger than the dataset");
+      for (int64_t i = 0; i < nrows; ++i)
+        if (rows[i] < 0 || (size_t)rows[i] >= h->L.num_data || (i > 0 && rows[i] <= rows[i - 1]))
+          throw std::invalid_argument("bagged rows must be ascending and inside the dataset");
+    }
+    Tree t = h->L.train(g, hs, rows, rows ? (size_t)nrows : 0);
     export_tree(t, out);
     return 0;
   } catch (const std::exception& e) {
@@ -1757,6 +1785,33 @@ int bko_learner_add_prediction_to_score(bko_learner* h, const bk_tree_out* t, do
     const int32_t* ix = L.idx_of((size_t)leaf);
     for (size_t i = 0; i < L.leaf_cnt[(size_t)leaf]; ++i) score[ix[i]] += t->leaf_value[leaf];
   }
+  return 0;
+}
+
+// add_out_of_bag_prediction (serial.rs:1995) through add_prediction_for_rows
+// (serial.rs:2009): the last tree's splits replayed, in order, on a fresh partition of
+// `rows` with the training routing; each row then gets its leaf's value.
+int bko_learner_add_out_of_bag_prediction(bko_learner* h, const bk_tree_out* t, const int32_t* rows, int64_t n,
+                                          double* score) {
+  Learner& L = h->L;
+  if (t->num_leaves <= 1 || n <= 0) return 0;
+  std::vector<int32_t> ix(rows, rows + n), lb, rb;
+  std::vector<size_t> begin((size_t)t->num_leaves, 0), cnt((size_t)t->num_leaves, 0);
+  cnt[0] = (size_t)n;
+  for (const auto& s : L.splits) {
+    size_t b = begin[(size_t)s.leaf], c = cnt[(size_t)s.leaf];
+    lb.clear();
+    rb.clear();
+    for (size_t i = b; i < b + c; ++i) (s.rule.route_left(s.rule.col[ix[i]]) ? lb : rb).push_back(ix[i]);
+    std::copy(lb.begin(), lb.end(), ix.begin() + (long)b);
+    std::copy(rb.begin(), rb.end(), ix.begin() + (long)(b + lb.size()));
+    cnt[(size_t)s.leaf] = lb.size();
+    begin[(size_t)s.right] = b + lb.size();
+    cnt[(size_t)s.right] = rb.size();
+  }
+  for (int leaf = 0; leaf < t->num_leaves; ++leaf)
+    for (size_t i = begin[(size_t)leaf]; i < begin[(size_t)leaf] + cnt[(size_t)leaf]; ++i)
+      score[ix[i]] += t->leaf_value[leaf];
   return 0;
 }
 

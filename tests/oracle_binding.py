@@ -29,8 +29,9 @@ def load():
         "bko_last_error": ([], C.c_char_p),
         "bko_learner_create": ([C.POINTER(bg.DatasetView), C.POINTER(bg.LearnerParamsC), C.c_int, C.POINTER(P)],
                                C.c_int),
-        "bko_learner_train": ([P, P, P, C.POINTER(bg.TreeOutC)], C.c_int),
+        "bko_learner_train": ([P, P, P, P, C.c_int64, C.POINTER(bg.TreeOutC)], C.c_int),
         "bko_learner_add_prediction_to_score": ([P, C.POINTER(bg.TreeOutC), P], C.c_int),
+        "bko_learner_add_out_of_bag_prediction": ([P, C.POINTER(bg.TreeOutC), P, C.c_int64, P], C.c_int),
         "bko_learner_leaf_rows": ([P, C.c_int, P, C.POINTER(C.c_int64)], C.c_int),
         "bko_learner_stats": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "bko_learner_quant_state": ([P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
@@ -74,10 +75,12 @@ class OracleLearner:
         self.handle = out
         self.tree = bg.TreeBuffers(self.params.num_leaves)
 
-    def train(self, gradients, hessians):
+    def train(self, gradients, hessians, rows=None):
         g = np.ascontiguousarray(gradients, dtype=np.float32)
         h = np.ascontiguousarray(hessians, dtype=np.float32)
-        rc = self._lib.bko_learner_train(self.handle, _p(g), _p(h), C.byref(self.tree.c))
+        r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        rc = self._lib.bko_learner_train(self.handle, _p(g), _p(h), _p(r), 0 if r is None else len(r),
+                                         C.byref(self.tree.c))
         if rc != 0:
             raise RuntimeError(self._lib.bko_last_error().decode())
         return self.tree.as_dict()
@@ -85,6 +88,11 @@ class OracleLearner:
     def add_prediction_to_score(self, leaf_values, score):
         self.tree.set_leaf_values(leaf_values)
         self._lib.bko_learner_add_prediction_to_score(self.handle, C.byref(self.tree.c), _p(score))
+
+    def add_out_of_bag_prediction(self, leaf_values, rows, score):
+        self.tree.set_leaf_values(leaf_values)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        self._lib.bko_learner_add_out_of_bag_prediction(self.handle, C.byref(self.tree.c), _p(r), len(r), _p(score))
 
     def leaf_rows(self, leaf):
         out = np.zeros(self.dataset.num_data, np.int32)

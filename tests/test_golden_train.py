@@ -2,7 +2,8 @@
 
 Each case under tests/golden/train/ is a copy of the reference's fixture
 tests/fixtures/train/<case>/ (x.npy, y.npy, params.json, model.txt, plus w.npy for
-weights and group.npy for query sizes; produced by
+weights and group.npy for query sizes; the bagging_* and goss* cases train each tree
+on a row sample drawn by boosting.RowSampler; produced by
 scripts/gen_train_fixtures.py with lightgbm==4.7.0, 1 thread, force_col_wise, seed 7).
 The dataset is binned by this repository's host binning (bankai_gpu.Dataset), the
 trees grown by the CPU oracle through the boosting loop of boosting.py, and every
@@ -80,9 +81,11 @@ def train_case(case, learner_factory):
     ds = bg.Dataset(x, dataset_params(p), categorical=spec["categorical"])
     objective = make_objective(p, y, w, spec.get("group"))
     lp = learner_params(p)
-    learners = [learner_factory(ds, lp, objective.const_hess)]
+    sampler = bst.RowSampler(p, y, objective.num_class)
+    # train.rs:533: a sampler that rescales hessians (GOSS) turns the constant hessian off
+    learners = [learner_factory(ds, lp, objective.const_hess and not sampler.changes_hessians)]
     gbdt = bst.GBDT(learners, objective, ds.num_data, learning_rate=p.get("learning_rate", 0.1),
-                    boost_from_average=p.get("boost_from_average", True))
+                    boost_from_average=p.get("boost_from_average", True), sampler=sampler)
     gbdt.train(spec["num_boost_round"])
     return ds, gbdt, model, objective
 
