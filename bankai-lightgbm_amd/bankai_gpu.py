@@ -150,6 +150,11 @@ EXPORTS = {
     "bk_learner_create": ([C.POINTER(DatasetView), C.POINTER(LearnerParamsC), C.c_int, C.POINTER(_P)], C.c_int),
     "bk_learner_train": ([_P, _P, _P, _P, C.c_int64, C.POINTER(TreeOutC)], C.c_int),
     "bk_learner_train_device": ([_P, _P, _P, C.POINTER(TreeOutC)], C.c_int),
+    "bk_learner_train_device_bagged": ([_P, _P, _P, _P, C.c_int64, C.POINTER(TreeOutC)], C.c_int),
+    "bk_learner_add_out_of_bag_prediction": ([_P, C.POINTER(TreeOutC), _P, C.c_int64, _P], C.c_int),
+    "bk_learner_add_out_of_bag_prediction_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
+    "bk_learner_last_splits": ([_P, _P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "bk_learner_add_prediction_for_rows": ([_P, C.POINTER(TreeOutC), _P, C.c_int32, _P, C.c_int64, _P], C.c_int),
     "bk_learner_add_prediction_to_score": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
     "bk_learner_add_prediction_to_score_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
     "bk_learner_leaf_rows": ([_P, C.c_int32, _P, C.POINTER(C.c_int64)], C.c_int),
@@ -335,8 +340,23 @@ class LearnerParams:
         return LearnerParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})
 
 
+SPLIT_RECORD_BYTES = 128  # sizeof(bk_split_record)
+
+
 class TreeBuffers:
     """Host arrays a bk_tree_out points into; `as_dict()` returns trimmed copies."""
+
+    @classmethod
+    def from_dict(cls, d, num_leaves_cap):
+        """A bk_tree_out over copies of a tree dict (as train() returns it)."""
+        tb = cls(num_leaves_cap, cat_cap=max(len(d["cat_threshold"]), 1))
+        for k, v in tb.a.items():
+            src = np.asarray(d[k])
+            v[: len(src)] = src
+        tb.c.num_leaves = d["num_leaves"]
+        tb.c.num_cat = d["num_cat"]
+        tb.c.cat_threshold_len = len(d["cat_threshold"])
+        return tb
 
     def __init__(self, num_leaves, cat_cap=1 << 16):
         ni, nl = max(num_leaves - 1, 1), num_leaves
@@ -389,14 +409,55 @@ class Learner:
         self.handle = out
         self.tree = TreeBuffers(self.params.num_leaves)
 
-    def train(self, gradients, hessians):
-        """SerialTreeLearner::train on host arrays; returns the tree as a dict."""
+    def train(self, gradients, hessians, rows=None):
+        """SerialTreeLearner::train on host arrays; returns the tree as a dict.
+        rows: the bag (ascending row ids) the tree is grown on, or None for all rows."""
         g = np.ascontiguousarray(gradients, dtype=np.float32)
         h = np.ascontiguousarray(hessians, dtype=np.float32)
         if g.shape[0] != self.dataset.num_data or h.shape[0] != self.dataset.num_data:
             raise BankaiError("gradients and hessians must have num_data entries")
-        _check(self._lib.bk_learner_train(self.handle, _ptr(g), _ptr(h), None, 0, C.byref(self.tree.c)))
+        r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        _check(self._lib.bk_learner_train(self.handle, _ptr(g), _ptr(h), None if r is None else _ptr(r),
+                                          0 if r is None else len(r), C.byref(self.tree.c)))
         return self.tree.as_dict()
+
+    def train_device_bagged(self, d_gradients_ptr, d_hessians_ptr, d_rows_ptr, num_rows):
+        """train() on device arrays with a device bag of num_rows ascending int32 rows."""
+        _check(self._lib.bk_learner_train_device_bagged(self.handle, C.c_void_p(d_gradients_ptr),
+                                                        C.c_void_p(d_hessians_ptr), C.c_void_p(d_rows_ptr),
+                                                        int(num_rows), C.byref(self.tree.c)))
+        return self.tree.as_dict()
+
+    def add_out_of_bag_prediction(self, leaf_values, rows, score):
+        """score[row] += leaf_values[leaf(row)] for the given rows (host float64 array)."""
+        self.tree.set_leaf_values(leaf_values)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        _check(self._lib.bk_learner_add_out_of_bag_prediction(self.handle, C.byref(self.tree.c), _ptr(r), len(r),
+                                                              _ptr(score)))
+
+    def last_splits(self):
+        """The last tree's split records (an opaque uint8 array, one row per internal node)."""
+        n = C.c_int32()
+        _check(self._lib.bk_learner_last_splits(self.handle, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), SPLIT_RECORD_BYTES), np.uint8)
+        _check(self._lib.bk_learner_last_splits(self.handle, _ptr(out), n.value, C.byref(n)))
+        return out[: n.value].copy()
+
+    def add_prediction_for_rows(self, tree, leaf_values, splits, rows, score):
+        """An earlier tree (the dict train() returned, with leaf_values in place of its
+        own) replayed on rows through its split records; score[row] += its leaf value."""
+        d = dict(tree)
+        d["leaf_value"] = np.asarray(leaf_values, np.float64)
+        tb = TreeBuffers.from_dict(d, self.params.num_leaves)
+        sp = np.ascontiguousarray(splits, np.uint8)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        _check(self._lib.bk_learner_add_prediction_for_rows(self.handle, C.byref(tb.c), _ptr(sp), len(sp), _ptr(r),
+                                                            len(r), _ptr(score)))
+
+    def add_out_of_bag_prediction_device(self, leaf_values, d_score_ptr):
+        self.tree.set_leaf_values(leaf_values)
+        _check(self._lib.bk_learner_add_out_of_bag_prediction_device(self.handle, C.byref(self.tree.c),
+                                                                     C.c_void_p(d_score_ptr)))
 
     def train_device(self, d_gradients_ptr, d_hessians_ptr):
         """train() on device arrays given as raw pointers (e.g. tensor.data_ptr())."""

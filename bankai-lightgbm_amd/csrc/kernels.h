@@ -287,8 +287,10 @@ struct NodeRule {
 // Leaf of every row: the route bitmaps of the nodes are filled from their rules
 // (one lane per bin value), then each row walks the tree from node 0 over its
 // record in buffer 0 (original row order). num_nodes == 0: every row is in leaf 0.
+// bag_bits (bit r: row r was in the tree's bag; null: every row was): a row
+// outside the bag gets ~leaf.
 void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
-                         int32_t* leaf_of, void* stream);
+                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream);
 // Leaf renewal sums: per-block partials over leaf_of, the certificate per leaf,
 // then the sequential chain for flagged leaves (d_flags == nullptr runs the
 // chain for every leaf).
@@ -300,9 +302,20 @@ void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const d
                       double* d_sums, int* d_flags, void* stream);
 void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int num_leaves, const float* g,
                        const float* h, const int* d_flags, double* d_sums, void* stream);
-void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
-                           void* stream);
-int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream);
+// score[row] += d_leaf_values[leaf] over the bag's rows (out_of_bag = 0) or the rows
+// outside the bag (out_of_bag = 1).
+void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, int out_of_bag,
+                           double* score, void* stream);
+// Per-block f64 sums of g and h over rows[0..n) (null rows: rows 0 .. n-1);
+// returns the number of blocks (partials at d_partials[2b], [2b+1]).
+int launch_sum_gh(const float* g, const float* h, const int32_t* rows, int n, int num_data, double* d_partials,
+                  void* stream);
+// The bag's root segment: row bag[j] of buffer 0 (with its payload) becomes row j
+// of dst_buf; bag rows are marked in bag_bits (zeroed before); quantized sums of
+// the bag's packed pairs are added to sums[0..2] (int g, int h, |int g|). Rows
+// not ascending or outside the dataset set err bit 8.
+void launch_gather_bag(const DevState& s, const int32_t* bag, int nbag, int dst_buf, uint32_t* bag_bits,
+                       long long* sums, void* stream);
 // Copies n bytes with a kernel on `stream`. Either side may be pinned host memory
 // (its device address): small transfers then stay on the compute queue instead
 // of a DMA engine, whose completion the queue would wait for.

@@ -1779,6 +1779,60 @@ __global__ void k_pack_f32(const float* __restrict__ g, const float* __restrict_
     gh[i] = make_float2(g[i], const_hess ? 1.0f : h[i]);
 }
 
+// Root segment of a bagged tree (DataPartition::init_with, data_partition.rs:100-107):
+// row bag[j] of buffer 0, with its payload patched in from gh0, becomes row j of
+// buffer dst_buf, so the bag sits ascending in one segment as the root leaf. Lanes
+// take (row, 16-byte chunk) pairs: the writes are coalesced, the reads whole
+// records. The bag's rows are also marked in bag_bits (zeroed by the caller) and,
+// for the quantized form, their packed pairs summed exactly: sums[0] = sum of the
+// int8 gradients, sums[1] = of the uint8 hessians, sums[2] = of |gradient|. A
+// row outside [0, num_data) or not above its predecessor sets err bit 8 and is
+// written as a zero record, so every later kernel stays inside its bins.
+template <int FORM>
+__global__ void __launch_bounds__(256) k_gather_bag(DevState s, const int32_t* __restrict__ bag, int nbag,
+                                                    int dst_buf, uint32_t* __restrict__ bag_bits,
+                                                    long long* __restrict__ sums) {
+  using PT = typename Pay<FORM>::T;
+  const int stride = s.stride, nc = stride >> 4;
+  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
+  uint8_t* dst = rec_of(s, dst_buf);
+  long long sg = 0, sh = 0, sa = 0;
+  const long long total = (long long)nbag * nc;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(q / nc), c = (int)(q - (long long)j * nc);
+    const int row = bag[j];
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (row >= 0 && row < s.num_data && (j == 0 || bag[j - 1] < row)) {
+      v = *(const uint4*)(s.rec[0] + (size_t)row * stride + 16 * c);
+      if (c == pay_chunk) {
+        const PT p = ((const PT*)s.gh0)[row];
+        patch_payload(v, pay_in, p);
+        if constexpr (FORM == FORM_QUANT) {
+          const long long gq = (long long)(int8_t)(p >> 8);
+          sg += gq;
+          sh += (long long)(uint8_t)(p & 0xff);
+          sa += gq < 0 ? -gq : gq;
+        }
+      }
+      if (c == 0) atomicOr(&bag_bits[row >> 5], 1u << (row & 31));
+    } else if (c == 0) {
+      atomicOr(s.err_flag, 8);
+    }
+    *(uint4*)(dst + (size_t)j * stride + 16 * c) = v;
+  }
+  if constexpr (FORM == FORM_QUANT) {
+    sg = wave_sum64(sg);
+    sh = wave_sum64(sh);
+    sa = wave_sum64(sa);
+    if ((threadIdx.x & 63) == 0 && (sg | sh | sa) != 0) {
+      atomicAdd((unsigned long long*)&sums[0], (unsigned long long)sg);
+      atomicAdd((unsigned long long*)&sums[1], (unsigned long long)sh);
+      atomicAdd((unsigned long long*)&sums[2], (unsigned long long)sa);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // renew_int_grad_tree_output (serial.rs:634-655) sums each leaf's f32 gradients
 // and hessians in f64, one addition at a time in ascending row order.
@@ -1832,10 +1886,17 @@ __global__ void __launch_bounds__(256) k_node_bitmaps(const NodeRule* __restrict
   }
 }
 
+// Whether a row was in the tree's bag (no bitmap: every row was).
+__device__ __forceinline__ bool in_bag(const uint32_t* bag_bits, long long r) {
+  return !bag_bits || ((bag_bits[r >> 5] >> (r & 31)) & 1u);
+}
+
 // Rows of a grid stride walk the tree; two rows per step of each lane, so two
-// independent chains of node loads are in flight.
+// independent chains of node loads are in flight. A row outside the tree's bag
+// gets ~leaf: the leaf the tree sends it to, marked as not trained on.
 template <typename NodePtr>
-__device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num_nodes, int32_t* __restrict__ leaf_of) {
+__device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num_nodes,
+                                          const uint32_t* __restrict__ bag_bits, int32_t* __restrict__ leaf_of) {
   const int stride = s.stride, nc = stride >> 4;
   const long long n = s.num_data, step = 2ll * gridDim.x * blockDim.x;
   for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < n; r0 += step) {
@@ -1874,27 +1935,29 @@ __device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num
         n1 = ((nd[n1].left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? nd[n1].left : nd[n1].right;
       }
     }
-    leaf_of[r0] = n0 < 0 ? ~n0 : 0;
-    if (has1) leaf_of[r1] = n1 < 0 ? ~n1 : 0;
+    const int l0 = n0 < 0 ? ~n0 : 0, l1 = n1 < 0 ? ~n1 : 0;
+    leaf_of[r0] = in_bag(bag_bits, r0) ? l0 : ~l0;
+    if (has1) leaf_of[r1] = in_bag(bag_bits, r1) ? l1 : ~l1;
   }
 }
 
 __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev* __restrict__ nodes, int num_nodes,
-                                                      int in_lds, int32_t* __restrict__ leaf_of) {
+                                                      int in_lds, const uint32_t* __restrict__ bag_bits,
+                                                      int32_t* __restrict__ leaf_of) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (num_nodes == 0) {
     for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < s.num_data;
          row += (long long)gridDim.x * blockDim.x)
-      leaf_of[row] = 0;
+      leaf_of[row] = in_bag(bag_bits, row) ? 0 : ~0;
     return;
   }
   if (in_lds) {
     const int words = num_nodes * (int)(sizeof(NodeDev) / 4);
     for (int i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t*)smem)[i] = ((const uint32_t*)nodes)[i];
     __syncthreads();
-    walk_rows(s, (const NodeDev*)smem, num_nodes, leaf_of);  // LDS loads
+    walk_rows(s, (const NodeDev*)smem, num_nodes, bag_bits, leaf_of);  // LDS loads
   } else {
-    walk_rows(s, nodes, num_nodes, leaf_of);
+    walk_rows(s, nodes, num_nodes, bag_bits, leaf_of);
   }
 }
 
@@ -1918,7 +1981,7 @@ __global__ void __launch_bounds__(256) k_renew_par(int n, const int32_t* __restr
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int l = leaf_of[i];
-    if ((unsigned)l >= (unsigned)num_leaves) continue;  // never taken: the leaf ranges tile every row
+    if ((unsigned)l >= (unsigned)num_leaves) continue;  // a row outside the bag (~leaf)
     const float x = g[i], y = h[i];
     atomicAdd(&acc[2 * l], (double)x);
     atomicAdd(&acc[2 * l + 1], (double)y);
@@ -2052,10 +2115,13 @@ __global__ void __launch_bounds__(256) k_renew_sums(int n, const int32_t* __rest
   if (threadIdx.x == 64 && (need & 2)) out[2 * leaf + 1] = ah;
 }
 
+// score[row] += vals[leaf of row] over the rows of the tree's bag (out_of_bag = 0)
+// or over the rows outside it (out_of_bag = 1, whose leaf_of holds ~leaf).
 __global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, const double* __restrict__ vals,
-                                 int num_leaves, double* __restrict__ score) {
+                                 int num_leaves, int out_of_bag, double* __restrict__ score) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int l = leaf_of[i];
+    const int v = leaf_of[i];
+    const int l = out_of_bag ? ~v : v;
     if (l >= 0 && l < num_leaves) score[i] += vals[l];
   }
 }
@@ -2199,12 +2265,16 @@ __global__ void k_binary_gradients(const double* __restrict__ score, const float
 
 
 // Per-block f64 sums of gradients and hessians (root totals of the float forms).
-__global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ h, int n, double* out) {
+// rows: the rows to sum (a bag, each inside [0, num_data)), or null for rows 0 .. n-1.
+__global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ h, const int32_t* __restrict__ rows,
+                         int n, int num_data, double* out) {
   __shared__ double rg[4], rh[4];
   double sg = 0.0, sh = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    sg += (double)g[i];
-    sh += (double)h[i];
+    const int r = rows ? rows[i] : i;
+    if (r < 0 || r >= num_data) continue;  // flagged by the bag's gather
+    sg += (double)g[r];
+    sh += (double)h[r];
   }
   for (int o = 32; o > 0; o >>= 1) {
     sg += __shfl_down(sg, o, 64);
@@ -2501,15 +2571,16 @@ void launch_copy(void* dst, const void* src, size_t n, void* stream) {
   BK_HIP(hipGetLastError());
 }
 
-int launch_sum_gh(const float* g, const float* h, int n, double* d_partials, void* stream) {
+int launch_sum_gh(const float* g, const float* h, const int32_t* rows, int n, int num_data, double* d_partials,
+                  void* stream) {
   const int nb = grid_for(n, 256, 1024);
-  launch(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, n, d_partials);
+  launch(k_sum_gh, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, h, rows, n, num_data, d_partials);
   BK_HIP(hipGetLastError());
   return nb;
 }
 
 void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
-                         int32_t* leaf_of, void* stream) {
+                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream) {
   if (num_nodes > 0) {
     launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes);
     BK_HIP(hipGetLastError());
@@ -2518,7 +2589,7 @@ void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_
   const int in_lds = lds <= (size_t)kLdsBytes ? 1 : 0;
   allow_lds(k_leaf_of_tree);
   launch(k_leaf_of_tree, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), in_lds ? lds : 0, (hipStream_t)stream, s,
-         d_nodes, num_nodes, in_lds, leaf_of);
+         d_nodes, num_nodes, in_lds, bag_bits, leaf_of);
   BK_HIP(hipGetLastError());
 }
 
@@ -2543,10 +2614,22 @@ void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const d
   BK_HIP(hipGetLastError());
 }
 
-void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, double* score,
-                           void* stream) {
+void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, int out_of_bag,
+                           double* score, void* stream) {
   launch(k_add_score_rows, dim3(grid_for(n, 256, 8192)), dim3(256), 0, (hipStream_t)stream, n, leaf_of,
-                     d_leaf_values, num_leaves, score);
+                     d_leaf_values, num_leaves, out_of_bag, score);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_gather_bag(const DevState& s, const int32_t* bag, int nbag, int dst_buf, uint32_t* bag_bits,
+                       long long* sums, void* stream) {
+  if (nbag <= 0) return;
+  const int nc = s.stride >> 4;
+  const dim3 grid(grid_for((long long)nbag * nc, 256, 8192));
+  if (s.form == FORM_QUANT)
+    launch(k_gather_bag<FORM_QUANT>, grid, dim3(256), 0, (hipStream_t)stream, s, bag, nbag, dst_buf, bag_bits, sums);
+  else
+    launch(k_gather_bag<FORM_F64>, grid, dim3(256), 0, (hipStream_t)stream, s, bag, nbag, dst_buf, bag_bits, sums);
   BK_HIP(hipGetLastError());
 }
 

@@ -45,6 +45,8 @@
 
 namespace bk {
 
+static_assert(sizeof(RuleDev) <= sizeof(((bk_split_record*)nullptr)->rule), "split record too small for a rule");
+
 std::string& last_error() {
   static thread_local std::string e;
   return e;
@@ -491,6 +493,11 @@ struct Learner {
   int* d_order = nullptr;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
+  int32_t* d_replay = nullptr;      // leaf of every row under a replayed tree, allocated at first use
+  int32_t* d_bag = nullptr;         // a host-given bag's rows, allocated at the first bagged tree
+  uint32_t* d_bag_bits = nullptr;   // bit r: row r is in the current tree's bag
+  bool last_bagged = false;         // the last tree was trained on a bag
+  bool sums_bagged = false;         // the kept quantized root sums are over a bag
   NodeDev* d_nodes = nullptr;     // internal nodes of the last tree (row-to-leaf walk)
   NodeRule* d_node_rules = nullptr;
   double* d_renew_part_sums = nullptr;
@@ -594,7 +601,7 @@ struct Learner {
                     (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
-                    (void*)d_renew_part_meta, (void*)d_renew_flags})
+                    (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes})
@@ -983,13 +990,35 @@ struct Learner {
     return sp;
   }
 
+  void ensure_bag_buffers() {
+    if (d_bag_bits) return;
+    d_bag = dmalloc<int32_t>((size_t)N + 1);
+    d_bag_bits = dmalloc<uint32_t>((size_t)N / 32 + 1);
+  }
+
+  // The root segment of a bagged tree: the bag's records, ascending, at the start
+  // of buffer 1 (DataPartition::init_with, data_partition.rs:100-107); the rows
+  // are marked for the row-to-leaf map, and the quantized pairs summed into
+  // d_sums3 (zeroed here).
+  void gather_bag(const int32_t* d_rows, int nrows) {
+    BK_HIPC(hipMemsetAsync(d_bag_bits, 0, sizeof(uint32_t) * ((size_t)N / 32 + 1), stream));
+    BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
+    const double db = (double)nrows * (4.0 + 2.0 * stride);
+    timed(4, db, db, [&] { launch_gather_bag(ds, d_rows, nrows, 1, d_bag_bits, d_sums3, stream); });
+  }
+
   // Discretization and root sums (serial.rs:658-776, gradient_discretizer/mod.rs:134-201).
-  LeafSplits before_train(const float* g, const float* h) {
+  // d_rows: the tree's bag on the device (ascending rows), or null for every row.
+  // The discretizer's scales always come from every row; the root sums run over
+  // the bag.
+  LeafSplits before_train(const float* g, const float* h, const int32_t* d_rows = nullptr, int nrows = 0) {
     pool.reset();
     for (auto& s : best) s.reset();
     amax.init(&best);
     std::fill(segs.begin(), segs.end(), Segment{0, 0, 0});
-    segs[0] = Segment{0, 0, N};
+    const bool bag = d_rows != nullptr;
+    const int count = bag ? nrows : N;
+    segs[0] = bag ? Segment{1, 0, nrows} : Segment{0, 0, N};
     stats = bk_train_stats{};
     double sg = 0.0, sh = 0.0;
     long long isum = 0;
@@ -1023,12 +1052,21 @@ struct Learner {
           launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0,
                             stochastic ? 1 : 0, (int16_t*)d_gh0, d_sums3, stream);
         });
-        long long sums[4];
-        launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream);
-        sync();
-        std::memcpy(sums, h_misc, sizeof sums);
-        last_int_g = sums[0];
-        last_int_h = sums[1];
+      }
+      // the bag's sums come from the packed pairs in place, fresh or kept from the
+      // previous tree when this discretization was refused; kept pairs of a bagged
+      // tree are summed again over every row when this tree has no bag
+      if (bag) gather_bag(d_rows, nrows);
+      const bool resum_all = !ok && !bag && sums_bagged;
+      sums_bagged = (ok || bag) ? bag : sums_bagged;
+      if (ok || bag || resum_all) {
+        long long sums[4] = {0, 0, 0, 0};
+        if (!resum_all) {
+          launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream);
+          sync();
+          std::memcpy(sums, h_misc, sizeof sums);
+        }
+        if (resum_all) sums_bagged = false;
         // exact when every sequential partial sum fits the f64 significand
         auto exact = [](long long sum_abs, double scale) {
           if (scale == 0.0) return true;
@@ -1039,18 +1077,34 @@ struct Learner {
           unsigned __int128 prod = (unsigned __int128)(uint64_t)sum_abs * mant;
           return prod < ((unsigned __int128)1 << 53);
         };
-        if (exact(sums[2], grad_scale) && exact(sums[1], hess_scale)) {
+        if (!resum_all && exact(sums[2], grad_scale) && exact(sums[1], hess_scale)) {
+          last_int_g = sums[0];
+          last_int_h = sums[1];
           last_root_g = (double)sums[0] * grad_scale;
           last_root_h = (double)sums[1] * hess_scale;
         } else {
+          // sequential f64 sums in row order over the bag or every row
           std::vector<int16_t> pk((size_t)N);
+          std::vector<int32_t> rows((size_t)(bag ? nrows : 0));
           BK_HIPC(hipMemcpyAsync(pk.data(), d_gh0, sizeof(int16_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+          if (bag && nrows > 0)
+            BK_HIPC(hipMemcpyAsync(rows.data(), d_rows, sizeof(int32_t) * (size_t)nrows, hipMemcpyDeviceToHost,
+                                   stream));
           sync();
           double a = 0.0, b = 0.0;
-          for (int i = 0; i < N; ++i) {
-            a += (double)(int8_t)(pk[(size_t)i] >> 8) * grad_scale;
-            b += (double)(uint8_t)(pk[(size_t)i] & 0xff) * hess_scale;
+          long long ig = 0, ih = 0;
+          for (int j = 0; j < count; ++j) {
+            const int i = bag ? rows[(size_t)j] : j;
+            if (i < 0 || i >= N) continue;  // flagged by the gather, reported at the end of the tree
+            const int8_t gq = (int8_t)(pk[(size_t)i] >> 8);
+            const uint8_t hq = (uint8_t)(pk[(size_t)i] & 0xff);
+            a += (double)gq * grad_scale;
+            b += (double)hq * hess_scale;
+            ig += gq;
+            ih += hq;
           }
+          last_int_g = ig;
+          last_int_h = ih;
           last_root_g = a;
           last_root_h = b;
         }
@@ -1061,7 +1115,8 @@ struct Learner {
       isum = (long long)(((unsigned long long)last_int_g << 32) + (unsigned long long)last_int_h);
     } else {
       launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
-      int nb = launch_sum_gh(g, h, N, d_sum_partials, stream);
+      if (bag) gather_bag(d_rows, nrows);
+      int nb = launch_sum_gh(g, h, d_rows, count, N, d_sum_partials, stream);
       if ((size_t)nb * 2 * sizeof(double) + 16 > kMiscBytes) throw std::runtime_error("set-up landing area too small");
       launch_copy(dev_alias(h_misc) + 16, d_sum_partials, sizeof(double) * 2 * (size_t)nb, stream);
       launch_copy(dev_alias(h_misc), h, sizeof(float), stream);
@@ -1077,7 +1132,7 @@ struct Learner {
       h0 = (double)hh0;
     }
     double carried = smaller.weight;
-    LeafSplits root{0, N, sg, sh, carried, isum};
+    LeafSplits root{0, count, sg, sh, carried, isum};
     smaller = root;
     larger = LeafSplits{};
     return root;
@@ -1407,11 +1462,11 @@ struct Learner {
       set_best(larger.leaf, has_large ? fold(1, pool.splittable[(size_t)larger_slot].data()) : HostSplit());
   }
 
-  Tree train_sequential(const float* g, const float* h) {
-    before_train(g, h);
+  Tree train_sequential(const float* g, const float* h, const int32_t* d_rows, int nrows) {
+    const LeafSplits root = before_train(g, h, d_rows, nrows);
     Tree t(P.num_leaves);
     t.set_leaf_output(0, leaf_output_root(P, smaller.sum_g, smaller.sum_h));
-    if (before_find_best_split(t, 0, -1)) {
+    if (root.num_data > 0 && before_find_best_split(t, 0, -1)) {
       compute_used();
       hist_seg.assign(1, segs[0]);
       const HistPlan hp = plan_hist(1);
@@ -1422,8 +1477,8 @@ struct Learner {
       si.small.nblk = hist_count[0];
       const int nr = plan_scan(1);
       upload_round(0, &hp, 1, nr, true);
-      run_hist(hp, N, true);
-      stats.committed_hist_rows += N;
+      run_hist(hp, root.num_data, true);
+      stats.committed_hist_rows += root.num_data;
       run_scan(1, nr);
       fold_sequential(false);
     }
@@ -1505,8 +1560,8 @@ struct Learner {
   // Batched form: every round expands the top frontier leaves in one launch of
   // each kernel; commits replay the reference's best-first order exactly.
   // ===========================================================================
-  Tree train_batched(const float* g, const float* h) {
-    LeafSplits root = before_train(g, h);
+  Tree train_batched(const float* g, const float* h, const int32_t* d_rows, int nrows) {
+    LeafSplits root = before_train(g, h, d_rows, nrows);
     Tree t(P.num_leaves);
     t.set_leaf_output(0, leaf_output_root(P, root.sum_g, root.sum_h));
     nodes.clear();
@@ -1514,7 +1569,7 @@ struct Learner {
     nodes.reserve(max_nodes);
     if (node_flags.size() < max_nodes * F) node_flags.assign(max_nodes * F, 1);
     Node r0;
-    r0.seg = Segment{0, 0, N};
+    r0.seg = segs[0];
     r0.ls = root;
     r0.slot = 0;
     std::fill(node_flags.begin(), node_flags.begin() + F, (char)1);
@@ -1523,7 +1578,7 @@ struct Learner {
     leaf_node[0] = 0;
     int next_slot = 1;
     // the root search (serial.rs:591-604 with the root's before_find_best_split)
-    if (N >= P.min_data_in_leaf * 2) {
+    if (root.num_data > 0 && root.num_data >= P.min_data_in_leaf * 2) {
       for (int f = 0; f < F; ++f) h_used[f] = 1;
       hist_seg.assign(1, nodes[0].seg);
       const HistPlan hp = plan_hist(1);
@@ -1536,8 +1591,8 @@ struct Learner {
       si.out_off = 0;
       const int nr = plan_scan(1);
       upload_round(0, &hp, 1, nr, true);
-      run_hist(hp, N, true);
-      stats.committed_hist_rows += N;
+      run_hist(hp, root.num_data, true);
+      stats.committed_hist_rows += root.num_data;
       run_scan(1, nr);
       nodes[0].best = fold(0, flags_of(0));
     } else {
@@ -1708,16 +1763,18 @@ struct Learner {
       rules = d_node_rules;
     }
     timed(6, (double)N * (stride + 4.0), (double)N * (stride + 4.0),
-          [&] { launch_leaf_of_tree(ds, rules, d_nodes, ni, d_leaf_of, stream); });
+          [&] { launch_leaf_of_tree(ds, rules, d_nodes, ni, last_bagged ? d_bag_bits : nullptr, d_leaf_of, stream); });
     leaf_of_valid = true;
   }
 
-  Tree train(const float* g, const float* h) {
+  // d_rows: the tree's bag on the device (ascending rows inside the dataset), or null
+  Tree train(const float* g, const float* h, const int32_t* d_rows = nullptr, int nrows = 0) {
     const auto t0 = std::chrono::steady_clock::now();
     wait_ms = 0.0;
     leaf_of_valid = false;
     prof.tree_on = prof.trees++ % prof.every == 0;
-    Tree t = batched ? train_batched(g, h) : train_sequential(g, h);
+    Tree t = batched ? train_batched(g, h, d_rows, nrows) : train_sequential(g, h, d_rows, nrows);
+    last_bagged = d_rows != nullptr;
     last_num_leaves = t.num_leaves;
     last_left.assign(t.left.begin(), t.left.begin() + std::max(t.num_leaves - 1, 0));
     last_right.assign(t.right.begin(), t.right.begin() + std::max(t.num_leaves - 1, 0));
@@ -1753,6 +1810,14 @@ struct Learner {
     sync();
     int err = 0;
     std::memcpy(&err, h_tail, sizeof(int));
+    if (err) {
+      // the flag is cleared so the learner stays usable after a refused call
+      BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
+      sync();
+      leaf_of_valid = true;  // no tree: nothing to map
+      last_num_leaves = 0;
+    }
+    if (err & 8) throw std::invalid_argument("bagged rows must be ascending and inside the dataset");
     if (err) throw std::runtime_error("partition count mismatch on the device");
     if (renew) {
       const double* sums = (const double*)(h_tail + 16);
@@ -1804,9 +1869,11 @@ struct Learner {
       throw std::invalid_argument("the tree does not match the learner's last tree (num_leaves)");
   }
 
-  void add_score_device(const bk_tree_out* t, double* d_score) {
+  // score[row] += leaf value over the last tree's bag (every row when it had none),
+  // or over the rows outside its bag (out_of_bag)
+  void add_score_device(const bk_tree_out* t, double* d_score, bool out_of_bag = false) {
     check_tree_matches(t);
-    if (t->num_leaves <= 1) return;
+    if (t->num_leaves <= 1 || (out_of_bag && !last_bagged)) return;
     ensure_leaf_of();
     stage_wait();
     for (int i = 0; i < t->num_leaves; ++i) h_vals[i] = t->leaf_value[i];
@@ -1814,8 +1881,9 @@ struct Learner {
     staged();
     // no synchronization: the next use of h_vals waits for this copy (stage_wait),
     // and the score is read by later work on the same stream
-    timed(6, (double)N * 20.0, (double)N * 20.0,
-          [&] { launch_add_score_rows(N, d_leaf_of, d_leaf_vals, t->num_leaves, d_score, stream); });
+    timed(6, (double)N * 20.0, (double)N * 20.0, [&] {
+      launch_add_score_rows(N, d_leaf_of, d_leaf_vals, t->num_leaves, out_of_bag ? 1 : 0, d_score, stream);
+    });
   }
 
   void add_score_host(const bk_tree_out* t, double* score) {
@@ -1826,7 +1894,79 @@ struct Learner {
     BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
     sync();
     staged_pending = false;
-    for (int i = 0; i < N; ++i) score[i] += t->leaf_value[lo[(size_t)i]];
+    for (int i = 0; i < N; ++i)
+      if (lo[(size_t)i] >= 0) score[i] += t->leaf_value[lo[(size_t)i]];
+  }
+
+  // add_out_of_bag_prediction (serial.rs:1995-2001): the last tree replayed on
+  // `rows` with the training routing, which is the leaf the walk found for them
+  void add_oob_host(const bk_tree_out* t, const int32_t* rows, int64_t n, double* score) {
+    check_tree_matches(t);
+    if (t->num_leaves <= 1 || n <= 0) return;
+    for (int64_t j = 0; j < n; ++j)
+      if (rows[j] < 0 || rows[j] >= N) throw std::invalid_argument("row outside the dataset");
+    ensure_leaf_of();
+    std::vector<int32_t> lo((size_t)N);
+    BK_HIPC(hipMemcpyAsync(lo.data(), d_leaf_of, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+    sync();
+    staged_pending = false;
+    for (int64_t j = 0; j < n; ++j) {
+      const int v = lo[(size_t)rows[j]];
+      score[rows[j]] += t->leaf_value[v < 0 ? ~v : v];
+    }
+  }
+
+  // last_splits (serial.rs:2003): the routing of every split of the last tree, by
+  // internal node (the order the splits were made)
+  int last_splits(bk_split_record* out, int cap) const {
+    const int ni = std::max(last_num_leaves - 1, 0);
+    if (out && cap < ni) throw std::invalid_argument("split record capacity too small");
+    for (int k = 0; out && k < ni; ++k) {
+      bk_split_record r{};
+      r.group = tree_nodes[(size_t)k].group;
+      std::memcpy(r.rule, &tree_nodes[(size_t)k].rule, sizeof(RuleDev));
+      out[k] = r;
+    }
+    return ni;
+  }
+
+  // add_prediction_for_rows (serial.rs:2009-2040): any earlier tree of this
+  // learner, replayed on `rows` through its split records with the training
+  // routing; score[row] += that tree's leaf value. The walk covers every row of
+  // the dataset on the device, into a map of its own.
+  void add_prediction_for_rows(const bk_tree_out* t, const bk_split_record* splits, int nsplits, const int32_t* rows,
+                               int64_t n, double* score) {
+    const int nl = t->num_leaves;
+    if (nl <= 1 || n <= 0) return;
+    if (nsplits != nl - 1) throw std::invalid_argument("one split record per internal node is needed");
+    if (nl > P.num_leaves) throw std::invalid_argument("tree larger than the learner's num_leaves");
+    for (int64_t j = 0; j < n; ++j)
+      if (rows[j] < 0 || rows[j] >= N) throw std::invalid_argument("row outside the dataset");
+    const int ni = nl - 1;
+    auto child_ok = [&](int c) { return c >= 0 ? c < ni : ~c < nl; };
+    std::vector<NodeRule> rules((size_t)ni);
+    for (int k = 0; k < ni; ++k) {
+      if (splits[k].group < 0 || splits[k].group >= G) throw std::invalid_argument("split record outside the dataset");
+      if (!child_ok(t->left_child[k]) || !child_ok(t->right_child[k]))
+        throw std::invalid_argument("tree children out of range");
+      NodeRule& r = rules[(size_t)k];
+      std::memcpy(&r.rule, splits[k].rule, sizeof(RuleDev));
+      r.group = splits[k].group;
+      r.left = t->left_child[k];
+      r.right = t->right_child[k];
+      r.pad = 0;
+    }
+    if (!d_replay) d_replay = dmalloc<int32_t>((size_t)N);
+    leaf_of_valid = false;  // the node buffers are shared with the last tree's map
+    stage_wait();
+    std::memcpy(h_nodes, rules.data(), sizeof(NodeRule) * (size_t)ni);
+    launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream);
+    launch_leaf_of_tree(ds, d_node_rules, d_nodes, ni, nullptr, d_replay, stream);
+    std::vector<int32_t> lo((size_t)N);
+    BK_HIPC(hipMemcpyAsync(lo.data(), d_replay, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
+    sync();
+    staged_pending = false;
+    for (int64_t j = 0; j < n; ++j) score[rows[j]] += t->leaf_value[lo[(size_t)rows[j]]];
   }
 
   // Root histogram over all rows in the learner's form: what construct_histograms
@@ -1908,12 +2048,19 @@ int bk_learner_train(bk_learner* l, const float* g, const float* h, const int32_
                      bk_tree_out* tree) {
   return bk::guard([&] {
     if (!l || !g || !h || !tree) throw std::invalid_argument("null argument");
-    if (rows) throw std::invalid_argument("bagged row subsets are not supported by the GPU learner");
-    (void)num_rows;
     bk::Learner& L = l->L;
+    if (rows) {
+      if (num_rows < 0 || num_rows > L.N) throw std::invalid_argument("bag larger than the dataset");
+      for (int64_t i = 0; i < num_rows; ++i)
+        if (rows[i] < 0 || rows[i] >= L.N || (i > 0 && rows[i] <= rows[i - 1]))
+          throw std::invalid_argument("bagged rows must be ascending and inside the dataset");
+      L.ensure_bag_buffers();
+      if (num_rows > 0)
+        BK_HIPC(hipMemcpyAsync(L.d_bag, rows, sizeof(int32_t) * (size_t)num_rows, hipMemcpyHostToDevice, L.stream));
+    }
     BK_HIPC(hipMemcpyAsync(L.d_grad, g, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
     BK_HIPC(hipMemcpyAsync(L.d_hess, h, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
-    bk::Tree t = L.train(L.d_grad, L.d_hess);
+    bk::Tree t = L.train(L.d_grad, L.d_hess, rows ? L.d_bag : nullptr, rows ? (int)num_rows : 0);
     L.export_tree(t, tree);
   });
 }
@@ -1932,6 +2079,49 @@ int bk_learner_add_prediction_to_score(bk_learner* l, const bk_tree_out* tree, d
 
 int bk_learner_add_prediction_to_score_device(bk_learner* l, const bk_tree_out* tree, double* d_score) {
   return bk::guard([&] { l->L.add_score_device(tree, d_score); });
+}
+
+int bk_learner_train_device_bagged(bk_learner* l, const float* dg, const float* dh, const int32_t* d_rows,
+                                   int64_t num_rows, bk_tree_out* tree) {
+  return bk::guard([&] {
+    if (!l || !dg || !dh || !d_rows || !tree) throw std::invalid_argument("null argument");
+    bk::Learner& L = l->L;
+    if (num_rows < 0 || num_rows > L.N) throw std::invalid_argument("bag larger than the dataset");
+    L.ensure_bag_buffers();
+    bk::Tree t = L.train(dg, dh, d_rows, (int)num_rows);
+    L.export_tree(t, tree);
+  });
+}
+
+int bk_learner_add_out_of_bag_prediction(bk_learner* l, const bk_tree_out* tree, const int32_t* rows,
+                                         int64_t num_rows, double* score) {
+  return bk::guard([&] {
+    if (!l || !tree || (num_rows > 0 && (!rows || !score))) throw std::invalid_argument("null argument");
+    l->L.add_oob_host(tree, rows, num_rows, score);
+  });
+}
+
+int bk_learner_last_splits(bk_learner* l, bk_split_record* out, int32_t capacity, int32_t* num_out) {
+  return bk::guard([&] {
+    if (!l || !num_out) throw std::invalid_argument("null argument");
+    *num_out = l->L.last_splits(out, capacity);
+  });
+}
+
+int bk_learner_add_prediction_for_rows(bk_learner* l, const bk_tree_out* tree, const bk_split_record* splits,
+                                       int32_t num_splits, const int32_t* rows, int64_t num_rows, double* score) {
+  return bk::guard([&] {
+    if (!l || !tree || (num_rows > 0 && (!rows || !score)) || (num_splits > 0 && !splits))
+      throw std::invalid_argument("null argument");
+    l->L.add_prediction_for_rows(tree, splits, num_splits, rows, num_rows, score);
+  });
+}
+
+int bk_learner_add_out_of_bag_prediction_device(bk_learner* l, const bk_tree_out* tree, double* d_score) {
+  return bk::guard([&] {
+    if (!l || !tree || !d_score) throw std::invalid_argument("null argument");
+    l->L.add_score_device(tree, d_score, true);
+  });
 }
 
 int bk_learner_leaf_rows(bk_learner* l, int32_t leaf, int32_t* rows_out, int64_t* num_out) {

@@ -143,7 +143,10 @@ int bk_learner_create(const bk_dataset_view* ds, const bk_learner_params* params
                       int is_constant_hessian, bk_learner** out);
 
 /* SerialTreeLearner::train (crates/lgbm-core/src/treelearner/serial.rs:569) with host
- * gradient/hessian arrays of num_data entries. rows: ascending bagged subset or NULL. */
+ * gradient/hessian arrays of num_data entries. rows: the bagged subset (num_rows
+ * distinct rows, ascending, each in [0, num_data)) the tree is grown on, or NULL for
+ * every row (DataPartition::init_with / init, data_partition.rs:91-107). The
+ * quantized scales always come from every row, as in the reference. */
 int bk_learner_train(bk_learner* learner, const float* gradients, const float* hessians,
                      const int32_t* rows, int64_t num_rows, bk_tree_out* tree);
 
@@ -152,14 +155,56 @@ int bk_learner_train(bk_learner* learner, const float* gradients, const float* h
 int bk_learner_train_device(bk_learner* learner, const float* d_gradients,
                             const float* d_hessians, bk_tree_out* tree);
 
+/* bk_learner_train on device arrays with a bag in device memory (d_rows: num_rows
+ * ascending rows). Rows out of order or outside the dataset fail the call (-1)
+ * when the tree ends. */
+int bk_learner_train_device_bagged(bk_learner* learner, const float* d_gradients,
+                                   const float* d_hessians, const int32_t* d_rows,
+                                   int64_t num_rows, bk_tree_out* tree);
+
 /* SerialTreeLearner::add_prediction_to_score (crates/lgbm-core/src/treelearner/serial.rs:1974):
- * score[row] += leaf_value[leaf(row)] for the last trained tree, host scores. */
+ * score[row] += leaf_value[leaf(row)] for the last trained tree, host scores, over
+ * the rows the tree was trained on (its bag, or every row). */
 int bk_learner_add_prediction_to_score(bk_learner* learner, const bk_tree_out* tree,
                                        double* score);
 
 /* The same on a device score array of num_data doubles. */
 int bk_learner_add_prediction_to_score_device(bk_learner* learner, const bk_tree_out* tree,
                                               double* d_score);
+
+/* SerialTreeLearner::add_out_of_bag_prediction (serial.rs:1995): the last tree
+ * replayed on `rows` (any rows of the dataset, usually those outside the bag,
+ * train.rs:1282-1291) with the training routing; score[row] += its leaf's value. */
+int bk_learner_add_out_of_bag_prediction(bk_learner* learner, const bk_tree_out* tree,
+                                         const int32_t* rows, int64_t num_rows, double* score);
+
+/* The same on a device score array, over every row outside the last tree's bag
+ * (nothing when it had no bag). */
+int bk_learner_add_out_of_bag_prediction_device(bk_learner* learner, const bk_tree_out* tree,
+                                                double* d_score);
+
+/* How one split of a tree routes rows (SplitRecord, crates/lgbm-core/src/treelearner/
+ * split_info.rs and serial.rs:2003): the group whose bin is read and the learner's
+ * routing rule, stored as it was applied. Opaque to the caller, who keeps the
+ * records of each tree to replay it later. */
+typedef struct bk_split_record {
+  int32_t group;
+  uint8_t rule[124];
+} bk_split_record;
+
+/* SerialTreeLearner::last_splits (serial.rs:2003): the records of the last tree's
+ * splits, one per internal node in node order, into out (capacity records; out may
+ * be NULL to ask for the count). *num_out receives num_leaves - 1. */
+int bk_learner_last_splits(bk_learner* learner, bk_split_record* out, int32_t capacity,
+                           int32_t* num_out);
+
+/* SerialTreeLearner::add_prediction_for_rows (serial.rs:2009): an earlier tree of
+ * this learner (its children and leaf values from `tree`, its routing from the
+ * records last_splits gave for it) replayed on `rows` of the dataset;
+ * score[row] += the leaf value (DART and rollback, train.rs:249,961,1071). */
+int bk_learner_add_prediction_for_rows(bk_learner* learner, const bk_tree_out* tree,
+                                       const bk_split_record* splits, int32_t num_splits,
+                                       const int32_t* rows, int64_t num_rows, double* score);
 
 /* DataPartition::indices_on_leaf (crates/lgbm-core/src/treelearner/data_partition.rs:109-112),
  * read through the public `partition` field by Trainer::renew_tree_output

@@ -53,18 +53,21 @@ def binary_10k():
     return bg.Dataset(x, bg.DatasetParams(num_threads=8)), y
 
 
-def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise):
-    """Boosts the oracle; the GPU learner trains each tree on the oracle's gradients."""
-    o = ob.OracleLearner(ds, params, obj.const_hess)
-    gpu = bg.Learner(ds, params, obj.const_hess)
+def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise, sampler=None):
+    """Boosts the oracle; the GPU learner trains each tree on the oracle's gradients
+    (and on the same bag, when a row sampler is given)."""
+    const_hess = obj.const_hess and not (sampler is not None and sampler.changes_hessians)
+    o = ob.OracleLearner(ds, params, const_hess)
+    gpu = bg.Learner(ds, params, const_hess)
     score = np.zeros(ds.num_data)
     init = obj.boost_from_score(0)
     score += init
     trees = []
     for it in range(iters):
         g, h = obj.gradients(score)
-        to = o.train(g, h)
-        tg = gpu.train(g, h)
+        rows = sampler.sample(it, g, h) if sampler is not None else None
+        to = o.train(g, h, rows)
+        tg = gpu.train(g, h, rows)
         compare(tg, to, f"iter {it}")
         for leaf in range(to["num_leaves"]):
             assert np.array_equal(gpu.leaf_rows(leaf), o.leaf_rows(leaf)), (it, leaf)
@@ -72,6 +75,10 @@ def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise):
         s_gpu = score.copy()
         gpu.add_prediction_to_score(vals, s_gpu)
         o.add_prediction_to_score(vals, score)
+        oob = bst.out_of_bag(rows, ds.num_data)
+        if oob is not None:
+            gpu.add_out_of_bag_prediction(vals, oob, s_gpu)
+            o.add_out_of_bag_prediction(vals, oob, score)
         assert np.array_equal(s_gpu, score)
         trees.append((tg, to))
     return trees
@@ -381,3 +388,133 @@ def test_lambdarank_gradients_device_match_host():
         sh = np.max(np.abs(want_h[lo:hi])) + 1e-30
         assert np.all(np.abs(g[lo:hi] - want_g[lo:hi]) <= 1e-5 * sg + 4 * np.spacing(np.abs(want_g[lo:hi])))
         assert np.all(np.abs(h[lo:hi] - want_h[lo:hi]) <= 1e-5 * sh + 4 * np.spacing(np.abs(want_h[lo:hi])))
+
+
+BAG_PARAMS = {
+    "bagging": {"bagging_fraction": 0.6, "bagging_freq": 1, "bagging_seed": 5},
+    "bagging_freq3": {"bagging_fraction": 0.3, "bagging_freq": 3, "bagging_seed": 9},
+    "balanced": {"objective": "binary", "bagging_freq": 1, "pos_bagging_fraction": 0.5, "neg_bagging_fraction": 0.8},
+    "goss": {"data_sample_strategy": "goss", "top_rate": 0.2, "other_rate": 0.1, "learning_rate": 0.5},
+}
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("stochastic", [0, 1])
+@pytest.mark.parametrize("kind", sorted(BAG_PARAMS))
+def test_bagged_quantized_bit_exact(binary_10k, kind, stochastic, batched):
+    """Trees on a row sample (bagging, balanced bagging, GOSS): the quantized GPU
+    learner against the oracle, bit for bit, with the rows of every leaf and the
+    in-bag plus out-of-bag score updates."""
+    announce(f"bagged quantized {kind} stochastic={stochastic} batched={batched}")
+    ds, y = binary_10k
+    sampler = bst.RowSampler(BAG_PARAMS[kind], y)
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=stochastic,
+                         quant_train_renew_leaf=1, seed=3, form=bg.FORM_QUANTIZED, batched=batched)
+    trees = lockstep(ds, bst.BinaryObjective(y), p, iters=5, sampler=sampler)
+    assert any(t[1]["leaf_count"].sum() < ds.num_data for t in trees)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("form", [bg.FORM_PLAIN_F64, bg.FORM_F32_COMPENSATED])
+def test_bagged_float_forms(binary_10k, form, batched):
+    announce(f"bagged float form {form} batched={batched}")
+    ds, y = binary_10k
+    sampler = bst.RowSampler(BAG_PARAMS["bagging"], y)
+    p = bg.LearnerParams(num_leaves=31, form=form, batched=batched)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=4, compare=assert_trees_close, sampler=sampler)
+
+
+def test_bagged_device_entry_points(binary_10k):
+    """train_device_bagged and the device out-of-bag score update agree with the
+    host entry points; a bag out of order fails the call and the learner stays usable."""
+    announce("bagged device entry points")
+    import torch
+    assert torch.cuda.is_available()
+    ds, y = binary_10k
+    n = ds.num_data
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=3, form=bg.FORM_QUANTIZED, batched=1)
+    obj = bst.BinaryObjective(y)
+    g, h = obj.gradients(np.zeros(n))
+    rows = bst.RowSampler(BAG_PARAMS["bagging"], y).sample(0, g, h)
+    host = bg.Learner(ds, p, False)
+    dev = bg.Learner(ds, p, False)
+    th = host.train(g, h, rows)
+    dg = torch.from_numpy(g).cuda()
+    dh = torch.from_numpy(h).cuda()
+    dr = torch.from_numpy(rows).cuda()
+    torch.cuda.synchronize()
+    td = dev.train_device_bagged(dg.data_ptr(), dh.data_ptr(), dr.data_ptr(), len(rows))
+    assert_trees_bitwise(td, th, "device bag")
+    vals = th["leaf_value"] * 0.1
+    s_host = np.zeros(n)
+    host.add_prediction_to_score(vals, s_host)
+    host.add_out_of_bag_prediction(vals, bst.out_of_bag(rows, n), s_host)
+    ds_score = torch.zeros(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    dev.add_prediction_to_score_device(vals, ds_score.data_ptr())
+    dev.add_out_of_bag_prediction_device(vals, ds_score.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(ds_score.cpu().numpy(), s_host)
+    # rows out of order: every kernel stays inside the dataset, the call fails
+    bad = dr.flip(0).contiguous()
+    torch.cuda.synchronize()
+    with pytest.raises(bg.BankaiError, match="ascending"):
+        dev.train_device_bagged(dg.data_ptr(), dh.data_ptr(), bad.data_ptr(), len(rows))
+    with pytest.raises(bg.BankaiError, match="ascending"):
+        host.train(g, h, rows[::-1].copy())
+    # both learners still train (the refused device call drew its noise offset, so
+    # the trees are checked for their bag, not against each other)
+    for lr_ in (dev, host):
+        t = lr_.train(g, h, rows)
+        assert t["num_leaves"] > 1 and t["leaf_count"].sum() == len(rows)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_replay_earlier_trees_from_split_records(batched):
+    """last_splits / add_prediction_for_rows: every tree of a bagged run, replayed at
+    the end through its split records, gives each row the leaf value the oracle's
+    in-bag and out-of-bag updates gave it when the tree was grown (categorical splits
+    included)."""
+    announce(f"replay of earlier trees from split records, batched={batched}")
+    x, y = synth.multiclass_50(20_000, seed=4)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63, num_threads=8), categorical=synth.MULTICLASS_CATEGORICAL)
+    yb = (y < 5).astype(float)
+    obj = bst.BinaryObjective(yb)
+    sampler = bst.RowSampler(BAG_PARAMS["bagging"], yb)
+    p = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=2, form=bg.FORM_QUANTIZED, batched=batched)
+    o = ob.OracleLearner(ds, p, False)
+    gpu = bg.Learner(ds, p, False)
+    n = ds.num_data
+    score = np.full(n, obj.boost_from_score(0))
+    kept = []
+    for it in range(4):
+        g, h = obj.gradients(score)
+        rows = sampler.sample(it, g, h)
+        to = o.train(g, h, rows)
+        tg = gpu.train(g, h, rows)
+        assert_trees_bitwise(tg, to, f"iter {it}")
+        vals = to["leaf_value"] * 0.1
+        contrib = np.zeros(n)
+        o.add_prediction_to_score(vals, contrib)
+        o.add_out_of_bag_prediction(vals, bst.out_of_bag(rows, n), contrib)
+        kept.append((tg, vals, gpu.last_splits(), contrib))
+        score += contrib
+    assert any(t[0]["num_cat"] > 0 for t in kept)
+    all_rows = np.arange(n, dtype=np.int32)
+    some = np.sort(np.random.default_rng(0).choice(n, 777, replace=False)).astype(np.int32)
+    for k, (tree, vals, splits, contrib) in enumerate(kept):
+        assert len(splits) == tree["num_leaves"] - 1
+        replay = np.zeros(n)
+        gpu.add_prediction_for_rows(tree, vals, splits, all_rows, replay)
+        assert np.array_equal(replay, contrib), k
+        part = np.zeros(n)
+        gpu.add_prediction_for_rows(tree, vals, splits, some, part)
+        assert np.array_equal(part[some], contrib[some]) and not part[np.setdiff1d(all_rows, some)].any(), k
+    # the last tree's own map is rebuilt after the replays
+    last_tree, last_vals, _, last_contrib = kept[-1]
+    s = np.zeros(n)
+    gpu.add_prediction_to_score(last_vals, s)
+    gpu.add_out_of_bag_prediction(last_vals, bst.out_of_bag(rows, n), s)
+    assert np.array_equal(s, last_contrib)
