@@ -61,7 +61,8 @@ class LearnerParamsC(C.Structure):
         ("histogram_pool_size", C.c_double), ("use_quantized_grad", C.c_int32),
         ("num_grad_quant_bins", C.c_int32), ("stochastic_rounding", C.c_int32),
         ("quant_train_renew_leaf", C.c_int32), ("seed", C.c_int32), ("num_threads", C.c_int32),
-        ("form", C.c_int32), ("batched", C.c_int32),
+        ("form", C.c_int32), ("batched", C.c_int32), ("feature_fraction", C.c_double),
+        ("feature_fraction_bynode", C.c_double), ("feature_fraction_seed", C.c_int32),
     ]
 
 
@@ -335,6 +336,9 @@ class LearnerParams:
     num_threads: int = 1
     form: int = FORM_PLAIN_F64
     batched: int = 0
+    feature_fraction: float = 1.0
+    feature_fraction_bynode: float = 1.0
+    feature_fraction_seed: int = 2
 
     def to_c(self):
         return LearnerParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})

@@ -1565,14 +1565,17 @@ __device__ void fold_side(const ScanItem& it, int k, int side, const SplitOut* o
   const SplitOut* out = out_all + it.out_off + (side ? F : 0);
   double bg = kMinScore;
   int bk = INT_MAX, bf = -1;
+  // used[f]: bit 0 searched (its splittable flag is kept), bit 1 / bit 2 it may be
+  // the smaller / larger leaf's best (the by-node feature samples)
   for (int f = lane; f < F; f += 64) {
     int sp = 0;
-    if (active && used[f]) {
+    if (active && (used[f] & 1)) {
       const double gain = out[f].gain;
       const int feat = out[f].feature;
       sp = out[f].splittable ? 1 : 0;
       const int key = feat == -1 ? INT_MAX : feat;
-      if (!(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
+      const bool eligible = (used[f] >> (1 + side)) & 1;
+      if (eligible && !(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
         bg = gain;
         bk = key;
         bf = f;

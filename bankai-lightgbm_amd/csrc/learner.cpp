@@ -405,6 +405,85 @@ struct Prof {
 
 }  // namespace
 
+// LightGBM's sampling LCG (utils/random.rs:6-80): x = x * 214013 + 2531011.
+struct Lcg {
+  uint32_t x = 0;
+  uint32_t step() { return x = x * 214013u + 2531011u; }
+  float next_float() { return (float)((step() >> 16) & 0x7FFFu) / 32768.0f; }
+  int next_int(int lo, int hi) { return (int)(step() & 0x7FFFFFFFu) % (hi - lo) + lo; }
+  // k sorted distinct values of 0..n-1 (Random::sample, random.rs:52-79): a
+  // selection scan when k is large against n / log2 k, else Floyd's algorithm
+  std::vector<int> sample(int n, int k) {
+    std::vector<int> out;
+    if (k > n || k <= 0) return out;
+    out.reserve((size_t)k);
+    if (k == n) {
+      for (int i = 0; i < n; ++i) out.push_back(i);
+    } else if (k > 1 && (double)k > (double)n / std::log2((double)k)) {
+      for (int i = 0; i < n; ++i)
+        if ((double)next_float() < (double)(k - (int)out.size()) / (double)(n - i)) out.push_back(i);
+    } else {
+      std::vector<char> taken((size_t)n, 0);
+      for (int r = n - k; r < n; ++r) {
+        const int v = next_int(0, r + 1);
+        taken[(size_t)(taken[(size_t)v] ? r : v)] = 1;
+      }
+      for (int i = 0; i < n; ++i)
+        if (taken[(size_t)i]) out.push_back(i);
+    }
+    return out;
+  }
+};
+
+// Feature sampling per tree and per node (ColSampler, treelearner/col_sampler.rs:
+// 14-176, without interaction constraints). One LCG stream: a by-tree draw at set-up
+// and at every tree, by-node draws at every search (smaller leaf, then larger).
+struct FeatureSampler {
+  Lcg rnd;
+  std::vector<int> valid;  // inner features in real-index order
+  std::vector<int> in_tree;  // indices into `valid` of the tree's sample
+  std::vector<char> bytree;  // per inner feature
+  double bynode = 1.0;
+  bool per_tree = false;
+  int tree_cnt = 0;
+
+  static int count(int total, double fraction) {
+    return std::max(std::min(1, total), (int)((double)total * fraction + 0.5));
+  }
+  void init(std::vector<int> order, double ff, double ffn, int32_t seed) {
+    valid = std::move(order);
+    bytree.assign(valid.size(), 1);
+    rnd.x = (uint32_t)seed;
+    bynode = ffn;
+    per_tree = ff < 1.0;
+    tree_cnt = per_tree ? count((int)valid.size(), ff) : (int)valid.size();
+    next_tree();
+  }
+  void next_tree() {
+    if (!per_tree) return;
+    std::fill(bytree.begin(), bytree.end(), (char)0);
+    in_tree = rnd.sample((int)valid.size(), tree_cnt);
+    for (int i : in_tree) bytree[(size_t)valid[(size_t)i]] = 1;
+  }
+  bool by_node_active() const { return bynode < 1.0; }
+  std::vector<char> by_node() {
+    std::vector<char> m(valid.size(), bynode >= 1.0 ? 1 : 0);
+    if (bynode >= 1.0) return m;
+    if (per_tree) {
+      for (int p : rnd.sample((int)in_tree.size(), count((int)in_tree.size(), bynode)))
+        m[(size_t)valid[(size_t)in_tree[(size_t)p]]] = 1;
+    } else {
+      for (int p : rnd.sample((int)valid.size(), count((int)valid.size(), bynode))) m[(size_t)valid[(size_t)p]] = 1;
+    }
+    return m;
+  }
+};
+
+// Bits of a search item's per-feature word: searched (its histogram is scanned and
+// its splittable flag kept), and whether it may be the smaller / larger leaf's best
+// (the by-node samples).
+constexpr int kUseSearch = 1, kUseSmall = 2, kUseLarge = 4, kUseAll = 7;
+
 // A leaf of the growing tree in the batched form: its rows, sums, best split,
 // and (after a speculative expansion) its two evaluated children.
 struct Node {
@@ -415,6 +494,7 @@ struct Node {
   int slot = -1;  // pool slot of its histogram
   int child[2] = {-1, -1};
   bool expanded = false;
+  bool kids_searched = false;  // its children's splits were searched
 };
 
 struct Learner {
@@ -550,6 +630,13 @@ struct Learner {
   std::vector<Node> nodes;
   std::vector<int> leaf_node;
   std::vector<char> node_flags;  // splittable flags of node i at [i * F, (i + 1) * F)
+  FeatureSampler cols;
+  // by-node sampling in the batched form: every searched node keeps its per-feature
+  // candidates and searched bits; its best is folded on the host when the reference
+  // would draw its sample (when its parent's split is committed)
+  SplitOut* h_out = nullptr;           // pinned copy of a round's per-feature results
+  std::vector<SplitOut> node_cands;    // node i: [i * F, (i + 1) * F)
+  std::vector<uint8_t> node_searched;  // node i: [i * F, (i + 1) * F)
   char* flags_of(int node) { return node_flags.data() + (size_t)node * F; }
 
   static constexpr size_t kMiscBytes = 16 + sizeof(double) * 2 * (2048 + 2);
@@ -604,7 +691,8 @@ struct Learner {
                     (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc,
-                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes})
+                    (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
+                    (void*)h_out})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -632,6 +720,9 @@ struct Learner {
   void create(const bk_dataset_view* v, const bk_learner_params* p, int is_const_hess) {
     P = *p;
     if (P.extra_trees) throw std::invalid_argument("extra_trees is not supported by the GPU learner");
+    if (!(P.feature_fraction > 0.0 && P.feature_fraction <= 1.0) ||
+        !(P.feature_fraction_bynode > 0.0 && P.feature_fraction_bynode <= 1.0))
+      throw std::invalid_argument("feature_fraction and feature_fraction_bynode must be in (0, 1]");
     if (v->num_data <= 0) throw std::invalid_argument("empty dataset");
     if (v->num_data > (1LL << 30)) throw std::invalid_argument("more than 2^30 rows");
     if (P.num_leaves < 2) throw std::invalid_argument("num_leaves must be at least 2");
@@ -904,6 +995,14 @@ struct Learner {
     best.assign((size_t)P.num_leaves, HostSplit());
     segs.assign((size_t)P.num_leaves, Segment{0, 0, 0});
     used_feat.assign((size_t)F, 1);
+    {
+      std::vector<int> order((size_t)F);
+      for (int f = 0; f < F; ++f) order[(size_t)f] = f;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return v->feature_real_index[a] < v->feature_real_index[b]; });
+      cols.init(std::move(order), P.feature_fraction, P.feature_fraction_bynode, P.feature_fraction_seed);
+      if (cols.by_node_active()) pinned(&h_out, (size_t)F * 2 * max_items);
+    }
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
     sync();
@@ -990,6 +1089,18 @@ struct Learner {
     return sp;
   }
 
+  // group_used on the device: every group (all) or those with a feature in the
+  // tree's sample
+  void set_groups_used(bool all) {
+    stage_wait();
+    for (int g = 0; g < G; ++g) h_group_used[g] = all ? 1 : 0;
+    if (!all)
+      for (int f = 0; f < F; ++f)
+        if (cols.bytree[(size_t)f]) h_group_used[feats[(size_t)f].group] = 1;
+    launch_copy(d_group_used, dev_alias(h_group_used), sizeof(int) * (size_t)G, stream);
+    staged();
+  }
+
   void ensure_bag_buffers() {
     if (d_bag_bits) return;
     d_bag = dmalloc<int32_t>((size_t)N + 1);
@@ -1019,6 +1130,12 @@ struct Learner {
     const bool bag = d_rows != nullptr;
     const int count = bag ? nrows : N;
     segs[0] = bag ? Segment{1, 0, nrows} : Segment{0, 0, N};
+    // the tree's feature sample (col_sampler.rs:87-101): histograms only for the
+    // groups of its features
+    if (cols.per_tree) {
+      cols.next_tree();
+      set_groups_used(false);
+    }
     stats = bk_train_stats{};
     double sg = 0.0, sh = 0.0;
     long long isum = 0;
@@ -1326,6 +1443,9 @@ struct Learner {
       launch_reduce(ds, d_ritems, nr, stream);
       launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, d_fold, d_flags, stream);
     });
+    // the batched form with by-node sampling folds on the host, later
+    if (batched && cols.by_node_active())
+      launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)nitems, stream);
     sync();
     stats.rounds += 1;
   }
@@ -1345,13 +1465,40 @@ struct Learner {
 
   // The folded best split of (item, side) k = 2 * item + side and the
   // splittable flags of its features (serial.rs:1500-1566; folded on the device).
-  HostSplit fold(int k, char* splittable) const {
+  // searched: the features whose flags the search set (null: all); the flags of
+  // the others are left as they are, as the reference leaves them
+  HostSplit fold(int k, char* splittable, const int* searched = nullptr) const {
     HostSplit b;
     if (h_fold[k].inner >= 0) {
       b.o = h_fold[k].o;
       b.inner = h_fold[k].inner;
     }
-    for (int f = 0; f < F; ++f) splittable[f] = (char)h_flags[(size_t)k * F + f];
+    for (int f = 0; f < F; ++f)
+      if (!searched || searched[f]) splittable[f] = (char)h_flags[(size_t)k * F + f];
+    return b;
+  }
+
+  // The fold of one side's per-feature candidates over the features that were
+  // searched and are in the by-node sample: the highest gain, ties to the lower
+  // real feature index, NaN never (split_info.rs:9-96 better_than), in feature order.
+  HostSplit fold_host(const SplitOut* out, const uint8_t* searched, const std::vector<char>& sample) const {
+    HostSplit b;
+    double bg = kMinScore;
+    int bk = INT_MAX, bf = -1;
+    for (int f = 0; f < F; ++f) {
+      if (!searched[f] || !sample[(size_t)f]) continue;
+      const double gain = out[f].gain;
+      const int key = out[f].feature == -1 ? INT_MAX : out[f].feature;
+      if (!(gain != gain) && (gain > bg || (gain == bg && key < bk))) {
+        bg = gain;
+        bk = key;
+        bf = f;
+      }
+    }
+    if (bf >= 0) {
+      b.o = out[bf];
+      b.inner = bf;
+    }
     return b;
   }
 
@@ -1443,23 +1590,37 @@ struct Learner {
   }
 
   // is_feature_used from the parent's splittable flags (serial.rs:838-852)
+  // with the tree's feature sample and the by-node samples, drawn here for the
+  // smaller leaf and then the larger one (serial.rs:1329-1337)
   void compute_used() {
     stage_wait();
+    std::vector<char> sm, lg;
+    if (cols.by_node_active()) {
+      sm = cols.by_node();
+      if (larger.leaf >= 0) lg = cols.by_node();
+    }
     for (int f = 0; f < F; ++f) {
-      used_feat[(size_t)f] = 1;
-      if (parent_slot >= 0 && !pool.splittable[(size_t)parent_slot][(size_t)f]) {
+      used_feat[(size_t)f] = cols.bytree[(size_t)f];
+      if (used_feat[(size_t)f] && parent_slot >= 0 && !pool.splittable[(size_t)parent_slot][(size_t)f]) {
         pool.splittable[smaller_slot][(size_t)f] = 0;
         used_feat[(size_t)f] = 0;
       }
-      h_used[f] = used_feat[(size_t)f];
+      int w = 0;
+      if (used_feat[(size_t)f]) {
+        w = kUseSearch;
+        if (sm.empty() || sm[(size_t)f]) w |= kUseSmall;
+        if (lg.empty() || lg[(size_t)f]) w |= kUseLarge;
+      }
+      h_used[f] = w;
     }
   }
 
   void fold_sequential(bool has_large) {
-    HostSplit sb = fold(0, pool.splittable[smaller_slot].data());
+    HostSplit sb = fold(0, pool.splittable[smaller_slot].data(), used_feat.data());
     set_best(smaller.leaf, sb);
     if (larger.leaf >= 0)
-      set_best(larger.leaf, has_large ? fold(1, pool.splittable[(size_t)larger_slot].data()) : HostSplit());
+      set_best(larger.leaf,
+               has_large ? fold(1, pool.splittable[(size_t)larger_slot].data(), used_feat.data()) : HostSplit());
   }
 
   Tree train_sequential(const float* g, const float* h, const int32_t* d_rows, int nrows) {
@@ -1574,12 +1735,22 @@ struct Learner {
     r0.slot = 0;
     std::fill(node_flags.begin(), node_flags.begin() + F, (char)1);
     nodes.push_back(r0);
+    const bool host_fold = cols.by_node_active();
+    if (host_fold && node_cands.size() < max_nodes * F) {
+      node_cands.resize(max_nodes * F);
+      node_searched.resize(max_nodes * F);
+    }
+    // a searched node's candidates, kept for its fold at its parent's commit
+    auto keep = [&](int node, int k2, int side) {
+      std::memcpy(&node_cands[(size_t)node * F], h_out + ((size_t)2 * k2 + side) * F, sizeof(SplitOut) * (size_t)F);
+      for (int f = 0; f < F; ++f) node_searched[(size_t)node * F + f] = (uint8_t)(h_used[k2 * F + f] & kUseSearch);
+    };
     leaf_node.assign((size_t)P.num_leaves, -1);
     leaf_node[0] = 0;
     int next_slot = 1;
     // the root search (serial.rs:591-604 with the root's before_find_best_split)
     if (root.num_data > 0 && root.num_data >= P.min_data_in_leaf * 2) {
-      for (int f = 0; f < F; ++f) h_used[f] = 1;
+      for (int f = 0; f < F; ++f) h_used[f] = cols.bytree[(size_t)f] ? kUseAll : 0;
       hist_seg.assign(1, nodes[0].seg);
       const HistPlan hp = plan_hist(1);
       ScanItem& si = h_sitems[0];
@@ -1595,6 +1766,10 @@ struct Learner {
       stats.committed_hist_rows += root.num_data;
       run_scan(1, nr);
       nodes[0].best = fold(0, flags_of(0));
+      if (host_fold) {
+        keep(0, 0, 0);
+        nodes[0].best = fold_host(&node_cands[0], &node_searched[0], cols.by_node());
+      }
     } else {
       nodes[0].best.o.gain = kMinScore;
     }
@@ -1620,6 +1795,16 @@ struct Learner {
         const int cl = nodes[(size_t)x].child[0], cr = nodes[(size_t)x].child[1];
         nodes[(size_t)cl].ls.leaf = (int)bl;
         nodes[(size_t)cr].ls.leaf = right;
+        // the reference searches these children, and draws their by-node samples
+        // (smaller child first), only when another split may follow
+        if (host_fold && nodes[(size_t)x].kids_searched && t.num_leaves < P.num_leaves) {
+          const bool left_small = info.o.exact_left < info.o.exact_right;
+          const int sm = left_small ? cl : cr, lg = left_small ? cr : cl;
+          const std::vector<char> ms = cols.by_node();
+          const std::vector<char> ml = cols.by_node();
+          nodes[(size_t)sm].best = fold_host(&node_cands[(size_t)sm * F], &node_searched[(size_t)sm * F], ms);
+          nodes[(size_t)lg].best = fold_host(&node_cands[(size_t)lg * F], &node_searched[(size_t)lg * F], ml);
+        }
         leaf_node[bl] = cl;
         leaf_node[(size_t)right] = cr;
         set_best((int)bl, nodes[(size_t)cl].best);
@@ -1687,13 +1872,14 @@ struct Learner {
           si.used_off = ns * F;
           si.out_off = ns * 2 * F;
           const char* xf = flags_of(x);
-          for (int f = 0; f < F; ++f) h_used[ns * F + f] = xf[f] ? 1 : 0;
+          for (int f = 0; f < F; ++f) h_used[ns * F + f] = xf[f] && cols.bytree[(size_t)f] ? kUseAll : 0;
           ++ns;
         }
         if (nodes.size() + 2 > max_nodes) throw std::runtime_error("node table too small");
         X.child[0] = (int)nodes.size();
         X.child[1] = (int)nodes.size() + 1;
         X.expanded = true;
+        X.kids_searched = search;
         nodes.push_back(cl);
         nodes.push_back(cr);
         ++ni;
@@ -1728,6 +1914,11 @@ struct Learner {
         Node& lg = nodes[(size_t)X.child[left_small ? 1 : 0]];
         sm.best = fold(2 * k, flags_of(X.child[left_small ? 0 : 1]));
         lg.best = fold(2 * k + 1, flags_of(X.child[left_small ? 1 : 0]));
+        if (host_fold) {
+          keep(X.child[left_small ? 0 : 1], k, 0);
+          keep(X.child[left_small ? 1 : 0], k, 1);
+          sm.best = lg.best = HostSplit();  // folded when X is committed
+        }
         ++k;
       }
     }
@@ -1979,6 +2170,7 @@ struct Learner {
       throw std::invalid_argument(quant ? "the quantized learner returns integer histograms"
                                         : "float learners return f64 histograms");
     before_train(g, h);
+    if (cols.per_tree) set_groups_used(true);  // every group, whatever the tree's sample
     hist_seg.assign(1, segs[0]);
     const HistPlan hp = plan_hist(1);
     h_ritems[0] = ReduceItem{0, hist_first[0], hist_count[0], 0};

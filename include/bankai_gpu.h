@@ -99,6 +99,9 @@ typedef struct bk_learner_params {
   int32_t num_threads;            /* fixes the rounding-noise block layout       */
   int32_t form;                   /* BK_FORM_*                                   */
   int32_t batched;                /* 1 = speculative depth-batched growth        */
+  double feature_fraction;        /* ColSampler (treelearner/col_sampler.rs): by-tree  */
+  double feature_fraction_bynode; /* and by-node fractions, each in (0, 1]             */
+  int32_t feature_fraction_seed;  /* seeds the sampler's LCG (Config::set derives it)  */
 } bk_learner_params;
 
 /*

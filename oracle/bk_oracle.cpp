@@ -17,9 +17,10 @@
 // reference of tests/quantized_split_search.rs, brute force over all thresholds,
 // and end-to-end the upstream golden models tests/fixtures/train/*/model.txt.
 //
-// Scope: dense single-value groups, numerical and categorical features, no
-// monotone constraints, no forced splits, no CEGB, no linear trees, no extra
-// trees, feature_fraction = 1. Those inputs are refused.
+// Scope: dense single-value groups, numerical and categorical features, bagged
+// row subsets, by-tree and by-node feature sampling; no monotone or interaction
+// constraints, no forced splits, no CEGB, no linear trees, no extra trees. Those
+// inputs are refused.
 
 #include <algorithm>
 #include <cmath>
@@ -64,6 +65,102 @@ inline int8_t sat_i8(double x) {
 // ---------------------------------------------------------------------------
 // std::mt19937 and libc++'s distributions (gradient_discretizer/mt19937.rs:48-128)
 // ---------------------------------------------------------------------------
+// The LCG of LightGBM's samplers (utils/random.rs:6-80).
+struct LcgRandom {
+  uint32_t x = 0;
+  explicit LcgRandom(int32_t seed = 0) : x((uint32_t)seed) {}
+  int32_t rand_int16() {
+    x = x * 214013u + 2531011u;
+    return (int32_t)((x >> 16) & 0x7FFFu);
+  }
+  int32_t rand_int32() {
+    x = x * 214013u + 2531011u;
+    return (int32_t)(x & 0x7FFFFFFFu);
+  }
+  int32_t next_int(int32_t lo, int32_t hi) { return rand_int32() % (hi - lo) + lo; }
+  float next_float() { return (float)rand_int16() / 32768.0f; }
+  // Random::sample: k sorted distinct indices of 0..n-1 (random.rs:52-79)
+  std::vector<int> sample(int n, int k) {
+    std::vector<int> ret;
+    if (k > n || k <= 0) return ret;
+    if (k == n) {
+      for (int i = 0; i < n; ++i) ret.push_back(i);
+      return ret;
+    }
+    if (k > 1 && (double)k > (double)n / std::log2((double)k)) {
+      for (int i = 0; i < n; ++i) {
+        const double prob = (double)(k - (int)ret.size()) / (double)(n - i);
+        if ((double)next_float() < prob) ret.push_back(i);
+      }
+    } else {
+      std::vector<int> set;
+      auto insert = [&](int v) {
+        auto it = std::lower_bound(set.begin(), set.end(), v);
+        if (it != set.end() && *it == v) return false;
+        set.insert(it, v);
+        return true;
+      };
+      for (int r = n - k; r < n; ++r)
+        if (!insert(next_int(0, r + 1))) insert(r);
+      ret = set;
+    }
+    return ret;
+  }
+};
+
+// ColSampler (treelearner/col_sampler.rs:14-176) without interaction constraints:
+// the by-tree sample is drawn at set-up and at every tree, the by-node samples at
+// every search, smaller leaf first.
+struct ColSampler {
+  double frac_bynode = 1.0;
+  LcgRandom rnd;
+  std::vector<int> valid_inner;  // inner index of each feature, in real-index order
+  int nf = 0;
+  bool need_reset_bytree = false;
+  int used_cnt_bytree = 0;
+  std::vector<char> bytree;
+  std::vector<int> used_idx;  // indices into valid_inner of the by-tree sample
+
+  static int get_cnt(int total, double fraction) {
+    const int mn = std::min(1, total);
+    return std::max(mn, (int)((double)total * fraction + 0.5));
+  }
+
+  void init(const std::vector<int>& valid, int num_features, double ff, double ffn, int32_t seed) {
+    valid_inner = valid;
+    nf = num_features;
+    frac_bynode = ffn;
+    rnd = LcgRandom(seed);
+    need_reset_bytree = ff < 1.0;
+    used_cnt_bytree = need_reset_bytree ? get_cnt((int)valid.size(), ff) : (int)valid.size();
+    bytree.assign((size_t)nf, 1);
+    reset_by_tree();
+  }
+
+  void reset_by_tree() {
+    if (!need_reset_bytree) return;
+    std::fill(bytree.begin(), bytree.end(), (char)0);
+    used_idx = rnd.sample((int)valid_inner.size(), used_cnt_bytree);
+    for (int i : used_idx) bytree[(size_t)valid_inner[(size_t)i]] = 1;
+  }
+
+  std::vector<char> by_node() {
+    std::vector<char> ret((size_t)nf, 0);
+    if (frac_bynode >= 1.0) {
+      std::fill(ret.begin(), ret.end(), (char)1);
+      return ret;
+    }
+    if (need_reset_bytree) {
+      const int cnt = get_cnt((int)used_idx.size(), frac_bynode);
+      for (int p : rnd.sample((int)used_idx.size(), cnt)) ret[(size_t)valid_inner[(size_t)used_idx[(size_t)p]]] = 1;
+    } else {
+      const int cnt = get_cnt((int)valid_inner.size(), frac_bynode);
+      for (int p : rnd.sample((int)valid_inner.size(), cnt)) ret[(size_t)valid_inner[(size_t)p]] = 1;
+    }
+    return ret;
+  }
+};
+
 struct Mt19937 {
   uint32_t st[624];
   int idx;
@@ -1205,6 +1302,7 @@ struct Learner {
   std::vector<float> og, oh;
   Discretizer* disc = nullptr;
   double grad_scale = 0.0, hess_scale = 0.0;
+  ColSampler cols;
   // per-tree stats
   int64_t stat_hist_rows = 0, stat_part_rows = 0;
   // The last tree's splits in the order they were made (serial.rs:2003 last_splits):
@@ -1397,6 +1495,7 @@ struct Learner {
   void find_best_splits(const Tree& t, const float* g, const float* h) {
     std::vector<char> used((size_t)nf, 0);
     for (int f = 0; f < nf; ++f) {
+      if (!cols.bytree[(size_t)f]) continue;  // outside the tree's sample: not touched
       if (parent_slot >= 0 && !pool.splittable[(size_t)parent_slot][(size_t)f]) {
         pool.splittable[smaller_slot][(size_t)f] = 0;
         continue;
@@ -1410,6 +1509,9 @@ struct Learner {
       if (quant) construct_int_hist(used, (size_t)larger.leaf_index, (size_t)larger_slot);
       else construct_hist(used, (size_t)larger.leaf_index, (size_t)larger_slot, g, h);
     }
+    // by-node samples, smaller leaf first (serial.rs:1329-1337)
+    const std::vector<char> smask = cols.by_node();
+    const std::vector<char> lmask = larger.leaf_index >= 0 ? cols.by_node() : std::vector<char>((size_t)nf, 0);
     double sp_out = parent_output(t, smaller);
     double lp_out = larger.leaf_index >= 0 ? parent_output(t, larger) : 0.0;
     long ls = larger.leaf_index >= 0 ? larger_slot : -1;
@@ -1455,10 +1557,10 @@ struct Learner {
     for (int f = 0; f < nf; ++f) {
       if (!used[(size_t)f]) continue;
       pool.splittable[smaller_slot][(size_t)f] = rs[(size_t)f].first;
-      if (rs[(size_t)f].second.better_than(sb)) sb = rs[(size_t)f].second;
+      if (smask[(size_t)f] && rs[(size_t)f].second.better_than(sb)) sb = rs[(size_t)f].second;
       if (ls >= 0) {
         pool.splittable[(size_t)ls][(size_t)f] = rl[(size_t)f].first;
-        if (rl[(size_t)f].second.better_than(lb)) lb = rl[(size_t)f].second;
+        if (lmask[(size_t)f] && rl[(size_t)f].second.better_than(lb)) lb = rl[(size_t)f].second;
       }
     }
     best[(size_t)smaller.leaf_index] = sb;
@@ -1565,6 +1667,7 @@ struct Learner {
     }
     for (auto& s : best) s.reset();
     splits.clear();
+    cols.reset_by_tree();
     stat_hist_rows = stat_part_rows = 0;
     const size_t cnt = leaf_cnt[0];
     double sg = 0.0, sh = 0.0;
@@ -1746,6 +1849,16 @@ int bko_learner_create(const bk_dataset_view* ds, const bk_learner_params* p, in
     L.pool.init(nl, std::max(L.total_bins, (size_t)1), (size_t)L.nf, cache, L.quant);
     L.best.assign(nl, SplitInfo());
     L.indices.resize(L.num_data);
+    if (!(p->feature_fraction > 0.0 && p->feature_fraction <= 1.0) ||
+        !(p->feature_fraction_bynode > 0.0 && p->feature_fraction_bynode <= 1.0))
+      throw std::invalid_argument("feature_fraction and feature_fraction_bynode must be in (0, 1]");
+    {
+      std::vector<int> valid((size_t)L.nf);
+      for (int f = 0; f < L.nf; ++f) valid[(size_t)f] = f;
+      std::stable_sort(valid.begin(), valid.end(),
+                       [&](int a, int b) { return L.feats[(size_t)a].real_index < L.feats[(size_t)b].real_index; });
+      L.cols.init(valid, L.nf, p->feature_fraction, p->feature_fraction_bynode, p->feature_fraction_seed);
+    }
     if (L.quant) {
       L.disc = new Discretizer(L.num_data, p->num_grad_quant_bins, L.const_hess);
       L.stochastic = p->stochastic_rounding != 0;

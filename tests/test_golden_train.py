@@ -56,6 +56,10 @@ def learner_params(p):
               "max_cat_to_onehot", "min_data_per_group"):
         if k in p:
             setattr(lp, k, p[k])
+    for k in ("feature_fraction", "feature_fraction_bynode"):
+        if k in p:
+            setattr(lp, k, p[k])
+    lp.feature_fraction_seed = bst.resolve_seed(p, "feature_fraction_seed")
     lp.num_threads = 1
     return lp
 

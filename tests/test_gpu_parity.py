@@ -518,3 +518,56 @@ def test_replay_earlier_trees_from_split_records(batched):
     gpu.add_prediction_to_score(last_vals, s)
     gpu.add_out_of_bag_prediction(last_vals, bst.out_of_bag(rows, n), s)
     assert np.array_equal(s, last_contrib)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("ff,ffn", [(0.5, 1.0), (1.0, 0.4), (0.7, 0.5)])
+def test_feature_sampling_quantized_bit_exact(binary_10k, ff, ffn, batched):
+    """By-tree and by-node feature samples (ColSampler): the quantized GPU learner
+    against the oracle, bit for bit. The batched form folds by-node samples on the
+    host when each parent split is committed, in the reference's draw order."""
+    announce(f"feature sampling ff={ff} ffn={ffn} batched={batched}")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=3, form=bg.FORM_QUANTIZED, batched=batched,
+                         feature_fraction=ff, feature_fraction_bynode=ffn, feature_fraction_seed=11)
+    trees = lockstep(ds, bst.BinaryObjective(y), p, iters=5)
+    used = set()
+    for tg, _ in trees:
+        used.update(tg["split_feature"].tolist())
+    assert len(used) > 1
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_feature_sampling_with_bagging_float(binary_10k, batched):
+    announce(f"feature sampling with bagging, plain f64, batched={batched}")
+    ds, y = binary_10k
+    sampler = bst.RowSampler(BAG_PARAMS["bagging"], y)
+    p = bg.LearnerParams(num_leaves=31, form=bg.FORM_PLAIN_F64, batched=batched, feature_fraction=0.6,
+                         feature_fraction_bynode=0.5, feature_fraction_seed=5)
+    lockstep(ds, bst.BinaryObjective(y), p, iters=4, compare=assert_trees_close, sampler=sampler)
+
+
+SAMPLED_CASES = ["bagging", "bagging_feature_fraction", "cat_bagging", "feature_fraction", "feature_fraction_both",
+                 "feature_fraction_bynode", "goss"]
+
+
+@pytest.mark.parametrize("case", SAMPLED_CASES)
+def test_golden_sampled_models_batched(case):
+    """The upstream models grown on row and feature samples, reproduced by the
+    batched form (speculative rounds, commits in the reference's order)."""
+    announce(f"golden {case} plain f64 batched on the GPU")
+    import test_golden_train as tg
+
+    def factory(ds, lp, ch):
+        lp.batched = 1
+        return bg.Learner(ds, lp, ch)
+    ds, gbdt, model, objective = tg.train_case(case, factory)
+    tg.compare_models(ds, gbdt, model, objective)
+
+
+def test_feature_fraction_out_of_range_refused(binary_10k):
+    ds, _ = binary_10k
+    for kw in ({"feature_fraction": 0.0}, {"feature_fraction_bynode": 1.5}):
+        with pytest.raises(bg.BankaiError, match="feature_fraction"):
+            bg.Learner(ds, bg.LearnerParams(**kw))
