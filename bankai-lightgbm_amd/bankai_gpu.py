@@ -141,7 +141,8 @@ class DatasetParamsC(C.Structure):
                 ("bin_construct_sample_cnt", C.c_int32), ("data_random_seed", C.c_int32),
                 ("use_missing", C.c_int32), ("zero_as_missing", C.c_int32),
                 ("feature_pre_filter", C.c_int32), ("min_data_in_leaf", C.c_int32),
-                ("num_threads", C.c_int32)]
+                ("num_threads", C.c_int32), ("enable_bundle", C.c_int32), ("is_enable_sparse", C.c_int32),
+                ("gpu_bundle_limit", C.c_int32)]
 
 
 # Every entry point the headers declare, with its argument types.
@@ -178,6 +179,8 @@ EXPORTS = {
     "bk_dataset_get_view": ([_P, C.POINTER(DatasetView)], C.c_int),
     "bk_dataset_feature_info": ([_P, C.c_int32, C.c_char_p, C.c_int32], C.c_int),
     "bk_dataset_free": ([_P], C.c_int),
+    "bk_dataset_bundle_layout": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, _P, _P, _P, _P,
+                                  C.POINTER(C.c_int32)], C.c_int),
 }
 
 _lib = None
@@ -220,9 +223,33 @@ class DatasetParams:
     feature_pre_filter: int = 1
     min_data_in_leaf: int = 20
     num_threads: int = 0
+    enable_bundle: int = 1
+    is_enable_sparse: int = 1
+    gpu_bundle_limit: int = 0
 
     def to_c(self):
         return DatasetParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})
+
+
+def bundle_layout(data, params=None, categorical=()):
+    """The group layout Dataset::Construct gives `data` (bk_dataset_bundle_layout):
+    per column its group and sub-feature index (-1 when unused), per group its bins
+    and whether it is multi-valued; also for layouts the dataset cannot store."""
+    lib = load_library()
+    x = np.ascontiguousarray(data, dtype=np.float64)
+    nrow, ncol = x.shape
+    p = (params or DatasetParams()).to_c()
+    cats = np.asarray(list(categorical), dtype=np.int32)
+    fg = np.zeros(ncol, np.int32)
+    fs = np.zeros(ncol, np.int32)
+    gb = np.zeros(ncol, np.int64)
+    gm = np.zeros(ncol, np.int8)
+    ng = C.c_int32()
+    _check(lib.bk_dataset_bundle_layout(_ptr(x), nrow, ncol, C.byref(p), _ptr(cats) if len(cats) else None, len(cats),
+                                        _ptr(fg), _ptr(fs), _ptr(gb), _ptr(gm), C.byref(ng)))
+    n = ng.value
+    return {"feature_group": fg, "feature_sub": fs, "group_num_total_bin": gb[:n].copy(),
+            "group_multi_val": gm[:n].astype(bool)}
 
 
 def derive_data_random_seed(seed):

@@ -8,10 +8,12 @@
 //   bin finding         crates/lgbm-core/src/io/bin.rs:50-553 (GreedyFindBin,
 //                       FindBinWithZeroAsOneBin, NeedFilter, BinMapper::find_bin)
 //   value_to_bin        crates/lgbm-core/src/io/bin.rs:558-589
-//   layout / groups     crates/lgbm-core/src/io/dataset.rs:19-76,276-447
+//   layout / groups     crates/lgbm-core/src/io/dataset.rs:19-76,365-447
+//   feature bundling    crates/lgbm-core/src/io/dataset.rs:95-363 (FixSampleIndices,
+//                       FindGroups, FastFeatureBundling)
 //   fill_dense          crates/lgbm-core/src/io/dataset.rs:633-689
-// Feature bundling (EFB) is not implemented: construction refuses inputs where
-// two features could share a group, which dense columns never do.
+// Every group is stored as one u8 column: bundles of more than 256 bins and
+// multi-value groups are refused (the reference's u16 and multi-value bins).
 
 #include <algorithm>
 #include <cmath>
@@ -19,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -392,8 +395,9 @@ struct bk_dataset {
   int32_t num_total_features = 0;
   std::vector<BinMapper> mappers;          // per real column
   std::vector<char> used;                  // per real column
-  // layout (one group per used feature, shuffled as upstream does)
-  std::vector<int32_t> group_real;         // real column of each group
+  // layout: the real columns of each group in sub-feature order (bundled and
+  // shuffled as upstream does); inner features are numbered group by group
+  std::vector<std::vector<int32_t>> group_feats;
   std::vector<std::vector<uint8_t>> cols;  // per group
   std::vector<const uint8_t*> col_ptrs;
   std::vector<int32_t> g_num_total_bin, g_feature_start, g_feature_cnt, g_bin_offsets;
@@ -406,70 +410,84 @@ struct bk_dataset {
 
 namespace {
 
+// Group columns (fill_dense, dataset.rs:670-689): a row's value of each
+// sub-feature, pushed in real-column order, that is not the most frequent bin is
+// stored at its sub-feature's offset; within a bundle a later column overwrites.
 void fill_columns(bk_dataset* ds, const double* rowmajor_f64, const float* rowmajor_f32, int64_t nrow, int32_t ncol,
                   int nthreads) {
-  size_t ng = ds->group_real.size();
+  size_t ng = ds->group_feats.size();
   ds->cols.assign(ng, std::vector<uint8_t>((size_t)nrow, 0));
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (long g = 0; g < (long)ng; ++g) {
-    int32_t real = ds->group_real[(size_t)g];
-    const BinMapper& m = ds->mappers[(size_t)real];
+    const std::vector<int32_t>& feats = ds->group_feats[(size_t)g];
+    const int32_t* offsets = ds->g_bin_offsets.data() + ds->g_feature_start[(size_t)g] + g;
+    std::vector<size_t> order(feats.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return feats[a] < feats[b]; });
     uint8_t* col = ds->cols[(size_t)g].data();
     for (int64_t r = 0; r < nrow; ++r) {
-      double v = rowmajor_f64 ? rowmajor_f64[r * ncol + real] : (double)rowmajor_f32[r * ncol + real];
-      uint32_t bin = m.value_to_bin(v);
-      if (bin == m.most_freq_bin) continue;
-      if (m.most_freq_bin == 0) bin -= 1;
-      col[r] = (uint8_t)(bin + 1u);
+      for (size_t sub : order) {
+        const int32_t real = feats[sub];
+        const BinMapper& m = ds->mappers[(size_t)real];
+        double v = rowmajor_f64 ? rowmajor_f64[r * ncol + real] : (double)rowmajor_f32[r * ncol + real];
+        uint32_t bin = m.value_to_bin(v);
+        if (bin == m.most_freq_bin) continue;
+        if (m.most_freq_bin == 0) bin -= 1;
+        col[r] = (uint8_t)(bin + (uint32_t)offsets[sub]);
+      }
     }
   }
   ds->col_ptrs.resize(ng);
   for (size_t g = 0; g < ng; ++g) ds->col_ptrs[g] = ds->cols[g].data();
 }
 
-void build_layout(bk_dataset* ds) {
-  // groups in used-feature order, then the upstream shuffle (dataset.rs:353-359)
-  std::vector<int32_t> used_features;
-  for (int32_t i = 0; i < ds->num_total_features; ++i)
-    if (ds->used[(size_t)i]) used_features.push_back(i);
-  std::vector<int32_t> groups = used_features;
-  Lcg rand((int32_t)ds->num_data);
-  int ng = (int)groups.size();
-  for (int i = 0; i < ng - 1; ++i) {
-    int j = rand.next_short(i + 1, ng);
-    std::swap(groups[(size_t)i], groups[(size_t)j]);
-  }
-  ds->group_real = groups;
-  size_t n = groups.size();
-  ds->g_num_total_bin.resize(n);
-  ds->g_feature_start.resize(n);
-  ds->g_feature_cnt.assign(n, 1);
+// Groups and inner features (Dataset::Construct, dataset.rs:378-447 with
+// FeatureGroupLayout::new, dataset.rs:35-76, single-value groups): bin 0 of a group
+// is shared by its sub-features' most frequent bins; each sub-feature's other bins
+// follow at its offset.
+void build_layout(bk_dataset* ds, std::vector<std::vector<int32_t>> groups) {
+  ds->group_feats = std::move(groups);
+  const size_t ng = ds->group_feats.size();
+  size_t nf = 0;
+  for (const auto& g : ds->group_feats) nf += g.size();
+  ds->g_num_total_bin.resize(ng);
+  ds->g_feature_start.resize(ng);
+  ds->g_feature_cnt.resize(ng);
   ds->g_bin_offsets.clear();
-  ds->f_num_bin.resize(n);
-  ds->f_missing.resize(n);
-  ds->f_default.resize(n);
-  ds->f_mfb.resize(n);
-  ds->f_type.resize(n);
-  ds->f_real.resize(n);
-  ds->f_penalty.assign(n, 1.0);
-  ds->f_upper.assign(n, nullptr);
-  ds->f_bin2cat.assign(n, nullptr);
-  for (size_t g = 0; g < n; ++g) {
-    const BinMapper& m = ds->mappers[(size_t)groups[g]];
-    int nb = m.num_bin - (m.most_freq_bin == 0 ? 1 : 0);
-    ds->g_num_total_bin[g] = 1 + nb;
-    ds->g_feature_start[g] = (int32_t)g;
-    ds->g_bin_offsets.push_back(1);
-    ds->g_bin_offsets.push_back(1 + nb);
-    ds->f_num_bin[g] = m.num_bin;
-    ds->f_missing[g] = m.missing_type;
-    ds->f_default[g] = (int32_t)m.default_bin;
-    ds->f_mfb[g] = (int32_t)m.most_freq_bin;
-    ds->f_type[g] = m.bin_type;
-    ds->f_real[g] = groups[g];
-    if (m.bin_type == BK_BIN_NUMERICAL) ds->f_upper[g] = m.upper.data();
-    else ds->f_bin2cat[g] = m.bin2cat.data();
-    if (ds->g_num_total_bin[g] > 256) throw std::invalid_argument("feature with more than 255 bins is not supported");
+  ds->f_num_bin.resize(nf);
+  ds->f_missing.resize(nf);
+  ds->f_default.resize(nf);
+  ds->f_mfb.resize(nf);
+  ds->f_type.resize(nf);
+  ds->f_real.resize(nf);
+  ds->f_penalty.assign(nf, 1.0);
+  ds->f_upper.assign(nf, nullptr);
+  ds->f_bin2cat.assign(nf, nullptr);
+  size_t f = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    ds->g_feature_start[g] = (int32_t)f;
+    ds->g_feature_cnt[g] = (int32_t)ds->group_feats[g].size();
+    int32_t total = 1;
+    ds->g_bin_offsets.push_back(total);
+    for (int32_t real : ds->group_feats[g]) {
+      const BinMapper& m = ds->mappers[(size_t)real];
+      total += m.num_bin - (m.most_freq_bin == 0 ? 1 : 0);
+      ds->g_bin_offsets.push_back(total);
+      ds->f_num_bin[f] = m.num_bin;
+      ds->f_missing[f] = m.missing_type;
+      ds->f_default[f] = (int32_t)m.default_bin;
+      ds->f_mfb[f] = (int32_t)m.most_freq_bin;
+      ds->f_type[f] = m.bin_type;
+      ds->f_real[f] = real;
+      if (m.bin_type == BK_BIN_NUMERICAL) ds->f_upper[f] = m.upper.data();
+      else ds->f_bin2cat[f] = m.bin2cat.data();
+      ++f;
+    }
+    ds->g_num_total_bin[g] = total;
+    if (total > 256)
+      throw std::invalid_argument(
+          "feature group with more than 256 bins (a u16 bundle) is not supported: set gpu_bundle_limit or disable "
+          "bundling");
   }
   ds->infos.assign((size_t)ds->num_total_features, "none");
   for (int32_t i = 0; i < ds->num_total_features; ++i) {
@@ -488,18 +506,203 @@ void build_layout(bk_dataset* ds) {
   }
 }
 
+// The sample a bundling decision reads: per column, the sample positions of its
+// non-zero values and the values (SampleSet, io/sample.rs).
+struct SampleView {
+  const std::vector<int32_t>* rows;
+  const std::vector<std::vector<double>>* values;
+  const std::vector<std::vector<int32_t>>* indices;
+  int32_t total;     // sampled rows
+  int32_t num_data;  // rows of the dataset (seeds the bundling LCG)
+};
+
+// FixSampleIndices (dataset.rs:116-143): when the most frequent bin is not the
+// zero bin, the positions whose bin differs from it (zeros included); empty
+// otherwise (the non-zero positions then stand).
+std::vector<int32_t> fix_sample_indices(const BinMapper& m, int32_t total, const std::vector<int32_t>& ix,
+                                        const std::vector<double>& vals) {
+  std::vector<int32_t> ret;
+  if (m.default_bin == m.most_freq_bin) return ret;
+  int32_t i = 0;
+  size_t j = 0;
+  while (i < total) {
+    if (j < ix.size() && ix[j] < i) {
+      ++j;
+    } else if (j < ix.size() && ix[j] == i) {
+      if (m.value_to_bin(vals[j]) != m.most_freq_bin) ret.push_back(i);
+      ++i;
+    } else {
+      ret.push_back(i);
+      ++i;
+    }
+  }
+  return ret;
+}
+
+// Conflicting positions of `ix` already marked, or -1 once above max_cnt.
+int32_t conflict_count(const std::vector<char>& mark, const std::vector<int32_t>& ix, int32_t max_cnt) {
+  int32_t ret = 0;
+  for (int32_t i : ix) {
+    if (mark[(size_t)i]) ++ret;
+    if (ret > max_cnt) return -1;
+  }
+  return ret;
+}
+
+// FindGroups (dataset.rs:146-279): greedy exclusive bundling in `order`. A feature
+// joins the first of (the newest group, then up to 99 others drawn at random) whose
+// marked sample positions it conflicts with at most the group's remaining budget
+// (total / 10000 overall) and at most half its own non-zero count; otherwise it
+// opens a group. With gpu_limit a group stays within 256 bins. With sparse
+// groups enabled, groups used on under 40% of the sample are regrouped into one
+// last group, multi-valued when its features conflict (always, with gpu_limit).
+void find_groups(const std::vector<BinMapper>& mappers, const std::vector<int32_t>& order,
+                 const std::vector<const std::vector<int32_t>*>& ix, const std::vector<int32_t>& num_per_col,
+                 int32_t total, int32_t num_data, bool gpu_limit, bool sparse,
+                 std::vector<std::vector<int32_t>>& groups_out, std::vector<char>& multi_out) {
+  const int32_t kMaxSearchGroup = 100, kMaxBinPerGroup = 256;
+  const int32_t max_conflict = total / 10000;
+  Lcg rand(num_data);
+  std::vector<std::vector<int32_t>> groups;
+  std::vector<std::vector<char>> marks;
+  std::vector<int32_t> used_rows, total_cnt, num_bins;
+  for (int32_t f : order) {
+    const BinMapper& m = mappers[(size_t)f];
+    const int32_t nz = num_per_col[(size_t)f];
+    const int32_t mfb_zero = m.most_freq_bin == 0 ? 1 : 0;
+    std::vector<int32_t> avail;
+    for (size_t g = 0; g < groups.size(); ++g) {
+      const int32_t cur_bins = num_bins[g] + m.num_bin - mfb_zero;
+      if (total_cnt[g] + nz <= total + max_conflict && (!gpu_limit || cur_bins <= kMaxBinPerGroup))
+        avail.push_back((int32_t)g);
+    }
+    std::vector<int32_t> search;
+    if (!avail.empty()) {
+      const int32_t last = (int32_t)avail.size() - 1;
+      search.push_back(avail.back());
+      for (int32_t i : rand.sample(last, std::min(last, kMaxSearchGroup - 1))) search.push_back(avail[(size_t)i]);
+    }
+    int32_t best = -1, best_cnt = -1;
+    for (int32_t g : search) {
+      const int32_t rest = max_conflict - total_cnt[(size_t)g] + used_rows[(size_t)g];
+      const int32_t c = conflict_count(marks[(size_t)g], *ix[(size_t)f], rest);
+      if (c >= 0 && c <= rest && c <= nz / 2) {
+        best = g;
+        best_cnt = c;
+        break;
+      }
+    }
+    if (best >= 0) {
+      groups[(size_t)best].push_back(f);
+      total_cnt[(size_t)best] += nz;
+      used_rows[(size_t)best] += nz - best_cnt;
+      for (int32_t i : *ix[(size_t)f]) marks[(size_t)best][(size_t)i] = 1;
+      // the reference adds here with the default bin, elsewhere with the most frequent one
+      num_bins[(size_t)best] += m.num_bin - (m.default_bin == 0 ? 1 : 0);
+    } else {
+      groups.push_back({f});
+      marks.emplace_back((size_t)total, (char)0);
+      for (int32_t i : *ix[(size_t)f]) marks.back()[(size_t)i] = 1;
+      total_cnt.push_back(nz);
+      used_rows.push_back(nz);
+      num_bins.push_back(1 + m.num_bin - mfb_zero);
+    }
+  }
+  groups_out.clear();
+  multi_out.clear();
+  if (!sparse) {
+    groups_out = std::move(groups);
+    multi_out.assign(groups_out.size(), 0);
+    return;
+  }
+  std::vector<int32_t> second;
+  for (size_t g = 0; g < groups.size(); ++g) {
+    if ((double)used_rows[g] / (double)total >= 0.4) {
+      groups_out.push_back(groups[g]);
+      multi_out.push_back(0);
+    } else {
+      second.insert(second.end(), groups[g].begin(), groups[g].end());
+    }
+  }
+  if (second.empty()) return;
+  std::vector<char> mark((size_t)total, 0);
+  bool is_multi = gpu_limit;
+  int32_t conflicts = 0;
+  std::vector<int32_t> group;
+  for (int32_t f : second) {
+    group.push_back(f);
+    if (is_multi) continue;
+    const int32_t c = conflict_count(mark, *ix[(size_t)f], max_conflict - conflicts);
+    conflicts += c;
+    if (c < 0 || conflicts > max_conflict) {
+      is_multi = true;
+      continue;
+    }
+    for (int32_t i : *ix[(size_t)f]) mark[(size_t)i] = 1;
+  }
+  groups_out.push_back(group);
+  multi_out.push_back(is_multi ? 1 : 0);
+}
+
+// FastFeatureBundling (dataset.rs:282-363): bundles in column order and in
+// decreasing non-zero count, keeps the fewer groups (the first on a tie), then
+// shuffles the groups with the LCG seeded by num_data.
+void fast_feature_bundling(const std::vector<BinMapper>& mappers, const SampleView& sv,
+                           const std::vector<int32_t>& used, bool gpu_limit, bool sparse,
+                           std::vector<std::vector<int32_t>>& groups, std::vector<char>& multi) {
+  const size_t ncol = sv.indices->size();
+  std::vector<size_t> nz(used.size());
+  for (size_t i = 0; i < used.size(); ++i) nz[i] = (*sv.indices)[(size_t)used[i]].size();
+  std::vector<size_t> by_cnt(used.size());
+  for (size_t i = 0; i < by_cnt.size(); ++i) by_cnt[i] = i;
+  std::stable_sort(by_cnt.begin(), by_cnt.end(), [&](size_t a, size_t b) { return nz[a] > nz[b]; });
+  std::vector<int32_t> order_by_cnt;
+  for (size_t i : by_cnt) order_by_cnt.push_back(used[i]);
+  std::vector<std::vector<int32_t>> fixed(ncol);
+  std::vector<int32_t> num_per_col(ncol, 0);
+  std::vector<const std::vector<int32_t>*> ix(ncol);
+  for (size_t f = 0; f < ncol; ++f) ix[f] = &(*sv.indices)[f];
+  for (int32_t f : used) {
+    fixed[(size_t)f] = fix_sample_indices(mappers[(size_t)f], sv.total, (*sv.indices)[(size_t)f], (*sv.values)[(size_t)f]);
+    if (fixed[(size_t)f].empty()) {
+      num_per_col[(size_t)f] = (int32_t)(*sv.indices)[(size_t)f].size();
+    } else {
+      num_per_col[(size_t)f] = (int32_t)fixed[(size_t)f].size();
+      ix[(size_t)f] = &fixed[(size_t)f];
+    }
+  }
+  std::vector<std::vector<int32_t>> g2;
+  std::vector<char> m2;
+  find_groups(mappers, used, ix, num_per_col, sv.total, sv.num_data, gpu_limit, sparse, groups, multi);
+  find_groups(mappers, order_by_cnt, ix, num_per_col, sv.total, sv.num_data, gpu_limit, sparse, g2, m2);
+  if (groups.size() > g2.size()) {
+    groups = std::move(g2);
+    multi = std::move(m2);
+  }
+  Lcg rand(sv.num_data);
+  const int32_t ng = (int32_t)groups.size();
+  for (int32_t i = 0; i < ng - 1; ++i) {
+    const int32_t j = rand.next_short(i + 1, ng);
+    std::swap(groups[(size_t)i], groups[(size_t)j]);
+    std::swap(multi[(size_t)i], multi[(size_t)j]);
+  }
+}
+
+// Sampling, bin finding and bundling: the mappers of every column and the groups
+// (real columns per group, sub-feature order) with their multi-value flags.
+struct Plan {
+  std::vector<BinMapper> mappers;
+  std::vector<char> used;
+  std::vector<std::vector<int32_t>> groups;
+  std::vector<char> multi;
+};
+
 template <typename T>
-bk_dataset* construct(const T* data, int64_t nrow, int32_t ncol, const bk_dataset_params* p, const int32_t* cat,
-                      int32_t ncat) {
+Plan plan_dataset(const T* data, int64_t nrow, int32_t ncol, const bk_dataset_params* p, const int32_t* cat,
+                  int32_t ncat, int nthreads) {
   if (nrow <= 0 || ncol <= 0) throw std::invalid_argument("empty matrix");
   if (nrow > 2147483647LL) throw std::invalid_argument("too many rows");
-  int nthreads = p->num_threads > 0 ? p->num_threads : 1;
-#ifdef _OPENMP
-  if (p->num_threads <= 0) nthreads = omp_get_max_threads();
-#endif
-  auto* ds = new bk_dataset();
-  ds->num_data = nrow;
-  ds->num_total_features = ncol;
+  Plan pl;
   // sample rows (sample.rs:15-18) and gather non-zero values (sample.rs:79-98)
   int32_t sample_cnt = std::min((int32_t)nrow, p->bin_construct_sample_cnt);
   std::vector<int32_t> rows = Lcg(p->data_random_seed).sample((int32_t)nrow, sample_cnt);
@@ -517,48 +720,74 @@ bk_dataset* construct(const T* data, int64_t nrow, int32_t ncol, const bk_datase
   }
   std::set<int32_t> cats(cat, cat + ncat);
   int filter_cnt = (int)(((double)p->min_data_in_leaf * (double)rows.size()) / (double)nrow);
-  ds->mappers.resize((size_t)ncol);
-  std::vector<size_t> nonzero((size_t)ncol);
-  for (int32_t k = 0; k < ncol; ++k) nonzero[(size_t)k] = values[(size_t)k].size();
+  pl.mappers.resize((size_t)ncol);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int32_t k = 0; k < ncol; ++k) {
     FindBinParams fp{rows.size(), p->max_bin, p->min_data_in_bin, filter_cnt, p->feature_pre_filter != 0,
                      cats.count(k) ? BK_BIN_CATEGORICAL : BK_BIN_NUMERICAL, p->use_missing != 0,
                      p->zero_as_missing != 0};
-    ds->mappers[(size_t)k] = find_bin(values[(size_t)k], fp);
+    pl.mappers[(size_t)k] = find_bin(values[(size_t)k], fp);
   }
-  ds->used.assign((size_t)ncol, 0);
-  for (int32_t k = 0; k < ncol; ++k) ds->used[(size_t)k] = !ds->mappers[(size_t)k].is_trivial;
-  // Bundling precondition (dataset.rs:164-172): with every used column's non-zero
-  // sample count above half the sample, no two columns can share a group.
-  size_t total = rows.size();
-  size_t max_conflict = total / 10000;
-  std::vector<size_t> cnts;
+  pl.used.assign((size_t)ncol, 0);
+  std::vector<int32_t> used_features;
   for (int32_t k = 0; k < ncol; ++k) {
-    if (!ds->used[(size_t)k]) continue;
-    const BinMapper& m = ds->mappers[(size_t)k];
-    size_t c = nonzero[(size_t)k];
-    if (m.default_bin != m.most_freq_bin) {
-      // FixSampleIndices: rows whose bin is not the most frequent one
-      c = 0;
-      const std::vector<int32_t>& ix = sidx[(size_t)k];
-      size_t j = 0;
-      std::vector<double> vals;
-      for (size_t i = 0; i < total; ++i) {
-        double v = 0.0;
-        if (j < ix.size() && (size_t)ix[j] == i) v = (double)data[(int64_t)rows[i] * ncol + k], ++j;
-        if (m.value_to_bin(v) != m.most_freq_bin) ++c;
-      }
-    }
-    cnts.push_back(c);
+    pl.used[(size_t)k] = !pl.mappers[(size_t)k].is_trivial;
+    if (pl.used[(size_t)k]) used_features.push_back(k);
   }
-  std::sort(cnts.begin(), cnts.end());
-  if (cnts.size() >= 2 && cnts[0] + cnts[1] <= total + max_conflict)
-    throw std::invalid_argument("columns sparse enough to bundle: feature bundling is not implemented");
-  build_layout(ds);
-  if (std::is_same<T, double>::value) fill_columns(ds, (const double*)data, nullptr, nrow, ncol, nthreads);
-  else fill_columns(ds, nullptr, (const float*)data, nrow, ncol, nthreads);
-  return ds;
+  if (p->enable_bundle && !used_features.empty()) {
+    SampleView sv{&rows, &values, &sidx, (int32_t)rows.size(), (int32_t)nrow};
+    fast_feature_bundling(pl.mappers, sv, used_features, p->gpu_bundle_limit != 0, p->is_enable_sparse != 0,
+                          pl.groups, pl.multi);
+  } else {
+    for (int32_t k : used_features) pl.groups.push_back({k});
+    pl.multi.assign(pl.groups.size(), 0);
+  }
+  return pl;
+}
+
+// Bins of a group as FeatureGroupLayout::new counts them (dataset.rs:35-76),
+// multi-value groups included: bin 0 is shared unless the group is a dense
+// multi-value one (mean sparse rate under 0.25), which keeps no shared bin, or
+// only one when it is group 0 and its first feature's most frequent bin is not 0.
+int64_t group_num_total_bin(const Plan& pl, size_t g) {
+  const std::vector<int32_t>& feats = pl.groups[g];
+  double sparse = 0.0;
+  for (int32_t f : feats) sparse += pl.mappers[(size_t)f].sparse_rate;
+  sparse /= (double)feats.size();
+  int64_t offset = 1;
+  bool dense_multi = false;
+  if (sparse < (double)0.25f && pl.multi[g]) {
+    offset = 0;
+    dense_multi = true;
+  }
+  int64_t total = offset;
+  if (g == 0 && !feats.empty() && dense_multi && pl.mappers[(size_t)feats[0]].most_freq_bin > 0) total = 1;
+  for (int32_t f : feats) {
+    const BinMapper& m = pl.mappers[(size_t)f];
+    total += m.num_bin - (m.most_freq_bin == 0 ? offset : 0);
+  }
+  return total;
+}
+
+template <typename T>
+bk_dataset* construct(const T* data, int64_t nrow, int32_t ncol, const bk_dataset_params* p, const int32_t* cat,
+                      int32_t ncat) {
+  int nthreads = p->num_threads > 0 ? p->num_threads : 1;
+#ifdef _OPENMP
+  if (p->num_threads <= 0) nthreads = omp_get_max_threads();
+#endif
+  Plan pl = plan_dataset(data, nrow, ncol, p, cat, ncat, nthreads);
+  for (char m : pl.multi)
+    if (m) throw std::invalid_argument("multi-value feature groups (sparse bundles with conflicts) are not supported");
+  auto ds = std::make_unique<bk_dataset>();
+  ds->num_data = nrow;
+  ds->num_total_features = ncol;
+  ds->mappers = std::move(pl.mappers);
+  ds->used = std::move(pl.used);
+  build_layout(ds.get(), std::move(pl.groups));
+  if (std::is_same<T, double>::value) fill_columns(ds.get(), (const double*)data, nullptr, nrow, ncol, nthreads);
+  else fill_columns(ds.get(), nullptr, (const float*)data, nrow, ncol, nthreads);
+  return ds.release();
 }
 
 }  // namespace
@@ -583,10 +812,10 @@ int bk_dataset_from_dense_f32_reference(const float* data, int64_t nrow, int32_t
     ds->num_data = nrow;
     fill_columns(ds, nullptr, data, nrow, ncol, 1);
     // re-point metadata into the copy
-    for (size_t g = 0; g < ds->group_real.size(); ++g) {
-      const BinMapper& m = ds->mappers[(size_t)ds->group_real[g]];
-      ds->f_upper[g] = m.bin_type == BK_BIN_NUMERICAL ? m.upper.data() : nullptr;
-      ds->f_bin2cat[g] = m.bin_type == BK_BIN_CATEGORICAL ? m.bin2cat.data() : nullptr;
+    for (size_t f = 0; f < ds->f_real.size(); ++f) {
+      const BinMapper& m = ds->mappers[(size_t)ds->f_real[f]];
+      ds->f_upper[f] = m.bin_type == BK_BIN_NUMERICAL ? m.upper.data() : nullptr;
+      ds->f_bin2cat[f] = m.bin_type == BK_BIN_CATEGORICAL ? m.bin2cat.data() : nullptr;
     }
     *out = ds;
   });
@@ -596,8 +825,8 @@ int bk_dataset_get_view(const bk_dataset* ds, bk_dataset_view* v) {
   return bk::guard([&] {
     std::memset(v, 0, sizeof *v);
     v->num_data = ds->num_data;
-    v->num_groups = (int32_t)ds->group_real.size();
-    v->num_features = (int32_t)ds->group_real.size();
+    v->num_groups = (int32_t)ds->group_feats.size();
+    v->num_features = (int32_t)ds->f_real.size();
     v->num_total_features = ds->num_total_features;
     v->group_bins = ds->col_ptrs.data();
     v->group_num_total_bin = ds->g_num_total_bin.data();
@@ -622,6 +851,26 @@ int bk_dataset_feature_info(const bk_dataset* ds, int32_t col, char* buf, int32_
     const std::string& s = ds->infos[(size_t)col];
     if ((int32_t)s.size() + 1 > cap) throw std::invalid_argument("buffer too small");
     std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+int bk_dataset_bundle_layout(const double* data, int64_t nrow, int32_t ncol, const bk_dataset_params* p,
+                             const int32_t* categorical, int32_t num_categorical, int32_t* feature_group,
+                             int32_t* feature_sub, int64_t* group_num_total_bin_out, int8_t* group_multi_val,
+                             int32_t* num_groups) {
+  return bk::guard([&] {
+    int nthreads = p->num_threads > 0 ? p->num_threads : 1;
+    Plan pl = plan_dataset(data, nrow, ncol, p, categorical, num_categorical, nthreads);
+    for (int32_t k = 0; k < ncol; ++k) feature_group[k] = feature_sub[k] = -1;
+    for (size_t g = 0; g < pl.groups.size(); ++g) {
+      for (size_t j = 0; j < pl.groups[g].size(); ++j) {
+        feature_group[pl.groups[g][j]] = (int32_t)g;
+        feature_sub[pl.groups[g][j]] = (int32_t)j;
+      }
+      group_num_total_bin_out[g] = group_num_total_bin(pl, g);
+      group_multi_val[g] = (int8_t)pl.multi[g];
+    }
+    *num_groups = (int32_t)pl.groups.size();
   });
 }
 

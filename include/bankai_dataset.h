@@ -29,6 +29,10 @@ typedef struct bk_dataset_params {
   int32_t feature_pre_filter;       /* default 1       */
   int32_t min_data_in_leaf;         /* default 20 (pre-filter count) */
   int32_t num_threads;              /* 0 = all cores   */
+  int32_t enable_bundle;            /* default 1: exclusive feature bundling (EFB) */
+  int32_t is_enable_sparse;         /* default 1: regroup sparse groups (FindGroups) */
+  int32_t gpu_bundle_limit;         /* default 0; 1 = the bundling of device_type "gpu": groups of
+                                       at most 256 bins (dataset.rs:159-181,259) */
 } bk_dataset_params;
 
 typedef struct bk_dataset bk_dataset;
@@ -47,6 +51,17 @@ int bk_dataset_from_dense_f32_reference(const float* data, int64_t nrow, int32_t
 int bk_dataset_get_view(const bk_dataset* ds, bk_dataset_view* view);
 /* Model-text `feature_infos` entry of real column `col` ("none" when unused). */
 int bk_dataset_feature_info(const bk_dataset* ds, int32_t col, char* buf, int32_t cap);
+/* The layout Dataset::Construct would give these rows (io/dataset.rs:378-447 with
+ * FastFeatureBundling, dataset.rs:282-363), without building the bins: for every
+ * column its group and sub-feature index (-1: unused), and per group (capacity ncol)
+ * its number of bins as FeatureGroupLayout::new counts them and whether it is
+ * multi-valued. Groups this library cannot store (more than 256 bins, multi-valued)
+ * are reported too; bk_dataset_from_dense refuses them. */
+int bk_dataset_bundle_layout(const double* data, int64_t nrow, int32_t ncol, const bk_dataset_params* params,
+                             const int32_t* categorical, int32_t num_categorical, int32_t* feature_group,
+                             int32_t* feature_sub, int64_t* group_num_total_bin, int8_t* group_multi_val,
+                             int32_t* num_groups);
+
 int bk_dataset_free(bk_dataset* ds);
 
 #ifdef __cplusplus

@@ -1,0 +1,241 @@
+"""Dataset layout (f1, and the bundling part of f4) against upstream binary datasets.
+
+Each case under tests/golden/datasets/ is a copy of the reference's fixture
+tests/fixtures/datasets/<case>/ (x.npy, params.json and dataset.bin, the
+`Dataset.save_binary` of lightgbm 4.7.0, written by scripts/gen_dataset_fixtures.py).
+The header of dataset.bin (io/binary.rs:219-287 reads the same fields) fixes which
+columns are used, how they are bundled into groups (exclusive feature bundling,
+io/dataset.rs:95-363), the order of the groups after the shuffle, every group's bin
+boundaries and sub-feature order, and whether a group is multi-valued. The layout
+built by bk_dataset_from_dense must match it field for field, or, when the upstream
+layout holds a group this build cannot store (more than 256 bins, or multi-valued),
+construction must refuse with a message naming it.
+"""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import bankai_gpu as bg
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "datasets")
+CASES = sorted(os.listdir(GOLDEN)) if os.path.isdir(GOLDEN) else []
+TOKEN = b"______LightGBM_Binary_File_Token______\n"
+
+
+class _Reader:
+    """Little-endian reader of the 8-byte aligned layout of Dataset::SaveBinaryFile."""
+
+    def __init__(self, buf):
+        self.b, self.p = buf, 0
+
+    def take(self, n):
+        out = self.b[self.p:self.p + n]
+        assert len(out) == n, "truncated"
+        self.p += n
+        return out
+
+    def u64(self):
+        return struct.unpack("<Q", self.take(8))[0]
+
+    def ai32(self):  # i32 padded to 8 bytes
+        v = struct.unpack("<i", self.take(4))[0]
+        self.take(4)
+        return v
+
+    def abool(self):
+        return self.take(8)[0] != 0
+
+    def ai32s(self, n):
+        v = list(struct.unpack(f"<{n}i", self.take(4 * n))) if n else []
+        self.take((-4 * n) % 8)
+        return v
+
+    def abytes(self, n):
+        v = self.take(n)
+        self.take((-n) % 8)
+        return v
+
+
+def read_layout(path):
+    r = _Reader(open(path, "rb").read())
+    assert r.abytes(len(TOKEN)) == TOKEN
+    size = r.u64()
+    start = r.p
+    h = {}
+    for k in ("num_data", "num_features", "num_total_features", "label_idx", "max_bin", "bin_construct_sample_cnt",
+              "min_data_in_bin"):
+        h[k] = r.ai32()
+    for k in ("use_missing", "zero_as_missing", "has_raw"):
+        h[k] = r.abool()
+    ntf, nf = h["num_total_features"], h["num_features"]
+    h["used_feature_map"] = r.ai32s(ntf)
+    ng = h["num_groups"] = r.ai32()
+    h["real_feature_idx"] = r.ai32s(nf)
+    h["feature2group"] = r.ai32s(nf)
+    h["feature2subfeature"] = r.ai32s(nf)
+    h["group_bin_boundaries"] = [r.u64() for _ in range(ng + 1)]
+    h["group_feature_start"] = r.ai32s(ng)
+    h["group_feature_cnt"] = r.ai32s(ng)
+    r.ai32s(ntf)  # max_bin_by_feature
+    for _ in range(ntf):
+        r.abytes(r.ai32())  # feature names
+    for _ in range(ntf):
+        r.take(8 * r.ai32())  # forced bin bounds
+    assert r.p - start == size, "header size"
+    r.take(r.u64())  # metadata
+    multi = []
+    for _ in range(ng):
+        gsize = r.u64()
+        g0 = r.p
+        multi.append(r.abool())
+        r.take(gsize - (r.p - g0))
+    h["is_multi_val"] = multi
+    return h
+
+
+def dataset_params(spec):
+    p = spec["params"]
+    dp = bg.DatasetParams(num_threads=1)
+    for k in ("max_bin", "min_data_in_bin", "use_missing", "zero_as_missing", "enable_bundle", "is_enable_sparse",
+              "feature_pre_filter"):
+        if k in p:
+            setattr(dp, k, int(p[k]))
+    return dp
+
+
+def view_layout(ds):
+    v = ds.view
+    ng, nf = v.num_groups, v.num_features
+    start = [v.group_feature_start[g] for g in range(ng)]
+    cnt = [v.group_feature_cnt[g] for g in range(ng)]
+    bounds = [0]
+    for g in range(ng):
+        bounds.append(bounds[-1] + v.group_num_total_bin[g])
+    real = [v.feature_real_index[f] for f in range(nf)]
+    f2g, f2s = [], []
+    for g in range(ng):
+        for s_ in range(cnt[g]):
+            f2g.append(g)
+            f2s.append(s_)
+    used = [-1] * v.num_total_features
+    for f, rf in enumerate(real):
+        used[rf] = f
+    return {"num_features": nf, "num_groups": ng, "group_feature_start": start, "group_feature_cnt": cnt,
+            "group_bin_boundaries": bounds, "real_feature_idx": real, "feature2group": f2g,
+            "feature2subfeature": f2s, "used_feature_map": used}
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_bundling_plan_matches_upstream(case):
+    """Every grouping decision (used columns, bundles, sub-feature order, group order
+    after the shuffle, bins per group, multi-value flags), also for layouts the
+    dataset cannot store."""
+    d = os.path.join(GOLDEN, case)
+    spec = json.load(open(os.path.join(d, "params.json")))
+    want = read_layout(os.path.join(d, "dataset.bin"))
+    x = np.load(os.path.join(d, "x.npy"))
+    got = bg.bundle_layout(x, dataset_params(spec), spec["categorical"])
+    ng = want["num_groups"]
+    assert len(got["group_num_total_bin"]) == ng, case
+    want_group = [-1] * want["num_total_features"]
+    want_sub = [-1] * want["num_total_features"]
+    for f, real in enumerate(want["real_feature_idx"]):
+        want_group[real] = want["feature2group"][f]
+        want_sub[real] = want["feature2subfeature"][f]
+    assert got["feature_group"].tolist() == want_group, case
+    assert got["feature_sub"].tolist() == want_sub, case
+    assert got["group_num_total_bin"].tolist() == np.diff(want["group_bin_boundaries"]).tolist(), case
+    assert got["group_multi_val"].tolist() == want["is_multi_val"], case
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_layout_matches_upstream_binary_dataset(case):
+    d = os.path.join(GOLDEN, case)
+    spec = json.load(open(os.path.join(d, "params.json")))
+    want = read_layout(os.path.join(d, "dataset.bin"))
+    x = np.load(os.path.join(d, "x.npy"))
+    sizes = np.diff(want["group_bin_boundaries"])
+    storable = not any(want["is_multi_val"]) and all(s <= 256 for s in sizes)
+    if not storable:
+        with pytest.raises(bg.BankaiError, match="256 bins|multi-value"):
+            bg.Dataset(x, dataset_params(spec), categorical=spec["categorical"])
+        return
+    ds = bg.Dataset(x, dataset_params(spec), categorical=spec["categorical"])
+    got = view_layout(ds)
+    for k, v in got.items():
+        assert v == want[k], (case, k, v[:12] if isinstance(v, list) else v,
+                              want[k][:12] if isinstance(want[k], list) else want[k])
+
+
+def test_bundles_exist_among_the_pinned_cases():
+    """The plan pins include real bundles (a group of 40 one-hot columns, a
+    multi-value group of 29 sparse columns)."""
+    bundled = [c for c in CASES if max(read_layout(os.path.join(GOLDEN, c, "dataset.bin"))["group_feature_cnt"]) > 1]
+    assert len(bundled) >= 3, bundled
+
+
+def test_gpu_bundle_limit_keeps_groups_storable():
+    """gpu_bundle_limit (the reference's device_type "gpu" bundling): groups of at most
+    256 bins, every used feature in exactly one group, and bundled columns exclusive
+    on the rows up to the conflict budget. Parity unpinned beyond these invariants: no
+    fixture was written with device_type "gpu"."""
+    rng = np.random.default_rng(3)
+    n, d = 4000, 30
+    x = np.zeros((n, d))
+    owner = rng.integers(0, d, size=n)
+    x[np.arange(n), owner] = rng.normal(size=n) + 3
+    x[:, 0] = rng.normal(size=n)
+    dp = bg.DatasetParams(num_threads=1, gpu_bundle_limit=1, is_enable_sparse=0)
+    ds = bg.Dataset(x, dp)
+    lay = view_layout(ds)
+    sizes = np.diff(lay["group_bin_boundaries"])
+    assert sizes.max() <= 256
+    assert sorted(lay["real_feature_idx"]) == sorted(set(lay["real_feature_idx"]))
+    assert max(lay["group_feature_cnt"]) > 1
+
+
+def one_hot_frame(n=6000, k=40, seed=5):
+    """k mutually exclusive 0/1 indicator columns (one bundle of k + 1 bins) after
+    four dense columns; the label depends on both."""
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 4 + k))
+    x[:, :4] = rng.normal(size=(n, 4))
+    owner = rng.integers(0, k, size=n)
+    x[np.arange(n), 4 + owner] = 1.0
+    effect = rng.normal(size=k)
+    y = x[:, 0] + 0.5 * x[:, 1] ** 2 + effect[owner] + 0.1 * rng.normal(size=n)
+    return x, y
+
+
+def test_bundled_dataset_layout():
+    x, _ = one_hot_frame()
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=1))
+    lay = view_layout(ds)
+    g = int(np.argmax(lay["group_feature_cnt"]))
+    assert lay["group_feature_cnt"][g] == 40
+    assert np.diff(lay["group_bin_boundaries"])[g] == 41
+
+
+@pytest.mark.parametrize("quantized", [0, 1])
+def test_oracle_trees_identical_bundled_or_not(quantized):
+    """Exclusive columns: the bundle's per-feature histograms (most frequent bin
+    recovered by fix_histogram) equal the unbundled ones, so the oracle grows the
+    same trees on the bundled and the unbundled layout (real feature ids,
+    thresholds, counts and values)."""
+    import boosting as bst
+    import oracle_binding as ob
+    x, y = one_hot_frame()
+    lp = bg.LearnerParams(num_leaves=31, min_data_in_leaf=10)
+    if quantized:
+        lp.use_quantized_grad, lp.num_grad_quant_bins, lp.seed, lp.form = 1, 4, 7, bg.FORM_QUANTIZED
+    trees = []
+    for bundle in (1, 0):
+        ds = bg.Dataset(x, bg.DatasetParams(num_threads=1, enable_bundle=bundle))
+        obj = bst.RegressionL2Objective(y)
+        gbdt = bst.GBDT([ob.OracleLearner(ds, lp, obj.const_hess)], obj, ds.num_data)
+        gbdt.train(8)
+        trees.append([t.to_model_string() for t in gbdt.models])
+    assert trees[0] == trees[1]

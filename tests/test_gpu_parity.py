@@ -571,3 +571,52 @@ def test_feature_fraction_out_of_range_refused(binary_10k):
     for kw in ({"feature_fraction": 0.0}, {"feature_fraction_bynode": 1.5}):
         with pytest.raises(bg.BankaiError, match="feature_fraction"):
             bg.Learner(ds, bg.LearnerParams(**kw))
+
+
+def sparse_frame(n, d=30, seed=6):
+    """d - 1 sparse columns of about 60 distinct values, each row non-zero in about
+    one of them (0.02% extra conflicts, within the bundles' budget), after one dense
+    column: under the 256-bin limit they bundle three or four to a group."""
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, d))
+    owner = rng.integers(0, d, size=n)
+    x[np.arange(n), owner] = np.round(rng.normal(size=n) + 3, 1)
+    x[:, 0] = rng.normal(size=n)
+    x[rng.random((n, d)) < 0.0002] = -1.5
+    return x, x[:, 0] + x[:, 1] + 0.5 * x[:, 2]
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("kind", ["one_hot", "sparse_gpu_limit"])
+def test_bundled_groups_quantized_bit_exact(kind, batched):
+    """Feature groups of several bundled columns (exclusive feature bundling): the
+    GPU learner's histograms, fix of each sub-feature's most frequent bin, search
+    offsets and routing of bundled bins against the oracle, bit for bit."""
+    announce(f"bundled groups {kind} quantized batched={batched}")
+    import test_dataset_layout as tdl
+    if kind == "one_hot":
+        x, y = tdl.one_hot_frame(n=50_000)
+        dp = bg.DatasetParams(num_threads=8)
+    else:
+        x, y = sparse_frame(50_000)
+        dp = bg.DatasetParams(num_threads=8, gpu_bundle_limit=1, is_enable_sparse=0)
+    ds = bg.Dataset(x, dp)
+    assert max(ds.view.group_feature_cnt[g] for g in range(ds.view.num_groups)) > 1
+    p = bg.LearnerParams(num_leaves=63, min_data_in_leaf=10, use_quantized_grad=1, num_grad_quant_bins=8,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=5, form=bg.FORM_QUANTIZED,
+                         batched=batched)
+    trees = lockstep(ds, bst.RegressionL2Objective(y), p, iters=5)
+    split_reals = set()
+    for tg, _ in trees:
+        split_reals.update(tg["split_feature"].tolist())
+    assert split_reals & set(range(4, x.shape[1])), "no split on a bundled column"
+
+
+@pytest.mark.parametrize("form", [bg.FORM_PLAIN_F64, bg.FORM_F32_COMPENSATED])
+def test_bundled_groups_float_forms(form):
+    announce(f"bundled groups float form {form}")
+    import test_dataset_layout as tdl
+    x, y = tdl.one_hot_frame(n=50_000)
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=8))
+    p = bg.LearnerParams(num_leaves=31, min_data_in_leaf=10, form=form, batched=1)
+    lockstep(ds, bst.RegressionL2Objective(y), p, iters=4, compare=assert_trees_close)
