@@ -320,6 +320,9 @@ void launch_gather_bag(const DevState& s, const int32_t* bag, int nbag, int dst_
 // (its device address): small transfers then stay on the compute queue instead
 // of a DMA engine, whose completion the queue would wait for.
 void launch_copy(void* dst, const void* src, size_t n, void* stream);
+// Writes seq into *word (device address of pinned host memory) after the work
+// enqueued on `stream` before it.
+void launch_stamp(unsigned* word, unsigned seq, void* stream);
 void launch_softmax_gradients(const double* score, const float* label, int n, int num_class, float* g, float* h,
                               void* stream);
 // LambdaRank tables and switches (objective/rank.rs:183-240).

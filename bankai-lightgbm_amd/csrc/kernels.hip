@@ -2565,6 +2565,17 @@ __global__ void __launch_bounds__(256) k_copy(uint8_t* __restrict__ dst, const u
   for (size_t i = done + i0; i < n; i += step) dst[i] = src[i];
 }
 
+// Stamps `seq` into a word of pinned host memory: launched after a stream's work,
+// it lets the host spin on the word instead of sleeping in a blocked wait.
+__global__ void k_stamp(unsigned* word, unsigned seq) {
+  if (threadIdx.x == 0) __hip_atomic_store(word, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_stamp(unsigned* word, unsigned seq, void* stream) {
+  launch(k_stamp, dim3(1), dim3(64), 0, (hipStream_t)stream, word, seq);
+  BK_HIP(hipGetLastError());
+}
+
 void launch_copy(void* dst, const void* src, size_t n, void* stream) {
   if (n == 0) return;
   const int vec = ((((uintptr_t)dst) | ((uintptr_t)src)) & 15u) == 0 ? 1 : 0;

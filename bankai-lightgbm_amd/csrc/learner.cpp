@@ -570,6 +570,8 @@ struct Learner {
   double* d_renew = nullptr;  // device address of the renewal sums inside h_tail
   uint8_t* h_tail = nullptr;  // pinned: [error flag, 16 bytes][renewal sums, 2 per leaf]
   uint8_t* h_misc = nullptr;  // pinned landing area of the per-tree set-up's small reads
+  unsigned* h_stamp = nullptr;  // pinned word the stamp kernel writes (sync_spin)
+  unsigned stamp_seq = 0;
   int* d_order = nullptr;
   double* d_leaf_vals = nullptr;
   int32_t* d_leaf_of = nullptr;
@@ -666,6 +668,33 @@ struct Learner {
     staged_pending = false;  // every staged copy has completed
   }
 
+  // The same wait for work whose results land in pinned memory through kernels
+  // (no pageable copies before it): a one-lane kernel stamps a sequence number
+  // into pinned memory after that work, and the host spins on it. A blocked wait
+  // on the stream sleeps on an interrupt and wakes tens of microseconds after the
+  // work ends; the spin sees the stamp within a PCIe write. After kSpinMs without
+  // the stamp the blocked wait takes over, which also reports any fault.
+  static constexpr double kSpinMs = 200.0;
+  void sync_spin() {
+    const unsigned seq = ++stamp_seq;
+    launch_stamp(dev_alias(h_stamp), seq, stream);
+    const auto w0 = std::chrono::steady_clock::now();
+    bool seen = false;
+    for (unsigned i = 1;; ++i) {
+      if (__atomic_load_n(h_stamp, __ATOMIC_ACQUIRE) == seq) {
+        seen = true;
+        break;
+      }
+      if ((i & 4095u) == 0u &&
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count() > kSpinMs)
+        break;
+      __builtin_ia32_pause();
+    }
+    if (!seen) BK_HIPC(hipStreamSynchronize(stream));
+    wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    staged_pending = false;
+  }
+
   ~Learner() { release(); }
 
   void release() {
@@ -690,7 +719,7 @@ struct Learner {
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay})
       f(p);
-    for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc,
+    for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc, (void*)h_stamp,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
                     (void*)h_out})
       fh(p);
@@ -920,6 +949,8 @@ struct Learner {
     // the renewal kernels write the leaf sums straight into pinned host memory
     pinned(&h_tail, 16 + sizeof(double) * 2 * (size_t)P.num_leaves);
     pinned(&h_misc, kMiscBytes);
+    pinned(&h_stamp, 16);
+    *h_stamp = 0u;
     BK_HIPC(hipHostGetDevicePointer((void**)&d_renew, h_tail + 16, 0));
     d_order = dmalloc<int>((size_t)P.num_leaves);
     d_leaf_vals = dmalloc<double>((size_t)P.num_leaves);
@@ -1144,7 +1175,7 @@ struct Learner {
       launch_max_abs(g, h, N, d_max4, stream);
       unsigned mx[4];
       launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
-      sync();
+      sync_spin();
       std::memcpy(mx, h_misc, sizeof mx);
       float max_g, max_h;
       std::memcpy(&max_g, &mx[0], 4);
@@ -1180,7 +1211,7 @@ struct Learner {
         long long sums[4] = {0, 0, 0, 0};
         if (!resum_all) {
           launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream);
-          sync();
+          sync_spin();
           std::memcpy(sums, h_misc, sizeof sums);
         }
         if (resum_all) sums_bagged = false;
@@ -1237,7 +1268,7 @@ struct Learner {
       if ((size_t)nb * 2 * sizeof(double) + 16 > kMiscBytes) throw std::runtime_error("set-up landing area too small");
       launch_copy(dev_alias(h_misc) + 16, d_sum_partials, sizeof(double) * 2 * (size_t)nb, stream);
       launch_copy(dev_alias(h_misc), h, sizeof(float), stream);
-      sync();
+      sync_spin();
       float hh0 = 0.0f;
       std::memcpy(&hh0, h_misc, sizeof(float));
       std::vector<double> part((size_t)nb * 2);
@@ -1446,7 +1477,7 @@ struct Learner {
     // the batched form with by-node sampling folds on the host, later
     if (batched && cols.by_node_active())
       launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)nitems, stream);
-    sync();
+    sync_spin();
     stats.rounds += 1;
   }
 
@@ -1998,7 +2029,7 @@ struct Learner {
     }
     // one synchronization for the error flag and the renewal sums
     launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
-    sync();
+    sync_spin();
     int err = 0;
     std::memcpy(&err, h_tail, sizeof(int));
     if (err) {
