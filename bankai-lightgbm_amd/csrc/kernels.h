@@ -250,13 +250,13 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
                  const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
 // Rows per partition tile for a record stride (2048 16-byte chunks per tile).
 int partition_tile_rows(int stride);
-// Single-pass stable partition of every item: tasks are the items' tiles in
-// order (chunk == partition_tile_rows). d_status holds one word per task,
-// d_ticket a 64-bit counter of tiles taken (ticket_base = its value before this
-// launch), epoch a nonzero tag below 2^30 that differs from the previous launch.
-void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks,
-                      unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
-                      unsigned epoch, void* stream);
+// Single-pass partition of every item (row order kept within a tile, not across
+// tiles): tasks are the items' tiles in order (chunk == partition_tile_rows).
+// d_cursors: 2 banks x max_items x (left, right) u32, zeroed before the first
+// launch; a launch uses bank (epoch & 1) and zeroes the other, so consecutive
+// launches on one stream alternate epoch parity.
+void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
+                      int max_items, unsigned epoch, void* stream);
 // Sums the partial histograms of every item into its pool slot; the search then
 // reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
 // from its slot.

@@ -467,29 +467,29 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 }
 
 // ---------------------------------------------------------------------------
-// Single-pass stable partition (data_partition.rs:124-231): left rows first,
-// then right rows, both in row order. Every item is cut into tiles of
-// part_tile_rows(stride) rows; blocks take tiles in ticket order (an atomic
-// counter), so a tile only waits on tiles that are already running. Per tile:
-// the records are loaded as 16-byte chunks into registers (consecutive lanes,
-// consecutive chunks), the lanes holding the split group's byte route their
-// rows, a block scan ranks the rows, and the tile's left count is published
-// for the tiles after it while wave 0 looks back over the tiles before it
-// (decoupled look-back) for its left offset. Then the chunks are written to
-// their destinations: two contiguous runs per tile. Rows carry no id: the leaf
-// of every row is found afterwards by walking the tree over the records in
-// their original order.
-// status word: [flag 2 bits: 1 aggregate, 2 inclusive][epoch 30 bits][count 32 bits].
+// Single-pass partition (data_partition.rs:124-231): a leaf's rows move to the
+// other buffer, left rows first, then right rows. Every item is cut into tiles
+// of part_tile_rows(stride) rows, one block per tile. Per tile: the records are
+// loaded as 16-byte chunks into registers (consecutive lanes, consecutive
+// chunks), the lanes holding the split group's byte route their rows, a block
+// scan ranks the rows, and one atomic add per side on the item's cursors claims
+// the tile's two output runs; the chunks are then written straight to them.
+//
+// The reference's partition is stable; this one keeps row order within a tile,
+// not across tiles. Nothing computed from the segments depends on that order:
+// quantized histograms are exact integer sums, the float forms already add in
+// atomic order, and the renewal sums, score updates and leaf rows read the row
+// of every leaf from the tree walk over the records in their original order.
+// Without an order to keep, no tile waits on another (no look-back).
+//
+// Cursors: two banks of (left, right) u32 per item; a launch uses bank
+// (epoch & 1) and zeroes the other for the next launch. A tile whose claimed run
+// would leave its side of the segment (the routed counts disagree with the
+// search's exact counts) writes nothing and sets err bit 2.
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
 constexpr int kPartChunks = 8;  // 16-byte chunks per thread per tile
-constexpr int kSpinLimit = 1 << 22;
 constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
-
-__device__ __forceinline__ unsigned long long status_word(unsigned flag, unsigned epoch, unsigned value) {
-  return ((unsigned long long)flag << 62) | ((unsigned long long)(epoch & 0x3fffffffu) << 32) |
-         (unsigned long long)value;
-}
 
 __device__ __forceinline__ long long wave_sum64(long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -504,10 +504,9 @@ __host__ __device__ inline size_t partition_lds(int R, int stride) {
 
 template <int FORM>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
-                                                            int nitems, int ntasks,
-                                                            unsigned long long* status, unsigned long long* ticket,
-                                                            unsigned long long ticket_base, unsigned epoch) {
-  __shared__ int sh_tile, sh_left_before;
+                                                            int nitems, unsigned* __restrict__ cursors,
+                                                            int max_items, unsigned epoch) {
+  __shared__ int sh_lbase, sh_rbase;
   __shared__ int scratch[32];
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
@@ -516,16 +515,18 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const int rpt = (R + kPartThreads - 1) / kPartThreads;
   uint32_t* dstp = (uint32_t*)smem;
   uint8_t* lflag = (uint8_t*)(dstp + R);
-  // the items' first tiles go to LDS while the ticket is taken: the item search
-  // then costs one global load latency instead of one per bisection step
+  const int t = blockIdx.x;
+  // the cursors of the next launch start at zero
+  unsigned* other = cursors + (size_t)((epoch + 1u) & 1u) * max_items * 2;
+  for (int i = t * kPartThreads + (int)threadIdx.x; i < 2 * max_items; i += (int)gridDim.x * kPartThreads) other[i] = 0u;
+  // the items' first tiles go to LDS: the item search then costs one global load
+  // latency instead of one per bisection step
   __shared__ int sh_first[kPartSearchItems];
   const bool lds_search = nitems <= kPartSearchItems;
-  if (threadIdx.x == 0) sh_tile = (int)(atomicAdd(ticket, 1ull) - ticket_base);
-  if (lds_search)
+  if (lds_search) {
     for (int i = threadIdx.x; i < nitems; i += kPartThreads) sh_first[i] = items[i].first_task;
-  __syncthreads();
-  const int t = sh_tile;
-  if (t < 0 || t >= ntasks) return;  // one ticket per block: never taken
+    __syncthreads();
+  }
   int item;
   if (lds_search) {
     int lo = 0, hi = nitems - 1;
@@ -543,9 +544,8 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const Segment seg = it.seg;
   const RuleDev& rule = it.rule;  // by reference: a local copy of its bitset would live in scratch
   const int left_total = it.left_total;
-  const long long seg_end = (long long)seg.begin + seg.count;
   const long long t0 = (long long)seg.begin + (long long)blk * it.chunk;
-  const int rows = (int)min((long long)R, seg_end - t0);
+  const int rows = (int)min((long long)R, (long long)seg.begin + seg.count - t0);
   const uint8_t* src = rec_of(s, seg.buf) + (size_t)t0 * stride;
   uint8_t* dst = rec_of(s, it.dst_buf);
   const int total = rows * nc;
@@ -575,52 +575,11 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   }
   int pl, pr, tl, tr;
   block_scan2<kPartThreads>(__popc(lm), __popc(vm & ~lm), &pl, &pr, &tl, &tr, scratch);
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    long long excl = 0;
-    if (blk == 0) {
-      if (lane == 0)
-        __hip_atomic_store(&status[t], status_word(2, epoch, (unsigned)tl), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&status[t], status_word(1, epoch, (unsigned)tl), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      int j = t - 1, spins = 0;
-      for (;;) {
-        const int idx = j - lane;
-        unsigned flag = 2u, val = 0u;  // before the item's first tile: nothing to add
-        if (idx >= it.first_task) {
-          const unsigned long long w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          flag = ((unsigned)(w >> 32) & 0x3fffffffu) == (epoch & 0x3fffffffu) ? (unsigned)(w >> 62) : 0u;
-          val = (unsigned)w;
-        }
-        if (__any(flag == 0u)) {
-          if (++spins > kSpinLimit) {  // a predecessor never published: report, do not hang
-            if (lane == 0) atomicOr(s.err_flag, 4);
-            excl = 0;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        const unsigned long long inc = __ballot(flag == 2u);
-        if (inc) {
-          const int k = __ffsll((long long)inc) - 1;
-          excl += wave_sum64(lane <= k ? (long long)val : 0ll);
-          break;
-        }
-        excl += wave_sum64((long long)val);
-        j -= 64;
-      }
-      if (lane == 0)
-        __hip_atomic_store(&status[t], status_word(2, epoch, (unsigned)(excl + tl)), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (lane == 0) sh_left_before = (int)excl;
+  if (threadIdx.x == 0) {
+    unsigned* cur = cursors + (size_t)(epoch & 1u) * max_items * 2 + 2 * (size_t)item;
+    sh_lbase = tl ? (int)atomicAdd(&cur[0], (unsigned)tl) : 0;
+    sh_rbase = tr ? (int)atomicAdd(&cur[1], (unsigned)tr) : 0;
   }
-  __syncthreads();
-  const int left_before = sh_left_before;
   // tile-local destination of every row: left rows at [0, tl), right rows at
   // [tl, tl + tr), both in row order
   {
@@ -631,28 +590,28 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     }
   }
   __syncthreads();
-  const long long lpos = (long long)seg.begin + left_before;
-  const long long rpos = (long long)seg.begin + left_total + ((t0 - seg.begin) - left_before);
-  if (lpos < (long long)seg.begin || lpos + tl > seg_end || rpos < (long long)seg.begin || rpos + tr > seg_end) {
+  const int lbase = sh_lbase, rbase = sh_rbase;
+  if (lbase + tl > left_total || rbase + tr > seg.count - left_total) {
     if (threadIdx.x == 0) atomicOr(s.err_flag, 2);  // inconsistent counts: write nothing
-  } else {
-    // every lane writes its chunks straight to their destination rows; the rows
-    // of a tile land in two contiguous runs, so the lanes of a wave cover
-    // contiguous 16-byte pieces of them
-    const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
+    return;
+  }
+  const long long lpos = (long long)seg.begin + lbase;
+  const long long rpos = (long long)seg.begin + left_total + rbase;
+  // every lane writes its chunks straight to their destination rows; the rows of
+  // a tile land in two contiguous runs, so the lanes of a wave cover contiguous
+  // 16-byte pieces of them
+  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
 #pragma unroll
-    for (int k = 0; k < kPartChunks; ++k) {
-      const int q = (int)threadIdx.x + k * kPartThreads;
-      if (q < total) {
-        const int r = q / nc, c = q - r * nc;
-        const int l = (int)dstp[r];
-        const long long d = l < tl ? lpos + l : rpos + (l - tl);
-        if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
-        store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
-      }
+  for (int k = 0; k < kPartChunks; ++k) {
+    const int q = (int)threadIdx.x + k * kPartThreads;
+    if (q < total) {
+      const int r = q / nc, c = q - r * nc;
+      const int l = (int)dstp[r];
+      const long long d = l < tl ? lpos + l : rpos + (l - tl);
+      if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
+      store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
     }
   }
-  if (threadIdx.x == 0 && blk == it.ntasks - 1 && left_before + tl != left_total) atomicOr(s.err_flag, 1);
 }
 
 // Records of buffer 0 from column-major bins: a block stages 64 rows of every
@@ -2429,10 +2388,10 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
 
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
-void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks,
-                      unsigned long long* d_status, unsigned long long* d_ticket, unsigned long long ticket_base,
-                      unsigned epoch, void* stream) {
+void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
+                      int max_items, unsigned epoch, void* stream) {
   if (ntasks <= 0) return;
+  if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
   const int R = partition_tile_rows(s.stride);
   if (R < 1) throw std::runtime_error("row record too large for the partition tile");
   const size_t lds = partition_lds(R, s.stride);
@@ -2441,13 +2400,13 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
   switch (s.form) {
     case FORM_QUANT:
       allow_lds(k_partition<FORM_QUANT>);
-      launch(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
-                         nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
+      launch(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
+             d_cursors, max_items, epoch);
       break;
     default:
       allow_lds(k_partition<FORM_F64>);
-      launch(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items,
-                         nitems, ntasks, d_status, d_ticket, ticket_base, epoch);
+      launch(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
+             d_cursors, max_items, epoch);
       break;
   }
   BK_HIP(hipGetLastError());

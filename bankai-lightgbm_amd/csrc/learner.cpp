@@ -517,8 +517,7 @@ struct Learner {
   size_t num_slots = 0;   // device slots (batched form: one per evaluated leaf)
   int max_items = 0, max_partials = 0, max_part_tiles = 0;
   int part_tile = 0;                   // rows per partition tile
-  unsigned long long ticket_base = 0;  // partition tiles taken before the next launch
-  unsigned epoch = 0;                  // tag of the last partition launch
+  unsigned epoch = 0;  // partition launches so far: the parity picks the cursor bank
 
   // device memory
   hipStream_t stream = nullptr;
@@ -535,8 +534,7 @@ struct Learner {
   double* d_part_g = nullptr;
   double* d_part_h = nullptr;
   unsigned* d_part_c = nullptr;
-  unsigned long long* d_status = nullptr;
-  unsigned long long* d_ticket = nullptr;
+  unsigned* d_cursors = nullptr;  // partition cursors: 2 banks x max_items x (left, right)
   int* d_err = nullptr;
   int* d_tile_first = nullptr;
   // staging arenas: every round's items go up in one copy, results come back in one
@@ -712,7 +710,7 @@ struct Learner {
     }
     for (void* p : {(void*)d_grad, (void*)d_hess, (void*)d_gnoise, (void*)d_hnoise, (void*)d_feats,
                     (void*)d_group_bin_start, (void*)d_pool, (void*)d_pool_cnt, (void*)d_part_q, (void*)d_part_g,
-                    (void*)d_part_h, (void*)d_part_c, (void*)d_status, (void*)d_ticket, (void*)d_err, (void*)d_tile_first,
+                    (void*)d_part_h, (void*)d_part_c, (void*)d_cursors, (void*)d_err, (void*)d_tile_first,
                     (void*)d_gh0, (void*)d_arena, (void*)d_group_used, (void*)d_out,
                     (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
@@ -895,10 +893,8 @@ struct Learner {
       d_part_h = dmalloc<double>((size_t)max_partials * TB);
       d_part_c = dmalloc<unsigned>((size_t)max_partials * TB);
     }
-    d_status = dmalloc<unsigned long long>((size_t)max_part_tiles);
-    BK_HIPC(hipMemsetAsync(d_status, 0, sizeof(unsigned long long) * (size_t)max_part_tiles, stream));
-    d_ticket = dmalloc<unsigned long long>(1);
-    BK_HIPC(hipMemsetAsync(d_ticket, 0, sizeof(unsigned long long), stream));
+    d_cursors = dmalloc<unsigned>((size_t)4 * max_items);
+    BK_HIPC(hipMemsetAsync(d_cursors, 0, sizeof(unsigned) * (size_t)4 * max_items, stream));
     d_err = dmalloc<int>(1);
     BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
     d_tile_first = dmalloc<int>(tile_first.size());
@@ -1444,12 +1440,11 @@ struct Learner {
   // Partition of the uploaded items (tiles 0 .. ntasks-1).
   void run_partition(int nitems, int ntasks, double part_rows) {
     if (ntasks <= 0) return;
-    if (ntasks > max_part_tiles) throw std::runtime_error("partition status buffer too small");
-    epoch = epoch >= 0x3ffffffeu ? 1u : epoch + 1u;
+    if (ntasks > max_part_tiles) throw std::runtime_error("too many partition tiles");
+    ++epoch;
     timed(2, part_rows * 9.0, part_rows * 2.0 * stride, [&] {
-      launch_partition(ds, d_pitems, nitems, ntasks, d_status, d_ticket, ticket_base, epoch, stream);
+      launch_partition(ds, d_pitems, nitems, ntasks, d_cursors, max_items, epoch, stream);
     });
-    ticket_base += (unsigned long long)ntasks;
     stats.part_rows += (int64_t)part_rows;
   }
 
