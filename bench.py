@@ -329,7 +329,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the learner has no multi-GPU path (rows would shard across GPUs "
+              "with one histogram all-reduce per build; not built).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
