@@ -459,10 +459,15 @@ struct FeatureSampler {
     tree_cnt = per_tree ? count((int)valid.size(), ff) : (int)valid.size();
     next_tree();
   }
-  void next_tree() {
+  // the sample of tree `slot` of every `period` trees drawn from the stream (a
+  // learner that grows one class tree of each iteration takes the draws of all)
+  void next_tree(int slot = 0, int period = 1) {
     if (!per_tree) return;
+    for (int j = 0; j < period; ++j) {
+      std::vector<int> pick = rnd.sample((int)valid.size(), tree_cnt);
+      if (j == slot) in_tree = std::move(pick);
+    }
     std::fill(bytree.begin(), bytree.end(), (char)0);
-    in_tree = rnd.sample((int)valid.size(), tree_cnt);
     for (int i : in_tree) bytree[(size_t)valid[(size_t)i]] = 1;
   }
   bool by_node_active() const { return bynode < 1.0; }
@@ -1160,7 +1165,7 @@ struct Learner {
     // the tree's feature sample (col_sampler.rs:87-101): histograms only for the
     // groups of its features
     if (cols.per_tree) {
-      cols.next_tree();
+      cols.next_tree(tree_slot, tree_period);
       set_groups_used(false);
     }
     stats = bk_train_stats{};
@@ -2415,6 +2420,10 @@ int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
   return bk::guard([&] {
     if (trees_per_iteration < 1 || slot < 0 || slot >= trees_per_iteration)
       throw std::invalid_argument("slot must lie in [0, trees_per_iteration)");
+    if (trees_per_iteration > 1 && l->L.cols.by_node_active())
+      throw std::invalid_argument(
+          "by-node feature sampling draws as many samples as the other trees' searches: the trees of one "
+          "iteration need one learner");
     l->L.tree_slot = slot;
     l->L.tree_period = trees_per_iteration;
   });

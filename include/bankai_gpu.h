@@ -274,10 +274,13 @@ int bk_learner_lambdarank_gradients_device(bk_learner* learner, const double* d_
  * on its own stream and host thread, so the class trees grow concurrently. The
  * learner consumes the stochastic-rounding rotation draws of every tree of the
  * sequence (GradientDiscretizer, crates/lgbm-core/src/treelearner/gradient_discretizer/mod.rs:172-201)
+ * and the by-tree feature samples of every tree (ColSampler, col_sampler.rs:87-101),
  * and keeps its own, so the trees equal those of one learner training the
  * whole sequence in order, provided no tree of the sequence has its
  * discretization refused (all-zero or non-finite gradients), which skips a draw.
- * No reference counterpart: the reference trains the class trees one by one. */
+ * Refused (-1) with by-node feature sampling, whose draws depend on the other
+ * trees' searches. No reference counterpart: the reference trains the class trees
+ * one by one. */
 int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iteration);
 
 /* Per-tree counters of the last train call. hist_rows / part_rows count every

@@ -329,16 +329,18 @@ def test_softmax_gradients_device_match_host():
 
 
 @pytest.mark.parametrize("batched", [0, 1])
-def test_concurrent_class_learners_match_one_learner(batched):
+@pytest.mark.parametrize("ff", [1.0, 0.6])
+def test_concurrent_class_learners_match_one_learner(batched, ff):
     """One learner per class, trained concurrently from host threads (each on its
     own stream, set_tree_slot(k, K)), grows bit for bit the trees of one learner
-    training the classes in order: the stochastic-rounding draws follow the
-    shared sequence."""
-    announce(f"3 class learners in threads vs one learner, batched={batched}")
+    training the classes in order: the stochastic-rounding draws and the by-tree
+    feature samples follow the shared sequence."""
+    announce(f"3 class learners in threads vs one learner, batched={batched}, feature_fraction={ff}")
     from concurrent.futures import ThreadPoolExecutor
     x, y = synth.multiclass_50(30_000, seed=6, num_class=3)
     ds = bg.Dataset(x, bg.DatasetParams(max_bin=63), categorical=synth.MULTICLASS_CATEGORICAL)
-    p = quantized(bg.LearnerParams(num_leaves=31, form=bg.FORM_QUANTIZED, batched=batched), 7)
+    p = quantized(bg.LearnerParams(num_leaves=31, form=bg.FORM_QUANTIZED, batched=batched, feature_fraction=ff,
+                                   feature_fraction_seed=9), 7)
     obj = bst.MulticlassObjective(y, 3)
     n = ds.num_data
     one = bg.Learner(ds, p, False)
@@ -620,3 +622,11 @@ def test_bundled_groups_float_forms(form):
     ds = bg.Dataset(x, bg.DatasetParams(num_threads=8))
     p = bg.LearnerParams(num_leaves=31, min_data_in_leaf=10, form=form, batched=1)
     lockstep(ds, bst.RegressionL2Objective(y), p, iters=4, compare=assert_trees_close)
+
+
+def test_tree_slot_refused_with_by_node_sampling(binary_10k):
+    ds, _ = binary_10k
+    lr = bg.Learner(ds, bg.LearnerParams(feature_fraction_bynode=0.5))
+    lr.set_tree_slot(0, 1)
+    with pytest.raises(bg.BankaiError, match="by-node"):
+        lr.set_tree_slot(1, 3)
