@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // search's exact counts) writes nothing and sets err bit 2.
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
-constexpr int kPartChunks = 8;  // 16-byte chunks per thread per tile
+constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
 constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
 
 __device__ __forceinline__ long long wave_sum64(long long v) {
