@@ -5,7 +5,9 @@
  * Replaces `Dataset::from_dense` (crates/lgbm-core/src/io/dataset.rs:904-915):
  * `SampleSet::from_dense` + `find_bins` (crates/lgbm-core/src/io/sample.rs:31-196),
  * `BinMapper::find_bin` (crates/lgbm-core/src/io/bin.rs:342-553),
- * `DatasetLayout::construct` (dataset.rs:378-447) and `fill_dense` (dataset.rs:633-689).
+ * `DatasetLayout::construct` (dataset.rs:378-447) with `FastFeatureBundling`
+ * (dataset.rs:282-363) and `fill_dense` (dataset.rs:633-689). Groups are stored one
+ * byte per row: bundles of more than 256 bins and multi-value groups are refused.
  * Same error convention as bankai_gpu.h (0 / -1 and bk_last_error()).
  */
 #ifndef BANKAI_DATASET_H
