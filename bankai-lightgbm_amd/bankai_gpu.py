@@ -142,7 +142,7 @@ class DatasetParamsC(C.Structure):
                 ("use_missing", C.c_int32), ("zero_as_missing", C.c_int32),
                 ("feature_pre_filter", C.c_int32), ("min_data_in_leaf", C.c_int32),
                 ("num_threads", C.c_int32), ("enable_bundle", C.c_int32), ("is_enable_sparse", C.c_int32),
-                ("gpu_bundle_limit", C.c_int32)]
+                ("gpu_bundle_limit", C.c_int32), ("unbundle_groups", C.c_int32)]
 
 
 # Every entry point the headers declare, with its argument types.
@@ -226,6 +226,7 @@ class DatasetParams:
     enable_bundle: int = 1
     is_enable_sparse: int = 1
     gpu_bundle_limit: int = 0
+    unbundle_groups: int = 0
 
     def to_c(self):
         return DatasetParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})

@@ -12,8 +12,10 @@
 //   feature bundling    crates/lgbm-core/src/io/dataset.rs:95-363 (FixSampleIndices,
 //                       FindGroups, FastFeatureBundling)
 //   fill_dense          crates/lgbm-core/src/io/dataset.rs:633-689
-// Every group is stored as one u8 column: bundles of more than 256 bins and
-// multi-value groups are refused (the reference's u16 and multi-value bins).
+// Every stored group is one u8 column. A group of the reference's layout that does
+// not fit (a bundle of more than 256 bins, kept in u16 bins by the reference, or a
+// multi-value group) is stored exploded: one column per sub-feature, holding the
+// values the reference's group holds, in the reference's inner feature order.
 
 #include <algorithm>
 #include <cmath>
@@ -398,7 +400,12 @@ struct bk_dataset {
   // layout: the real columns of each group in sub-feature order (bundled and
   // shuffled as upstream does); inner features are numbered group by group
   std::vector<std::vector<int32_t>> group_feats;
-  std::vector<std::vector<uint8_t>> cols;  // per group
+  // the groups of the reference's layout and how each is stored: as one group, or
+  // exploded into one stored group per sub-feature (a multi-value group, or a
+  // bundle of more than 256 bins)
+  std::vector<std::vector<int32_t>> plan_groups;
+  std::vector<char> plan_multi, plan_exploded;
+  std::vector<std::vector<uint8_t>> cols;  // per stored group
   std::vector<const uint8_t*> col_ptrs;
   std::vector<int32_t> g_num_total_bin, g_feature_start, g_feature_cnt, g_bin_offsets;
   std::vector<int32_t> f_num_bin, f_missing, f_default, f_mfb, f_type, f_real;
@@ -410,21 +417,37 @@ struct bk_dataset {
 
 namespace {
 
-// Group columns (fill_dense, dataset.rs:670-689): a row's value of each
+// Group columns (fill_dense, dataset.rs:633-689): a row's value of each
 // sub-feature, pushed in real-column order, that is not the most frequent bin is
-// stored at its sub-feature's offset; within a bundle a later column overwrites.
+// stored at its sub-feature's offset; within a single-value bundle a later column
+// overwrites, so a row holds one sub-feature's value and every other sub-feature
+// reads as its most frequent bin. An exploded group keeps exactly that: a
+// multi-value group stores every sub-feature's value in its own column; a bundle
+// stores, in the column of the last sub-feature the row pushed, that value, and
+// bin 0 (the most frequent bin) in the others. Each feature's histogram and each
+// split's routing are then those of the reference's group.
 void fill_columns(bk_dataset* ds, const double* rowmajor_f64, const float* rowmajor_f32, int64_t nrow, int32_t ncol,
                   int nthreads) {
-  size_t ng = ds->group_feats.size();
+  const size_t ng = ds->group_feats.size(), np = ds->plan_groups.size();
   ds->cols.assign(ng, std::vector<uint8_t>((size_t)nrow, 0));
+  std::vector<size_t> first(np + 1, 0);  // first stored group of each plan group
+  for (size_t q = 0; q < np; ++q) first[q + 1] = first[q] + (ds->plan_exploded[q] ? ds->plan_groups[q].size() : 1);
+  if (first[np] != ng) throw std::logic_error("stored groups disagree with the plan");
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-  for (long g = 0; g < (long)ng; ++g) {
-    const std::vector<int32_t>& feats = ds->group_feats[(size_t)g];
-    const int32_t* offsets = ds->g_bin_offsets.data() + ds->g_feature_start[(size_t)g] + g;
+  for (long q = 0; q < (long)np; ++q) {
+    const std::vector<int32_t>& feats = ds->plan_groups[(size_t)q];
+    const bool exploded = ds->plan_exploded[(size_t)q] != 0, multi = ds->plan_multi[(size_t)q] != 0;
     std::vector<size_t> order(feats.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return feats[a] < feats[b]; });
-    uint8_t* col = ds->cols[(size_t)g].data();
+    // stored column and bin offset of each sub-feature
+    std::vector<uint8_t*> col(feats.size());
+    std::vector<int32_t> off(feats.size());
+    for (size_t j = 0; j < feats.size(); ++j) {
+      const size_t g = first[(size_t)q] + (exploded ? j : 0);
+      col[j] = ds->cols[g].data();
+      off[j] = ds->g_bin_offsets[(size_t)ds->g_feature_start[g] + g + (exploded ? 0 : j)];
+    }
     for (int64_t r = 0; r < nrow; ++r) {
       for (size_t sub : order) {
         const int32_t real = feats[sub];
@@ -433,7 +456,9 @@ void fill_columns(bk_dataset* ds, const double* rowmajor_f64, const float* rowma
         uint32_t bin = m.value_to_bin(v);
         if (bin == m.most_freq_bin) continue;
         if (m.most_freq_bin == 0) bin -= 1;
-        col[r] = (uint8_t)(bin + (uint32_t)offsets[sub]);
+        if (exploded && !multi)
+          for (size_t j = 0; j < feats.size(); ++j) col[j][r] = 0;  // an earlier push is overwritten
+        col[sub][r] = (uint8_t)(bin + (uint32_t)off[sub]);
       }
     }
   }
@@ -484,10 +509,7 @@ void build_layout(bk_dataset* ds, std::vector<std::vector<int32_t>> groups) {
       ++f;
     }
     ds->g_num_total_bin[g] = total;
-    if (total > 256)
-      throw std::invalid_argument(
-          "feature group with more than 256 bins (a u16 bundle) is not supported: set gpu_bundle_limit or disable "
-          "bundling");
+    if (total > 256) throw std::logic_error("a stored group holds more than 256 bins");
   }
   ds->infos.assign((size_t)ds->num_total_features, "none");
   for (int32_t i = 0; i < ds->num_total_features; ++i) {
@@ -777,14 +799,27 @@ bk_dataset* construct(const T* data, int64_t nrow, int32_t ncol, const bk_datase
   if (p->num_threads <= 0) nthreads = omp_get_max_threads();
 #endif
   Plan pl = plan_dataset(data, nrow, ncol, p, cat, ncat, nthreads);
-  for (char m : pl.multi)
-    if (m) throw std::invalid_argument("multi-value feature groups (sparse bundles with conflicts) are not supported");
   auto ds = std::make_unique<bk_dataset>();
   ds->num_data = nrow;
   ds->num_total_features = ncol;
+  // a group that cannot be one byte per row (multi-valued, or more than 256 bins),
+  // or any bundle when asked, is stored exploded, one column per sub-feature
+  std::vector<std::vector<int32_t>> stored;
+  ds->plan_exploded.assign(pl.groups.size(), 0);
+  for (size_t g = 0; g < pl.groups.size(); ++g) {
+    const bool explode = pl.groups[g].size() > 1 &&
+                         (pl.multi[g] || group_num_total_bin(pl, g) > 256 || p->unbundle_groups != 0);
+    ds->plan_exploded[g] = explode ? 1 : 0;
+    if (explode)
+      for (int32_t f : pl.groups[g]) stored.push_back({f});
+    else
+      stored.push_back(pl.groups[g]);
+  }
+  ds->plan_groups = pl.groups;
+  ds->plan_multi = pl.multi;
   ds->mappers = std::move(pl.mappers);
   ds->used = std::move(pl.used);
-  build_layout(ds.get(), std::move(pl.groups));
+  build_layout(ds.get(), std::move(stored));
   if (std::is_same<T, double>::value) fill_columns(ds.get(), (const double*)data, nullptr, nrow, ncol, nthreads);
   else fill_columns(ds.get(), nullptr, (const float*)data, nrow, ncol, nthreads);
   return ds.release();

@@ -7,7 +7,10 @@
  * `BinMapper::find_bin` (crates/lgbm-core/src/io/bin.rs:342-553),
  * `DatasetLayout::construct` (dataset.rs:378-447) with `FastFeatureBundling`
  * (dataset.rs:282-363) and `fill_dense` (dataset.rs:633-689). Groups are stored one
- * byte per row: bundles of more than 256 bins and multi-value groups are refused.
+ * byte per row; a multi-value group or a bundle of more than 256 bins is stored
+ * exploded, one column per sub-feature, holding exactly the values the reference's
+ * group holds (for a bundle, the last column a row pushes), with the reference's
+ * inner feature order.
  * Same error convention as bankai_gpu.h (0 / -1 and bk_last_error()).
  */
 #ifndef BANKAI_DATASET_H
@@ -35,6 +38,8 @@ typedef struct bk_dataset_params {
   int32_t is_enable_sparse;         /* default 1: regroup sparse groups (FindGroups) */
   int32_t gpu_bundle_limit;         /* default 0; 1 = the bundling of device_type "gpu": groups of
                                        at most 256 bins (dataset.rs:159-181,259) */
+  int32_t unbundle_groups;          /* default 0; 1 = store every bundle exploded (one column per
+                                       sub-feature); groups that cannot be one byte per row always are */
 } bk_dataset_params;
 
 typedef struct bk_dataset bk_dataset;
@@ -57,8 +62,9 @@ int bk_dataset_feature_info(const bk_dataset* ds, int32_t col, char* buf, int32_
  * FastFeatureBundling, dataset.rs:282-363), without building the bins: for every
  * column its group and sub-feature index (-1: unused), and per group (capacity ncol)
  * its number of bins as FeatureGroupLayout::new counts them and whether it is
- * multi-valued. Groups this library cannot store (more than 256 bins, multi-valued)
- * are reported too; bk_dataset_from_dense refuses them. */
+ * multi-valued. Groups that do not fit one byte per row (more than 256 bins,
+ * multi-valued) are reported as the reference plans them; bk_dataset_from_dense
+ * stores them exploded. */
 int bk_dataset_bundle_layout(const double* data, int64_t nrow, int32_t ncol, const bk_dataset_params* params,
                              const int32_t* categorical, int32_t num_categorical, int32_t* feature_group,
                              int32_t* feature_sub, int64_t* group_num_total_bin, int8_t* group_multi_val,

@@ -153,21 +153,35 @@ def test_bundling_plan_matches_upstream(case):
 
 @pytest.mark.parametrize("case", CASES)
 def test_layout_matches_upstream_binary_dataset(case):
+    """The stored layout: the file's groups where they fit one byte per row; a group
+    that does not (more than 256 bins, or multi-valued) stored exploded, its
+    sub-features as consecutive single-feature groups. The inner feature order is
+    the file's in both cases."""
     d = os.path.join(GOLDEN, case)
     spec = json.load(open(os.path.join(d, "params.json")))
     want = read_layout(os.path.join(d, "dataset.bin"))
     x = np.load(os.path.join(d, "x.npy"))
-    sizes = np.diff(want["group_bin_boundaries"])
-    storable = not any(want["is_multi_val"]) and all(s <= 256 for s in sizes)
-    if not storable:
-        with pytest.raises(bg.BankaiError, match="256 bins|multi-value"):
-            bg.Dataset(x, dataset_params(spec), categorical=spec["categorical"])
-        return
     ds = bg.Dataset(x, dataset_params(spec), categorical=spec["categorical"])
     got = view_layout(ds)
-    for k, v in got.items():
-        assert v == want[k], (case, k, v[:12] if isinstance(v, list) else v,
-                              want[k][:12] if isinstance(want[k], list) else want[k])
+    assert got["real_feature_idx"] == want["real_feature_idx"], case
+    assert got["used_feature_map"] == want["used_feature_map"], case
+    sizes = np.diff(want["group_bin_boundaries"])
+    v = ds.view
+    g = 0
+    for q in range(want["num_groups"]):
+        cnt = want["group_feature_cnt"][q]
+        if cnt == 1 or (not want["is_multi_val"][q] and sizes[q] <= 256):
+            assert got["group_feature_cnt"][g] == cnt and got["group_bin_boundaries"][g + 1] - \
+                got["group_bin_boundaries"][g] == sizes[q], (case, q)
+            g += 1
+        else:
+            for j in range(cnt):
+                f = want["group_feature_start"][q] + j
+                assert got["group_feature_cnt"][g] == 1 and got["feature2group"][f] == g, (case, q, j)
+                nb = v.feature_num_bin[f] - (1 if v.feature_most_freq_bin[f] == 0 else 0)
+                assert v.group_num_total_bin[g] == 1 + nb, (case, q, j)
+                g += 1
+    assert g == got["num_groups"], case
 
 
 def test_bundles_exist_among_the_pinned_cases():
@@ -239,3 +253,62 @@ def test_oracle_trees_identical_bundled_or_not(quantized):
         gbdt.train(8)
         trees.append([t.to_model_string() for t in gbdt.models])
     assert trees[0] == trees[1]
+
+
+def _oracle_trees(x, y, dp, lp, rounds=6):
+    import boosting as bst
+    import oracle_binding as ob
+    ds = bg.Dataset(x, dp)
+    obj = bst.RegressionL2Objective(y)
+    gbdt = bst.GBDT([ob.OracleLearner(ds, lp, obj.const_hess)], obj, ds.num_data)
+    gbdt.train(rounds)
+    return ds, [t.to_model_string() for t in gbdt.models]
+
+
+@pytest.mark.parametrize("case", ["one_hot_bundle", "sparse_bundle"])
+def test_exploded_groups_train_like_unbundled(case):
+    """The fixture's wide bundle of exclusive columns (one_hot_bundle: 1682 bins) and
+    its multi-value group (sparse_bundle) are stored exploded. A multi-value group
+    keeps every value and exclusive columns lose none, so the oracle grows the same
+    trees as on the unbundled layout."""
+    d = os.path.join(GOLDEN, case)
+    x = np.load(os.path.join(d, "x.npy"))
+    y = x[:, 0] + np.nan_to_num(x[:, 1:6]).sum(axis=1) + 0.1 * np.random.default_rng(1).normal(size=len(x))
+    lp = bg.LearnerParams(num_leaves=15, min_data_in_leaf=10)
+    ds_b, bundled = _oracle_trees(x, y, bg.DatasetParams(num_threads=1), lp)
+    assert ds_b.view.num_groups > bg.bundle_layout(x, bg.DatasetParams(num_threads=1))["group_num_total_bin"].size
+    _, plain = _oracle_trees(x, y, bg.DatasetParams(num_threads=1, enable_bundle=0), lp)
+    assert bundled == plain
+
+
+@pytest.mark.parametrize("quantized", [0, 1])
+def test_exploded_bundle_keeps_the_last_pushed_value(quantized):
+    """Bundles with conflicting rows (two columns non-zero in one row: the bundle keeps
+    the later column's value) stored as bundles and stored exploded give the same
+    trees: the exploded columns keep exactly the values the bundle holds."""
+    rng = np.random.default_rng(9)
+    n, d = 50_000, 24  # the conflict budget is n / 10000 rows per bundle
+    x = np.zeros((n, d))
+    owner = rng.integers(0, d, size=n)
+    x[np.arange(n), owner] = np.round(rng.normal(size=n) + 3, 1)
+    x[:, 0] = rng.normal(size=n)
+    conflict = rng.random((n, d)) < 0.0002
+    x[conflict] = -1.5
+    y = x[:, 0] + x[:, 1] - x[:, 2] + 0.3 * (x[:, 3] != 0)
+    lp = bg.LearnerParams(num_leaves=15, min_data_in_leaf=5)
+    if quantized:
+        lp.use_quantized_grad, lp.num_grad_quant_bins, lp.seed, lp.form = 1, 4, 7, bg.FORM_QUANTIZED
+    base = bg.DatasetParams(num_threads=1, gpu_bundle_limit=1, is_enable_sparse=0)
+    plan = bg.bundle_layout(x, base)
+    fg = plan["feature_group"]
+    conflicting = 0
+    for g in set(fg[fg >= 0].tolist()):
+        cols = np.flatnonzero(fg == g)
+        if len(cols) > 1:
+            conflicting += int(((x[:, cols] != 0).sum(axis=1) > 1).sum())
+    assert conflicting > 0, "no bundle holds a conflicting row"
+    ds_b, as_bundles = _oracle_trees(x, y, base, lp)
+    ds_e, exploded = _oracle_trees(x, y, bg.DatasetParams(num_threads=1, gpu_bundle_limit=1, is_enable_sparse=0,
+                                                          unbundle_groups=1), lp)
+    assert ds_e.view.num_groups > ds_b.view.num_groups
+    assert as_bundles == exploded

@@ -630,3 +630,21 @@ def test_tree_slot_refused_with_by_node_sampling(binary_10k):
     lr.set_tree_slot(0, 1)
     with pytest.raises(bg.BankaiError, match="by-node"):
         lr.set_tree_slot(1, 3)
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_exploded_groups_quantized_bit_exact(batched):
+    """Sparse columns under the reference's default bundling plan u16 bundles and a
+    multi-value group; they are stored exploded (one column per sub-feature) and the
+    GPU learner matches the oracle bit for bit on them."""
+    announce(f"exploded groups quantized batched={batched}")
+    x, y = sparse_frame(50_000)
+    dp = bg.DatasetParams(num_threads=8)
+    plan = bg.bundle_layout(x, dp)
+    assert plan["group_multi_val"].any() or (plan["group_num_total_bin"] > 256).any()
+    ds = bg.Dataset(x, dp)
+    assert ds.view.num_groups > len(plan["group_num_total_bin"])
+    p = bg.LearnerParams(num_leaves=63, min_data_in_leaf=10, use_quantized_grad=1, num_grad_quant_bins=8,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=5, form=bg.FORM_QUANTIZED,
+                         batched=batched)
+    lockstep(ds, bst.RegressionL2Objective(y), p, iters=4)
