@@ -156,6 +156,9 @@ EXPORTS = {
     "bk_learner_add_out_of_bag_prediction": ([_P, C.POINTER(TreeOutC), _P, C.c_int64, _P], C.c_int),
     "bk_learner_add_out_of_bag_prediction_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
     "bk_learner_last_splits": ([_P, _P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "bk_learner_histogram_cache_capacity": ([_P, C.POINTER(C.c_int64)], C.c_int),
+    "bk_learner_reset_histogram_pool_config": ([_P], C.c_int),
+    "bk_learner_continue_after_dataset_reset": ([_P, _P], C.c_int),
     "bk_learner_add_prediction_for_rows": ([_P, C.POINTER(TreeOutC), _P, C.c_int32, _P, C.c_int64, _P], C.c_int),
     "bk_learner_add_prediction_to_score": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
     "bk_learner_add_prediction_to_score_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
@@ -466,6 +469,17 @@ class Learner:
         r = np.ascontiguousarray(rows, dtype=np.int32)
         _check(self._lib.bk_learner_add_out_of_bag_prediction(self.handle, C.byref(self.tree.c), _ptr(r), len(r),
                                                               _ptr(score)))
+
+    def histogram_cache_capacity(self):
+        out = C.c_int64()
+        _check(self._lib.bk_learner_histogram_cache_capacity(self.handle, C.byref(out)))
+        return out.value
+
+    def reset_histogram_pool_config(self):
+        _check(self._lib.bk_learner_reset_histogram_pool_config(self.handle))
+
+    def continue_after_dataset_reset(self, previous):
+        _check(self._lib.bk_learner_continue_after_dataset_reset(self.handle, previous.handle))
 
     def last_splits(self):
         """The last tree's split records (an opaque uint8 array, one row per internal node)."""

@@ -2324,6 +2324,28 @@ int bk_learner_add_out_of_bag_prediction(bk_learner* l, const bk_tree_out* tree,
   });
 }
 
+int bk_learner_histogram_cache_capacity(bk_learner* l, int64_t* out) {
+  return bk::guard([&] {
+    if (!l || !out) throw std::invalid_argument("null argument");
+    *out = (int64_t)l->L.cache;
+  });
+}
+
+int bk_learner_reset_histogram_pool_config(bk_learner* l) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    l->L.cols.next_tree();
+  });
+}
+
+int bk_learner_continue_after_dataset_reset(bk_learner* l, const bk_learner* previous) {
+  return bk::guard([&] {
+    if (!l || !previous) throw std::invalid_argument("null argument");
+    l->L.cols.rnd = previous->L.cols.rnd;
+    l->L.cols.next_tree();
+  });
+}
+
 int bk_learner_last_splits(bk_learner* l, bk_split_record* out, int32_t capacity, int32_t* num_out) {
   return bk::guard([&] {
     if (!l || !num_out) throw std::invalid_argument("null argument");

@@ -186,6 +186,19 @@ int bk_learner_add_out_of_bag_prediction(bk_learner* learner, const bk_tree_out*
 int bk_learner_add_out_of_bag_prediction_device(bk_learner* learner, const bk_tree_out* tree,
                                                 double* d_score);
 
+/* SerialTreeLearner::histogram_cache_capacity (serial.rs:250): the histogram slots
+ * of the reference's pool (num_leaves, or what histogram_pool_size allows). */
+int bk_learner_histogram_cache_capacity(bk_learner* learner, int64_t* capacity);
+
+/* SerialTreeLearner::reset_histogram_pool_config (serial.rs:256), the budget-only
+ * reset: the pool is kept and the feature sampler draws a by-tree sample from its
+ * continuing stream. */
+int bk_learner_reset_histogram_pool_config(bk_learner* learner);
+
+/* SerialTreeLearner::continue_after_dataset_reset (serial.rs:266): the feature
+ * sampler continues `previous`'s stream and draws a by-tree sample from it. */
+int bk_learner_continue_after_dataset_reset(bk_learner* learner, const bk_learner* previous);
+
 /* How one split of a tree routes rows (SplitRecord, crates/lgbm-core/src/treelearner/
  * split_info.rs and serial.rs:2003): the group whose bin is read and the learner's
  * routing rule, stored as it was applied. Opaque to the caller, who keeps the

@@ -1952,6 +1952,15 @@ int bko_learner_quant_state(bko_learner* h, int16_t* packed, double* gs, double*
   return 0;
 }
 
+// histogram_cache_capacity (serial.rs:250), reset_histogram_pool_config (256-264)
+// and continue_after_dataset_reset (266-272) for learners without extra trees.
+int64_t bko_learner_histogram_cache_capacity(bko_learner* h) { return (int64_t)h->L.pool.size(); }
+void bko_learner_reset_histogram_pool_config(bko_learner* h) { h->L.cols.reset_by_tree(); }
+void bko_learner_continue_after_dataset_reset(bko_learner* h, const bko_learner* prev) {
+  h->L.cols.rnd = prev->L.cols.rnd;
+  h->L.cols.reset_by_tree();
+}
+
 void bko_learner_free(bko_learner* h) { delete h; }
 
 // ---- primitive entry points, for the known-answer tests -------------------

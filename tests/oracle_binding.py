@@ -36,6 +36,9 @@ def load():
         "bko_learner_stats": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "bko_learner_quant_state": ([P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "bko_learner_free": ([P], None),
+        "bko_learner_histogram_cache_capacity": ([P], C.c_int64),
+        "bko_learner_reset_histogram_pool_config": ([P], None),
+        "bko_learner_continue_after_dataset_reset": ([P, P], None),
         "bko_mt_new": ([C.c_uint32], P),
         "bko_mt_next": ([P], C.c_uint32),
         "bko_mt_uniform_real": ([P], C.c_double),
@@ -110,6 +113,15 @@ class OracleLearner:
         gs, hs = C.c_double(), C.c_double()
         self._lib.bko_learner_quant_state(self.handle, _p(packed), C.byref(gs), C.byref(hs))
         return packed, gs.value, hs.value
+
+    def histogram_cache_capacity(self):
+        return self._lib.bko_learner_histogram_cache_capacity(self.handle)
+
+    def reset_histogram_pool_config(self):
+        self._lib.bko_learner_reset_histogram_pool_config(self.handle)
+
+    def continue_after_dataset_reset(self, previous):
+        self._lib.bko_learner_continue_after_dataset_reset(self.handle, previous.handle)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -648,3 +648,28 @@ def test_exploded_groups_quantized_bit_exact(batched):
                          stochastic_rounding=1, quant_train_renew_leaf=1, seed=5, form=bg.FORM_QUANTIZED,
                          batched=batched)
     lockstep(ds, bst.RegressionL2Objective(y), p, iters=4)
+
+
+@pytest.mark.parametrize("pool_mb", [0.0, 0.5])
+def test_pool_capacity_and_config_resets(binary_10k, pool_mb):
+    """histogram_cache_capacity, the budget-only reset_histogram_pool_config and
+    continue_after_dataset_reset against the oracle: the capacity matches, and the
+    feature sampler's extra by-tree draw and continued stream give the same trees."""
+    announce(f"pool capacity and config resets, pool {pool_mb} MB")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=31, histogram_pool_size=pool_mb, feature_fraction=0.6, feature_fraction_seed=4,
+                         use_quantized_grad=1, num_grad_quant_bins=4, seed=2, form=bg.FORM_QUANTIZED)
+    o, gpu = ob.OracleLearner(ds, p, False), bg.Learner(ds, p, False)
+    assert gpu.histogram_cache_capacity() == o.histogram_cache_capacity()
+    if pool_mb:
+        assert gpu.histogram_cache_capacity() < 31
+    obj = bst.BinaryObjective(y)
+    g, h = obj.gradients(np.zeros(ds.num_data))
+    assert_trees_bitwise(gpu.train(g, h), o.train(g, h), "before reset")
+    gpu.reset_histogram_pool_config()
+    o.reset_histogram_pool_config()
+    assert_trees_bitwise(gpu.train(g, h), o.train(g, h), "after reset")
+    o2, gpu2 = ob.OracleLearner(ds, p, False), bg.Learner(ds, p, False)
+    o2.continue_after_dataset_reset(o)
+    gpu2.continue_after_dataset_reset(gpu)
+    assert_trees_bitwise(gpu2.train(g, h), o2.train(g, h), "continued")
