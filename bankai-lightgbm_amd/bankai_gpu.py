@@ -63,6 +63,8 @@ class LearnerParamsC(C.Structure):
         ("quant_train_renew_leaf", C.c_int32), ("seed", C.c_int32), ("num_threads", C.c_int32),
         ("form", C.c_int32), ("batched", C.c_int32), ("feature_fraction", C.c_double),
         ("feature_fraction_bynode", C.c_double), ("feature_fraction_seed", C.c_int32),
+        ("num_interaction_constraints", C.c_int32), ("interaction_constraint_offsets", C.POINTER(C.c_int32)),
+        ("interaction_constraint_features", C.POINTER(C.c_int32)),
     ]
 
 
@@ -370,9 +372,21 @@ class LearnerParams:
     feature_fraction: float = 1.0
     feature_fraction_bynode: float = 1.0
     feature_fraction_seed: int = 2
+    interaction_constraints: tuple = ()  # lists of real feature ids
 
     def to_c(self):
-        return LearnerParamsC(**{f.name: getattr(self, f.name) for f in fields(self)})
+        kw = {f.name: getattr(self, f.name) for f in fields(self) if f.name != "interaction_constraints"}
+        c = LearnerParamsC(**kw)
+        groups = [list(g) for g in self.interaction_constraints]
+        offsets = np.zeros(len(groups) + 1, np.int32)
+        for i, g in enumerate(groups):
+            offsets[i + 1] = offsets[i] + len(g)
+        feats = np.asarray([f for g in groups for f in g] or [0], np.int32)
+        c.num_interaction_constraints = len(groups)
+        c.interaction_constraint_offsets = offsets.ctypes.data_as(C.POINTER(C.c_int32))
+        c.interaction_constraint_features = feats.ctypes.data_as(C.POINTER(C.c_int32))
+        c._keep = (offsets, feats)  # the arrays live as long as the struct
+        return c
 
 
 SPLIT_RECORD_BYTES = 128  # sizeof(bk_split_record)

@@ -436,8 +436,9 @@ struct Lcg {
 };
 
 // Feature sampling per tree and per node (ColSampler, treelearner/col_sampler.rs:
-// 14-176, without interaction constraints). One LCG stream: a by-tree draw at set-up
-// and at every tree, by-node draws at every search (smaller leaf, then larger).
+// 14-218). One LCG stream: a by-tree draw at set-up and at every tree, by-node draws
+// at every search (smaller leaf, then larger). Interaction constraints restrict a
+// leaf's features to those allowed by the real features split on along its branch.
 struct FeatureSampler {
   Lcg rnd;
   std::vector<int> valid;  // inner features in real-index order
@@ -446,6 +447,8 @@ struct FeatureSampler {
   double bynode = 1.0;
   bool per_tree = false;
   int tree_cnt = 0;
+  std::vector<std::vector<int>> constraints;  // sorted real feature ids per constraint
+  std::vector<int> real_of;                   // real feature id of each inner feature
 
   static int count(int total, double fraction) {
     return std::max(std::min(1, total), (int)((double)total * fraction + 0.5));
@@ -470,16 +473,47 @@ struct FeatureSampler {
     std::fill(bytree.begin(), bytree.end(), (char)0);
     for (int i : in_tree) bytree[(size_t)valid[(size_t)i]] = 1;
   }
-  bool by_node_active() const { return bynode < 1.0; }
-  std::vector<char> by_node() {
-    std::vector<char> m(valid.size(), bynode >= 1.0 ? 1 : 0);
-    if (bynode >= 1.0) return m;
-    if (per_tree) {
-      for (int p : rnd.sample((int)in_tree.size(), count((int)in_tree.size(), bynode)))
-        m[(size_t)valid[(size_t)in_tree[(size_t)p]]] = 1;
-    } else {
-      for (int p : rnd.sample((int)valid.size(), count((int)valid.size(), bynode))) m[(size_t)valid[(size_t)p]] = 1;
+  // by-node masks differ between leaves (sampling or constraints); sampling draws
+  bool by_node_active() const { return bynode < 1.0 || !constraints.empty(); }
+  bool by_node_draws() const { return bynode < 1.0; }
+  // the branch's own features and every constraint holding all of them (every
+  // constraint for the root), as allowed_features does (col_sampler.rs:178-201)
+  std::vector<int> allowed(const std::vector<int>& branch) const {
+    std::vector<int> out(branch);
+    for (const auto& c : constraints) {
+      bool all = true;
+      for (int f : branch) all = all && std::binary_search(c.begin(), c.end(), f);
+      if (all) out.insert(out.end(), c.begin(), c.end());
     }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
+  // get_by_node (col_sampler.rs:111-176) for a leaf whose path split on `branch`
+  std::vector<char> by_node(const std::vector<int>& branch) {
+    const bool constrained = !constraints.empty();
+    const std::vector<int> ok = constrained ? allowed(branch) : std::vector<int>();
+    auto admits = [&](int inner) { return !constrained || std::binary_search(ok.begin(), ok.end(), real_of[(size_t)inner]); };
+    std::vector<char> m(valid.size(), 0);
+    if (bynode >= 1.0) {
+      for (size_t f = 0; f < m.size(); ++f) m[f] = admits((int)f) ? 1 : 0;
+      return m;
+    }
+    // candidates (indices into `valid`): the tree's sample or every feature, those
+    // the constraints admit; the count comes from the unfiltered set
+    std::vector<int> cand;
+    int cnt;
+    if (per_tree) {
+      cnt = count((int)in_tree.size(), bynode);
+      for (int i : in_tree)
+        if (admits(valid[(size_t)i])) cand.push_back(i);
+    } else {
+      cnt = count((int)valid.size(), bynode);
+      for (int i = 0; i < (int)valid.size(); ++i)
+        if (admits(valid[(size_t)i])) cand.push_back(i);
+    }
+    if (constrained) cnt = std::min(cnt, (int)cand.size());
+    for (int p : rnd.sample((int)cand.size(), cnt)) m[(size_t)valid[(size_t)cand[(size_t)p]]] = 1;
     return m;
   }
 };
@@ -636,6 +670,7 @@ struct Learner {
   std::vector<int> leaf_node;
   std::vector<char> node_flags;  // splittable flags of node i at [i * F, (i + 1) * F)
   FeatureSampler cols;
+  std::vector<std::vector<int>> branch;  // real features split on along each leaf's path
   // by-node sampling in the batched form: every searched node keeps its per-feature
   // candidates and searched bits; its best is folded on the host when the reference
   // would draw its sample (when its parent's split is committed)
@@ -1033,6 +1068,21 @@ struct Learner {
       std::stable_sort(order.begin(), order.end(),
                        [&](int a, int b) { return v->feature_real_index[a] < v->feature_real_index[b]; });
       cols.init(std::move(order), P.feature_fraction, P.feature_fraction_bynode, P.feature_fraction_seed);
+      cols.real_of.assign(v->feature_real_index, v->feature_real_index + F);
+      if (P.num_interaction_constraints < 0) throw std::invalid_argument("negative interaction constraint count");
+      if (P.num_interaction_constraints > 0 &&
+          (!P.interaction_constraint_offsets || !P.interaction_constraint_features))
+        throw std::invalid_argument("interaction constraints without their arrays");
+      for (int i = 0; i < P.num_interaction_constraints; ++i) {
+        const int a = P.interaction_constraint_offsets[i], b = P.interaction_constraint_offsets[i + 1];
+        if (a < 0 || b < a) throw std::invalid_argument("interaction constraint offsets must ascend from 0");
+        std::vector<int> c(P.interaction_constraint_features + a, P.interaction_constraint_features + b);
+        std::sort(c.begin(), c.end());
+        cols.constraints.push_back(std::move(c));
+      }
+      // the arrays are the caller's: they are not kept
+      P.interaction_constraint_offsets = nullptr;
+      P.interaction_constraint_features = nullptr;
       if (cols.by_node_active()) pinned(&h_out, (size_t)F * 2 * max_items);
     }
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
@@ -1159,6 +1209,7 @@ struct Learner {
     for (auto& s : best) s.reset();
     amax.init(&best);
     std::fill(segs.begin(), segs.end(), Segment{0, 0, 0});
+    branch.assign((size_t)P.num_leaves, std::vector<int>());
     const bool bag = d_rows != nullptr;
     const int count = bag ? nrows : N;
     segs[0] = bag ? Segment{1, 0, nrows} : Segment{0, 0, N};
@@ -1563,6 +1614,11 @@ struct Learner {
       t.th_bin[(size_t)node] = (uint32_t)info.o.threshold;
       t.th[(size_t)node] = upper[(size_t)inner][(size_t)info.o.threshold];
     }
+    // both children's branch features: the parent's path and this split's real
+    // feature (serial.rs:1719-1722)
+    branch[(size_t)t.num_leaves] = branch[(size_t)best_leaf];
+    branch[(size_t)t.num_leaves].push_back(info.o.feature);
+    branch[(size_t)best_leaf].push_back(info.o.feature);
     NodeRule& nd = tree_nodes[(size_t)(t.num_leaves - 1)];
     nd.rule = rule_of(info);
     nd.group = m.group;
@@ -1627,8 +1683,8 @@ struct Learner {
     stage_wait();
     std::vector<char> sm, lg;
     if (cols.by_node_active()) {
-      sm = cols.by_node();
-      if (larger.leaf >= 0) lg = cols.by_node();
+      sm = cols.by_node(branch[(size_t)smaller.leaf]);
+      if (larger.leaf >= 0) lg = cols.by_node(branch[(size_t)larger.leaf]);
     }
     for (int f = 0; f < F; ++f) {
       used_feat[(size_t)f] = cols.bytree[(size_t)f];
@@ -1799,7 +1855,7 @@ struct Learner {
       nodes[0].best = fold(0, flags_of(0));
       if (host_fold) {
         keep(0, 0, 0);
-        nodes[0].best = fold_host(&node_cands[0], &node_searched[0], cols.by_node());
+        nodes[0].best = fold_host(&node_cands[0], &node_searched[0], cols.by_node(branch[0]));
       }
     } else {
       nodes[0].best.o.gain = kMinScore;
@@ -1831,8 +1887,8 @@ struct Learner {
         if (host_fold && nodes[(size_t)x].kids_searched && t.num_leaves < P.num_leaves) {
           const bool left_small = info.o.exact_left < info.o.exact_right;
           const int sm = left_small ? cl : cr, lg = left_small ? cr : cl;
-          const std::vector<char> ms = cols.by_node();
-          const std::vector<char> ml = cols.by_node();
+          const std::vector<char> ms = cols.by_node(branch[(size_t)(left_small ? (int)bl : right)]);
+          const std::vector<char> ml = cols.by_node(branch[(size_t)(left_small ? right : (int)bl)]);
           nodes[(size_t)sm].best = fold_host(&node_cands[(size_t)sm * F], &node_searched[(size_t)sm * F], ms);
           nodes[(size_t)lg].best = fold_host(&node_cands[(size_t)lg * F], &node_searched[(size_t)lg * F], ml);
         }
@@ -2442,7 +2498,7 @@ int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
   return bk::guard([&] {
     if (trees_per_iteration < 1 || slot < 0 || slot >= trees_per_iteration)
       throw std::invalid_argument("slot must lie in [0, trees_per_iteration)");
-    if (trees_per_iteration > 1 && l->L.cols.by_node_active())
+    if (trees_per_iteration > 1 && l->L.cols.by_node_draws())
       throw std::invalid_argument(
           "by-node feature sampling draws as many samples as the other trees' searches: the trees of one "
           "iteration need one learner");

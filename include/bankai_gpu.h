@@ -102,6 +102,13 @@ typedef struct bk_learner_params {
   double feature_fraction;        /* ColSampler (treelearner/col_sampler.rs): by-tree  */
   double feature_fraction_bynode; /* and by-node fractions, each in (0, 1]             */
   int32_t feature_fraction_seed;  /* seeds the sampler's LCG (Config::set derives it)  */
+  /* interaction_constraints (Config::interaction_constraints_vector): constraint i is the
+   * real feature ids interaction_constraint_features[interaction_constraint_offsets[i] ..
+   * interaction_constraint_offsets[i + 1]); none when num_interaction_constraints is 0.
+   * The arrays are read at creation only. */
+  int32_t num_interaction_constraints;
+  const int32_t* interaction_constraint_offsets;
+  const int32_t* interaction_constraint_features;
 } bk_learner_params;
 
 /*

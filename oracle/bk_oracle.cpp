@@ -108,9 +108,10 @@ struct LcgRandom {
   }
 };
 
-// ColSampler (treelearner/col_sampler.rs:14-176) without interaction constraints:
-// the by-tree sample is drawn at set-up and at every tree, the by-node samples at
-// every search, smaller leaf first.
+// ColSampler (treelearner/col_sampler.rs:14-218): the by-tree sample is drawn at
+// set-up and at every tree, the by-node samples at every search, smaller leaf
+// first; interaction constraints restrict a leaf's features to those allowed by
+// the real features split on along its branch.
 struct ColSampler {
   double frac_bynode = 1.0;
   LcgRandom rnd;
@@ -120,6 +121,26 @@ struct ColSampler {
   int used_cnt_bytree = 0;
   std::vector<char> bytree;
   std::vector<int> used_idx;  // indices into valid_inner of the by-tree sample
+  std::vector<std::vector<int>> constraints;  // sorted real feature ids per constraint
+  std::vector<int> real_of;                   // real feature id of each inner feature
+
+  // allowed_features (col_sampler.rs:178-201): the branch's own features, plus every
+  // constraint that holds all of them (every constraint for an empty branch)
+  std::vector<int> allowed(const std::vector<int>& branch) const {
+    std::vector<int> out(branch.begin(), branch.end());
+    for (const auto& c : constraints) {
+      bool all = true;
+      for (int f : branch)
+        if (!std::binary_search(c.begin(), c.end(), f)) {
+          all = false;
+          break;
+        }
+      if (all) out.insert(out.end(), c.begin(), c.end());
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
 
   static int get_cnt(int total, double fraction) {
     const int mn = std::min(1, total);
@@ -144,19 +165,31 @@ struct ColSampler {
     for (int i : used_idx) bytree[(size_t)valid_inner[(size_t)i]] = 1;
   }
 
-  std::vector<char> by_node() {
+  // get_by_node (col_sampler.rs:111-176)
+  std::vector<char> by_node(const std::vector<int>& branch) {
     std::vector<char> ret((size_t)nf, 0);
+    const bool constrained = !constraints.empty();
+    const std::vector<int> ok = constrained ? allowed(branch) : std::vector<int>();
+    auto is_ok = [&](int inner) { return std::binary_search(ok.begin(), ok.end(), real_of[(size_t)inner]); };
     if (frac_bynode >= 1.0) {
-      std::fill(ret.begin(), ret.end(), (char)1);
+      for (int f = 0; f < nf; ++f) ret[(size_t)f] = !constrained || is_ok(f);
       return ret;
     }
+    // candidates: the tree's sample (indices into valid_inner) or every feature,
+    // filtered by the constraints; the count is taken before the filter
+    std::vector<int> cand;
+    int cnt;
     if (need_reset_bytree) {
-      const int cnt = get_cnt((int)used_idx.size(), frac_bynode);
-      for (int p : rnd.sample((int)used_idx.size(), cnt)) ret[(size_t)valid_inner[(size_t)used_idx[(size_t)p]]] = 1;
+      cnt = get_cnt((int)used_idx.size(), frac_bynode);
+      for (int i : used_idx)
+        if (!constrained || is_ok(valid_inner[(size_t)i])) cand.push_back(i);
     } else {
-      const int cnt = get_cnt((int)valid_inner.size(), frac_bynode);
-      for (int p : rnd.sample((int)valid_inner.size(), cnt)) ret[(size_t)valid_inner[(size_t)p]] = 1;
+      cnt = get_cnt((int)valid_inner.size(), frac_bynode);
+      for (int i = 0; i < (int)valid_inner.size(); ++i)
+        if (!constrained || is_ok(valid_inner[(size_t)i])) cand.push_back(i);
     }
+    if (constrained) cnt = std::min(cnt, (int)cand.size());
+    for (int p : rnd.sample((int)cand.size(), cnt)) ret[(size_t)valid_inner[(size_t)cand[(size_t)p]]] = 1;
     return ret;
   }
 };
@@ -1303,6 +1336,7 @@ struct Learner {
   Discretizer* disc = nullptr;
   double grad_scale = 0.0, hess_scale = 0.0;
   ColSampler cols;
+  std::vector<std::vector<int>> branch;  // real features split on along each leaf's path
   // per-tree stats
   int64_t stat_hist_rows = 0, stat_part_rows = 0;
   // The last tree's splits in the order they were made (serial.rs:2003 last_splits):
@@ -1510,8 +1544,9 @@ struct Learner {
       else construct_hist(used, (size_t)larger.leaf_index, (size_t)larger_slot, g, h);
     }
     // by-node samples, smaller leaf first (serial.rs:1329-1337)
-    const std::vector<char> smask = cols.by_node();
-    const std::vector<char> lmask = larger.leaf_index >= 0 ? cols.by_node() : std::vector<char>((size_t)nf, 0);
+    const std::vector<char> smask = cols.by_node(branch[(size_t)smaller.leaf_index]);
+    const std::vector<char> lmask =
+        larger.leaf_index >= 0 ? cols.by_node(branch[(size_t)larger.leaf_index]) : std::vector<char>((size_t)nf, 0);
     double sp_out = parent_output(t, smaller);
     double lp_out = larger.leaf_index >= 0 ? parent_output(t, larger) : 0.0;
     long ls = larger.leaf_index >= 0 ? larger_slot : -1;
@@ -1636,6 +1671,11 @@ struct Learner {
       t.num_leaves += 1;
       right = t.num_leaves - 1;
     }
+    // the branch features of both children: the parent's path and this split's
+    // real feature (serial.rs:1719-1722)
+    branch[(size_t)right] = branch[(size_t)best_leaf];
+    branch[(size_t)right].push_back(info.feature);
+    branch[(size_t)best_leaf].push_back(info.feature);
     LeafSplits l{best_leaf, info.left_count, info.left_sum_gradient, info.left_sum_hessian, info.left_output,
                  info.left_int_sum};
     LeafSplits r{right, info.right_count, info.right_sum_gradient, info.right_sum_hessian, info.right_output,
@@ -1667,6 +1707,7 @@ struct Learner {
     }
     for (auto& s : best) s.reset();
     splits.clear();
+    branch.assign((size_t)num_leaves_cfg, std::vector<int>());
     cols.reset_by_tree();
     stat_hist_rows = stat_part_rows = 0;
     const size_t cnt = leaf_cnt[0];
@@ -1858,6 +1899,14 @@ int bko_learner_create(const bk_dataset_view* ds, const bk_learner_params* p, in
       std::stable_sort(valid.begin(), valid.end(),
                        [&](int a, int b) { return L.feats[(size_t)a].real_index < L.feats[(size_t)b].real_index; });
       L.cols.init(valid, L.nf, p->feature_fraction, p->feature_fraction_bynode, p->feature_fraction_seed);
+      L.cols.real_of.resize((size_t)L.nf);
+      for (int f = 0; f < L.nf; ++f) L.cols.real_of[(size_t)f] = L.feats[(size_t)f].real_index;
+      for (int i = 0; i < p->num_interaction_constraints; ++i) {
+        std::vector<int> c(p->interaction_constraint_features + p->interaction_constraint_offsets[i],
+                           p->interaction_constraint_features + p->interaction_constraint_offsets[i + 1]);
+        std::sort(c.begin(), c.end());
+        L.cols.constraints.push_back(c);
+      }
     }
     if (L.quant) {
       L.disc = new Discretizer(L.num_data, p->num_grad_quant_bins, L.const_hess);

@@ -60,6 +60,11 @@ def learner_params(p):
         if k in p:
             setattr(lp, k, p[k])
     lp.feature_fraction_seed = bst.resolve_seed(p, "feature_fraction_seed")
+    if p.get("interaction_constraints"):
+        # Config's "[0,1,2],[3,4,5]" form (config/mod.rs:537-552)
+        text = p["interaction_constraints"]
+        lp.interaction_constraints = tuple(tuple(int(v) for v in grp.split(",") if v.strip())
+                                           for grp in text.replace("]", "").split("[") if grp.strip(", "))
     lp.num_threads = 1
     return lp
 

@@ -551,7 +551,7 @@ def test_feature_sampling_with_bagging_float(binary_10k, batched):
 
 
 SAMPLED_CASES = ["bagging", "bagging_feature_fraction", "cat_bagging", "feature_fraction", "feature_fraction_both",
-                 "feature_fraction_bynode", "goss"]
+                 "feature_fraction_bynode", "goss", "interaction", "interaction_bynode", "interaction_sampled"]
 
 
 @pytest.mark.parametrize("case", SAMPLED_CASES)
@@ -673,3 +673,21 @@ def test_pool_capacity_and_config_resets(binary_10k, pool_mb):
     o2.continue_after_dataset_reset(o)
     gpu2.continue_after_dataset_reset(gpu)
     assert_trees_bitwise(gpu2.train(g, h), o2.train(g, h), "continued")
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("ffn", [1.0, 0.6])
+def test_interaction_constraints_quantized_bit_exact(binary_10k, ffn, batched):
+    """Interaction constraints (a leaf may split only on features allowed by the ones
+    on its path): the quantized GPU learner against the oracle, bit for bit, alone and
+    with by-node sampling."""
+    announce(f"interaction constraints ffn={ffn} batched={batched}")
+    ds, y = binary_10k
+    cons = ((0, 1, 2, 3), (3, 4, 5, 6, 7), tuple(range(8, 20)))
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=3, form=bg.FORM_QUANTIZED, batched=batched,
+                         feature_fraction_bynode=ffn, feature_fraction_seed=21, interaction_constraints=cons)
+    trees = lockstep(ds, bst.BinaryObjective(y), p, iters=4)
+    allowed = set(range(20))
+    for tg, _ in trees:
+        assert set(tg["split_feature"].tolist()) <= allowed
