@@ -60,6 +60,9 @@ def learner_params(p):
         if k in p:
             setattr(lp, k, p[k])
     lp.feature_fraction_seed = bst.resolve_seed(p, "feature_fraction_seed")
+    mono = [int(v) for v in str(p.get("monotone_constraints", "")).replace(" ", "").split(",") if v]
+    if any(mono):
+        raise NotImplementedError("monotone constraints are not supported by the learner")
     if p.get("interaction_constraints"):
         # Config's "[0,1,2],[3,4,5]" form (config/mod.rs:537-552)
         text = p["interaction_constraints"]
