@@ -1903,6 +1903,47 @@ __device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num
   }
 }
 
+// The same walk with each lane's two records staged in LDS (records of up to 64
+// bytes): a level then reads the group's byte with one LDS load instead of a
+// select chain over the record's registers. A lane's records start every
+// pstride = stride + 4 bytes, an odd number of words, so the 64 lanes of a wave
+// read from 64 different banks. Each lane reads only what it wrote itself.
+__device__ __forceinline__ void walk_rows_staged(const DevState& s, const NodeDev* nd, int num_nodes,
+                                                 unsigned char* recs, const uint32_t* __restrict__ bag_bits,
+                                                 int32_t* __restrict__ leaf_of) {
+  const int stride = s.stride, nw = stride >> 2, pstride = stride + 4;
+  unsigned char* my0 = recs + (size_t)threadIdx.x * pstride;
+  unsigned char* my1 = recs + (size_t)(blockDim.x + threadIdx.x) * pstride;
+  const long long n = s.num_data, step = 2ll * gridDim.x * blockDim.x;
+  for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < n; r0 += step) {
+    const long long r1 = r0 + (long long)gridDim.x * blockDim.x;
+    const bool has1 = r1 < n;
+    const uint32_t* src0 = (const uint32_t*)(s.rec[0] + (size_t)r0 * stride);
+    const uint32_t* src1 = (const uint32_t*)(s.rec[0] + (size_t)(has1 ? r1 : r0) * stride);
+    for (int w = 0; w < nw; ++w) {
+      ((uint32_t*)my0)[w] = src0[w];
+      ((uint32_t*)my1)[w] = src1[w];
+    }
+    int n0 = 0, n1 = 0;  // current node, or ~leaf once the walk ended
+    for (int k = 0; k < num_nodes && (n0 >= 0 || n1 >= 0); ++k) {
+      if (n0 >= 0) {
+        const uint32_t bin = my0[nd[n0].group];
+        n0 = ((nd[n0].left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? nd[n0].left : nd[n0].right;
+      }
+      if (n1 >= 0) {
+        const uint32_t bin = my1[nd[n1].group];
+        n1 = ((nd[n1].left_bins[bin >> 5] >> (bin & 31u)) & 1u) ? nd[n1].left : nd[n1].right;
+      }
+    }
+    const int l0 = n0 < 0 ? ~n0 : 0, l1 = n1 < 0 ? ~n1 : 0;
+    leaf_of[r0] = in_bag(bag_bits, r0) ? l0 : ~l0;
+    if (has1) leaf_of[r1] = in_bag(bag_bits, r1) ? l1 : ~l1;
+  }
+}
+
+// LDS bytes of the staged walk's records: two per lane of a 256-lane block.
+__host__ __device__ inline size_t walk_stage_bytes(int stride) { return (size_t)2 * 256 * (stride + 4); }
+
 __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev* __restrict__ nodes, int num_nodes,
                                                       int in_lds, const uint32_t* __restrict__ bag_bits,
                                                       int32_t* __restrict__ leaf_of) {
@@ -1917,7 +1958,10 @@ __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev*
     const int words = num_nodes * (int)(sizeof(NodeDev) / 4);
     for (int i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t*)smem)[i] = ((const uint32_t*)nodes)[i];
     __syncthreads();
-    walk_rows(s, (const NodeDev*)smem, num_nodes, bag_bits, leaf_of);  // LDS loads
+    if (in_lds == 2)  // records staged after the nodes
+      walk_rows_staged(s, (const NodeDev*)smem, num_nodes, smem + (size_t)words * 4, bag_bits, leaf_of);
+    else
+      walk_rows(s, (const NodeDev*)smem, num_nodes, bag_bits, leaf_of);  // LDS loads
   } else {
     walk_rows(s, nodes, num_nodes, bag_bits, leaf_of);
   }
@@ -2558,8 +2602,10 @@ void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_
     launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes);
     BK_HIP(hipGetLastError());
   }
-  const size_t lds = (size_t)num_nodes * sizeof(NodeDev);
-  const int in_lds = lds <= (size_t)kLdsBytes ? 1 : 0;
+  const size_t node_bytes = (size_t)num_nodes * sizeof(NodeDev);
+  const bool staged = s.stride <= 64 && node_bytes + walk_stage_bytes(s.stride) <= (size_t)kLdsBytes;
+  const int in_lds = staged ? 2 : (node_bytes <= (size_t)kLdsBytes ? 1 : 0);
+  const size_t lds = staged ? node_bytes + walk_stage_bytes(s.stride) : node_bytes;
   allow_lds(k_leaf_of_tree);
   launch(k_leaf_of_tree, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), in_lds ? lds : 0, (hipStream_t)stream, s,
          d_nodes, num_nodes, in_lds, bag_bits, leaf_of);
