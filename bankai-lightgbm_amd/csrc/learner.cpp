@@ -739,9 +739,9 @@ struct Learner {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
-    auto fh = [](void* p) {
+    auto fh = [this](void* p) {
       if (p) {
-        aliases().erase(p);
+        alias_of.erase(p);
         (void)hipHostFree(p);
       }
     };
@@ -767,21 +767,21 @@ struct Learner {
     stream = nullptr;
   }
 
+  // Device address of each pinned host buffer this learner owns, looked up once at
+  // allocation; the map is the learner's own, so a learner may move between threads.
+  std::unordered_map<void*, void*> alias_of;
   template <typename T>
-  static void pinned(T** p, size_t n) {
+  void pinned(T** p, size_t n) {
     BK_HIPC(hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault));
-    aliases()[(void*)*p] = nullptr;
-  }
-  // Device address of a pinned host buffer (looked up once per buffer).
-  static std::unordered_map<void*, void*>& aliases() {
-    static thread_local std::unordered_map<void*, void*> m;
-    return m;
+    void* d = nullptr;
+    BK_HIPC(hipHostGetDevicePointer(&d, (void*)*p, 0));
+    alias_of[(void*)*p] = d;
   }
   template <typename T>
-  static T* dev_alias(T* h) {
-    void*& d = aliases()[(void*)h];
-    if (!d) BK_HIPC(hipHostGetDevicePointer(&d, (void*)h, 0));
-    return (T*)d;
+  T* dev_alias(T* h) {
+    auto it = alias_of.find((void*)h);
+    if (it == alias_of.end()) throw std::logic_error("dev_alias of a buffer this learner did not pin");
+    return (T*)it->second;
   }
 
   void create(const bk_dataset_view* v, const bk_learner_params* p, int is_const_hess) {
