@@ -509,7 +509,8 @@ void build_layout(bk_dataset* ds, std::vector<std::vector<int32_t>> groups) {
       ++f;
     }
     ds->g_num_total_bin[g] = total;
-    if (total > 256) throw std::logic_error("a stored group holds more than 256 bins");
+    if (total > 256)
+      throw std::invalid_argument("a feature of more than 256 bins cannot be stored one byte per row");
   }
   ds->infos.assign((size_t)ds->num_total_features, "none");
   for (int32_t i = 0; i < ds->num_total_features; ++i) {
@@ -794,6 +795,9 @@ int64_t group_num_total_bin(const Plan& pl, size_t g) {
 template <typename T>
 bk_dataset* construct(const T* data, int64_t nrow, int32_t ncol, const bk_dataset_params* p, const int32_t* cat,
                       int32_t ncat) {
+  // bins are stored one byte per row and feature: u16 bins are not built
+  if (p->max_bin < 2 || p->max_bin > 255)
+    throw std::invalid_argument("max_bin must be in [2, 255]: bins are stored one byte per row");
   int nthreads = p->num_threads > 0 ? p->num_threads : 1;
 #ifdef _OPENMP
   if (p->num_threads <= 0) nthreads = omp_get_max_threads();

@@ -25,7 +25,7 @@ extern "C" {
 
 /* The Config fields dataset construction reads (crates/lgbm-core/src/config/auto.rs:92-99,244-251). */
 typedef struct bk_dataset_params {
-  int32_t max_bin;                  /* default 255     */
+  int32_t max_bin;                  /* default 255; in [2, 255]: one byte per bin, u16 bins are not built */
   int32_t min_data_in_bin;          /* default 3       */
   int32_t bin_construct_sample_cnt; /* default 200000  */
   int32_t data_random_seed;         /* default 1       */

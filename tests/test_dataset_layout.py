@@ -312,3 +312,12 @@ def test_exploded_bundle_keeps_the_last_pushed_value(quantized):
                                                           unbundle_groups=1), lp)
     assert ds_e.view.num_groups > ds_b.view.num_groups
     assert as_bundles == exploded
+
+
+@pytest.mark.parametrize("max_bin", [1, 256, 511])
+def test_max_bin_outside_one_byte_bins_is_refused(max_bin):
+    """Bins are stored one byte per row: max_bin outside [2, 255] is refused with
+    an error that names it, not by a failure deeper in construction."""
+    x = np.random.default_rng(0).normal(size=(5_000, 3))
+    with pytest.raises(bg.BankaiError, match="max_bin"):
+        bg.Dataset(x, bg.DatasetParams(num_threads=1, max_bin=max_bin))
