@@ -166,7 +166,7 @@ EXPORTS = {
     "bk_learner_add_prediction_to_score_device": ([_P, C.POINTER(TreeOutC), _P], C.c_int),
     "bk_learner_leaf_rows": ([_P, C.c_int32, _P, C.POINTER(C.c_int64)], C.c_int),
     "bk_learner_reset_constant_hessian": ([_P, C.c_int], C.c_int),
-    "bk_learner_binary_gradients_device": ([_P, _P, _P, C.c_double, _P, _P], C.c_int),
+    "bk_learner_binary_gradients_device": ([_P, _P, _P, _P, C.c_double, _P, _P, _P], C.c_int),
     "bk_learner_softmax_gradients_device": ([_P, _P, _P, C.c_int, _P, _P], C.c_int),
     "bk_learner_set_tree_slot": ([_P, C.c_int, C.c_int], C.c_int),
     "bk_learner_lambdarank_gradients_device": ([_P, _P, _P, _P, _P, _P], C.c_int),
@@ -175,6 +175,7 @@ EXPORTS = {
     "bk_learner_kernel_stats": ([_P, C.c_int, C.POINTER(KernelStatsC)], C.c_int),
     "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
+    "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
                               C.c_int),
@@ -214,6 +215,23 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _score_ptr(score, num_data):
+    """A score array the library writes float64 values into: it must be a
+    C-contiguous, writeable float64 numpy array of at least num_data entries."""
+    if not isinstance(score, np.ndarray) or score.dtype != np.float64 or not score.flags["C_CONTIGUOUS"] \
+            or not score.flags["WRITEABLE"] or score.ndim != 1 or score.shape[0] < num_data:
+        raise BankaiError(f"score must be a writeable C-contiguous float64 array of at least {num_data} entries")
+    return _ptr(score)
+
+
+def _rows_array(rows, num_data):
+    """Row ids as C-contiguous int32, each inside [0, num_data)."""
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    if r.ndim != 1 or (len(r) and (int(r.min()) < 0 or int(r.max()) >= num_data)):
+        raise BankaiError("rows must be a 1-d array of row ids inside the dataset")
+    return r
 
 
 @dataclass
@@ -480,9 +498,10 @@ class Learner:
     def add_out_of_bag_prediction(self, leaf_values, rows, score):
         """score[row] += leaf_values[leaf(row)] for the given rows (host float64 array)."""
         self.tree.set_leaf_values(leaf_values)
-        r = np.ascontiguousarray(rows, dtype=np.int32)
+        r = _rows_array(rows, self.dataset.num_data)
+        sp = _score_ptr(score, self.dataset.num_data)
         _check(self._lib.bk_learner_add_out_of_bag_prediction(self.handle, C.byref(self.tree.c), _ptr(r), len(r),
-                                                              _ptr(score)))
+                                                              sp))
 
     def histogram_cache_capacity(self):
         out = C.c_int64()
@@ -510,9 +529,10 @@ class Learner:
         d["leaf_value"] = np.asarray(leaf_values, np.float64)
         tb = TreeBuffers.from_dict(d, self.params.num_leaves)
         sp = np.ascontiguousarray(splits, np.uint8)
-        r = np.ascontiguousarray(rows, dtype=np.int32)
+        r = _rows_array(rows, self.dataset.num_data)
+        dst = _score_ptr(score, self.dataset.num_data)
         _check(self._lib.bk_learner_add_prediction_for_rows(self.handle, C.byref(tb.c), _ptr(sp), len(sp), _ptr(r),
-                                                            len(r), _ptr(score)))
+                                                            len(r), dst))
 
     def add_out_of_bag_prediction_device(self, leaf_values, d_score_ptr):
         self.tree.set_leaf_values(leaf_values)
@@ -528,17 +548,25 @@ class Learner:
     def add_prediction_to_score(self, leaf_values, score):
         """score[row] += leaf_values[leaf(row)] for the last tree (host float64 array, in place)."""
         self.tree.set_leaf_values(leaf_values)
-        _check(self._lib.bk_learner_add_prediction_to_score(self.handle, C.byref(self.tree.c), _ptr(score)))
+        sp = _score_ptr(score, self.dataset.num_data)
+        _check(self._lib.bk_learner_add_prediction_to_score(self.handle, C.byref(self.tree.c), sp))
 
     def add_prediction_to_score_device(self, leaf_values, d_score_ptr):
         self.tree.set_leaf_values(leaf_values)
         _check(self._lib.bk_learner_add_prediction_to_score_device(self.handle, C.byref(self.tree.c),
                                                                    C.c_void_p(d_score_ptr)))
 
-    def binary_gradients_device(self, d_score_ptr, d_label_ptr, sigmoid, d_g_ptr, d_h_ptr):
+    def binary_gradients_device(self, d_score_ptr, d_label_ptr, sigmoid, d_g_ptr, d_h_ptr, label_weights=(1.0, 1.0),
+                                d_weights_ptr=None):
+        """BinaryLogloss gradients on the device: label_weights = (negative, positive)
+        as the objective derives them (BinaryObjective.label_weights); optional
+        per-row sample weights (a device float32 pointer)."""
+        lw = (C.c_double * 2)(float(label_weights[0]), float(label_weights[1]))
         _check(self._lib.bk_learner_binary_gradients_device(self.handle, C.c_void_p(d_score_ptr),
-                                                            C.c_void_p(d_label_ptr), float(sigmoid),
-                                                            C.c_void_p(d_g_ptr), C.c_void_p(d_h_ptr)))
+                                                            C.c_void_p(d_label_ptr),
+                                                            C.c_void_p(d_weights_ptr) if d_weights_ptr else None,
+                                                            float(sigmoid), lw, C.c_void_p(d_g_ptr),
+                                                            C.c_void_p(d_h_ptr)))
 
     def lambdarank_gradients_device(self, d_score_ptr, d_label_ptr, tables, d_g_ptr, d_h_ptr):
         """tables: a RankTablesC whose arrays are device pointers."""
@@ -578,6 +606,10 @@ class Learner:
             gh = np.zeros((tb, 2), np.float64)
             _check(self._lib.bk_learner_root_histogram(self.handle, _ptr(g), _ptr(h), _ptr(gh), None, _ptr(cnt)))
         return gh, cnt
+
+    def set_hist_tiling(self, min_rows=0, max_blocks=0):
+        """Forces the histogram tiling (0, 0: the autotuner's choice)."""
+        _check(self._lib.bk_learner_set_hist_tiling(self.handle, int(min_rows), int(max_blocks)))
 
     def reset_constant_hessian(self, value):
         _check(self._lib.bk_learner_reset_constant_hessian(self.handle, int(value)))

@@ -338,7 +338,9 @@ struct RankDev {
 constexpr int kRankMaxDocs = 2048;  // documents of one query held in LDS by the gradient kernel
 void launch_lambdarank_gradients(const double* score, const float* label, const RankDev& r, float* g, float* h,
                                  void* stream);
-void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
-                             void* stream);
+// Binary log-loss (objective/binary.rs:71-90): label weights lw_neg / lw_pos,
+// optional sample weights (null: none).
+void launch_binary_gradients(const double* score, const float* label, const float* weights, int n, double sigmoid,
+                             double lw_neg, double lw_pos, float* g, float* h, void* stream);
 
 }  // namespace bk

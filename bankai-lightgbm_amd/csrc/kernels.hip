@@ -2257,15 +2257,25 @@ __global__ void __launch_bounds__(256) k_lambdarank(const double* __restrict__ s
   }
 }
 
-// Binary log-loss gradients (objective/binary.rs:71-90) for a device-side boosting loop.
-__global__ void k_binary_gradients(const double* __restrict__ score, const float* __restrict__ label, int n,
-                                   double sigmoid, float* __restrict__ g, float* __restrict__ h) {
+// Binary log-loss gradients (objective/binary.rs:71-90) for a device-side boosting
+// loop: the same products in the same order (response * label_weight [* w]).
+__global__ void k_binary_gradients(const double* __restrict__ score, const float* __restrict__ label,
+                                   const float* __restrict__ weights, int n, double sigmoid, double lw_neg,
+                                   double lw_pos, float* __restrict__ g, float* __restrict__ h) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double lab = label[i] > 0.0f ? 1.0 : -1.0;
+    const bool pos = label[i] > 0.0f;
+    const double lab = pos ? 1.0 : -1.0;
+    const double lw = pos ? lw_pos : lw_neg;
     const double response = -lab * sigmoid / (1.0 + exp(lab * sigmoid * score[i]));
     const double ar = fabs(response);
-    g[i] = (float)response;
-    h[i] = (float)(ar * (sigmoid - ar));
+    if (weights) {
+      const double w = (double)weights[i];
+      g[i] = (float)(response * lw * w);
+      h[i] = (float)(ar * (sigmoid - ar) * lw * w);
+    } else {
+      g[i] = (float)(response * lw);
+      h[i] = (float)(ar * (sigmoid - ar) * lw);
+    }
   }
 }
 
@@ -2548,10 +2558,10 @@ void launch_lambdarank_gradients(const double* score, const float* label, const 
   BK_HIP(hipGetLastError());
 }
 
-void launch_binary_gradients(const double* score, const float* label, int n, double sigmoid, float* g, float* h,
-                             void* stream) {
-  launch(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score,
-                     label, n, sigmoid, g, h);
+void launch_binary_gradients(const double* score, const float* label, const float* weights, int n, double sigmoid,
+                             double lw_neg, double lw_pos, float* g, float* h, void* stream) {
+  launch(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score, label, weights,
+         n, sigmoid, lw_neg, lw_pos, g, h);
   BK_HIP(hipGetLastError());
 }
 

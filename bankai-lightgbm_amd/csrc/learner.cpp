@@ -901,8 +901,10 @@ struct Learner {
     max_items = P.num_leaves + 1;
     part_tile = partition_tile_rows(stride);
     max_part_tiles = N / part_tile + max_items + 8;
-    // a build has at most one row block per 2048 rows (the tuner's smallest row count)
-    max_partials = std::min(kHistMaxBlocks, N / 2048 + 1) + 2 * max_items + 8;
+    // a build has at most one row block per 2048 rows (the tuner's smallest row
+    // count), and at least one per rows_per_flush rows: a partial slot never holds
+    // more rows than one LDS accumulation (see plan_hist)
+    max_partials = std::max(std::min(kHistMaxBlocks, N / 2048 + 1), N / rows_per_flush + 2) + 2 * max_items + 8;
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // marks the end of staging transfers, which only read pinned host memory
@@ -1105,8 +1107,8 @@ struct Learner {
     std::lock_guard<std::mutex> lock(mu);
     auto it = picked.find(key);
     if (it != picked.end()) {
-      hist_min_rows = it->second.first;
-      hist_max_blocks = it->second.second;
+      hist_min_rows = tuned_min_rows = it->second.first;
+      hist_max_blocks = tuned_max_blocks = it->second.second;
       return;
     }
     hipEvent_t e0, e1;
@@ -1145,8 +1147,8 @@ struct Learner {
       }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hist_min_rows = best_c.first;
-    hist_max_blocks = best_c.second;
+    hist_min_rows = tuned_min_rows = best_c.first;
+    hist_max_blocks = tuned_max_blocks = best_c.second;
     picked[key] = best_c;
   }
 
@@ -1442,6 +1444,7 @@ struct Learner {
   // Segment i's partials are hist_first[i] .. hist_first[i] + hist_count[i] - 1.
   static constexpr int kHistMaxBlocks = 2048;  // largest row-block count the tuner may pick
   int hist_max_blocks = 1024, hist_min_rows = 4096;  // picked by autotune_hist
+  int tuned_max_blocks = 1024, tuned_min_rows = 4096;  // the tuner's choice (kept under an override)
   HistPlan plan_hist(int n) {
     HistPlan hp;
     hist_first.assign((size_t)n, 0);
@@ -1456,6 +1459,10 @@ struct Learner {
     }
     if (hp.ni == 0) return hp;
     int nblocks = (int)std::min<long long>(hist_max_blocks, std::max<long long>(1, hp.total / hist_min_rows));
+    // a block's partial slot holds the sum of all its rows, packed as the LDS bins
+    // are: more rows than one LDS accumulation may hold could carry one packed field
+    // into the next, so no block takes more than rows_per_flush rows
+    nblocks = (int)std::max<long long>(nblocks, (hp.total + rows_per_flush - 1) / rows_per_flush);
     hp.rpb = (int)((hp.total + nblocks - 1) / nblocks);
     hp.nblocks = (int)((hp.total + hp.rpb - 1) / hp.rpb);
     // slots: block b gets one per item it touches
@@ -2426,7 +2433,10 @@ int bk_learner_add_out_of_bag_prediction_device(bk_learner* l, const bk_tree_out
 }
 
 int bk_learner_leaf_rows(bk_learner* l, int32_t leaf, int32_t* rows_out, int64_t* num_out) {
-  return bk::guard([&] { l->L.leaf_rows(leaf, rows_out, num_out); });
+  return bk::guard([&] {
+    if (!l || !rows_out || !num_out) throw std::invalid_argument("null argument");
+    l->L.leaf_rows(leaf, rows_out, num_out);
+  });
 }
 
 int bk_learner_root_histogram(bk_learner* l, const float* g, const float* h, double* hist_out,
@@ -2448,12 +2458,18 @@ int bk_learner_reset_constant_hessian(bk_learner* l, int is_constant_hessian) {
   });
 }
 
-int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, const float* d_label, double sigmoid,
+int bk_learner_binary_gradients_device(bk_learner* l, const double* d_score, const float* d_label,
+                                       const float* d_weights, double sigmoid, const double label_weights[2],
                                        float* d_g, float* d_h) {
   return bk::guard([&] {
+    if (!l || !d_score || !d_label || !label_weights || !d_g || !d_h) throw std::invalid_argument("null argument");
+    if (!(sigmoid > 0.0)) throw std::invalid_argument("sigmoid must be greater than zero");
     bk::Learner& L = l->L;
-    L.timed(6, (double)L.N * 20.0, (double)L.N * 20.0,
-            [&] { bk::launch_binary_gradients(d_score, d_label, L.N, sigmoid, d_g, d_h, L.stream); });
+    const double b = (double)L.N * (d_weights ? 24.0 : 20.0);
+    L.timed(6, b, b, [&] {
+      bk::launch_binary_gradients(d_score, d_label, d_weights, L.N, sigmoid, label_weights[0], label_weights[1], d_g,
+                                  d_h, L.stream);
+    });
   });
 }
 
@@ -2504,6 +2520,21 @@ int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
           "iteration need one learner");
     l->L.tree_slot = slot;
     l->L.tree_period = trees_per_iteration;
+  });
+}
+
+int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_blocks) {
+  return bk::guard([&] {
+    bk::Learner& L = l->L;
+    if (min_rows == 0 && max_blocks == 0) {
+      L.hist_min_rows = L.tuned_min_rows;
+      L.hist_max_blocks = L.tuned_max_blocks;
+      return;
+    }
+    if (min_rows < 1 || max_blocks < 1 || max_blocks > bk::Learner::kHistMaxBlocks)
+      throw std::invalid_argument("histogram tiling: min_rows >= 1 and 1 <= max_blocks <= 2048");
+    L.hist_min_rows = min_rows;
+    L.hist_max_blocks = max_blocks;
   });
 }
 

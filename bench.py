@@ -87,7 +87,8 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     trees = []
 
     def gradients():
-        learner.binary_gradients_device(score.data_ptr(), label.data_ptr(), 1.0, g.data_ptr(), h.data_ptr())
+        learner.binary_gradients_device(score.data_ptr(), label.data_ptr(), obj.sigmoid, g.data_ptr(), h.data_ptr(),
+                                        label_weights=obj.label_weights)
 
     def step(record):
         # one boosting iteration: the tree, the score update (Tree::apply_shrinkage's

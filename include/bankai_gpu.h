@@ -255,12 +255,17 @@ int bk_learner_root_histogram(bk_learner* learner, const float* gradients, const
 /* SerialTreeLearner::reset_constant_hessian (crates/lgbm-core/src/treelearner/serial.rs:279). */
 int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessian);
 
-/* Binary log-loss gradients and hessians (crates/lgbm-core/src/objective/binary.rs:71-90,
- * unweighted, label_weights 1) from a device score array, written to device arrays,
- * on the learner's stream. A device-side boosting loop uses it; it has no FFI
- * counterpart in the reference, whose objective runs on the host. */
+/* Binary log-loss gradients and hessians (crates/lgbm-core/src/objective/binary.rs:71-90)
+ * from a device score array, written to device arrays, on the learner's stream.
+ * label_weights = {negative, positive} as BinaryLogloss::new derives them from
+ * is_unbalance and scale_pos_weight (binary.rs:44-52; {1, 1} without them);
+ * d_weights: per-row sample weights, or NULL. The device exp is not the host
+ * libm's: results agree with the host objective within 2 f32 ulps. A device-side
+ * boosting loop uses it; it has no FFI counterpart in the reference, whose
+ * objective runs on the host. */
 int bk_learner_binary_gradients_device(bk_learner* learner, const double* d_score, const float* d_label,
-                                       double sigmoid, float* d_gradients, float* d_hessians);
+                                       const float* d_weights, double sigmoid, const double label_weights[2],
+                                       float* d_gradients, float* d_hessians);
 
 /* Softmax gradients and hessians (crates/lgbm-core/src/objective/multiclass.rs:68-90,
  * unweighted) for num_class classes from a device score array of num_class * num_data
@@ -302,6 +307,13 @@ int bk_learner_lambdarank_gradients_device(bk_learner* learner, const double* d_
  * trees' searches. No reference counterpart: the reference trains the class trees
  * one by one. */
 int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iteration);
+
+/* Histogram tiling override: rows per workgroup at least min_rows, row blocks at
+ * most max_blocks (0, 0: back to what the autotuner picked at creation). Results
+ * do not depend on the tiling (integer histograms are exact; the float forms stay
+ * within their tolerance); tests use it to check that, and to force row blocks
+ * larger than one LDS accumulation. No reference counterpart (tuning only). */
+int bk_learner_set_hist_tiling(bk_learner* learner, int32_t min_rows, int32_t max_blocks);
 
 /* Per-tree counters of the last train call. hist_rows / part_rows count every
  * row the kernels processed (the batched form includes speculative expansions

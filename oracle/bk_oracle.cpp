@@ -1739,7 +1739,9 @@ struct Learner {
     double carried = smaller.weight;
     smaller = LeafSplits{0, (int)cnt, sg, sh, carried, isum};
     larger = LeafSplits{};
+    root_sums = smaller;
   }
+  LeafSplits root_sums;  // the root's sums of the last tree (serial.rs:741-761)
 
   // grow_tree (serial.rs:578-628) + renew_int_grad_tree_output (634-655)
   Tree train(const float* g, const float* h, const int32_t* rows, size_t nrows) {
@@ -1999,6 +2001,23 @@ int bko_learner_quant_state(bko_learner* h, int16_t* packed, double* gs, double*
   *gs = L.grad_scale;
   *hs = L.hess_scale;
   return 0;
+}
+
+// The last tree's root sums as before_train computed them (serial.rs:741-761):
+// sequential f64 sums in row order, and the packed integer sum (quantized).
+int bko_learner_root_sums(bko_learner* h, double* sum_g, double* sum_h, int64_t* int_sum) {
+  *sum_g = h->L.root_sums.sum_g;
+  *sum_h = h->L.root_sums.sum_h;
+  *int_sum = h->L.root_sums.int_sum;
+  return 0;
+}
+
+// Threading::BlockInfo (gradient_discretizer/mod.rs:19-29), as the noise layout uses it.
+void bko_block_info(int64_t num_threads, int64_t cnt, int64_t min_cnt, int64_t* n_block, int64_t* size) {
+  size_t nb = 0, bs = 0;
+  block_info((size_t)num_threads, (size_t)cnt, (size_t)min_cnt, &nb, &bs);
+  *n_block = (int64_t)nb;
+  *size = (int64_t)bs;
 }
 
 // histogram_cache_capacity (serial.rs:250), reset_histogram_pool_config (256-264)

@@ -36,6 +36,8 @@ def load():
         "bko_learner_stats": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "bko_learner_quant_state": ([P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "bko_learner_free": ([P], None),
+        "bko_learner_root_sums": ([P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
+        "bko_block_info": ([C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], None),
         "bko_learner_histogram_cache_capacity": ([P], C.c_int64),
         "bko_learner_reset_histogram_pool_config": ([P], None),
         "bko_learner_continue_after_dataset_reset": ([P, P], None),
@@ -114,6 +116,12 @@ class OracleLearner:
         self._lib.bko_learner_quant_state(self.handle, _p(packed), C.byref(gs), C.byref(hs))
         return packed, gs.value, hs.value
 
+    def root_sums(self):
+        """(sum_g, sum_h, packed int sum) of the last tree's root (before_train)."""
+        sg, sh, isum = C.c_double(), C.c_double(), C.c_int64()
+        self._lib.bko_learner_root_sums(self.handle, C.byref(sg), C.byref(sh), C.byref(isum))
+        return sg.value, sh.value, isum.value
+
     def histogram_cache_capacity(self):
         return self._lib.bko_learner_histogram_cache_capacity(self.handle)
 
@@ -162,6 +170,13 @@ def discretize(g, h, bins, const_hess=False, stochastic=False, seed=0, num_threa
     if rc != 0:
         return None
     return packed, gs.value, hs.value
+
+
+def block_info(num_threads, cnt, min_cnt):
+    """Threading::BlockInfo as the oracle's noise layout computes it: (blocks, block size)."""
+    nb, bs = C.c_int64(), C.c_int64()
+    load().bko_block_info(int(num_threads), int(cnt), int(min_cnt), C.byref(nb), C.byref(bs))
+    return nb.value, bs.value
 
 
 def noise(n, seed, num_threads):

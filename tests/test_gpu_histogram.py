@@ -89,6 +89,48 @@ def test_root_histogram_quantized_exact(binary_20k, bins, stochastic, const_hess
     assert np.array_equal(cnt, bin_counts(ds))
 
 
+def oracle_int_root_histogram(ds, p, g, h, const_hess):
+    """Exact per-bin sums of the oracle's discretized gradients for its first tree."""
+    o = ob.OracleLearner(ds, p, const_hess)
+    o.train(g, h)
+    packed, _, _ = o.quant_state()
+    qg = (packed >> 8).astype(np.int64)
+    qh = (packed & 0xFF).astype(np.int64)
+    want = []
+    for col, nb in zip(ds.group_columns(), ds.group_num_total_bin()):
+        s = np.zeros((int(nb), 2), np.int64)
+        np.add.at(s[:, 0], col, qg)
+        np.add.at(s[:, 1], col, qh)
+        want.append(s)
+    return np.concatenate(want)
+
+
+# (min_rows, max_blocks): the autotuner's extremes, and row blocks asked to be far
+# larger than one LDS accumulation of 127-bin gradients holds (6144 rows), which
+# the planner must split so that no packed partial field overflows
+TILINGS = [(2048, 2048), (8192, 512), (65536, 4), (1 << 20, 1)]
+
+
+@pytest.mark.parametrize("bins", [4, 127])
+@pytest.mark.parametrize("tiling", TILINGS)
+def test_root_histogram_any_tiling_exact(bins, tiling):
+    announce(f"root int histogram 200k x 28 bins={bins} tiling={tiling}")
+    x, _ = synth.binary_28(200_000, seed=8)
+    ds = bg.Dataset(x)
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal(ds.num_data).astype(np.float32)
+    h = rng.uniform(0.01, 0.25, ds.num_data).astype(np.float32)
+    # every hessian quantizes to the top bins: the hessian fields fill fastest
+    h[::2] = 0.25
+    p = bg.LearnerParams(num_leaves=7, use_quantized_grad=1, num_grad_quant_bins=bins, stochastic_rounding=0,
+                         seed=4, num_threads=1, form=bg.FORM_QUANTIZED)
+    gpu = bg.Learner(ds, p, False)
+    gpu.set_hist_tiling(*tiling)
+    ih, cnt = gpu.root_histogram(g, h)
+    assert np.array_equal(ih, oracle_int_root_histogram(ds, p, g, h, False))
+    assert np.array_equal(cnt, bin_counts(ds))
+
+
 @pytest.mark.parametrize("const_hess", [0, 1])
 def test_root_histogram_f64(binary_20k, const_hess):
     announce(f"root f64 histogram const_hess={const_hess}")

@@ -328,6 +328,47 @@ def test_softmax_gradients_device_match_host():
     assert np.all(np.abs(h - want_h) <= 2 * ulp_h)
 
 
+@pytest.mark.parametrize("case", ["plain", "unbalance", "scale_pos", "weighted", "sigmoid"])
+def test_binary_gradients_device_match_host(case):
+    """The device binary log-loss gradients (the bench loop's objective) against
+    objective/binary.rs:71-90 restated on the host, with its label weights
+    (is_unbalance, scale_pos_weight) and sample weights: within 2 f32 ulps (the
+    device exp is not the host libm's). Scores span +-40 so that the response
+    saturates at both ends."""
+    announce(f"binary gradients on the device, 200k rows, {case}")
+    import torch
+    n = 200_000
+    x, y = synth.binary_28(n, seed=12)
+    ds = bg.Dataset(x[:1000])  # any dataset: the kernel reads only score and label
+    learner = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
+    rng = np.random.default_rng(21)
+    score = np.concatenate([rng.normal(0.0, 2.0, n - 2000), rng.uniform(-40, 40, 2000)])
+    w = rng.uniform(0.1, 3.0, n).astype(np.float32) if case == "weighted" else None
+    kw = {"unbalance": dict(is_unbalance=True), "scale_pos": dict(scale_pos_weight=2.5),
+          "sigmoid": dict(sigmoid=0.7)}.get(case, {})
+    obj = bst.BinaryObjective(y, weights=w, **kw)
+    want_g, want_h = obj.gradients(score)
+    dev = torch.device("cuda:0")
+    ts = torch.as_tensor(score, device=dev)
+    tl = torch.as_tensor(y.astype(np.float32), device=dev)
+    tw = torch.as_tensor(w, device=dev) if w is not None else None
+    tg = torch.empty(n, dtype=torch.float32, device=dev)
+    th = torch.empty(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    learner.binary_gradients_device(ts.data_ptr(), tl.data_ptr(), obj.sigmoid, tg.data_ptr(), th.data_ptr(),
+                                    label_weights=obj.label_weights,
+                                    d_weights_ptr=tw.data_ptr() if tw is not None else None)
+    torch.cuda.synchronize()
+    learner.close()
+    g, h = tg.cpu().numpy(), th.cpu().numpy()
+    assert np.all(np.abs(g - want_g) <= 2 * np.spacing(np.abs(want_g)))
+    assert np.all(np.abs(h - want_h) <= 2 * np.spacing(np.abs(want_h)))
+    # the weights reach the values: a dropped label or sample weight fails here
+    if case != "plain":
+        plain_g, _ = bst.BinaryObjective(y).gradients(score)
+        assert not np.allclose(g, plain_g)
+
+
 @pytest.mark.parametrize("batched", [0, 1])
 @pytest.mark.parametrize("ff", [1.0, 0.6])
 def test_concurrent_class_learners_match_one_learner(batched, ff):
