@@ -649,3 +649,146 @@ def parse_model_header(text):
             k, v = line.split("=", 1)
             out[k] = v
     return out
+
+
+# ---------------------------------------------------------------------------
+# Prediction from model text and the AUC metric
+# ---------------------------------------------------------------------------
+def parse_model(text):
+    """Gbdt::from_model_string's tree part (model_text.rs:88-167): the header's
+    num_tree_per_iteration and every tree as arrays (split_feature, threshold,
+    decision_type, children, leaf_value and the categorical bitsets)."""
+    header = parse_model_header(text)
+    trees = []
+    for b in parse_model_trees(text):
+        n = int(b["num_leaves"])
+        arr = lambda k, dt: np.array(b[k].split(), dt) if b.get(k) else np.zeros(0, dt)  # noqa: E731
+        t = {"num_leaves": n, "num_cat": int(b.get("num_cat", 0)), "leaf_value": arr("leaf_value", np.float64)}
+        if n > 1:
+            t.update(split_feature=arr("split_feature", np.int64), threshold=arr("threshold", np.float64),
+                     decision_type=arr("decision_type", np.int64), left_child=arr("left_child", np.int64),
+                     right_child=arr("right_child", np.int64))
+        t["cat_boundaries"] = arr("cat_boundaries", np.int64) if t["num_cat"] > 0 else np.zeros(1, np.int64)
+        t["cat_threshold"] = arr("cat_threshold", np.int64) if t["num_cat"] > 0 else np.zeros(0, np.int64)
+        if b.get("is_linear", "0") != "0":
+            raise NotImplementedError("linear trees are not predicted here")
+        trees.append(t)
+    return int(header.get("num_tree_per_iteration", 1)), trees
+
+
+def _leaf_of_rows(t, x, torch):
+    """Tree::get_leaf (tree.rs:718-783) for every row of x (a float64 tensor,
+    rows x features): numerical_decision with the missing-value rules and
+    categorical_decision with the bitsets, level by level over the active rows."""
+    dev = x.device
+    n = x.shape[0]
+    leaf = torch.zeros(n, dtype=torch.int64, device=dev)
+    if t["num_leaves"] <= 1:
+        return leaf
+    asT = lambda a, dt: torch.as_tensor(np.asarray(a), dtype=dt, device=dev)  # noqa: E731
+    sf, th = asT(t["split_feature"], torch.int64), asT(t["threshold"], torch.float64)
+    dt = asT(t["decision_type"], torch.int64)
+    lc, rc = asT(t["left_child"], torch.int64), asT(t["right_child"], torch.int64)
+    cb, ct = asT(t["cat_boundaries"], torch.int64), asT(t["cat_threshold"], torch.int64)
+    rows = torch.arange(n, device=dev)
+    node = torch.zeros(n, dtype=torch.int64, device=dev)
+    while rows.numel() > 0:
+        nd = node
+        fval = x[rows, sf[nd]]
+        d = dt[nd]
+        miss = (d >> 2) & 3
+        default_left = (d & 2) != 0
+        nan = torch.isnan(fval)
+        # numerical: NaN reads as 0 unless NaN is the missing value
+        fnum = torch.where(nan & (miss != 2), torch.zeros_like(fval), fval)
+        is_zero = (fnum >= -K_ZERO_THRESHOLD) & (fnum <= K_ZERO_THRESHOLD)
+        missing = ((miss == 1) & is_zero) | ((miss == 2) & nan)
+        go_left = torch.where(missing, default_left, fnum <= th[nd])
+        cat = (d & 1) != 0
+        if bool(cat.any()):
+            # C++ static_cast<int>: truncation toward zero; NaN and negatives go right
+            iv = torch.where(nan, torch.full_like(fval, -1.0), fval).clamp(-1.0, 2.0 ** 31 - 1).trunc().to(torch.int64)
+            ci = th[nd].to(torch.int64).clamp(min=0, max=max(cb.numel() - 2, 0))
+            lo, hi = cb[ci], cb[ci + 1]
+            word = lo + torch.div(iv.clamp(min=0), 32, rounding_mode="floor")
+            inside = (iv >= 0) & (word < hi)
+            bits = ct[word.clamp(max=max(ct.numel() - 1, 0))] if ct.numel() > 0 else torch.zeros_like(word)
+            in_set = inside & (((bits >> (iv.clamp(min=0) % 32)) & 1) == 1)
+            go_left = torch.where(cat, in_set, go_left)
+        child = torch.where(go_left, lc[nd], rc[nd])
+        done = child < 0
+        leaf[rows[done]] = ~child[done]
+        keep = ~done
+        rows = rows[keep]
+        node = child[keep]
+    return leaf
+
+
+def predict_raw(trees, x, num_tree_per_iteration=1):
+    """Raw scores (the predictor's RawScore kind, predictor.rs:162-196): for each
+    row and class k, the sum over iterations of tree (i * K + k)'s leaf value, in
+    tree order from 0.0. x: a numpy array (computed with torch on the CPU) or a
+    torch tensor (computed on its device). trees: dicts as parse_model returns
+    them or as the learner's train() returns them. Returns float64, shape
+    (rows,) for one class, else (rows, K)."""
+    import torch
+
+    as_numpy = isinstance(x, np.ndarray)
+    xt = torch.as_tensor(np.ascontiguousarray(x, np.float64)) if as_numpy else x.to(torch.float64)
+    K = int(num_tree_per_iteration)
+    out = torch.zeros((xt.shape[0], K), dtype=torch.float64, device=xt.device)
+    for i, t in enumerate(trees):
+        vals = torch.as_tensor(np.asarray(t["leaf_value"][: max(t["num_leaves"], 1)], np.float64), device=xt.device)
+        out[:, i % K] += vals[_leaf_of_rows(t, xt, torch)]
+    out = out[:, 0] if K == 1 else out
+    return out.cpu().numpy() if as_numpy else out
+
+
+def auc(label, score, weights=None):
+    """AUCMetric (metric/binary.rs:73-135): rows by descending score (ties keep row
+    order), positives and negatives accumulated per distinct score,
+    accum = madd(cur_neg, madd(cur_pos, 0.5, sum_pos), accum) (unfused, as
+    utils/common.rs:207-216 on x86_64); 1.0 when one class is absent.
+
+    Per distinct score the sums are taken in row order as the reference's loop
+    does; the running sums over the groups are sequential (np.add.accumulate), so
+    the result has the reference's rounding. Unweighted group sums are counts,
+    exact in any order; weighted ones run the reference's loop literally."""
+    label = np.asarray(label)
+    score = np.asarray(score, np.float64)
+    n = len(label)
+    order = np.argsort(-score, kind="stable")
+    s = score[order]
+    pos = label[order] > 0
+    if weights is None:
+        sum_w = float(n)
+        starts = np.flatnonzero(np.concatenate([[True], s[1:] != s[:-1]]))
+        cnt_pos = np.add.reduceat(pos.astype(np.float64), starts)
+        cnt_all = np.diff(np.concatenate([starts, [n]])).astype(np.float64)
+        cnt_neg = cnt_all - cnt_pos
+        sum_pos_before = np.concatenate([[0.0], np.add.accumulate(cnt_pos)[:-1]])
+        inner = cnt_pos * 0.5 + sum_pos_before
+        accum = float(np.add.accumulate(cnt_neg * inner)[-1]) if n else 0.0
+        sum_pos = float(np.add.accumulate(cnt_pos)[-1]) if n else 0.0
+    else:
+        w = np.asarray(weights, np.float32)[order].astype(np.float64)
+        sum_w = 0.0
+        for v in np.asarray(weights, np.float32).astype(np.float64):
+            sum_w += float(v)
+        cur_pos = sum_pos = accum = cur_neg = 0.0
+        thr = s[0] if n else 0.0
+        for i in range(n):
+            if s[i] != thr:
+                thr = s[i]
+                accum = cur_neg * (cur_pos * 0.5 + sum_pos) + accum
+                sum_pos += cur_pos
+                cur_neg = cur_pos = 0.0
+            if pos[i]:
+                cur_pos += w[i]
+            else:
+                cur_neg += w[i]
+        accum = cur_neg * (cur_pos * 0.5 + sum_pos) + accum
+        sum_pos += cur_pos
+    if sum_pos > 0.0 and sum_pos != sum_w:
+        return accum / (sum_pos * (sum_w - sum_pos))
+    return 1.0

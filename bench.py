@@ -26,45 +26,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def predict_raw(trees, x):
-    """Sum of tree outputs for raw features (numerical splits without missing values)."""
-    import numpy as np
-
-    out = np.zeros(x.shape[0])
-    rows = np.arange(x.shape[0])
-    for t in trees:
-        if t["num_leaves"] <= 1:
-            out += t["leaf_value"][0]
-            continue
-        node = np.zeros(x.shape[0], np.int64)
-        active = np.ones(x.shape[0], bool)
-        leaf = np.zeros(x.shape[0], np.int64)
-        while active.any():
-            idx = rows[active]
-            n = node[idx]
-            f = t["split_feature"][n]
-            go_left = x[idx, f] <= t["threshold"][n]
-            child = np.where(go_left, t["left_child"][n], t["right_child"][n])
-            done = child < 0
-            leaf[idx[done]] = ~child[done]
-            node[idx[~done]] = child[~done]
-            active[idx[done]] = False
-        out += t["leaf_value"][leaf]
-    return out
-
-
-def auc(y, s):
-    import numpy as np
-
-    order = np.argsort(s, kind="mergesort")
-    ys = y[order]
-    ranks = np.empty(len(s))
-    ranks[order] = np.arange(1, len(s) + 1)
-    npos = ys.sum()
-    nneg = len(ys) - npos
-    return float((ranks[y > 0].sum() - npos * (npos + 1) / 2) / (npos * nneg))
-
-
 def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     """profile: False, True (every kernel kind) or a list of kind names to time,
     on the launches of every `every_trees`-th tree."""
@@ -280,16 +241,36 @@ def traffic_from_profile(kernel):
     return None if k is None else k["bytes_per_launch"]
 
 
-def cpu_baseline(ds, y, params, budget_s=25.0):
-    """Times the oracle (C++ restatement of the reference CPU learner) on this host."""
+def host_cpus():
+    """Threads the CPU baseline uses: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when the host sets it (the GPU box gives a process its share of
+    cores that way); the CPU model string from /proc/cpuinfo."""
+    visible = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(visible, cap) if cap > 0 else visible
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, visible, model
+
+
+def cpu_baseline(ds, y, params, threads, budget_s=25.0, max_iters=5):
+    """Times the oracle (C++ restatement of the reference CPU learner) on this host
+    with `threads` OpenMP threads: whole boosting iterations (host gradients, one
+    tree, the score update) until budget_s or max_iters."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
     import boosting as bst
     import oracle_binding as ob
 
-    threads = min(os.cpu_count() or 1, 16)
     params.num_threads = threads
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     obj = bst.BinaryObjective(y)
     score = np.full(ds.num_data, obj.boost_from_score(0))
     learner = ob.OracleLearner(ds, params, False)
@@ -301,9 +282,10 @@ def cpu_baseline(ds, y, params, budget_s=25.0):
         learner.add_prediction_to_score(t["leaf_value"] * 0.1, score)
         done += 1
         el = time.perf_counter() - t0
-        if el > budget_s or done >= 5:
+        if el > budget_s or done >= max_iters:
             break
-    return done / el, threads, done, el
+    learner.close()
+    return done / el, done, el
 
 
 def main():
@@ -321,6 +303,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
+    ap.add_argument("--auc-iters", type=int, default=500,
+                    help="boosting iterations of both forms before the holdout AUC gate (0 skips it)")
     ap.add_argument("--multiclass-rows", type=int, default=2_000_000,
                     help="rows of the 10-class workload measured beside the headline (0 skips it)")
     ap.add_argument("--multiclass-steps", type=int, default=3)
@@ -412,8 +396,17 @@ def main():
     result["device_copy_GBs"] = device_copy_gbs(args.copy_mib)
 
     if args.holdout > 0:
+        import boosting as bst
+
         hx, hy = synth.binary_28(args.holdout, seed=seed, intercept=intercept, stream=3)
-        result["holdout_auc"] = auc(hy, predict_raw(qtrees, hx))
+        hxt = torch.as_tensor(hx, device="cuda:0")
+
+        def holdout_auc(trees):
+            # the host predictor and AUC pinned by the reference's fixtures
+            # (tests/test_predict_metric.py), the tree walk run on the device
+            return bst.auc(hy, bst.predict_raw(trees, hxt).cpu().numpy())
+
+        result["holdout_auc"] = holdout_auc(qtrees)
     if args.sequential_steps > 0:
         sparams = dataclasses.replace(qparams, batched=0)
         sel, _, _, _ = run_gpu(ds, y, sparams, args.sequential_steps, 1, lr, profile=False)
@@ -433,12 +426,17 @@ def main():
         result["plain_f64_batched_iters_per_sec"] = args.plain_steps / bel
         result["speedup_vs_plain_f64_batched"] = its / (args.plain_steps / bel)
         log(f"plain f64 batched: {args.plain_steps / bel:.2f} it/s")
-        # AUC at equal tree counts
-        k = len(ptrees)
-        if args.holdout > 0:
-            result["holdout_auc_at_equal_trees"] = {"trees": k,
-                                                    "quantized": auc(hy, predict_raw(qtrees[:k], hx)),
-                                                    "plain_f64": auc(hy, predict_raw(ptrees, hx))}
+        # the accuracy gate: both forms trained for the workload's full iteration
+        # count (the FP64 form with batched growth, which commits the drop-in
+        # learner's trees), holdout AUC of each
+        if args.holdout > 0 and args.auc_iters > 0:
+            _, _, _, qa = run_gpu(ds, y, qparams, args.auc_iters, 0, lr, profile=False)
+            _, _, _, pa = run_gpu(ds, y, bparams, args.auc_iters, 0, lr, profile=False)
+            aq, ap = holdout_auc(qa), holdout_auc(pa)
+            result["holdout_auc_gate"] = {"iterations": args.auc_iters, "quantized_batched": aq,
+                                          "plain_f64_batched": ap, "gap": abs(aq - ap),
+                                          "holdout_rows": args.holdout}
+            log(f"holdout AUC after {args.auc_iters} iterations: quantized {aq:.5f}, f64 {ap:.5f}")
     if args.multiclass_rows > 0:
         mq = bg.LearnerParams(num_leaves=127, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                               stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
@@ -472,10 +470,16 @@ def main():
     if not args.no_cpu_baseline:
         cparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                    stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, form=bg.FORM_QUANTIZED)
-        v, cores, done, el = cpu_baseline(ds, y, cparams)
-        result["cpu_baseline"] = {"value": v, "unit": "iter/s", "cores": cores, "kind": "port",
+        threads, visible, model = host_cpus()
+        v, done, el = cpu_baseline(ds, y, cparams, threads)
+        v1, done1, el1 = cpu_baseline(ds, y, cparams, 1, budget_s=20.0, max_iters=1)
+        result["cpu_baseline"] = {"value": v, "unit": "iter/s", "cores": threads, "kind": "port",
                                   "sample": f"{done} boosting iterations of the same workload (quantized, full "
-                                            f"{args.rows} rows) in {el:.1f}s"}
+                                            f"{args.rows} rows) in {el:.1f}s on {threads} threads",
+                                  "one_thread": {"value": v1, "unit": "iter/s",
+                                                 "sample": f"{done1} iteration(s) in {el1:.1f}s"},
+                                  "cpu_model": model, "host_cpus_visible": visible}
+        log(f"cpu baseline: {v:.3f} iter/s on {threads} threads ({model}), {v1:.3f} iter/s on 1 thread")
     print(json.dumps(result), flush=True)
 
 

@@ -339,7 +339,7 @@ def test_binary_gradients_device_match_host(case):
     import torch
     n = 200_000
     x, y = synth.binary_28(n, seed=12)
-    ds = bg.Dataset(x[:1000])  # any dataset: the kernel reads only score and label
+    ds = bg.Dataset(x)  # the entry point covers the learner's num_data rows
     learner = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
     rng = np.random.default_rng(21)
     score = np.concatenate([rng.normal(0.0, 2.0, n - 2000), rng.uniform(-40, 40, 2000)])
