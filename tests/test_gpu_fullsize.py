@@ -85,3 +85,165 @@ def test_fullsize_bagged_feature_sampled_bit_exact():
         gpu.add_out_of_bag_prediction(vals, oob, s2)
         ora.add_out_of_bag_prediction(vals, oob, score)
         assert np.array_equal(s2, score)
+
+
+TREE_KEYS = ("split_feature", "threshold_in_bin", "decision_type", "left_child", "right_child", "split_gain",
+             "leaf_value", "leaf_weight", "leaf_count", "internal_value", "internal_weight", "internal_count",
+             "cat_boundaries", "cat_threshold")
+
+
+def assert_bitwise(a, b, ctx):
+    assert a["num_leaves"] == b["num_leaves"] and a["num_cat"] == b["num_cat"], ctx
+    for k in TREE_KEYS:
+        assert a[k].tobytes() == b[k].tobytes(), (ctx, k)
+
+
+def assert_close(a, b, ctx, rtol=1e-6):
+    """The float forms' bar: the same splits, leaf values within 1e-6 relative."""
+    assert a["num_leaves"] == b["num_leaves"], ctx
+    for k in ("split_feature", "threshold_in_bin", "decision_type", "left_child", "right_child", "leaf_count",
+              "internal_count", "cat_threshold"):
+        assert np.array_equal(a[k], b[k]), (ctx, k)
+    for k in ("leaf_value", "leaf_weight", "internal_value", "internal_weight"):
+        assert np.allclose(a[k], b[k], rtol=rtol, atol=1e-12), (ctx, k)
+
+
+def quantized_params(num_leaves, seed, batched=1):
+    return bg.LearnerParams(num_leaves=num_leaves, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                            stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
+                            form=bg.FORM_QUANTIZED, batched=batched)
+
+
+def device_gradients(fn, n_out):
+    """Runs a device objective (fn(g_ptr, h_ptr)) and returns host copies."""
+    import torch
+    g = torch.empty(n_out, dtype=torch.float32, device="cuda:0")
+    h = torch.empty(n_out, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    fn(g.data_ptr(), h.data_ptr())
+    torch.cuda.synchronize()
+    return g.cpu().numpy(), h.cpu().numpy()
+
+
+def test_fullsize_multiclass_2Mx50_one_iteration():
+    """The 10-class workload at its size: 2,000,000 x 50 (40 normal columns, 10
+    categorical over 32 categories), max_bin 63, 127 leaves. One boosting
+    iteration (10 class trees) from the host softmax gradients: quantized trees
+    bit for bit against the oracle, from one learner in class order and from ten
+    concurrent learners (the bench's form); plain FP64 batched trees within the
+    float tolerance. The device softmax gradients match the host's within 2 f32
+    ulps at this size."""
+    import torch
+    print("\n[case] full size multiclass 2M x 50, 10 classes, 127 leaves: device gradients", flush=True)
+    n, K = 2_000_000, 10
+    x, y = synth.multiclass_50(n, seed=0, num_class=K)
+    ds = bg.Dataset(x, bg.DatasetParams(max_bin=63, num_threads=16), categorical=synth.MULTICLASS_CATEGORICAL)
+    del x
+    obj = bst.MulticlassObjective(y, K)
+    score = np.concatenate([np.full(n, obj.boost_from_score(k)) for k in range(K)])
+    score += np.random.default_rng(4).normal(0.0, 0.5, score.shape)  # distinct scores per class
+    g, h = obj.gradients(score)
+    probe = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
+    ts = torch.as_tensor(score, device="cuda:0")
+    tl = torch.as_tensor(y.astype(np.float32), device="cuda:0")
+    dg, dh = device_gradients(lambda gp, hp: probe.softmax_gradients_device(ts.data_ptr(), tl.data_ptr(), K, gp, hp),
+                              n * K)
+    probe.close()
+    assert np.all(np.abs(dg - g) <= 2 * np.spacing(np.abs(g)))
+    assert np.all(np.abs(dh - h) <= 2 * np.spacing(np.abs(h)))
+    for name, p, concurrent in (("quantized", quantized_params(127, 3), False),
+                                ("quantized, concurrent class learners", quantized_params(127, 3), True),
+                                ("plain f64 batched", bg.LearnerParams(num_leaves=127, min_data_in_leaf=20,
+                                                                       form=bg.FORM_PLAIN_F64, batched=1), False)):
+        print(f"[case] full size multiclass 2M x 50: {name}, 10 class trees vs oracle", flush=True)
+        ora = ob.OracleLearner(ds, p, False)
+        gpus = [bg.Learner(ds, p, False) for _ in range(K if concurrent else 1)]
+        if concurrent:
+            for k, lk in enumerate(gpus):
+                lk.set_tree_slot(k, K)
+        for k in range(K):
+            sl = slice(k * n, (k + 1) * n)
+            to = ora.train(g[sl], h[sl])
+            tg = gpus[k if concurrent else 0].train(g[sl], h[sl])
+            assert to["num_leaves"] > 100, (name, k)
+            if p.form == bg.FORM_QUANTIZED:
+                assert_bitwise(tg, to, (name, k))
+            else:
+                assert_close(tg, to, (name, k))
+        for lk in gpus:
+            lk.close()
+        ora.close()
+
+
+@pytest.fixture(scope="module")
+def ranking_5m():
+    print("\n[case] full size ranking 5M x 136: data and binning", flush=True)
+    x, y, q = synth.ranking_136(5_000_000, seed=0)
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=16))
+    del x
+    return ds, y, q
+
+
+def test_fullsize_lambdarank_gradients_5Mx136(ranking_5m):
+    """Device LambdaRank gradients on every one of the ~56,000 uneven queries of the
+    5,000,000-document workload against rank.rs restated on the host, within the
+    bound of test_lambdarank_gradients_device_match_host."""
+    import torch
+    ds, y, q = ranking_5m
+    print("[case] full size ranking 5M x 136: device vs host LambdaRank gradients", flush=True)
+    obj = bst.LambdarankObjective(y, q)
+    assert max(q) <= 2048, "a query exceeds the 2048 documents the device kernel holds in LDS"
+    score = np.random.default_rng(9).normal(0.0, 1.0, size=len(y))
+    want_g, want_h = obj.gradients(score)
+    learner = bg.Learner(ds, bg.LearnerParams(num_leaves=7), False)
+    tables, keep = bg.rank_tables_on_device(obj, torch.device("cuda:0"))
+    ts = torch.as_tensor(score, device="cuda:0")
+    tl = torch.as_tensor(y.astype(np.float32), device="cuda:0")
+    g, h = device_gradients(lambda gp, hp: learner.lambdarank_gradients_device(ts.data_ptr(), tl.data_ptr(), tables,
+                                                                               gp, hp), len(y))
+    learner.close()
+    b = obj.bounds
+    for lo, hi in zip(b[:-1], b[1:]):
+        sg = np.max(np.abs(want_g[lo:hi])) + 1e-30
+        sh = np.max(np.abs(want_h[lo:hi])) + 1e-30
+        assert np.all(np.abs(g[lo:hi] - want_g[lo:hi]) <= 1e-5 * sg + 4 * np.spacing(np.abs(want_g[lo:hi])))
+        assert np.all(np.abs(h[lo:hi] - want_h[lo:hi]) <= 1e-5 * sh + 4 * np.spacing(np.abs(want_h[lo:hi])))
+
+
+@pytest.mark.parametrize("form", [bg.FORM_QUANTIZED, bg.FORM_PLAIN_F64])
+def test_fullsize_lambdarank_trees_5Mx136(ranking_5m, form):
+    """Two boosting iterations of the ranking workload at its size (5,000,000 x 136,
+    255 leaves): each tree grown by the GPU learner (batched) and by the oracle on
+    the same gradients (the device LambdaRank gradients of the current score,
+    copied to the host), quantized bit for bit, plain FP64 within the float
+    tolerance; the score update of the GPU learner equals the oracle's."""
+    import torch
+    ds, y, q = ranking_5m
+    name = "quantized" if form == bg.FORM_QUANTIZED else "plain f64 batched"
+    obj = bst.LambdarankObjective(y, q)
+    p = quantized_params(255, 5) if form == bg.FORM_QUANTIZED else \
+        bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, form=bg.FORM_PLAIN_F64, batched=1)
+    gpu = bg.Learner(ds, p, False)
+    ora = ob.OracleLearner(ds, p, False)
+    tables, keep = bg.rank_tables_on_device(obj, torch.device("cuda:0"))
+    tl = torch.as_tensor(y.astype(np.float32), device="cuda:0")
+    score = np.zeros(len(y))
+    for it in range(2):
+        print(f"[case] full size ranking 5M x 136 {name}: tree {it}", flush=True)
+        ts = torch.as_tensor(score, device="cuda:0")
+        g, h = device_gradients(lambda gp, hp: gpu.lambdarank_gradients_device(ts.data_ptr(), tl.data_ptr(), tables,
+                                                                               gp, hp), len(y))
+        to = ora.train(g, h)
+        tg = gpu.train(g, h)
+        assert to["num_leaves"] == 255
+        if form == bg.FORM_QUANTIZED:
+            assert_bitwise(tg, to, (name, it))
+        else:
+            assert_close(tg, to, (name, it))
+        vals = to["leaf_value"] * 0.1
+        s2 = score.copy()
+        gpu.add_prediction_to_score(vals, s2)
+        ora.add_prediction_to_score(vals, score)
+        assert np.array_equal(s2, score)
+    gpu.close()
+    ora.close()
