@@ -176,6 +176,7 @@ EXPORTS = {
     "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
     "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
+    "bk_learner_root_histogram_noalloc": ([_P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
                               C.c_int),
@@ -606,6 +607,12 @@ class Learner:
             gh = np.zeros((tb, 2), np.float64)
             _check(self._lib.bk_learner_root_histogram(self.handle, _ptr(g), _ptr(h), _ptr(gh), None, _ptr(cnt)))
         return gh, cnt
+
+    def root_histogram_noalloc(self, d_gradients_ptr, d_hessians_ptr, d_hist_ptr, d_count_ptr):
+        """bk_learner_root_histogram_noalloc: device inputs and outputs (raw pointers)."""
+        _check(self._lib.bk_learner_root_histogram_noalloc(self.handle, C.c_void_p(d_gradients_ptr),
+                                                           C.c_void_p(d_hessians_ptr), C.c_void_p(d_hist_ptr),
+                                                           C.c_void_p(d_count_ptr)))
 
     def set_hist_tiling(self, min_rows=0, max_blocks=0):
         """Forces the histogram tiling (0, 0: the autotuner's choice)."""

@@ -340,6 +340,15 @@ void launch_lambdarank_gradients(const double* score, const float* label, const 
                                  void* stream);
 // Binary log-loss (objective/binary.rs:71-90): label weights lw_neg / lw_pos,
 // optional sample weights (null: none).
+// Scratch memory of the allocation-free entry point: a file-scope device array
+// of kScratchBytes in kernels.hip. scratch_device_ptr() returns its address on
+// the current device (looked up once per device with hipGetSymbolAddress); every
+// caller on one device shares it.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+void* scratch_device_ptr();
+// hist[2 * b + 1] *= (double)hessians[0] for b < total_bins (a constant hessian
+// counts rows; construct_histograms scales them, serial.rs:982-988).
+void launch_scale_const_hess(double* hist, int total_bins, const float* hessians, void* stream);
 void launch_binary_gradients(const double* score, const float* label, const float* weights, int n, double sigmoid,
                              double lw_neg, double lw_pos, float* g, float* h, void* stream);
 

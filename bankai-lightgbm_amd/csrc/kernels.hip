@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 
 #include <cfloat>
 #include <cmath>
@@ -2562,6 +2563,40 @@ void launch_binary_gradients(const double* score, const float* label, const floa
                              double lw_neg, double lw_pos, float* g, float* h, void* stream) {
   launch(k_binary_gradients, dim3(grid_for(n, 256, 4096)), dim3(256), 0, (hipStream_t)stream, score, label, weights,
          n, sigmoid, lw_neg, lw_pos, g, h);
+  BK_HIP(hipGetLastError());
+}
+
+// The allocation-free entry point's scratch (partial histograms): one array per
+// device, held until the process ends. Host code never names it directly: its
+// name is a host-side stand-in, so every use goes through scratch_device_ptr().
+__device__ __align__(256) static unsigned char scratch[kScratchBytes];
+
+void* scratch_device_ptr() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* ptr[kMaxDevices] = {};
+  int dev = -1;
+  BK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    BK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
+__global__ void k_scale_const_hess(double* __restrict__ hist, int total_bins, const float* __restrict__ hessians) {
+  const double h0 = (double)hessians[0];
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < total_bins; b += gridDim.x * blockDim.x)
+    hist[2 * b + 1] = hist[2 * b + 1] * h0;
+}
+
+void launch_scale_const_hess(double* hist, int total_bins, const float* hessians, void* stream) {
+  launch(k_scale_const_hess, dim3(grid_for(total_bins, 256, 1024)), dim3(256), 0, (hipStream_t)stream, hist,
+         total_bins, hessians);
   BK_HIP(hipGetLastError());
 }
 

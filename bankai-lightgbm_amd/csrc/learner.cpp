@@ -1202,6 +1202,44 @@ struct Learner {
     timed(4, db, db, [&] { launch_gather_bag(ds, d_rows, nrows, 1, d_bag_bits, d_sums3, stream); });
   }
 
+  // The discretizer (gradient_discretizer/mod.rs:134-201) into buffer 0's payload:
+  // scales from every row, one rotation draw of the noise per tree, the packed
+  // int8 pairs, and their exact sums in d_sums3. Returns false when the
+  // reference refuses to discretize (non-finite or all-zero gradients), which
+  // keeps the previous discretization (serial.rs:692-703).
+  bool discretize(const float* g, const float* h) {
+    BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
+    launch_max_abs(g, h, N, d_max4, stream);
+    unsigned mx[4];
+    launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
+    sync_spin();
+    std::memcpy(mx, h_misc, sizeof mx);
+    float max_g, max_h;
+    std::memcpy(&max_g, &mx[0], 4);
+    std::memcpy(&max_h, &mx[1], 4);
+    const bool ok = mx[2] == 0u && max_g != 0.0f && (const_hess || max_h != 0.0f);
+    if (!ok) return false;
+    grad_scale = (double)max_g / (double)(bins / 2);
+    hess_scale = const_hess ? (double)max_h : (double)max_h / (double)bins;
+    double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
+    // the rotation of the noise: one draw per tree; a learner that trains tree
+    // tree_slot of every tree_period trees of a shared sequence draws for all of
+    // them and keeps its own (serial.rs:692-703 draws only when discretizing)
+    int start = 0;
+    if (stochastic)
+      for (int j = 0; j < tree_period; ++j) {
+        const int v = (int)use_start_eng.uint_incl((uint32_t)N);
+        if (j == tree_slot) start = v;
+      }
+    BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
+    const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
+    timed(4, db, db, [&] {
+      launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0, stochastic ? 1 : 0,
+                        (int16_t*)d_gh0, d_sums3, stream);
+    });
+    return true;
+  }
+
   // Discretization and root sums (serial.rs:658-776, gradient_discretizer/mod.rs:134-201).
   // d_rows: the tree's bag on the device (ascending rows), or null for every row.
   // The discretizer's scales always come from every row; the root sums run over
@@ -1225,36 +1263,7 @@ struct Learner {
     double sg = 0.0, sh = 0.0;
     long long isum = 0;
     if (quant) {
-      BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
-      launch_max_abs(g, h, N, d_max4, stream);
-      unsigned mx[4];
-      launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
-      sync_spin();
-      std::memcpy(mx, h_misc, sizeof mx);
-      float max_g, max_h;
-      std::memcpy(&max_g, &mx[0], 4);
-      std::memcpy(&max_h, &mx[1], 4);
-      bool ok = mx[2] == 0u && max_g != 0.0f && (const_hess || max_h != 0.0f);
-      if (ok) {
-        grad_scale = (double)max_g / (double)(bins / 2);
-        hess_scale = const_hess ? (double)max_h : (double)max_h / (double)bins;
-        double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
-        // the rotation of the noise: one draw per tree; a learner that trains tree
-        // tree_slot of every tree_period trees of a shared sequence draws for all of
-        // them and keeps its own (serial.rs:692-703 draws only when discretizing)
-        int start = 0;
-        if (stochastic)
-          for (int j = 0; j < tree_period; ++j) {
-            const int v = (int)use_start_eng.uint_incl((uint32_t)N);
-            if (j == tree_slot) start = v;
-          }
-        BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
-        const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
-        timed(4, db, db, [&] {
-          launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0,
-                            stochastic ? 1 : 0, (int16_t*)d_gh0, d_sums3, stream);
-        });
-      }
+      const bool ok = discretize(g, h);
       // the bag's sums come from the packed pairs in place, fresh or kept from the
       // previous tree when this discretization was refused; kept pairs of a bagged
       // tree are summed again over every row when this tree has no bag
@@ -2292,6 +2301,62 @@ struct Learner {
     if (out_cnt) std::memcpy(out_cnt, cnt.data(), sizeof(int32_t) * cnt.size());
   }
 
+  // root_histogram on device arrays, allocating nothing: the discretization (or
+  // the float payload) goes to buffer 0 as in train, the partial histograms to
+  // the file-scope scratch array, and the reduction straight into the caller's
+  // d_hist (2 x total_bins f64, or int64 when quantized) and d_count (total_bins
+  // int32). No host containers either: the plan is one item over all rows. Waits
+  // for the stream before returning. Calls that overlap on one device race on the
+  // scratch.
+  void root_histogram_noalloc(const float* g, const float* h, void* d_hist, int32_t* d_count) {
+    if (!g || !h || !d_hist || !d_count) throw std::invalid_argument("null argument");
+    // partials of this tiling: at least one block per rows_per_flush rows (one
+    // LDS accumulation per slot), at most what the scratch holds
+    const size_t per_partial = (size_t)TB * (form == FORM_QUANT ? 8 : 20);
+    const long long cap = (long long)(kScratchBytes / per_partial);
+    const long long need = ((long long)N + rows_per_flush - 1) / rows_per_flush;
+    if (need > cap || need > kHistMaxBlocks)
+      throw std::invalid_argument("root_histogram_noalloc: " + std::to_string(need) + " partial histograms of " +
+                                  std::to_string(per_partial) + " bytes exceed the " +
+                                  std::to_string(kScratchBytes >> 20) + " MiB scratch");
+    long long nb = std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows));
+    nb = std::max(std::min(nb, std::min<long long>(cap, kHistMaxBlocks)), need);
+    const int rpb = (int)(((long long)N + nb - 1) / nb);
+    const int nblocks = (int)(((long long)N + rpb - 1) / rpb);
+    if (quant) {
+      discretize(g, h);
+    } else {
+      launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
+    }
+    set_groups_used(true);
+    begin_uploads();
+    h_hitems[0] = HistItem{Segment{0, 0, N}, 0, 0};
+    for (int b = 0; b < nblocks; ++b) h_hblock[b] = b;
+    h_ritems[0] = ReduceItem{0, 0, nblocks, 0};
+    d_hitems = put(h_hitems, 1);
+    d_hblock = put(h_hblock, (size_t)nblocks);
+    d_ritems = put(h_ritems, 1);
+    send(true);
+    DevState s = ds;
+    uint8_t* sc = (uint8_t*)scratch_device_ptr();
+    const size_t plane = (size_t)nblocks * TB;
+    if (quant) {
+      s.part_q = (unsigned long long*)sc;
+    } else {
+      s.part_g = (double*)sc;
+      s.part_h = s.part_g + plane;
+      s.part_c = (unsigned*)(s.part_h + plane);
+    }
+    s.pool = (double*)d_hist;  // slot 0 of this view is the caller's output
+    s.pool_cnt = d_count;
+    launch_hist(s, d_hitems, 1, N, nblocks, rpb, d_hblock, d_group_used, max_tile_bins, d_tile_first,
+                (int)tile_first.size() - 1, rows_per_flush, stream);
+    launch_reduce(s, d_ritems, 1, stream);
+    if (!quant && const_hess) launch_scale_const_hess((double*)d_hist, TB, h, stream);
+    sync();
+    if (cols.per_tree) set_groups_used(false);
+  }
+
   // Rows of a leaf in ascending order (the order of the reference's partition),
   // read off the row-to-leaf map.
   void leaf_rows(int leaf, int32_t* out, int64_t* n) {
@@ -2447,6 +2512,14 @@ int bk_learner_root_histogram(bk_learner* l, const float* g, const float* h, dou
     BK_HIPC(hipMemcpyAsync(L.d_grad, g, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
     BK_HIPC(hipMemcpyAsync(L.d_hess, h, sizeof(float) * (size_t)L.N, hipMemcpyHostToDevice, L.stream));
     L.root_histogram(L.d_grad, L.d_hess, hist_out, int_hist_out, count_out);
+  });
+}
+
+int bk_learner_root_histogram_noalloc(bk_learner* l, const float* d_gradients, const float* d_hessians,
+                                      void* d_hist_out, int32_t* d_count_out) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    l->L.root_histogram_noalloc(d_gradients, d_hessians, d_hist_out, d_count_out);
   });
 }
 

@@ -252,6 +252,22 @@ int bk_learner_leaf_rows(bk_learner* learner, int32_t leaf, int32_t* rows_out,
 int bk_learner_root_histogram(bk_learner* learner, const float* gradients, const float* hessians,
                               double* hist_out, int64_t* int_hist_out, int32_t* count_out);
 
+/* bk_learner_root_histogram without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. Inputs are
+ * device arrays of num_data gradients and hessians; outputs are device arrays the
+ * caller allocated: d_hist_out holds 2 x total_bins values (double for the float
+ * forms, int64_t for the quantized form, laid out as bk_learner_root_histogram's
+ * outputs) and d_count_out total_bins int32 counts. The partial histograms live in
+ * a file-scope 16 MiB device array shared by every call on a device; a call whose
+ * partials cannot fit (total_bins x 8 bytes quantized, x 20 bytes float, times
+ * ceil(num_data / rows of one LDS accumulation)) fails with an error that says so.
+ * Calls that overlap on one device race on that array. The call runs on the
+ * learner's stream and returns when the outputs are written; the caller's inputs
+ * must be ready on the device before the call. */
+int bk_learner_root_histogram_noalloc(bk_learner* learner, const float* d_gradients, const float* d_hessians,
+                                      void* d_hist_out, int32_t* d_count_out);
+
 /* SerialTreeLearner::reset_constant_hessian (crates/lgbm-core/src/treelearner/serial.rs:279). */
 int bk_learner_reset_constant_hessian(bk_learner* learner, int is_constant_hessian);
 
