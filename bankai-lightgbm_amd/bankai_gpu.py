@@ -135,7 +135,7 @@ class KernelStatsC(C.Structure):
     _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("algo_bytes", C.c_double), ("moved_bytes", C.c_double)]
 
 
-KERNEL_KINDS = ("root_hist", "child_hist", "partition", "search", "discretize", "renew", "score_gradients")
+KERNEL_KINDS = ("root_hist", "child_hist", "partition", "search", "discretize", "renew", "score_gradients", "plan")
 
 
 class DatasetParamsC(C.Structure):
@@ -176,6 +176,7 @@ EXPORTS = {
     "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
     "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
+    "bk_learner_set_round_planner": ([_P, C.c_int], C.c_int),
     "bk_learner_root_histogram_noalloc": ([_P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
     "bk_dataset_from_dense": ([_P, C.c_int64, C.c_int32, C.POINTER(DatasetParamsC), _P, C.c_int32, C.POINTER(_P)],
@@ -613,6 +614,10 @@ class Learner:
         _check(self._lib.bk_learner_root_histogram_noalloc(self.handle, C.c_void_p(d_gradients_ptr),
                                                            C.c_void_p(d_hessians_ptr), C.c_void_p(d_hist_ptr),
                                                            C.c_void_p(d_count_ptr)))
+
+    def set_round_planner(self, on_device):
+        """Batched growth planned on the device (True, the default) or the host."""
+        _check(self._lib.bk_learner_set_round_planner(self.handle, int(bool(on_device))))
 
     def set_hist_tiling(self, min_rows=0, max_blocks=0):
         """Forces the histogram tiling (0, 0: the autotuner's choice)."""

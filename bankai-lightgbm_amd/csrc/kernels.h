@@ -231,6 +231,51 @@ struct ScanItem {
   int has_large, use_subtract, used_off, out_off;
 };
 
+// ---------------------------------------------------------------------------
+// Device round planner (batched growth without a host round trip per round)
+// ---------------------------------------------------------------------------
+// Work of one planned round, written by k_plan and read by the round's kernels,
+// which are launched with grids sized for the most work a round can hold: blocks
+// beyond the planned work exit at once.
+struct RoundHdr {
+  int npart, ntasks;                              // partition items and their tiles
+  int nhist, hist_total, hist_nblocks, hist_rpb;  // histogram items, rows, row blocks, rows per block
+  int nscan;                                      // search items (one reduce item each)
+  int pad;
+};
+
+constexpr int kPlanMaxLeaves = 256;  // num_leaves the device planner takes (more: the host plans)
+
+// A node of the batched growth: a leaf that was or may be expanded. Its children
+// are nodes child[0] (left) and child[1] (right) once it is expanded.
+struct PlanNode {
+  Segment seg;
+  int depth, slot, expanded, kids_searched;
+  int child[2];
+  int num_data, pad;
+  double sum_g, sum_h, weight;  // the leaf's sums and output (LeafSplits)
+  long long int_sum;
+  FoldOut best;                 // its best split (inner < 0: none)
+};
+
+// Tree state of the planner, in device memory.
+struct PlanState {
+  int num_leaves, num_nodes, next_slot, done;
+  int ncommit, nabsorb, rounds, pad;  // rounds: with work; pad: set on a table overflow
+  int search_rounds, pad2;            // rounds with a search
+  long long hist_rows, part_rows;     // rows the round kernels processed
+  RoundHdr hdr;
+  int absorb[2 * kPlanMaxLeaves];     // search item k of the last round: nodes of its smaller and larger side
+  int leaf_node[kPlanMaxLeaves];      // node of every leaf
+  int commit[2 * kPlanMaxLeaves];     // committed split c: (leaf, node), in the reference's order
+};
+
+struct PlanParams {
+  int num_leaves, F, max_depth, min_data_in_leaf;
+  int hist_min_rows, hist_max_blocks, rows_per_flush, part_tile;
+  int max_nodes, num_slots, max_partials, pad1;
+};
+
 // Kernel timing. While a source is set (per host thread), every launch below takes
 // a fresh (start, stop) pair of timing events (hipEvent_t as void*) from it and is
 // launched with hipExtLaunchKernel, which stamps the kernel's own begin and end
@@ -245,9 +290,13 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
 size_t hist_lds_bytes(int form, int max_tile_bins);
 // Block b covers concatenated rows [b * rows_per_block, (b + 1) * rows_per_block)
 // and writes one partial histogram per item it touches, from slot d_block_slot[b] on.
+// With hdr (a planned round): nitems, total_rows, nblocks and rows_per_block are
+// read from it, the arguments give the grid (nblocks: the most row blocks), and
+// the partial of (block b, item i) is slot b + i (d_block_slot unused).
 void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
-                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream);
+                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream,
+                 const RoundHdr* hdr = nullptr);
 // Rows per partition tile for a record stride (2048 16-byte chunks per tile).
 int partition_tile_rows(int stride);
 // Single-pass partition of every item (row order kept within a tile, not across
@@ -255,17 +304,27 @@ int partition_tile_rows(int stride);
 // d_cursors: 2 banks x max_items x (left, right) u32, zeroed before the first
 // launch; a launch uses bank (epoch & 1) and zeroes the other, so consecutive
 // launches on one stream alternate epoch parity.
+// With hdr: nitems and ntasks are read from it; the arguments give the grid.
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                      int max_items, unsigned epoch, void* stream);
+                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr);
 // Sums the partial histograms of every item into its pool slot; the search then
 // reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
 // from its slot.
-void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream);
+void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream,
+                   const RoundHdr* hdr = nullptr);
 // Split search of every (feature, item), then the fold of the best candidate of
 // every (item, side) into d_best[2 * item + side] and of every feature's
 // splittable flag into d_flags[(2 * item + side) * F + f].
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream);
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr = nullptr);
+// One round of planning: absorbs the last round's folded splits into their
+// nodes, replays the reference's best-first commits while the best leaf is
+// expanded, picks the next frontier (the best unexpanded leaves with positive
+// gain, at most the remaining leaf budget) and writes the round's items and
+// st->hdr. A complete tree sets st->done and plans no work.
+void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const FoldOut* fold, const uint8_t* fold_flags,
+                 const FeatDev* feats, const int* bytree, const PlanParams& pp, PartItem* pitems, HistItem* hitems,
+                 ScanItem* sitems, ReduceItem* ritems, int* used, void* stream);
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,

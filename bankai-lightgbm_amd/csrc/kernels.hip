@@ -373,9 +373,15 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
                                                        const int* __restrict__ block_slot,
                                                        const int* __restrict__ group_used,
                                                        const int* __restrict__ tile_first, int rows_per_flush,
-                                                       int nblocks, int ntiles) {
+                                                       int nblocks, int ntiles, const RoundHdr* __restrict__ hdr) {
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
+  if (hdr) {  // a planned round: the work comes from the planner, the grid covers the most it can hold
+    nitems = hdr->nhist;
+    total_rows = hdr->hist_total;
+    rows_per_block = hdr->hist_rpb;
+    nblocks = hdr->hist_nblocks;
+  }
   // Workgroups go to the XCDs round robin (workgroup i to XCD i % 8). The
   // group tiles of one row block are given workgroups i, i + 8, ... so they
   // run on one XCD one after the other and the later tiles read the rows from
@@ -383,6 +389,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   const int xcd = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
   const int rb = (k / ntiles) * kXcds + xcd, tile = k % ntiles;
   if (rb >= nblocks) return;  // block-uniform
+  if (total_rows <= 0) return;  // block-uniform
   const long long start = (long long)rb * rows_per_block;
   const long long end = min(start + (long long)rows_per_block, (long long)total_rows);
   if (start >= end) return;  // block-uniform
@@ -416,7 +423,9 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
     }
     i = lo;
   }
-  int slot = block_slot[rb];
+  // partial of (block, item): from the host's slot table, or block + item in a
+  // planned round (distinct, and consecutive over an item's blocks)
+  int slot = block_slot ? block_slot[rb] : rb + i;
   for (long long x = start; x < end; ++i, ++slot) {
     const HistItem it = items[i];
     const Segment seg = it.seg;
@@ -506,7 +515,8 @@ __host__ __device__ inline size_t partition_lds(int R, int stride) {
 template <int FORM>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
                                                             int nitems, unsigned* __restrict__ cursors,
-                                                            int max_items, unsigned epoch) {
+                                                            int max_items, unsigned epoch,
+                                                            const RoundHdr* __restrict__ hdr) {
   __shared__ int sh_lbase, sh_rbase;
   __shared__ int scratch[32];
   extern __shared__ __align__(16) unsigned char smem[];
@@ -520,6 +530,10 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   // the cursors of the next launch start at zero
   unsigned* other = cursors + (size_t)((epoch + 1u) & 1u) * max_items * 2;
   for (int i = t * kPartThreads + (int)threadIdx.x; i < 2 * max_items; i += (int)gridDim.x * kPartThreads) other[i] = 0u;
+  if (hdr) {  // a planned round: the grid covers the most tiles a round can hold
+    nitems = hdr->npart;
+    if (t >= hdr->ntasks) return;  // block-uniform
+  }
   // the items' first tiles go to LDS: the item search then costs one global load
   // latency instead of one per bisection step
   __shared__ int sh_first[kPartSearchItems];
@@ -1568,7 +1582,9 @@ __device__ void fold_side(const ScanItem& it, int k, int side, const SplitOut* o
 
 template <int FORM>
 __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanItem* __restrict__ items, ScanParams p,
-                                                       const int* __restrict__ used_all, SplitOut* out_all) {
+                                                       const int* __restrict__ used_all, SplitOut* out_all,
+                                                       const RoundHdr* __restrict__ hdr) {
+  if (hdr && (int)blockIdx.y >= hdr->nscan) return;  // block-uniform
   __shared__ SliceLds S[2];
   __shared__ ScanPrefix X[2];
   __shared__ int elig[2][kMaxBins];
@@ -1637,9 +1653,289 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
 // candidates it reads (a kernel boundary orders them across the XCDs' L2s).
 __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
                                              const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
-                                             uint8_t* __restrict__ flags) {
+                                             uint8_t* __restrict__ flags, const RoundHdr* __restrict__ hdr) {
   const int k = blockIdx.x >> 1, side = blockIdx.x & 1;
+  if (hdr && k >= hdr->nscan) return;
   fold_side(items[k], k, side, out_all, used_all, F, best, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Device round planner: the host loop of batched growth, one block per round
+// (see launch_plan). Leaves' best splits are ranked by better_than
+// (split_info.rs:67-85) with the leaf index last, which is argmax_split's first
+// maximum (split_info.rs:88-96) and the stable order of the frontier sort.
+// ---------------------------------------------------------------------------
+constexpr int kPlanThreads = 256;
+static_assert(kPlanMaxLeaves <= kPlanThreads, "one thread per frontier item");
+
+__device__ __forceinline__ bool plan_better(double ga, int ka, int la, double gb, int kb, int lb) {
+  if (ga != gb) return ga > gb;
+  if (ka != kb) return ka < kb;
+  return la < lb;
+}
+
+__device__ __forceinline__ int plan_key(const SplitOut& o) { return o.feature == -1 ? INT_MAX : o.feature; }
+
+__global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ st, PlanNode* __restrict__ nodes,
+                                                       uint8_t* __restrict__ node_flags,
+                                                       const FoldOut* __restrict__ fold,
+                                                       const uint8_t* __restrict__ fold_flags,
+                                                       const FeatDev* __restrict__ feats,
+                                                       const int* __restrict__ bytree, PlanParams pp,
+                                                       PartItem* __restrict__ pitems, HistItem* __restrict__ hitems,
+                                                       ScanItem* __restrict__ sitems, ReduceItem* __restrict__ ritems,
+                                                       int* __restrict__ used) {
+  __shared__ double lgain[kPlanMaxLeaves];
+  __shared__ int lkey[kPlanMaxLeaves], lnode[kPlanMaxLeaves], order[kPlanMaxLeaves];
+  __shared__ unsigned char lexp[kPlanMaxLeaves];
+  __shared__ int sh_nl, sh_done, sh_ncand;
+  __shared__ int scratch[32];
+  const int tid = threadIdx.x, F = pp.F;
+  if (st->done) {  // block-uniform: the tree is complete, no work
+    if (tid == 0) {
+      st->hdr = RoundHdr{};
+      st->nabsorb = 0;
+    }
+    return;
+  }
+  // (1) the last round's folded splits go to the nodes that were searched
+  const int na = st->nabsorb;
+  constexpr int kWords = (int)(sizeof(FoldOut) / 4);
+  for (int i = tid; i < na * 2 * kWords; i += kPlanThreads) {
+    const int k = i / (2 * kWords), r = i - k * 2 * kWords, side = r / kWords, w = r - side * kWords;
+    const int nd = st->absorb[2 * k + side];
+    if (nd >= 0) ((uint32_t*)&nodes[nd].best)[w] = ((const uint32_t*)(fold + 2 * k + side))[w];
+  }
+  for (int i = tid; i < na * 2 * F; i += kPlanThreads) {
+    const int k = i / (2 * F), r = i - k * 2 * F, side = r / F, f = r - side * F;
+    const int nd = st->absorb[2 * k + side];
+    if (nd >= 0) node_flags[(size_t)nd * F + f] = fold_flags[(size_t)(2 * k + side) * F + f];
+  }
+  __syncthreads();
+  // (2) the leaves' best splits, then the commits in the reference's order while
+  // the best leaf has been expanded (one wave)
+  int nl = st->num_leaves;
+  for (int l = tid; l < nl; l += kPlanThreads) {
+    const int x = st->leaf_node[l];
+    lnode[l] = x;
+    lgain[l] = nodes[x].best.o.gain;
+    lkey[l] = plan_key(nodes[x].best.o);
+    lexp[l] = (unsigned char)nodes[x].expanded;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int lane = tid;
+    int done = 0, ncommit = st->ncommit;
+    for (;;) {
+      if (nl >= pp.num_leaves) {
+        done = 1;
+        break;
+      }
+      bool nan = false;
+      for (int l = lane; l < nl; l += 64) nan |= lgain[l] != lgain[l];
+      int bl;
+      if (__ballot(nan) != 0ull) {
+        // a NaN gain is never better nor worse: the reference's plain scan decides
+        int b = 0;
+        if (lane == 0)
+          for (int l = 1; l < nl; ++l) {
+            const double x = lgain[l], y = lgain[b];
+            if (x != y ? x > y : lkey[l] < lkey[b]) b = l;
+          }
+        bl = __shfl(b, 0, 64);
+      } else {
+        double bg = 0.0;
+        int bk = INT_MAX, bi = INT_MAX;
+        for (int l = lane; l < nl; l += 64)
+          if (bi == INT_MAX || plan_better(lgain[l], lkey[l], l, bg, bk, bi)) {
+            bg = lgain[l];
+            bk = lkey[l];
+            bi = l;
+          }
+        for (int o = 32; o > 0; o >>= 1) {
+          const double og = __shfl_xor(bg, o, 64);
+          const int ok = __shfl_xor(bk, o, 64), oi = __shfl_xor(bi, o, 64);
+          if (oi != INT_MAX && (bi == INT_MAX || plan_better(og, ok, oi, bg, bk, bi))) {
+            bg = og;
+            bk = ok;
+            bi = oi;
+          }
+        }
+        bl = bi;
+      }
+      if (lgain[bl] <= 0.0) {
+        done = 1;
+        break;
+      }
+      if (!lexp[bl]) break;
+      const int x = lnode[bl], right = nl;
+      const int c0 = nodes[x].child[0], c1 = nodes[x].child[1];
+      if (lane == 0) {
+        st->commit[2 * ncommit] = bl;
+        st->commit[2 * ncommit + 1] = x;
+        st->leaf_node[bl] = c0;
+        st->leaf_node[right] = c1;
+        lnode[bl] = c0;
+        lnode[right] = c1;
+        lgain[bl] = nodes[c0].best.o.gain;
+        lkey[bl] = plan_key(nodes[c0].best.o);
+        lexp[bl] = (unsigned char)nodes[c0].expanded;
+        lgain[right] = nodes[c1].best.o.gain;
+        lkey[right] = plan_key(nodes[c1].best.o);
+        lexp[right] = (unsigned char)nodes[c1].expanded;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      ++ncommit;
+      ++nl;
+    }
+    if (lane == 0) {
+      st->ncommit = ncommit;
+      st->num_leaves = nl;
+      sh_done = done;
+      sh_nl = nl;
+    }
+  }
+  __syncthreads();
+  nl = sh_nl;
+  // (3) the frontier: unexpanded leaves with positive gain, best first
+  int cand = 0;
+  if (!sh_done)
+    for (int l = tid; l < nl; l += kPlanThreads) {
+      if (lexp[l] || !(lgain[l] > 0.0)) continue;
+      int r = 0;
+      for (int m = 0; m < nl; ++m)
+        if (m != l && !lexp[m] && lgain[m] > 0.0 && plan_better(lgain[m], lkey[m], m, lgain[l], lkey[l], l)) ++r;
+      order[r] = l;
+      ++cand;
+    }
+  const int ncand = block_sum<kPlanThreads>(cand, scratch);
+  const int ni = sh_done ? 0 : min(ncand, pp.num_leaves - nl);
+  // (4) the round's items, one thread per expanded leaf: partition item, two
+  // child nodes, and when the children are searchable (before_find_best_split's
+  // max_depth and min_data_in_leaf rules, serial.rs:779-799) a search item for
+  // the pair, a histogram item for the smaller child (its rows are contiguous
+  // after the partition) and its reduce item
+  const int j = tid;
+  int x = -1, lc = 0, rc = 0, depth = 0, tiles = 0, srch = 0, hrows = 0, cnt = 0;
+  Segment seg{0, 0, 0};
+  if (j < ni) {
+    x = lnode[order[j]];
+    seg = nodes[x].seg;
+    lc = nodes[x].best.o.exact_left;
+    rc = nodes[x].best.o.exact_right;
+    depth = nodes[x].depth + 1;
+    cnt = seg.count;
+    tiles = cnt > 0 ? (cnt + pp.part_tile - 1) / pp.part_tile : 0;
+    srch = !(pp.max_depth > 0 && depth >= pp.max_depth) &&
+           !(rc < pp.min_data_in_leaf * 2 && lc < pp.min_data_in_leaf * 2);
+    hrows = srch ? min(lc, rc) : 0;
+  }
+  int first_task, sidx, ntasks, nscan, row0, hidx, htotal, nhist;
+  block_scan2<kPlanThreads>(tiles, srch, &first_task, &sidx, &ntasks, &nscan, scratch);
+  block_scan2<kPlanThreads>(hrows, hrows > 0 ? 1 : 0, &row0, &hidx, &htotal, &nhist, scratch);
+  const int part_rows = block_sum<kPlanThreads>(cnt, scratch);
+  // histogram tiling of the round: as the host plans it
+  int nblocks = 0, rpb = 0;
+  if (htotal > 0) {
+    long long nb = min((long long)pp.hist_max_blocks, max(1ll, (long long)htotal / pp.hist_min_rows));
+    nb = max(nb, ((long long)htotal + pp.rows_per_flush - 1) / pp.rows_per_flush);
+    rpb = (int)(((long long)htotal + nb - 1) / nb);
+    nblocks = (htotal + rpb - 1) / rpb;
+  }
+  const int num_nodes = st->num_nodes, next_slot = st->next_slot;
+  // tables too small for this round (they are sized so that this cannot happen):
+  // nothing is written, the tree stops and the host reports it
+  if (num_nodes + 2 * ni > pp.max_nodes || next_slot + nscan > pp.num_slots || nblocks + nhist > pp.max_partials) {
+    if (tid == 0) {
+      st->done = 1;
+      st->pad = 1;
+      st->nabsorb = 0;
+      st->hdr = RoundHdr{};
+    }
+    return;
+  }
+  if (j < ni) {
+    const FoldOut& B = nodes[x].best;
+    const SplitOut& o = B.o;
+    const int dst = seg.buf == 1 ? 2 : 1;
+    const int base = num_nodes + 2 * j;
+    const bool left_small = lc < rc;
+    const int sm_slot = srch ? next_slot + sidx : -1;
+    const int lg_slot = srch ? nodes[x].slot : -1;
+    for (int side = 0; side < 2; ++side) {
+      PlanNode& c = nodes[base + side];
+      c.seg = side == 0 ? Segment{dst, seg.begin, lc} : Segment{dst, seg.begin + lc, rc};
+      c.depth = depth;
+      c.slot = ((side == 0) == left_small) ? sm_slot : lg_slot;
+      c.expanded = 0;
+      c.kids_searched = 0;
+      c.child[0] = c.child[1] = -1;
+      c.num_data = side == 0 ? lc : rc;
+      c.pad = 0;
+      c.sum_g = side == 0 ? o.left_sum_gradient : o.right_sum_gradient;
+      c.sum_h = side == 0 ? o.left_sum_hessian : o.right_sum_hessian;
+      c.weight = side == 0 ? o.left_output : o.right_output;
+      c.int_sum = side == 0 ? o.left_int_sum : o.right_int_sum;
+      c.best.o = SplitOut{};
+      c.best.o.gain = kMinScore;
+      c.best.o.feature = -1;
+      c.best.o.default_left = 1;
+      c.best.inner = -1;
+    }
+    nodes[x].child[0] = base;
+    nodes[x].child[1] = base + 1;
+    nodes[x].expanded = 1;
+    nodes[x].kids_searched = srch;
+    const FeatDev m = feats[B.inner];
+    PartItem& pi = pitems[j];
+    pi.seg = seg;
+    pi.dst_buf = dst;
+    pi.split_group = m.group;
+    pi.left_total = lc;
+    pi.pad0 = 0;
+    pi.first_task = first_task;
+    pi.ntasks = tiles;
+    pi.chunk = pp.part_tile;
+    pi.pad1 = 0;
+    pi.rule = m.bin_type == 1 ? make_rule_categorical(m, o.cat_bits)
+                              : make_rule_numerical(m, (uint32_t)o.threshold, o.default_left != 0);
+    if (srch) {
+      const int sm = base + (left_small ? 0 : 1), lg = base + (left_small ? 1 : 0);
+      const PlanNode& S = nodes[sm];
+      const PlanNode& L = nodes[lg];
+      ScanItem si{};
+      si.small = LeafArgs{-1, S.slot, S.num_data, S.sum_g, S.sum_h, S.weight, S.int_sum, 0, 0};
+      si.large = LeafArgs{-1, L.slot, L.num_data, L.sum_g, L.sum_h, L.weight, L.int_sum, 0, 0};
+      if (hrows > 0) {
+        const int b0 = row0 / rpb, b1 = (row0 + hrows - 1) / rpb;
+        si.small.part_first = b0 + hidx;
+        si.small.nblk = b1 - b0 + 1;
+        hitems[hidx] = HistItem{S.seg, row0, 0};
+      }
+      si.has_large = 1;
+      si.use_subtract = 1;
+      si.used_off = sidx * F;
+      si.out_off = sidx * 2 * F;
+      sitems[sidx] = si;
+      ritems[sidx] = ReduceItem{S.slot, si.small.part_first, si.small.nblk, 0};
+      st->absorb[2 * sidx] = sm;
+      st->absorb[2 * sidx + 1] = lg;
+      const uint8_t* fl = node_flags + (size_t)x * F;
+      for (int f = 0; f < F; ++f) used[sidx * F + f] = fl[f] && bytree[f] ? 7 : 0;
+    }
+  }
+  if (tid == 0) {
+    st->done = (sh_done || ni == 0) ? 1 : 0;
+    st->num_nodes = num_nodes + 2 * ni;
+    st->next_slot = next_slot + nscan;
+    st->nabsorb = nscan;
+    st->rounds += ni > 0 ? 1 : 0;
+    st->search_rounds += nscan > 0 ? 1 : 0;
+    st->hist_rows += htotal;
+    st->part_rows += part_rows;
+    st->hdr = RoundHdr{ni, ntasks, nhist, htotal, nblocks, rpb, nscan, 0};
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2320,10 +2616,12 @@ __global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ 
 // sums are then added in lane order, so float forms reduce in a fixed order.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __restrict__ items) {
+__global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __restrict__ items,
+                                                const RoundHdr* __restrict__ hdr) {
   __shared__ long long ra[8][32], rb[8][32];
   __shared__ double da[8][32], db[8][32];
   __shared__ long long rc[8][32];
+  if (hdr && (int)blockIdx.y >= hdr->nscan) return;  // block-uniform
   const ReduceItem it = items[blockIdx.y];
   const int lb = threadIdx.x & 31, sp = threadIdx.x >> 5;
   const int bin = blockIdx.x * 32 + lb;
@@ -2410,8 +2708,8 @@ size_t hist_lds_bytes(int form, int max_tile_bins) {
 
 void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
-                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream) {
-  if (nblocks <= 0 || total_rows <= 0) return;
+                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream, const RoundHdr* hdr) {
+  if (nblocks <= 0 || (!hdr && total_rows <= 0)) return;
   const unsigned groups8 = (unsigned)((nblocks + kXcds - 1) / kXcds);
   dim3 grid(groups8 * kXcds * (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
@@ -2423,28 +2721,37 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
       allow_lds(k_hist<FORM_QUANT>);
       launch(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles);
+                         ntiles, hdr);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
       launch(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles);
+                         ntiles, hdr);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
       launch(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles);
+                         ntiles, hdr);
       break;
   }
+  BK_HIP(hipGetLastError());
+}
+
+void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const FoldOut* fold, const uint8_t* fold_flags,
+                 const FeatDev* feats, const int* bytree, const PlanParams& pp, PartItem* pitems, HistItem* hitems,
+                 ScanItem* sitems, ReduceItem* ritems, int* used, void* stream) {
+  if (pp.num_leaves > kPlanMaxLeaves) throw std::invalid_argument("the device planner takes at most 256 leaves");
+  launch(k_plan, dim3(1), dim3(kPlanThreads), 0, (hipStream_t)stream, st, nodes, node_flags, fold, fold_flags, feats,
+         bytree, pp, pitems, hitems, sitems, ritems, used);
   BK_HIP(hipGetLastError());
 }
 
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                      int max_items, unsigned epoch, void* stream) {
+                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr) {
   if (ntasks <= 0) return;
   if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
   const int R = partition_tile_rows(s.stride);
@@ -2456,12 +2763,12 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
     case FORM_QUANT:
       allow_lds(k_partition<FORM_QUANT>);
       launch(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch);
+             d_cursors, max_items, epoch, hdr);
       break;
     default:
       allow_lds(k_partition<FORM_F64>);
       launch(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch);
+             d_cursors, max_items, epoch, hdr);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -2476,19 +2783,19 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
   BK_HIP(hipGetLastError());
 }
 
-void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream) {
+void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream, const RoundHdr* hdr) {
   if (nitems <= 0) return;
   dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)nitems);
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
-      launch(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items, hdr);
       break;
     case FORM_F64:
-      launch(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items, hdr);
       break;
     default:
-      launch(k_reduce<FORM_F32C>, grid, dim3(256), 0, st, s, d_items);
+      launch(k_reduce<FORM_F32C>, grid, dim3(256), 0, st, s, d_items, hdr);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -2496,24 +2803,24 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
 
 
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream) {
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr) {
   if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
   dim3 grid((unsigned)s.num_features, (unsigned)nitems);
   switch (s.form) {
     case FORM_QUANT:
-      launch(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out, hdr);
       break;
     case FORM_F64:
-      launch(k_scan<FORM_F64>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_F64>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out, hdr);
       break;
     default:
-      launch(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out);
+      launch(k_scan<FORM_F32C>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out, hdr);
       break;
   }
   BK_HIP(hipGetLastError());
   launch(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, st, d_items, d_out, d_used, s.num_features,
-                     d_best, d_flags);
+                     d_best, d_flags, hdr);
   BK_HIP(hipGetLastError());
 }
 

@@ -398,6 +398,11 @@ struct Prof {
   void reset() {
     for (int k = 0; k < 8; ++k) ms[k] = 0, launches[k] = 0, algo[k] = 0, moved[k] = 0;
   }
+  // bytes of launches whose work the device decided (known after the tree)
+  void add_bytes(int kind, double a, double m) {
+    algo[kind] += a;
+    moved[kind] += m;
+  }
   ~Prof() {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
@@ -674,6 +679,27 @@ struct Learner {
   // by-node sampling in the batched form: every searched node keeps its per-feature
   // candidates and searched bits; its best is folded on the host when the reference
   // would draw its sample (when its parent's split is committed)
+  // device round planner: batched growth whose rounds queue without a host wait
+  bool plan_on_device = true;          // bk_learner_set_round_planner
+  PlanState* d_pstate = nullptr;
+  PlanNode* d_pnodes = nullptr;
+  uint8_t* d_pnflags = nullptr;        // splittable flags of every node
+  FoldOut* d_pfold = nullptr;          // the round's folded splits (device, read by the planner)
+  uint8_t* d_pfflags = nullptr;
+  PartItem* d_ppart = nullptr;
+  HistItem* d_phist = nullptr;
+  ScanItem* d_pscan = nullptr;
+  ReduceItem* d_pred = nullptr;
+  int* d_pused = nullptr;
+  int* d_bytree = nullptr;
+  uint8_t* h_plan = nullptr;           // pinned: the state, then the nodes
+  PlanState plan_init{};
+  std::vector<int> bytree_int;
+  std::vector<uint8_t> ones_f;
+  int plan_max_nodes = 0, hist_grid_blocks = 0, last_plan_rounds = 0;
+  bool planner_ok() const {
+    return plan_on_device && batched && d_pstate && !cols.by_node_active() && P.num_leaves <= kPlanMaxLeaves;
+  }
   SplitOut* h_out = nullptr;           // pinned copy of a round's per-feature results
   std::vector<SplitOut> node_cands;    // node i: [i * F, (i + 1) * F)
   std::vector<uint8_t> node_searched;  // node i: [i * F, (i + 1) * F)
@@ -755,11 +781,13 @@ struct Learner {
                     (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
-                    (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay})
+                    (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay,
+                    (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
+                    (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc, (void*)h_stamp,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
-                    (void*)h_out})
+                    (void*)h_out, (void*)h_plan})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -967,7 +995,8 @@ struct Learner {
                                  up(sizeof(int) * (size_t)F * max_items);
       const size_t tree_bytes = up(sizeof(Segment) * (size_t)P.num_leaves) + up(sizeof(int) * (size_t)P.num_leaves) +
                                 up(sizeof(NodeRule) * (size_t)P.num_leaves);
-      arena_bytes = std::max(round_bytes, tree_bytes) + 256;
+      const size_t plan_bytes = up(sizeof(PlanState)) + up(sizeof(PlanNode)) + 2 * up(sizeof(int) * (size_t)F);
+      arena_bytes = std::max(round_bytes + plan_bytes, tree_bytes) + 256;
       d_arena = dmalloc<uint8_t>(arena_bytes);
       pinned(&h_arena, arena_bytes);
       off_fold = up((size_t)2 * max_items * F);
@@ -1086,6 +1115,27 @@ struct Learner {
       P.interaction_constraint_offsets = nullptr;
       P.interaction_constraint_features = nullptr;
       if (cols.by_node_active()) pinned(&h_out, (size_t)F * 2 * max_items);
+    }
+    if (batched && P.num_leaves <= kPlanMaxLeaves && !cols.by_node_active()) {
+      plan_max_nodes = P.num_leaves * 4 + 4;
+      d_pstate = dmalloc<PlanState>(1);
+      d_pnodes = dmalloc<PlanNode>((size_t)plan_max_nodes);
+      d_pnflags = dmalloc<uint8_t>((size_t)plan_max_nodes * F);
+      d_pfold = dmalloc<FoldOut>((size_t)2 * max_items);
+      d_pfflags = dmalloc<uint8_t>((size_t)2 * max_items * F);
+      d_ppart = dmalloc<PartItem>((size_t)max_items);
+      d_phist = dmalloc<HistItem>((size_t)max_items);
+      d_pscan = dmalloc<ScanItem>((size_t)max_items);
+      d_pred = dmalloc<ReduceItem>((size_t)max_items);
+      d_pused = dmalloc<int>((size_t)F * max_items);
+      d_bytree = dmalloc<int>((size_t)F);
+      pinned(&h_plan, sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes);
+      bytree_int.assign((size_t)F, 1);
+      ones_f.assign((size_t)F, 1);
+      // the most row blocks a planned round's histogram may cut (its rows are at most N)
+      long long nb = std::min<long long>(kHistMaxBlocks, std::max<long long>(1, N / 2048));
+      nb = std::max<long long>(nb, ((long long)N + rows_per_flush - 1) / rows_per_flush);
+      hist_grid_blocks = (int)nb;
     }
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
@@ -2033,6 +2083,164 @@ struct Learner {
     return t;
   }
 
+  // ===========================================================================
+  // Batched form with the rounds planned on the device: the host launches the
+  // root's search, then rounds of [plan, partition, histogram, reduce, search]
+  // without waiting between them; k_plan replays the commits and picks each
+  // round's work. The host waits once the expected number of rounds is queued,
+  // reads the planner's state (complete or not) and the node table, and rebuilds
+  // the tree by replaying the same commits (commit_split), so the result is the
+  // host-planned batched form's tree, bit for bit.
+  // ===========================================================================
+  Tree train_planned(const float* g, const float* h, const int32_t* d_rows, int nrows) {
+    LeafSplits root = before_train(g, h, d_rows, nrows);
+    Tree t(P.num_leaves);
+    t.set_leaf_output(0, leaf_output_root(P, root.sum_g, root.sum_h));
+    const bool search_root = root.num_data > 0 && root.num_data >= P.min_data_in_leaf * 2;
+    const ScanParams sp = scan_params();
+    // the root's search, as train_batched runs it
+    HistPlan hp;
+    int nr = 0;
+    if (search_root) {
+      for (int f = 0; f < F; ++f) h_used[f] = cols.bytree[(size_t)f] ? kUseAll : 0;
+      hist_seg.assign(1, segs[0]);
+      hp = plan_hist(1);
+      ScanItem& si = h_sitems[0];
+      si = ScanItem{};
+      si.small = leaf_args(t, root, 0);
+      si.small.part_first = hist_first[0];
+      si.small.nblk = hist_count[0];
+      nr = plan_scan(1);
+    }
+    upload_round(0, &hp, search_root ? 1 : 0, nr, true);
+    // the planner's initial state: leaf 0 is node 0, searched by the root item
+    plan_init = PlanState{};
+    plan_init.num_leaves = 1;
+    plan_init.num_nodes = 1;
+    plan_init.next_slot = 1;
+    plan_init.nabsorb = search_root ? 1 : 0;
+    plan_init.absorb[0] = 0;
+    plan_init.absorb[1] = -1;
+    plan_init.leaf_node[0] = 0;
+    PlanNode n0{};
+    n0.seg = segs[0];
+    n0.child[0] = n0.child[1] = -1;
+    n0.num_data = root.num_data;
+    n0.sum_g = root.sum_g;
+    n0.sum_h = root.sum_h;
+    n0.weight = root.weight;
+    n0.int_sum = root.int_sum;
+    n0.best.o.gain = kMinScore;
+    n0.best.o.feature = -1;
+    n0.best.o.default_left = 1;
+    n0.best.inner = -1;
+    for (int f = 0; f < F; ++f) bytree_int[(size_t)f] = cols.bytree[(size_t)f] ? 1 : 0;
+    const PlanState* u_state = put(&plan_init, 1);
+    const PlanNode* u_node = put(&n0, 1);
+    const int* u_bytree = put(bytree_int.data(), (size_t)F);
+    const uint8_t* u_ones = put(ones_f.data(), (size_t)F);
+    send(true);
+    launch_copy(d_pstate, u_state, sizeof(PlanState), stream);
+    launch_copy(d_pnodes, u_node, sizeof(PlanNode), stream);
+    launch_copy(d_bytree, u_bytree, sizeof(int) * (size_t)F, stream);
+    launch_copy(d_pnflags, u_ones, (size_t)F, stream);
+    if (search_root) {
+      run_hist(hp, root.num_data, true);
+      stats.committed_hist_rows += root.num_data;
+      timed(3, 0.0, 0.0, [&] {
+        launch_reduce(ds, d_ritems, nr, stream);
+        launch_scan(ds, d_sitems, 1, sp, d_used, d_out, d_pfold, d_pfflags, stream);
+      });
+    }
+    PlanParams pp{};
+    pp.num_leaves = P.num_leaves;
+    pp.F = F;
+    pp.max_depth = P.max_depth;
+    pp.min_data_in_leaf = P.min_data_in_leaf;
+    pp.hist_min_rows = hist_min_rows;
+    pp.hist_max_blocks = hist_max_blocks;
+    pp.rows_per_flush = rows_per_flush;
+    pp.part_tile = part_tile;
+    pp.max_nodes = plan_max_nodes;
+    pp.num_slots = (int)num_slots;
+    pp.max_partials = max_partials;
+    const RoundHdr* hdr = &d_pstate->hdr;
+    auto plan = [&] {
+      timed(7, 0.0, 0.0, [&] {
+        launch_plan(d_pstate, d_pnodes, d_pnflags, d_pfold, d_pfflags, d_feats, d_bytree, pp, d_ppart, d_phist,
+                    d_pscan, d_pred, d_pused, stream);
+      });
+    };
+    // grids for the most work a round can hold: every row partitioned, every row
+    // in a histogram, 2^(r-1) items in round r (the tree's depth at round r is at
+    // most r - 1), at most max_items
+    const long long hb = std::max<long long>(std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows)),
+                                             ((long long)N + rows_per_flush - 1) / rows_per_flush);
+    plan();
+    int r = 1, want = std::max(4, last_plan_rounds + 1);
+    const size_t back = sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes;
+    const PlanState* hs = (const PlanState*)h_plan;
+    for (;;) {
+      for (int k = 0; k < want; ++k, ++r) {
+        const int items = std::min(max_items, 1 << std::min(r - 1, 20));
+        ++epoch;
+        timed(2, 0.0, 0.0, [&] {
+          launch_partition(ds, d_ppart, max_items, max_part_tiles, d_cursors, max_items, epoch, stream, hdr);
+        });
+        timed(1, 0.0, 0.0, [&] {
+          launch_hist(ds, d_phist, 0, 0, (int)hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
+                      (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
+        });
+        timed(3, 0.0, 0.0, [&] {
+          launch_reduce(ds, d_pred, items, stream, hdr);
+          launch_scan(ds, d_pscan, items, sp, d_pused, d_out, d_pfold, d_pfflags, stream, hdr);
+        });
+        plan();
+      }
+      // the state and the node table are two allocations: two copies
+      launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream);
+      launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
+      sync_spin();
+      if (hs->done) break;
+      // every planned round expands at least one node, and a tree expands fewer
+      // than 2 * num_leaves: more rounds mean a planner that never finishes
+      if (r > 2 * P.num_leaves + 8) throw std::runtime_error("device planner did not finish the tree");
+      want = 4;
+    }
+    if (hs->pad) throw std::runtime_error("device planner overflow (nodes, slots or partials)");
+    last_plan_rounds = hs->rounds;
+    const PlanNode* hn = (const PlanNode*)(h_plan + sizeof(PlanState));
+    if (hs->ncommit < 0 || hs->ncommit > P.num_leaves - 1) throw std::runtime_error("device planner: bad commit count");
+    for (int c = 0; c < hs->ncommit; ++c) {
+      const int bl = hs->commit[2 * c], x = hs->commit[2 * c + 1];
+      HostSplit info;
+      info.o = hn[x].best.o;
+      info.inner = hn[x].best.inner;
+      if (bl < 0 || bl >= t.num_leaves || x < 0 || x >= hs->num_nodes || info.inner < 0 || info.inner >= F)
+        throw std::runtime_error("device planner: bad commit");
+      commit_split(t, bl, info);
+    }
+    if (t.num_leaves != hs->num_leaves) throw std::runtime_error("device planner: leaf count disagrees");
+    if (hs->num_nodes < 1 || hs->num_nodes > plan_max_nodes) throw std::runtime_error("device planner: bad node count");
+    for (int l = 0; l < t.num_leaves; ++l) {
+      const int x = hs->leaf_node[l];
+      if (x < 0 || x >= hs->num_nodes) throw std::runtime_error("device planner: bad leaf node");
+      segs[(size_t)l] = hn[x].seg;
+    }
+    int expanded = 0;
+    for (int i = 0; i < hs->num_nodes; ++i) expanded += hn[i].expanded ? 1 : 0;
+    stats.speculative_waste = expanded - (t.num_leaves - 1);
+    stats.hist_rows += hs->hist_rows;
+    stats.part_rows += hs->part_rows;
+    stats.rounds += hs->search_rounds + (search_root ? 1 : 0);
+    if (prof.tree_on) {
+      const double hr = (double)hs->hist_rows, pr = (double)hs->part_rows;
+      if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
+      if ((prof.mask >> 2) & 1u) prof.add_bytes(2, pr * 9.0, pr * 2.0 * stride);
+    }
+    return t;
+  }
+
   // Leaf of every row of the last tree: every row walks the tree over its record
   // in original order (the partition moves no row ids).
   // internal nodes of the last tree as the walk takes them
@@ -2067,7 +2275,8 @@ struct Learner {
     wait_ms = 0.0;
     leaf_of_valid = false;
     prof.tree_on = prof.trees++ % prof.every == 0;
-    Tree t = batched ? train_batched(g, h, d_rows, nrows) : train_sequential(g, h, d_rows, nrows);
+    Tree t = !batched ? train_sequential(g, h, d_rows, nrows)
+                      : (planner_ok() ? train_planned(g, h, d_rows, nrows) : train_batched(g, h, d_rows, nrows));
     last_bagged = d_rows != nullptr;
     last_num_leaves = t.num_leaves;
     last_left.assign(t.left.begin(), t.left.begin() + std::max(t.num_leaves - 1, 0));
@@ -2596,6 +2805,13 @@ int bk_learner_set_tree_slot(bk_learner* l, int slot, int trees_per_iteration) {
   });
 }
 
+int bk_learner_set_round_planner(bk_learner* l, int on_device) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    l->L.plan_on_device = on_device != 0;
+  });
+}
+
 int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_blocks) {
   return bk::guard([&] {
     bk::Learner& L = l->L;
@@ -2604,8 +2820,8 @@ int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_bloc
       L.hist_max_blocks = L.tuned_max_blocks;
       return;
     }
-    if (min_rows < 1 || max_blocks < 1 || max_blocks > bk::Learner::kHistMaxBlocks)
-      throw std::invalid_argument("histogram tiling: min_rows >= 1 and 1 <= max_blocks <= 2048");
+    if (min_rows < 2048 || max_blocks < 1 || max_blocks > bk::Learner::kHistMaxBlocks)
+      throw std::invalid_argument("histogram tiling: min_rows >= 2048 and 1 <= max_blocks <= 2048");
     L.hist_min_rows = min_rows;
     L.hist_max_blocks = max_blocks;
   });

@@ -331,6 +331,15 @@ int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iterat
  * larger than one LDS accumulation. No reference counterpart (tuning only). */
 int bk_learner_set_hist_tiling(bk_learner* learner, int32_t min_rows, int32_t max_blocks);
 
+/* Who plans the rounds of batched growth: 1 (the default) the device, whose
+ * planner kernel replays the commits and picks each round's work, so a tree's
+ * rounds queue without a host wait; 0 the host, waiting for every round's
+ * results. Both grow the same tree. The device planner takes up to 256 leaves and
+ * no by-node feature sampling or interaction constraints; the host plans
+ * otherwise. No reference counterpart (the reference grows one split per step,
+ * crates/lgbm-core/src/treelearner/serial.rs:606-621). */
+int bk_learner_set_round_planner(bk_learner* learner, int on_device);
+
 /* Per-tree counters of the last train call. hist_rows / part_rows count every
  * row the kernels processed (the batched form includes speculative expansions
  * that were never committed); committed_hist_rows / committed_part_rows count
@@ -356,7 +365,7 @@ int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 /* Per-kernel timing with HIP events recorded on the learner's stream around
  * every launch (off by default). Kinds: 0 root histogram, 1 child histograms,
  * 2 partition, 3 split search, 4 discretizer, 5 leaf renewal, 6 score update
- * and gradients. algo_bytes follows the per-row byte model of the histogram
+ * and gradients, 7 the device round planner. algo_bytes follows the per-row byte model of the histogram
  * (rows x (features + payload), plus a 4-byte row index for a child) and of the
  * partition (rows x 9: one bin, row id read, row id write), the reference's
  * index-based model; moved_bytes counts what the kernel actually reads and

@@ -328,6 +328,57 @@ def test_softmax_gradients_device_match_host():
     assert np.all(np.abs(h - want_h) <= 2 * ulp_h)
 
 
+PLANNER_CASES = {
+    "quantized_255": dict(num_leaves=255, min_data_in_leaf=5),
+    "quantized_depth4": dict(num_leaves=63, max_depth=4),
+    "quantized_min_data": dict(num_leaves=127, min_data_in_leaf=300),
+    "quantized_bytree": dict(num_leaves=63, feature_fraction=0.6, feature_fraction_seed=4),
+    "f64_255": dict(num_leaves=255, form=bg.FORM_PLAIN_F64),
+    "f32c_31": dict(num_leaves=31, form=bg.FORM_F32_COMPENSATED),
+}
+
+
+@pytest.mark.parametrize("bagged", [0, 1])
+@pytest.mark.parametrize("case", sorted(PLANNER_CASES))
+def test_device_round_planner_matches_host_planner(binary_10k, case, bagged):
+    """Batched growth planned on the device (k_plan replays the commits and picks
+    every round's work; no host wait per round) against the host-planned batched
+    form, tree by tree for 3 boosting iterations: bit for bit, with the same
+    rounds, leaf segments (leaf rows) and score updates. The quantized trees also
+    equal the oracle's."""
+    announce(f"device round planner vs host planner: {case} bagged={bagged}")
+    ds, y = binary_10k
+    kw = dict(PLANNER_CASES[case])
+    form = kw.pop("form", bg.FORM_QUANTIZED)
+    p = bg.LearnerParams(form=form, batched=1, **kw)
+    if form == bg.FORM_QUANTIZED:
+        quantized(p, 7)
+    dev, host = bg.Learner(ds, p, False), bg.Learner(ds, p, False)
+    host.set_round_planner(False)
+    o = ob.OracleLearner(ds, p, False) if form == bg.FORM_QUANTIZED else None
+    obj = bst.BinaryObjective(y)
+    sampler = bst.RowSampler({"bagging_fraction": 0.7, "bagging_freq": 1, "bagging_seed": 3}, y) if bagged else None
+    score = np.full(ds.num_data, obj.boost_from_score(0))
+    for it in range(3):
+        g, h = obj.gradients(score)
+        rows = sampler.sample(it, g, h) if sampler is not None else None
+        td, th = dev.train(g, h, rows), host.train(g, h, rows)
+        assert_trees_bitwise(td, th, f"iter {it}")
+        assert dev.last_stats()["rounds"] == host.last_stats()["rounds"]
+        assert dev.last_stats()["splits"] == host.last_stats()["splits"]
+        if o is not None:
+            assert_trees_bitwise(td, o.train(g, h, rows), f"iter {it} vs oracle")
+        for leaf in (0, td["num_leaves"] // 2, td["num_leaves"] - 1):
+            assert np.array_equal(dev.leaf_rows(leaf), host.leaf_rows(leaf)), (it, leaf)
+        vals = td["leaf_value"] * 0.1
+        s2 = score.copy()
+        dev.add_prediction_to_score(vals, s2)
+        host.add_prediction_to_score(vals, score)
+        assert np.array_equal(s2, score)
+    dev.close()
+    host.close()
+
+
 @pytest.mark.parametrize("case", ["plain", "unbalance", "scale_pos", "weighted", "sigmoid"])
 def test_binary_gradients_device_match_host(case):
     """The device binary log-loss gradients (the bench loop's objective) against
