@@ -322,10 +322,16 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
 // expanded, picks the next frontier (the best unexpanded leaves with positive
 // gain, at most the remaining leaf budget) and writes the round's items and
 // st->hdr. A complete tree sets st->done and plans no work.
+// The planner's initial state, root node, tree feature sample and root flags,
+// copied from an uploaded staging area in one launch.
+void launch_plan_init(const PlanState* s_state, const PlanNode* s_node, const int* s_bytree, const uint8_t* s_flags,
+                      PlanState* st, PlanNode* node0, int* bytree, uint8_t* flags0, int F, void* stream);
 void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const FoldOut* fold, const uint8_t* fold_flags,
                  const FeatDev* feats, const int* bytree, const PlanParams& pp, PartItem* pitems, HistItem* hitems,
                  ScanItem* sitems, ReduceItem* ritems, int* used, void* stream);
-void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream);
+// max |g|, max h and a bad-value flag into d_out4 (zeroed before); d_zero4 (may
+// be null): four int64 zeroed on the way, for the discretizer's sums.
+void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, long long* d_zero4, void* stream);
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
                        long long* d_sums3, void* stream);

@@ -144,6 +144,19 @@ __device__ __forceinline__ int block_sum(int v, int* scratch) {
   return t;
 }
 
+// Block-wide minimum over NT threads (result in every thread).
+template <int NT>
+__device__ __forceinline__ int block_min(int v, int* scratch) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_down(v, o, 64));
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = INT_MAX;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t = min(t, scratch[w]);
+  __syncthreads();
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // Histogram accumulators in LDS for one group tile.
 // ---------------------------------------------------------------------------
@@ -1580,16 +1593,17 @@ __device__ void fold_side(const ScanItem& it, int k, int side, const SplitOut* o
   if (lane == 0) dst->inner = bf;
 }
 
+// The search of feature blockIdx.x for search item y (one block; returns are
+// block-uniform, except for the waves that search no leaf, which leave before
+// the last wave-level steps).
 template <int FORM>
-__global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanItem* __restrict__ items, ScanParams p,
-                                                       const int* __restrict__ used_all, SplitOut* out_all,
-                                                       const RoundHdr* __restrict__ hdr) {
-  if (hdr && (int)blockIdx.y >= hdr->nscan) return;  // block-uniform
+__device__ __forceinline__ void scan_item(const DevState& s, const ScanItem* __restrict__ items, int y,
+                                          const ScanParams& p, const int* __restrict__ used_all, SplitOut* out_all) {
   __shared__ SliceLds S[2];
   __shared__ ScanPrefix X[2];
   __shared__ int elig[2][kMaxBins];
   const int f = blockIdx.x;
-  const ScanItem& item = items[blockIdx.y];
+  const ScanItem& item = items[y];
   const int* used = used_all + item.used_off;
   SplitOut* out = out_all + item.out_off;
   const LeafArgs small = item.small, large = item.large;
@@ -1649,6 +1663,19 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
   if (lane == 0) out[(wl == 0 ? 0 : s.num_features) + f] = o;
 }
 
+// Items blockIdx.y, blockIdx.y + gridDim.y, ...: a planned round's grid is capped
+// below the most items a round may hold.
+template <int FORM>
+__global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanItem* __restrict__ items, ScanParams p,
+                                                       const int* __restrict__ used_all, SplitOut* out_all,
+                                                       const RoundHdr* __restrict__ hdr) {
+  const int nitems = hdr ? hdr->nscan : (int)gridDim.y;
+  for (int y = blockIdx.y; y < nitems; y += gridDim.y) {
+    scan_item<FORM>(s, items, y, p, used_all, out_all);
+    __syncthreads();  // the LDS slices are reused by the next item
+  }
+}
+
 // One wave per (item, side): the fold runs after the search launch, whose
 // candidates it reads (a kernel boundary orders them across the XCDs' L2s).
 __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
@@ -1667,6 +1694,7 @@ __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items,
 // ---------------------------------------------------------------------------
 constexpr int kPlanThreads = 256;
 static_assert(kPlanMaxLeaves <= kPlanThreads, "one thread per frontier item");
+static_assert(kPlanMaxLeaves % 64 == 0, "the commit replay holds kPlanMaxLeaves / 64 leaves per lane");
 
 __device__ __forceinline__ bool plan_better(double ga, int ka, int la, double gb, int kb, int lb) {
   if (ga != gb) return ga > gb;
@@ -1685,10 +1713,14 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
                                                        PartItem* __restrict__ pitems, HistItem* __restrict__ hitems,
                                                        ScanItem* __restrict__ sitems, ReduceItem* __restrict__ ritems,
                                                        int* __restrict__ used) {
-  __shared__ double lgain[kPlanMaxLeaves];
-  __shared__ int lkey[kPlanMaxLeaves], lnode[kPlanMaxLeaves], order[kPlanMaxLeaves];
+  __shared__ double lgain[kPlanMaxLeaves], cgain[2][kPlanMaxLeaves];
+  __shared__ int lkey[kPlanMaxLeaves], lnode[kPlanMaxLeaves], order[kPlanMaxLeaves], cand[kPlanMaxLeaves];
+  __shared__ int ckey[2][kPlanMaxLeaves], cnode[2][kPlanMaxLeaves];
   __shared__ unsigned char lexp[kPlanMaxLeaves];
-  __shared__ int sh_nl, sh_done, sh_ncand;
+  __shared__ int scan_node[kPlanMaxLeaves], commit_leaf[kPlanMaxLeaves], commit_node[kPlanMaxLeaves];
+  __shared__ int sh_nl, sh_done, sh_first, sh_ncommit;
+  __shared__ double sbg[kPlanThreads];
+  __shared__ int sbk[kPlanThreads], sbi[kPlanThreads];
   __shared__ int scratch[32];
   const int tid = threadIdx.x, F = pp.F;
   if (st->done) {  // block-uniform: the tree is complete, no work
@@ -1698,118 +1730,227 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     }
     return;
   }
-  // (1) the last round's folded splits go to the nodes that were searched
+  // (1) the last round's folded splits go to the nodes that were searched: one
+  // thread per (item, side), its loads independent of each other
   const int na = st->nabsorb;
   constexpr int kWords = (int)(sizeof(FoldOut) / 4);
-  for (int i = tid; i < na * 2 * kWords; i += kPlanThreads) {
-    const int k = i / (2 * kWords), r = i - k * 2 * kWords, side = r / kWords, w = r - side * kWords;
-    const int nd = st->absorb[2 * k + side];
-    if (nd >= 0) ((uint32_t*)&nodes[nd].best)[w] = ((const uint32_t*)(fold + 2 * k + side))[w];
-  }
-  for (int i = tid; i < na * 2 * F; i += kPlanThreads) {
-    const int k = i / (2 * F), r = i - k * 2 * F, side = r / F, f = r - side * F;
-    const int nd = st->absorb[2 * k + side];
-    if (nd >= 0) node_flags[(size_t)nd * F + f] = fold_flags[(size_t)(2 * k + side) * F + f];
+  if (tid < 2 * na) {
+    const int nd = st->absorb[tid];
+    if (nd >= 0) {
+      uint32_t w[kWords];
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) w[i] = ((const uint32_t*)(fold + tid))[i];
+#pragma unroll
+      for (int i = 0; i < kWords; ++i) ((uint32_t*)&nodes[nd].best)[i] = w[i];
+      for (int f = 0; f < F; ++f) node_flags[(size_t)nd * F + f] = fold_flags[(size_t)tid * F + f];
+    }
   }
   __syncthreads();
-  // (2) the leaves' best splits, then the commits in the reference's order while
-  // the best leaf has been expanded (one wave)
+  // (2) every leaf's best split, and for an expanded leaf its children's: a leaf
+  // committed in this round becomes its two children, which were searched but
+  // not expanded, so the commits below read LDS only
   int nl = st->num_leaves;
   for (int l = tid; l < nl; l += kPlanThreads) {
     const int x = st->leaf_node[l];
     lnode[l] = x;
     lgain[l] = nodes[x].best.o.gain;
     lkey[l] = plan_key(nodes[x].best.o);
-    lexp[l] = (unsigned char)nodes[x].expanded;
+    const int e = nodes[x].expanded;
+    lexp[l] = (unsigned char)e;
+    if (e) {
+      for (int side = 0; side < 2; ++side) {
+        const int c = nodes[x].child[side];
+        cnode[side][l] = c;
+        cgain[side][l] = nodes[c].best.o.gain;
+        ckey[side][l] = plan_key(nodes[c].best.o);
+      }
+    }
   }
   __syncthreads();
-  if (tid < 64) {
+  // The commits in the reference's order while the best leaf has been expanded.
+  // A committed leaf's children are not expanded, so they are never committed
+  // in this round: the committed leaves are a prefix of the current leaves in
+  // better_than order, and the walk stops at the first position where a child
+  // made so far, or the next leaf itself if unexpanded, is the best leaf. The
+  // leaves are ranked once (one thread each); lane 0 walks. With a NaN gain
+  // (never better nor worse) the order is not total and the reference's plain
+  // scan is replayed commit by commit instead.
+  bool nan = false;
+  for (int l = tid; l < nl; l += kPlanThreads)
+    nan |= lgain[l] != lgain[l] || (lexp[l] && (cgain[0][l] != cgain[0][l] || cgain[1][l] != cgain[1][l]));
+  const int any_nan = __syncthreads_or(nan ? 1 : 0);
+  if (!any_nan) {
+    // rank every leaf (one thread each)
+    if (tid < nl) {
+      const double g = lgain[tid];
+      const int k = lkey[tid];
+      int r = 0;
+#pragma unroll 8
+      for (int m = 0; m < nl; ++m) r += (m != tid && plan_better(lgain[m], lkey[m], m, g, k, tid)) ? 1 : 0;
+      order[r] = tid;
+    }
+    __syncthreads();
+    // position i holds leaf order[i] and, were it committed as commit i, its two
+    // children: the left one takes its index, the right one index nl + i; cb[i]
+    // is the best child over positions 0..i (an inclusive scan of better_than)
+    const int nl0 = nl;
+    double cbg = 0.0;
+    int cbk = INT_MAX, cbi = -1;
+    if (tid < nl0) {
+      const int l = order[tid];
+      if (lexp[l]) {
+        cbg = cgain[0][l];
+        cbk = ckey[0][l];
+        cbi = l;
+        if (plan_better(cgain[1][l], ckey[1][l], nl0 + tid, cbg, cbk, cbi)) {
+          cbg = cgain[1][l];
+          cbk = ckey[1][l];
+          cbi = nl0 + tid;
+        }
+      }
+    }
+    for (int o = 1; o < kPlanThreads; o <<= 1) {
+      sbg[tid] = cbg;
+      sbk[tid] = cbk;
+      sbi[tid] = cbi;
+      __syncthreads();
+      if (tid >= o && sbi[tid - o] >= 0 &&
+          (cbi < 0 || plan_better(sbg[tid - o], sbk[tid - o], sbi[tid - o], cbg, cbk, cbi))) {
+        cbg = sbg[tid - o];
+        cbk = sbk[tid - o];
+        cbi = sbi[tid - o];
+      }
+      __syncthreads();
+    }
+    sbg[tid] = cbg;
+    sbk[tid] = cbk;
+    sbi[tid] = cbi;
+    __syncthreads();
+    // the walk stops at the first position i where the leaf budget is full, the
+    // best leaf (a child of positions < i, or order[i]) has gain <= 0, or it is
+    // not expanded (a child never is); positions before it are committed
+    int stop = kPlanThreads, why = 0;  // why: 1 done, 2 an unexpanded best leaf
+    if (tid <= nl0) {
+      const int i = tid;
+      const bool have_leaf = i < nl0;
+      const int l = have_leaf ? order[i] : 0;
+      const bool child_top = i > 0 && sbi[i - 1] >= 0 &&
+                             (!have_leaf || plan_better(sbg[i - 1], sbk[i - 1], sbi[i - 1], lgain[l], lkey[l], l));
+      if (nl0 + i >= pp.num_leaves || (child_top ? sbg[i - 1] : lgain[l]) <= 0.0) {
+        stop = i;
+        why = 1;
+      } else if (child_top || !lexp[l]) {
+        stop = i;
+        why = 2;
+      }
+    }
+    // first stopping position (position nl0 always stops: only children remain)
+    const int first = block_min<kPlanThreads>(stop * 4 + why, scratch);
+    const int ncm = first >> 2;
+    if (tid < ncm) {
+      const int l = order[tid];
+      commit_leaf[tid] = l;
+      commit_node[tid] = lnode[l];
+    }
+    __syncthreads();
+    if (tid < ncm) {
+      const int l = order[tid], right = nl0 + tid;
+      const int c0 = cnode[0][l], c1 = cnode[1][l];
+      const double g0 = cgain[0][l], g1 = cgain[1][l];
+      const int k0 = ckey[0][l], k1 = ckey[1][l];
+      lnode[l] = c0;
+      lgain[l] = g0;
+      lkey[l] = k0;
+      lexp[l] = 0;
+      lnode[right] = c1;
+      lgain[right] = g1;
+      lkey[right] = k1;
+      lexp[right] = 0;
+    }
+    if (tid == 0) {
+      sh_done = (first & 3) == 1 ? 1 : 0;
+      sh_nl = nl0 + ncm;
+      sh_first = st->ncommit;
+      sh_ncommit = st->ncommit + ncm;
+    }
+  } else if (tid < 64) {
+    // one wave, commit by commit: lane 0 scans for the first maximum
     const int lane = tid;
-    int done = 0, ncommit = st->ncommit;
+    int done = 0, ncommit = st->ncommit, first_commit = ncommit;
     for (;;) {
       if (nl >= pp.num_leaves) {
         done = 1;
         break;
       }
-      bool nan = false;
-      for (int l = lane; l < nl; l += 64) nan |= lgain[l] != lgain[l];
-      int bl;
-      if (__ballot(nan) != 0ull) {
-        // a NaN gain is never better nor worse: the reference's plain scan decides
-        int b = 0;
-        if (lane == 0)
-          for (int l = 1; l < nl; ++l) {
-            const double x = lgain[l], y = lgain[b];
-            if (x != y ? x > y : lkey[l] < lkey[b]) b = l;
-          }
-        bl = __shfl(b, 0, 64);
-      } else {
-        double bg = 0.0;
-        int bk = INT_MAX, bi = INT_MAX;
-        for (int l = lane; l < nl; l += 64)
-          if (bi == INT_MAX || plan_better(lgain[l], lkey[l], l, bg, bk, bi)) {
-            bg = lgain[l];
-            bk = lkey[l];
-            bi = l;
-          }
-        for (int o = 32; o > 0; o >>= 1) {
-          const double og = __shfl_xor(bg, o, 64);
-          const int ok = __shfl_xor(bk, o, 64), oi = __shfl_xor(bi, o, 64);
-          if (oi != INT_MAX && (bi == INT_MAX || plan_better(og, ok, oi, bg, bk, bi))) {
-            bg = og;
-            bk = ok;
-            bi = oi;
-          }
+      int b = 0;
+      if (lane == 0)
+        for (int l = 1; l < nl; ++l) {
+          const double x = lgain[l], y = lgain[b];
+          if (x != y ? x > y : lkey[l] < lkey[b]) b = l;
         }
-        bl = bi;
-      }
+      const int bl = __shfl(b, 0, 64);
       if (lgain[bl] <= 0.0) {
         done = 1;
         break;
       }
       if (!lexp[bl]) break;
-      const int x = lnode[bl], right = nl;
-      const int c0 = nodes[x].child[0], c1 = nodes[x].child[1];
+      const int right = nl;
       if (lane == 0) {
-        st->commit[2 * ncommit] = bl;
-        st->commit[2 * ncommit + 1] = x;
-        st->leaf_node[bl] = c0;
-        st->leaf_node[right] = c1;
-        lnode[bl] = c0;
-        lnode[right] = c1;
-        lgain[bl] = nodes[c0].best.o.gain;
-        lkey[bl] = plan_key(nodes[c0].best.o);
-        lexp[bl] = (unsigned char)nodes[c0].expanded;
-        lgain[right] = nodes[c1].best.o.gain;
-        lkey[right] = plan_key(nodes[c1].best.o);
-        lexp[right] = (unsigned char)nodes[c1].expanded;
+        commit_leaf[ncommit - first_commit] = bl;
+        commit_node[ncommit - first_commit] = lnode[bl];
+        lnode[right] = cnode[1][bl];
+        lgain[right] = cgain[1][bl];
+        lkey[right] = ckey[1][bl];
+        lexp[right] = 0;
+        lnode[bl] = cnode[0][bl];
+        lgain[bl] = cgain[0][bl];
+        lkey[bl] = ckey[0][bl];
+        lexp[bl] = 0;
       }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       ++ncommit;
       ++nl;
     }
     if (lane == 0) {
-      st->ncommit = ncommit;
-      st->num_leaves = nl;
       sh_done = done;
       sh_nl = nl;
+      sh_first = first_commit;
+      sh_ncommit = ncommit;
     }
   }
   __syncthreads();
   nl = sh_nl;
+  // the commits and the leaves' nodes to the state
+  for (int c = sh_first + tid; c < sh_ncommit; c += kPlanThreads) {
+    st->commit[2 * c] = commit_leaf[c - sh_first];
+    st->commit[2 * c + 1] = commit_node[c - sh_first];
+  }
+  for (int l = tid; l < nl; l += kPlanThreads) st->leaf_node[l] = lnode[l];
+  if (tid == 0) {
+    st->ncommit = sh_ncommit;
+    st->num_leaves = nl;
+  }
   // (3) the frontier: unexpanded leaves with positive gain, best first
-  int cand = 0;
-  if (!sh_done)
-    for (int l = tid; l < nl; l += kPlanThreads) {
-      if (lexp[l] || !(lgain[l] > 0.0)) continue;
-      int r = 0;
-      for (int m = 0; m < nl; ++m)
-        if (m != l && !lexp[m] && lgain[m] > 0.0 && plan_better(lgain[m], lkey[m], m, lgain[l], lkey[l], l)) ++r;
-      order[r] = l;
-      ++cand;
+  int is_cand = 0;
+  if (tid < nl && !sh_done) is_cand = !lexp[tid] && lgain[tid] > 0.0;
+  int cpos, unused, ncand, unused2;
+  block_scan2<kPlanThreads>(is_cand, 0, &cpos, &unused, &ncand, &unused2, scratch);
+  if (is_cand) cand[cpos] = tid;
+  __syncthreads();
+  if (tid < ncand) {
+    const int l = cand[tid];
+    const double g = lgain[l];
+    const int k = lkey[l];
+    int r = 0;
+#pragma unroll 8
+    for (int i = 0; i < ncand; ++i) {
+      const int m = cand[i];
+      r += (m != l && plan_better(lgain[m], lkey[m], m, g, k, l)) ? 1 : 0;
     }
-  const int ncand = block_sum<kPlanThreads>(cand, scratch);
+    order[r] = l;
+  }
+  __syncthreads();
   const int ni = sh_done ? 0 : min(ncand, pp.num_leaves - nl);
   // (4) the round's items, one thread per expanded leaf: partition item, two
   // child nodes, and when the children are searchable (before_find_best_split's
@@ -1856,27 +1997,34 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     return;
   }
   if (j < ni) {
-    const FoldOut& B = nodes[x].best;
-    const SplitOut& o = B.o;
+    const SplitOut o = nodes[x].best.o;
+    const int inner = nodes[x].best.inner, xslot = nodes[x].slot;
     const int dst = seg.buf == 1 ? 2 : 1;
     const int base = num_nodes + 2 * j;
     const bool left_small = lc < rc;
     const int sm_slot = srch ? next_slot + sidx : -1;
-    const int lg_slot = srch ? nodes[x].slot : -1;
+    const int lg_slot = srch ? xslot : -1;
+    // the children (side 0 left, 1 right) as LeafSplits of the split (serial.rs:1685-1876)
+    LeafArgs kid[2];
+    Segment kseg[2] = {Segment{dst, seg.begin, lc}, Segment{dst, seg.begin + lc, rc}};
+    kid[0] = LeafArgs{-1, left_small ? sm_slot : lg_slot, lc, o.left_sum_gradient, o.left_sum_hessian, o.left_output,
+                      o.left_int_sum, 0, 0};
+    kid[1] = LeafArgs{-1, left_small ? lg_slot : sm_slot, rc, o.right_sum_gradient, o.right_sum_hessian,
+                      o.right_output, o.right_int_sum, 0, 0};
     for (int side = 0; side < 2; ++side) {
       PlanNode& c = nodes[base + side];
-      c.seg = side == 0 ? Segment{dst, seg.begin, lc} : Segment{dst, seg.begin + lc, rc};
+      c.seg = kseg[side];
       c.depth = depth;
-      c.slot = ((side == 0) == left_small) ? sm_slot : lg_slot;
+      c.slot = kid[side].slot;
       c.expanded = 0;
       c.kids_searched = 0;
       c.child[0] = c.child[1] = -1;
-      c.num_data = side == 0 ? lc : rc;
+      c.num_data = kid[side].num_data;
       c.pad = 0;
-      c.sum_g = side == 0 ? o.left_sum_gradient : o.right_sum_gradient;
-      c.sum_h = side == 0 ? o.left_sum_hessian : o.right_sum_hessian;
-      c.weight = side == 0 ? o.left_output : o.right_output;
-      c.int_sum = side == 0 ? o.left_int_sum : o.right_int_sum;
+      c.sum_g = kid[side].sum_g;
+      c.sum_h = kid[side].sum_h;
+      c.weight = kid[side].parent_output;
+      c.int_sum = kid[side].int_sum;
       c.best.o = SplitOut{};
       c.best.o.gain = kMinScore;
       c.best.o.feature = -1;
@@ -1887,7 +2035,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     nodes[x].child[1] = base + 1;
     nodes[x].expanded = 1;
     nodes[x].kids_searched = srch;
-    const FeatDev m = feats[B.inner];
+    const FeatDev m = feats[inner];
     PartItem& pi = pitems[j];
     pi.seg = seg;
     pi.dst_buf = dst;
@@ -1901,29 +2049,33 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     pi.rule = m.bin_type == 1 ? make_rule_categorical(m, o.cat_bits)
                               : make_rule_numerical(m, (uint32_t)o.threshold, o.default_left != 0);
     if (srch) {
-      const int sm = base + (left_small ? 0 : 1), lg = base + (left_small ? 1 : 0);
-      const PlanNode& S = nodes[sm];
-      const PlanNode& L = nodes[lg];
+      const int sm_side = left_small ? 0 : 1;
       ScanItem si{};
-      si.small = LeafArgs{-1, S.slot, S.num_data, S.sum_g, S.sum_h, S.weight, S.int_sum, 0, 0};
-      si.large = LeafArgs{-1, L.slot, L.num_data, L.sum_g, L.sum_h, L.weight, L.int_sum, 0, 0};
+      si.small = kid[sm_side];
+      si.large = kid[1 - sm_side];
       if (hrows > 0) {
         const int b0 = row0 / rpb, b1 = (row0 + hrows - 1) / rpb;
         si.small.part_first = b0 + hidx;
         si.small.nblk = b1 - b0 + 1;
-        hitems[hidx] = HistItem{S.seg, row0, 0};
+        hitems[hidx] = HistItem{kseg[sm_side], row0, 0};
       }
       si.has_large = 1;
       si.use_subtract = 1;
       si.used_off = sidx * F;
       si.out_off = sidx * 2 * F;
       sitems[sidx] = si;
-      ritems[sidx] = ReduceItem{S.slot, si.small.part_first, si.small.nblk, 0};
-      st->absorb[2 * sidx] = sm;
-      st->absorb[2 * sidx + 1] = lg;
-      const uint8_t* fl = node_flags + (size_t)x * F;
-      for (int f = 0; f < F; ++f) used[sidx * F + f] = fl[f] && bytree[f] ? 7 : 0;
+      ritems[sidx] = ReduceItem{sm_slot, si.small.part_first, si.small.nblk, 0};
+      st->absorb[2 * sidx] = base + sm_side;
+      st->absorb[2 * sidx + 1] = base + 1 - sm_side;
+      scan_node[sidx] = x;
     }
+  }
+  __syncthreads();
+  // every search item's features: the expanded node's splittable flags (its
+  // children inherit them, serial.rs:838-852) within the tree's feature sample
+  for (int i = tid; i < nscan * F; i += kPlanThreads) {
+    const int k = i / F, f = i - k * F;
+    used[i] = node_flags[(size_t)scan_node[k] * F + f] && bytree[f] ? 7 : 0;
   }
   if (tid == 0) {
     st->done = (sh_done || ni == 0) ? 1 : 0;
@@ -1941,7 +2093,9 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
 // ---------------------------------------------------------------------------
 // Gradient discretization (gradient_discretizer/mod.rs:116-201)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_max_abs(const float* __restrict__ g, const float* __restrict__ h, int n, unsigned* out) {
+__global__ void __launch_bounds__(256) k_max_abs(const float* __restrict__ g, const float* __restrict__ h, int n, unsigned* out,
+                                                 long long* zero4) {
+  if (zero4 && blockIdx.x == 0 && threadIdx.x < 4) zero4[threadIdx.x] = 0;  // the discretizer's sums, read later
   unsigned mg = 0u, mh = 0u;
   int bad = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -2621,61 +2775,66 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
   __shared__ long long ra[8][32], rb[8][32];
   __shared__ double da[8][32], db[8][32];
   __shared__ long long rc[8][32];
-  if (hdr && (int)blockIdx.y >= hdr->nscan) return;  // block-uniform
-  const ReduceItem it = items[blockIdx.y];
-  const int lb = threadIdx.x & 31, sp = threadIdx.x >> 5;
-  const int bin = blockIdx.x * 32 + lb;
-  const size_t stride = (size_t)s.total_bins;
-  long long g = 0, h = 0, c = 0;
-  double fg = 0.0, fh = 0.0;
-  if (bin < s.total_bins) {
-    const size_t base = (size_t)it.part_first * stride + bin;
-    if (FORM == FORM_QUANT) {
-      for (int j = sp; j < it.nblk; j += 8) {
-        long long gg, hh, cc;
-        qunpack(s.part_q[base + (size_t)j * stride], &gg, &hh, &cc);
-        g += gg;
-        h += hh;
-        c += cc;
-      }
-    } else {
-      for (int j = sp; j < it.nblk; j += 8) {
-        fg += s.part_g[base + (size_t)j * stride];
-        fh += s.part_h[base + (size_t)j * stride];
-        c += s.part_c[base + (size_t)j * stride];
+  // items blockIdx.y, blockIdx.y + gridDim.y, ... (a planned round's grid is
+  // capped below the most items a round may hold)
+  const int nitems = hdr ? hdr->nscan : (int)gridDim.y;
+  for (int y = blockIdx.y; y < nitems; y += gridDim.y) {
+    const ReduceItem it = items[y];
+    const int lb = threadIdx.x & 31, sp = threadIdx.x >> 5;
+    const int bin = blockIdx.x * 32 + lb;
+    const size_t stride = (size_t)s.total_bins;
+    long long g = 0, h = 0, c = 0;
+    double fg = 0.0, fh = 0.0;
+    if (bin < s.total_bins) {
+      const size_t base = (size_t)it.part_first * stride + bin;
+      if (FORM == FORM_QUANT) {
+        for (int j = sp; j < it.nblk; j += 8) {
+          long long gg, hh, cc;
+          qunpack(s.part_q[base + (size_t)j * stride], &gg, &hh, &cc);
+          g += gg;
+          h += hh;
+          c += cc;
+        }
+      } else {
+        for (int j = sp; j < it.nblk; j += 8) {
+          fg += s.part_g[base + (size_t)j * stride];
+          fh += s.part_h[base + (size_t)j * stride];
+          c += s.part_c[base + (size_t)j * stride];
+        }
       }
     }
-  }
-  ra[sp][lb] = g;
-  rb[sp][lb] = h;
-  rc[sp][lb] = c;
-  da[sp][lb] = fg;
-  db[sp][lb] = fh;
-  __syncthreads();
-  if (sp == 0 && bin < s.total_bins) {
-    double* pool = s.pool + ((size_t)it.slot * s.total_bins) * 2;
-    int* cnt = s.pool_cnt + (size_t)it.slot * s.total_bins;
-    long long tc = 0;
-    if (FORM == FORM_QUANT) {
-      long long tg = 0, th = 0;
-      for (int k = 0; k < 8; ++k) {
-        tg += ra[k][lb];
-        th += rb[k][lb];
-        tc += rc[k][lb];
+    ra[sp][lb] = g;
+    rb[sp][lb] = h;
+    rc[sp][lb] = c;
+    da[sp][lb] = fg;
+    db[sp][lb] = fh;
+    __syncthreads();
+    if (sp == 0 && bin < s.total_bins) {
+      double* pool = s.pool + ((size_t)it.slot * s.total_bins) * 2;
+      int* cnt = s.pool_cnt + (size_t)it.slot * s.total_bins;
+      long long tc = 0;
+      if (FORM == FORM_QUANT) {
+        long long tg = 0, th = 0;
+        for (int k = 0; k < 8; ++k) {
+          tg += ra[k][lb];
+          th += rb[k][lb];
+          tc += rc[k][lb];
+        }
+        ((long long*)pool)[2 * bin] = tg;
+        ((long long*)pool)[2 * bin + 1] = th;
+      } else {
+        double tg = 0.0, th = 0.0;
+        for (int k = 0; k < 8; ++k) {
+          tg += da[k][lb];
+          th += db[k][lb];
+          tc += rc[k][lb];
+        }
+        pool[2 * bin] = tg;
+        pool[2 * bin + 1] = th;
       }
-      ((long long*)pool)[2 * bin] = tg;
-      ((long long*)pool)[2 * bin + 1] = th;
-    } else {
-      double tg = 0.0, th = 0.0;
-      for (int k = 0; k < 8; ++k) {
-        tg += da[k][lb];
-        th += db[k][lb];
-        tc += rc[k][lb];
-      }
-      pool[2 * bin] = tg;
-      pool[2 * bin + 1] = th;
+      cnt[bin] = (int)tc;
     }
-    cnt[bin] = (int)tc;
+    __syncthreads();  // the LDS planes are reused by the next item
   }
 }
 
@@ -2739,6 +2898,27 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
   BK_HIP(hipGetLastError());
 }
 
+__global__ void __launch_bounds__(256) k_plan_init(const PlanState* __restrict__ s_state,
+                                                  const PlanNode* __restrict__ s_node,
+                                                  const int* __restrict__ s_bytree, const uint8_t* __restrict__ s_flags,
+                                                  PlanState* st, PlanNode* node0, int* bytree, uint8_t* flags0, int F) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (int i = tid; i < (int)(sizeof(PlanState) / 4); i += nt) ((uint32_t*)st)[i] = ((const uint32_t*)s_state)[i];
+  for (int i = tid; i < (int)(sizeof(PlanNode) / 4); i += nt) ((uint32_t*)node0)[i] = ((const uint32_t*)s_node)[i];
+  for (int i = tid; i < F; i += nt) {
+    bytree[i] = s_bytree[i];
+    flags0[i] = s_flags[i];
+  }
+}
+
+void launch_plan_init(const PlanState* s_state, const PlanNode* s_node, const int* s_bytree, const uint8_t* s_flags,
+                      PlanState* st, PlanNode* node0, int* bytree, uint8_t* flags0, int F, void* stream) {
+  static_assert(sizeof(PlanState) % 4 == 0 && sizeof(PlanNode) % 4 == 0, "word copies");
+  launch(k_plan_init, dim3(4), dim3(256), 0, (hipStream_t)stream, s_state, s_node, s_bytree, s_flags, st, node0,
+         bytree, flags0, F);
+  BK_HIP(hipGetLastError());
+}
+
 void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const FoldOut* fold, const uint8_t* fold_flags,
                  const FeatDev* feats, const int* bytree, const PlanParams& pp, PartItem* pitems, HistItem* hitems,
                  ScanItem* sitems, ReduceItem* ritems, int* used, void* stream) {
@@ -2785,7 +2965,8 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
 
 void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream, const RoundHdr* hdr) {
   if (nitems <= 0) return;
-  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)nitems);
+  // a planned round's kernel loops over its items: a capped grid holds them
+  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)(hdr ? std::min(nitems, 16) : nitems));
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
@@ -2806,7 +2987,7 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
                  SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr) {
   if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid((unsigned)s.num_features, (unsigned)nitems);
+  dim3 grid((unsigned)s.num_features, (unsigned)(hdr ? std::min(nitems, 64) : nitems));
   switch (s.form) {
     case FORM_QUANT:
       launch(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out, hdr);
@@ -2824,9 +3005,8 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
   BK_HIP(hipGetLastError());
 }
 
-void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, void* stream) {
-  launch(k_max_abs, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, n,
-                     d_out4);
+void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, long long* d_zero4, void* stream) {
+  launch(k_max_abs, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, n, d_out4, d_zero4);
   BK_HIP(hipGetLastError());
 }
 

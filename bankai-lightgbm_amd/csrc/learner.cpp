@@ -1259,7 +1259,7 @@ struct Learner {
   // keeps the previous discretization (serial.rs:692-703).
   bool discretize(const float* g, const float* h) {
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
-    launch_max_abs(g, h, N, d_max4, stream);
+    launch_max_abs(g, h, N, d_max4, d_sums3, stream);
     unsigned mx[4];
     launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
     sync_spin();
@@ -1281,7 +1281,6 @@ struct Learner {
         const int v = (int)use_start_eng.uint_incl((uint32_t)N);
         if (j == tree_slot) start = v;
       }
-    BK_HIPC(hipMemsetAsync(d_sums3, 0, sizeof(long long) * 4, stream));
     const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
     timed(4, db, db, [&] {
       launch_discretize(g, h, d_gnoise, d_hnoise, N, start, inv_g, inv_h, const_hess ? 1 : 0, stochastic ? 1 : 0,
@@ -2112,7 +2111,15 @@ struct Learner {
       si.small.nblk = hist_count[0];
       nr = plan_scan(1);
     }
-    upload_round(0, &hp, search_root ? 1 : 0, nr, true);
+    // one upload: the root's items and the planner's initial state
+    begin_uploads();
+    if (search_root) {
+      d_hitems = put(h_hitems, (size_t)hp.ni);
+      d_hblock = put(h_hblock, (size_t)hp.nblocks);
+      d_sitems = put(h_sitems, 1);
+      d_ritems = put(h_ritems, (size_t)nr);
+      d_used = put(h_used, (size_t)F);
+    }
     // the planner's initial state: leaf 0 is node 0, searched by the root item
     plan_init = PlanState{};
     plan_init.num_leaves = 1;
@@ -2140,10 +2147,7 @@ struct Learner {
     const int* u_bytree = put(bytree_int.data(), (size_t)F);
     const uint8_t* u_ones = put(ones_f.data(), (size_t)F);
     send(true);
-    launch_copy(d_pstate, u_state, sizeof(PlanState), stream);
-    launch_copy(d_pnodes, u_node, sizeof(PlanNode), stream);
-    launch_copy(d_bytree, u_bytree, sizeof(int) * (size_t)F, stream);
-    launch_copy(d_pnflags, u_ones, (size_t)F, stream);
+    launch_plan_init(u_state, u_node, u_bytree, u_ones, d_pstate, d_pnodes, d_bytree, d_pnflags, F, stream);
     if (search_root) {
       run_hist(hp, root.num_data, true);
       stats.committed_hist_rows += root.num_data;
@@ -2177,7 +2181,9 @@ struct Learner {
     const long long hb = std::max<long long>(std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows)),
                                              ((long long)N + rows_per_flush - 1) / rows_per_flush);
     plan();
-    int r = 1, want = std::max(4, last_plan_rounds + 1);
+    // as many rounds as the last tree planned (trees of one model have similar
+    // depths): the host waits once; a deeper tree takes more rounds in batches
+    int r = 1, want = last_plan_rounds > 0 ? last_plan_rounds : 8;
     const size_t back = sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes;
     const PlanState* hs = (const PlanState*)h_plan;
     for (;;) {

@@ -26,6 +26,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+ROUND_PLANNER_ON_DEVICE = True
+
+
 def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     """profile: False, True (every kernel kind) or a list of kind names to time,
     on the launches of every `every_trees`-th tree."""
@@ -38,6 +41,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     obj = bst.BinaryObjective(y)
     init = obj.boost_from_score(0)
     learner = bg.Learner(ds, params, False)
+    learner.set_round_planner(ROUND_PLANNER_ON_DEVICE)
     n = ds.num_data
     dev = torch.device("cuda:0")
     score = torch.full((n,), init, dtype=torch.float64, device=dev)
@@ -312,7 +316,11 @@ def main():
                     help="documents of the LambdaRank workload measured beside the headline (0 skips it)")
     ap.add_argument("--ranking-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
+    ap.add_argument("--round-planner", choices=("device", "host"), default="device",
+                    help="who plans the rounds of batched growth")
     args = ap.parse_args()
+    global ROUND_PLANNER_ON_DEVICE
+    ROUND_PLANNER_ON_DEVICE = args.round_planner == "device"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the learner has no multi-GPU path (rows would shard across GPUs "
               "with one histogram all-reduce per build; not built).", file=sys.stderr)
