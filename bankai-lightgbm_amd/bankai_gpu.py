@@ -615,9 +615,10 @@ class Learner:
                                                            C.c_void_p(d_hessians_ptr), C.c_void_p(d_hist_ptr),
                                                            C.c_void_p(d_count_ptr)))
 
-    def set_round_planner(self, on_device):
-        """Batched growth planned on the device (True, the default) or the host."""
-        _check(self._lib.bk_learner_set_round_planner(self.handle, int(bool(on_device))))
+    def set_round_planner(self, mode):
+        """Batched growth planned on the host (0, the default), on the device for
+        the whole tree (1), or on the device for the rounds only (2)."""
+        _check(self._lib.bk_learner_set_round_planner(self.handle, int(mode)))
 
     def set_hist_tiling(self, min_rows=0, max_blocks=0):
         """Forces the histogram tiling (0, 0: the autotuner's choice)."""

@@ -165,6 +165,7 @@ struct ScanParams {
   int min_data_in_leaf, max_cat_threshold, max_cat_to_onehot, min_data_per_group;
   double grad_scale, hess_scale, h0;  // quantized scales; h0 = hessians[0] for a constant hessian
   int const_hess;
+  const double* dyn;  // when set: grad_scale, hess_scale, h0 are read from the device (a DiscState)
 };
 
 // Device buffers of the learner that kernels read.
@@ -276,6 +277,37 @@ struct PlanParams {
   int max_nodes, num_slots, max_partials, pad1;
 };
 
+// ---------------------------------------------------------------------------
+// Device-resident trees: the discretizer's scales, the root sums and the tree's
+// final tables are computed on the device, so a tree runs without a host wait
+// until its end.
+// ---------------------------------------------------------------------------
+// The discretizer's state, kept across trees (a refused discretization keeps the
+// previous one, serial.rs:692-703). The search reads gs, hs, h0 (ScanParams::dyn).
+struct DiscState {
+  double gs, hs, h0;
+  double inv_g, inv_h;
+  double root_g, root_h;      // root sums of the last accepted discretization
+  long long int_g, int_h;     // and their exact integer sums
+  int ok, start, pad0, pad1;  // ok: this tree's discretization was accepted
+};
+// Scalars of the root set-up.
+struct RootArgs {
+  int num_data, min_data_in_leaf, quantized, const_hess;
+  int nparts, bins, pad0, pad1;  // nparts: f64 partial sums of g and h (float forms)
+  double l1, l2, max_delta_step;
+};
+// Scales from max |g|, max h (zeroed for the next tree) and the noise rotation
+// drawn by the host (*start_word).
+void launch_disc_prep(unsigned* d_max4, const int* start_word, int bins, int const_hess, DiscState* d, void* stream);
+// The root's sums (exact integer sums times the scales when quantized; the
+// partial sums in order otherwise), then the planner's initial state, the root
+// node, the tree's feature sample and the root's histogram, reduce and search
+// items with the round header.
+void launch_root_prep(DiscState* d, const long long* sums3, const double* partials, const float* hessians,
+                      const RootArgs& ra, const PlanParams& pp, PlanState* st, PlanNode* nodes, uint8_t* node_flags,
+                      const int* bytree_src, int* bytree, HistItem* hitems, ScanItem* sitems, ReduceItem* ritems,
+                      int* used, int* err, void* stream);
 // Kernel timing. While a source is set (per host thread), every launch below takes
 // a fresh (start, stop) pair of timing events (hipEvent_t as void*) from it and is
 // launched with hipExtLaunchKernel, which stamps the kernel's own begin and end
@@ -332,9 +364,11 @@ void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const Fold
 // max |g|, max h and a bad-value flag into d_out4 (zeroed before); d_zero4 (may
 // be null): four int64 zeroed on the way, for the discretizer's sums.
 void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, long long* d_zero4, void* stream);
+// d_state (may be null): start and inverse scales from it; nothing is written when
+// its discretization was refused.
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
-                       long long* d_sums3, void* stream);
+                       long long* d_sums3, void* stream, const DiscState* d_state = nullptr);
 void launch_pack_f32(const float* g, const float* h, int n, int const_hess, float* gh, void* stream);
 // One internal node of the last tree for the row-to-leaf traversal: the route
 // of every bin value under the rule its split applied in the partition (bit b:
@@ -354,8 +388,15 @@ struct NodeRule {
 // record in buffer 0 (original row order). num_nodes == 0: every row is in leaf 0.
 // bag_bits (bit r: row r was in the tree's bag; null: every row was): a row
 // outside the bag gets ~leaf.
+// d_num_nodes (may be null): the node count on the device, num_nodes then the most.
 void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
-                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream);
+                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream, const int* d_num_nodes = nullptr);
+// The committed tree's walk rules (rule, group, children: a later commit of the
+// same leaf index, or ~leaf), the leaf segments and the leaves by row count
+// (descending, for the renewal chains) for num_leaves leaf slots; *d_count =
+// committed splits.
+void launch_finalize(const PlanState* st, const PlanNode* nodes, const FeatDev* feats, NodeRule* rules, int* d_count,
+                     Segment* segs, int* order, int num_leaves, void* stream);
 // Leaf renewal sums: per-block partials over leaf_of, the certificate per leaf,
 // then the sequential chain for flagged leaves (d_flags == nullptr runs the
 // chain for every leaf).

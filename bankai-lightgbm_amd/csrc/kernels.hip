@@ -1598,7 +1598,13 @@ __device__ void fold_side(const ScanItem& it, int k, int side, const SplitOut* o
 // the last wave-level steps).
 template <int FORM>
 __device__ __forceinline__ void scan_item(const DevState& s, const ScanItem* __restrict__ items, int y,
-                                          const ScanParams& p, const int* __restrict__ used_all, SplitOut* out_all) {
+                                          const ScanParams& p0, const int* __restrict__ used_all, SplitOut* out_all) {
+  ScanParams p = p0;
+  if (p0.dyn) {  // scales computed on the device for this tree
+    p.grad_scale = p0.dyn[0];
+    p.hess_scale = p0.dyn[1];
+    p.h0 = p0.dyn[2];
+  }
   __shared__ SliceLds S[2];
   __shared__ ScanPrefix X[2];
   __shared__ int elig[2][kMaxBins];
@@ -2132,7 +2138,14 @@ __global__ void __launch_bounds__(256) k_max_abs(const float* __restrict__ g, co
 
 __global__ void __launch_bounds__(256) k_discretize(const float* __restrict__ g, const float* __restrict__ h, const double* __restrict__ gn,
                              const double* __restrict__ hn, int n, int start, double inv_g, double inv_h,
-                             int const_hess, int stochastic, int16_t* __restrict__ packed, long long* sums) {
+                             int const_hess, int stochastic, int16_t* __restrict__ packed, long long* sums,
+                             const DiscState* __restrict__ dst) {
+  if (dst) {  // a device-resident tree: the set-up kernel decided
+    if (!dst->ok) return;  // refused: the previous discretization is kept
+    start = dst->start;
+    inv_g = dst->inv_g;
+    inv_h = dst->inv_h;
+  }
   long long sg = 0, sh = 0, sa = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double gd = (double)g[i];
@@ -2282,7 +2295,9 @@ __device__ __forceinline__ uint32_t record_byte(const uint4& c0, const uint4& c1
 }
 
 // Route bitmap of node blockIdx.x: lane b evaluates the node's rule on bin b.
-__global__ void __launch_bounds__(256) k_node_bitmaps(const NodeRule* __restrict__ rules, NodeDev* __restrict__ nodes) {
+__global__ void __launch_bounds__(256) k_node_bitmaps(const NodeRule* __restrict__ rules, NodeDev* __restrict__ nodes,
+                                                      const int* __restrict__ d_count) {
+  if (d_count && (int)blockIdx.x >= *d_count) return;  // block-uniform
   const NodeRule& r = rules[blockIdx.x];
   const uint32_t b = threadIdx.x;
   const unsigned long long m = __ballot(route_left(r.rule, b));
@@ -2397,8 +2412,9 @@ __host__ __device__ inline size_t walk_stage_bytes(int stride) { return (size_t)
 
 __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev* __restrict__ nodes, int num_nodes,
                                                       int in_lds, const uint32_t* __restrict__ bag_bits,
-                                                      int32_t* __restrict__ leaf_of) {
+                                                      int32_t* __restrict__ leaf_of, const int* __restrict__ d_count) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (d_count) num_nodes = min(*d_count, num_nodes);  // the LDS holds the most
   if (num_nodes == 0) {
     for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < s.num_data;
          row += (long long)gridDim.x * blockDim.x)
@@ -2898,6 +2914,189 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
   BK_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Device-resident trees
+// ---------------------------------------------------------------------------
+// update_scales (gradient_discretizer/mod.rs:134-165) from the maxima, one lane.
+__global__ void k_disc_prep(unsigned* __restrict__ max4, const int* __restrict__ start_word, int bins, int const_hess,
+                            DiscState* __restrict__ d) {
+  if (threadIdx.x != 0) return;
+  const unsigned m0 = max4[0], m1 = max4[1], m2 = max4[2];
+  max4[0] = max4[1] = max4[2] = max4[3] = 0u;  // the next tree's maxima start from zero
+  const float max_g = __uint_as_float(m0), max_h = __uint_as_float(m1);
+  const int ok = m2 == 0u && max_g != 0.0f && (const_hess || max_h != 0.0f);
+  d->ok = ok;
+  if (!ok) return;
+  d->gs = (double)max_g / (double)(bins / 2);
+  d->hs = const_hess ? (double)max_h : (double)max_h / (double)bins;
+  d->inv_g = 1.0 / d->gs;
+  d->inv_h = 1.0 / d->hs;
+  d->start = *start_word;
+}
+
+// Σ|x| * odd part of the scale's significand < 2^53: every sequential partial sum
+// of x_i * scale is exact, so the reference's ordered sum equals the exact one.
+__device__ bool sum_exact(long long sum_abs, double scale) {
+  if (scale == 0.0) return true;
+  int e;
+  const double m = frexp(scale, &e);
+  unsigned long long mant = (unsigned long long)ldexp(m, 53);
+  while (mant && !(mant & 1ull)) mant >>= 1;
+  return (unsigned __int128)(unsigned long long)sum_abs * mant < ((unsigned __int128)1 << 53);
+}
+
+__device__ double leaf_output_root_dev(double g, double h, double l1, double l2, double mds) {
+  const double reg = fmax(fabs(g) - l1, 0.0);
+  double ret = -(sgn(g) * reg) / (h + l2);
+  if (mds > 0.0 && fabs(ret) > mds) ret = sgn(ret) * mds;
+  if (ret < -DBL_MAX) return -DBL_MAX;  // NaN passes through, as on the host
+  if (ret > DBL_MAX) return DBL_MAX;
+  return ret;
+}
+
+__global__ void __launch_bounds__(256) k_root_prep(DiscState* __restrict__ d, const long long* __restrict__ sums3,
+                                                   const double* __restrict__ partials,
+                                                   const float* __restrict__ hessians, RootArgs ra, PlanParams pp,
+                                                   PlanState* __restrict__ st, PlanNode* __restrict__ nodes,
+                                                   uint8_t* __restrict__ node_flags,
+                                                   const int* __restrict__ bytree_src, int* __restrict__ bytree,
+                                                   HistItem* __restrict__ hitems, ScanItem* __restrict__ sitems,
+                                                   ReduceItem* __restrict__ ritems, int* __restrict__ used,
+                                                   int* __restrict__ err) {
+  const int tid = threadIdx.x, F = pp.F, N = ra.num_data;
+  for (int f = tid; f < F; f += blockDim.x) {
+    const int b = bytree_src[f];
+    bytree[f] = b;
+    used[f] = b ? 7 : 0;
+    node_flags[f] = 1;
+  }
+  for (int i = tid; i < (int)(sizeof(PlanState) / 4); i += blockDim.x) ((uint32_t*)st)[i] = 0u;
+  __syncthreads();
+  if (tid != 0) return;
+  double sg = 0.0, sh = 0.0;
+  long long isum = 0;
+  if (ra.quantized) {
+    if (d->ok) {
+      // before_train's root sums (serial.rs:741-761)
+      if (!sum_exact(sums3[2], d->gs) || !sum_exact(sums3[1], d->hs)) atomicOr(err, 16);
+      d->int_g = sums3[0];
+      d->int_h = sums3[1];
+      d->root_g = (double)sums3[0] * d->gs;
+      d->root_h = (double)sums3[1] * d->hs;
+    }
+    sg = d->root_g;
+    sh = d->root_h;
+    isum = (long long)(((unsigned long long)d->int_g << 32) + (unsigned long long)d->int_h);
+  } else {
+    for (int b = 0; b < ra.nparts; ++b) {
+      sg += partials[2 * b];
+      sh += partials[2 * b + 1];
+    }
+    d->h0 = (double)hessians[0];
+  }
+  const bool search = N > 0 && N >= ra.min_data_in_leaf * 2;
+  PlanNode& n0 = nodes[0];
+  n0.seg = Segment{0, 0, N};
+  n0.depth = 0;
+  n0.slot = 0;
+  n0.expanded = 0;
+  n0.kids_searched = 0;
+  n0.child[0] = n0.child[1] = -1;
+  n0.num_data = N;
+  n0.pad = 0;
+  n0.sum_g = sg;
+  n0.sum_h = sh;
+  n0.weight = 0.0;
+  n0.int_sum = isum;
+  n0.best.o = SplitOut{};
+  n0.best.o.gain = kMinScore;
+  n0.best.o.feature = -1;
+  n0.best.o.default_left = 1;
+  n0.best.inner = -1;
+  st->num_leaves = 1;
+  st->num_nodes = 1;
+  st->next_slot = 1;
+  st->nabsorb = search ? 1 : 0;
+  st->absorb[0] = 0;
+  st->absorb[1] = -1;
+  st->leaf_node[0] = 0;
+  // the root's histogram over every row, tiled as the host plans it
+  long long nb = min((long long)pp.hist_max_blocks, max(1ll, (long long)N / pp.hist_min_rows));
+  nb = max(nb, ((long long)N + pp.rows_per_flush - 1) / pp.rows_per_flush);
+  const int rpb = (int)(((long long)N + nb - 1) / nb);
+  const int nblocks = N > 0 ? (N + rpb - 1) / rpb : 0;
+  hitems[0] = HistItem{Segment{0, 0, N}, 0, 0};
+  ritems[0] = ReduceItem{0, 0, nblocks, 0};
+  ScanItem si{};
+  si.small = LeafArgs{0, 0, N, sg, sh, leaf_output_root_dev(sg, sh, ra.l1, ra.l2, ra.max_delta_step), isum, 0,
+                      nblocks};
+  sitems[0] = si;
+  st->hdr = RoundHdr{0, 0, search ? 1 : 0, N, nblocks, rpb, search ? 1 : 0, 0};
+}
+
+__global__ void __launch_bounds__(256) k_finalize(const PlanState* __restrict__ st, const PlanNode* __restrict__ nodes,
+                                                  const FeatDev* __restrict__ feats, NodeRule* __restrict__ rules,
+                                                  int* __restrict__ d_count, Segment* __restrict__ segs,
+                                                  int* __restrict__ order, int num_leaves) {
+  __shared__ int cnt[kPlanMaxLeaves];
+  const int tid = threadIdx.x, nc = st->ncommit, nl = st->num_leaves;
+  if (tid == 0) *d_count = nc;
+  for (int c = tid; c < nc; c += blockDim.x) {
+    const int leaf = st->commit[2 * c], x = st->commit[2 * c + 1];
+    // the node's children: the first later commit that splits the same leaf index
+    // (the left child keeps it; the right child is leaf c + 1), else the leaf
+    int left = ~leaf, right = ~(c + 1);
+    for (int c2 = nc - 1; c2 > c; --c2) {
+      const int l2 = st->commit[2 * c2];
+      if (l2 == leaf) left = c2;
+      if (l2 == c + 1) right = c2;
+    }
+    const SplitOut& o = nodes[x].best.o;
+    const FeatDev m = feats[nodes[x].best.inner];
+    NodeRule r{};
+    r.rule = m.bin_type == 1 ? make_rule_categorical(m, o.cat_bits)
+                             : make_rule_numerical(m, (uint32_t)o.threshold, o.default_left != 0);
+    r.group = m.group;
+    r.left = left;
+    r.right = right;
+    rules[c] = r;
+  }
+  for (int l = tid; l < num_leaves; l += blockDim.x) {
+    const Segment sg = l < nl ? nodes[st->leaf_node[l]].seg : Segment{0, 0, 0};
+    segs[l] = sg;
+    cnt[l] = sg.count;
+  }
+  __syncthreads();
+  // leaves by row count, descending, ties by index
+  for (int l = tid; l < num_leaves; l += blockDim.x) {
+    int r = 0;
+    for (int m = 0; m < num_leaves; ++m) r += (cnt[m] > cnt[l] || (cnt[m] == cnt[l] && m < l)) ? 1 : 0;
+    order[r] = l;
+  }
+}
+
+void launch_disc_prep(unsigned* d_max4, const int* start_word, int bins, int const_hess, DiscState* d, void* stream) {
+  launch(k_disc_prep, dim3(1), dim3(64), 0, (hipStream_t)stream, d_max4, start_word, bins, const_hess, d);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_root_prep(DiscState* d, const long long* sums3, const double* partials, const float* hessians,
+                      const RootArgs& ra, const PlanParams& pp, PlanState* st, PlanNode* nodes, uint8_t* node_flags,
+                      const int* bytree_src, int* bytree, HistItem* hitems, ScanItem* sitems, ReduceItem* ritems,
+                      int* used, int* err, void* stream) {
+  launch(k_root_prep, dim3(1), dim3(256), 0, (hipStream_t)stream, d, sums3, partials, hessians, ra, pp, st, nodes,
+         node_flags, bytree_src, bytree, hitems, sitems, ritems, used, err);
+  BK_HIP(hipGetLastError());
+}
+
+void launch_finalize(const PlanState* st, const PlanNode* nodes, const FeatDev* feats, NodeRule* rules, int* d_count,
+                     Segment* segs, int* order, int num_leaves, void* stream) {
+  if (num_leaves > kPlanMaxLeaves) throw std::invalid_argument("the device planner takes at most 256 leaves");
+  launch(k_finalize, dim3(1), dim3(256), 0, (hipStream_t)stream, st, nodes, feats, rules, d_count, segs, order,
+         num_leaves);
+  BK_HIP(hipGetLastError());
+}
+
 __global__ void __launch_bounds__(256) k_plan_init(const PlanState* __restrict__ s_state,
                                                   const PlanNode* __restrict__ s_node,
                                                   const int* __restrict__ s_bytree, const uint8_t* __restrict__ s_flags,
@@ -3012,9 +3211,9 @@ void launch_max_abs(const float* g, const float* h, int n, unsigned* d_out4, lon
 
 void launch_discretize(const float* g, const float* h, const double* gnoise, const double* hnoise, int n, int start,
                        double inv_g, double inv_h, int const_hess, int stochastic, int16_t* packed,
-                       long long* d_sums3, void* stream) {
+                       long long* d_sums3, void* stream, const DiscState* d_state) {
   launch(k_discretize, dim3(grid_for(n, 256 * 8, 1024)), dim3(256), 0, (hipStream_t)stream, g, h, gnoise,
-                     hnoise, n, start, inv_g, inv_h, const_hess, stochastic, packed, d_sums3);
+                     hnoise, n, start, inv_g, inv_h, const_hess, stochastic, packed, d_sums3, d_state);
   BK_HIP(hipGetLastError());
 }
 
@@ -3129,9 +3328,10 @@ int launch_sum_gh(const float* g, const float* h, const int32_t* rows, int n, in
 }
 
 void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
-                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream) {
+                         const uint32_t* bag_bits, int32_t* leaf_of, void* stream, const int* d_num_nodes) {
   if (num_nodes > 0) {
-    launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes);
+    launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes,
+           d_num_nodes);
     BK_HIP(hipGetLastError());
   }
   const size_t node_bytes = (size_t)num_nodes * sizeof(NodeDev);
@@ -3140,7 +3340,7 @@ void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_
   const size_t lds = staged ? node_bytes + walk_stage_bytes(s.stride) : node_bytes;
   allow_lds(k_leaf_of_tree);
   launch(k_leaf_of_tree, dim3(grid_for(s.num_data, 256, 4096)), dim3(256), in_lds ? lds : 0, (hipStream_t)stream, s,
-         d_nodes, num_nodes, in_lds, bag_bits, leaf_of);
+         d_nodes, num_nodes, in_lds, bag_bits, leaf_of, d_num_nodes);
   BK_HIP(hipGetLastError());
 }
 

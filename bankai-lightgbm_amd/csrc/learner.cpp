@@ -680,7 +680,7 @@ struct Learner {
   // candidates and searched bits; its best is folded on the host when the reference
   // would draw its sample (when its parent's split is committed)
   // device round planner: batched growth whose rounds queue without a host wait
-  bool plan_on_device = true;          // bk_learner_set_round_planner
+  bool plan_on_device = false;         // bk_learner_set_round_planner
   PlanState* d_pstate = nullptr;
   PlanNode* d_pnodes = nullptr;
   uint8_t* d_pnflags = nullptr;        // splittable flags of every node
@@ -697,6 +697,22 @@ struct Learner {
   std::vector<int> bytree_int;
   std::vector<uint8_t> ones_f;
   int plan_max_nodes = 0, hist_grid_blocks = 0, last_plan_rounds = 0;
+  // device-resident trees: one host wait per tree
+  bool resident_on = false;            // bk_learner_set_round_planner(…, 1) turns it on
+  DiscState* d_disc = nullptr;
+  int* d_node_count = nullptr;
+  int* h_start = nullptr;              // pinned: the noise rotation drawn for the tree
+  int* h_bytree = nullptr;             // pinned: the tree's feature sample
+  DiscState* h_disc = nullptr;         // pinned copy of the discretizer state at a tree's end
+  // trees whose root sums and renewal need no host fallback: integer root sums
+  // are exact in any order when every partial sum fits the f64 significand,
+  // which holds by construction for power-of-two bins and N * (bins + 1) < 2^29
+  // (the scales carry at most the 24 significand bits of an f32 maximum)
+  bool resident_ok(bool bagged) const {
+    if (!resident_on || !planner_ok() || bagged || !d_disc) return false;
+    if (quant && ((bins & (bins - 1)) != 0 || (double)N * (bins + 1) >= (double)(1 << 29))) return false;
+    return true;
+  }
   bool planner_ok() const {
     return plan_on_device && batched && d_pstate && !cols.by_node_active() && P.num_leaves <= kPlanMaxLeaves;
   }
@@ -783,11 +799,12 @@ struct Learner {
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay,
                     (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
-                    (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree})
+                    (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree,
+                    (void*)d_disc, (void*)d_node_count})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc, (void*)h_stamp,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
-                    (void*)h_out, (void*)h_plan})
+                    (void*)h_out, (void*)h_plan, (void*)h_start, (void*)h_bytree, (void*)h_disc})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -1010,6 +1027,7 @@ struct Learner {
       h_fold = (FoldOut*)(h_back + off_fold);
     }
     d_max4 = dmalloc<unsigned>(4);
+    BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
     d_sums3 = dmalloc<long long>(4);
     d_sum_partials = dmalloc<double>(2048 + 2);
     d_segs = dmalloc<Segment>((size_t)P.num_leaves);
@@ -1130,6 +1148,12 @@ struct Learner {
       d_pused = dmalloc<int>((size_t)F * max_items);
       d_bytree = dmalloc<int>((size_t)F);
       pinned(&h_plan, sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes);
+      d_disc = dmalloc<DiscState>(1);
+      BK_HIPC(hipMemsetAsync(d_disc, 0, sizeof(DiscState), stream));
+      d_node_count = dmalloc<int>(1);
+      pinned(&h_start, 1);
+      pinned(&h_bytree, (size_t)F);
+      pinned(&h_disc, 1);
       bytree_int.assign((size_t)F, 1);
       ones_f.assign((size_t)F, 1);
       // the most row blocks a planned round's histogram may cut (its rows are at most N)
@@ -1262,6 +1286,8 @@ struct Learner {
     launch_max_abs(g, h, N, d_max4, d_sums3, stream);
     unsigned mx[4];
     launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
+    // the maxima start from zero for the next tree on every path
+    BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
     sync_spin();
     std::memcpy(mx, h_misc, sizeof mx);
     float max_g, max_h;
@@ -1372,6 +1398,22 @@ struct Learner {
       // on refusal the previous discretization and its sums are kept (serial.rs:692-703)
       sg = last_root_g;
       sh = last_root_h;
+      if (d_disc) {
+        // a later device-resident tree keeps this state when its discretization is refused
+        stage_wait();
+        DiscState dsn{};
+        dsn.gs = grad_scale;
+        dsn.hs = hess_scale;
+        dsn.inv_g = grad_scale != 0.0 ? 1.0 / grad_scale : 0.0;
+        dsn.inv_h = hess_scale != 0.0 ? 1.0 / hess_scale : 0.0;
+        dsn.root_g = last_root_g;
+        dsn.root_h = last_root_h;
+        dsn.int_g = last_int_g;
+        dsn.int_h = last_int_h;
+        *h_disc = dsn;
+        launch_copy(d_disc, dev_alias(h_disc), sizeof(DiscState), stream);
+        staged();
+      }
       isum = (long long)(((unsigned long long)last_int_g << 32) + (unsigned long long)last_int_h);
     } else {
       launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
@@ -2082,6 +2124,95 @@ struct Learner {
     return t;
   }
 
+  PlanParams plan_params() const {
+    PlanParams pp{};
+    pp.num_leaves = P.num_leaves;
+    pp.F = F;
+    pp.max_depth = P.max_depth;
+    pp.min_data_in_leaf = P.min_data_in_leaf;
+    pp.hist_min_rows = hist_min_rows;
+    pp.hist_max_blocks = hist_max_blocks;
+    pp.rows_per_flush = rows_per_flush;
+    pp.part_tile = part_tile;
+    pp.max_nodes = plan_max_nodes;
+    pp.num_slots = (int)num_slots;
+    pp.max_partials = max_partials;
+    return pp;
+  }
+  // the most row blocks a planned histogram cuts: every row in one build
+  int plan_hist_grid() const {
+    return (int)std::max<long long>(std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows)),
+                                    ((long long)N + rows_per_flush - 1) / rows_per_flush);
+  }
+  void launch_plan_round() {
+    const PlanParams pp = plan_params();
+    timed(7, 0.0, 0.0, [&] {
+      launch_plan(d_pstate, d_pnodes, d_pnflags, d_pfold, d_pfflags, d_feats, d_bytree, pp, d_ppart, d_phist, d_pscan,
+                  d_pred, d_pused, stream);
+    });
+  }
+  // `count` planned rounds: the kernels of the round the last plan set up, then
+  // the next plan. Grids hold the most work a round can: every row partitioned,
+  // every row in a histogram, 2^(r-1) items in round r (the tree's depth at
+  // round r is at most r - 1), at most max_items.
+  void launch_rounds(int& r, int count, const ScanParams& sp, int hb) {
+    const RoundHdr* hdr = &d_pstate->hdr;
+    for (int k = 0; k < count; ++k, ++r) {
+      const int items = std::min(max_items, 1 << std::min(r - 1, 20));
+      ++epoch;
+      timed(2, 0.0, 0.0, [&] {
+        launch_partition(ds, d_ppart, max_items, max_part_tiles, d_cursors, max_items, epoch, stream, hdr);
+      });
+      timed(1, 0.0, 0.0, [&] {
+        launch_hist(ds, d_phist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
+                    (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
+      });
+      timed(3, 0.0, 0.0, [&] {
+        launch_reduce(ds, d_pred, items, stream, hdr);
+        launch_scan(ds, d_pscan, items, sp, d_pused, d_out, d_pfold, d_pfflags, stream, hdr);
+      });
+      launch_plan_round();
+    }
+  }
+
+  // The tree from the planner's state and node table read back into h_plan: the
+  // reference's commits replayed in order (commit_split), the leaves' segments,
+  // and the round statistics.
+  void rebuild_from_plan(Tree& t, bool search_root) {
+    const PlanState* hs = (const PlanState*)h_plan;
+    if (hs->pad) throw std::runtime_error("device planner overflow (nodes, slots or partials)");
+    last_plan_rounds = hs->rounds;
+    const PlanNode* hn = (const PlanNode*)(h_plan + sizeof(PlanState));
+    if (hs->ncommit < 0 || hs->ncommit > P.num_leaves - 1) throw std::runtime_error("device planner: bad commit count");
+    for (int c = 0; c < hs->ncommit; ++c) {
+      const int bl = hs->commit[2 * c], x = hs->commit[2 * c + 1];
+      HostSplit info;
+      info.o = hn[x].best.o;
+      info.inner = hn[x].best.inner;
+      if (bl < 0 || bl >= t.num_leaves || x < 0 || x >= hs->num_nodes || info.inner < 0 || info.inner >= F)
+        throw std::runtime_error("device planner: bad commit");
+      commit_split(t, bl, info);
+    }
+    if (t.num_leaves != hs->num_leaves) throw std::runtime_error("device planner: leaf count disagrees");
+    if (hs->num_nodes < 1 || hs->num_nodes > plan_max_nodes) throw std::runtime_error("device planner: bad node count");
+    for (int l = 0; l < t.num_leaves; ++l) {
+      const int x = hs->leaf_node[l];
+      if (x < 0 || x >= hs->num_nodes) throw std::runtime_error("device planner: bad leaf node");
+      segs[(size_t)l] = hn[x].seg;
+    }
+    int expanded = 0;
+    for (int i = 0; i < hs->num_nodes; ++i) expanded += hn[i].expanded ? 1 : 0;
+    stats.speculative_waste = expanded - (t.num_leaves - 1);
+    stats.hist_rows += hs->hist_rows;
+    stats.part_rows += hs->part_rows;
+    stats.rounds += hs->search_rounds + (search_root ? 1 : 0);
+    if (prof.tree_on) {
+      const double hr = (double)hs->hist_rows, pr = (double)hs->part_rows;
+      if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
+      if ((prof.mask >> 2) & 1u) prof.add_bytes(2, pr * 9.0, pr * 2.0 * stride);
+    }
+  }
+
   // ===========================================================================
   // Batched form with the rounds planned on the device: the host launches the
   // root's search, then rounds of [plan, partition, histogram, reduce, search]
@@ -2156,53 +2287,15 @@ struct Learner {
         launch_scan(ds, d_sitems, 1, sp, d_used, d_out, d_pfold, d_pfflags, stream);
       });
     }
-    PlanParams pp{};
-    pp.num_leaves = P.num_leaves;
-    pp.F = F;
-    pp.max_depth = P.max_depth;
-    pp.min_data_in_leaf = P.min_data_in_leaf;
-    pp.hist_min_rows = hist_min_rows;
-    pp.hist_max_blocks = hist_max_blocks;
-    pp.rows_per_flush = rows_per_flush;
-    pp.part_tile = part_tile;
-    pp.max_nodes = plan_max_nodes;
-    pp.num_slots = (int)num_slots;
-    pp.max_partials = max_partials;
-    const RoundHdr* hdr = &d_pstate->hdr;
-    auto plan = [&] {
-      timed(7, 0.0, 0.0, [&] {
-        launch_plan(d_pstate, d_pnodes, d_pnflags, d_pfold, d_pfflags, d_feats, d_bytree, pp, d_ppart, d_phist,
-                    d_pscan, d_pred, d_pused, stream);
-      });
-    };
-    // grids for the most work a round can hold: every row partitioned, every row
-    // in a histogram, 2^(r-1) items in round r (the tree's depth at round r is at
-    // most r - 1), at most max_items
-    const long long hb = std::max<long long>(std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows)),
-                                             ((long long)N + rows_per_flush - 1) / rows_per_flush);
-    plan();
+    const int hb = plan_hist_grid();
+    launch_plan_round();
     // as many rounds as the last tree planned (trees of one model have similar
     // depths): the host waits once; a deeper tree takes more rounds in batches
     int r = 1, want = last_plan_rounds > 0 ? last_plan_rounds : 8;
     const size_t back = sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes;
     const PlanState* hs = (const PlanState*)h_plan;
     for (;;) {
-      for (int k = 0; k < want; ++k, ++r) {
-        const int items = std::min(max_items, 1 << std::min(r - 1, 20));
-        ++epoch;
-        timed(2, 0.0, 0.0, [&] {
-          launch_partition(ds, d_ppart, max_items, max_part_tiles, d_cursors, max_items, epoch, stream, hdr);
-        });
-        timed(1, 0.0, 0.0, [&] {
-          launch_hist(ds, d_phist, 0, 0, (int)hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
-                      (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
-        });
-        timed(3, 0.0, 0.0, [&] {
-          launch_reduce(ds, d_pred, items, stream, hdr);
-          launch_scan(ds, d_pscan, items, sp, d_pused, d_out, d_pfold, d_pfflags, stream, hdr);
-        });
-        plan();
-      }
+      launch_rounds(r, want, sp, hb);
       // the state and the node table are two allocations: two copies
       launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream);
       launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
@@ -2213,36 +2306,147 @@ struct Learner {
       if (r > 2 * P.num_leaves + 8) throw std::runtime_error("device planner did not finish the tree");
       want = 4;
     }
-    if (hs->pad) throw std::runtime_error("device planner overflow (nodes, slots or partials)");
-    last_plan_rounds = hs->rounds;
+    rebuild_from_plan(t, search_root);
+    return t;
+  }
+
+  // ===========================================================================
+  // Device-resident tree: the planned batched form with the discretizer's
+  // scales, the root's sums and search, the final walk rules, the row-to-leaf
+  // walk and the leaf renewal all queued on the device; the host waits once, at
+  // the tree's end (more rounds than expected: once more per batch of 4), reads
+  // the planner's tables and rebuilds the tree with the same commit replay.
+  // ===========================================================================
+  Tree train_resident(const float* g, const float* h) {
+    stats = bk_train_stats{};
+    branch.assign((size_t)P.num_leaves, std::vector<int>());
+    if (cols.per_tree) {
+      cols.next_tree(tree_slot, tree_period);
+      set_groups_used(false);
+    }
+    stage_wait();
+    for (int f = 0; f < F; ++f) h_bytree[f] = cols.bytree[(size_t)f] ? 1 : 0;
+    // the noise rotation of this tree, drawn on a copy of the engine: the engine
+    // advances only when the device accepts the discretization (serial.rs:692-703)
+    Mt eng = use_start_eng;
+    int start = 0;
+    if (quant && stochastic)
+      for (int j = 0; j < tree_period; ++j) {
+        const int v = (int)eng.uint_incl((uint32_t)N);
+        if (j == tree_slot) start = v;
+      }
+    *h_start = start;
+    int nparts = 0;
+    if (quant) {
+      const double db = (double)N * (8.0 + (stochastic ? 16.0 : 0.0) + 2.0);
+      timed(4, db, db, [&] {
+        launch_max_abs(g, h, N, d_max4, d_sums3, stream);
+        launch_disc_prep(d_max4, dev_alias(h_start), bins, const_hess ? 1 : 0, d_disc, stream);
+        launch_discretize(g, h, d_gnoise, d_hnoise, N, 0, 0.0, 0.0, const_hess ? 1 : 0, stochastic ? 1 : 0,
+                          (int16_t*)d_gh0, d_sums3, stream, d_disc);
+      });
+    } else {
+      launch_pack_f32(g, h, N, const_hess ? 1 : 0, (float*)d_gh0, stream);
+      nparts = launch_sum_gh(g, h, nullptr, N, N, d_sum_partials, stream);
+    }
+    RootArgs ra{};
+    ra.num_data = N;
+    ra.min_data_in_leaf = P.min_data_in_leaf;
+    ra.quantized = quant ? 1 : 0;
+    ra.const_hess = const_hess ? 1 : 0;
+    ra.nparts = nparts;
+    ra.bins = bins;
+    ra.l1 = P.lambda_l1;
+    ra.l2 = P.lambda_l2;
+    ra.max_delta_step = P.max_delta_step;
+    const PlanParams pp = plan_params();
+    launch_root_prep(d_disc, d_sums3, d_sum_partials, h, ra, pp, d_pstate, d_pnodes, d_pnflags, dev_alias(h_bytree),
+                     d_bytree, d_phist, d_pscan, d_pred, d_pused, d_err, stream);
+    ScanParams sp = scan_params();
+    sp.dyn = (const double*)d_disc;
+    const bool search_root = N >= P.min_data_in_leaf * 2;
+    const int hb = plan_hist_grid();
+    const RoundHdr* hdr = &d_pstate->hdr;
+    // the root's round: its histogram, reduce and search from the root set-up
+    timed(0, (double)N * (G + payload_bytes()), (double)N * (stride + payload_bytes()), [&] {
+      launch_hist(ds, d_phist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
+                  (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
+    });
+    if (search_root) {
+      stats.hist_rows += N;
+      stats.committed_hist_rows += N;
+    }
+    timed(3, 0.0, 0.0, [&] {
+      launch_reduce(ds, d_pred, 1, stream, hdr);
+      launch_scan(ds, d_pscan, 1, sp, d_pused, d_out, d_pfold, d_pfflags, stream, hdr);
+    });
+    launch_plan_round();
+    int r = 1, want = last_plan_rounds > 0 ? last_plan_rounds : 8;
+    const size_t back = sizeof(PlanState) + sizeof(PlanNode) * (size_t)plan_max_nodes;
+    const PlanState* hs = (const PlanState*)h_plan;
+    for (;;) {
+      launch_rounds(r, want, sp, hb);
+      // the tree's walk rules, segments and renewal order, the row-to-leaf walk,
+      // the renewal sums, and every table the host reads, before one wait
+      launch_finalize(d_pstate, d_pnodes, d_feats, d_node_rules, d_node_count, d_segs, d_order, P.num_leaves, stream);
+      timed(6, (double)N * (stride + 4.0), (double)N * (stride + 4.0), [&] {
+        launch_leaf_of_tree(ds, d_node_rules, d_nodes, P.num_leaves - 1, nullptr, d_leaf_of, stream, d_node_count);
+      });
+      if (renew) {
+        const int* flags = nullptr;
+        if (d_renew_part_sums) {
+          timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
+            launch_renew_par(N, d_leaf_of, g, h, P.num_leaves, d_renew_part_sums, d_renew_part_meta, stream);
+            launch_renew_fix(d_segs, P.num_leaves, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew,
+                             d_renew_flags, stream);
+          });
+          flags = d_renew_flags;
+        }
+        timed(5, 0.0, 0.0,
+              [&] { launch_renew_sums(N, d_leaf_of, d_order, P.num_leaves, g, h, flags, d_renew, stream); });
+      }
+      launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream);
+      launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
+      launch_copy(dev_alias(h_disc), d_disc, sizeof(DiscState), stream);
+      launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
+      sync_spin();
+      if (hs->done) break;
+      if (r > 2 * P.num_leaves + 8) throw std::runtime_error("device planner did not finish the tree");
+      want = 4;
+    }
+    int err = 0;
+    std::memcpy(&err, h_tail, sizeof(int));
+    if (err) {
+      BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
+      sync();
+      leaf_of_valid = true;
+      last_num_leaves = 0;
+      if (err & 16) throw std::runtime_error("root sums are not exact in every order (device-resident tree)");
+      throw std::runtime_error("partition count mismatch on the device");
+    }
+    // the discretizer's state as the host path keeps it
+    if (quant && h_disc->ok) {
+      use_start_eng = eng;
+      grad_scale = h_disc->gs;
+      hess_scale = h_disc->hs;
+      last_int_g = h_disc->int_g;
+      last_int_h = h_disc->int_h;
+      last_root_g = h_disc->root_g;
+      last_root_h = h_disc->root_h;
+      sums_bagged = false;
+    }
+    if (!quant) h0 = h_disc->h0;
     const PlanNode* hn = (const PlanNode*)(h_plan + sizeof(PlanState));
-    if (hs->ncommit < 0 || hs->ncommit > P.num_leaves - 1) throw std::runtime_error("device planner: bad commit count");
-    for (int c = 0; c < hs->ncommit; ++c) {
-      const int bl = hs->commit[2 * c], x = hs->commit[2 * c + 1];
-      HostSplit info;
-      info.o = hn[x].best.o;
-      info.inner = hn[x].best.inner;
-      if (bl < 0 || bl >= t.num_leaves || x < 0 || x >= hs->num_nodes || info.inner < 0 || info.inner >= F)
-        throw std::runtime_error("device planner: bad commit");
-      commit_split(t, bl, info);
-    }
-    if (t.num_leaves != hs->num_leaves) throw std::runtime_error("device planner: leaf count disagrees");
-    if (hs->num_nodes < 1 || hs->num_nodes > plan_max_nodes) throw std::runtime_error("device planner: bad node count");
-    for (int l = 0; l < t.num_leaves; ++l) {
-      const int x = hs->leaf_node[l];
-      if (x < 0 || x >= hs->num_nodes) throw std::runtime_error("device planner: bad leaf node");
-      segs[(size_t)l] = hn[x].seg;
-    }
-    int expanded = 0;
-    for (int i = 0; i < hs->num_nodes; ++i) expanded += hn[i].expanded ? 1 : 0;
-    stats.speculative_waste = expanded - (t.num_leaves - 1);
-    stats.hist_rows += hs->hist_rows;
-    stats.part_rows += hs->part_rows;
-    stats.rounds += hs->search_rounds + (search_root ? 1 : 0);
-    if (prof.tree_on) {
-      const double hr = (double)hs->hist_rows, pr = (double)hs->part_rows;
-      if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
-      if ((prof.mask >> 2) & 1u) prof.add_bytes(2, pr * 9.0, pr * 2.0 * stride);
+    Tree t(P.num_leaves);
+    t.set_leaf_output(0, leaf_output_root(P, hn[0].sum_g, hn[0].sum_h));
+    smaller = LeafSplits{0, N, hn[0].sum_g, hn[0].sum_h, 0.0, hn[0].int_sum};
+    larger = LeafSplits{};
+    rebuild_from_plan(t, search_root);
+    if (renew) {
+      const double* sums = (const double*)(h_tail + 16);
+      for (int leaf = 0; leaf < t.num_leaves; ++leaf)
+        t.set_leaf_output(leaf, leaf_output_plain(sums[(size_t)2 * leaf], sums[(size_t)2 * leaf + 1], P.lambda_l1,
+                                                  P.lambda_l2, P.max_delta_step));
     }
     return t;
   }
@@ -2281,6 +2485,19 @@ struct Learner {
     wait_ms = 0.0;
     leaf_of_valid = false;
     prof.tree_on = prof.trees++ % prof.every == 0;
+    if (resident_ok(d_rows != nullptr)) {
+      Tree t = train_resident(g, h);
+      last_bagged = false;
+      last_num_leaves = t.num_leaves;
+      last_left.assign(t.left.begin(), t.left.begin() + std::max(t.num_leaves - 1, 0));
+      last_right.assign(t.right.begin(), t.right.begin() + std::max(t.num_leaves - 1, 0));
+      leaf_of_valid = true;  // walked on the device
+      if (prof.on && prof.used > Prof::kResolveAt) prof.resolve();
+      const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      stats.wait_ms = wait_ms;
+      stats.host_ms = total - wait_ms;
+      return t;
+    }
     Tree t = !batched ? train_sequential(g, h, d_rows, nrows)
                       : (planner_ok() ? train_planned(g, h, d_rows, nrows) : train_batched(g, h, d_rows, nrows));
     last_bagged = d_rows != nullptr;
@@ -2815,6 +3032,7 @@ int bk_learner_set_round_planner(bk_learner* l, int on_device) {
   return bk::guard([&] {
     if (!l) throw std::invalid_argument("null argument");
     l->L.plan_on_device = on_device != 0;
+    l->L.resident_on = on_device == 1;
   });
 }
 

@@ -26,7 +26,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-ROUND_PLANNER_ON_DEVICE = True
+ROUND_PLANNER = 0
 
 
 def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
@@ -41,7 +41,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     obj = bst.BinaryObjective(y)
     init = obj.boost_from_score(0)
     learner = bg.Learner(ds, params, False)
-    learner.set_round_planner(ROUND_PLANNER_ON_DEVICE)
+    learner.set_round_planner(ROUND_PLANNER)
     n = ds.num_data
     dev = torch.device("cuda:0")
     score = torch.full((n,), init, dtype=torch.float64, device=dev)
@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--rows", type=int, default=10_500_000)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--sequential-steps", type=int, default=3)
+    ap.add_argument("--device-tree-steps", type=int, default=10,
+                    help="iterations timed with the trees planned on the device (0 skips it)")
     ap.add_argument("--breakdown-steps", type=int, default=5)
     ap.add_argument("--timed-every", type=int, default=4,
                     help="the measured run times the dominant kernel's launches of every k-th tree (each timed "
@@ -316,11 +318,11 @@ def main():
                     help="documents of the LambdaRank workload measured beside the headline (0 skips it)")
     ap.add_argument("--ranking-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
-    ap.add_argument("--round-planner", choices=("device", "host"), default="device",
-                    help="who plans the rounds of batched growth")
+    ap.add_argument("--round-planner", choices=("tree", "rounds", "host"), default="host",
+                    help="who plans batched growth: the device for whole trees, the device for the rounds, the host")
     args = ap.parse_args()
-    global ROUND_PLANNER_ON_DEVICE
-    ROUND_PLANNER_ON_DEVICE = args.round_planner == "device"
+    global ROUND_PLANNER
+    ROUND_PLANNER = {"tree": 1, "rounds": 2, "host": 0}[args.round_planner]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the learner has no multi-GPU path (rows would shard across GPUs "
               "with one histogram all-reduce per build; not built).", file=sys.stderr)
@@ -415,6 +417,14 @@ def main():
             return bst.auc(hy, bst.predict_raw(trees, hxt).cpu().numpy())
 
         result["holdout_auc"] = holdout_auc(qtrees)
+    if args.device_tree_steps > 0:
+        # the same learner with its trees planned and finished on the device (one
+        # host wait per tree); the host-planned rounds above measure faster
+        saved, ROUND_PLANNER = ROUND_PLANNER, 1
+        del_, _, _, _ = run_gpu(ds, y, qparams, args.device_tree_steps, 2, lr, profile=False)
+        ROUND_PLANNER = saved
+        result["device_planned_tree_iters_per_sec"] = args.device_tree_steps / del_
+        log(f"quantized, trees planned on the device: {args.device_tree_steps / del_:.2f} it/s")
     if args.sequential_steps > 0:
         sparams = dataclasses.replace(qparams, batched=0)
         sel, _, _, _ = run_gpu(ds, y, sparams, args.sequential_steps, 1, lr, profile=False)

@@ -331,12 +331,19 @@ int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iterat
  * larger than one LDS accumulation. No reference counterpart (tuning only). */
 int bk_learner_set_hist_tiling(bk_learner* learner, int32_t min_rows, int32_t max_blocks);
 
-/* Who plans the rounds of batched growth: 1 (the default) the device, whose
- * planner kernel replays the commits and picks each round's work, so a tree's
- * rounds queue without a host wait; 0 the host, waiting for every round's
- * results. Both grow the same tree. The device planner takes up to 256 leaves and
- * no by-node feature sampling or interaction constraints; the host plans
- * otherwise. No reference counterpart (the reference grows one split per step,
+/* Who plans the rounds of batched growth: 0 (the default, measured fastest on
+ * MI355X: the host sees a round's end within microseconds and plans it in about
+ * ten) the host, waiting for every round's results; 1 the device, for the whole
+ * tree: its planner kernel replays the commits and picks each round's work,
+ * and the discretizer's scales, the root's sums, the row-to-leaf walk and the
+ * leaf renewal also run on the device, so the host waits once per tree; 2 the
+ * device plans the rounds, the host the tree's start and end. All grow the same
+ * tree. The device
+ * planner takes up to 256 leaves and no by-node feature sampling or interaction
+ * constraints; a whole-tree device run also needs no bag and, when quantized,
+ * power-of-two gradient bins with num_data * (bins + 1) < 2^29 (the root sums are
+ * then exact in any order). Otherwise the next mode down runs. No reference
+ * counterpart (the reference grows one split per step,
  * crates/lgbm-core/src/treelearner/serial.rs:606-621). */
 int bk_learner_set_round_planner(bk_learner* learner, int on_device);
 

@@ -15,9 +15,12 @@ import synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("batched", [1, 0])
-def test_fullsize_quantized_trees_bit_exact(batched):
-    print(f"\n[case] full size 10.5M x 28 quantized batched={batched}, 2 trees, GPU vs oracle", flush=True)
+@pytest.mark.parametrize("batched,planner", [(1, 0), (0, 0), (1, 1)])
+def test_fullsize_quantized_trees_bit_exact(batched, planner):
+    """planner 1: the batched trees planned and finished on the device (one host
+    wait per tree)."""
+    print(f"\n[case] full size 10.5M x 28 quantized batched={batched} planner={planner}, 2 trees, GPU vs oracle",
+          flush=True)
     n = 10_500_000
     x, y = synth.binary_28(n, seed=0, intercept=synth.solve_intercept(0))
     ds = bg.Dataset(x, bg.DatasetParams(num_threads=16))
@@ -28,6 +31,7 @@ def test_fullsize_quantized_trees_bit_exact(batched):
     obj = bst.BinaryObjective(y)
     score = np.full(n, obj.boost_from_score(0))
     gpu = bg.Learner(ds, p)
+    gpu.set_round_planner(planner)
     ora = ob.OracleLearner(ds, p)
     for it in range(2):
         g, h = obj.gradients(score)

@@ -342,10 +342,11 @@ PLANNER_CASES = {
 @pytest.mark.parametrize("case", sorted(PLANNER_CASES))
 def test_device_round_planner_matches_host_planner(binary_10k, case, bagged):
     """Batched growth planned on the device (k_plan replays the commits and picks
-    every round's work; no host wait per round) against the host-planned batched
-    form, tree by tree for 3 boosting iterations: bit for bit, with the same
-    rounds, leaf segments (leaf rows) and score updates. The quantized trees also
-    equal the oracle's."""
+    every round's work; no host wait per round), for whole trees (scales, root
+    sums, walk and renewal on the device too; the unbagged cases) and for the
+    rounds only, against the host-planned batched form, tree by tree for 3
+    boosting iterations: bit for bit, with the same rounds, leaf segments (leaf
+    rows) and score updates. The quantized trees also equal the oracle's."""
     announce(f"device round planner vs host planner: {case} bagged={bagged}")
     ds, y = binary_10k
     kw = dict(PLANNER_CASES[case])
@@ -353,8 +354,10 @@ def test_device_round_planner_matches_host_planner(binary_10k, case, bagged):
     p = bg.LearnerParams(form=form, batched=1, **kw)
     if form == bg.FORM_QUANTIZED:
         quantized(p, 7)
-    dev, host = bg.Learner(ds, p, False), bg.Learner(ds, p, False)
-    host.set_round_planner(False)
+    dev, host, mid = bg.Learner(ds, p, False), bg.Learner(ds, p, False), bg.Learner(ds, p, False)
+    dev.set_round_planner(1)
+    host.set_round_planner(0)
+    mid.set_round_planner(2)
     o = ob.OracleLearner(ds, p, False) if form == bg.FORM_QUANTIZED else None
     obj = bst.BinaryObjective(y)
     sampler = bst.RowSampler({"bagging_fraction": 0.7, "bagging_freq": 1, "bagging_seed": 3}, y) if bagged else None
@@ -362,21 +365,25 @@ def test_device_round_planner_matches_host_planner(binary_10k, case, bagged):
     for it in range(3):
         g, h = obj.gradients(score)
         rows = sampler.sample(it, g, h) if sampler is not None else None
-        td, th = dev.train(g, h, rows), host.train(g, h, rows)
+        td, th, tm = dev.train(g, h, rows), host.train(g, h, rows), mid.train(g, h, rows)
         assert_trees_bitwise(td, th, f"iter {it}")
-        assert dev.last_stats()["rounds"] == host.last_stats()["rounds"]
-        assert dev.last_stats()["splits"] == host.last_stats()["splits"]
+        assert_trees_bitwise(tm, th, f"iter {it} rounds-only planner")
+        for other in (dev, mid):
+            assert other.last_stats()["rounds"] == host.last_stats()["rounds"]
+            assert other.last_stats()["splits"] == host.last_stats()["splits"]
         if o is not None:
             assert_trees_bitwise(td, o.train(g, h, rows), f"iter {it} vs oracle")
         for leaf in (0, td["num_leaves"] // 2, td["num_leaves"] - 1):
             assert np.array_equal(dev.leaf_rows(leaf), host.leaf_rows(leaf)), (it, leaf)
         vals = td["leaf_value"] * 0.1
-        s2 = score.copy()
+        s2, s3 = score.copy(), score.copy()
         dev.add_prediction_to_score(vals, s2)
+        mid.add_prediction_to_score(vals, s3)
         host.add_prediction_to_score(vals, score)
-        assert np.array_equal(s2, score)
+        assert np.array_equal(s2, score) and np.array_equal(s3, score)
     dev.close()
     host.close()
+    mid.close()
 
 
 @pytest.mark.parametrize("case", ["plain", "unbalance", "scale_pos", "weighted", "sigmoid"])
