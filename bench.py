@@ -27,6 +27,8 @@ def log(*a):
 
 
 ROUND_PLANNER = 0
+HIST_TILING = None  # (min_rows, max_blocks) forced, or None for the autotuner's choice
+TREE_LOG = None     # a list collecting every tree's row counts (--tree-log), for PMC summaries
 
 
 def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
@@ -42,6 +44,8 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     init = obj.boost_from_score(0)
     learner = bg.Learner(ds, params, False)
     learner.set_round_planner(ROUND_PLANNER)
+    if HIST_TILING:
+        learner.set_hist_tiling(*HIST_TILING)
     n = ds.num_data
     dev = torch.device("cuda:0")
     score = torch.full((n,), init, dtype=torch.float64, device=dev)
@@ -60,6 +64,8 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
         # values) and the gradients of the next iteration are queued before the
         # host-side bookkeeping of the tree, which then overlaps the device work
         t = learner.train_device(g.data_ptr(), h.data_ptr())
+        if TREE_LOG is not None:
+            TREE_LOG.append(learner.last_stats())
         learner.add_prediction_to_score_device(bst._round0(t["leaf_value"] * lr), score.data_ptr())
         gradients()
         if record:
@@ -233,16 +239,43 @@ def device_copy_gbs(mib=1024, reps=10):
     return 2.0 * mib * 1024 * 1024 / (ms * 1e-3) / 1e9
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a copy of known size)."""
+# algorithmic bytes per row of each kernel kind (the reference's index-based model):
+# root histogram F + P, child histogram F + P + 4 (row index), partition 9
+def algo_bytes_per_row(kind, groups, payload):
+    return {"root_hist": groups + payload, "child_hist": groups + payload + 4, "partition": 9.0}.get(kind)
+
+
+def traffic_per_row(kernel):
+    """Measured HBM bytes per processed row of `kernel` (fetch + write) from the
+    committed rocprofv3 PMC summary: FETCH_SIZE and WRITE_SIZE summed over the
+    kernel's dispatches of a counted run, scaled by the same counters on a copy of
+    known size, over the rows those dispatches processed (per-tree row counts of
+    the same run). None when the summary does not cover the kernel."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     k = d.get("kernels", {}).get(kernel)
-    return None if k is None else k["bytes_per_launch"]
+    return None if k is None else k.get("bytes_per_row")
+
+
+def roofline_entry(kind, ks, rows, groups, payload):
+    """Roofline of one kernel kind over a timed run: algorithmic bytes (rows x the
+    per-row model) over the summed HIP-event durations of its launches, against
+    the 8 TB/s HBM peak; traffic = measured bytes per row x the same rows, per
+    launch (the same launch mix as the algorithmic bytes)."""
+    launches = max(ks["launches"], 1)
+    ms = ks["ms"]
+    per_row = algo_bytes_per_row(kind, groups, payload)
+    algo = rows * per_row
+    achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    tpr = traffic_per_row(kind)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": (tpr * rows / launches) if tpr else None,
+            "kernel": kind, "launches": ks["launches"], "avg_launch_us": 1000.0 * ms / launches,
+            "rows": rows, "algo_bytes_per_row": per_row, "algo_bytes_per_launch": algo / launches,
+            "traffic_bytes_per_row": tpr}
 
 
 def host_cpus():
@@ -318,11 +351,17 @@ def main():
                     help="documents of the LambdaRank workload measured beside the headline (0 skips it)")
     ap.add_argument("--ranking-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the device copy measured beside the learner")
+    ap.add_argument("--tree-log", default="",
+                    help="write every tree's row counts (histogram, partition) to this JSON file")
+    ap.add_argument("--hist-tiling", default="",
+                    help="force the histogram tiling as min_rows,max_blocks (default: the autotuner's choice)")
     ap.add_argument("--round-planner", choices=("tree", "rounds", "host"), default="host",
                     help="who plans batched growth: the device for whole trees, the device for the rounds, the host")
     args = ap.parse_args()
-    global ROUND_PLANNER
+    global ROUND_PLANNER, HIST_TILING, TREE_LOG
+    TREE_LOG = [] if args.tree_log else None
     ROUND_PLANNER = {"tree": 1, "rounds": 2, "host": 0}[args.round_planner]
+    HIST_TILING = tuple(int(v) for v in args.hist_tiling.split(",")) if args.hist_tiling else None
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the learner has no multi-GPU path (rows would shard across GPUs "
               "with one histogram all-reduce per build; not built).", file=sys.stderr)
@@ -376,15 +415,26 @@ def main():
     # roofline of the dominant kernel: algorithmic bytes per launch over the
     # average launch duration (HIP events on the learner's stream, measured run)
     ks = kstats[dom]
-    launches = max(ks["launches"], 1)
-    achieved = ks["algo_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
-    result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(dom), "kernel": dom,
-                          "launches": ks["launches"], "timed_trees": f"every {args.timed_every}th tree",
-                          "avg_launch_us": 1000.0 * ks["ms"] / launches,
-                          "algo_bytes_per_launch": ks["algo_bytes"] / launches,
-                          "moved_bytes_per_launch": ks["moved_bytes"] / launches,
-                          "moved_GBs": ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0}
+    groups, payload = ds.num_features, 2
+    per_row = algo_bytes_per_row(dom, groups, payload)
+    if per_row:
+        result["roofline"] = roofline_entry(dom, ks, ks["algo_bytes"] / per_row, groups, payload)
+    else:
+        launches = max(ks["launches"], 1)
+        achieved = ks["algo_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
+        result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                              "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / launches}
+    result["roofline"]["timed_trees"] = f"every {args.timed_every}th tree"
+    result["roofline"]["moved_bytes_per_launch"] = ks["moved_bytes"] / max(ks["launches"], 1)
+    result["roofline"]["moved_GBs"] = ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
+    # the histogram kernel's own rooflines (root and child builds) from the
+    # breakdown run, where every launch of every kind is timed
+    result["roofline_hist"] = {
+        "root": roofline_entry("root_hist", kall["root_hist"], kall["root_hist"]["algo_bytes"] / (groups + payload),
+                               groups, payload),
+        "child": roofline_entry("child_hist", kall["child_hist"],
+                                kall["child_hist"]["algo_bytes"] / (groups + payload + 4), groups, payload)}
     bs = args.breakdown_steps
     result["kernels"] = {k: {"ms_per_iter": v["ms"] / bs, "launches_per_iter": v["launches"] / bs,
                              "algo_GBs": v["algo_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0,
@@ -498,6 +548,9 @@ def main():
                                                  "sample": f"{done1} iteration(s) in {el1:.1f}s"},
                                   "cpu_model": model, "host_cpus_visible": visible}
         log(f"cpu baseline: {v:.3f} iter/s on {threads} threads ({model}), {v1:.3f} iter/s on 1 thread")
+    if args.tree_log:
+        with open(args.tree_log, "w") as f:
+            json.dump({"rows": args.rows, "trees": TREE_LOG}, f)
     print(json.dumps(result), flush=True)
 
 
