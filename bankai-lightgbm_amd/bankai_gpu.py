@@ -125,7 +125,7 @@ def rank_tables_on_device(objective, device):
 
 class TrainStatsC(C.Structure):
     _fields_ = [("hist_rows", C.c_int64), ("part_rows", C.c_int64), ("splits", C.c_int32),
-                ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("reserved", C.c_int32),
+                ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("hist_bin_bytes", C.c_int32),
                 ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
                 ("host_ms", C.c_double), ("wait_ms", C.c_double), ("hist_min_rows", C.c_int32),
                 ("hist_max_blocks", C.c_int32)]
@@ -176,6 +176,7 @@ EXPORTS = {
     "bk_learner_root_histogram": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
     "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
+    "bk_learner_set_hist_bin_width": ([_P, C.c_int32], C.c_int),
     "bk_learner_set_round_planner": ([_P, C.c_int], C.c_int),
     "bk_learner_root_histogram_noalloc": ([_P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
@@ -624,6 +625,11 @@ class Learner:
         """Forces the histogram tiling (0, 0: the autotuner's choice)."""
         _check(self._lib.bk_learner_set_hist_tiling(self.handle, int(min_rows), int(max_blocks)))
 
+    def set_hist_bin_width(self, nbytes=0):
+        """Bytes per quantized histogram bin of batched growth: 4 (no row counts;
+        the partition counts the children), 8 (with counts), 0 (4 where it applies)."""
+        _check(self._lib.bk_learner_set_hist_bin_width(self.handle, int(nbytes)))
+
     def reset_constant_hessian(self, value):
         _check(self._lib.bk_learner_reset_constant_hessian(self.handle, int(value)))
 
@@ -633,7 +639,7 @@ class Learner:
         return {"hist_rows": s.hist_rows, "part_rows": s.part_rows, "splits": s.splits, "rounds": s.rounds,
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
-                "speculative_waste": s.speculative_waste,
+                "speculative_waste": s.speculative_waste, "hist_bin_bytes": s.hist_bin_bytes,
                 "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks}}
 
     def profile(self, kinds=True, every_trees=1):

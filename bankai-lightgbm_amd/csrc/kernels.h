@@ -178,6 +178,10 @@ struct ScanParams {
 struct DevState {
   int num_data, npad, num_groups, num_features, total_bins, form;
   int stride, pay_off;
+  // quantized histograms with 32-bit bins (16-bit gradient and hessian sums, no
+  // row count): partials are u32 in part_q, pool slots carry no counts, and the
+  // search leaves exact_left/right to the partition (-1)
+  int hist32, pad_h32;
   uint8_t* rec[3];
   void* gh0;                     // payload of buffer 0: int16 packed (quantized) or float2 (g, h)
   const FeatDev* feats;
@@ -204,8 +208,11 @@ struct HistItem {
   int row0, pad;
 };
 
-// One leaf split: its rows move from seg (buffer seg.buf) to dst_buf, left rows
-// first, in tiles of `chunk` rows (tasks first_task .. first_task + ntasks - 1).
+// One leaf split: its rows move from seg (buffer seg.buf) to dst_buf, in tiles of
+// `chunk` rows (tasks first_task .. first_task + ntasks - 1). Left rows fill the
+// segment from its start; right rows follow them when left_total (the left
+// child's exact count) is known, and fill it from its end when left_total < 0,
+// the cursors then giving the children's counts.
 struct PartItem {
   Segment seg;
   int dst_buf, split_group, left_total, pad0;
@@ -271,6 +278,22 @@ struct PlanState {
   int commit[2 * kPlanMaxLeaves];     // committed split c: (leaf, node), in the reference's order
 };
 
+// A split whose children's counts come from the partition (left_total < 0): the
+// children as search arguments (sums, outputs, integer sums; num_data and slot are
+// filled in from the counts), the children's depth, and the slots the pair
+// takes when searched: new_slot for the smaller child, parent_slot (the split
+// leaf's) for the larger.
+struct ChildSpec {
+  LeafArgs kid[2];
+  int depth, new_slot, parent_slot, pad;
+};
+// Rules and tiling of the items derived from the partition.
+struct ChildParams {
+  int max_depth, min_data_in_leaf, hist_min_rows, hist_max_blocks;
+  int rows_per_flush, max_partials, F, pad;
+};
+constexpr int kChildThreads = 1024;  // splits one derivation takes (one thread each)
+
 struct PlanParams {
   int num_leaves, F, max_depth, min_data_in_leaf;
   int hist_min_rows, hist_max_blocks, rows_per_flush, part_tile;
@@ -319,7 +342,7 @@ void set_event_source(EventPairSource fn, void* ctx);
 // Launchers (kernels.hip). All run on `stream` (a hipStream_t passed as void*).
 // Records of buffer 0 from column-major bins (group g at cols + g * col_stride).
 void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride, void* stream);
-size_t hist_lds_bytes(int form, int max_tile_bins);
+size_t hist_lds_bytes(int form, int hist32, int max_tile_bins);
 // Block b covers concatenated rows [b * rows_per_block, (b + 1) * rows_per_block)
 // and writes one partial histogram per item it touches, from slot d_block_slot[b] on.
 // With hdr (a planned round): nitems, total_rows, nblocks and rows_per_block are
@@ -339,6 +362,19 @@ int partition_tile_rows(int stride);
 // With hdr: nitems and ntasks are read from it; the arguments give the grid.
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
                       int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr);
+// After a partition whose items have left_total < 0: per split j, the children's
+// exact counts from bank (epoch & 1) of the cursors (to counts[2j], counts[2j + 1],
+// host-mapped memory), and the round's items as the host plans them for known
+// counts: a search item for every split whose children are searchable
+// (before_find_best_split's max_depth and min_data_in_leaf rules), a histogram
+// item for its smaller child and its reduce item, the histogram tiling, and hdr
+// (nhist, hist_total, hist_nblocks, hist_rpb, nscan) for the launches that follow.
+// counts[2 * nitems] is a status word: 0, or bit 1 (counts that do not add up to
+// the segment) / bit 5 (a tiling beyond max_partials), and then no work is planned.
+// nitems <= kChildThreads.
+void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cursors, int max_items, unsigned epoch,
+                     const ChildSpec* d_specs, const ChildParams& cp, HistItem* hitems, ScanItem* sitems,
+                     ReduceItem* ritems, RoundHdr* hdr, int* counts, void* stream);
 // Sums the partial histograms of every item into its pool slot; the search then
 // reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
 // from its slot.

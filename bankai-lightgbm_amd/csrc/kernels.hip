@@ -82,6 +82,19 @@ __device__ __forceinline__ void qunpack(unsigned long long s, long long* g, long
   *g = ((long long)s) >> 43;
 }
 
+// 32-bit form: bits 16..31 gradient sum (signed), 0..15 hessian sum, no count.
+// A sum of n rows is exact while n * max h < 2^16 and n * max |g| < 2^15: the
+// hessian field never carries and the two's-complement word is G * 2^16 + H.
+__device__ __forceinline__ unsigned qpack32(int16_t p) {
+  const int g = (int)(int8_t)(p >> 8);
+  const unsigned h = (unsigned)(uint8_t)(p & 0xff);
+  return ((unsigned)g << 16) + h;
+}
+__device__ __forceinline__ void qunpack32(unsigned w, long long* g, long long* h) {
+  *h = (long long)(w & 0xffffu);
+  *g = (long long)(((int)w) >> 16);
+}
+
 __device__ __forceinline__ int sat_i32_dev(double x) {
   if (x != x) return 0;
   if (x >= 2147483647.0) return 2147483647;
@@ -174,6 +187,25 @@ struct LdsHist<FORM_QUANT> {
   __device__ __forceinline__ void add(int b, unsigned long long v) { atomicAdd(&q[b], v); }
   __device__ void flush(const DevState& s, int blk, int bin_lo, int nb, bool first) {
     unsigned long long* dst = s.part_q + (size_t)blk * s.total_bins + bin_lo;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[i] = first ? q[i] : dst[i] + q[i];
+  }
+};
+
+// Kind of k_hist / k_reduce for the quantized payload with 32-bit bins.
+constexpr int kHistQ32 = 3;
+
+template <>
+struct LdsHist<kHistQ32> {
+  unsigned* q;
+  static constexpr int kBytesPerBin = 4;
+  __device__ void bind(unsigned char* smem, int nb) { q = (unsigned*)smem; }
+  __device__ void zero(int nb) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) q[i] = 0u;
+  }
+  __device__ __forceinline__ void add(int b, unsigned v) { atomicAdd(&q[b], v); }
+  // partials are u32 words in part_q's memory, total_bins per slot
+  __device__ void flush(const DevState& s, int blk, int bin_lo, int nb, bool first) {
+    unsigned* dst = (unsigned*)s.part_q + (size_t)blk * s.total_bins + bin_lo;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) dst[i] = first ? q[i] : dst[i] + q[i];
   }
 };
@@ -274,6 +306,10 @@ template <>
 struct Pay<FORM_QUANT> {
   using T = int16_t;
 };
+template <>
+struct Pay<kHistQ32> {
+  using T = int16_t;
+};
 
 // Buffer pointers by buffer number. Selecting with branches keeps the kernel
 // argument in constant memory; indexing its arrays with a runtime value would
@@ -323,6 +359,7 @@ __device__ __forceinline__ void patch_payload(uint4& v, int off, float2 p) {
 }
 
 __device__ __forceinline__ void hist_add(LdsHist<FORM_QUANT>& L, int b, int16_t p) { L.add(b, qpack(p)); }
+__device__ __forceinline__ void hist_add(LdsHist<kHistQ32>& L, int b, int16_t p) { L.add(b, qpack32(p)); }
 template <int FORM>
 __device__ __forceinline__ void hist_add(LdsHist<FORM>& L, int b, float2 p) {
   L.add(b, p.x, p.y);
@@ -506,9 +543,12 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // Without an order to keep, no tile waits on another (no look-back).
 //
 // Cursors: two banks of (left, right) u32 per item; a launch uses bank
-// (epoch & 1) and zeroes the other for the next launch. A tile whose claimed run
-// would leave its side of the segment (the routed counts disagree with the
-// search's exact counts) writes nothing and sets err bit 2.
+// (epoch & 1) and zeroes the other for the next launch. With the left child's
+// count known (left_total >= 0) right rows start after the left ones; a tile
+// whose claimed run would leave its side of the segment (the routed counts
+// disagree with the search's exact counts) writes nothing and sets err bit 2.
+// Without it (left_total < 0) right rows fill the segment from its end and the
+// final cursors are the children's exact counts.
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
 constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
@@ -619,12 +659,17 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   }
   __syncthreads();
   const int lbase = sh_lbase, rbase = sh_rbase;
-  if (lbase + tl > left_total || rbase + tr > seg.count - left_total) {
+  // left_total < 0: the right rows fill the segment from its end (each tile's run
+  // in row order), and the cursors end at the children's counts
+  const bool from_end = left_total < 0;
+  if (from_end ? lbase + tl + rbase + tr > seg.count
+               : (lbase + tl > left_total || rbase + tr > seg.count - left_total)) {
     if (threadIdx.x == 0) atomicOr(s.err_flag, 2);  // inconsistent counts: write nothing
     return;
   }
   const long long lpos = (long long)seg.begin + lbase;
-  const long long rpos = (long long)seg.begin + left_total + rbase;
+  const long long rpos = from_end ? (long long)seg.begin + seg.count - rbase - tr
+                                  : (long long)seg.begin + left_total + rbase;
   // every lane writes its chunks straight to their destination rows; the rows of
   // a tile land in two contiguous runs, so the lanes of a wave cover contiguous
   // 16-byte pieces of them
@@ -1322,6 +1367,7 @@ __device__ void load_slice_raw(const DevState& s, const FeatDev& m, int slot, co
   const double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
   const long long* pooli = (const long long*)pool;
   const int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
+  const bool counts = !s.hist32;  // 32-bit quantized bins: the slots keep no counts
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
@@ -1331,7 +1377,7 @@ __device__ void load_slice_raw(const DevState& s, const FeatDev& m, int slot, co
       S.g[i] = pool[2 * bin];
       S.h[i] = p.const_hess ? pool[2 * bin + 1] * p.h0 : pool[2 * bin + 1];
     }
-    S.c[i] = cnt[bin];
+    S.c[i] = counts ? cnt[bin] : 0;
   }
 }
 
@@ -1389,6 +1435,7 @@ __device__ void store_slice(const DevState& s, const FeatDev& m, int slot, const
   double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
   long long* pooli = (long long*)pool;
   int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
+  const bool counts = !s.hist32;
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
@@ -1398,7 +1445,7 @@ __device__ void store_slice(const DevState& s, const FeatDev& m, int slot, const
       pool[2 * bin] = S.g[i];
       pool[2 * bin + 1] = S.h[i];
     }
-    cnt[bin] = S.c[i];
+    if (counts) cnt[bin] = S.c[i];
   }
 }
 
@@ -1408,6 +1455,7 @@ __device__ void subtract_slice(const DevState& s, const FeatDev& m, int slot, co
   double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
   long long* pooli = (long long*)pool;
   int* cnt = s.pool_cnt + (size_t)slot * s.total_bins;
+  const bool counts = !s.hist32;
   for (int i = threadIdx.x; i < m.hist_len; i += blockDim.x) {
     const int bin = m.hist_off + i;
     if (FORM == FORM_QUANT) {
@@ -1417,7 +1465,7 @@ __device__ void subtract_slice(const DevState& s, const FeatDev& m, int slot, co
       L.g[i] = pool[2 * bin] - small.g[i];
       L.h[i] = pool[2 * bin + 1] - small.h[i];
     }
-    L.c[i] = cnt[bin] - small.c[i];
+    L.c[i] = counts ? cnt[bin] - small.c[i] : 0;
   }
 }
 
@@ -1665,7 +1713,10 @@ __device__ __forceinline__ void scan_item(const DevState& s, const ScanItem* __r
   }
   if (!enabled) return;
   o.feature = m.real;
-  if (o.gain > kMinScore) exact_counts(m, S[wl], a.num_data, o);
+  if (o.gain > kMinScore) {
+    if (s.hist32) o.exact_left = o.exact_right = -1;  // the partition counts them
+    else exact_counts(m, S[wl], a.num_data, o);
+  }
   if (lane == 0) out[(wl == 0 ? 0 : s.num_features) + f] = o;
 }
 
@@ -2093,6 +2144,103 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     st->hist_rows += htotal;
     st->part_rows += part_rows;
     st->hdr = RoundHdr{ni, ntasks, nhist, htotal, nblocks, rpb, nscan, 0};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A round's items from the partition's counts (quantized 32-bit bins, whose
+// histograms count no rows): one thread per split. The children's counts are
+// the final cursors of the partition that just ran; the items follow the host's
+// plan for known counts: the search item of a searchable pair (smaller child
+// first, its histogram built, the larger one by subtraction in the parent's
+// slot), the smaller child's histogram item (its rows are contiguous after the
+// partition), its reduce item, and the histogram tiling.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __restrict__ pitems, int ni,
+                                                            const unsigned* __restrict__ cur,
+                                                            const ChildSpec* __restrict__ specs, ChildParams cp,
+                                                            HistItem* __restrict__ hitems,
+                                                            ScanItem* __restrict__ sitems,
+                                                            ReduceItem* __restrict__ ritems,
+                                                            RoundHdr* __restrict__ hdr, int* __restrict__ counts) {
+  __shared__ int scratch[32];
+  __shared__ int status;
+  const int j = threadIdx.x;
+  if (j == 0) status = 0;
+  __syncthreads();
+  int lc = 0, rc = 0, srch = 0, hrows = 0;
+  Segment seg{0, 0, 0};
+  int dst = 0;
+  if (j < ni) {
+    seg = pitems[j].seg;
+    dst = pitems[j].dst_buf;
+    lc = (int)cur[2 * j];
+    rc = (int)cur[2 * j + 1];
+    counts[2 * j] = lc;
+    counts[2 * j + 1] = rc;
+    if (lc < 0 || rc < 0 || (long long)lc + rc != seg.count) {
+      atomicOr(&status, 2);
+    } else {
+      const int depth = specs[j].depth;
+      srch = !(cp.max_depth > 0 && depth >= cp.max_depth) &&
+             !(rc < cp.min_data_in_leaf * 2 && lc < cp.min_data_in_leaf * 2);
+      hrows = srch ? min(lc, rc) : 0;
+    }
+  }
+  int sidx, row0, nscan, htotal, hidx, unused0, nhist, unused1;
+  block_scan2<kChildThreads>(srch, hrows, &sidx, &row0, &nscan, &htotal, scratch);
+  block_scan2<kChildThreads>(hrows > 0 ? 1 : 0, 0, &hidx, &unused0, &nhist, &unused1, scratch);
+  int nblocks = 0, rpb = 0;
+  if (htotal > 0) {  // as the host cuts a build of htotal rows
+    long long nb = min((long long)cp.hist_max_blocks, max(1ll, (long long)htotal / cp.hist_min_rows));
+    nb = max(nb, ((long long)htotal + cp.rows_per_flush - 1) / cp.rows_per_flush);
+    rpb = (int)(((long long)htotal + nb - 1) / nb);
+    nblocks = (htotal + rpb - 1) / rpb;
+  }
+  if (nblocks + nhist > cp.max_partials) {  // the host sizes the partials so that this cannot happen
+    if (j == 0) status |= 32;
+    nscan = nhist = htotal = nblocks = 0;
+    srch = 0;
+  }
+  __syncthreads();
+  if (status != 0) {  // no work: the host reads the status and stops
+    if (j == 0) {
+      counts[2 * ni] = status;
+      *hdr = RoundHdr{};
+    }
+    return;  // block-uniform
+  }
+  if (j < ni && srch) {
+    const ChildSpec& cs = specs[j];
+    const bool left_small = lc < rc;
+    const int sm = left_small ? 0 : 1;
+    LeafArgs kid[2] = {cs.kid[0], cs.kid[1]};
+    kid[0].num_data = lc;
+    kid[1].num_data = rc;
+    kid[sm].slot = cs.new_slot;
+    kid[1 - sm].slot = cs.parent_slot;
+    kid[0].part_first = kid[1].part_first = 0;
+    kid[0].nblk = kid[1].nblk = 0;
+    const Segment kseg[2] = {Segment{dst, seg.begin, lc}, Segment{dst, seg.begin + lc, rc}};
+    ScanItem si{};
+    si.small = kid[sm];
+    si.large = kid[1 - sm];
+    if (hrows > 0) {
+      const int b0 = row0 / rpb, b1 = (row0 + hrows - 1) / rpb;
+      si.small.part_first = b0 + hidx;  // partial of (block b, item i) is slot b + i
+      si.small.nblk = b1 - b0 + 1;
+      hitems[hidx] = HistItem{kseg[sm], row0, 0};
+    }
+    si.has_large = 1;
+    si.use_subtract = 1;
+    si.used_off = j * cp.F;
+    si.out_off = sidx * 2 * cp.F;
+    sitems[sidx] = si;
+    ritems[sidx] = ReduceItem{si.small.slot, si.small.part_first, si.small.nblk, 0};
+  }
+  if (j == 0) {
+    counts[2 * ni] = 0;
+    *hdr = RoundHdr{ni, 0, nhist, htotal, nblocks, rpb, nscan, 0};
   }
 }
 
@@ -2811,6 +2959,14 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
           h += hh;
           c += cc;
         }
+      } else if (FORM == kHistQ32) {
+        const unsigned* pq = (const unsigned*)s.part_q;
+        for (int j = sp; j < it.nblk; j += 8) {
+          long long gg, hh;
+          qunpack32(pq[base + (size_t)j * stride], &gg, &hh);
+          g += gg;
+          h += hh;
+        }
       } else {
         for (int j = sp; j < it.nblk; j += 8) {
           fg += s.part_g[base + (size_t)j * stride];
@@ -2829,7 +2985,7 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
       double* pool = s.pool + ((size_t)it.slot * s.total_bins) * 2;
       int* cnt = s.pool_cnt + (size_t)it.slot * s.total_bins;
       long long tc = 0;
-      if (FORM == FORM_QUANT) {
+      if (FORM == FORM_QUANT || FORM == kHistQ32) {
         long long tg = 0, th = 0;
         for (int k = 0; k < 8; ++k) {
           tg += ra[k][lb];
@@ -2848,7 +3004,7 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
         pool[2 * bin] = tg;
         pool[2 * bin + 1] = th;
       }
-      cnt[bin] = (int)tc;
+      if (FORM != kHistQ32) cnt[bin] = (int)tc;  // the 32-bit form keeps no counts
     }
     __syncthreads();  // the LDS planes are reused by the next item
   }
@@ -2876,8 +3032,8 @@ void allow_lds(K kernel) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-size_t hist_lds_bytes(int form, int max_tile_bins) {
-  const size_t per_bin = form == FORM_QUANT ? 8 : 20;
+size_t hist_lds_bytes(int form, int hist32, int max_tile_bins) {
+  const size_t per_bin = form == FORM_QUANT ? (hist32 ? 4 : 8) : 20;
   return ((size_t)max_tile_bins * per_bin + 15) & ~(size_t)15;
 }
 
@@ -2889,10 +3045,17 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
   dim3 grid(groups8 * kXcds * (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
   const size_t pay = s.form == FORM_QUANT ? 2 : 8;
-  const size_t lds = hist_lds_bytes(s.form, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
+  const size_t lds =
+      hist_lds_bytes(s.form, s.hist32, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("histogram tile exceeds the LDS budget");
   switch (s.form) {
     case FORM_QUANT:
+      if (s.hist32) {
+        allow_lds(k_hist<kHistQ32>);
+        launch(k_hist<kHistQ32>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows, rows_per_block,
+               d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks, ntiles, hdr);
+        break;
+      }
       allow_lds(k_hist<FORM_QUANT>);
       launch(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
@@ -3127,6 +3290,17 @@ void launch_plan(PlanState* st, PlanNode* nodes, uint8_t* node_flags, const Fold
   BK_HIP(hipGetLastError());
 }
 
+void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cursors, int max_items, unsigned epoch,
+                     const ChildSpec* d_specs, const ChildParams& cp, HistItem* hitems, ScanItem* sitems,
+                     ReduceItem* ritems, RoundHdr* hdr, int* counts, void* stream) {
+  if (nitems <= 0) return;
+  if (nitems > kChildThreads || nitems > max_items) throw std::runtime_error("too many splits in one round");
+  const unsigned* cur = d_cursors + (size_t)(epoch & 1u) * max_items * 2;
+  launch(k_children, dim3(1), dim3(kChildThreads), 0, (hipStream_t)stream, d_pitems, nitems, cur, d_specs, cp, hitems,
+         sitems, ritems, hdr, counts);
+  BK_HIP(hipGetLastError());
+}
+
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
@@ -3169,7 +3343,8 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
-      launch(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items, hdr);
+      if (s.hist32) launch(k_reduce<kHistQ32>, grid, dim3(256), 0, st, s, d_items, hdr);
+      else launch(k_reduce<FORM_QUANT>, grid, dim3(256), 0, st, s, d_items, hdr);
       break;
     case FORM_F64:
       launch(k_reduce<FORM_F64>, grid, dim3(256), 0, st, s, d_items, hdr);

@@ -697,6 +697,18 @@ struct Learner {
   std::vector<int> bytree_int;
   std::vector<uint8_t> ones_f;
   int plan_max_nodes = 0, hist_grid_blocks = 0, last_plan_rounds = 0;
+  // quantized histograms with 32-bit bins (host-planned batched rounds): the
+  // histograms carry no row counts, the partition counts the children and
+  // k_children derives the round's histogram, reduce and search items
+  bool hist32_ok = false;              // this learner may use them
+  int hist_width = 0;                  // bk_learner_set_hist_bin_width: 0 picks, 4 or 8 forces
+  int rows_per_flush64 = 0, rows_per_flush32 = 0;
+  std::vector<ChildSpec> v_cspec;
+  HistItem* d_chist = nullptr;
+  ScanItem* d_cscan = nullptr;
+  ReduceItem* d_cred = nullptr;
+  RoundHdr* d_chdr = nullptr;
+  int* h_kcount = nullptr;             // pinned: the children's counts, then a status word
   // device-resident trees: one host wait per tree
   bool resident_on = false;            // bk_learner_set_round_planner(…, 1) turns it on
   DiscState* d_disc = nullptr;
@@ -800,11 +812,12 @@ struct Learner {
                     (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay,
                     (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
                     (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree,
-                    (void*)d_disc, (void*)d_node_count})
+                    (void*)d_disc, (void*)d_node_count, (void*)d_chist, (void*)d_cscan, (void*)d_cred,
+                    (void*)d_chdr})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc, (void*)h_stamp,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
-                    (void*)h_out, (void*)h_plan, (void*)h_start, (void*)h_bytree, (void*)h_disc})
+                    (void*)h_out, (void*)h_plan, (void*)h_start, (void*)h_bytree, (void*)h_disc, (void*)h_kcount})
       fh(p);
     if (staged_ev) (void)hipEventDestroy(staged_ev);
     staged_ev = nullptr;
@@ -930,9 +943,14 @@ struct Learner {
     if (form == FORM_QUANT) {
       int maxq = bins + 1;
       rows_per_flush = std::max(kTileRows, ((1 << 20) - 1) / maxq / kTileRows * kTileRows);
+      // 32-bit bins: n rows stay exact while n * max h < 2^16 and n * max |g| < 2^15;
+      // a discretized row has h <= bins + 1 and |g| <= bins / 2 + 1 (the rounding
+      // of the scaled maximum may add one), so n * (bins + 2) < 2^16 bounds both
+      rows_per_flush32 = ((1 << 16) - 1) / (bins + 2) / kTileRows * kTileRows;
     } else {
       rows_per_flush = 1 << 30;
     }
+    rows_per_flush64 = rows_per_flush;
     cache = (size_t)P.num_leaves;
     if (P.histogram_pool_size > 0.0) {
       size_t bytes = 0;
@@ -944,6 +962,8 @@ struct Learner {
     batched = P.batched != 0 && cache == (size_t)P.num_leaves;
     num_slots = batched ? (size_t)P.num_leaves * 2 + 2 : cache;
     max_items = P.num_leaves + 1;
+    hist32_ok = form == FORM_QUANT && batched && rows_per_flush32 >= 2 * kTileRows && max_items <= kChildThreads;
+    if (hist32_ok) rows_per_flush = std::min(rows_per_flush, rows_per_flush32);  // sizes the partials for both
     part_tile = partition_tile_rows(stride);
     max_part_tiles = N / part_tile + max_items + 8;
     // a build has at most one row block per 2048 rows (the tuner's smallest row
@@ -1009,7 +1029,7 @@ struct Learner {
       const size_t round_bytes = up(sizeof(PartItem) * (size_t)max_items) + up(sizeof(HistItem) * (size_t)max_items) +
                                  up(sizeof(int) * (size_t)kHistMaxBlocks) + up(sizeof(ScanItem) * (size_t)max_items) +
                                  up(sizeof(ReduceItem) * 2 * (size_t)max_items) +
-                                 up(sizeof(int) * (size_t)F * max_items);
+                                 up(sizeof(int) * (size_t)F * max_items) + up(sizeof(ChildSpec) * (size_t)max_items);
       const size_t tree_bytes = up(sizeof(Segment) * (size_t)P.num_leaves) + up(sizeof(int) * (size_t)P.num_leaves) +
                                 up(sizeof(NodeRule) * (size_t)P.num_leaves);
       const size_t plan_bytes = up(sizeof(PlanState)) + up(sizeof(PlanNode)) + 2 * up(sizeof(int) * (size_t)F);
@@ -1025,6 +1045,14 @@ struct Learner {
       h_flags = h_back;
       d_fold = (FoldOut*)(d_back + off_fold);
       h_fold = (FoldOut*)(h_back + off_fold);
+    }
+    if (hist32_ok) {
+      v_cspec.resize((size_t)max_items);
+      d_chist = dmalloc<HistItem>((size_t)max_items);
+      d_cscan = dmalloc<ScanItem>((size_t)max_items);
+      d_cred = dmalloc<ReduceItem>((size_t)max_items);
+      d_chdr = dmalloc<RoundHdr>(1);
+      pinned(&h_kcount, (size_t)2 * max_items + 1);
     }
     d_max4 = dmalloc<unsigned>(4);
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
@@ -1164,7 +1192,16 @@ struct Learner {
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
     sync();
+    use_hist32(hist32_wanted());  // the tuner times the bins the trees will use
     autotune_hist();
+  }
+
+  // 32-bit bins for the next tree: quantized, batched with host-planned rounds,
+  // and not turned off by bk_learner_set_hist_bin_width
+  bool hist32_wanted() const { return hist32_ok && hist_width != 8 && !planner_ok(); }
+  void use_hist32(bool on) {
+    ds.hist32 = on ? 1 : 0;
+    rows_per_flush = on ? rows_per_flush32 : rows_per_flush64;
   }
 
   // Autotuner of the histogram tiling (rows per workgroup at least hist_min_rows,
@@ -1173,11 +1210,11 @@ struct Learner {
   // a fifth of them (a child-sized build), and keeps the candidate with the least
   // time for one root and three child builds. The payload is whatever buffer 0
   // holds: the kernel's work does not depend on its values. The choice is kept
-  // per (rows, groups, bins, form) for the process.
+  // per (rows, groups, bins, form and bin width) for the process.
   void autotune_hist() {
     static std::mutex mu;
     static std::map<std::array<int, 4>, std::pair<int, int>> picked;
-    const std::array<int, 4> key{N, G, TB, form};
+    const std::array<int, 4> key{N, G, TB, form * 2 + ds.hist32};
     std::lock_guard<std::mutex> lock(mu);
     auto it = picked.find(key);
     if (it != picked.end()) {
@@ -1335,6 +1372,7 @@ struct Learner {
       set_groups_used(false);
     }
     stats = bk_train_stats{};
+    stats.hist_bin_bytes = form == FORM_QUANT ? (ds.hist32 ? 4 : 8) : 0;
     double sg = 0.0, sh = 0.0;
     long long isum = 0;
     if (quant) {
@@ -1915,6 +1953,98 @@ struct Learner {
   // Batched form: every round expands the top frontier leaves in one launch of
   // each kernel; commits replay the reference's best-first order exactly.
   // ===========================================================================
+  // The most row blocks plan_hist cuts for any build of at most `rows` rows (0:
+  // none): the block target nb grows with the rows, and a build of r rows takes
+  // ceil(r / ceil(r / nb(r))) <= nb(r) blocks.
+  int hist_blocks_for(long long rows) const {
+    if (rows <= 0) return 0;
+    long long nb = std::min<long long>(hist_max_blocks, std::max<long long>(1, rows / hist_min_rows));
+    nb = std::max<long long>(nb, (rows + rows_per_flush - 1) / rows_per_flush);
+    return (int)nb;
+  }
+
+  // One round of batched growth with 32-bit bins: the partition counts the
+  // children (right rows fill each segment from its end), k_children turns the
+  // counts into the round's histogram, reduce and search items, and the host
+  // reads the counts back with the folded splits. The round's histogram grid is
+  // sized for hist_bound rows (every smaller child holds at most half of its
+  // parent's rows); the device plans at most that.
+  template <typename Keep>
+  void run_derived_round(int ni, int nt, double part_rows, long long hist_bound, const std::vector<int>& order,
+                         bool host_fold, Keep&& keep) {
+    begin_uploads();
+    d_pitems = put(h_pitems, (size_t)ni);
+    const ChildSpec* u_specs = put(v_cspec.data(), (size_t)ni);
+    d_used = put(h_used, (size_t)F * ni);
+    send(true);
+    run_partition(ni, nt, part_rows);
+    ChildParams cp{};
+    cp.max_depth = P.max_depth;
+    cp.min_data_in_leaf = P.min_data_in_leaf;
+    cp.hist_min_rows = hist_min_rows;
+    cp.hist_max_blocks = hist_max_blocks;
+    cp.rows_per_flush = rows_per_flush;
+    cp.max_partials = max_partials;
+    cp.F = F;
+    const int hb = hist_blocks_for(hist_bound);
+    if (hb > max_partials) throw std::runtime_error("partial buffer too small");
+    const ScanParams sp = scan_params();
+    timed(7, 0.0, 0.0, [&] {
+      launch_children(d_pitems, ni, d_cursors, max_items, epoch, u_specs, cp, d_chist, d_cscan, d_cred, d_chdr,
+                      dev_alias(h_kcount), stream);
+    });
+    if (hb > 0)
+      timed(1, 0.0, 0.0, [&] {
+        launch_hist(ds, d_chist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
+                    (int)tile_first.size() - 1, rows_per_flush, stream, d_chdr);
+      });
+    timed(3, 0.0, 0.0, [&] {
+      launch_reduce(ds, d_cred, ni, stream, d_chdr);
+      launch_scan(ds, d_cscan, ni, sp, d_used, d_out, d_fold, d_flags, stream, d_chdr);
+    });
+    if (host_fold) launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
+    sync_spin();
+    stats.rounds += 1;
+    if (h_kcount[2 * ni] != 0) throw std::runtime_error("partition count mismatch on the device");
+    long long hist_rows = 0;
+    int k = 0;
+    for (int j = 0; j < ni; ++j) {
+      Node& X = nodes[(size_t)leaf_node[(size_t)order[(size_t)j]]];
+      const int lc = h_kcount[2 * j], rc = h_kcount[2 * j + 1];
+      if (lc < 0 || rc < 0 || (long long)lc + rc != X.seg.count)
+        throw std::runtime_error("partition count mismatch on the device");
+      const int dst = X.seg.buf == 1 ? 2 : 1;
+      const int xl = X.child[0], xr = X.child[1];
+      Node& cl = nodes[(size_t)xl];
+      Node& cr = nodes[(size_t)xr];
+      cl.seg = Segment{dst, X.seg.begin, lc};
+      cr.seg = Segment{dst, X.seg.begin + lc, rc};
+      cl.ls.num_data = lc;
+      cr.ls.num_data = rc;
+      const bool search = children_searchable(X.depth + 1, lc, rc);
+      X.kids_searched = search;
+      if (!search) continue;
+      const bool left_small = lc < rc;
+      const int xs = left_small ? xl : xr, xg = left_small ? xr : xl;
+      nodes[(size_t)xs].slot = v_cspec[(size_t)j].new_slot;
+      nodes[(size_t)xg].slot = X.slot;
+      hist_rows += std::min(lc, rc);
+      nodes[(size_t)xs].best = fold(2 * k, flags_of(xs));
+      nodes[(size_t)xg].best = fold(2 * k + 1, flags_of(xg));
+      if (host_fold) {
+        keep(xs, k, 0, j);
+        keep(xg, k, 1, j);
+        nodes[(size_t)xs].best = nodes[(size_t)xg].best = HostSplit();  // folded when X is committed
+      }
+      ++k;
+    }
+    stats.hist_rows += hist_rows;
+    if (prof.tree_on) {
+      const double hr = (double)hist_rows;
+      if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
+    }
+  }
+
   Tree train_batched(const float* g, const float* h, const int32_t* d_rows, int nrows) {
     LeafSplits root = before_train(g, h, d_rows, nrows);
     Tree t(P.num_leaves);
@@ -1934,11 +2064,14 @@ struct Learner {
       node_cands.resize(max_nodes * F);
       node_searched.resize(max_nodes * F);
     }
-    // a searched node's candidates, kept for its fold at its parent's commit
-    auto keep = [&](int node, int k2, int side) {
+    // a searched node's candidates (search item k2, feature row urow of the
+    // round's used table), kept for its fold at its parent's commit
+    auto keep = [&](int node, int k2, int side, int urow) {
       std::memcpy(&node_cands[(size_t)node * F], h_out + ((size_t)2 * k2 + side) * F, sizeof(SplitOut) * (size_t)F);
-      for (int f = 0; f < F; ++f) node_searched[(size_t)node * F + f] = (uint8_t)(h_used[k2 * F + f] & kUseSearch);
+      for (int f = 0; f < F; ++f) node_searched[(size_t)node * F + f] = (uint8_t)(h_used[urow * F + f] & kUseSearch);
     };
+    // 32-bit bins: the children's counts come from the partition (k_children)
+    const bool derive = ds.hist32 != 0;
     leaf_node.assign((size_t)P.num_leaves, -1);
     leaf_node[0] = 0;
     int next_slot = 1;
@@ -1961,7 +2094,7 @@ struct Learner {
       run_scan(1, nr);
       nodes[0].best = fold(0, flags_of(0));
       if (host_fold) {
-        keep(0, 0, 0);
+        keep(0, 0, 0, 0);
         nodes[0].best = fold_host(&node_cands[0], &node_searched[0], cols.by_node(branch[0]));
       }
     } else {
@@ -1984,9 +2117,12 @@ struct Learner {
         }
         const int x = leaf_node[bl];
         if (!nodes[(size_t)x].expanded) break;
-        const HostSplit info = best[bl];
-        const int right = commit_split(t, (int)bl, info);
         const int cl = nodes[(size_t)x].child[0], cr = nodes[(size_t)x].child[1];
+        HostSplit info = best[bl];
+        // the children's exact counts (the search's, or the partition's with 32-bit bins)
+        info.o.exact_left = nodes[(size_t)cl].ls.num_data;
+        info.o.exact_right = nodes[(size_t)cr].ls.num_data;
+        const int right = commit_split(t, (int)bl, info);
         nodes[(size_t)cl].ls.leaf = (int)bl;
         nodes[(size_t)cr].ls.leaf = right;
         // the reference searches these children, and draws their by-node samples
@@ -2018,13 +2154,55 @@ struct Learner {
       if (order.size() > budget) order.resize(budget);
       int ni = 0, nt = 0, ns = 0;
       double part_rows = 0.0, hist_rows = 0.0;
+      long long hist_bound = 0;  // 32-bit bins: the most rows the round's histograms take
       hist_seg.clear();
       for (int l : order) {
         const int x = leaf_node[(size_t)l];
         Node& X = nodes[(size_t)x];
         const HostSplit& info = X.best;
-        const int lc = info.o.exact_left, rc = info.o.exact_right;
         const int dst = X.seg.buf == 1 ? 2 : 1;
+        if (derive) {
+          Node cl, cr;
+          cl.depth = cr.depth = X.depth + 1;
+          cl.ls = LeafSplits{-1, 0, info.o.left_sum_gradient, info.o.left_sum_hessian, info.o.left_output,
+                             info.o.left_int_sum};
+          cr.ls = LeafSplits{-1, 0, info.o.right_sum_gradient, info.o.right_sum_hessian, info.o.right_output,
+                             info.o.right_int_sum};
+          cl.best.o.gain = cr.best.o.gain = kMinScore;
+          PartItem& pi = h_pitems[ni];
+          pi = PartItem{};
+          pi.seg = X.seg;
+          pi.dst_buf = dst;
+          pi.split_group = feats[(size_t)info.inner].group;
+          pi.left_total = -1;  // right rows fill the segment from its end
+          pi.chunk = part_tile;
+          pi.rule = rule_of(info);
+          pi.first_task = add_tasks(X.seg.count, pi.chunk, nt);
+          pi.ntasks = nt - pi.first_task;
+          part_rows += X.seg.count;
+          ChildSpec& cs = v_cspec[(size_t)ni];
+          cs = ChildSpec{};
+          cs.kid[0] = leaf_args_with(cl.ls, -1, cl.ls.weight);
+          cs.kid[1] = leaf_args_with(cr.ls, -1, cr.ls.weight);
+          cs.depth = X.depth + 1;
+          // the smaller child's slot, taken whether or not the pair is searched
+          if ((size_t)next_slot >= num_slots) throw std::runtime_error("histogram slots exhausted");
+          cs.new_slot = next_slot++;
+          cs.parent_slot = X.slot;
+          const char* xf = flags_of(x);
+          for (int f = 0; f < F; ++f) h_used[ni * F + f] = xf[f] && cols.bytree[(size_t)f] ? kUseAll : 0;
+          hist_bound += X.seg.count / 2;
+          if (nodes.size() + 2 > max_nodes) throw std::runtime_error("node table too small");
+          X.child[0] = (int)nodes.size();
+          X.child[1] = (int)nodes.size() + 1;
+          X.expanded = true;
+          X.kids_searched = false;  // known from the counts
+          nodes.push_back(cl);
+          nodes.push_back(cr);
+          ++ni;
+          continue;
+        }
+        const int lc = info.o.exact_left, rc = info.o.exact_right;
         Node cl, cr;
         cl.depth = cr.depth = X.depth + 1;
         cl.seg = Segment{dst, X.seg.begin, lc};
@@ -2079,6 +2257,10 @@ struct Learner {
         ++ni;
       }
       if (ni == 0) break;
+      if (derive) {
+        run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, keep);
+        continue;
+      }
       HistPlan hp;
       int nr = 0;
       if (ns > 0) {
@@ -2109,8 +2291,8 @@ struct Learner {
         sm.best = fold(2 * k, flags_of(X.child[left_small ? 0 : 1]));
         lg.best = fold(2 * k + 1, flags_of(X.child[left_small ? 1 : 0]));
         if (host_fold) {
-          keep(X.child[left_small ? 0 : 1], k, 0);
-          keep(X.child[left_small ? 1 : 0], k, 1);
+          keep(X.child[left_small ? 0 : 1], k, 0, k);
+          keep(X.child[left_small ? 1 : 0], k, 1, k);
           sm.best = lg.best = HostSplit();  // folded when X is committed
         }
         ++k;
@@ -2319,6 +2501,7 @@ struct Learner {
   // ===========================================================================
   Tree train_resident(const float* g, const float* h) {
     stats = bk_train_stats{};
+    stats.hist_bin_bytes = form == FORM_QUANT ? (ds.hist32 ? 4 : 8) : 0;
     branch.assign((size_t)P.num_leaves, std::vector<int>());
     if (cols.per_tree) {
       cols.next_tree(tree_slot, tree_period);
@@ -2485,6 +2668,7 @@ struct Learner {
     wait_ms = 0.0;
     leaf_of_valid = false;
     prof.tree_on = prof.trees++ % prof.every == 0;
+    use_hist32(hist32_wanted());
     if (resident_ok(d_rows != nullptr)) {
       Tree t = train_resident(g, h);
       last_bagged = false;
@@ -2704,6 +2888,7 @@ struct Learner {
     if (quant ? out_gh != nullptr : out_int != nullptr)
       throw std::invalid_argument(quant ? "the quantized learner returns integer histograms"
                                         : "float learners return f64 histograms");
+    use_hist32(false);  // the histogram with its row counts
     before_train(g, h);
     if (cols.per_tree) set_groups_used(true);  // every group, whatever the tree's sample
     hist_seg.assign(1, segs[0]);
@@ -2742,6 +2927,7 @@ struct Learner {
   // scratch.
   void root_histogram_noalloc(const float* g, const float* h, void* d_hist, int32_t* d_count) {
     if (!g || !h || !d_hist || !d_count) throw std::invalid_argument("null argument");
+    use_hist32(false);  // the histogram with its row counts
     // partials of this tiling: at least one block per rows_per_flush rows (one
     // LDS accumulation per slot), at most what the scratch holds
     const size_t per_partial = (size_t)TB * (form == FORM_QUANT ? 8 : 20);
@@ -3038,6 +3224,7 @@ int bk_learner_set_round_planner(bk_learner* l, int on_device) {
 
 int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_blocks) {
   return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
     bk::Learner& L = l->L;
     if (min_rows == 0 && max_blocks == 0) {
       L.hist_min_rows = L.tuned_min_rows;
@@ -3051,8 +3238,20 @@ int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_bloc
   });
 }
 
+int bk_learner_set_hist_bin_width(bk_learner* l, int32_t bytes) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    if (bytes != 0 && bytes != 4 && bytes != 8) throw std::invalid_argument("histogram bin width: 0, 4 or 8 bytes");
+    if (bytes == 4 && !l->L.hist32_ok)
+      throw std::invalid_argument("32-bit histogram bins need the quantized batched form, num_grad_quant_bins <= 13 "
+                                  "and num_leaves <= 1023");
+    l->L.hist_width = bytes;
+  });
+}
+
 int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
   return bk::guard([&] {
+    if (!l || !out) throw std::invalid_argument("null argument");
     *out = l->L.stats;
     out->hist_min_rows = l->L.hist_min_rows;
     out->hist_max_blocks = l->L.hist_max_blocks;

@@ -331,6 +331,21 @@ int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iterat
  * larger than one LDS accumulation. No reference counterpart (tuning only). */
 int bk_learner_set_hist_tiling(bk_learner* learner, int32_t min_rows, int32_t max_blocks);
 
+/* Bytes per histogram bin of the quantized learner's batched growth with
+ * host-planned rounds: 4 (16-bit gradient and hessian sums, no row count: the
+ * children's exact counts then come from the partition, whose right rows fill
+ * each segment from its end), 8 (21-bit sums and a row count, exact counts
+ * from the search), or 0 (the default: 4 where it applies). 4 applies to
+ * num_grad_quant_bins <= 13 (a row block then holds at least 4096 rows within
+ * the 16-bit fields) and num_leaves <= 1023; asking for 4 elsewhere is refused
+ * (-1). The sequential form and the device planner use 8. Both widths grow the
+ * same tree, bit for bit. The reference picks 8-, 16- or 32-bit histogram
+ * fields per leaf from its row count
+ * (crates/lgbm-core/src/treelearner/gradient_discretizer/mod.rs:223-232); this
+ * choice is per learner and by bins, since a GPU block's rows are bounded
+ * instead. */
+int bk_learner_set_hist_bin_width(bk_learner* learner, int32_t bytes);
+
 /* Who plans the rounds of batched growth: 0 (the default, measured fastest on
  * MI355X: the host sees a round's end within microseconds and plans it in about
  * ten) the host, waiting for every round's results; 1 the device, for the whole
@@ -359,7 +374,7 @@ typedef struct bk_train_stats {
   int32_t splits;
   int32_t rounds;
   int32_t speculative_waste;
-  int32_t reserved;
+  int32_t hist_bin_bytes;   /* bytes per quantized histogram bin of the last tree (0: float forms) */
   int64_t committed_hist_rows;
   int64_t committed_part_rows;
   double host_ms;  /* wall time of the train call not spent blocked on the device */

@@ -53,12 +53,15 @@ def binary_10k():
     return bg.Dataset(x, bg.DatasetParams(num_threads=8)), y
 
 
-def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise, sampler=None):
+def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise, sampler=None, setup=None, check=None):
     """Boosts the oracle; the GPU learner trains each tree on the oracle's gradients
-    (and on the same bag, when a row sampler is given)."""
+    (and on the same bag, when a row sampler is given). setup(gpu) runs once after
+    the learner is made, check(gpu) after every tree."""
     const_hess = obj.const_hess and not (sampler is not None and sampler.changes_hessians)
     o = ob.OracleLearner(ds, params, const_hess)
     gpu = bg.Learner(ds, params, const_hess)
+    if setup is not None:
+        setup(gpu)
     score = np.zeros(ds.num_data)
     init = obj.boost_from_score(0)
     score += init
@@ -68,6 +71,8 @@ def lockstep(ds, obj, params, iters, lr=0.1, compare=assert_trees_bitwise, sampl
         rows = sampler.sample(it, g, h) if sampler is not None else None
         to = o.train(g, h, rows)
         tg = gpu.train(g, h, rows)
+        if check is not None:
+            check(gpu)
         compare(tg, to, f"iter {it}")
         for leaf in range(to["num_leaves"]):
             assert np.array_equal(gpu.leaf_rows(leaf), o.leaf_rows(leaf)), (it, leaf)
@@ -92,6 +97,61 @@ def test_quantized_bit_exact(binary_10k, bins, stochastic, renew, batched):
     p = bg.LearnerParams(num_leaves=31, use_quantized_grad=1, num_grad_quant_bins=bins, stochastic_rounding=stochastic,
                          quant_train_renew_leaf=renew, seed=7, num_threads=4, form=bg.FORM_QUANTIZED, batched=batched)
     lockstep(ds, bst.BinaryObjective(y), p, iters=6)
+
+
+# 32-bit bins (no row counts: the partition counts the children, right rows from
+# the segment's end) and 64-bit bins (exact counts from the search) must grow the
+# oracle's trees; the cases reach unsearchable children (max_depth, min_data),
+# empty smaller children, bags, by-node sampling and the pool's largest trees
+WIDTH_CASES = {
+    "bins4_stochastic": dict(num_leaves=31, num_grad_quant_bins=4, stochastic_rounding=1, quant_train_renew_leaf=1),
+    "bins8_nearest": dict(num_leaves=63, num_grad_quant_bins=8, stochastic_rounding=0),
+    "bins13_depth3": dict(num_leaves=31, num_grad_quant_bins=13, max_depth=3),
+    "min_data_1500": dict(num_leaves=255, num_grad_quant_bins=4, min_data_in_leaf=1500),
+    "min_data_1": dict(num_leaves=255, num_grad_quant_bins=2, min_data_in_leaf=1, min_sum_hessian_in_leaf=0.0),
+    "bynode": dict(num_leaves=31, num_grad_quant_bins=4, feature_fraction_bynode=0.5, feature_fraction_seed=5),
+}
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("case", sorted(WIDTH_CASES))
+def test_hist_bin_widths_bit_exact(binary_10k, case, width):
+    announce(f"batched quantized 10k x 28 {case} with {width}-byte histogram bins")
+    ds, y = binary_10k
+    p = bg.LearnerParams(use_quantized_grad=1, seed=11, num_threads=4, form=bg.FORM_QUANTIZED, batched=1,
+                         **WIDTH_CASES[case])
+
+    def check(gpu):
+        assert gpu.last_stats()["hist_bin_bytes"] == width
+
+    lockstep(ds, bst.BinaryObjective(y), p, iters=4, setup=lambda gpu: gpu.set_hist_bin_width(width), check=check)
+
+
+def test_hist_bin_width_bagged_and_default(binary_10k):
+    announce("32-bit bins: bagged trees, and the default width")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=3, form=bg.FORM_QUANTIZED, batched=1)
+    sampler = bst.RowSampler(dict(bagging_fraction=0.6, bagging_freq=1, bagging_seed=9), y)
+
+    def check(gpu):
+        assert gpu.last_stats()["hist_bin_bytes"] == 4  # the default where it applies
+
+    lockstep(ds, bst.BinaryObjective(y), p, iters=4, sampler=sampler, check=check)
+
+
+def test_hist_bin_width_refused_where_it_cannot_apply(binary_10k):
+    announce("32-bit bins refused: 127 gradient bins, sequential growth")
+    ds, _ = binary_10k
+    for kw in (dict(num_grad_quant_bins=127, batched=1), dict(num_grad_quant_bins=4, batched=0)):
+        p = bg.LearnerParams(num_leaves=15, use_quantized_grad=1, form=bg.FORM_QUANTIZED, **kw)
+        gpu = bg.Learner(ds, p, False)
+        with pytest.raises(bg.BankaiError):
+            gpu.set_hist_bin_width(4)
+        gpu.set_hist_bin_width(8)
+        gpu.set_hist_bin_width(0)
+        with pytest.raises(bg.BankaiError):
+            gpu.set_hist_bin_width(2)
 
 
 @pytest.mark.parametrize("leaves", [2, 3, 63, 255])
