@@ -2004,7 +2004,6 @@ struct Learner {
     });
     if (host_fold) launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
     sync_spin();
-    stats.rounds += 1;
     if (h_kcount[2 * ni] != 0) throw std::runtime_error("partition count mismatch on the device");
     long long hist_rows = 0;
     int k = 0;
@@ -2039,6 +2038,7 @@ struct Learner {
       ++k;
     }
     stats.hist_rows += hist_rows;
+    if (k > 0) stats.rounds += 1;  // rounds count search launches with work
     if (prof.tree_on) {
       const double hr = (double)hist_rows;
       if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
