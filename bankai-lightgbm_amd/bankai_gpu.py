@@ -128,7 +128,8 @@ class TrainStatsC(C.Structure):
                 ("rounds", C.c_int32), ("speculative_waste", C.c_int32), ("hist_bin_bytes", C.c_int32),
                 ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
                 ("host_ms", C.c_double), ("wait_ms", C.c_double), ("hist_min_rows", C.c_int32),
-                ("hist_max_blocks", C.c_int32)]
+                ("hist_max_blocks", C.c_int32), ("hist_tile_groups", C.c_int32), ("hist_ntiles", C.c_int32),
+                ("hist_sub_histograms", C.c_int32), ("pad0", C.c_int32)]
 
 
 class KernelStatsC(C.Structure):
@@ -177,6 +178,7 @@ EXPORTS = {
     "bk_learner_profile_trees": ([_P, C.c_int], C.c_int),
     "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
     "bk_learner_set_hist_bin_width": ([_P, C.c_int32], C.c_int),
+    "bk_learner_set_hist_layout": ([_P, C.c_int32, C.c_int32], C.c_int),
     "bk_learner_set_round_planner": ([_P, C.c_int], C.c_int),
     "bk_learner_root_histogram_noalloc": ([_P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
@@ -625,6 +627,11 @@ class Learner:
         """Forces the histogram tiling (0, 0: the autotuner's choice)."""
         _check(self._lib.bk_learner_set_hist_tiling(self.handle, int(min_rows), int(max_blocks)))
 
+    def set_hist_layout(self, groups_per_tile=0, sub_histograms=0):
+        """Forces the histogram layout: a new LDS tile every groups_per_tile groups
+        (a multiple of 16) and 1 or 2 LDS copies per tile (0, 0: the autotuner's)."""
+        _check(self._lib.bk_learner_set_hist_layout(self.handle, int(groups_per_tile), int(sub_histograms)))
+
     def set_hist_bin_width(self, nbytes=0):
         """Bytes per quantized histogram bin of batched growth: 4 (no row counts;
         the partition counts the children), 8 (with counts), 0 (4 where it applies)."""
@@ -640,7 +647,9 @@ class Learner:
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
                 "speculative_waste": s.speculative_waste, "hist_bin_bytes": s.hist_bin_bytes,
-                "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks}}
+                "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks,
+                                "tile_groups": s.hist_tile_groups, "ntiles": s.hist_ntiles,
+                                "sub_histograms": s.hist_sub_histograms}}
 
     def profile(self, kinds=True, every_trees=1):
         """Per-kernel HIP-event timing: True for every kind, False for none, or a

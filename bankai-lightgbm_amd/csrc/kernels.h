@@ -25,6 +25,7 @@ constexpr int kLdsBytes = 120 * 1024; // dynamic LDS budget per workgroup on gfx
 constexpr int kHistLdsBytes = 72 * 1024;  // LDS histogram budget: two row blocks fit on one CU
 constexpr int kMaxGroups = 1024;      // groups per row record
 constexpr int kScanThreads = 256;    // threads of the split-search blocks
+constexpr int kMaxSubHist = 2;       // LDS copies of a histogram tile (waves alternate between them)
 
 // Per inner feature, as the kernels read it.
 struct FeatDev {
@@ -348,10 +349,13 @@ size_t hist_lds_bytes(int form, int hist32, int max_tile_bins);
 // With hdr (a planned round): nitems, total_rows, nblocks and rows_per_block are
 // read from it, the arguments give the grid (nblocks: the most row blocks), and
 // the partial of (block b, item i) is slot b + i (d_block_slot unused).
+// Group tiles: tile t covers groups d_tile_first[t] .. d_tile_first[t + 1] - 1
+// (max_tile_bins bins at most); nsub LDS copies of a tile's histogram, wave w
+// adding to copy w % nsub.
 void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
                  const int* d_tile_first, int ntiles, int rows_per_flush, void* stream,
-                 const RoundHdr* hdr = nullptr);
+                 const RoundHdr* hdr = nullptr, int nsub = 1);
 // Rows per partition tile for a record stride (2048 16-byte chunks per tile).
 int partition_tile_rows(int stride);
 // Single-pass partition of every item (row order kept within a tile, not across

@@ -423,7 +423,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
                                                        const int* __restrict__ block_slot,
                                                        const int* __restrict__ group_used,
                                                        const int* __restrict__ tile_first, int rows_per_flush,
-                                                       int nblocks, int ntiles, const RoundHdr* __restrict__ hdr) {
+                                                       int nblocks, int ntiles, int nsub,
+                                                       const RoundHdr* __restrict__ hdr) {
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
   if (hdr) {  // a planned round: the work comes from the planner, the grid covers the most it can hold
@@ -445,12 +446,30 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   if (start >= end) return;  // block-uniform
   const int g0 = tile_first[tile], g1 = tile_first[tile + 1];
   const int bin_lo = s.group_bin_start[g0], nb = s.group_bin_start[g1] - bin_lo;
+  // nsub copies of the tile's histogram; wave w adds to copy w % nsub, and a
+  // flush adds the copies up (fewer same-bin atomics per copy)
+  const int hb = hist_bytes_of<FORM>(nb);
   LdsHist<FORM> L;
-  L.bind(smem, nb);
-  int* gbs = (int*)(smem + hist_bytes_of<FORM>(nb));
+  L.bind(smem + (size_t)hb * (((int)threadIdx.x >> 6) % nsub), nb);
+  int* gbs = (int*)(smem + (size_t)hb * nsub);
   PT* pay = (PT*)(gbs + ((s.num_groups + 3) & ~3));
   init_gbs(s, gbs, g0, g1, bin_lo, group_used);
-  L.zero(nb);
+  auto zero_all = [&]() {
+    for (int c = 0; c < nsub; ++c) {
+      LdsHist<FORM> Z;
+      Z.bind(smem + (size_t)hb * c, nb);
+      Z.zero(nb);
+    }
+  };
+  // thread i writes partial bin i of every copy in turn, so the adds are ordered
+  auto flush_all = [&](int slot, bool first) {
+    for (int c = 0; c < nsub; ++c) {
+      LdsHist<FORM> Z;
+      Z.bind(smem + (size_t)hb * c, nb);
+      Z.flush(s, slot, bin_lo, nb, first && c == 0);
+    }
+  };
+  zero_all();
   const int c0 = g0 >> 4, nc = ((g1 - 1) >> 4) + 1 - c0;
   const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
   const int rstep = tc / nc;
@@ -508,18 +527,18 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
       since += rows;
       __syncthreads();
       if (since >= rows_per_flush) {
-        L.flush(s, slot, bin_lo, nb, first);
+        flush_all(slot, first);
         __syncthreads();
-        L.zero(nb);
+        zero_all();
         __syncthreads();
         first = false;
         since = 0;
       }
     }
     if (since > 0 || first) {
-      L.flush(s, slot, bin_lo, nb, first);
+      flush_all(slot, first);
       __syncthreads();
-      L.zero(nb);
+      zero_all();
       __syncthreads();
     }
     x = iend;
@@ -3039,39 +3058,41 @@ size_t hist_lds_bytes(int form, int hist32, int max_tile_bins) {
 
 void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int total_rows, int nblocks,
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
-                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream, const RoundHdr* hdr) {
+                 const int* d_tile_first, int ntiles, int rows_per_flush, void* stream, const RoundHdr* hdr,
+                 int nsub) {
   if (nblocks <= 0 || (!hdr && total_rows <= 0)) return;
+  if (nsub < 1 || nsub > kMaxSubHist) throw std::runtime_error("sub-histogram count out of range");
   const unsigned groups8 = (unsigned)((nblocks + kXcds - 1) / kXcds);
   dim3 grid(groups8 * kXcds * (unsigned)ntiles);
   hipStream_t st = (hipStream_t)stream;
   const size_t pay = s.form == FORM_QUANT ? 2 : 8;
-  const size_t lds =
-      hist_lds_bytes(s.form, s.hist32, max_tile_bins) + 4 * (size_t)((s.num_groups + 3) & ~3) + pay * kHistRows;
+  const size_t lds = hist_lds_bytes(s.form, s.hist32, max_tile_bins) * nsub + 4 * (size_t)((s.num_groups + 3) & ~3) +
+                     pay * kHistRows;
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("histogram tile exceeds the LDS budget");
   switch (s.form) {
     case FORM_QUANT:
       if (s.hist32) {
         allow_lds(k_hist<kHistQ32>);
         launch(k_hist<kHistQ32>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows, rows_per_block,
-               d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks, ntiles, hdr);
+               d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks, ntiles, nsub, hdr);
         break;
       }
       allow_lds(k_hist<FORM_QUANT>);
       launch(k_hist<FORM_QUANT>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles, hdr);
+                         ntiles, nsub, hdr);
       break;
     case FORM_F64:
       allow_lds(k_hist<FORM_F64>);
       launch(k_hist<FORM_F64>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles, hdr);
+                         ntiles, nsub, hdr);
       break;
     default:
       allow_lds(k_hist<FORM_F32C>);
       launch(k_hist<FORM_F32C>, grid, dim3(kHistThreads), lds, st, s, d_items, nitems, total_rows,
                          rows_per_block, d_block_slot, d_group_used, d_tile_first, rows_per_flush, nblocks,
-                         ntiles, hdr);
+                         ntiles, nsub, hdr);
       break;
   }
   BK_HIP(hipGetLastError());

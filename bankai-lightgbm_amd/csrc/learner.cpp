@@ -554,7 +554,10 @@ struct Learner {
   std::vector<std::vector<int32_t>> bin2cat;
   // group tiles for LDS: small ones for the partition, large ones for segment histograms
   std::vector<int> tile_first;
+  std::vector<int> group_bins;  // bins of every group
   int max_tile_bins = 0;
+  int tile_width = 16, hist_nsub = 1;  // histogram layout: groups per tile, LDS copies per tile
+  int tuned_width = 16, tuned_nsub = 1;
   int stride = 0, pay_off = 0;  // row record layout
   int rows_per_flush = 0;
   size_t cache = 0;       // histogram slots of the reference pool
@@ -918,27 +921,8 @@ struct Learner {
     const int pay_bytes = form == FORM_QUANT ? 2 : 8;
     pay_off = (G + pay_bytes - 1) / pay_bytes * pay_bytes;
     stride = (pay_off + pay_bytes + 15) / 16 * 16;
-    // LDS histogram tiles of whole groups, aligned to the 16-group chunks of a
-    // record (a histogram block reads one chunk per row): smaller tiles mean
-    // more blocks and more resident waves per CU for the same bytes read
-    const int per_bin = form == FORM_QUANT ? 8 : 20;
-    const int tile_cap = kHistLdsBytes / per_bin;
-    tile_first.assign(1, 0);
-    max_tile_bins = 0;
-    {
-      int cur = 0;
-      for (int g = 0; g < G; ++g) {
-        int nb = v->group_num_total_bin[g];
-        if (cur > 0 && (cur + nb > tile_cap || g % 16 == 0)) {
-          tile_first.push_back(g);
-          max_tile_bins = std::max(max_tile_bins, cur);
-          cur = 0;
-        }
-        cur += nb;
-      }
-      tile_first.push_back(G);
-      max_tile_bins = std::max(max_tile_bins, cur);
-    }
+    group_bins.assign(v->group_num_total_bin, v->group_num_total_bin + G);
+    build_tiles(16, 1);
     // rows one LDS accumulation may hold: the packed quantized fields stay exact
     if (form == FORM_QUANT) {
       int maxq = bins + 1;
@@ -1004,7 +988,7 @@ struct Learner {
     BK_HIPC(hipMemsetAsync(d_cursors, 0, sizeof(unsigned) * (size_t)4 * max_items, stream));
     d_err = dmalloc<int>(1);
     BK_HIPC(hipMemsetAsync(d_err, 0, sizeof(int), stream));
-    d_tile_first = dmalloc<int>(tile_first.size());
+    d_tile_first = dmalloc<int>((size_t)G + 1);  // the most tiles: one per group
     BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
                            stream));
     d_group_used = dmalloc<int>((size_t)G);
@@ -1196,6 +1180,43 @@ struct Learner {
     autotune_hist();
   }
 
+  // LDS histogram tiles of whole groups: a new tile every `width` groups (a
+  // multiple of the 16 groups of a record chunk; a histogram block reads the
+  // tile's chunks of every row), or when nsub copies of the tile would exceed the
+  // LDS histogram budget. Narrow tiles mean more blocks and more resident waves
+  // per CU for the same rows; a tile of the whole record reads every row once.
+  void build_tiles(int width, int nsub) {
+    const int per_bin = form == FORM_QUANT ? 8 : 20;  // the widest bin a tree may use
+    const int tile_cap = kHistLdsBytes / (per_bin * nsub);
+    tile_first.assign(1, 0);
+    max_tile_bins = 0;
+    int cur = 0;
+    for (int g = 0; g < G; ++g) {
+      const int nb = group_bins[(size_t)g];
+      if (nb > tile_cap) throw std::invalid_argument("a feature group has more bins than the LDS histogram holds");
+      if (cur > 0 && (cur + nb > tile_cap || g % width == 0)) {
+        tile_first.push_back(g);
+        max_tile_bins = std::max(max_tile_bins, cur);
+        cur = 0;
+      }
+      cur += nb;
+    }
+    tile_first.push_back(G);
+    max_tile_bins = std::max(max_tile_bins, cur);
+    tile_width = width;
+    hist_nsub = nsub;
+    if (d_tile_first) {
+      BK_HIPC(hipMemcpyAsync(d_tile_first, tile_first.data(), sizeof(int) * tile_first.size(), hipMemcpyHostToDevice,
+                             stream));
+      sync();
+    }
+  }
+  void hist_launch(const DevState& s, const HistItem* items, int ni, int total, int nblocks, int rpb,
+                   const int* block_slot, const RoundHdr* hdr) {
+    launch_hist(s, items, ni, total, nblocks, rpb, block_slot, d_group_used, max_tile_bins, d_tile_first,
+                (int)tile_first.size() - 1, rows_per_flush, stream, hdr, hist_nsub);
+  }
+
   // 32-bit bins for the next tree: quantized, batched with host-planned rounds,
   // and not turned off by bk_learner_set_hist_bin_width
   bool hist32_wanted() const { return hist32_ok && hist_width != 8 && !planner_ok(); }
@@ -1204,63 +1225,97 @@ struct Learner {
     rows_per_flush = on ? rows_per_flush32 : rows_per_flush64;
   }
 
-  // Autotuner of the histogram tiling (rows per workgroup at least hist_min_rows,
-  // at most hist_max_blocks row blocks): times the histogram kernel on this
+  // Autotuner of the histogram tiling, three knobs: the layout (groups per
+  // tile: 16, 32 or the whole record; 1 or 2 LDS copies per tile), then the row
+  // tiling (rows per workgroup at least hist_min_rows, at most hist_max_blocks
+  // row blocks) with the chosen layout. It times the histogram kernel on this
   // dataset's records for every candidate, over all rows (a root build) and over
   // a fifth of them (a child-sized build), and keeps the candidate with the least
   // time for one root and three child builds. The payload is whatever buffer 0
   // holds: the kernel's work does not depend on its values. The choice is kept
-  // per (rows, groups, bins, form and bin width) for the process.
+  // per (rows, groups, bins, form and bin width) for the process. Results do not
+  // depend on it: the quantized histograms are exact integer sums.
   void autotune_hist() {
     static std::mutex mu;
-    static std::map<std::array<int, 4>, std::pair<int, int>> picked;
+    static std::map<std::array<int, 4>, std::array<int, 4>> picked;
     const std::array<int, 4> key{N, G, TB, form * 2 + ds.hist32};
     std::lock_guard<std::mutex> lock(mu);
     auto it = picked.find(key);
     if (it != picked.end()) {
-      hist_min_rows = tuned_min_rows = it->second.first;
-      hist_max_blocks = tuned_max_blocks = it->second.second;
+      apply_tuned(it->second);
       return;
     }
     hipEvent_t e0, e1;
     BK_HIPC(hipEventCreate(&e0));
     BK_HIPC(hipEventCreate(&e1));
+    // one root build and three child-sized builds (a fifth of the rows)
+    auto time_now = [&]() {
+      double t = 0.0;
+      for (int part = 0; part < 2; ++part) {
+        const int rows = part == 0 ? N : std::max(N / 5, 1);
+        hist_seg.assign(1, Segment{0, 0, rows});
+        const HistPlan hp = plan_hist(1);
+        begin_uploads();
+        d_hitems = put(h_hitems, (size_t)hp.ni);
+        d_hblock = put(h_hblock, (size_t)hp.nblocks);
+        send(true);
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms up, the second is timed
+          BK_HIPC(hipEventRecord(e0, stream));
+          hist_launch(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, nullptr);
+          BK_HIPC(hipEventRecord(e1, stream));
+          sync();
+          BK_HIPC(hipEventElapsedTime(&ms, e0, e1));
+        }
+        t += (part == 0 ? 1.0 : 3.0) * ms;
+      }
+      return t;
+    };
+    // (1) the layout (groups per tile, LDS copies) at a middle row tiling
+    hist_min_rows = 4096;
+    hist_max_blocks = 1024;
+    std::array<int, 4> best_c{hist_min_rows, hist_max_blocks, 16, 1};
     double best_t = 1e300;
-    std::pair<int, int> best_c{hist_min_rows, hist_max_blocks};
+    std::vector<std::vector<int>> seen;
+    for (const auto& wn : {std::array<int, 2>{16, 1}, std::array<int, 2>{32, 1}, std::array<int, 2>{kMaxGroups, 1},
+                           std::array<int, 2>{16, 2}, std::array<int, 2>{32, 2}}) {
+      build_tiles(wn[0], wn[1]);
+      std::vector<int> sig = tile_first;  // layouts that cut the same tiles are timed once
+      sig.push_back(wn[1]);
+      if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+      seen.push_back(sig);
+      const double t = time_now();
+      if (t < best_t) {
+        best_t = t;
+        best_c[2] = wn[0];
+        best_c[3] = wn[1];
+      }
+    }
+    build_tiles(best_c[2], best_c[3]);
+    // (2) the row tiling (rows per workgroup at least, row blocks at most) with it
+    best_t = 1e300;
     for (int min_rows : {2048, 4096, 8192})
       for (int max_blocks : {512, 1024, 2048}) {
         hist_min_rows = min_rows;
         hist_max_blocks = max_blocks;
-        double t = 0.0;
-        for (int part = 0; part < 2; ++part) {
-          const int rows = part == 0 ? N : std::max(N / 5, 1);
-          hist_seg.assign(1, Segment{0, 0, rows});
-          const HistPlan hp = plan_hist(1);
-          begin_uploads();
-          d_hitems = put(h_hitems, (size_t)hp.ni);
-          d_hblock = put(h_hblock, (size_t)hp.nblocks);
-          send(true);
-          float ms = 0.0f;
-          for (int rep = 0; rep < 2; ++rep) {  // the first run warms up, the second is timed
-            BK_HIPC(hipEventRecord(e0, stream));
-            launch_hist(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, d_group_used,
-                        max_tile_bins, d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
-            BK_HIPC(hipEventRecord(e1, stream));
-            sync();
-            BK_HIPC(hipEventElapsedTime(&ms, e0, e1));
-          }
-          t += (part == 0 ? 1.0 : 3.0) * ms;
-        }
+        const double t = time_now();
         if (t < best_t) {
           best_t = t;
-          best_c = {min_rows, max_blocks};
+          best_c[0] = min_rows;
+          best_c[1] = max_blocks;
         }
       }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hist_min_rows = tuned_min_rows = best_c.first;
-    hist_max_blocks = tuned_max_blocks = best_c.second;
+    apply_tuned(best_c);
     picked[key] = best_c;
+  }
+  void apply_tuned(const std::array<int, 4>& c) {
+    hist_min_rows = tuned_min_rows = c[0];
+    hist_max_blocks = tuned_max_blocks = c[1];
+    tuned_width = c[2];
+    tuned_nsub = c[3];
+    build_tiles(c[2], c[3]);
   }
 
   ScanParams scan_params() const {
@@ -1632,8 +1687,7 @@ struct Learner {
     if (hp.ni == 0) return;
     const double hb = rows * (G + payload_bytes() + (root ? 0 : 4));
     timed(root ? 0 : 1, hb, rows * (stride + (root ? payload_bytes() : 0)), [&] {
-      launch_hist(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, d_group_used, max_tile_bins,
-                  d_tile_first, (int)tile_first.size() - 1, rows_per_flush, stream);
+      hist_launch(ds, d_hitems, hp.ni, (int)hp.total, hp.nblocks, hp.rpb, d_hblock, nullptr);
     });
     stats.hist_rows += (int64_t)rows;
   }
@@ -1995,8 +2049,7 @@ struct Learner {
     });
     if (hb > 0)
       timed(1, 0.0, 0.0, [&] {
-        launch_hist(ds, d_chist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
-                    (int)tile_first.size() - 1, rows_per_flush, stream, d_chdr);
+        hist_launch(ds, d_chist, 0, 0, hb, 0, nullptr, d_chdr);
       });
     timed(3, 0.0, 0.0, [&] {
       launch_reduce(ds, d_cred, ni, stream, d_chdr);
@@ -2346,8 +2399,7 @@ struct Learner {
         launch_partition(ds, d_ppart, max_items, max_part_tiles, d_cursors, max_items, epoch, stream, hdr);
       });
       timed(1, 0.0, 0.0, [&] {
-        launch_hist(ds, d_phist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
-                    (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
+        hist_launch(ds, d_phist, 0, 0, hb, 0, nullptr, hdr);
       });
       timed(3, 0.0, 0.0, [&] {
         launch_reduce(ds, d_pred, items, stream, hdr);
@@ -2552,8 +2604,7 @@ struct Learner {
     const RoundHdr* hdr = &d_pstate->hdr;
     // the root's round: its histogram, reduce and search from the root set-up
     timed(0, (double)N * (G + payload_bytes()), (double)N * (stride + payload_bytes()), [&] {
-      launch_hist(ds, d_phist, 0, 0, hb, 0, nullptr, d_group_used, max_tile_bins, d_tile_first,
-                  (int)tile_first.size() - 1, rows_per_flush, stream, hdr);
+      hist_launch(ds, d_phist, 0, 0, hb, 0, nullptr, hdr);
     });
     if (search_root) {
       stats.hist_rows += N;
@@ -2967,8 +3018,7 @@ struct Learner {
     }
     s.pool = (double*)d_hist;  // slot 0 of this view is the caller's output
     s.pool_cnt = d_count;
-    launch_hist(s, d_hitems, 1, N, nblocks, rpb, d_hblock, d_group_used, max_tile_bins, d_tile_first,
-                (int)tile_first.size() - 1, rows_per_flush, stream);
+    hist_launch(s, d_hitems, 1, N, nblocks, rpb, d_hblock, nullptr);
     launch_reduce(s, d_ritems, 1, stream);
     if (!quant && const_hess) launch_scale_const_hess((double*)d_hist, TB, h, stream);
     sync();
@@ -3238,6 +3288,21 @@ int bk_learner_set_hist_tiling(bk_learner* l, int32_t min_rows, int32_t max_bloc
   });
 }
 
+int bk_learner_set_hist_layout(bk_learner* l, int32_t groups_per_tile, int32_t sub_histograms) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    bk::Learner& L = l->L;
+    if (groups_per_tile == 0 && sub_histograms == 0) {
+      L.build_tiles(L.tuned_width, L.tuned_nsub);
+      return;
+    }
+    if (groups_per_tile < 16 || groups_per_tile % 16 != 0 || sub_histograms < 1 ||
+        sub_histograms > bk::kMaxSubHist)
+      throw std::invalid_argument("histogram layout: groups_per_tile a positive multiple of 16, 1 or 2 sub-histograms");
+    L.build_tiles(groups_per_tile, sub_histograms);
+  });
+}
+
 int bk_learner_set_hist_bin_width(bk_learner* l, int32_t bytes) {
   return bk::guard([&] {
     if (!l) throw std::invalid_argument("null argument");
@@ -3255,6 +3320,9 @@ int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
     *out = l->L.stats;
     out->hist_min_rows = l->L.hist_min_rows;
     out->hist_max_blocks = l->L.hist_max_blocks;
+    out->hist_tile_groups = l->L.tile_width;
+    out->hist_ntiles = (int32_t)l->L.tile_first.size() - 1;
+    out->hist_sub_histograms = l->L.hist_nsub;
   });
 }
 

@@ -331,6 +331,15 @@ int bk_learner_set_tree_slot(bk_learner* learner, int slot, int trees_per_iterat
  * larger than one LDS accumulation. No reference counterpart (tuning only). */
 int bk_learner_set_hist_tiling(bk_learner* learner, int32_t min_rows, int32_t max_blocks);
 
+/* Histogram layout override: a new LDS tile every groups_per_tile groups (a
+ * positive multiple of 16, the groups of one 16-byte record chunk; a tile is
+ * also cut where its bins would exceed the LDS budget), and sub_histograms LDS
+ * copies of each tile (1 or 2; wave w adds to copy w % sub_histograms). (0, 0):
+ * back to the autotuner's choice. The autotuner picks the row tiling and this
+ * layout per dataset shape; results do not depend on either. No reference
+ * counterpart (tuning only). */
+int bk_learner_set_hist_layout(bk_learner* learner, int32_t groups_per_tile, int32_t sub_histograms);
+
 /* Bytes per histogram bin of the quantized learner's batched growth with
  * host-planned rounds: 4 (16-bit gradient and hessian sums, no row count: the
  * children's exact counts then come from the partition, whose right rows fill
@@ -379,8 +388,12 @@ typedef struct bk_train_stats {
   int64_t committed_part_rows;
   double host_ms;  /* wall time of the train call not spent blocked on the device */
   double wait_ms;  /* wall time blocked on the device (synchronizations)           */
-  int32_t hist_min_rows;    /* histogram tiling picked by the autotuner at creation: */
-  int32_t hist_max_blocks;  /* rows per workgroup at least / row blocks at most     */
+  int32_t hist_min_rows;    /* histogram tiling in use (the autotuner's unless forced): */
+  int32_t hist_max_blocks;  /* rows per workgroup at least / row blocks at most        */
+  int32_t hist_tile_groups; /* groups per LDS tile (a new tile every that many groups) */
+  int32_t hist_ntiles;      /* group tiles of a histogram build                       */
+  int32_t hist_sub_histograms; /* LDS copies of a tile's histogram                    */
+  int32_t pad0;
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 

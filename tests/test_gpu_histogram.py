@@ -111,12 +111,23 @@ def oracle_int_root_histogram(ds, p, g, h, const_hess):
 TILINGS = [(2048, 2048), (8192, 512), (65536, 4), (1 << 20, 1)]
 
 
-@pytest.mark.parametrize("bins", [4, 127])
-@pytest.mark.parametrize("tiling", TILINGS)
-def test_root_histogram_any_tiling_exact(bins, tiling):
-    announce(f"root int histogram 200k x 28 bins={bins} tiling={tiling}")
+# (groups per tile, LDS copies per tile): every layout the autotuner may pick;
+# 1024 groups per tile is the whole record
+LAYOUTS = [(16, 1), (32, 1), (1024, 1), (16, 2), (32, 2)]
+
+
+@pytest.fixture(scope="module")
+def binary_200k():
     x, _ = synth.binary_28(200_000, seed=8)
-    ds = bg.Dataset(x)
+    return bg.Dataset(x)
+
+
+@pytest.mark.parametrize("bins", [4, 127])
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tiling", TILINGS)
+def test_root_histogram_any_tiling_exact(binary_200k, bins, tiling, layout):
+    announce(f"root int histogram 200k x 28 bins={bins} tiling={tiling} layout={layout}")
+    ds = binary_200k
     rng = np.random.default_rng(11)
     g = rng.standard_normal(ds.num_data).astype(np.float32)
     h = rng.uniform(0.01, 0.25, ds.num_data).astype(np.float32)
@@ -126,9 +137,41 @@ def test_root_histogram_any_tiling_exact(bins, tiling):
                          seed=4, num_threads=1, form=bg.FORM_QUANTIZED)
     gpu = bg.Learner(ds, p, False)
     gpu.set_hist_tiling(*tiling)
+    gpu.set_hist_layout(*layout)
     ih, cnt = gpu.root_histogram(g, h)
     assert np.array_equal(ih, oracle_int_root_histogram(ds, p, g, h, False))
     assert np.array_equal(cnt, bin_counts(ds))
+    t = gpu.last_stats()["hist_tiling"]
+    assert (t["min_rows"], t["max_blocks"], t["tile_groups"], t["sub_histograms"]) == tiling + layout
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_root_histogram_layouts_wide_records(layout):
+    """100 features with NaN: tiles are also cut where a tile's bins would exceed
+    the LDS budget (not only every groups_per_tile groups)."""
+    announce(f"root int histogram 100k x 100 layout={layout}")
+    x, _ = synth.regression_100(100_000, seed=3)
+    ds = bg.Dataset(x)
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal(ds.num_data).astype(np.float32)
+    h = rng.uniform(0.01, 0.25, ds.num_data).astype(np.float32)
+    p = bg.LearnerParams(num_leaves=7, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=0,
+                         seed=4, num_threads=1, form=bg.FORM_QUANTIZED)
+    gpu = bg.Learner(ds, p, False)
+    gpu.set_hist_layout(*layout)
+    ih, cnt = gpu.root_histogram(g, h)
+    assert np.array_equal(ih, oracle_int_root_histogram(ds, p, g, h, False))
+    assert np.array_equal(cnt, bin_counts(ds))
+    assert gpu.last_stats()["hist_tiling"]["ntiles"] >= 2
+
+
+def test_hist_layout_refused_out_of_range(binary_200k):
+    announce("histogram layout arguments out of range")
+    gpu = bg.Learner(binary_200k, bg.LearnerParams(num_leaves=7, use_quantized_grad=1, form=bg.FORM_QUANTIZED), False)
+    for bad in ((8, 1), (24, 1), (16, 3), (16, 0), (0, 1)):
+        with pytest.raises(bg.BankaiError):
+            gpu.set_hist_layout(*bad)
+    gpu.set_hist_layout(0, 0)
 
 
 @pytest.mark.parametrize("const_hess", [0, 1])

@@ -154,6 +154,31 @@ def test_hist_bin_width_refused_where_it_cannot_apply(binary_10k):
             gpu.set_hist_bin_width(2)
 
 
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("layout", [(16, 1), (32, 1), (1024, 1), (16, 2), (32, 2)])
+@pytest.mark.parametrize("tiling", [(2048, 2048), (65536, 4)])
+def test_any_hist_tiling_grows_the_same_trees(binary_10k, tiling, layout, width):
+    """Every tiling the autotuner may pick (rows per block, groups per tile, LDS
+    copies per tile), with either bin width, grows the oracle's trees bit for bit."""
+    announce(f"batched quantized 10k x 28 tiling={tiling} layout={layout} width={width}")
+    ds, y = binary_10k
+    p = bg.LearnerParams(num_leaves=63, use_quantized_grad=1, num_grad_quant_bins=4, stochastic_rounding=1,
+                         quant_train_renew_leaf=1, seed=5, num_threads=4, form=bg.FORM_QUANTIZED, batched=1)
+
+    def setup(gpu):
+        gpu.set_hist_tiling(*tiling)
+        gpu.set_hist_layout(*layout)
+        gpu.set_hist_bin_width(width)
+
+    def check(gpu):
+        s = gpu.last_stats()
+        t = s["hist_tiling"]
+        assert (t["min_rows"], t["max_blocks"], t["tile_groups"], t["sub_histograms"]) == tiling + layout
+        assert s["hist_bin_bytes"] == width
+
+    lockstep(ds, bst.BinaryObjective(y), p, iters=3, setup=setup, check=check)
+
+
 @pytest.mark.parametrize("leaves", [2, 3, 63, 255])
 def test_batched_quantized_leaf_budgets(binary_10k, leaves):
     """Speculative expansion must never change the tree: small and large leaf budgets,
