@@ -129,7 +129,7 @@ class TrainStatsC(C.Structure):
                 ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
                 ("host_ms", C.c_double), ("wait_ms", C.c_double), ("hist_min_rows", C.c_int32),
                 ("hist_max_blocks", C.c_int32), ("hist_tile_groups", C.c_int32), ("hist_ntiles", C.c_int32),
-                ("hist_sub_histograms", C.c_int32), ("pad0", C.c_int32)]
+                ("hist_sub_histograms", C.c_int32), ("partition_histograms", C.c_int32)]
 
 
 class KernelStatsC(C.Structure):
@@ -179,6 +179,7 @@ EXPORTS = {
     "bk_learner_set_hist_tiling": ([_P, C.c_int32, C.c_int32], C.c_int),
     "bk_learner_set_hist_bin_width": ([_P, C.c_int32], C.c_int),
     "bk_learner_set_hist_layout": ([_P, C.c_int32, C.c_int32], C.c_int),
+    "bk_learner_set_partition_histograms": ([_P, C.c_int32], C.c_int),
     "bk_learner_set_round_planner": ([_P, C.c_int], C.c_int),
     "bk_learner_root_histogram_noalloc": ([_P, _P, _P, _P, _P], C.c_int),
     "bk_learner_free": ([_P], C.c_int),
@@ -632,6 +633,12 @@ class Learner:
         (a multiple of 16) and 1 or 2 LDS copies per tile (0, 0: the autotuner's)."""
         _check(self._lib.bk_learner_set_hist_layout(self.handle, int(groups_per_tile), int(sub_histograms)))
 
+    def set_partition_histograms(self, on=True):
+        """With 4-byte bins: the partition also builds one child's histogram per
+        split (True) or a separate launch builds the smaller child's (False, the
+        default, measured faster)."""
+        _check(self._lib.bk_learner_set_partition_histograms(self.handle, 1 if on else 0))
+
     def set_hist_bin_width(self, nbytes=0):
         """Bytes per quantized histogram bin of batched growth: 4 (no row counts;
         the partition counts the children), 8 (with counts), 0 (4 where it applies)."""
@@ -647,6 +654,7 @@ class Learner:
                 "committed_hist_rows": s.committed_hist_rows, "committed_part_rows": s.committed_part_rows,
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
                 "speculative_waste": s.speculative_waste, "hist_bin_bytes": s.hist_bin_bytes,
+                "partition_histograms": s.partition_histograms,
                 "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks,
                                 "tile_groups": s.hist_tile_groups, "ntiles": s.hist_ntiles,
                                 "sub_histograms": s.hist_sub_histograms}}

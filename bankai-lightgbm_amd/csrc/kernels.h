@@ -213,10 +213,12 @@ struct HistItem {
 // `chunk` rows (tasks first_task .. first_task + ntasks - 1). Left rows fill the
 // segment from its start; right rows follow them when left_total (the left
 // child's exact count) is known, and fill it from its end when left_total < 0,
-// the cursors then giving the children's counts.
+// the cursors then giving the children's counts. hist_side (0 left, 1 right, -1
+// none): the child whose histogram a histogram-building partition accumulates,
+// one partial per tile (task).
 struct PartItem {
   Segment seg;
-  int dst_buf, split_group, left_total, pad0;
+  int dst_buf, split_group, left_total, hist_side;
   int first_task, ntasks, chunk, pad1;
   RuleDev rule;
 };
@@ -288,10 +290,13 @@ struct ChildSpec {
   LeafArgs kid[2];
   int depth, new_slot, parent_slot, pad;
 };
-// Rules and tiling of the items derived from the partition.
+// Rules and tiling of the items derived from the partition. part_hist: the
+// partition built the histogram of each split's hist_side child (one partial per
+// tile), which is then the search item's built side, and no histogram items are
+// planned.
 struct ChildParams {
   int max_depth, min_data_in_leaf, hist_min_rows, hist_max_blocks;
-  int rows_per_flush, max_partials, F, pad;
+  int rows_per_flush, max_partials, F, part_hist;
 };
 constexpr int kChildThreads = 1024;  // splits one derivation takes (one thread each)
 
@@ -366,6 +371,17 @@ int partition_tile_rows(int stride);
 // With hdr: nitems and ntasks are read from it; the arguments give the grid.
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
                       int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr);
+// Whether the partition can build histograms (quantized 32-bit bins): the
+// record's chunks divide the block's threads evenly and every bin of the record
+// fits in LDS beside the partition's own tables.
+bool partition_hist_fits(const DevState& s);
+// The partition with histograms: for every item with hist_side >= 0, tile t also
+// adds the rows it routes to that side into an LDS histogram of the whole record
+// (groups with d_group_used[g] == 0 skipped) and writes it to partial slot t of
+// part_q (u32 words, total_bins per slot). Tiles are at most 4096 rows, within a
+// 32-bit bin's exact range. Quantized learners with hist32 only.
+void launch_partition_hist(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
+                           int max_items, unsigned epoch, const int* d_group_used, void* stream);
 // After a partition whose items have left_total < 0: per split j, the children's
 // exact counts from bank (epoch & 1) of the cursors (to counts[2j], counts[2j + 1],
 // host-mapped memory), and the round's items as the host plans them for known
@@ -374,7 +390,8 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
 // item for its smaller child and its reduce item, the histogram tiling, and hdr
 // (nhist, hist_total, hist_nblocks, hist_rpb, nscan) for the launches that follow.
 // counts[2 * nitems] is a status word: 0, or bit 1 (counts that do not add up to
-// the segment) / bit 5 (a tiling beyond max_partials), and then no work is planned.
+// the segment) / bit 2 (a searchable split whose histogram the partition did not
+// build) / bit 5 (a tiling beyond max_partials), and then no work is planned.
 // nitems <= kChildThreads.
 void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cursors, int max_items, unsigned epoch,
                      const ChildSpec* d_specs, const ChildParams& cp, HistItem* hitems, ScanItem* sitems,

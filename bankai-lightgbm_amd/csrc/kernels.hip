@@ -578,17 +578,22 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
   return v;
 }
 
-// LDS of the partition: [destinations R x u32][route flags R bytes]
+// LDS of the partition: [destinations R x u16][route flags R bytes]; with
+// histograms also [payloads R x int16][histogram total_bins x u32]
 __host__ __device__ inline size_t partition_lds(int R, int stride) {
   (void)stride;
-  return ((size_t)R * 5 + 15) & ~(size_t)15;
+  return ((size_t)R * 3 + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t partition_hist_lds(int R, int stride, int total_bins) {
+  return partition_lds(R, stride) + (((size_t)R * 2 + 15) & ~(size_t)15) + (size_t)total_bins * 4;
 }
 
-template <int FORM>
+template <int FORM, bool HIST>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
                                                             int nitems, unsigned* __restrict__ cursors,
                                                             int max_items, unsigned epoch,
-                                                            const RoundHdr* __restrict__ hdr) {
+                                                            const RoundHdr* __restrict__ hdr,
+                                                            const int* __restrict__ group_used) {
   __shared__ int sh_lbase, sh_rbase;
   __shared__ int scratch[32];
   extern __shared__ __align__(16) unsigned char smem[];
@@ -596,8 +601,10 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const int stride = s.stride, nc = stride >> 4;
   const int R = kPartThreads * kPartChunks / nc;  // rows per tile
   const int rpt = (R + kPartThreads - 1) / kPartThreads;
-  uint32_t* dstp = (uint32_t*)smem;
+  uint16_t* dstp = (uint16_t*)smem;  // R <= 8192
   uint8_t* lflag = (uint8_t*)(dstp + R);
+  int16_t* pay_lds = (int16_t*)(smem + partition_lds(R, stride));
+  unsigned* hist = (unsigned*)((unsigned char*)pay_lds + (((size_t)R * 2 + 15) & ~(size_t)15));
   const int t = blockIdx.x;
   // the cursors of the next launch start at zero
   unsigned* other = cursors + (size_t)((epoch + 1u) & 1u) * max_items * 2;
@@ -637,18 +644,29 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   uint8_t* dst = rec_of(s, it.dst_buf);
   const int total = rows * nc;
   const int cs = it.split_group >> 4, bs = it.split_group & 15;
+  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
+  // histogram of one side (block-uniform): the tile's rows of that side, over the
+  // whole record; a thread's chunk is threadIdx.x % nc for every k (nc divides
+  // the block), so its 16 group bin offsets stay in registers
+  const int side = HIST ? it.hist_side : -1;
+  if (HIST && side >= 0)
+    for (int i = threadIdx.x; i < s.total_bins; i += kPartThreads) hist[i] = 0u;
   uint4 v[kPartChunks];
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) v[k] = load_stream(src + 16 * (size_t)q);
   }
+  if (HIST && side >= 0 && seg.buf == 0)
+    for (int r = threadIdx.x; r < rows; r += kPartThreads) pay_lds[r] = ((const int16_t*)s.gh0)[t0 + r];
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) {
-      const int r = q / nc;
-      if (q - r * nc == cs) lflag[r] = route_left(rule, (lane_word(v[k], bs >> 2) >> (8 * (bs & 3))) & 0xffu) ? 1 : 0;
+      const int r = q / nc, c = q - r * nc;
+      if (c == cs) lflag[r] = route_left(rule, (lane_word(v[k], bs >> 2) >> (8 * (bs & 3))) & 0xffu) ? 1 : 0;
+      if (HIST && side >= 0 && seg.buf != 0 && c == pay_chunk)
+        pay_lds[r] = (int16_t)((lane_word(v[k], pay_in >> 2) >> (8 * (pay_in & 3))) & 0xffffu);
     }
   }
   __syncthreads();
@@ -673,7 +691,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     int il = pl, ir = tl + pr;
     for (int i = 0; i < rpt; ++i) {
       const int r = (int)threadIdx.x * rpt + i;
-      if ((vm >> i) & 1) dstp[r] = ((lm >> i) & 1) ? il++ : ir++;
+      if ((vm >> i) & 1) dstp[r] = (uint16_t)(((lm >> i) & 1) ? il++ : ir++);
     }
   }
   __syncthreads();
@@ -692,7 +710,6 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   // every lane writes its chunks straight to their destination rows; the rows of
   // a tile land in two contiguous runs, so the lanes of a wave cover contiguous
   // 16-byte pieces of them
-  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
@@ -700,10 +717,43 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
       const int r = q / nc, c = q - r * nc;
       const int l = (int)dstp[r];
       const long long d = l < tl ? lpos + l : rpos + (l - tl);
-      if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, ((const PT*)s.gh0)[t0 + r]);
+      if (seg.buf == 0 && c == pay_chunk) {
+        PT pv;
+        if constexpr (HIST) pv = side >= 0 ? (PT)pay_lds[r] : ((const PT*)s.gh0)[t0 + r];
+        else pv = ((const PT*)s.gh0)[t0 + r];
+        patch_payload(v[k], pay_in, pv);
+      }
       store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
     }
   }
+  if (!HIST || side < 0) return;  // block-uniform
+  // the side's rows into the LDS histogram (the stores above are in flight)
+  {
+    const int my_c = (int)threadIdx.x % nc;
+    int off[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int g = 16 * my_c + j;
+      off[j] = (g < s.num_groups && group_used[g]) ? s.group_bin_start[g] : -1;
+    }
+    const uint8_t want = side == 0 ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < kPartChunks; ++k) {
+      const int q = (int)threadIdx.x + k * kPartThreads;
+      if (q < total) {
+        const int r = q / nc;
+        if (lflag[r] == want) {
+          const unsigned w = qpack32(pay_lds[r]);
+#pragma unroll
+          for (int j = 0; j < 16; ++j)
+            if (off[j] >= 0) atomicAdd(&hist[off[j] + (int)((lane_word(v[k], j >> 2) >> (8 * (j & 3))) & 0xffu)], w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  unsigned* part = (unsigned*)s.part_q + (size_t)t * s.total_bins;
+  for (int i = threadIdx.x; i < s.total_bins; i += kPartThreads) part[i] = hist[i];
 }
 
 // Records of buffer 0 from column-major bins: a block stages 64 rows of every
@@ -2117,7 +2167,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
     pi.dst_buf = dst;
     pi.split_group = m.group;
     pi.left_total = lc;
-    pi.pad0 = 0;
+    pi.hist_side = -1;
     pi.first_task = first_task;
     pi.ntasks = tiles;
     pi.chunk = pp.part_tile;
@@ -2203,7 +2253,8 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
       const int depth = specs[j].depth;
       srch = !(cp.max_depth > 0 && depth >= cp.max_depth) &&
              !(rc < cp.min_data_in_leaf * 2 && lc < cp.min_data_in_leaf * 2);
-      hrows = srch ? min(lc, rc) : 0;
+      if (srch && cp.part_hist && pitems[j].hist_side < 0) atomicOr(&status, 4);  // searched but not built
+      hrows = srch && !cp.part_hist ? min(lc, rc) : 0;
     }
   }
   int sidx, row0, nscan, htotal, hidx, unused0, nhist, unused1;
@@ -2231,8 +2282,8 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
   }
   if (j < ni && srch) {
     const ChildSpec& cs = specs[j];
-    const bool left_small = lc < rc;
-    const int sm = left_small ? 0 : 1;
+    // the built side: the smaller child, or the side the partition built
+    const int sm = cp.part_hist ? pitems[j].hist_side : (lc < rc ? 0 : 1);
     LeafArgs kid[2] = {cs.kid[0], cs.kid[1]};
     kid[0].num_data = lc;
     kid[1].num_data = rc;
@@ -2244,7 +2295,10 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
     ScanItem si{};
     si.small = kid[sm];
     si.large = kid[1 - sm];
-    if (hrows > 0) {
+    if (cp.part_hist) {  // one partial per partition tile of the split
+      si.small.part_first = pitems[j].first_task;
+      si.small.nblk = pitems[j].ntasks;
+    } else if (hrows > 0) {
       const int b0 = row0 / rpb, b1 = (row0 + hrows - 1) / rpb;
       si.small.part_first = b0 + hidx;  // partial of (block b, item i) is slot b + i
       si.small.nblk = b1 - b0 + 1;
@@ -3333,18 +3387,39 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
   const size_t lds = partition_lds(R, s.stride);
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("partition tile exceeds the LDS budget");
   hipStream_t st = (hipStream_t)stream;
+  const int* no_groups = nullptr;
   switch (s.form) {
     case FORM_QUANT:
-      allow_lds(k_partition<FORM_QUANT>);
-      launch(k_partition<FORM_QUANT>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr);
+      allow_lds(k_partition<FORM_QUANT, false>);
+      launch(k_partition<FORM_QUANT, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
+             d_cursors, max_items, epoch, hdr, no_groups);
       break;
     default:
-      allow_lds(k_partition<FORM_F64>);
-      launch(k_partition<FORM_F64>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr);
+      allow_lds(k_partition<FORM_F64, false>);
+      launch(k_partition<FORM_F64, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
+             d_cursors, max_items, epoch, hdr, no_groups);
       break;
   }
+  BK_HIP(hipGetLastError());
+}
+
+bool partition_hist_fits(const DevState& s) {
+  const int nc = s.stride >> 4;
+  if (s.form != FORM_QUANT || !s.hist32 || nc < 1 || kPartThreads % nc != 0) return false;
+  const int R = partition_tile_rows(s.stride);
+  return partition_hist_lds(R, s.stride, s.total_bins) <= (size_t)kLdsBytes;
+}
+
+void launch_partition_hist(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
+                           int max_items, unsigned epoch, const int* d_group_used, void* stream) {
+  if (ntasks <= 0) return;
+  if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
+  if (!partition_hist_fits(s)) throw std::runtime_error("the partition cannot build these histograms");
+  const int R = partition_tile_rows(s.stride);
+  const size_t lds = partition_hist_lds(R, s.stride, s.total_bins);
+  allow_lds(k_partition<FORM_QUANT, true>);
+  launch(k_partition<FORM_QUANT, true>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, (hipStream_t)stream, s,
+         d_items, nitems, d_cursors, max_items, epoch, (const RoundHdr*)nullptr, d_group_used);
   BK_HIP(hipGetLastError());
 }
 

@@ -705,6 +705,8 @@ struct Learner {
   // k_children derives the round's histogram, reduce and search items
   bool hist32_ok = false;              // this learner may use them
   int hist_width = 0;                  // bk_learner_set_hist_bin_width: 0 picks, 4 or 8 forces
+  bool part_hist_ok = false;           // the partition can build the built child's histogram
+  bool part_hist_on = false;           // bk_learner_set_partition_histograms (measured slower: off)
   int rows_per_flush64 = 0, rows_per_flush32 = 0;
   std::vector<ChildSpec> v_cspec;
   HistItem* d_chist = nullptr;
@@ -954,6 +956,8 @@ struct Learner {
     // count), and at least one per rows_per_flush rows: a partial slot never holds
     // more rows than one LDS accumulation (see plan_hist)
     max_partials = std::max(std::min(kHistMaxBlocks, N / 2048 + 1), N / rows_per_flush + 2) + 2 * max_items + 8;
+    // a histogram-building partition writes one partial per tile
+    if (hist32_ok) max_partials = std::max(max_partials, max_part_tiles);
     // device allocation
     BK_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // marks the end of staging transfers, which only read pinned host memory
@@ -1176,6 +1180,11 @@ struct Learner {
     for (int g = 0; g < G; ++g) h_group_used[g] = 1;
     BK_HIPC(hipMemcpyAsync(d_group_used, h_group_used, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream));
     sync();
+    if (hist32_ok) {
+      DevState s32 = ds;
+      s32.hist32 = 1;
+      part_hist_ok = partition_hist_fits(s32) && part_tile <= rows_per_flush32 && max_partials >= max_part_tiles;
+    }
     use_hist32(hist32_wanted());  // the tuner times the bins the trees will use
     autotune_hist();
   }
@@ -2025,13 +2034,22 @@ struct Learner {
   // parent's rows); the device plans at most that.
   template <typename Keep>
   void run_derived_round(int ni, int nt, double part_rows, long long hist_bound, const std::vector<int>& order,
-                         bool host_fold, Keep&& keep) {
+                         bool host_fold, bool part_hist, Keep&& keep) {
     begin_uploads();
     d_pitems = put(h_pitems, (size_t)ni);
     const ChildSpec* u_specs = put(v_cspec.data(), (size_t)ni);
     d_used = put(h_used, (size_t)F * ni);
     send(true);
-    run_partition(ni, nt, part_rows);
+    if (part_hist) {
+      if (nt > max_part_tiles) throw std::runtime_error("too many partition tiles");
+      ++epoch;
+      timed(2, part_rows * 9.0, part_rows * 2.0 * stride, [&] {
+        launch_partition_hist(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, d_group_used, stream);
+      });
+      stats.part_rows += (int64_t)part_rows;
+    } else {
+      run_partition(ni, nt, part_rows);
+    }
     ChildParams cp{};
     cp.max_depth = P.max_depth;
     cp.min_data_in_leaf = P.min_data_in_leaf;
@@ -2040,7 +2058,8 @@ struct Learner {
     cp.rows_per_flush = rows_per_flush;
     cp.max_partials = max_partials;
     cp.F = F;
-    const int hb = hist_blocks_for(hist_bound);
+    cp.part_hist = part_hist ? 1 : 0;
+    const int hb = part_hist ? 0 : hist_blocks_for(hist_bound);
     if (hb > max_partials) throw std::runtime_error("partial buffer too small");
     const ScanParams sp = scan_params();
     timed(7, 0.0, 0.0, [&] {
@@ -2065,6 +2084,8 @@ struct Learner {
       const int lc = h_kcount[2 * j], rc = h_kcount[2 * j + 1];
       if (lc < 0 || rc < 0 || (long long)lc + rc != X.seg.count)
         throw std::runtime_error("partition count mismatch on the device");
+      const int side = h_pitems[j].hist_side;
+      if (part_hist && side >= 0) hist_rows += side == 0 ? lc : rc;  // built in the partition
       const int dst = X.seg.buf == 1 ? 2 : 1;
       const int xl = X.child[0], xr = X.child[1];
       Node& cl = nodes[(size_t)xl];
@@ -2076,11 +2097,11 @@ struct Learner {
       const bool search = children_searchable(X.depth + 1, lc, rc);
       X.kids_searched = search;
       if (!search) continue;
-      const bool left_small = lc < rc;
+      const bool left_small = part_hist ? side == 0 : lc < rc;  // the built side
       const int xs = left_small ? xl : xr, xg = left_small ? xr : xl;
       nodes[(size_t)xs].slot = v_cspec[(size_t)j].new_slot;
       nodes[(size_t)xg].slot = X.slot;
-      hist_rows += std::min(lc, rc);
+      if (!part_hist) hist_rows += std::min(lc, rc);
       nodes[(size_t)xs].best = fold(2 * k, flags_of(xs));
       nodes[(size_t)xg].best = fold(2 * k + 1, flags_of(xg));
       if (host_fold) {
@@ -2093,8 +2114,10 @@ struct Learner {
     stats.hist_rows += hist_rows;
     if (k > 0) stats.rounds += 1;  // rounds count search launches with work
     if (prof.tree_on) {
+      // a histogram built in the partition adds its bytes to the partition's
       const double hr = (double)hist_rows;
-      if ((prof.mask >> 1) & 1u) prof.add_bytes(1, hr * (G + payload_bytes() + 4), hr * stride);
+      const int kind = part_hist ? 2 : 1;
+      if ((prof.mask >> kind) & 1u) prof.add_bytes(kind, hr * (G + payload_bytes() + 4), part_hist ? 0.0 : hr * stride);
     }
   }
 
@@ -2123,8 +2146,10 @@ struct Learner {
       std::memcpy(&node_cands[(size_t)node * F], h_out + ((size_t)2 * k2 + side) * F, sizeof(SplitOut) * (size_t)F);
       for (int f = 0; f < F; ++f) node_searched[(size_t)node * F + f] = (uint8_t)(h_used[urow * F + f] & kUseSearch);
     };
-    // 32-bit bins: the children's counts come from the partition (k_children)
+    // 32-bit bins: the children's counts come from the partition (k_children),
+    // which may also build the histograms (part_hist)
     const bool derive = ds.hist32 != 0;
+    const bool part_hist = derive && part_hist_ok && part_hist_on;
     leaf_node.assign((size_t)P.num_leaves, -1);
     leaf_node[0] = 0;
     int next_slot = 1;
@@ -2228,6 +2253,11 @@ struct Learner {
           pi.dst_buf = dst;
           pi.split_group = feats[(size_t)info.inner].group;
           pi.left_total = -1;  // right rows fill the segment from its end
+          // the child whose histogram the partition builds: the smaller by the
+          // search's estimated counts (any side gives the same integer histograms)
+          pi.hist_side = -1;
+          if (part_hist && !(P.max_depth > 0 && X.depth + 1 >= P.max_depth))
+            pi.hist_side = info.o.left_count < info.o.right_count ? 0 : 1;
           pi.chunk = part_tile;
           pi.rule = rule_of(info);
           pi.first_task = add_tasks(X.seg.count, pi.chunk, nt);
@@ -2311,7 +2341,7 @@ struct Learner {
       }
       if (ni == 0) break;
       if (derive) {
-        run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, keep);
+        run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, part_hist, keep);
         continue;
       }
       HistPlan hp;
@@ -3303,6 +3333,13 @@ int bk_learner_set_hist_layout(bk_learner* l, int32_t groups_per_tile, int32_t s
   });
 }
 
+int bk_learner_set_partition_histograms(bk_learner* l, int32_t on) {
+  return bk::guard([&] {
+    if (!l) throw std::invalid_argument("null argument");
+    l->L.part_hist_on = on != 0;
+  });
+}
+
 int bk_learner_set_hist_bin_width(bk_learner* l, int32_t bytes) {
   return bk::guard([&] {
     if (!l) throw std::invalid_argument("null argument");
@@ -3323,6 +3360,7 @@ int bk_learner_last_stats(bk_learner* l, bk_train_stats* out) {
     out->hist_tile_groups = l->L.tile_width;
     out->hist_ntiles = (int32_t)l->L.tile_first.size() - 1;
     out->hist_sub_histograms = l->L.hist_nsub;
+    out->partition_histograms = l->L.ds.hist32 && l->L.part_hist_ok && l->L.part_hist_on;
   });
 }
 

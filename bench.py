@@ -28,6 +28,7 @@ def log(*a):
 
 ROUND_PLANNER = 0
 HIST_TILING = None  # (min_rows, max_blocks) forced, or None for the autotuner's choice
+HIST_OPTS = {}      # bin width, partition-built histograms: forced values, else the learner's defaults
 TREE_LOG = None     # a list collecting every tree's row counts (--tree-log), for PMC summaries
 
 
@@ -46,6 +47,10 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
     learner.set_round_planner(ROUND_PLANNER)
     if HIST_TILING:
         learner.set_hist_tiling(*HIST_TILING)
+    if "bin_width" in HIST_OPTS:
+        learner.set_hist_bin_width(HIST_OPTS["bin_width"])
+    if "partition_histograms" in HIST_OPTS:
+        learner.set_partition_histograms(HIST_OPTS["partition_histograms"])
     n = ds.num_data
     dev = torch.device("cuda:0")
     score = torch.full((n,), init, dtype=torch.float64, device=dev)
@@ -260,15 +265,20 @@ def traffic_per_row(kernel):
     return None if k is None else k.get("bytes_per_row")
 
 
-def roofline_entry(kind, ks, rows, groups, payload):
+def bs_steps(args):
+    return max(args.breakdown_steps, 1)
+
+
+def roofline_entry(kind, ks, rows, groups, payload, algo=None):
     """Roofline of one kernel kind over a timed run: algorithmic bytes (rows x the
-    per-row model) over the summed HIP-event durations of its launches, against
-    the 8 TB/s HBM peak; traffic = measured bytes per row x the same rows, per
-    launch (the same launch mix as the algorithmic bytes)."""
+    per-row model, or `algo` when given) over the summed HIP-event durations of its
+    launches, against the 8 TB/s HBM peak; traffic = measured bytes per row x the
+    same rows, per launch (the same launch mix as the algorithmic bytes)."""
     launches = max(ks["launches"], 1)
     ms = ks["ms"]
-    per_row = algo_bytes_per_row(kind, groups, payload)
-    algo = rows * per_row
+    if algo is None:
+        algo = rows * algo_bytes_per_row(kind, groups, payload)
+    per_row = algo / rows if rows else 0.0
     achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     tpr = traffic_per_row(kind)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -355,6 +365,10 @@ def main():
                     help="write every tree's row counts (histogram, partition) to this JSON file")
     ap.add_argument("--hist-tiling", default="",
                     help="force the histogram tiling as min_rows,max_blocks (default: the autotuner's choice)")
+    ap.add_argument("--hist-bin-width", type=int, choices=(0, 4, 8), default=0,
+                    help="bytes per quantized histogram bin (0: the learner's choice)")
+    ap.add_argument("--partition-histograms", choices=("on", "off", "default"), default="default",
+                    help="4-byte bins: build one child's histogram inside the partition")
     ap.add_argument("--round-planner", choices=("tree", "rounds", "host"), default="host",
                     help="who plans batched growth: the device for whole trees, the device for the rounds, the host")
     args = ap.parse_args()
@@ -362,6 +376,10 @@ def main():
     TREE_LOG = [] if args.tree_log else None
     ROUND_PLANNER = {"tree": 1, "rounds": 2, "host": 0}[args.round_planner]
     HIST_TILING = tuple(int(v) for v in args.hist_tiling.split(",")) if args.hist_tiling else None
+    if args.hist_bin_width:
+        HIST_OPTS["bin_width"] = args.hist_bin_width
+    if args.partition_histograms != "default":
+        HIST_OPTS["partition_histograms"] = args.partition_histograms == "on"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the learner has no multi-GPU path (rows would shard across GPUs "
               "with one histogram all-reduce per build; not built).", file=sys.stderr)
@@ -416,8 +434,15 @@ def main():
     # average launch duration (HIP events on the learner's stream, measured run)
     ks = kstats[dom]
     groups, payload = ds.num_features, 2
+    stride = (groups + 1) // 2 * 2 + payload  # row record: group bins, the payload, padded to 16 bytes
+    stride = (stride + 15) // 16 * 16
     per_row = algo_bytes_per_row(dom, groups, payload)
-    if per_row:
+    if dom == "partition":
+        # rows from the bytes it moves (2 records per row); its algorithmic bytes
+        # also hold the child histograms a histogram-building partition adds
+        result["roofline"] = roofline_entry(dom, ks, ks["moved_bytes"] / (2 * stride), groups, payload,
+                                            algo=ks["algo_bytes"])
+    elif per_row:
         result["roofline"] = roofline_entry(dom, ks, ks["algo_bytes"] / per_row, groups, payload)
     else:
         launches = max(ks["launches"], 1)
@@ -432,9 +457,18 @@ def main():
     # breakdown run, where every launch of every kind is timed
     result["roofline_hist"] = {
         "root": roofline_entry("root_hist", kall["root_hist"], kall["root_hist"]["algo_bytes"] / (groups + payload),
-                               groups, payload),
-        "child": roofline_entry("child_hist", kall["child_hist"],
-                                kall["child_hist"]["algo_bytes"] / (groups + payload + 4), groups, payload)}
+                               groups, payload)}
+    if kall.get("child_hist", {}).get("launches", 0) > 0:
+        result["roofline_hist"]["child"] = roofline_entry("child_hist", kall["child_hist"],
+                                                          kall["child_hist"]["algo_bytes"] / (groups + payload + 4),
+                                                          groups, payload)
+    else:  # built inside the partition: its bytes and time are the partition's
+        part = kall["partition"]
+        part_rows = part["moved_bytes"] / (2 * stride)
+        result["roofline_hist"]["child"] = {
+            "built_in": "partition", "hist_algo_bytes_per_iter": (part["algo_bytes"] - 9.0 * part_rows) / bs_steps(
+                args),
+            "partition": roofline_entry("partition", part, part_rows, groups, payload, algo=part["algo_bytes"])}
     bs = args.breakdown_steps
     result["kernels"] = {k: {"ms_per_iter": v["ms"] / bs, "launches_per_iter": v["launches"] / bs,
                              "algo_GBs": v["algo_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0,

@@ -355,6 +355,18 @@ int bk_learner_set_hist_layout(bk_learner* learner, int32_t groups_per_tile, int
  * instead. */
 int bk_learner_set_hist_bin_width(bk_learner* learner, int32_t bytes);
 
+/* With 4-byte bins, on = 1 has the partition also build the histogram of one
+ * child of every split: each partition tile adds the rows it routes to that side
+ * into an LDS histogram of the whole record and writes it as one partial, so no
+ * separate histogram launch reads the child's rows again. The side is the
+ * smaller child by the search's estimated counts (the integer histograms, and
+ * the trees, do not depend on it). Applies where the record's bins fit in LDS
+ * beside the partition's tables (about 23,000 bins). on = 0, the default,
+ * builds the smaller child in its own launch: on MI355X at 10.5M x 28 that is
+ * faster (the atomics do not overlap the partition's streaming, and a partial
+ * per 4096-row tile costs more to reduce). No reference counterpart. */
+int bk_learner_set_partition_histograms(bk_learner* learner, int32_t on);
+
 /* Who plans the rounds of batched growth: 0 (the default, measured fastest on
  * MI355X: the host sees a round's end within microseconds and plans it in about
  * ten) the host, waiting for every round's results; 1 the device, for the whole
@@ -393,7 +405,7 @@ typedef struct bk_train_stats {
   int32_t hist_tile_groups; /* groups per LDS tile (a new tile every that many groups) */
   int32_t hist_ntiles;      /* group tiles of a histogram build                       */
   int32_t hist_sub_histograms; /* LDS copies of a tile's histogram                    */
-  int32_t pad0;
+  int32_t partition_histograms; /* 1: the partition built the children's histograms  */
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 

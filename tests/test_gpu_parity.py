@@ -113,18 +113,28 @@ WIDTH_CASES = {
 }
 
 
-@pytest.mark.parametrize("width", [4, 8])
+# (bin width, histograms built in the partition)
+WIDTHS = [(4, 1), (4, 0), (8, 0)]
+
+
+@pytest.mark.parametrize("width,part_hist", WIDTHS)
 @pytest.mark.parametrize("case", sorted(WIDTH_CASES))
-def test_hist_bin_widths_bit_exact(binary_10k, case, width):
-    announce(f"batched quantized 10k x 28 {case} with {width}-byte histogram bins")
+def test_hist_bin_widths_bit_exact(binary_10k, case, width, part_hist):
+    announce(f"batched quantized 10k x 28 {case} with {width}-byte histogram bins, partition-built={part_hist}")
     ds, y = binary_10k
     p = bg.LearnerParams(use_quantized_grad=1, seed=11, num_threads=4, form=bg.FORM_QUANTIZED, batched=1,
                          **WIDTH_CASES[case])
 
-    def check(gpu):
-        assert gpu.last_stats()["hist_bin_bytes"] == width
+    def setup(gpu):
+        gpu.set_hist_bin_width(width)
+        gpu.set_partition_histograms(part_hist)
 
-    lockstep(ds, bst.BinaryObjective(y), p, iters=4, setup=lambda gpu: gpu.set_hist_bin_width(width), check=check)
+    def check(gpu):
+        s = gpu.last_stats()
+        assert s["hist_bin_bytes"] == width
+        assert s["partition_histograms"] == part_hist
+
+    lockstep(ds, bst.BinaryObjective(y), p, iters=4, setup=setup, check=check)
 
 
 def test_hist_bin_width_bagged_and_default(binary_10k):
@@ -154,10 +164,10 @@ def test_hist_bin_width_refused_where_it_cannot_apply(binary_10k):
             gpu.set_hist_bin_width(2)
 
 
-@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("width,part_hist", WIDTHS)
 @pytest.mark.parametrize("layout", [(16, 1), (32, 1), (1024, 1), (16, 2), (32, 2)])
 @pytest.mark.parametrize("tiling", [(2048, 2048), (65536, 4)])
-def test_any_hist_tiling_grows_the_same_trees(binary_10k, tiling, layout, width):
+def test_any_hist_tiling_grows_the_same_trees(binary_10k, tiling, layout, width, part_hist):
     """Every tiling the autotuner may pick (rows per block, groups per tile, LDS
     copies per tile), with either bin width, grows the oracle's trees bit for bit."""
     announce(f"batched quantized 10k x 28 tiling={tiling} layout={layout} width={width}")
@@ -169,6 +179,7 @@ def test_any_hist_tiling_grows_the_same_trees(binary_10k, tiling, layout, width)
         gpu.set_hist_tiling(*tiling)
         gpu.set_hist_layout(*layout)
         gpu.set_hist_bin_width(width)
+        gpu.set_partition_histograms(part_hist)
 
     def check(gpu):
         s = gpu.last_stats()
