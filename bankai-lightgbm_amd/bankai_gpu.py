@@ -129,7 +129,9 @@ class TrainStatsC(C.Structure):
                 ("committed_hist_rows", C.c_int64), ("committed_part_rows", C.c_int64),
                 ("host_ms", C.c_double), ("wait_ms", C.c_double), ("hist_min_rows", C.c_int32),
                 ("hist_max_blocks", C.c_int32), ("hist_tile_groups", C.c_int32), ("hist_ntiles", C.c_int32),
-                ("hist_sub_histograms", C.c_int32), ("partition_histograms", C.c_int32)]
+                ("hist_sub_histograms", C.c_int32), ("partition_histograms", C.c_int32),
+                ("replay_ms", C.c_double), ("plan_ms", C.c_double), ("launch_ms", C.c_double),
+                ("readback_ms", C.c_double)]
 
 
 class KernelStatsC(C.Structure):
@@ -655,6 +657,8 @@ class Learner:
                 "host_ms": s.host_ms, "wait_ms": s.wait_ms,
                 "speculative_waste": s.speculative_waste, "hist_bin_bytes": s.hist_bin_bytes,
                 "partition_histograms": s.partition_histograms,
+                "host_phases_ms": {"replay": s.replay_ms, "plan": s.plan_ms, "launch": s.launch_ms,
+                                   "readback": s.readback_ms},
                 "hist_tiling": {"min_rows": s.hist_min_rows, "max_blocks": s.hist_max_blocks,
                                 "tile_groups": s.hist_tile_groups, "ntiles": s.hist_ntiles,
                                 "sub_histograms": s.hist_sub_histograms}}

@@ -299,6 +299,22 @@ struct ChildParams {
   int rows_per_flush, max_partials, F, part_hist;
 };
 constexpr int kChildThreads = 1024;  // splits one derivation takes (one thread each)
+constexpr int kPartDeriveMax = 512;  // splits the partition's last block derives (one thread each)
+// The derivation run by the partition's last block (on != 0; at most 512 splits),
+// with what launch_children takes: specs, params, the item tables, hdr, the
+// host-mapped counts, and a zeroed ticket word the partition's blocks count on
+// (the last one resets it).
+struct DeriveArgs {
+  const ChildSpec* specs;
+  HistItem* hitems;
+  ScanItem* sitems;
+  ReduceItem* ritems;
+  RoundHdr* hdr;
+  int* counts;
+  unsigned* ticket;
+  ChildParams cp;
+  int on, pad;
+};
 
 struct PlanParams {
   int num_leaves, F, max_depth, min_data_in_leaf;
@@ -369,8 +385,11 @@ int partition_tile_rows(int stride);
 // launch; a launch uses bank (epoch & 1) and zeroes the other, so consecutive
 // launches on one stream alternate epoch parity.
 // With hdr: nitems and ntasks are read from it; the arguments give the grid.
+// derive (may be null): the last block to finish derives the round's items as
+// launch_children would (left_total < 0 items, no hdr, nitems <= 512).
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr);
+                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr,
+                      const DeriveArgs* derive = nullptr);
 // Whether the partition can build histograms (quantized 32-bit bins): the
 // record's chunks divide the block's threads evenly and every bin of the record
 // fits in LDS beside the partition's own tables.
@@ -381,7 +400,8 @@ bool partition_hist_fits(const DevState& s);
 // part_q (u32 words, total_bins per slot). Tiles are at most 4096 rows, within a
 // 32-bit bin's exact range. Quantized learners with hist32 only.
 void launch_partition_hist(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                           int max_items, unsigned epoch, const int* d_group_used, void* stream);
+                           int max_items, unsigned epoch, const int* d_group_used, void* stream,
+                           const DeriveArgs* derive = nullptr);
 // After a partition whose items have left_total < 0: per split j, the children's
 // exact counts from bank (epoch & 1) of the cursors (to counts[2j], counts[2j + 1],
 // host-mapped memory), and the round's items as the host plans them for known
@@ -404,8 +424,12 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
 // Split search of every (feature, item), then the fold of the best candidate of
 // every (item, side) into d_best[2 * item + side] and of every feature's
 // splittable flag into d_flags[(2 * item + side) * F + f].
+// stamp (may be null): the fold's last wave to finish writes seq to *stamp (a
+// host-mapped word), after every fold result is visible to the host; ticket: a
+// zeroed word the fold's waves count on (reset by the last).
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr = nullptr);
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr = nullptr,
+                 unsigned* stamp = nullptr, unsigned seq = 0, unsigned* ticket = nullptr);
 // One round of planning: absorbs the last round's folded splits into their
 // nodes, replays the reference's best-first commits while the best leaf is
 // expanded, picks the next frontier (the best unexpanded leaves with positive

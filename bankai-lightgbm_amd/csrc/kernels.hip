@@ -572,6 +572,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 constexpr int kPartThreads = 512;
 constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
 constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
+static_assert(kPartDeriveMax == kPartThreads, "the partition's last block derives one split per thread");
 
 __device__ __forceinline__ long long wave_sum64(long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -588,12 +589,35 @@ __host__ __device__ inline size_t partition_hist_lds(int R, int stride, int tota
   return partition_lds(R, stride) + (((size_t)R * 2 + 15) & ~(size_t)15) + (size_t)total_bins * 4;
 }
 
+template <int NT>
+__device__ void derive_children(const PartItem* __restrict__ pitems, int ni, const unsigned* cur,
+                                const ChildSpec* __restrict__ specs, const ChildParams& cp,
+                                HistItem* __restrict__ hitems, ScanItem* __restrict__ sitems,
+                                ReduceItem* __restrict__ ritems, RoundHdr* __restrict__ hdr, int* __restrict__ counts);
+
+// End of a partition block: with a derivation, the block counts itself done
+// (thread 0's cursor atomics returned before, so they are complete) and the last
+// block derives the round's items from the final cursors.
+__device__ __forceinline__ void partition_done(const PartItem* __restrict__ items, int nitems,
+                                               const unsigned* cursors, int max_items, unsigned epoch,
+                                               const DeriveArgs& da) {
+  if (!da.on) return;
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(da.ticket, 1u) == gridDim.x - 1u ? 1 : 0;
+  __syncthreads();
+  if (!last) return;  // block-uniform
+  if (threadIdx.x == 0) *da.ticket = 0u;  // every block has counted: the next launch starts at zero
+  derive_children<kPartThreads>(items, nitems, cursors + (size_t)(epoch & 1u) * max_items * 2, da.specs, da.cp,
+                                da.hitems, da.sitems, da.ritems, da.hdr, da.counts);
+}
+
 template <int FORM, bool HIST>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
                                                             int nitems, unsigned* __restrict__ cursors,
                                                             int max_items, unsigned epoch,
                                                             const RoundHdr* __restrict__ hdr,
-                                                            const int* __restrict__ group_used) {
+                                                            const int* __restrict__ group_used, DeriveArgs da) {
   __shared__ int sh_lbase, sh_rbase;
   __shared__ int scratch[32];
   extern __shared__ __align__(16) unsigned char smem[];
@@ -699,9 +723,11 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   // left_total < 0: the right rows fill the segment from its end (each tile's run
   // in row order), and the cursors end at the children's counts
   const bool from_end = left_total < 0;
-  if (from_end ? lbase + tl + rbase + tr > seg.count
-               : (lbase + tl > left_total || rbase + tr > seg.count - left_total)) {
-    if (threadIdx.x == 0) atomicOr(s.err_flag, 2);  // inconsistent counts: write nothing
+  const bool bad = from_end ? lbase + tl + rbase + tr > seg.count
+                            : (lbase + tl > left_total || rbase + tr > seg.count - left_total);
+  if (bad) {  // inconsistent counts: the tile writes nothing (block-uniform)
+    if (threadIdx.x == 0) atomicOr(s.err_flag, 2);
+    partition_done(items, nitems, cursors, max_items, epoch, da);
     return;
   }
   const long long lpos = (long long)seg.begin + lbase;
@@ -726,7 +752,10 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
       store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
     }
   }
-  if (!HIST || side < 0) return;  // block-uniform
+  if (!HIST || side < 0) {  // block-uniform
+    partition_done(items, nitems, cursors, max_items, epoch, da);
+    return;
+  }
   // the side's rows into the LDS histogram (the stores above are in flight)
   {
     const int my_c = (int)threadIdx.x % nc;
@@ -754,6 +783,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   __syncthreads();
   unsigned* part = (unsigned*)s.part_q + (size_t)t * s.total_bins;
   for (int i = threadIdx.x; i < s.total_bins; i += kPartThreads) part[i] = hist[i];
+  partition_done(items, nitems, cursors, max_items, epoch, da);
 }
 
 // Records of buffer 0 from column-major bins: a block stages 64 rows of every
@@ -1804,12 +1834,21 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(DevState s, const ScanIte
 
 // One wave per (item, side): the fold runs after the search launch, whose
 // candidates it reads (a kernel boundary orders them across the XCDs' L2s).
+// With a stamp, every wave makes its results visible to the host and counts
+// itself done; the last one writes seq to the host-mapped stamp word.
 __global__ void __launch_bounds__(64) k_fold(const ScanItem* __restrict__ items, const SplitOut* __restrict__ out_all,
                                              const int* __restrict__ used_all, int F, FoldOut* __restrict__ best,
-                                             uint8_t* __restrict__ flags, const RoundHdr* __restrict__ hdr) {
+                                             uint8_t* __restrict__ flags, const RoundHdr* __restrict__ hdr,
+                                             unsigned* stamp, unsigned seq, unsigned* ticket) {
   const int k = blockIdx.x >> 1, side = blockIdx.x & 1;
-  if (hdr && k >= hdr->nscan) return;
-  fold_side(items[k], k, side, out_all, used_all, F, best, flags);
+  if (!(hdr && k >= hdr->nscan)) fold_side(items[k], k, side, out_all, used_all, F, best, flags);
+  if (!stamp) return;
+  __threadfence_system();  // this wave's results reach host memory before it counts itself done
+  if ((threadIdx.x & 63) == 0 && atomicAdd(ticket, 1u) == gridDim.x - 1u) {
+    *ticket = 0u;  // every wave has counted: the next launch starts at zero
+    __threadfence_system();
+    __hip_atomic_store(stamp, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2225,13 +2264,14 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
 // slot), the smaller child's histogram item (its rows are contiguous after the
 // partition), its reduce item, and the histogram tiling.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __restrict__ pitems, int ni,
-                                                            const unsigned* __restrict__ cur,
-                                                            const ChildSpec* __restrict__ specs, ChildParams cp,
-                                                            HistItem* __restrict__ hitems,
-                                                            ScanItem* __restrict__ sitems,
-                                                            ReduceItem* __restrict__ ritems,
-                                                            RoundHdr* __restrict__ hdr, int* __restrict__ counts) {
+// The derivation of one round over NT threads (ni <= NT), from the partition's
+// cursors cur (read with coherent loads: the partition's last block runs it
+// while other blocks' cursor atomics have just completed).
+template <int NT>
+__device__ void derive_children(const PartItem* __restrict__ pitems, int ni, const unsigned* cur,
+                                const ChildSpec* __restrict__ specs, const ChildParams& cp,
+                                HistItem* __restrict__ hitems, ScanItem* __restrict__ sitems,
+                                ReduceItem* __restrict__ ritems, RoundHdr* __restrict__ hdr, int* __restrict__ counts) {
   __shared__ int scratch[32];
   __shared__ int status;
   const int j = threadIdx.x;
@@ -2243,8 +2283,8 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
   if (j < ni) {
     seg = pitems[j].seg;
     dst = pitems[j].dst_buf;
-    lc = (int)cur[2 * j];
-    rc = (int)cur[2 * j + 1];
+    lc = (int)__hip_atomic_load(cur + 2 * j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    rc = (int)__hip_atomic_load(cur + 2 * j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     counts[2 * j] = lc;
     counts[2 * j + 1] = rc;
     if (lc < 0 || rc < 0 || (long long)lc + rc != seg.count) {
@@ -2258,8 +2298,8 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
     }
   }
   int sidx, row0, nscan, htotal, hidx, unused0, nhist, unused1;
-  block_scan2<kChildThreads>(srch, hrows, &sidx, &row0, &nscan, &htotal, scratch);
-  block_scan2<kChildThreads>(hrows > 0 ? 1 : 0, 0, &hidx, &unused0, &nhist, &unused1, scratch);
+  block_scan2<NT>(srch, hrows, &sidx, &row0, &nscan, &htotal, scratch);
+  block_scan2<NT>(hrows > 0 ? 1 : 0, 0, &hidx, &unused0, &nhist, &unused1, scratch);
   int nblocks = 0, rpb = 0;
   if (htotal > 0) {  // as the host cuts a build of htotal rows
     long long nb = min((long long)cp.hist_max_blocks, max(1ll, (long long)htotal / cp.hist_min_rows));
@@ -2315,6 +2355,16 @@ __global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __re
     counts[2 * ni] = 0;
     *hdr = RoundHdr{ni, 0, nhist, htotal, nblocks, rpb, nscan, 0};
   }
+}
+
+__global__ void __launch_bounds__(kChildThreads) k_children(const PartItem* __restrict__ pitems, int ni,
+                                                            const unsigned* __restrict__ cur,
+                                                            const ChildSpec* __restrict__ specs, ChildParams cp,
+                                                            HistItem* __restrict__ hitems,
+                                                            ScanItem* __restrict__ sitems,
+                                                            ReduceItem* __restrict__ ritems,
+                                                            RoundHdr* __restrict__ hdr, int* __restrict__ counts) {
+  derive_children<kChildThreads>(pitems, ni, cur, specs, cp, hitems, sitems, ritems, hdr, counts);
 }
 
 // ---------------------------------------------------------------------------
@@ -3006,6 +3056,8 @@ __global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ 
 // partial (32 consecutive bins of one partial are one coalesced read); the 8
 // sums are then added in lane order, so float forms reduce in a fixed order.
 // ---------------------------------------------------------------------------
+constexpr int kReduceMaxItemsY = 64;  // grid rows of a reduce whose item count the device decides
+
 template <int FORM>
 __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __restrict__ items,
                                                 const RoundHdr* __restrict__ hdr) {
@@ -3025,7 +3077,21 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
     if (bin < s.total_bins) {
       const size_t base = (size_t)it.part_first * stride + bin;
       if (FORM == FORM_QUANT) {
-        for (int j = sp; j < it.nblk; j += 8) {
+        int j = sp;
+        for (; j + 24 < it.nblk; j += 32) {
+          unsigned long long w[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w[u] = s.part_q[base + (size_t)(j + 8 * u) * stride];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            long long gg, hh, cc;
+            qunpack(w[u], &gg, &hh, &cc);
+            g += gg;
+            h += hh;
+            c += cc;
+          }
+        }
+        for (; j < it.nblk; j += 8) {
           long long gg, hh, cc;
           qunpack(s.part_q[base + (size_t)j * stride], &gg, &hh, &cc);
           g += gg;
@@ -3033,8 +3099,22 @@ __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __
           c += cc;
         }
       } else if (FORM == kHistQ32) {
+        // four partials in flight per step: the sum is latency-bound, not bandwidth-bound
         const unsigned* pq = (const unsigned*)s.part_q;
-        for (int j = sp; j < it.nblk; j += 8) {
+        int j = sp;
+        for (; j + 24 < it.nblk; j += 32) {
+          unsigned w[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w[u] = pq[base + (size_t)(j + 8 * u) * stride];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            long long gg, hh;
+            qunpack32(w[u], &gg, &hh);
+            g += gg;
+            h += hh;
+          }
+        }
+        for (; j < it.nblk; j += 8) {
           long long gg, hh;
           qunpack32(pq[base + (size_t)j * stride], &gg, &hh);
           g += gg;
@@ -3379,9 +3459,14 @@ void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cur
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
 
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr) {
+                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr, const DeriveArgs* derive) {
   if (ntasks <= 0) return;
   if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
+  DeriveArgs da{};
+  if (derive && derive->on) {
+    if (hdr || nitems > kPartDeriveMax) throw std::runtime_error("the partition derives at most 512 splits, unplanned");
+    da = *derive;
+  }
   const int R = partition_tile_rows(s.stride);
   if (R < 1) throw std::runtime_error("row record too large for the partition tile");
   const size_t lds = partition_lds(R, s.stride);
@@ -3392,12 +3477,12 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
     case FORM_QUANT:
       allow_lds(k_partition<FORM_QUANT, false>);
       launch(k_partition<FORM_QUANT, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr, no_groups);
+             d_cursors, max_items, epoch, hdr, no_groups, da);
       break;
     default:
       allow_lds(k_partition<FORM_F64, false>);
       launch(k_partition<FORM_F64, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr, no_groups);
+             d_cursors, max_items, epoch, hdr, no_groups, da);
       break;
   }
   BK_HIP(hipGetLastError());
@@ -3411,15 +3496,21 @@ bool partition_hist_fits(const DevState& s) {
 }
 
 void launch_partition_hist(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                           int max_items, unsigned epoch, const int* d_group_used, void* stream) {
+                           int max_items, unsigned epoch, const int* d_group_used, void* stream,
+                           const DeriveArgs* derive) {
   if (ntasks <= 0) return;
   if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
+  DeriveArgs da{};
+  if (derive && derive->on) {
+    if (nitems > kPartDeriveMax) throw std::runtime_error("the partition derives at most 512 splits");
+    da = *derive;
+  }
   if (!partition_hist_fits(s)) throw std::runtime_error("the partition cannot build these histograms");
   const int R = partition_tile_rows(s.stride);
   const size_t lds = partition_hist_lds(R, s.stride, s.total_bins);
   allow_lds(k_partition<FORM_QUANT, true>);
   launch(k_partition<FORM_QUANT, true>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, (hipStream_t)stream, s,
-         d_items, nitems, d_cursors, max_items, epoch, (const RoundHdr*)nullptr, d_group_used);
+         d_items, nitems, d_cursors, max_items, epoch, (const RoundHdr*)nullptr, d_group_used, da);
   BK_HIP(hipGetLastError());
 }
 
@@ -3435,7 +3526,7 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
 void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream, const RoundHdr* hdr) {
   if (nitems <= 0) return;
   // a planned round's kernel loops over its items: a capped grid holds them
-  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)(hdr ? std::min(nitems, 16) : nitems));
+  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)(hdr ? std::min(nitems, kReduceMaxItemsY) : nitems));
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
@@ -3454,7 +3545,9 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
 
 
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
-                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr) {
+                 SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr,
+                 unsigned* stamp, unsigned seq, unsigned* ticket) {
+  if (stamp && !ticket) throw std::runtime_error("a stamped fold needs its ticket word");
   if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
   dim3 grid((unsigned)s.num_features, (unsigned)(hdr ? std::min(nitems, 64) : nitems));
@@ -3471,7 +3564,7 @@ void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const S
   }
   BK_HIP(hipGetLastError());
   launch(k_fold, dim3((unsigned)(2 * nitems)), dim3(64), 0, st, d_items, d_out, d_used, s.num_features,
-                     d_best, d_flags, hdr);
+                     d_best, d_flags, hdr, stamp, seq, ticket);
   BK_HIP(hipGetLastError());
 }
 

@@ -714,6 +714,8 @@ struct Learner {
   ReduceItem* d_cred = nullptr;
   RoundHdr* d_chdr = nullptr;
   int* h_kcount = nullptr;             // pinned: the children's counts, then a status word
+  unsigned* d_tickets = nullptr;       // [0]: the partition's blocks (derivation), [1]: the fold's waves (stamp)
+  bool derive_in_partition = true;     // the partition's last block derives the round's items
   // device-resident trees: one host wait per tree
   bool resident_on = false;            // bk_learner_set_round_planner(…, 1) turns it on
   DiscState* d_disc = nullptr;
@@ -775,6 +777,11 @@ struct Learner {
   void sync_spin() {
     const unsigned seq = ++stamp_seq;
     launch_stamp(dev_alias(h_stamp), seq, stream);
+    wait_stamp(seq);
+  }
+  // The wait of sync_spin for a stamp that the stream's last kernel writes itself
+  // (a stamped fold).
+  void wait_stamp(unsigned seq) {
     const auto w0 = std::chrono::steady_clock::now();
     bool seen = false;
     for (unsigned i = 1;; ++i) {
@@ -818,7 +825,7 @@ struct Learner {
                     (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
                     (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree,
                     (void*)d_disc, (void*)d_node_count, (void*)d_chist, (void*)d_cscan, (void*)d_cred,
-                    (void*)d_chdr})
+                    (void*)d_chdr, (void*)d_tickets})
       f(p);
     for (void* p : {(void*)h_arena, (void*)h_back, (void*)h_tail, (void*)h_misc, (void*)h_stamp,
                     (void*)h_group_used, (void*)h_segs, (void*)h_order, (void*)h_vals, (void*)h_nodes,
@@ -1042,6 +1049,8 @@ struct Learner {
       d_chdr = dmalloc<RoundHdr>(1);
       pinned(&h_kcount, (size_t)2 * max_items + 1);
     }
+    d_tickets = dmalloc<unsigned>(2);
+    BK_HIPC(hipMemsetAsync(d_tickets, 0, sizeof(unsigned) * 2, stream));
     d_max4 = dmalloc<unsigned>(4);
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
     d_sums3 = dmalloc<long long>(4);
@@ -1729,14 +1738,21 @@ struct Learner {
   void run_scan(int nitems, int nr) {
     if (nitems <= 0) return;
     ScanParams sp = scan_params();
+    // the batched form with by-node sampling folds on the host, later: the
+    // candidates are copied back after the fold, and a stamp follows the copy
+    const bool copy_out = batched && cols.by_node_active();
+    const unsigned seq = copy_out ? 0u : ++stamp_seq;
     timed(3, 0.0, 0.0, [&] {
       launch_reduce(ds, d_ritems, nr, stream);
-      launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, d_fold, d_flags, stream);
+      launch_scan(ds, d_sitems, nitems, sp, d_used, d_out, d_fold, d_flags, stream, nullptr,
+                  copy_out ? nullptr : dev_alias(h_stamp), seq, d_tickets + 1);
     });
-    // the batched form with by-node sampling folds on the host, later
-    if (batched && cols.by_node_active())
+    if (copy_out) {
       launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)nitems, stream);
-    sync_spin();
+      sync_spin();
+    } else {
+      wait_stamp(seq);
+    }
     stats.rounds += 1;
   }
 
@@ -2035,21 +2051,12 @@ struct Learner {
   template <typename Keep>
   void run_derived_round(int ni, int nt, double part_rows, long long hist_bound, const std::vector<int>& order,
                          bool host_fold, bool part_hist, Keep&& keep) {
+    const auto l0 = std::chrono::steady_clock::now();
     begin_uploads();
     d_pitems = put(h_pitems, (size_t)ni);
     const ChildSpec* u_specs = put(v_cspec.data(), (size_t)ni);
     d_used = put(h_used, (size_t)F * ni);
     send(true);
-    if (part_hist) {
-      if (nt > max_part_tiles) throw std::runtime_error("too many partition tiles");
-      ++epoch;
-      timed(2, part_rows * 9.0, part_rows * 2.0 * stride, [&] {
-        launch_partition_hist(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, d_group_used, stream);
-      });
-      stats.part_rows += (int64_t)part_rows;
-    } else {
-      run_partition(ni, nt, part_rows);
-    }
     ChildParams cp{};
     cp.max_depth = P.max_depth;
     cp.min_data_in_leaf = P.min_data_in_leaf;
@@ -2062,20 +2069,41 @@ struct Learner {
     const int hb = part_hist ? 0 : hist_blocks_for(hist_bound);
     if (hb > max_partials) throw std::runtime_error("partial buffer too small");
     const ScanParams sp = scan_params();
-    timed(7, 0.0, 0.0, [&] {
-      launch_children(d_pitems, ni, d_cursors, max_items, epoch, u_specs, cp, d_chist, d_cscan, d_cred, d_chdr,
-                      dev_alias(h_kcount), stream);
+    // the derivation: in the partition's last block, or a launch of its own
+    DeriveArgs da{u_specs, d_chist, d_cscan, d_cred, d_chdr, dev_alias(h_kcount), d_tickets, cp,
+                  derive_in_partition && ni <= kPartDeriveMax ? 1 : 0, 0};
+    if (nt > max_part_tiles) throw std::runtime_error("too many partition tiles");
+    ++epoch;
+    timed(2, part_rows * 9.0, part_rows * 2.0 * stride, [&] {
+      if (part_hist)
+        launch_partition_hist(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, d_group_used, stream, &da);
+      else
+        launch_partition(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, stream, nullptr, &da);
     });
+    stats.part_rows += (int64_t)part_rows;
+    if (!da.on)
+      timed(7, 0.0, 0.0, [&] {
+        launch_children(d_pitems, ni, d_cursors, max_items, epoch, u_specs, cp, d_chist, d_cscan, d_cred, d_chdr,
+                        dev_alias(h_kcount), stream);
+      });
     if (hb > 0)
       timed(1, 0.0, 0.0, [&] {
         hist_launch(ds, d_chist, 0, 0, hb, 0, nullptr, d_chdr);
       });
+    const unsigned seq = host_fold ? 0u : ++stamp_seq;
     timed(3, 0.0, 0.0, [&] {
       launch_reduce(ds, d_cred, ni, stream, d_chdr);
-      launch_scan(ds, d_cscan, ni, sp, d_used, d_out, d_fold, d_flags, stream, d_chdr);
+      launch_scan(ds, d_cscan, ni, sp, d_used, d_out, d_fold, d_flags, stream, d_chdr,
+                  host_fold ? nullptr : dev_alias(h_stamp), seq, d_tickets + 1);
     });
-    if (host_fold) launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
-    sync_spin();
+    stats.launch_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - l0).count();
+    if (host_fold) {
+      launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
+      sync_spin();
+    } else {
+      wait_stamp(seq);
+    }
+    const auto b0 = std::chrono::steady_clock::now();
     if (h_kcount[2 * ni] != 0) throw std::runtime_error("partition count mismatch on the device");
     long long hist_rows = 0;
     int k = 0;
@@ -2113,6 +2141,7 @@ struct Learner {
     }
     stats.hist_rows += hist_rows;
     if (k > 0) stats.rounds += 1;  // rounds count search launches with work
+    stats.readback_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
     if (prof.tree_on) {
       // a histogram built in the partition adds its bytes to the partition's
       const double hr = (double)hist_rows;
@@ -2181,7 +2210,10 @@ struct Learner {
     set_best(0, nodes[0].best);
     std::vector<int> order;
     bool done = false;
+    using clk = std::chrono::steady_clock;
+    auto since_ms = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     while (!done) {
+      const auto r0 = clk::now();
       // replay commits in the reference's order while the needed children are known
       for (;;) {
         if (t.num_leaves >= P.num_leaves) {
@@ -2219,6 +2251,8 @@ struct Learner {
         set_best(right, nodes[(size_t)cr].best);
       }
       if (done) break;
+      stats.replay_ms += since_ms(r0);
+      const auto p0 = clk::now();
       // expand the best unexpanded frontier leaves, at most the remaining budget
       stage_wait();
       order.clear();
@@ -2340,6 +2374,7 @@ struct Learner {
         ++ni;
       }
       if (ni == 0) break;
+      stats.plan_ms += since_ms(p0);
       if (derive) {
         run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, part_hist, keep);
         continue;

@@ -86,6 +86,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
         learner.profile(profile, every_trees)
     stats = {"hist_rows": 0, "part_rows": 0, "splits": 0, "rounds": 0, "speculative_waste": 0,
              "committed_hist_rows": 0, "committed_part_rows": 0, "host_ms": 0.0, "wait_ms": 0.0}
+    phases = {"replay": 0.0, "plan": 0.0, "launch": 0.0, "readback": 0.0}
     tiling = None
     t0 = time.perf_counter()
     for i in range(steps):
@@ -93,6 +94,8 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
         s = learner.last_stats()
         for k in stats:
             stats[k] += s[k]
+        for k in phases:
+            phases[k] += s["host_phases_ms"][k]
         tiling = s["hist_tiling"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -107,6 +110,7 @@ def run_gpu(ds, y, params, steps, warmup, lr, profile, every_trees=1):
         out_trees.append(d)
     learner.close()
     stats["hist_tiling"] = tiling
+    stats["host_phases_ms"] = phases
     return elapsed, stats, kstats, out_trees
 
 
@@ -476,7 +480,8 @@ def main():
                          for k, v in kall.items()}
     total_ms = sum(v["ms"] for v in kall.values()) / bs
     result["kernel_ms_share"] = total_ms / (1000.0 * elapsed / args.steps)
-    result["rows_per_iter"] = {k: v / args.steps for k, v in stats.items() if k != "hist_tiling"}
+    result["rows_per_iter"] = {k: v / args.steps for k, v in stats.items() if k not in ("hist_tiling", "host_phases_ms")}
+    result["host_phases_ms_per_iter"] = {k: v / args.steps for k, v in stats["host_phases_ms"].items()}
     result["hist_tiling_autotuned"] = stats["hist_tiling"]
     # whole-iteration algorithmic bytes of the reference's work (committed splits only):
     # hist = sum R_b (F + P + I) with I = 0 for the root; part = sum Q_s * 9;

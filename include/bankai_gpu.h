@@ -406,6 +406,11 @@ typedef struct bk_train_stats {
   int32_t hist_ntiles;      /* group tiles of a histogram build                       */
   int32_t hist_sub_histograms; /* LDS copies of a tile's histogram                    */
   int32_t partition_histograms; /* 1: the partition built the children's histograms  */
+  /* host time of batched growth with 4-byte bins, by phase (within host_ms): */
+  double replay_ms;    /* the reference's commits replayed as children become known */
+  double plan_ms;      /* the frontier ranked and the round's items written            */
+  double launch_ms;    /* the round's upload and launches                              */
+  double readback_ms;  /* the children's counts and folded splits read back            */
 } bk_train_stats;
 int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
