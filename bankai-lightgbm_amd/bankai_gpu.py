@@ -138,7 +138,8 @@ class KernelStatsC(C.Structure):
     _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("algo_bytes", C.c_double), ("moved_bytes", C.c_double)]
 
 
-KERNEL_KINDS = ("root_hist", "child_hist", "partition", "search", "discretize", "renew", "score_gradients", "plan")
+KERNEL_KINDS = ("root_hist", "child_hist", "partition", "search", "discretize", "renew", "score_gradients", "plan",
+                "transfers")
 
 
 class DatasetParamsC(C.Structure):

@@ -510,6 +510,10 @@ void launch_copy(void* dst, const void* src, size_t n, void* stream);
 // Writes seq into *word (device address of pinned host memory) after the work
 // enqueued on `stream` before it.
 void launch_stamp(unsigned* word, unsigned seq, void* stream);
+// Keeps `stream` busy for `ticks` of the device's wall clock (timing runs only):
+// a timed launch submitted while it runs is stamped by its events when it is
+// already queued, not when the host starts submitting it.
+void launch_hold(unsigned long long ticks, void* stream);
 void launch_softmax_gradients(const double* score, const float* label, int n, int num_class, float* g, float* h,
                               void* stream);
 // LambdaRank tables and switches (objective/rank.rs:183-240).

@@ -3669,6 +3669,20 @@ __global__ void k_stamp(unsigned* word, unsigned seq) {
   if (threadIdx.x == 0) __hip_atomic_store(word, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// A bounded wait of `ticks` of the device's wall clock (one lane), queued ahead of
+// timed launches: launches submitted meanwhile wait in the queue behind it, so
+// their timing events are stamped when the kernel is already there.
+__global__ void k_hold(unsigned long long ticks) {
+  if (threadIdx.x != 0) return;
+  const unsigned long long t0 = wall_clock64();
+  while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+
+void launch_hold(unsigned long long ticks, void* stream) {
+  launch(k_hold, dim3(1), dim3(64), 0, (hipStream_t)stream, ticks);
+  BK_HIP(hipGetLastError());
+}
+
 void launch_stamp(unsigned* word, unsigned seq, void* stream) {
   launch(k_stamp, dim3(1), dim3(64), 0, (hipStream_t)stream, word, seq);
   BK_HIP(hipGetLastError());

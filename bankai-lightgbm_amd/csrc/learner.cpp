@@ -355,9 +355,10 @@ struct Prof {
     double algo, moved;
   };
   std::vector<Rec> pending;
-  double ms[8] = {0};
-  int64_t launches[8] = {0};
-  double algo[8] = {0}, moved[8] = {0};
+  static constexpr int kKinds = 9;
+  double ms[kKinds] = {0};
+  int64_t launches[kKinds] = {0};
+  double algo[kKinds] = {0}, moved[kKinds] = {0};
   hipEvent_t ev(size_t i) {
     while (pool.size() <= i) {
       hipEvent_t e;
@@ -396,7 +397,7 @@ struct Prof {
     used = 0;
   }
   void reset() {
-    for (int k = 0; k < 8; ++k) ms[k] = 0, launches[k] = 0, algo[k] = 0, moved[k] = 0;
+    for (int k = 0; k < kKinds; ++k) ms[k] = 0, launches[k] = 0, algo[k] = 0, moved[k] = 0;
   }
   // bytes of launches whose work the device decided (known after the tree)
   void add_bytes(int kind, double a, double m) {
@@ -741,6 +742,8 @@ struct Learner {
   char* flags_of(int node) { return node_flags.data() + (size_t)node * F; }
 
   static constexpr size_t kMiscBytes = 16 + sizeof(double) * 2 * (2048 + 2);
+  static constexpr double kHoldUs = 30.0;  // the hold ahead of a timed round
+  unsigned long long hold_ticks = 3000;    // kHoldUs in ticks of the device's wall clock
   int payload_bytes() const { return form == FORM_QUANT ? 2 : 8; }
   template <typename Fn>
   void timed(int kind, double algo, double moved, Fn&& launch) {
@@ -776,7 +779,7 @@ struct Learner {
   static constexpr double kSpinMs = 200.0;
   void sync_spin() {
     const unsigned seq = ++stamp_seq;
-    launch_stamp(dev_alias(h_stamp), seq, stream);
+    timed(8, 0.0, 0.0, [&] { launch_stamp(dev_alias(h_stamp), seq, stream); });
     wait_stamp(seq);
   }
   // The wait of sync_spin for a stamp that the stream's last kernel writes itself
@@ -1051,6 +1054,12 @@ struct Learner {
     }
     d_tickets = dmalloc<unsigned>(2);
     BK_HIPC(hipMemsetAsync(d_tickets, 0, sizeof(unsigned) * 2, stream));
+    {
+      int dev = 0, khz = 0;
+      BK_HIPC(hipGetDevice(&dev));
+      BK_HIPC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      hold_ticks = (unsigned long long)((khz > 0 ? khz : 100000) * (kHoldUs / 1000.0));
+    }
     d_max4 = dmalloc<unsigned>(4);
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
     d_sums3 = dmalloc<long long>(4);
@@ -1365,7 +1374,8 @@ struct Learner {
     if (!all)
       for (int f = 0; f < F; ++f)
         if (cols.bytree[(size_t)f]) h_group_used[feats[(size_t)f].group] = 1;
-    launch_copy(d_group_used, dev_alias(h_group_used), sizeof(int) * (size_t)G, stream);
+    timed(8, 0.0, 0.0,
+          [&] { launch_copy(d_group_used, dev_alias(h_group_used), sizeof(int) * (size_t)G, stream); });
     staged();
   }
 
@@ -1393,9 +1403,9 @@ struct Learner {
   // keeps the previous discretization (serial.rs:692-703).
   bool discretize(const float* g, const float* h) {
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
-    launch_max_abs(g, h, N, d_max4, d_sums3, stream);
+    timed(4, (double)N * 8.0, (double)N * 8.0, [&] { launch_max_abs(g, h, N, d_max4, d_sums3, stream); });
     unsigned mx[4];
-    launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream);
+    timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_misc), d_max4, sizeof mx, stream); });
     // the maxima start from zero for the next tree on every path
     BK_HIPC(hipMemsetAsync(d_max4, 0, sizeof(unsigned) * 4, stream));
     sync_spin();
@@ -1459,7 +1469,7 @@ struct Learner {
       if (ok || bag || resum_all) {
         long long sums[4] = {0, 0, 0, 0};
         if (!resum_all) {
-          launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream);
+          timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_misc), d_sums3, sizeof sums, stream); });
           sync_spin();
           std::memcpy(sums, h_misc, sizeof sums);
         }
@@ -1522,7 +1532,7 @@ struct Learner {
         dsn.int_g = last_int_g;
         dsn.int_h = last_int_h;
         *h_disc = dsn;
-        launch_copy(d_disc, dev_alias(h_disc), sizeof(DiscState), stream);
+        timed(8, 0.0, 0.0, [&] { launch_copy(d_disc, dev_alias(h_disc), sizeof(DiscState), stream); });
         staged();
       }
       isum = (long long)(((unsigned long long)last_int_g << 32) + (unsigned long long)last_int_h);
@@ -1531,8 +1541,10 @@ struct Learner {
       if (bag) gather_bag(d_rows, nrows);
       int nb = launch_sum_gh(g, h, d_rows, count, N, d_sum_partials, stream);
       if ((size_t)nb * 2 * sizeof(double) + 16 > kMiscBytes) throw std::runtime_error("set-up landing area too small");
-      launch_copy(dev_alias(h_misc) + 16, d_sum_partials, sizeof(double) * 2 * (size_t)nb, stream);
-      launch_copy(dev_alias(h_misc), h, sizeof(float), stream);
+      timed(8, 0.0, 0.0, [&] {
+        launch_copy(dev_alias(h_misc) + 16, d_sum_partials, sizeof(double) * 2 * (size_t)nb, stream);
+      });
+      timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_misc), h, sizeof(float), stream); });
       sync_spin();
       float hh0 = 0.0f;
       std::memcpy(&hh0, h_misc, sizeof(float));
@@ -1606,10 +1618,15 @@ struct Learner {
     long long total = 0;
   };
 
-  // starts a round's uploads: the previous round's transfers have completed
+  // starts a round's uploads: the previous round's transfers have completed.
+  // In a timed tree a hold is queued first: the round's upload and its first
+  // timed launch then wait in the queue behind it, so the launch's start event
+  // is not stamped before the host has submitted the kernel (the GPU is idle at
+  // a round's start, and submitting a timed launch takes the host ~10 us).
   void begin_uploads() {
     stage_wait();
     arena_cur = arena_sent = 0;
+    if (prof.mask != 0 && prof.tree_on) launch_hold(hold_ticks, stream);
   }
 
   // packs n elements into the arena; returns where they will be on the device
@@ -1628,7 +1645,7 @@ struct Learner {
   void send(bool sync_follows = false) {
     if (arena_cur > arena_sent) {
       const size_t from = arena_sent & ~(size_t)15;  // 16-byte aligned: the bytes before are sent already
-      launch_copy(d_arena + from, dev_alias(h_arena) + from, arena_cur - from, stream);
+      timed(8, 0.0, 0.0, [&] { launch_copy(d_arena + from, dev_alias(h_arena) + from, arena_cur - from, stream); });
       if (!sync_follows) staged();
     }
     arena_sent = arena_cur;
@@ -1748,7 +1765,9 @@ struct Learner {
                   copy_out ? nullptr : dev_alias(h_stamp), seq, d_tickets + 1);
     });
     if (copy_out) {
-      launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)nitems, stream);
+      timed(8, 0.0, 0.0, [&] {
+        launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)nitems, stream);
+      });
       sync_spin();
     } else {
       wait_stamp(seq);
@@ -2098,7 +2117,9 @@ struct Learner {
     });
     stats.launch_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - l0).count();
     if (host_fold) {
-      launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
+      timed(8, 0.0, 0.0, [&] {
+        launch_copy(dev_alias(h_out), d_out, sizeof(SplitOut) * (size_t)F * 2 * (size_t)ni, stream);
+      });
       sync_spin();
     } else {
       wait_stamp(seq);
@@ -2596,8 +2617,10 @@ struct Learner {
     for (;;) {
       launch_rounds(r, want, sp, hb);
       // the state and the node table are two allocations: two copies
-      launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream);
-      launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
+      timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream); });
+      timed(8, 0.0, 0.0, [&] {
+        launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
+      });
       sync_spin();
       if (hs->done) break;
       // every planned round expands at least one node, and a tree expands fewer
@@ -2704,10 +2727,12 @@ struct Learner {
         timed(5, 0.0, 0.0,
               [&] { launch_renew_sums(N, d_leaf_of, d_order, P.num_leaves, g, h, flags, d_renew, stream); });
       }
-      launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream);
-      launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
-      launch_copy(dev_alias(h_disc), d_disc, sizeof(DiscState), stream);
-      launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
+      timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream); });
+      timed(8, 0.0, 0.0, [&] {
+        launch_copy(dev_alias(h_plan) + sizeof(PlanState), d_pnodes, back - sizeof(PlanState), stream);
+      });
+      timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_disc), d_disc, sizeof(DiscState), stream); });
+      timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream); });
       sync_spin();
       if (hs->done) break;
       if (r > 2 * P.num_leaves + 8) throw std::runtime_error("device planner did not finish the tree");
@@ -2769,7 +2794,8 @@ struct Learner {
     if (!rules && ni > 0) {
       stage_wait();
       fill_nodes(h_nodes);
-      launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream);
+      timed(8, 0.0, 0.0,
+            [&] { launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream); });
       staged();
       rules = d_node_rules;
     }
@@ -2832,7 +2858,7 @@ struct Learner {
       timed(5, 0.0, 0.0, [&] { launch_renew_sums(N, d_leaf_of, u_order, nl, g, h, flags, d_renew, stream); });
     }
     // one synchronization for the error flag and the renewal sums
-    launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream);
+    timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream); });
     sync_spin();
     int err = 0;
     std::memcpy(&err, h_tail, sizeof(int));
@@ -2903,7 +2929,9 @@ struct Learner {
     ensure_leaf_of();
     stage_wait();
     for (int i = 0; i < t->num_leaves; ++i) h_vals[i] = t->leaf_value[i];
-    launch_copy(d_leaf_vals, dev_alias(h_vals), sizeof(double) * (size_t)t->num_leaves, stream);
+    timed(8, 0.0, 0.0, [&] {
+      launch_copy(d_leaf_vals, dev_alias(h_vals), sizeof(double) * (size_t)t->num_leaves, stream);
+    });
     staged();
     // no synchronization: the next use of h_vals waits for this copy (stage_wait),
     // and the score is read by later work on the same stream
@@ -2986,7 +3014,8 @@ struct Learner {
     leaf_of_valid = false;  // the node buffers are shared with the last tree's map
     stage_wait();
     std::memcpy(h_nodes, rules.data(), sizeof(NodeRule) * (size_t)ni);
-    launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream);
+    timed(8, 0.0, 0.0,
+          [&] { launch_copy(d_node_rules, dev_alias(h_nodes), sizeof(NodeRule) * (size_t)ni, stream); });
     launch_leaf_of_tree(ds, d_node_rules, d_nodes, ni, nullptr, d_replay, stream);
     std::vector<int32_t> lo((size_t)N);
     BK_HIPC(hipMemcpyAsync(lo.data(), d_replay, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToHost, stream));
@@ -3421,7 +3450,7 @@ int bk_learner_profile_trees(bk_learner* l, int every) {
 
 int bk_learner_kernel_stats(bk_learner* l, int kind, bk_kernel_stats* out) {
   return bk::guard([&] {
-    if (kind < 0 || kind >= 8) throw std::invalid_argument("kernel kind out of range");
+    if (kind < 0 || kind >= bk::Prof::kKinds) throw std::invalid_argument("kernel kind out of range");
     if (!l->L.prof.pending.empty()) {
       l->L.sync();
       l->L.prof.resolve();

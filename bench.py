@@ -349,10 +349,9 @@ def main():
     ap.add_argument("--sequential-steps", type=int, default=3)
     ap.add_argument("--device-tree-steps", type=int, default=10,
                     help="iterations timed with the trees planned on the device (0 skips it)")
-    ap.add_argument("--breakdown-steps", type=int, default=5)
-    ap.add_argument("--timed-every", type=int, default=4,
-                    help="the measured run times the dominant kernel's launches of every k-th tree (each timed "
-                         "launch costs ~25 us of host time)")
+    ap.add_argument("--breakdown-steps", type=int, default=0,
+                    help="iterations of the per-kernel breakdown run (0: the measured run's iterations, so both "
+                         "runs grow the same trees)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--holdout", type=int, default=1_050_000)
@@ -406,13 +405,16 @@ def main():
     qparams = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
                                stochastic_rounding=1, quant_train_renew_leaf=1, seed=seed, num_threads=16,
                                form=bg.FORM_QUANTIZED, batched=1)
-    # per-kernel breakdown: every launch timed (the events cost time, so this run
-    # is not the measured one); it names the dominant kernel
-    _, _, kall, _ = run_gpu(ds, y, qparams, args.breakdown_steps, 1, lr, profile=True)
+    # per-kernel breakdown: every launch of every kind timed with HIP events on the
+    # learner's stream, over the measured run's iterations (the same trees: the
+    # runs are deterministic); it names the dominant kernel and gives every
+    # roofline below. The events cost host time, so this run is not the measured one.
+    if args.breakdown_steps <= 0:
+        args.breakdown_steps = args.steps
+    _, _, kall, _ = run_gpu(ds, y, qparams, args.breakdown_steps, args.warmup, lr, profile=True)
     dom = max(kall, key=lambda k: kall[k]["ms"])
-    # the measured run: only the dominant kernel's launches carry HIP events
-    elapsed, stats, kstats, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=[dom],
-                                             every_trees=args.timed_every)
+    # the measured run: no events, the headline's whole-job throughput
+    elapsed, stats, _, qtrees = run_gpu(ds, y, qparams, args.steps, args.warmup, lr, profile=False)
     its = args.steps / elapsed
     log(f"quantized: {its:.2f} it/s ({1000 * elapsed / args.steps:.1f} ms/iter), dominant kernel {dom}")
 
@@ -435,8 +437,8 @@ def main():
                    "num_leaves": 255, "max_bin": 255, "form": "quantized", "growth": "batched"},
     }
     # roofline of the dominant kernel: algorithmic bytes per launch over the
-    # average launch duration (HIP events on the learner's stream, measured run)
-    ks = kstats[dom]
+    # average launch duration (HIP events on the learner's stream, breakdown run)
+    ks = kall[dom]
     groups, payload = ds.num_features, 2
     stride = (groups + 1) // 2 * 2 + payload  # row record: group bins, the payload, padded to 16 bytes
     stride = (stride + 15) // 16 * 16
@@ -454,7 +456,8 @@ def main():
         result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                               "launches": ks["launches"], "avg_launch_us": 1000.0 * ks["ms"] / launches}
-    result["roofline"]["timed_trees"] = f"every {args.timed_every}th tree"
+    result["roofline"]["timed_trees"] = ("every tree of the breakdown run: the measured run's iterations and trees, "
+                                         "timed in a run of their own")
     result["roofline"]["moved_bytes_per_launch"] = ks["moved_bytes"] / max(ks["launches"], 1)
     result["roofline"]["moved_GBs"] = ks["moved_bytes"] / (ks["ms"] * 1e-3) / 1e9 if ks["ms"] > 0 else 0.0
     # the histogram kernel's own rooflines (root and child builds) from the

@@ -416,8 +416,9 @@ int bk_learner_last_stats(bk_learner* learner, bk_train_stats* out);
 
 /* Per-kernel timing with HIP events recorded on the learner's stream around
  * every launch (off by default). Kinds: 0 root histogram, 1 child histograms,
- * 2 partition, 3 split search, 4 discretizer, 5 leaf renewal, 6 score update
- * and gradients, 7 the device round planner. algo_bytes follows the per-row byte model of the histogram
+ * 2 partition, 3 split search, 4 discretizer (with the gradient maxima), 5 leaf
+ * renewal, 6 score update, tree walk and gradients, 7 the device round planner,
+ * 8 transfers (the learner's small copies and stamps on its stream). algo_bytes follows the per-row byte model of the histogram
  * (rows x (features + payload), plus a 4-byte row index for a child) and of the
  * partition (rows x 9: one bin, row id read, row id write), the reference's
  * index-based model; moved_bytes counts what the kernel actually reads and
