@@ -482,6 +482,13 @@ void launch_finalize(const PlanState* st, const PlanNode* nodes, const FeatDev* 
 // then the sequential chain for flagged leaves (d_flags == nullptr runs the
 // chain for every leaf).
 int renew_parts(int n);
+// The walk (staged records) with the leaf renewal's per-block partials
+// (launch_renew_par's part_sums and part_meta, renew_parts(num_data) blocks) in
+// the same pass over the rows. Returns false, launching nothing, when the
+// records, nodes and accumulators do not fit the staged layout.
+bool launch_leaf_of_tree_renew(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
+                               const uint32_t* bag_bits, int32_t* leaf_of, const float* g, const float* h,
+                               int num_leaves, double* part_sums, int* part_meta, void* stream);
 size_t renew_par_lds(int num_leaves);
 void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float* h, int num_leaves,
                       double* part_sums, int* part_meta, void* stream);

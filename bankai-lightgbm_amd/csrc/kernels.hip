@@ -585,6 +585,10 @@ __host__ __device__ inline size_t partition_lds(int R, int stride) {
   (void)stride;
   return ((size_t)R * 3 + 15) & ~(size_t)15;
 }
+// The plain partition's LDS: its tables, then the tile's payloads.
+__host__ __device__ inline size_t partition_pay_lds(int R, int stride, int pay_bytes) {
+  return partition_lds(R, stride) + (((size_t)R * pay_bytes + 15) & ~(size_t)15);
+}
 __host__ __device__ inline size_t partition_hist_lds(int R, int stride, int total_bins) {
   return partition_lds(R, stride) + (((size_t)R * 2 + 15) & ~(size_t)15) + (size_t)total_bins * 4;
 }
@@ -627,8 +631,10 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const int rpt = (R + kPartThreads - 1) / kPartThreads;
   uint16_t* dstp = (uint16_t*)smem;  // R <= 8192
   uint8_t* lflag = (uint8_t*)(dstp + R);
-  int16_t* pay_lds = (int16_t*)(smem + partition_lds(R, stride));
-  unsigned* hist = (unsigned*)((unsigned char*)pay_lds + (((size_t)R * 2 + 15) & ~(size_t)15));
+  // the tile's payloads: from gh0 for a tile of buffer 0 (loaded beside the
+  // records, patched into them on the way out), and the histogram side's
+  PT* payt = (PT*)(smem + partition_lds(R, stride));
+  unsigned* hist = (unsigned*)((unsigned char*)payt + (((size_t)R * 2 + 15) & ~(size_t)15));  // HIST: PT is 2 bytes
   const int t = blockIdx.x;
   // the cursors of the next launch start at zero
   unsigned* other = cursors + (size_t)((epoch + 1u) & 1u) * max_items * 2;
@@ -681,16 +687,17 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) v[k] = load_stream(src + 16 * (size_t)q);
   }
-  if (HIST && side >= 0 && seg.buf == 0)
-    for (int r = threadIdx.x; r < rows; r += kPartThreads) pay_lds[r] = ((const int16_t*)s.gh0)[t0 + r];
+  if (seg.buf == 0)
+    for (int r = threadIdx.x; r < rows; r += kPartThreads) payt[r] = ((const PT*)s.gh0)[t0 + r];
 #pragma unroll
   for (int k = 0; k < kPartChunks; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) {
       const int r = q / nc, c = q - r * nc;
       if (c == cs) lflag[r] = route_left(rule, (lane_word(v[k], bs >> 2) >> (8 * (bs & 3))) & 0xffu) ? 1 : 0;
-      if (HIST && side >= 0 && seg.buf != 0 && c == pay_chunk)
-        pay_lds[r] = (int16_t)((lane_word(v[k], pay_in >> 2) >> (8 * (pay_in & 3))) & 0xffffu);
+      if constexpr (HIST)
+        if (side >= 0 && seg.buf != 0 && c == pay_chunk)
+          payt[r] = (PT)((lane_word(v[k], pay_in >> 2) >> (8 * (pay_in & 3))) & 0xffffu);
     }
   }
   __syncthreads();
@@ -743,12 +750,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
       const int r = q / nc, c = q - r * nc;
       const int l = (int)dstp[r];
       const long long d = l < tl ? lpos + l : rpos + (l - tl);
-      if (seg.buf == 0 && c == pay_chunk) {
-        PT pv;
-        if constexpr (HIST) pv = side >= 0 ? (PT)pay_lds[r] : ((const PT*)s.gh0)[t0 + r];
-        else pv = ((const PT*)s.gh0)[t0 + r];
-        patch_payload(v[k], pay_in, pv);
-      }
+      if (seg.buf == 0 && c == pay_chunk) patch_payload(v[k], pay_in, payt[r]);
       store_stream(dst + (size_t)d * stride + 16 * c, v[k]);
     }
   }
@@ -757,7 +759,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     return;
   }
   // the side's rows into the LDS histogram (the stores above are in flight)
-  {
+  if constexpr (HIST) {
     const int my_c = (int)threadIdx.x % nc;
     int off[16];
 #pragma unroll
@@ -772,7 +774,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
       if (q < total) {
         const int r = q / nc;
         if (lflag[r] == want) {
-          const unsigned w = qpack32(pay_lds[r]);
+          const unsigned w = qpack32(payt[r]);
 #pragma unroll
           for (int j = 0; j < 16; ++j)
             if (off[j] >= 0) atomicAdd(&hist[off[j] + (int)((lane_word(v[k], j >> 2) >> (8 * (j & 3))) & 0xffu)], w);
@@ -2322,19 +2324,20 @@ __device__ void derive_children(const PartItem* __restrict__ pitems, int ni, con
   }
   if (j < ni && srch) {
     const ChildSpec& cs = specs[j];
-    // the built side: the smaller child, or the side the partition built
+    // the built side: the smaller child, or the side the partition built (the
+    // children are picked by selects: a local array indexed at run time would
+    // live in per-lane scratch memory)
     const int sm = cp.part_hist ? pitems[j].hist_side : (lc < rc ? 0 : 1);
-    LeafArgs kid[2] = {cs.kid[0], cs.kid[1]};
-    kid[0].num_data = lc;
-    kid[1].num_data = rc;
-    kid[sm].slot = cs.new_slot;
-    kid[1 - sm].slot = cs.parent_slot;
-    kid[0].part_first = kid[1].part_first = 0;
-    kid[0].nblk = kid[1].nblk = 0;
-    const Segment kseg[2] = {Segment{dst, seg.begin, lc}, Segment{dst, seg.begin + lc, rc}};
     ScanItem si{};
-    si.small = kid[sm];
-    si.large = kid[1 - sm];
+    si.small = cs.kid[sm];
+    si.large = cs.kid[1 - sm];
+    si.small.num_data = sm ? rc : lc;
+    si.large.num_data = sm ? lc : rc;
+    si.small.slot = cs.new_slot;
+    si.large.slot = cs.parent_slot;
+    si.small.part_first = si.large.part_first = 0;
+    si.small.nblk = si.large.nblk = 0;
+    const Segment sm_seg = sm ? Segment{dst, seg.begin + lc, rc} : Segment{dst, seg.begin, lc};
     if (cp.part_hist) {  // one partial per partition tile of the split
       si.small.part_first = pitems[j].first_task;
       si.small.nblk = pitems[j].ntasks;
@@ -2342,7 +2345,7 @@ __device__ void derive_children(const PartItem* __restrict__ pitems, int ni, con
       const int b0 = row0 / rpb, b1 = (row0 + hrows - 1) / rpb;
       si.small.part_first = b0 + hidx;  // partial of (block b, item i) is slot b + i
       si.small.nblk = b1 - b0 + 1;
-      hitems[hidx] = HistItem{kseg[sm], row0, 0};
+      hitems[hidx] = HistItem{sm_seg, row0, 0};
     }
     si.has_large = 1;
     si.use_subtract = 1;
@@ -2593,9 +2596,16 @@ __device__ __forceinline__ bool in_bag(const uint32_t* bag_bits, long long r) {
 // Rows of a grid stride walk the tree; two rows per step of each lane, so two
 // independent chains of node loads are in flight. A row outside the tree's bag
 // gets ~leaf: the leaf the tree sends it to, marked as not trained on.
-template <typename NodePtr>
+// What a walk does with each row's leaf besides writing it: nothing, or the
+// leaf renewal's per-block partial sums (RenewAcc).
+struct NoEmit {
+  __device__ __forceinline__ void operator()(long long, int) const {}
+};
+
+template <typename NodePtr, typename Emit = NoEmit>
 __device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num_nodes,
-                                          const uint32_t* __restrict__ bag_bits, int32_t* __restrict__ leaf_of) {
+                                          const uint32_t* __restrict__ bag_bits, int32_t* __restrict__ leaf_of,
+                                          const Emit& emit = Emit()) {
   const int stride = s.stride, nc = stride >> 4;
   const long long n = s.num_data, step = 2ll * gridDim.x * blockDim.x;
   for (long long r0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; r0 < n; r0 += step) {
@@ -2635,8 +2645,13 @@ __device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num
       }
     }
     const int l0 = n0 < 0 ? ~n0 : 0, l1 = n1 < 0 ? ~n1 : 0;
-    leaf_of[r0] = in_bag(bag_bits, r0) ? l0 : ~l0;
-    if (has1) leaf_of[r1] = in_bag(bag_bits, r1) ? l1 : ~l1;
+    const int v0 = in_bag(bag_bits, r0) ? l0 : ~l0, v1 = in_bag(bag_bits, r1) ? l1 : ~l1;
+    leaf_of[r0] = v0;
+    emit(r0, v0);
+    if (has1) {
+      leaf_of[r1] = v1;
+      emit(r1, v1);
+    }
   }
 }
 
@@ -2645,9 +2660,10 @@ __device__ __forceinline__ void walk_rows(const DevState& s, NodePtr nd, int num
 // select chain over the record's registers. A lane's records start every
 // pstride = stride + 4 bytes, an odd number of words, so the 64 lanes of a wave
 // read from 64 different banks. Each lane reads only what it wrote itself.
+template <typename Emit = NoEmit>
 __device__ __forceinline__ void walk_rows_staged(const DevState& s, const NodeDev* nd, int num_nodes,
                                                  unsigned char* recs, const uint32_t* __restrict__ bag_bits,
-                                                 int32_t* __restrict__ leaf_of) {
+                                                 int32_t* __restrict__ leaf_of, const Emit& emit = Emit()) {
   const int stride = s.stride, nw = stride >> 2, pstride = stride + 4;
   unsigned char* my0 = recs + (size_t)threadIdx.x * pstride;
   unsigned char* my1 = recs + (size_t)(blockDim.x + threadIdx.x) * pstride;
@@ -2673,8 +2689,13 @@ __device__ __forceinline__ void walk_rows_staged(const DevState& s, const NodeDe
       }
     }
     const int l0 = n0 < 0 ? ~n0 : 0, l1 = n1 < 0 ? ~n1 : 0;
-    leaf_of[r0] = in_bag(bag_bits, r0) ? l0 : ~l0;
-    if (has1) leaf_of[r1] = in_bag(bag_bits, r1) ? l1 : ~l1;
+    const int v0 = in_bag(bag_bits, r0) ? l0 : ~l0, v1 = in_bag(bag_bits, r1) ? l1 : ~l1;
+    leaf_of[r0] = v0;
+    emit(r0, v0);
+    if (has1) {
+      leaf_of[r1] = v1;
+      emit(r1, v1);
+    }
   }
 }
 
@@ -2705,6 +2726,79 @@ __global__ void __launch_bounds__(256) k_leaf_of_tree(DevState s, const NodeDev*
   }
 }
 
+// The leaf renewal's per-block partials of one row (k_renew_par's accumulation):
+// LDS per leaf: sum g, sum h (f64), max|g|, max|h| (float bits), min lsb
+// exponents. A row outside the bag (a negative leaf) adds nothing. The sums are
+// used only where the certificate holds: then every partial sum is exact and the
+// order of the atomics does not matter.
+__device__ __forceinline__ void renew_add(double* acc, int* meta, int l, float x, float y) {
+  atomicAdd(&acc[2 * l], (double)x);
+  atomicAdd(&acc[2 * l + 1], (double)y);
+  // NaN and Inf compare above every finite magnitude as unsigned bits and
+  // also force INT_MIN exponents, so the certificate fails for them
+  const int ax = (int)(__float_as_uint(x) & 0x7fffffffu), ay = (int)(__float_as_uint(y) & 0x7fffffffu);
+  const int ex = lsb_exponent(x), ey = lsb_exponent(y);
+  if (ax > meta[4 * l]) atomicMax(&meta[4 * l], ax);
+  if (ay > meta[4 * l + 1]) atomicMax(&meta[4 * l + 1], ay);
+  if (ex < meta[4 * l + 2]) atomicMin(&meta[4 * l + 2], ex);
+  if (ey < meta[4 * l + 3]) atomicMin(&meta[4 * l + 3], ey);
+}
+
+__device__ __forceinline__ void renew_zero(double* acc, int* meta, int num_leaves) {
+  for (int i = threadIdx.x; i < num_leaves; i += blockDim.x) {
+    acc[2 * i] = 0.0;
+    acc[2 * i + 1] = 0.0;
+    meta[4 * i] = 0;
+    meta[4 * i + 1] = 0;
+    meta[4 * i + 2] = INT_MAX;
+    meta[4 * i + 3] = INT_MAX;
+  }
+}
+
+__device__ __forceinline__ void renew_store(const double* acc, const int* meta, int num_leaves,
+                                            double* __restrict__ part_sums, int* __restrict__ part_meta) {
+  double* ps = part_sums + (size_t)blockIdx.x * num_leaves * 2;
+  int* pm = part_meta + (size_t)blockIdx.x * num_leaves * 4;
+  for (int i = threadIdx.x; i < 2 * num_leaves; i += blockDim.x) ps[i] = acc[i];
+  for (int i = threadIdx.x; i < 4 * num_leaves; i += blockDim.x) pm[i] = meta[i];
+}
+
+struct RenewAcc {
+  double* acc;
+  int* meta;
+  const float* g;
+  const float* h;
+  int num_leaves;
+  __device__ __forceinline__ void operator()(long long r, int l) const {
+    if ((unsigned)l < (unsigned)num_leaves) renew_add(acc, meta, l, g[r], h[r]);
+  }
+};
+
+// The row-to-leaf walk with the leaf renewal's partials in the same pass (the
+// rows' leaves are at hand, so leaf_of is not read back): the nodes and the
+// staged records in LDS as k_leaf_of_tree's staged form, then the renewal's
+// per-leaf accumulators; one partial per block (the grid is renew_parts(n)).
+__global__ void __launch_bounds__(256) k_leaf_of_tree_renew(DevState s, const NodeDev* __restrict__ nodes,
+                                                            int num_nodes, const uint32_t* __restrict__ bag_bits,
+                                                            int32_t* __restrict__ leaf_of,
+                                                            const float* __restrict__ g,
+                                                            const float* __restrict__ h, int num_leaves,
+                                                            double* __restrict__ part_sums,
+                                                            int* __restrict__ part_meta) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int words = num_nodes * (int)(sizeof(NodeDev) / 4);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) ((uint32_t*)smem)[i] = ((const uint32_t*)nodes)[i];
+  unsigned char* recs = smem + (size_t)words * 4;
+  double* acc = (double*)(recs + ((walk_stage_bytes(s.stride) + 15) & ~(size_t)15));
+  int* meta = (int*)(acc + 2 * (size_t)num_leaves);
+  renew_zero(acc, meta, num_leaves);
+  __syncthreads();
+  walk_rows_staged(s, (const NodeDev*)smem, num_nodes, recs, bag_bits, leaf_of,
+                   RenewAcc{acc, meta, g, h, num_leaves});
+  __syncthreads();
+  renew_store(acc, meta, num_leaves, part_sums, part_meta);
+}
+
 // Per-block partial sums per leaf, in row order over a grid stride. LDS holds
 // per leaf: sum g, sum h (f64), max|g|, max|h| (float bits), min lsb exponents.
 __global__ void __launch_bounds__(256) k_renew_par(int n, const int32_t* __restrict__ leaf_of,
@@ -2714,35 +2808,15 @@ __global__ void __launch_bounds__(256) k_renew_par(int n, const int32_t* __restr
   extern __shared__ unsigned char smem[];
   double* acc = (double*)smem;                     // [leaf][2]
   int* meta = (int*)(acc + 2 * (size_t)num_leaves);  // [leaf][4]: maxabs g, maxabs h, lsb g, lsb h
-  for (int i = threadIdx.x; i < num_leaves; i += blockDim.x) {
-    acc[2 * i] = 0.0;
-    acc[2 * i + 1] = 0.0;
-    meta[4 * i] = 0;
-    meta[4 * i + 1] = 0;
-    meta[4 * i + 2] = INT_MAX;
-    meta[4 * i + 3] = INT_MAX;
-  }
+  renew_zero(acc, meta, num_leaves);
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int l = leaf_of[i];
     if ((unsigned)l >= (unsigned)num_leaves) continue;  // a row outside the bag (~leaf)
-    const float x = g[i], y = h[i];
-    atomicAdd(&acc[2 * l], (double)x);
-    atomicAdd(&acc[2 * l + 1], (double)y);
-    // NaN and Inf compare above every finite magnitude as unsigned bits and
-    // also force INT_MIN exponents, so the certificate fails for them
-    const int ax = (int)(__float_as_uint(x) & 0x7fffffffu), ay = (int)(__float_as_uint(y) & 0x7fffffffu);
-    const int ex = lsb_exponent(x), ey = lsb_exponent(y);
-    if (ax > meta[4 * l]) atomicMax(&meta[4 * l], ax);
-    if (ay > meta[4 * l + 1]) atomicMax(&meta[4 * l + 1], ay);
-    if (ex < meta[4 * l + 2]) atomicMin(&meta[4 * l + 2], ex);
-    if (ey < meta[4 * l + 3]) atomicMin(&meta[4 * l + 3], ey);
+    renew_add(acc, meta, l, g[i], h[i]);
   }
   __syncthreads();
-  double* ps = part_sums + (size_t)blockIdx.x * num_leaves * 2;
-  int* pm = part_meta + (size_t)blockIdx.x * num_leaves * 4;
-  for (int i = threadIdx.x; i < 2 * num_leaves; i += blockDim.x) ps[i] = acc[i];
-  for (int i = threadIdx.x; i < 4 * num_leaves; i += blockDim.x) pm[i] = meta[i];
+  renew_store(acc, meta, num_leaves, part_sums, part_meta);
 }
 
 // One block per leaf: folds the block partials, checks the certificate, writes
@@ -3469,7 +3543,7 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
   }
   const int R = partition_tile_rows(s.stride);
   if (R < 1) throw std::runtime_error("row record too large for the partition tile");
-  const size_t lds = partition_lds(R, s.stride);
+  const size_t lds = partition_pay_lds(R, s.stride, s.form == FORM_QUANT ? 2 : 8);
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("partition tile exceeds the LDS budget");
   hipStream_t st = (hipStream_t)stream;
   const int* no_groups = nullptr;
@@ -3723,6 +3797,23 @@ void launch_leaf_of_tree(const DevState& s, const NodeRule* d_rules, NodeDev* d_
 }
 
 int renew_parts(int n) { return grid_for(n, 256 * 16, 1024); }
+
+bool launch_leaf_of_tree_renew(const DevState& s, const NodeRule* d_rules, NodeDev* d_nodes, int num_nodes,
+                               const uint32_t* bag_bits, int32_t* leaf_of, const float* g, const float* h,
+                               int num_leaves, double* part_sums, int* part_meta, void* stream) {
+  if (num_nodes <= 0 || s.stride > 64) return false;
+  const size_t lds = (size_t)num_nodes * sizeof(NodeDev) + ((walk_stage_bytes(s.stride) + 15) & ~(size_t)15) +
+                     renew_par_lds(num_leaves);
+  if (lds > (size_t)kLdsBytes) return false;
+  launch(k_node_bitmaps, dim3((unsigned)num_nodes), dim3(256), 0, (hipStream_t)stream, d_rules, d_nodes,
+         (const int*)nullptr);
+  BK_HIP(hipGetLastError());
+  allow_lds(k_leaf_of_tree_renew);
+  launch(k_leaf_of_tree_renew, dim3(renew_parts(s.num_data)), dim3(256), lds, (hipStream_t)stream, s, d_nodes,
+         num_nodes, bag_bits, leaf_of, g, h, num_leaves, part_sums, part_meta);
+  BK_HIP(hipGetLastError());
+  return true;
+}
 
 size_t renew_par_lds(int num_leaves) { return (size_t)num_leaves * (2 * sizeof(double) + 4 * sizeof(int)); }
 

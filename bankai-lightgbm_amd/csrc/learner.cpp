@@ -2846,10 +2846,23 @@ struct Learner {
       send(true);
       const int* flags = nullptr;
       if (d_renew_part_sums && nl > 1) {
-        // certified parallel sums; the chain runs only for leaves the certificate rejects
-        ensure_leaf_of(u_nodes);
-        timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
-          launch_renew_par(N, d_leaf_of, g, h, nl, d_renew_part_sums, d_renew_part_meta, stream);
+        // certified parallel sums; the chain runs only for leaves the certificate rejects.
+        // The walk adds the partials as it finds each row's leaf (one pass over the rows).
+        bool fused = false;
+        if (!leaf_of_valid) {
+          timed(6, (double)N * (stride + 12.0), (double)N * (stride + 12.0), [&] {
+            fused = launch_leaf_of_tree_renew(ds, u_nodes, d_nodes, nl - 1, last_bagged ? d_bag_bits : nullptr,
+                                              d_leaf_of, g, h, nl, d_renew_part_sums, d_renew_part_meta, stream);
+          });
+          leaf_of_valid = fused;
+        }
+        if (!fused) {
+          ensure_leaf_of(u_nodes);
+          timed(5, (double)N * 12.0, (double)N * 12.0, [&] {
+            launch_renew_par(N, d_leaf_of, g, h, nl, d_renew_part_sums, d_renew_part_meta, stream);
+          });
+        }
+        timed(5, 0.0, 0.0, [&] {
           launch_renew_fix(u_segs, nl, renew_nparts, d_renew_part_sums, d_renew_part_meta, d_renew, d_renew_flags,
                            stream);
         });
