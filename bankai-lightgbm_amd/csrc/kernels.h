@@ -419,8 +419,10 @@ void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cur
 // Sums the partial histograms of every item into its pool slot; the search then
 // reads the raw histogram of the smaller leaf (and of a rebuilt larger leaf)
 // from its slot.
+// With hdr the grid covers min(nitems, hdr_grid_cap) items at a time (the
+// device decides how many of the nitems there are).
 void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream,
-                   const RoundHdr* hdr = nullptr);
+                   const RoundHdr* hdr = nullptr, int hdr_grid_cap = 64);
 // Split search of every (feature, item), then the fold of the best candidate of
 // every (item, side) into d_best[2 * item + side] and of every feature's
 // splittable flag into d_flags[(2 * item + side) * F + f].
@@ -429,7 +431,7 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
 // zeroed word the fold's waves count on (reset by the last).
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
                  SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr = nullptr,
-                 unsigned* stamp = nullptr, unsigned seq = 0, unsigned* ticket = nullptr);
+                 unsigned* stamp = nullptr, unsigned seq = 0, unsigned* ticket = nullptr, int hdr_grid_cap = 64);
 // One round of planning: absorbs the last round's folded splits into their
 // nodes, replays the reference's best-first commits while the best leaf is
 // expanded, picks the next frontier (the best unexpanded leaves with positive

@@ -3130,7 +3130,7 @@ __global__ void k_sum_gh(const float* __restrict__ g, const float* __restrict__ 
 // partial (32 consecutive bins of one partial are one coalesced read); the 8
 // sums are then added in lane order, so float forms reduce in a fixed order.
 // ---------------------------------------------------------------------------
-constexpr int kReduceMaxItemsY = 64;  // grid rows of a reduce whose item count the device decides
+constexpr int kReduceMaxItemsY = 1024;  // grid rows of a reduce whose item count the device decides
 
 template <int FORM>
 __global__ void __launch_bounds__(256) k_reduce(DevState s, const ReduceItem* __restrict__ items,
@@ -3597,10 +3597,12 @@ void launch_to_records(const DevState& s, const uint8_t* cols, size_t col_stride
   BK_HIP(hipGetLastError());
 }
 
-void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream, const RoundHdr* hdr) {
+void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, void* stream, const RoundHdr* hdr,
+                   int hdr_grid_cap) {
   if (nitems <= 0) return;
   // a planned round's kernel loops over its items: a capped grid holds them
-  dim3 grid((unsigned)((s.total_bins + 31) / 32), (unsigned)(hdr ? std::min(nitems, kReduceMaxItemsY) : nitems));
+  dim3 grid((unsigned)((s.total_bins + 31) / 32),
+            (unsigned)(hdr ? std::min(nitems, std::min(hdr_grid_cap, kReduceMaxItemsY)) : nitems));
   hipStream_t st = (hipStream_t)stream;
   switch (s.form) {
     case FORM_QUANT:
@@ -3620,11 +3622,11 @@ void launch_reduce(const DevState& s, const ReduceItem* d_items, int nitems, voi
 
 void launch_scan(const DevState& s, const ScanItem* d_items, int nitems, const ScanParams& sp, const int* d_used,
                  SplitOut* d_out, FoldOut* d_best, uint8_t* d_flags, void* stream, const RoundHdr* hdr,
-                 unsigned* stamp, unsigned seq, unsigned* ticket) {
+                 unsigned* stamp, unsigned seq, unsigned* ticket, int hdr_grid_cap) {
   if (stamp && !ticket) throw std::runtime_error("a stamped fold needs its ticket word");
   if (nitems <= 0) return;
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid((unsigned)s.num_features, (unsigned)(hdr ? std::min(nitems, 64) : nitems));
+  dim3 grid((unsigned)s.num_features, (unsigned)(hdr ? std::min(nitems, hdr_grid_cap) : nitems));
   switch (s.form) {
     case FORM_QUANT:
       launch(k_scan<FORM_QUANT>, grid, dim3(kScanThreads), 0, st, s, d_items, sp, d_used, d_out, hdr);

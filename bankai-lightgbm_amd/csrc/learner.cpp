@@ -2111,9 +2111,10 @@ struct Learner {
       });
     const unsigned seq = host_fold ? 0u : ++stamp_seq;
     timed(3, 0.0, 0.0, [&] {
-      launch_reduce(ds, d_cred, ni, stream, d_chdr);
+      // ni splits: at most ni search items, one grid row each
+      launch_reduce(ds, d_cred, ni, stream, d_chdr, ni);
       launch_scan(ds, d_cscan, ni, sp, d_used, d_out, d_fold, d_flags, stream, d_chdr,
-                  host_fold ? nullptr : dev_alias(h_stamp), seq, d_tickets + 1);
+                  host_fold ? nullptr : dev_alias(h_stamp), seq, d_tickets + 1, ni);
     });
     stats.launch_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - l0).count();
     if (host_fold) {
