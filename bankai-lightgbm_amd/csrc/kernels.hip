@@ -392,6 +392,20 @@ __device__ __forceinline__ int find_item(const Item* __restrict__ items, int nit
 
 // Adds one 16-byte chunk of a row with the chunk's group bin offsets in
 // registers (off[j] < 0: group outside the tile, unused, or padding).
+// The quantized payload of the row whose chunk v this lane holds, when the row's
+// nc chunks sit in consecutive lanes of one wave and one of them (pay_lane in the
+// row's lanes) holds the payload at byte pay_in: each lane takes the payload
+// word out of its own chunk and reads the payload lane's. Every lane of the wave
+// calls it.
+template <int FORM>
+__device__ __forceinline__ typename Pay<FORM>::T row_payload(const uint4& v, int pay_in, int nc, int pay_lane) {
+  const int lane = (int)threadIdx.x & 63;
+  const int src = lane - lane % nc + pay_lane;
+  static_assert(FORM == FORM_QUANT || FORM == kHistQ32, "2-byte payloads");
+  const int w = (int)(int16_t)((lane_word(v, pay_in >> 2) >> (8 * (pay_in & 3))) & 0xffffu);
+  return (int16_t)__shfl(w, src, 64);
+}
+
 template <int FORM>
 __device__ __forceinline__ void hist_chunk_regs(LdsHist<FORM>& L, const uint4& v, const int (&off)[16],
                                                 typename Pay<FORM>::T p) {
@@ -471,6 +485,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
   };
   zero_all();
   const int c0 = g0 >> 4, nc = ((g1 - 1) >> 4) + 1 - c0;
+  const int pay_chunk = s.pay_off >> 4, pay_in = s.pay_off & 15;
   const int tc = kHistThreads - kHistThreads % nc;  // threads of the chunk loop
   const int rstep = tc / nc;
   const int my_c = c0 + (int)threadIdx.x % nc;
@@ -502,12 +517,20 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
     const long long cb = (long long)seg.begin + (x - it.row0);
     const long long ce = (long long)seg.begin + (iend - it.row0);
     const uint8_t* src = rec_of(s, seg.buf) + 16 * my_c;
+    // a partition buffer's rows carry their payload: with the payload's chunk
+    // in the tile and a row's chunks in one wave, the lanes pass it across (no
+    // second read of the records); otherwise it is staged in LDS per piece
+    constexpr bool kQuantPay = FORM == FORM_QUANT || FORM == kHistQ32;  // 2-byte payloads
+    const bool regs_pay =
+        kQuantPay && seg.buf != 0 && pay_chunk >= c0 && pay_chunk < c0 + nc && 64 % nc == 0;  // block-uniform
     bool first = true;
     long long since = 0;
     for (long long t0 = cb; t0 < ce; t0 += kHistRows) {
       const int rows = (int)min((long long)kHistRows, ce - t0);
-      for (int r = threadIdx.x; r < rows; r += kHistThreads) pay[r] = load_payload<FORM>(s, seg.buf, t0 + r);
-      __syncthreads();
+      if (!regs_pay) {
+        for (int r = threadIdx.x; r < rows; r += kHistThreads) pay[r] = load_payload<FORM>(s, seg.buf, t0 + r);
+        __syncthreads();
+      }
       if ((int)threadIdx.x < tc) {
         const uint8_t* base = src + (size_t)t0 * s.stride;
         for (int r0 = (int)threadIdx.x / nc; r0 < rows; r0 += 4 * rstep) {
@@ -515,12 +538,17 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int r = r0 + u * rstep;
-            if (r < rows) v[u] = *(const uint4*)(base + (size_t)r * s.stride);
+            v[u] = r < rows ? *(const uint4*)(base + (size_t)r * s.stride) : make_uint4(0u, 0u, 0u, 0u);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int r = r0 + u * rstep;
-            if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, pay[r]);
+            if constexpr (kQuantPay) {
+              const PT p = regs_pay ? row_payload<FORM>(v[u], pay_in, nc, pay_chunk - c0) : (r < rows ? pay[r] : PT{});
+              if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, p);
+            } else {
+              if (r < rows) hist_chunk_regs<FORM>(L, v[u], off, pay[r]);
+            }
           }
         }
       }
