@@ -298,6 +298,23 @@ struct ChildParams {
   int max_depth, min_data_in_leaf, hist_min_rows, hist_max_blocks;
   int rows_per_flush, max_partials, F, part_hist;
 };
+// Row blocks of a histogram build of `rows` rows: about rows / min_rows, at most
+// max_blocks, but a small build spreads over up to kHistSpreadBlocks blocks of at
+// least kHistSpreadRows rows so that it occupies the chip, and no block takes
+// more than rows_per_flush rows. Grows with rows (a grid sized for a bound holds
+// every build within it).
+constexpr long long kHistSpreadBlocks = 256, kHistSpreadRows = 512;
+BK_HD inline long long hist_row_blocks(long long rows, int min_rows, int max_blocks, int rows_per_flush) {
+  if (rows <= 0) return 0;
+  long long nb = rows / min_rows;
+  if (nb < 1) nb = 1;
+  if (nb > max_blocks) nb = max_blocks;
+  long long spread = (rows + kHistSpreadRows - 1) / kHistSpreadRows;
+  if (spread > kHistSpreadBlocks) spread = kHistSpreadBlocks;
+  if (nb < spread) nb = spread;
+  const long long flush = (rows + rows_per_flush - 1) / rows_per_flush;
+  return nb < flush ? flush : nb;
+}
 constexpr int kChildThreads = 1024;  // splits one derivation takes (one thread each)
 constexpr int kPartDeriveMax = 512;  // splits the partition's last block derives (one thread each)
 // The derivation run by the partition's last block (on != 0; at most 512 splits),

@@ -2174,8 +2174,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(PlanState* __restrict__ s
   // histogram tiling of the round: as the host plans it
   int nblocks = 0, rpb = 0;
   if (htotal > 0) {
-    long long nb = min((long long)pp.hist_max_blocks, max(1ll, (long long)htotal / pp.hist_min_rows));
-    nb = max(nb, ((long long)htotal + pp.rows_per_flush - 1) / pp.rows_per_flush);
+    const long long nb = hist_row_blocks(htotal, pp.hist_min_rows, pp.hist_max_blocks, pp.rows_per_flush);
     rpb = (int)(((long long)htotal + nb - 1) / nb);
     nblocks = (htotal + rpb - 1) / rpb;
   }
@@ -2332,8 +2331,7 @@ __device__ void derive_children(const PartItem* __restrict__ pitems, int ni, con
   block_scan2<NT>(hrows > 0 ? 1 : 0, 0, &hidx, &unused0, &nhist, &unused1, scratch);
   int nblocks = 0, rpb = 0;
   if (htotal > 0) {  // as the host cuts a build of htotal rows
-    long long nb = min((long long)cp.hist_max_blocks, max(1ll, (long long)htotal / cp.hist_min_rows));
-    nb = max(nb, ((long long)htotal + cp.rows_per_flush - 1) / cp.rows_per_flush);
+    const long long nb = hist_row_blocks(htotal, cp.hist_min_rows, cp.hist_max_blocks, cp.rows_per_flush);
     rpb = (int)(((long long)htotal + nb - 1) / nb);
     nblocks = (htotal + rpb - 1) / rpb;
   }

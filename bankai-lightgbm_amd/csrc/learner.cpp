@@ -965,7 +965,9 @@ struct Learner {
     // a build has at most one row block per 2048 rows (the tuner's smallest row
     // count), and at least one per rows_per_flush rows: a partial slot never holds
     // more rows than one LDS accumulation (see plan_hist)
-    max_partials = std::max(std::min(kHistMaxBlocks, N / 2048 + 1), N / rows_per_flush + 2) + 2 * max_items + 8;
+    max_partials = std::max({std::min(kHistMaxBlocks, N / 2048 + 1), N / rows_per_flush + 2,
+                             (int)std::min<long long>(kHistSpreadBlocks, N / kHistSpreadRows + 1)}) +
+                   2 * max_items + 8;
     // a histogram-building partition writes one partial per tile
     if (hist32_ok) max_partials = std::max(max_partials, max_part_tiles);
     // device allocation
@@ -1686,11 +1688,10 @@ struct Learner {
       ++hp.ni;
     }
     if (hp.ni == 0) return hp;
-    int nblocks = (int)std::min<long long>(hist_max_blocks, std::max<long long>(1, hp.total / hist_min_rows));
     // a block's partial slot holds the sum of all its rows, packed as the LDS bins
     // are: more rows than one LDS accumulation may hold could carry one packed field
     // into the next, so no block takes more than rows_per_flush rows
-    nblocks = (int)std::max<long long>(nblocks, (hp.total + rows_per_flush - 1) / rows_per_flush);
+    const int nblocks = (int)hist_row_blocks(hp.total, hist_min_rows, hist_max_blocks, rows_per_flush);
     hp.rpb = (int)((hp.total + nblocks - 1) / nblocks);
     hp.nblocks = (int)((hp.total + hp.rpb - 1) / hp.rpb);
     // slots: block b gets one per item it touches
@@ -2055,10 +2056,7 @@ struct Learner {
   // none): the block target nb grows with the rows, and a build of r rows takes
   // ceil(r / ceil(r / nb(r))) <= nb(r) blocks.
   int hist_blocks_for(long long rows) const {
-    if (rows <= 0) return 0;
-    long long nb = std::min<long long>(hist_max_blocks, std::max<long long>(1, rows / hist_min_rows));
-    nb = std::max<long long>(nb, (rows + rows_per_flush - 1) / rows_per_flush);
-    return (int)nb;
+    return (int)hist_row_blocks(rows, hist_min_rows, hist_max_blocks, rows_per_flush);
   }
 
   // One round of batched growth with 32-bit bins: the partition counts the
@@ -2462,10 +2460,7 @@ struct Learner {
     return pp;
   }
   // the most row blocks a planned histogram cuts: every row in one build
-  int plan_hist_grid() const {
-    return (int)std::max<long long>(std::min<long long>(hist_max_blocks, std::max<long long>(1, N / hist_min_rows)),
-                                    ((long long)N + rows_per_flush - 1) / rows_per_flush);
-  }
+  int plan_hist_grid() const { return (int)hist_row_blocks(N, hist_min_rows, hist_max_blocks, rows_per_flush); }
   void launch_plan_round() {
     const PlanParams pp = plan_params();
     timed(7, 0.0, 0.0, [&] {
