@@ -1598,6 +1598,47 @@ __device__ void subtract_slice(const DevState& s, const FeatDev& m, int slot, co
   }
 }
 
+// Bin threadIdx.x of a slot's slice, in registers (the parent's fixed values).
+struct ParentBin {
+  long long ig, ih;
+  double g, h;
+  int c;
+};
+
+template <int FORM>
+__device__ __forceinline__ ParentBin load_parent_bin(const DevState& s, const FeatDev& m, int slot) {
+  ParentBin b{};
+  const int i = (int)threadIdx.x;
+  if (i >= m.hist_len) return b;
+  const int bin = m.hist_off + i;
+  const double* pool = s.pool + ((size_t)slot * s.total_bins) * 2;
+  if (FORM == FORM_QUANT) {
+    b.ig = ((const long long*)pool)[2 * bin];
+    b.ih = ((const long long*)pool)[2 * bin + 1];
+  } else {
+    b.g = pool[2 * bin];
+    b.h = pool[2 * bin + 1];
+  }
+  b.c = !s.hist32 ? s.pool_cnt[(size_t)slot * s.total_bins + bin] : 0;
+  return b;
+}
+
+// subtract_slice with the parent's bin already in registers.
+template <int FORM>
+__device__ __forceinline__ void subtract_parent_bin(const DevState& s, const FeatDev& m, int slot, const ParentBin& b,
+                                                    const SliceLds& small, SliceLds& L) {
+  const int i = (int)threadIdx.x;
+  if (i >= m.hist_len) return;
+  if (FORM == FORM_QUANT) {
+    L.ig[i] = b.ig - small.ig[i];
+    L.ih[i] = b.ih - small.ih[i];
+  } else {
+    L.g[i] = b.g - small.g[i];
+    L.h[i] = b.h - small.h[i];
+  }
+  L.c[i] = !s.hist32 ? b.c - small.c[i] : 0;
+}
+
 // Stable ctr order of the eligible bins (feature_histogram.rs:1478-1486):
 // rank = number of eligible bins with a smaller ctr, or an equal ctr and a lower index.
 template <int FORM>
@@ -1795,13 +1836,19 @@ __device__ __forceinline__ void scan_item(const DevState& s, const ScanItem* __r
   const FeatDev m = s.feats[f];
   // smaller leaf: raw histogram from its slot, fix, store
   __shared__ long long red[3 * (kScanThreads / 64)];
+  // the larger leaf's slot holds its parent's fixed slice: with one bin per
+  // thread it is loaded beside the smaller leaf's (one global round trip, not two)
+  const bool pre = has_large && use_subtract && m.hist_len <= (int)blockDim.x;  // block-uniform
+  ParentBin pb{};
+  if (pre) pb = load_parent_bin<FORM>(s, m, large.slot);
   load_slice_raw<FORM>(s, m, small.slot, p, S[0]);
   __syncthreads();
   fix_slice<FORM>(m, small, S[0], red);
   store_slice<FORM>(s, m, small.slot, S[0]);
   if (has_large) {
     if (use_subtract) {
-      subtract_slice<FORM>(s, m, large.slot, S[0], S[1]);
+      if (pre) subtract_parent_bin<FORM>(s, m, large.slot, pb, S[0], S[1]);
+      else subtract_slice<FORM>(s, m, large.slot, S[0], S[1]);
       __syncthreads();
     } else {
       load_slice_raw<FORM>(s, m, large.slot, p, S[1]);
