@@ -513,8 +513,50 @@ void launch_renew_par(int n, const int32_t* leaf_of, const float* g, const float
                       double* part_sums, int* part_meta, void* stream);
 void launch_renew_fix(const Segment* d_segs, int num_leaves, int nparts, const double* part_sums, const int* part_meta,
                       double* d_sums, int* d_flags, void* stream);
+// One piece: the values of one leaf among rows [p * piece_rows, (p + 1) * piece_rows)
+// of one component: their sum (exact where the piece's own certificate holds), the
+// sum of their magnitudes and the lowest set bit's exponent over them (INT_MAX: no
+// nonzero value, INT_MIN: a NaN or infinity).
+struct RenewPiece {
+  double tg, th, ag, ah;
+  int eg, eh;
+};
+// Piece storage for the piecewise chain: num_pieces x num_leaves entries (piece-major).
+// A piece added from a running sum s inside one binade [2^eb, 2^(eb+1)) of |s| (the
+// sum stays there throughout the piece): every addition rounds to a multiple of
+// u = 2^(eb-52), to the nearest one whatever s is, except a tie, which goes to the
+// even multiple. The piece adds u times s0 when s/u is even, u times s1 when odd.
+struct RenewBinade {
+  long long s0, s1;
+  int eb;  // INT_MIN: no summary
+  int pad;
+};
+// Piece storage for the piecewise chain: num_pieces x num_leaves entries (piece-major),
+// the binade summaries per (piece, leaf, gradients/hessians), and the list of pieces
+// whose summary is computed (piece, leaf, component, binade; a counter first).
+struct RenewWork {
+  int piece, leaf, comp, eb;
+};
+struct RenewPieces {
+  RenewPiece* pieces = nullptr;
+  RenewBinade* binade = nullptr;
+  RenewWork* work = nullptr;
+  int* work_count = nullptr;
+  int piece_rows = 0, num_pieces = 0, work_cap = 0;
+};
+constexpr int kRenewPieceRows = 4096;  // rows per piece at the least
+// Rows per piece for n rows and num_leaves leaves: kRenewPieceRows, doubled until the
+// pieces take at most max_bytes.
+int renew_piece_rows(long long n, int num_leaves, size_t max_bytes);
+size_t renew_pieces_lds(int num_leaves);
+// The sequential f64 sums of the flagged leaves (d_flags bit 0: gradients, bit 1:
+// hessians; null: both of every leaf) in ascending row order, into d_sums[2 * leaf + c].
+// With pieces (pieces.pieces != null) the chain runs piece by piece: a piece whose
+// every partial sum, started from the chain's running sum, is exactly representable
+// is added as one exact sum, the others value by value. Without, one block per leaf
+// adds every value.
 void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int num_leaves, const float* g,
-                       const float* h, const int* d_flags, double* d_sums, void* stream);
+                       const float* h, const int* d_flags, double* d_sums, const RenewPieces& pieces, void* stream);
 // score[row] += d_leaf_values[leaf] over the bag's rows (out_of_bag = 0) or the rows
 // outside the bag (out_of_bag = 1).
 void launch_add_score_rows(int n, const int32_t* leaf_of, const double* d_leaf_values, int num_leaves, int out_of_bag,

@@ -3006,6 +3006,394 @@ __global__ void __launch_bounds__(256) k_renew_sums(int n, const int32_t* __rest
   if (threadIdx.x == 64 && (need & 2)) out[2 * leaf + 1] = ah;
 }
 
+// Piecewise sequential sums. A leaf's sequence of values in ascending row order
+// is cut into pieces (its rows within consecutive row ranges). Started from the
+// chain's running sum s, a piece adds without any rounding when every partial
+// sum s + x_1 + ... + x_i is exactly representable: all of them are multiples of
+// 2^E, E = min(lowest set bit of s, lowest set bit of the piece's values), and
+// none exceeds |s| + sum|x| in magnitude, so |s| + sum|x| < 2^(53 + E) suffices
+// (an integer times 2^E below 2^53 in magnitude is a double, E >= -1074 here).
+// Then the chain's result after the piece is s + T exactly, T the piece's exact
+// sum (exact in any order under the same bound, so the atomics may add it), and
+// the piece costs one addition instead of one per value. Pieces that fail the
+// bound are added value by value, as the reference's loop does.
+
+// Exponent of the lowest set bit: INT_MAX for zero, INT_MIN for NaN or infinity.
+__device__ __forceinline__ int lowest_bit_f32(float x) {
+  const uint32_t b = __float_as_uint(x) & 0x7fffffffu;
+  if (b == 0u) return INT_MAX;
+  const int be = (int)(b >> 23);
+  if (be == 0xff) return INT_MIN;
+  const uint32_t m = b & 0x7fffffu;
+  return be == 0 ? -149 + __builtin_ctz(m) : (be - 150) + __builtin_ctz(m | 0x800000u);
+}
+
+__device__ __forceinline__ int lowest_bit_f64(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
+  if (b == 0ull) return INT_MAX;
+  const int be = (int)(b >> 52);
+  if (be == 0x7ff) return INT_MIN;
+  const unsigned long long m = b & 0xfffffffffffffull;
+  return be == 0 ? -1074 + __builtin_ctzll(m) : (be - 1075) + __builtin_ctzll(m | 0x10000000000000ull);
+}
+
+__device__ __forceinline__ void piece_add(double* t, double* a, int* e, float x) {
+  if (x == 0.0f) return;  // adds nothing to either sum (a running sum that starts at +0 stays +0)
+  atomicAdd(t, (double)x);
+  atomicAdd(a, fabs((double)x));
+  const int ex = lowest_bit_f32(x);
+  if (ex < *e) atomicMin(e, ex);
+}
+
+// One block per piece: for every flagged leaf, its values among the piece's rows.
+__global__ void __launch_bounds__(256) k_renew_pieces(int n, int piece_rows, const int32_t* __restrict__ leaf_of,
+                                                      const float* __restrict__ g, const float* __restrict__ h,
+                                                      int num_leaves, const int* __restrict__ flags,
+                                                      RenewPiece* __restrict__ pieces, int* work_count) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && work_count) *work_count = 0;  // k_renew_guess appends after this launch
+  RenewPiece* acc = (RenewPiece*)smem;
+  int* need = (int*)(acc + num_leaves);
+  int any = 0;
+  for (int l = threadIdx.x; l < num_leaves; l += blockDim.x) {
+    const int f = flags ? flags[l] : 3;
+    need[l] = f;
+    any |= f;
+    acc[l] = RenewPiece{0.0, 0.0, 0.0, 0.0, INT_MAX, INT_MAX};
+  }
+  if (!__syncthreads_or(any)) return;  // no leaf needs the chain: block-uniform
+  const long long r0 = (long long)blockIdx.x * piece_rows;
+  const long long r1 = min(r0 + (long long)piece_rows, (long long)n);
+  for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const int l = leaf_of[r];
+    if ((unsigned)l >= (unsigned)num_leaves) continue;  // a row outside the bag (~leaf)
+    const int f = need[l];
+    if (f & 1) piece_add(&acc[l].tg, &acc[l].ag, &acc[l].eg, g[r]);
+    if (f & 2) piece_add(&acc[l].th, &acc[l].ah, &acc[l].eh, h[r]);
+  }
+  __syncthreads();
+  RenewPiece* out = pieces + (size_t)blockIdx.x * num_leaves;
+  for (int l = threadIdx.x; l < num_leaves; l += blockDim.x)
+    if (need[l]) out[l] = acc[l];
+}
+
+__device__ __forceinline__ float lane_f32(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+__device__ __forceinline__ double lane_f64(double v, int k) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ long long lane_i64(long long v, int k) {
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), k);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), k);
+  return ((long long)hi << 32) | (long long)(unsigned)lo;
+}
+
+// Value-by-value addition of one leaf's values among rows [r0, r1), in row order;
+// called by a whole wave, every lane holding the same s. A batch of 4096 rows is
+// loaded at once (16 x 16 bytes of leaves and of values per lane; lane i holds rows
+// 4i .. 4i + 3 of every 256), then its matches are added in row order.
+__device__ double chain_rows(double s, long long r0, long long r1, int leaf, const int32_t* __restrict__ leaf_of,
+                             const float* __restrict__ x) {
+  const int lane = (int)threadIdx.x & 63;
+  constexpr int kVec = 16;  // uint4 per lane and array: 64 x 4 x 16 = 4096 rows
+  // 16-byte loads need 16-byte aligned rows (r0 is a multiple of 4096; a caller's
+  // value array may start anywhere)
+  const bool vec = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u && (r0 & 3) == 0;
+  for (long long b = r0; b < r1; b += 256 * kVec) {
+    if (vec && b + 256 * kVec <= r1) {
+      int4 lv[kVec];
+      float4 xv[kVec];
+#pragma unroll
+      for (int j = 0; j < kVec; ++j) {
+        lv[j] = *(const int4*)(leaf_of + b + 256 * j + 4 * lane);
+        xv[j] = *(const float4*)(x + b + 256 * j + 4 * lane);
+      }
+#pragma unroll
+      for (int j = 0; j < kVec; ++j) {
+        const unsigned long long m0 = __ballot(lv[j].x == leaf), m1 = __ballot(lv[j].y == leaf);
+        const unsigned long long m2 = __ballot(lv[j].z == leaf), m3 = __ballot(lv[j].w == leaf);
+        unsigned long long mm = m0 | m1 | m2 | m3;
+        while (mm) {  // wave-uniform: lanes in order, each lane's four rows in order
+          const int k = __builtin_ctzll(mm);
+          mm &= mm - 1ull;
+          if ((m0 >> k) & 1ull) s += (double)lane_f32(xv[j].x, k);
+          if ((m1 >> k) & 1ull) s += (double)lane_f32(xv[j].y, k);
+          if ((m2 >> k) & 1ull) s += (double)lane_f32(xv[j].z, k);
+          if ((m3 >> k) & 1ull) s += (double)lane_f32(xv[j].w, k);
+        }
+      }
+    } else {  // the last rows of the data: 64 at a time
+      for (long long c = b; c < r1; c += 64) {
+        const long long r = c + lane;
+        const bool mine = r < r1 && leaf_of[r] == leaf;
+        const float v = mine ? x[r] : 0.0f;
+        unsigned long long mm = __ballot(mine);
+        while (mm) {
+          const int k = __builtin_ctzll(mm);
+          mm &= mm - 1ull;
+          s += (double)lane_f32(v, k);
+        }
+      }
+    }
+  }
+  return s;
+}
+
+// The exact-piece test of the chain: from s, every partial sum of the piece is
+// exactly representable (sum|x| from at most 2^22 atomic adds is below the true
+// value by less than a factor 2^-31, so a * (1 + 2^-30) bounds it from above; a
+// rounded sum below a power of two means the exact sum is below it too).
+__device__ __forceinline__ bool piece_exact(double s, double a, int e, double margin) {
+  const int em = min(lowest_bit_f64(s), e);
+  return em != INT_MIN && fabs(s) + a * (1.0 + 0x1p-30) * margin < ldexp(1.0, 53 + em);
+}
+
+// Binade exponent of a normal double (|s| in [2^eb, 2^(eb+1))), INT_MIN otherwise.
+__device__ __forceinline__ int binade_of(double s) {
+  const int be = (int)(((unsigned long long)__double_as_longlong(s) >> 52) & 0x7ffull);
+  return (be == 0 || be == 0x7ff) ? INT_MIN : be - 1023;
+}
+
+// The binade test of the chain: s and every partial sum of the piece, rounded or
+// not, stay strictly inside s's binade. The drift of the rounded sums from the
+// exact ones is at most half a unit per addition, bounded here by piece_rows units.
+__device__ __forceinline__ bool piece_in_binade(double s, double a, int eb, int piece_rows) {
+  if (eb == INT_MIN || eb < -1022 || eb > 900 || binade_of(s) != eb) return false;
+  const double slack = a * (1.0 + 0x1p-30) + (double)piece_rows * ldexp(1.0, eb - 52);
+  return fabs(s) - slack > ldexp(1.0, eb) && fabs(s) + slack < ldexp(1.0, eb + 1);
+}
+
+// One block of two waves per leaf (gradients, hessians; each only when flagged):
+// the chain's running sum guessed piece by piece from the pieces' sums, and the
+// pieces that the exact test may reject listed for k_renew_binade with the binade
+// of the guess. The guess only picks pieces and binades: the chain checks both.
+__global__ void __launch_bounds__(128) k_renew_guess(int num_pieces, int num_leaves, const int* __restrict__ flags,
+                                                     const RenewPiece* __restrict__ pieces,
+                                                     RenewBinade* __restrict__ binade, RenewWork* __restrict__ work,
+                                                     int* work_count, int work_cap) {
+  const int leaf = blockIdx.x, comp = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int need = flags ? flags[leaf] : 3;
+  if (!((need >> comp) & 1)) return;  // wave-uniform
+  double s = 0.0;
+  for (int p0 = 0; p0 < num_pieces; p0 += 64) {
+    double t = 0.0, a = 0.0;
+    int e = INT_MAX;
+    if (p0 + lane < num_pieces) {
+      const RenewPiece& pc = pieces[(size_t)(p0 + lane) * num_leaves + leaf];
+      t = comp ? pc.th : pc.tg;
+      a = comp ? pc.ah : pc.ag;
+      e = comp ? pc.eh : pc.eg;
+    }
+    int mine_eb = INT_MIN;
+    const int cnt = min(64, num_pieces - p0);
+    for (int j = 0; j < cnt; ++j) {
+      const int ej = __builtin_amdgcn_readlane(e, j);
+      if (ej == INT_MAX || ej == INT_MIN) continue;
+      const double tj = lane_f64(t, j), aj = lane_f64(a, j);
+      // a margin of 4 on the test: the guess drifts from the chain's sum by roundings
+      if (!piece_exact(s, aj, ej, 4.0)) {
+        const int eb = binade_of(s);
+        if (lane == j) mine_eb = eb;
+      }
+      s += tj;
+    }
+    if (p0 + lane < num_pieces) {
+      int eb = mine_eb;
+      if (eb != INT_MIN) {
+        const int w = atomicAdd(work_count, 1);
+        if (w < work_cap) work[w] = RenewWork{p0 + lane, leaf, comp, eb};
+        else eb = INT_MIN;  // no room: the chain adds this piece value by value
+      }
+      binade[((size_t)(p0 + lane) * num_leaves + leaf) * 2 + comp].eb = eb;
+    }
+  }
+}
+
+// The 2-state summary of a run of additions inside one binade: the units added
+// and the parity of s/u after the run, for s/u even (0) and odd (1) before it.
+struct ParityMap {
+  long long s0, s1;
+  int o0, o1;
+};
+
+__device__ __forceinline__ ParityMap parity_then(const ParityMap& f, const ParityMap& g) {
+  ParityMap r;
+  r.s0 = f.s0 + (f.o0 ? g.s1 : g.s0);
+  r.s1 = f.s1 + (f.o1 ? g.s1 : g.s0);
+  r.o0 = f.o0 ? g.o1 : g.o0;
+  r.o1 = f.o1 ? g.o1 : g.o0;
+  return r;
+}
+
+// One addition of x (a value of the leaf) with s/u = k: s + x rounds to
+// (k + q + round(f)) u, x/u = q + f; a tie (f = 1/2) goes to the even one of k + q
+// and k + q + 1. bad is set when x/u is not a moderate number (not this binade's).
+__device__ __forceinline__ ParityMap parity_of(bool mine, float x, double inv_u, bool& bad) {
+  ParityMap m{0, 0, 0, 1};
+  if (!mine) return m;
+  const double y = (double)x * inv_u;  // exact: a power-of-two scaling within range
+  if (!(fabs(y) < 0x1p62)) {
+    bad = true;
+    return m;
+  }
+  const double q = floor(y), f = y - q;
+  const long long qi = (long long)q;
+  if (f == 0.5) {
+    m.s0 = qi + (qi & 1);        // k even: k + q even if q even, else k + q + 1
+    m.s1 = qi + ((qi & 1) ^ 1);  // k odd
+    m.o0 = m.o1 = 0;             // either way the result is even
+  } else {
+    const long long a = qi + (f > 0.5 ? 1 : 0);
+    m.s0 = m.s1 = a;
+    m.o0 = (int)(a & 1);
+    m.o1 = (int)((a & 1) ^ 1);
+  }
+  return m;
+}
+
+__device__ __forceinline__ ParityMap parity_shfl_down(const ParityMap& m, int o) {
+  ParityMap r;
+  r.s0 = __shfl_down(m.s0, o, 64);
+  r.s1 = __shfl_down(m.s1, o, 64);
+  r.o0 = __shfl_down(m.o0, o, 64);
+  r.o1 = __shfl_down(m.o1, o, 64);
+  return r;
+}
+
+// The lanes' maps in lane order, composed (lane 0 holds the result).
+__device__ __forceinline__ ParityMap parity_reduce(ParityMap m) {
+  const int lane = (int)threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const ParityMap r = parity_shfl_down(m, o);
+    if ((lane & (2 * o - 1)) == 0) m = parity_then(m, r);
+  }
+  return m;
+}
+
+__device__ __forceinline__ ParityMap parity_lane0(const ParityMap& m) {
+  ParityMap r;
+  const long long a = m.s0, b = m.s1;
+  r.s0 = ((long long)__builtin_amdgcn_readlane((int)(a >> 32), 0) << 32) |
+         (long long)(unsigned)__builtin_amdgcn_readlane((int)(a & 0xffffffffll), 0);
+  r.s1 = ((long long)__builtin_amdgcn_readlane((int)(b >> 32), 0) << 32) |
+         (long long)(unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), 0);
+  r.o0 = __builtin_amdgcn_readlane(m.o0, 0);
+  r.o1 = __builtin_amdgcn_readlane(m.o1, 0);
+  return r;
+}
+
+// One wave per listed piece: the summary of the leaf's values in the piece's
+// rows, in row order, for the listed binade.
+__global__ void __launch_bounds__(256) k_renew_binade(int n, int piece_rows, int num_leaves,
+                                                      const int32_t* __restrict__ leaf_of,
+                                                      const float* __restrict__ g, const float* __restrict__ h,
+                                                      RenewBinade* __restrict__ binade,
+                                                      const RenewWork* __restrict__ work, const int* work_count,
+                                                      int work_cap) {
+  const int lane = (int)threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  const int total = min(*work_count, work_cap);
+  for (int w = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); w < total; w += waves) {
+    const RenewWork it = work[w];
+    const int piece = it.piece, leaf = it.leaf, comp = it.comp, eb = it.eb;
+    const float* x = comp ? h : g;
+    const double inv_u = ldexp(1.0, 52 - eb);
+    const long long r0 = (long long)piece * piece_rows, r1 = min(r0 + (long long)piece_rows, (long long)n);
+    ParityMap acc{0, 0, 0, 1};
+    bool bad = false;
+    const bool vec = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u;
+    for (long long b = r0; b < r1; b += 256) {
+      ParityMap m{0, 0, 0, 1};
+      if (vec && b + 256 <= r1) {  // lane i: rows 4i .. 4i + 3 of the 256, in order
+        const int4 lv = *(const int4*)(leaf_of + b + 4 * lane);
+        const float4 xv = *(const float4*)(x + b + 4 * lane);
+        m = parity_then(m, parity_of(lv.x == leaf, xv.x, inv_u, bad));
+        m = parity_then(m, parity_of(lv.y == leaf, xv.y, inv_u, bad));
+        m = parity_then(m, parity_of(lv.z == leaf, xv.z, inv_u, bad));
+        m = parity_then(m, parity_of(lv.w == leaf, xv.w, inv_u, bad));
+      } else {  // lane i: rows 4i .. 4i + 3, those below r1
+        for (int t = 0; t < 4; ++t) {
+          const long long r = b + 4 * lane + t;
+          const bool mine = r < r1 && leaf_of[r] == leaf;
+          m = parity_then(m, parity_of(mine, mine ? x[r] : 0.0f, inv_u, bad));
+        }
+      }
+      acc = parity_then(acc, parity_lane0(parity_reduce(m)));
+    }
+    const bool any_bad = __ballot(bad) != 0ull;
+    if (lane == 0) {
+      RenewBinade& out = binade[((size_t)piece * num_leaves + leaf) * 2 + comp];
+      out.s0 = acc.s0;
+      out.s1 = acc.s1;
+      out.eb = any_bad ? INT_MIN : eb;
+    }
+  }
+}
+
+// One block of two waves per leaf: wave 0 chains the gradients, wave 1 the
+// hessians (each only when flagged), over the pieces in row order: an exact piece
+// adds its sum, a piece inside the running sum's binade with a summary for that
+// binade adds its units, any other piece adds its values one by one.
+__global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int num_pieces,
+                                                     const int32_t* __restrict__ leaf_of,
+                                                     const float* __restrict__ g, const float* __restrict__ h,
+                                                     int num_leaves, const int* __restrict__ flags,
+                                                     const RenewPiece* __restrict__ pieces,
+                                                     const RenewBinade* __restrict__ binade, double* out) {
+  const int leaf = blockIdx.x, comp = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int need = flags ? flags[leaf] : 3;
+  if (!((need >> comp) & 1)) return;  // wave-uniform
+  const float* x = comp ? h : g;
+  double s = 0.0;
+  for (int p0 = 0; p0 < num_pieces; p0 += 64) {
+    double t = 0.0, a = 0.0;
+    int e = INT_MAX, beb = INT_MIN;
+    long long bs0 = 0, bs1 = 0;
+    if (p0 + lane < num_pieces) {
+      const size_t at = (size_t)(p0 + lane) * num_leaves + leaf;
+      const RenewPiece& pc = pieces[at];
+      t = comp ? pc.th : pc.tg;
+      a = comp ? pc.ah : pc.ag;
+      e = comp ? pc.eh : pc.eg;
+      if (binade) {
+        const RenewBinade& bn = binade[at * 2 + comp];
+        beb = bn.eb;
+        bs0 = bn.s0;
+        bs1 = bn.s1;
+      }
+    }
+    const int cnt = min(64, num_pieces - p0);
+    for (int j = 0; j < cnt; ++j) {
+      const int ej = __builtin_amdgcn_readlane(e, j);
+      if (ej == INT_MAX) continue;  // no nonzero value of the leaf in this piece
+      const double aj = lane_f64(a, j);
+      if (piece_exact(s, aj, ej, 1.0)) {
+        s += lane_f64(t, j);
+        continue;
+      }
+      const int ebj = __builtin_amdgcn_readlane(beb, j);
+      if (ej != INT_MIN && piece_in_binade(s, aj, ebj, piece_rows)) {
+        // s = k u with k an integer below 2^53 in magnitude; the piece adds s0 or
+        // s1 units by the parity of k, and the result stays in the binade
+        const double u = ldexp(1.0, ebj - 52);
+        const long long k = (long long)(s * ldexp(1.0, 52 - ebj));
+        const long long add = (k & 1) ? lane_i64(bs1, j) : lane_i64(bs0, j);
+        s = (double)(k + add) * u;
+        continue;
+      }
+      const long long r0 = (long long)(p0 + j) * piece_rows;
+      s = chain_rows(s, r0, min(r0 + (long long)piece_rows, (long long)n), leaf, leaf_of, x);
+    }
+  }
+  if (lane == 0) out[2 * leaf + comp] = s;
+}
+
 // score[row] += vals[leaf of row] over the rows of the tree's bag (out_of_bag = 0)
 // or over the rows outside it (out_of_bag = 1, whose leaf_of holds ~leaf).
 __global__ void k_add_score_rows(int n, const int32_t* __restrict__ leaf_of, const double* __restrict__ vals,
@@ -3736,10 +4124,41 @@ void launch_pack_f32(const float* g, const float* h, int n, int const_hess, floa
   BK_HIP(hipGetLastError());
 }
 
+int renew_piece_rows(long long n, int num_leaves, size_t max_bytes) {
+  long long rows = kRenewPieceRows;
+  while (rows < n && (size_t)((n + rows - 1) / rows) * (size_t)num_leaves * sizeof(RenewPiece) > max_bytes) rows *= 2;
+  return (int)std::min<long long>(rows, 1ll << 22);  // the magnitude bound of k_renew_chain holds to 2^22 rows
+}
+
+size_t renew_pieces_lds(int num_leaves) { return (size_t)num_leaves * (sizeof(RenewPiece) + sizeof(int)); }
+
 void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int num_leaves, const float* g,
-                       const float* h, const int* d_flags, double* d_sums, void* stream) {
-  launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(256), 0, (hipStream_t)stream, n, leaf_of, d_order, g, h,
-         d_flags, d_sums);
+                       const float* h, const int* d_flags, double* d_sums, const RenewPieces& pieces, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = renew_pieces_lds(num_leaves);
+  if (pieces.pieces && lds <= (size_t)kLdsBytes && pieces.piece_rows > 0 &&
+      (long long)pieces.num_pieces * pieces.piece_rows >= n) {
+    const bool summaries = pieces.binade && pieces.work && pieces.work_count && pieces.work_cap > 0;
+    allow_lds(k_renew_pieces);
+    launch(k_renew_pieces, dim3((unsigned)std::max(pieces.num_pieces, 1)), dim3(256), lds, st, n, pieces.piece_rows,
+           leaf_of, g, h, num_leaves, d_flags, pieces.pieces, summaries ? pieces.work_count : (int*)nullptr);
+    BK_HIP(hipGetLastError());
+    if (summaries) {
+      launch(k_renew_guess, dim3((unsigned)num_leaves), dim3(128), 0, st, pieces.num_pieces, num_leaves, d_flags,
+             (const RenewPiece*)pieces.pieces, pieces.binade, pieces.work, pieces.work_count, pieces.work_cap);
+      BK_HIP(hipGetLastError());
+      // the listed pieces, one wave each (a block's waves exit once the list is done)
+      launch(k_renew_binade, dim3(1024), dim3(256), 0, st, n, pieces.piece_rows, num_leaves, leaf_of, g, h,
+             pieces.binade, (const RenewWork*)pieces.work, (const int*)pieces.work_count, pieces.work_cap);
+      BK_HIP(hipGetLastError());
+    }
+    launch(k_renew_chain, dim3((unsigned)num_leaves), dim3(128), 0, st, n, pieces.piece_rows, pieces.num_pieces,
+           leaf_of, g, h, num_leaves, d_flags, (const RenewPiece*)pieces.pieces,
+           summaries ? (const RenewBinade*)pieces.binade : (const RenewBinade*)nullptr, d_sums);
+    BK_HIP(hipGetLastError());
+    return;
+  }
+  launch(k_renew_sums, dim3((unsigned)num_leaves), dim3(256), 0, st, n, leaf_of, d_order, g, h, d_flags, d_sums);
   BK_HIP(hipGetLastError());
 }
 

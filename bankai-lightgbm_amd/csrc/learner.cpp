@@ -631,6 +631,7 @@ struct Learner {
   double* d_renew_part_sums = nullptr;
   int* d_renew_part_meta = nullptr;
   int* d_renew_flags = nullptr;
+  RenewPieces renew_pieces;  // the piecewise chain's per-piece sums (renewal leaves whose certificate fails)
   NodeRule* h_nodes = nullptr;
   std::vector<NodeRule> tree_nodes;  // rule and group of every committed split, by internal node
   std::vector<int> last_left, last_right;  // children of the last tree's internal nodes
@@ -824,7 +825,9 @@ struct Learner {
                     (void*)d_max4, (void*)d_sums3,
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
-                    (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)d_bag, (void*)d_bag_bits, (void*)d_replay,
+                    (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)renew_pieces.pieces, (void*)renew_pieces.binade,
+                    (void*)renew_pieces.work, (void*)renew_pieces.work_count, (void*)d_bag,
+                    (void*)d_bag_bits, (void*)d_replay,
                     (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
                     (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree,
                     (void*)d_disc, (void*)d_node_count, (void*)d_chist, (void*)d_cscan, (void*)d_cred,
@@ -1087,6 +1090,18 @@ struct Learner {
     tree_nodes.assign((size_t)P.num_leaves, NodeRule{});
     if (renew) {
       d_renew_flags = dmalloc<int>((size_t)P.num_leaves);
+      if (renew_pieces_lds(P.num_leaves) <= (size_t)kLdsBytes) {
+        renew_pieces.piece_rows = renew_piece_rows(N, P.num_leaves, (size_t)256 << 20);
+        renew_pieces.num_pieces = (int)(((long long)N + renew_pieces.piece_rows - 1) / renew_pieces.piece_rows);
+        const size_t entries = (size_t)std::max(renew_pieces.num_pieces, 1) * (size_t)P.num_leaves;
+        renew_pieces.pieces = dmalloc<RenewPiece>(entries);
+        // binade summaries per piece, leaf and component; the list holds any number of
+        // pieces the guess may pick (every piece of every leaf, both components)
+        renew_pieces.binade = dmalloc<RenewBinade>(entries * 2);
+        renew_pieces.work_cap = (int)std::min<size_t>(entries * 2, (size_t)1 << 30);
+        renew_pieces.work = dmalloc<RenewWork>((size_t)renew_pieces.work_cap);
+        renew_pieces.work_count = dmalloc<int>(1);
+      }
       if (renew_par_lds(P.num_leaves) <= (size_t)kLdsBytes) {
         renew_nparts = renew_parts(N);
         d_renew_part_sums = dmalloc<double>((size_t)renew_nparts * P.num_leaves * 2);
@@ -2721,7 +2736,9 @@ struct Learner {
           flags = d_renew_flags;
         }
         timed(5, 0.0, 0.0,
-              [&] { launch_renew_sums(N, d_leaf_of, d_order, P.num_leaves, g, h, flags, d_renew, stream); });
+              [&] {
+                launch_renew_sums(N, d_leaf_of, d_order, P.num_leaves, g, h, flags, d_renew, renew_pieces, stream);
+              });
       }
       timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_plan), d_pstate, sizeof(PlanState), stream); });
       timed(8, 0.0, 0.0, [&] {
@@ -2864,7 +2881,8 @@ struct Learner {
         });
         flags = d_renew_flags;
       }
-      timed(5, 0.0, 0.0, [&] { launch_renew_sums(N, d_leaf_of, u_order, nl, g, h, flags, d_renew, stream); });
+      timed(5, 0.0, 0.0,
+            [&] { launch_renew_sums(N, d_leaf_of, u_order, nl, g, h, flags, d_renew, renew_pieces, stream); });
     }
     // one synchronization for the error flag and the renewal sums
     timed(8, 0.0, 0.0, [&] { launch_copy(dev_alias(h_tail), d_err, sizeof(int), stream); });
