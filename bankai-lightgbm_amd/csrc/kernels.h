@@ -529,18 +529,30 @@ struct RenewPiece {
 struct RenewBinade {
   long long s0, s1;
   int eb;  // INT_MIN: no summary
-  int pad;
+  int work;  // the piece's entry in the list (its sub-range summaries), -1: none
+};
+// The same tests for one sixteenth of a listed piece's rows: the exact sum, the
+// sum of magnitudes and the lowest set bit of the leaf's values there, and the
+// binade summary for the binade the guessed running sum has at its start.
+constexpr int kRenewParts = 16;
+struct RenewPart {
+  double t, a;
+  long long s0, s1;
+  int e, eb;
 };
 // Piece storage for the piecewise chain: num_pieces x num_leaves entries (piece-major),
 // the binade summaries per (piece, leaf, gradients/hessians), and the list of pieces
-// whose summary is computed (piece, leaf, component, binade; a counter first).
+// whose summaries are computed (piece, leaf, component, binade, guessed running sum
+// at the piece's start; a counter first), kRenewParts sub-range summaries per entry.
 struct RenewWork {
   int piece, leaf, comp, eb;
+  double s;
 };
 struct RenewPieces {
   RenewPiece* pieces = nullptr;
   RenewBinade* binade = nullptr;
   RenewWork* work = nullptr;
+  RenewPart* parts = nullptr;
   int* work_count = nullptr;
   int piece_rows = 0, num_pieces = 0, work_cap = 0;
 };

@@ -3095,50 +3095,43 @@ __device__ __forceinline__ long long lane_i64(long long v, int k) {
 }
 
 // Value-by-value addition of one leaf's values among rows [r0, r1), in row order;
-// called by a whole wave, every lane holding the same s. A batch of 4096 rows is
-// loaded at once (16 x 16 bytes of leaves and of values per lane; lane i holds rows
-// 4i .. 4i + 3 of every 256), then its matches are added in row order.
+// called by a whole wave, every lane holding the same s. Up to 4096 rows are
+// loaded at once (lane i holds rows 4i .. 4i + 3 of every 256: 16-byte loads where
+// the rows are aligned and whole), then their matches are added in row order.
 __device__ double chain_rows(double s, long long r0, long long r1, int leaf, const int32_t* __restrict__ leaf_of,
                              const float* __restrict__ x) {
   const int lane = (int)threadIdx.x & 63;
-  constexpr int kVec = 16;  // uint4 per lane and array: 64 x 4 x 16 = 4096 rows
-  // 16-byte loads need 16-byte aligned rows (r0 is a multiple of 4096; a caller's
-  // value array may start anywhere)
-  const bool vec = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u && (r0 & 3) == 0;
-  for (long long b = r0; b < r1; b += 256 * kVec) {
-    if (vec && b + 256 * kVec <= r1) {
-      int4 lv[kVec];
-      float4 xv[kVec];
+  constexpr int kSteps = 16;  // 256-row steps per batch
+  const bool aligned = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u && (r0 & 3) == 0;
+  for (long long b = r0; b < r1; b += 256 * kSteps) {
+    int4 lv[kSteps];
+    float4 xv[kSteps];
 #pragma unroll
-      for (int j = 0; j < kVec; ++j) {
-        lv[j] = *(const int4*)(leaf_of + b + 256 * j + 4 * lane);
-        xv[j] = *(const float4*)(x + b + 256 * j + 4 * lane);
+    for (int j = 0; j < kSteps; ++j) {
+      const long long c = b + 256 * j, r = c + 4 * lane;
+      if (aligned && c + 256 <= r1) {
+        lv[j] = *(const int4*)(leaf_of + r);
+        xv[j] = *(const float4*)(x + r);
+      } else {
+        // rows past r1 read as leaf -1, which no leaf index equals
+        lv[j] = make_int4(r < r1 ? leaf_of[r] : -1, r + 1 < r1 ? leaf_of[r + 1] : -1,
+                          r + 2 < r1 ? leaf_of[r + 2] : -1, r + 3 < r1 ? leaf_of[r + 3] : -1);
+        xv[j] = make_float4(r < r1 ? x[r] : 0.0f, r + 1 < r1 ? x[r + 1] : 0.0f, r + 2 < r1 ? x[r + 2] : 0.0f,
+                            r + 3 < r1 ? x[r + 3] : 0.0f);
       }
+    }
 #pragma unroll
-      for (int j = 0; j < kVec; ++j) {
-        const unsigned long long m0 = __ballot(lv[j].x == leaf), m1 = __ballot(lv[j].y == leaf);
-        const unsigned long long m2 = __ballot(lv[j].z == leaf), m3 = __ballot(lv[j].w == leaf);
-        unsigned long long mm = m0 | m1 | m2 | m3;
-        while (mm) {  // wave-uniform: lanes in order, each lane's four rows in order
-          const int k = __builtin_ctzll(mm);
-          mm &= mm - 1ull;
-          if ((m0 >> k) & 1ull) s += (double)lane_f32(xv[j].x, k);
-          if ((m1 >> k) & 1ull) s += (double)lane_f32(xv[j].y, k);
-          if ((m2 >> k) & 1ull) s += (double)lane_f32(xv[j].z, k);
-          if ((m3 >> k) & 1ull) s += (double)lane_f32(xv[j].w, k);
-        }
-      }
-    } else {  // the last rows of the data: 64 at a time
-      for (long long c = b; c < r1; c += 64) {
-        const long long r = c + lane;
-        const bool mine = r < r1 && leaf_of[r] == leaf;
-        const float v = mine ? x[r] : 0.0f;
-        unsigned long long mm = __ballot(mine);
-        while (mm) {
-          const int k = __builtin_ctzll(mm);
-          mm &= mm - 1ull;
-          s += (double)lane_f32(v, k);
-        }
+    for (int j = 0; j < kSteps; ++j) {
+      const unsigned long long m0 = __ballot(lv[j].x == leaf), m1 = __ballot(lv[j].y == leaf);
+      const unsigned long long m2 = __ballot(lv[j].z == leaf), m3 = __ballot(lv[j].w == leaf);
+      unsigned long long mm = m0 | m1 | m2 | m3;
+      while (mm) {  // wave-uniform: lanes in order, each lane's four rows in order
+        const int k = __builtin_ctzll(mm);
+        mm &= mm - 1ull;
+        if ((m0 >> k) & 1ull) s += (double)lane_f32(xv[j].x, k);
+        if ((m1 >> k) & 1ull) s += (double)lane_f32(xv[j].y, k);
+        if ((m2 >> k) & 1ull) s += (double)lane_f32(xv[j].z, k);
+        if ((m3 >> k) & 1ull) s += (double)lane_f32(xv[j].w, k);
       }
     }
   }
@@ -3190,27 +3183,34 @@ __global__ void __launch_bounds__(128) k_renew_guess(int num_pieces, int num_lea
       a = comp ? pc.ah : pc.ag;
       e = comp ? pc.eh : pc.eg;
     }
-    int mine_eb = INT_MIN;
+    int mine_mark = 0, mine_eb = INT_MIN;
+    double mine_s = 0.0;
     const int cnt = min(64, num_pieces - p0);
     for (int j = 0; j < cnt; ++j) {
       const int ej = __builtin_amdgcn_readlane(e, j);
       if (ej == INT_MAX || ej == INT_MIN) continue;
       const double tj = lane_f64(t, j), aj = lane_f64(a, j);
       // a margin of 4 on the test: the guess drifts from the chain's sum by roundings
-      if (!piece_exact(s, aj, ej, 4.0)) {
-        const int eb = binade_of(s);
-        if (lane == j) mine_eb = eb;
+      if (!piece_exact(s, aj, ej, 4.0) && lane == j) {
+        mine_mark = 1;
+        mine_eb = binade_of(s);
+        mine_s = s;
       }
       s += tj;
     }
     if (p0 + lane < num_pieces) {
-      int eb = mine_eb;
-      if (eb != INT_MIN) {
+      int eb = INT_MIN, wi = -1;
+      if (mine_mark) {
         const int w = atomicAdd(work_count, 1);
-        if (w < work_cap) work[w] = RenewWork{p0 + lane, leaf, comp, eb};
-        else eb = INT_MIN;  // no room: the chain adds this piece value by value
+        if (w < work_cap) {  // else no room: the chain adds this piece value by value
+          work[w] = RenewWork{p0 + lane, leaf, comp, mine_eb, mine_s};
+          eb = mine_eb;
+          wi = w;
+        }
       }
-      binade[((size_t)(p0 + lane) * num_leaves + leaf) * 2 + comp].eb = eb;
+      RenewBinade& bn = binade[((size_t)(p0 + lane) * num_leaves + leaf) * 2 + comp];
+      bn.eb = eb;
+      bn.work = wi;
     }
   }
 }
@@ -3288,52 +3288,107 @@ __device__ __forceinline__ ParityMap parity_lane0(const ParityMap& m) {
   return r;
 }
 
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 // One wave per listed piece: the summary of the leaf's values in the piece's
-// rows, in row order, for the listed binade.
+// rows, in row order, for the listed binade (when it has one), and the same for
+// each of its kRenewParts sub-ranges with the binade the guessed running sum has
+// at the sub-range's start, with each sub-range's sum, sum of magnitudes and
+// lowest set bit. A sub-range's sum is exact wherever the chain uses it (the
+// exact test bounds every partial sum of any order).
 __global__ void __launch_bounds__(256) k_renew_binade(int n, int piece_rows, int num_leaves,
                                                       const int32_t* __restrict__ leaf_of,
                                                       const float* __restrict__ g, const float* __restrict__ h,
                                                       RenewBinade* __restrict__ binade,
-                                                      const RenewWork* __restrict__ work, const int* work_count,
-                                                      int work_cap) {
+                                                      const RenewWork* __restrict__ work, RenewPart* __restrict__ parts,
+                                                      const int* work_count, int work_cap) {
   const int lane = (int)threadIdx.x & 63;
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   const int total = min(*work_count, work_cap);
+  const int part_rows = piece_rows / kRenewParts;  // a multiple of 256
   for (int w = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); w < total; w += waves) {
     const RenewWork it = work[w];
     const int piece = it.piece, leaf = it.leaf, comp = it.comp, eb = it.eb;
     const float* x = comp ? h : g;
-    const double inv_u = ldexp(1.0, 52 - eb);
+    const bool whole = eb != INT_MIN;
+    const double inv_u = whole ? ldexp(1.0, 52 - eb) : 0.0;
     const long long r0 = (long long)piece * piece_rows, r1 = min(r0 + (long long)piece_rows, (long long)n);
+    const bool aligned = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u;
     ParityMap acc{0, 0, 0, 1};
     bool bad = false;
-    const bool vec = (((uintptr_t)x | (uintptr_t)leaf_of) & 15u) == 0u;
-    for (long long b = r0; b < r1; b += 256) {
-      ParityMap m{0, 0, 0, 1};
-      if (vec && b + 256 <= r1) {  // lane i: rows 4i .. 4i + 3 of the 256, in order
-        const int4 lv = *(const int4*)(leaf_of + b + 4 * lane);
-        const float4 xv = *(const float4*)(x + b + 4 * lane);
-        m = parity_then(m, parity_of(lv.x == leaf, xv.x, inv_u, bad));
-        m = parity_then(m, parity_of(lv.y == leaf, xv.y, inv_u, bad));
-        m = parity_then(m, parity_of(lv.z == leaf, xv.z, inv_u, bad));
-        m = parity_then(m, parity_of(lv.w == leaf, xv.w, inv_u, bad));
-      } else {  // lane i: rows 4i .. 4i + 3, those below r1
-        for (int t = 0; t < 4; ++t) {
-          const long long r = b + 4 * lane + t;
-          const bool mine = r < r1 && leaf_of[r] == leaf;
-          m = parity_then(m, parity_of(mine, mine ? x[r] : 0.0f, inv_u, bad));
+    double s_run = it.s;  // the guessed running sum, for the sub-ranges' binades
+    for (int k = 0; k < kRenewParts; ++k) {
+      const long long c0 = r0 + (long long)k * part_rows, c1 = min(c0 + (long long)part_rows, r1);
+      const int ebk = binade_of(s_run);
+      const bool has_k = ebk != INT_MIN && ebk >= -1022 && ebk <= 900;
+      const double inv_uk = has_k ? ldexp(1.0, 52 - ebk) : 0.0;
+      ParityMap pk{0, 0, 0, 1};
+      bool bad_k = !has_k;
+      double t = 0.0, a = 0.0;
+      int e = INT_MAX;
+      for (long long b = c0; b < c1; b += 256) {
+        int lv[4];
+        float xv[4];
+        const long long r = b + 4 * lane;
+        if (aligned && b + 256 <= c1) {  // lane i: rows 4i .. 4i + 3 of the 256, in order
+          const int4 l4 = *(const int4*)(leaf_of + r);
+          const float4 x4 = *(const float4*)(x + r);
+          lv[0] = l4.x, lv[1] = l4.y, lv[2] = l4.z, lv[3] = l4.w;
+          xv[0] = x4.x, xv[1] = x4.y, xv[2] = x4.z, xv[3] = x4.w;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            lv[q] = r + q < c1 ? leaf_of[r + q] : -1;
+            xv[q] = r + q < c1 ? x[r + q] : 0.0f;
+          }
         }
+        ParityMap mw{0, 0, 0, 1}, mk{0, 0, 0, 1};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool mine = lv[q] == leaf;
+          const float v = mine ? xv[q] : 0.0f;
+          if (whole) mw = parity_then(mw, parity_of(mine, v, inv_u, bad));
+          if (has_k) mk = parity_then(mk, parity_of(mine, v, inv_uk, bad_k));
+          if (mine && v != 0.0f) {
+            t += (double)v;
+            a += fabs((double)v);
+            e = min(e, lowest_bit_f32(v));
+          }
+        }
+        if (whole) acc = parity_then(acc, parity_lane0(parity_reduce(mw)));
+        if (has_k) pk = parity_then(pk, parity_lane0(parity_reduce(mk)));
       }
-      acc = parity_then(acc, parity_lane0(parity_reduce(m)));
+      t = wave_sum_f64(t);
+      a = wave_sum_f64(a);
+      e = wave_min_i32(e);
+      bad_k = __ballot(bad_k) != 0ull;
+      if (lane == 0) parts[(size_t)w * kRenewParts + k] = RenewPart{t, a, pk.s0, pk.s1, e, bad_k ? INT_MIN : ebk};
+      s_run += t;
     }
     const bool any_bad = __ballot(bad) != 0ull;
     if (lane == 0) {
       RenewBinade& out = binade[((size_t)piece * num_leaves + leaf) * 2 + comp];
       out.s0 = acc.s0;
       out.s1 = acc.s1;
-      out.eb = any_bad ? INT_MIN : eb;
+      out.eb = whole && !any_bad ? eb : INT_MIN;
     }
   }
+}
+
+// s = k u (u = 2^(eb-52), k an integer below 2^53 in magnitude) plus a run inside
+// s's binade that adds s0 units when k is even, s1 when odd; the result stays in
+// the binade, so it is exact.
+__device__ __forceinline__ double binade_add(double s, int eb, long long s0, long long s1) {
+  const long long k = (long long)(s * ldexp(1.0, 52 - eb));
+  return (double)(k + ((k & 1) ? s1 : s0)) * ldexp(1.0, eb - 52);
 }
 
 // One block of two waves per leaf: wave 0 chains the gradients, wave 1 the
@@ -3345,7 +3400,8 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
                                                      const float* __restrict__ g, const float* __restrict__ h,
                                                      int num_leaves, const int* __restrict__ flags,
                                                      const RenewPiece* __restrict__ pieces,
-                                                     const RenewBinade* __restrict__ binade, double* out) {
+                                                     const RenewBinade* __restrict__ binade,
+                                                     const RenewPart* __restrict__ parts, double* out) {
   const int leaf = blockIdx.x, comp = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const int need = flags ? flags[leaf] : 3;
   if (!((need >> comp) & 1)) return;  // wave-uniform
@@ -3353,7 +3409,7 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
   double s = 0.0;
   for (int p0 = 0; p0 < num_pieces; p0 += 64) {
     double t = 0.0, a = 0.0;
-    int e = INT_MAX, beb = INT_MIN;
+    int e = INT_MAX, beb = INT_MIN, bw = -1;
     long long bs0 = 0, bs1 = 0;
     if (p0 + lane < num_pieces) {
       const size_t at = (size_t)(p0 + lane) * num_leaves + leaf;
@@ -3364,6 +3420,7 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
       if (binade) {
         const RenewBinade& bn = binade[at * 2 + comp];
         beb = bn.eb;
+        bw = bn.work;
         bs0 = bn.s0;
         bs1 = bn.s1;
       }
@@ -3379,16 +3436,35 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
       }
       const int ebj = __builtin_amdgcn_readlane(beb, j);
       if (ej != INT_MIN && piece_in_binade(s, aj, ebj, piece_rows)) {
-        // s = k u with k an integer below 2^53 in magnitude; the piece adds s0 or
-        // s1 units by the parity of k, and the result stays in the binade
-        const double u = ldexp(1.0, ebj - 52);
-        const long long k = (long long)(s * ldexp(1.0, 52 - ebj));
-        const long long add = (k & 1) ? lane_i64(bs1, j) : lane_i64(bs0, j);
-        s = (double)(k + add) * u;
+        s = binade_add(s, ebj, lane_i64(bs0, j), lane_i64(bs1, j));
         continue;
       }
-      const long long r0 = (long long)(p0 + j) * piece_rows;
-      s = chain_rows(s, r0, min(r0 + (long long)piece_rows, (long long)n), leaf, leaf_of, x);
+      const long long r0 = (long long)(p0 + j) * piece_rows, r1 = min(r0 + (long long)piece_rows, (long long)n);
+      const int wj = __builtin_amdgcn_readlane(bw, j);
+      if (wj < 0 || ej == INT_MIN) {
+        s = chain_rows(s, r0, r1, leaf, leaf_of, x);
+        continue;
+      }
+      // the same tests on the piece's sub-ranges (lane k < kRenewParts loads the k-th)
+      const int part_rows = piece_rows / kRenewParts;
+      RenewPart pt{0.0, 0.0, 0, 0, INT_MAX, INT_MIN};
+      if (lane < kRenewParts) pt = parts[(size_t)wj * kRenewParts + lane];
+      for (int k = 0; k < kRenewParts; ++k) {
+        const int ek = __builtin_amdgcn_readlane(pt.e, k);
+        if (ek == INT_MAX) continue;
+        const double ak = lane_f64(pt.a, k);
+        if (piece_exact(s, ak, ek, 1.0)) {
+          s += lane_f64(pt.t, k);
+          continue;
+        }
+        const int ebk = __builtin_amdgcn_readlane(pt.eb, k);
+        if (ek != INT_MIN && piece_in_binade(s, ak, ebk, part_rows)) {
+          s = binade_add(s, ebk, lane_i64(pt.s0, k), lane_i64(pt.s1, k));
+          continue;
+        }
+        const long long c0 = r0 + (long long)k * part_rows;
+        s = chain_rows(s, c0, min(c0 + (long long)part_rows, r1), leaf, leaf_of, x);
+      }
     }
   }
   if (lane == 0) out[2 * leaf + comp] = s;
@@ -4138,7 +4214,8 @@ void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int nu
   const size_t lds = renew_pieces_lds(num_leaves);
   if (pieces.pieces && lds <= (size_t)kLdsBytes && pieces.piece_rows > 0 &&
       (long long)pieces.num_pieces * pieces.piece_rows >= n) {
-    const bool summaries = pieces.binade && pieces.work && pieces.work_count && pieces.work_cap > 0;
+    const bool summaries = pieces.binade && pieces.work && pieces.parts && pieces.work_count && pieces.work_cap > 0 &&
+                           pieces.piece_rows % (256 * kRenewParts) == 0;
     allow_lds(k_renew_pieces);
     launch(k_renew_pieces, dim3((unsigned)std::max(pieces.num_pieces, 1)), dim3(256), lds, st, n, pieces.piece_rows,
            leaf_of, g, h, num_leaves, d_flags, pieces.pieces, summaries ? pieces.work_count : (int*)nullptr);
@@ -4149,12 +4226,14 @@ void launch_renew_sums(int n, const int32_t* leaf_of, const int* d_order, int nu
       BK_HIP(hipGetLastError());
       // the listed pieces, one wave each (a block's waves exit once the list is done)
       launch(k_renew_binade, dim3(1024), dim3(256), 0, st, n, pieces.piece_rows, num_leaves, leaf_of, g, h,
-             pieces.binade, (const RenewWork*)pieces.work, (const int*)pieces.work_count, pieces.work_cap);
+             pieces.binade, (const RenewWork*)pieces.work, pieces.parts, (const int*)pieces.work_count,
+             pieces.work_cap);
       BK_HIP(hipGetLastError());
     }
     launch(k_renew_chain, dim3((unsigned)num_leaves), dim3(128), 0, st, n, pieces.piece_rows, pieces.num_pieces,
            leaf_of, g, h, num_leaves, d_flags, (const RenewPiece*)pieces.pieces,
-           summaries ? (const RenewBinade*)pieces.binade : (const RenewBinade*)nullptr, d_sums);
+           summaries ? (const RenewBinade*)pieces.binade : (const RenewBinade*)nullptr,
+           (const RenewPart*)pieces.parts, d_sums);
     BK_HIP(hipGetLastError());
     return;
   }

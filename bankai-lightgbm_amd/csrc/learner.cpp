@@ -826,7 +826,7 @@ struct Learner {
                     (void*)d_sum_partials, (void*)d_segs, (void*)d_order, (void*)d_leaf_vals,
                     (void*)d_leaf_of, (void*)d_nodes, (void*)d_node_rules, (void*)d_renew_part_sums,
                     (void*)d_renew_part_meta, (void*)d_renew_flags, (void*)renew_pieces.pieces, (void*)renew_pieces.binade,
-                    (void*)renew_pieces.work, (void*)renew_pieces.work_count, (void*)d_bag,
+                    (void*)renew_pieces.work, (void*)renew_pieces.parts, (void*)renew_pieces.work_count, (void*)d_bag,
                     (void*)d_bag_bits, (void*)d_replay,
                     (void*)d_pstate, (void*)d_pnodes, (void*)d_pnflags, (void*)d_pfold, (void*)d_pfflags,
                     (void*)d_ppart, (void*)d_phist, (void*)d_pscan, (void*)d_pred, (void*)d_pused, (void*)d_bytree,
@@ -1095,11 +1095,13 @@ struct Learner {
         renew_pieces.num_pieces = (int)(((long long)N + renew_pieces.piece_rows - 1) / renew_pieces.piece_rows);
         const size_t entries = (size_t)std::max(renew_pieces.num_pieces, 1) * (size_t)P.num_leaves;
         renew_pieces.pieces = dmalloc<RenewPiece>(entries);
-        // binade summaries per piece, leaf and component; the list holds any number of
-        // pieces the guess may pick (every piece of every leaf, both components)
+        // binade summaries per piece, leaf and component; the list holds up to 65536 of
+        // the pieces the guess picks, with their sub-range summaries (pieces beyond it
+        // are added value by value)
         renew_pieces.binade = dmalloc<RenewBinade>(entries * 2);
-        renew_pieces.work_cap = (int)std::min<size_t>(entries * 2, (size_t)1 << 30);
+        renew_pieces.work_cap = (int)std::min<size_t>(entries * 2, (size_t)65536);
         renew_pieces.work = dmalloc<RenewWork>((size_t)renew_pieces.work_cap);
+        renew_pieces.parts = dmalloc<RenewPart>((size_t)renew_pieces.work_cap * kRenewParts);
         renew_pieces.work_count = dmalloc<int>(1);
       }
       if (renew_par_lds(P.num_leaves) <= (size_t)kLdsBytes) {
