@@ -3183,21 +3183,20 @@ __global__ void __launch_bounds__(128) k_renew_guess(int num_pieces, int num_lea
       a = comp ? pc.ah : pc.ag;
       e = comp ? pc.eh : pc.eg;
     }
-    int mine_mark = 0, mine_eb = INT_MIN;
-    double mine_s = 0.0;
-    const int cnt = min(64, num_pieces - p0);
-    for (int j = 0; j < cnt; ++j) {
-      const int ej = __builtin_amdgcn_readlane(e, j);
-      if (ej == INT_MAX || ej == INT_MIN) continue;
-      const double tj = lane_f64(t, j), aj = lane_f64(a, j);
-      // a margin of 4 on the test: the guess drifts from the chain's sum by roundings
-      if (!piece_exact(s, aj, ej, 4.0) && lane == j) {
-        mine_mark = 1;
-        mine_eb = binade_of(s);
-        mine_s = s;
-      }
-      s += tj;
+    // the running sum at each piece's start: the batch's start plus an exclusive
+    // prefix of the pieces' sums (any order: it is a guess)
+    const bool valid = e != INT_MAX && e != INT_MIN;
+    const double tv = valid ? t : 0.0;
+    double incl = tv;
+    for (int o = 1; o < 64; o <<= 1) {
+      const double up = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += up;
     }
+    const double mine_s = s + (incl - tv);
+    // a margin of 4 on the test: the guess drifts from the chain's sum by roundings
+    const int mine_mark = valid && !piece_exact(mine_s, a, e, 4.0) ? 1 : 0;
+    const int mine_eb = mine_mark ? binade_of(mine_s) : INT_MIN;
+    s += __shfl(incl, 63, 64);
     if (p0 + lane < num_pieces) {
       int eb = INT_MIN, wi = -1;
       if (mine_mark) {
@@ -3423,6 +3422,21 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
         bw = bn.work;
         bs0 = bn.s0;
         bs1 = bn.s1;
+      }
+    }
+    // the whole batch at once: every partial sum of the batch's values, started
+    // from s, is exact when |s| + sum|x| < 2^(53 + E) over all of them (E the lowest
+    // set bit of s and of every value); the pieces' sums then add exactly in any order
+    {
+      const int eb_all = wave_min_i32(e);
+      const bool any_bad = __ballot(e == INT_MIN) != 0ull;
+      if (!any_bad) {
+        if (eb_all == INT_MAX) continue;  // no nonzero value of the leaf in the batch
+        const double a_all = wave_sum_f64(e == INT_MAX ? 0.0 : a);
+        if (piece_exact(s, a_all, eb_all, 1.0)) {
+          s += wave_sum_f64(e == INT_MAX ? 0.0 : t);
+          continue;
+        }
       }
     }
     const int cnt = min(64, num_pieces - p0);
