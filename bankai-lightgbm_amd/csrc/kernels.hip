@@ -3437,6 +3437,18 @@ __global__ void __launch_bounds__(128) k_renew_chain(int n, int piece_rows, int 
           s += wave_sum_f64(e == INT_MAX ? 0.0 : t);
           continue;
         }
+        // or the whole batch inside s's binade, every piece summarized for it: the
+        // pieces' 2-state maps compose in row order into the batch's
+        const int ebs = binade_of(s);
+        const bool summarized = e == INT_MAX || beb == ebs;
+        if (ebs != INT_MIN && __ballot(!summarized) == 0ull &&
+            piece_in_binade(s, a_all, ebs, piece_rows * min(64, num_pieces - p0))) {
+          const ParityMap m = e == INT_MAX ? ParityMap{0, 0, 0, 1}
+                                           : ParityMap{bs0, bs1, (int)(bs0 & 1), (int)((bs1 + 1) & 1)};
+          const ParityMap all = parity_lane0(parity_reduce(m));
+          s = binade_add(s, ebs, all.s0, all.s1);
+          continue;
+        }
       }
     }
     const int cnt = min(64, num_pieces - p0);

@@ -333,20 +333,23 @@ class ConfidentObjective:
         return (p - y).astype(np.float32), (p * (1.0 - p)).astype(np.float32)
 
 
+@pytest.mark.parametrize("leaves,min_data", [(127, 20), (7, 20000)])
 @pytest.mark.parametrize("bagged", [0, 1])
 @pytest.mark.parametrize("batched", [0, 1])
-def test_renew_piecewise_chain_bit_exact(batched, bagged):
+def test_renew_piecewise_chain_bit_exact(batched, bagged, leaves, min_data):
     """300,000 rows (74 pieces of 4096 rows), 127 leaves, gradients of a confident
     model: the renewal's chain adds a piece as one exact sum where that is exact
     from its running sum, by its binade summary where the running sum stays in one
     binade, and value by value otherwise (on the first unbagged tree: 8156, 1189 and
     801 pieces of the leaves whose whole-leaf certificate fails); every leaf value
-    equals the oracle's sequential row-order sums, with and without a bag."""
-    announce(f"renewal piecewise chain 300k x 28, batched={batched}, bagged={bagged}")
+    equals the oracle's sequential row-order sums, with and without a bag. With 7
+    large leaves, whole batches of 64 pieces stay in the running sum's binade (4 of
+    the first unbagged tree's 28 batches of failing leaves), whose summaries compose."""
+    announce(f"renewal piecewise chain 300k x 28, {leaves} leaves, batched={batched}, bagged={bagged}")
     n = 300_000
     x, _ = synth.binary_28(n, seed=12)
     ds = bg.Dataset(x)
-    p = bg.LearnerParams(num_leaves=127, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+    p = bg.LearnerParams(num_leaves=leaves, min_data_in_leaf=min_data, use_quantized_grad=1, num_grad_quant_bins=4,
                          stochastic_rounding=1, quant_train_renew_leaf=1, seed=6, form=bg.FORM_QUANTIZED,
                          batched=batched)
     sampler = bst.RowSampler({"bagging_fraction": 0.7, "bagging_freq": 1, "bagging_seed": 4},
