@@ -251,3 +251,71 @@ def test_fullsize_lambdarank_trees_5Mx136(ranking_5m, form):
         assert np.array_equal(s2, score)
     gpu.close()
     ora.close()
+
+
+def test_fullsize_late_renewal_equals_sequential_sums():
+    """The workload's tree 301, grown after 300 device iterations (gradients, trees and
+    score updates on the device): most gradients are tiny by then, so the whole-leaf
+    renewal certificate fails for many leaves and the piecewise chain runs (exact
+    pieces, binade summaries, sub-ranges, value by value). Every leaf value must equal
+    calculate_splitted_leaf_output of the leaf's gradient and hessian sums taken one
+    f64 addition at a time in ascending row order (serial.rs:634-655), computed here
+    in numpy from the leaf's rows."""
+    print("\n[case] full size 10.5M x 28 quantized batched, tree 301 renewal vs sequential sums", flush=True)
+    import torch
+
+    n = 10_500_000
+    x, y = synth.binary_28(n, seed=0, intercept=synth.solve_intercept(0))
+    ds = bg.Dataset(x, bg.DatasetParams(num_threads=16))
+    del x
+    p = bg.LearnerParams(num_leaves=255, min_data_in_leaf=20, use_quantized_grad=1, num_grad_quant_bins=4,
+                         stochastic_rounding=1, quant_train_renew_leaf=1, seed=0, num_threads=16,
+                         form=bg.FORM_QUANTIZED, batched=1)
+    obj = bst.BinaryObjective(y)
+    gpu = bg.Learner(ds, p)
+    dev = torch.device("cuda:0")
+    score = torch.full((n,), obj.boost_from_score(0), dtype=torch.float64, device=dev)
+    label = torch.as_tensor(y.astype(np.float32), device=dev)
+    g = torch.empty(n, dtype=torch.float32, device=dev)
+    h = torch.empty(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def gradients():
+        gpu.binary_gradients_device(score.data_ptr(), label.data_ptr(), obj.sigmoid, g.data_ptr(), h.data_ptr(),
+                                    label_weights=obj.label_weights)
+
+    gradients()
+    for it in range(300):
+        t = gpu.train_device(g.data_ptr(), h.data_ptr())
+        gpu.add_prediction_to_score_device(bst._round0(t["leaf_value"] * 0.1), score.data_ptr())
+        gradients()
+    torch.cuda.synchronize()
+    gh, hh = g.cpu().numpy(), h.cpu().numpy()
+    t = gpu.train_device(g.data_ptr(), h.data_ptr())
+    torch.cuda.synchronize()
+    assert t["num_leaves"] == 255
+
+    def lsb(v):
+        b = v.view(np.uint32) & 0x7FFFFFFF
+        m = (b & 0x7FFFFF).astype(np.int64)
+        e = (b >> 23).astype(np.int64)
+        m = np.where(e > 0, m | 0x800000, m)
+        low = m & -m
+        return np.where(e > 0, e - 150, -149) + np.log2(np.maximum(low, 1)).astype(np.int64)
+
+    uncertified = 0
+    for leaf in range(t["num_leaves"]):
+        rows = gpu.leaf_rows(leaf)
+        assert np.all(np.diff(rows) > 0) and len(rows) == t["leaf_count"][leaf]
+        vg, vh = gh[rows].astype(np.float64), hh[rows].astype(np.float64)
+        sg = float(np.add.accumulate(vg)[-1])  # one addition at a time, in row order
+        sh = float(np.add.accumulate(vh)[-1])
+        ret = -sg / sh
+        want = 0.0 if -bst.K_ZERO_THRESHOLD <= ret <= bst.K_ZERO_THRESHOLD else ret
+        assert t["leaf_value"][leaf] == want, leaf
+        for v in (gh[rows], hh[rows]):
+            nz = v[v != 0]
+            if len(nz) and not len(v) * float(np.max(np.abs(nz))) < 2.0 ** (52 + int(lsb(nz).min())):
+                uncertified += 1
+    # the whole-leaf certificate fails for many sums: the piecewise chain decided them
+    assert uncertified > 20, uncertified
