@@ -394,10 +394,14 @@ void launch_hist(const DevState& s, const HistItem* d_items, int nitems, int tot
                  int rows_per_block, const int* d_block_slot, const int* d_group_used, int max_tile_bins,
                  const int* d_tile_first, int ntiles, int rows_per_flush, void* stream,
                  const RoundHdr* hdr = nullptr, int nsub = 1);
-// Rows per partition tile for a record stride (2048 16-byte chunks per tile).
+// Rows per partition tile for a record stride (8192 16-byte chunks per tile),
+// and per short tile (2048 chunks), which rounds of few rows take to spread over
+// more blocks.
 int partition_tile_rows(int stride);
+int partition_tile_rows_small(int stride);
 // Single-pass partition of every item (row order kept within a tile, not across
-// tiles): tasks are the items' tiles in order (chunk == partition_tile_rows).
+// tiles): tasks are the items' tiles in order (chunk == partition_tile_rows, or
+// partition_tile_rows_small with small_tiles).
 // d_cursors: 2 banks x max_items x (left, right) u32, zeroed before the first
 // launch; a launch uses bank (epoch & 1) and zeroes the other, so consecutive
 // launches on one stream alternate epoch parity.
@@ -406,7 +410,7 @@ int partition_tile_rows(int stride);
 // launch_children would (left_total < 0 items, no hdr, nitems <= 512).
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
                       int max_items, unsigned epoch, void* stream, const RoundHdr* hdr = nullptr,
-                      const DeriveArgs* derive = nullptr);
+                      const DeriveArgs* derive = nullptr, bool small_tiles = false);
 // Whether the partition can build histograms (quantized 32-bit bins): the
 // record's chunks divide the block's threads evenly and every bin of the record
 // fits in LDS beside the partition's own tables.

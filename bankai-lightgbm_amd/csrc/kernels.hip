@@ -19,6 +19,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <unordered_set>
+
 #include <algorithm>
 #include <climits>
 #include <mutex>
@@ -599,6 +602,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(DevState s, const HistIte
 // ---------------------------------------------------------------------------
 constexpr int kPartThreads = 512;
 constexpr int kPartChunks = 16;  // 16-byte chunks per thread per tile
+constexpr int kPartChunksSmall = 4;  // the same for small rounds: shorter tiles, more blocks per CU
 constexpr int kPartSearchItems = 1024;  // items searched in LDS (more: bisection in global memory)
 static_assert(kPartDeriveMax == kPartThreads, "the partition's last block derives one split per thread");
 
@@ -644,7 +648,7 @@ __device__ __forceinline__ void partition_done(const PartItem* __restrict__ item
                                 da.hitems, da.sitems, da.ritems, da.hdr, da.counts);
 }
 
-template <int FORM, bool HIST>
+template <int FORM, bool HIST, int CH = kPartChunks>
 __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const PartItem* __restrict__ items,
                                                             int nitems, unsigned* __restrict__ cursors,
                                                             int max_items, unsigned epoch,
@@ -655,7 +659,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   extern __shared__ __align__(16) unsigned char smem[];
   using PT = typename Pay<FORM>::T;
   const int stride = s.stride, nc = stride >> 4;
-  const int R = kPartThreads * kPartChunks / nc;  // rows per tile
+  const int R = kPartThreads * CH / nc;  // rows per tile
   const int rpt = (R + kPartThreads - 1) / kPartThreads;
   uint16_t* dstp = (uint16_t*)smem;  // R <= 8192
   uint8_t* lflag = (uint8_t*)(dstp + R);
@@ -709,16 +713,16 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   const int side = HIST ? it.hist_side : -1;
   if (HIST && side >= 0)
     for (int i = threadIdx.x; i < s.total_bins; i += kPartThreads) hist[i] = 0u;
-  uint4 v[kPartChunks];
+  uint4 v[CH];
 #pragma unroll
-  for (int k = 0; k < kPartChunks; ++k) {
+  for (int k = 0; k < CH; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) v[k] = load_stream(src + 16 * (size_t)q);
   }
   if (seg.buf == 0)
     for (int r = threadIdx.x; r < rows; r += kPartThreads) payt[r] = ((const PT*)s.gh0)[t0 + r];
 #pragma unroll
-  for (int k = 0; k < kPartChunks; ++k) {
+  for (int k = 0; k < CH; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) {
       const int r = q / nc, c = q - r * nc;
@@ -772,7 +776,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
   // a tile land in two contiguous runs, so the lanes of a wave cover contiguous
   // 16-byte pieces of them
 #pragma unroll
-  for (int k = 0; k < kPartChunks; ++k) {
+  for (int k = 0; k < CH; ++k) {
     const int q = (int)threadIdx.x + k * kPartThreads;
     if (q < total) {
       const int r = q / nc, c = q - r * nc;
@@ -797,7 +801,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(DevState s, const Pa
     }
     const uint8_t want = side == 0 ? 1 : 0;
 #pragma unroll
-    for (int k = 0; k < kPartChunks; ++k) {
+    for (int k = 0; k < CH; ++k) {
       const int q = (int)threadIdx.x + k * kPartThreads;
       if (q < total) {
         const int r = q / nc;
@@ -3808,13 +3812,17 @@ inline int grid_for(long long n, int threads, int cap = 4096) {
 }
 
 
+// Raises the dynamic LDS limit of a kernel to kLdsBytes, once per kernel (kernels
+// of one signature, such as the forms of a template, are told apart by address);
+// learners on several host threads may call it at once.
 template <typename K>
 void allow_lds(K kernel) {
-  static bool done = false;
-  if (!done) {
-    BK_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-    done = true;
-  }
+  static std::mutex mu;
+  static std::unordered_set<const void*> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count((const void*)kernel)) return;
+  BK_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
+  done.insert((const void*)kernel);
 }
 
 }  // namespace
@@ -4094,9 +4102,11 @@ void launch_children(const PartItem* d_pitems, int nitems, const unsigned* d_cur
 }
 
 int partition_tile_rows(int stride) { return kPartThreads * kPartChunks / (stride >> 4); }
+int partition_tile_rows_small(int stride) { return kPartThreads * kPartChunksSmall / (stride >> 4); }
 
 void launch_partition(const DevState& s, const PartItem* d_items, int nitems, int ntasks, unsigned* d_cursors,
-                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr, const DeriveArgs* derive) {
+                      int max_items, unsigned epoch, void* stream, const RoundHdr* hdr, const DeriveArgs* derive,
+                      bool small_tiles) {
   if (ntasks <= 0) return;
   if (nitems > max_items) throw std::runtime_error("more partition items than cursors");
   DeriveArgs da{};
@@ -4104,23 +4114,29 @@ void launch_partition(const DevState& s, const PartItem* d_items, int nitems, in
     if (hdr || nitems > kPartDeriveMax) throw std::runtime_error("the partition derives at most 512 splits, unplanned");
     da = *derive;
   }
-  const int R = partition_tile_rows(s.stride);
+  const int R = small_tiles ? partition_tile_rows_small(s.stride) : partition_tile_rows(s.stride);
   if (R < 1) throw std::runtime_error("row record too large for the partition tile");
   const size_t lds = partition_pay_lds(R, s.stride, s.form == FORM_QUANT ? 2 : 8);
   if (lds > (size_t)kLdsBytes) throw std::runtime_error("partition tile exceeds the LDS budget");
   hipStream_t st = (hipStream_t)stream;
   const int* no_groups = nullptr;
-  switch (s.form) {
-    case FORM_QUANT:
-      allow_lds(k_partition<FORM_QUANT, false>);
-      launch(k_partition<FORM_QUANT, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr, no_groups, da);
-      break;
-    default:
-      allow_lds(k_partition<FORM_F64, false>);
-      launch(k_partition<FORM_F64, false>, dim3((unsigned)ntasks), dim3(kPartThreads), lds, st, s, d_items, nitems,
-             d_cursors, max_items, epoch, hdr, no_groups, da);
-      break;
+  const dim3 grid((unsigned)ntasks), block(kPartThreads);
+  if (s.form == FORM_QUANT && small_tiles) {
+    allow_lds(k_partition<FORM_QUANT, false, kPartChunksSmall>);
+    launch(k_partition<FORM_QUANT, false, kPartChunksSmall>, grid, block, lds, st, s, d_items, nitems, d_cursors,
+           max_items, epoch, hdr, no_groups, da);
+  } else if (s.form == FORM_QUANT) {
+    allow_lds(k_partition<FORM_QUANT, false>);
+    launch(k_partition<FORM_QUANT, false>, grid, block, lds, st, s, d_items, nitems, d_cursors, max_items, epoch, hdr,
+           no_groups, da);
+  } else if (small_tiles) {
+    allow_lds(k_partition<FORM_F64, false, kPartChunksSmall>);
+    launch(k_partition<FORM_F64, false, kPartChunksSmall>, grid, block, lds, st, s, d_items, nitems, d_cursors,
+           max_items, epoch, hdr, no_groups, da);
+  } else {
+    allow_lds(k_partition<FORM_F64, false>);
+    launch(k_partition<FORM_F64, false>, grid, block, lds, st, s, d_items, nitems, d_cursors, max_items, epoch, hdr,
+           no_groups, da);
   }
   BK_HIP(hipGetLastError());
 }

@@ -528,6 +528,11 @@ struct FeatureSampler {
 // its splittable flag kept), and whether it may be the smaller / larger leaf's best
 // (the by-node samples).
 constexpr int kUseSearch = 1, kUseSmall = 2, kUseLarge = 4, kUseAll = 7;
+// Rounds partitioning at most this many rows use the short partition tiles: in
+// long tiles such a round is at most 128 blocks for 256 CUs (measured over the
+// 500-iteration job at 10.5M rows: 5% less time than long tiles throughout,
+// against 3% with the bound at 2M rows and 4.5% at 256K).
+constexpr long long kSmallTileRoundRows = 1 << 19;
 
 // A leaf of the growing tree in the batched form: its rows, sums, best split,
 // and (after a speculative expansion) its two evaluated children.
@@ -565,6 +570,7 @@ struct Learner {
   size_t num_slots = 0;   // device slots (batched form: one per evaluated leaf)
   int max_items = 0, max_partials = 0, max_part_tiles = 0;
   int part_tile = 0;                   // rows per partition tile
+  int part_tile_small = 0;             // rows per short partition tile
   unsigned epoch = 0;  // partition launches so far: the parity picks the cursor bank
 
   // device memory
@@ -964,6 +970,7 @@ struct Learner {
     hist32_ok = form == FORM_QUANT && batched && rows_per_flush32 >= 2 * kTileRows && max_items <= kChildThreads;
     if (hist32_ok) rows_per_flush = std::min(rows_per_flush, rows_per_flush32);  // sizes the partials for both
     part_tile = partition_tile_rows(stride);
+    part_tile_small = partition_tile_rows_small(stride);
     max_part_tiles = N / part_tile + max_items + 8;
     // a build has at most one row block per 2048 rows (the tuner's smallest row
     // count), and at least one per rows_per_flush rows: a partial slot never holds
@@ -2084,7 +2091,7 @@ struct Learner {
   // parent's rows); the device plans at most that.
   template <typename Keep>
   void run_derived_round(int ni, int nt, double part_rows, long long hist_bound, const std::vector<int>& order,
-                         bool host_fold, bool part_hist, Keep&& keep) {
+                         bool host_fold, bool part_hist, bool small_tiles, Keep&& keep) {
     const auto l0 = std::chrono::steady_clock::now();
     begin_uploads();
     d_pitems = put(h_pitems, (size_t)ni);
@@ -2112,7 +2119,7 @@ struct Learner {
       if (part_hist)
         launch_partition_hist(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, d_group_used, stream, &da);
       else
-        launch_partition(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, stream, nullptr, &da);
+        launch_partition(ds, d_pitems, ni, nt, d_cursors, max_items, epoch, stream, nullptr, &da, small_tiles);
     });
     stats.part_rows += (int64_t)part_rows;
     if (!da.on)
@@ -2305,6 +2312,13 @@ struct Learner {
       double part_rows = 0.0, hist_rows = 0.0;
       long long hist_bound = 0;  // 32-bit bins: the most rows the round's histograms take
       hist_seg.clear();
+      // a derived round of few rows partitions in short tiles, so that its
+      // blocks still cover the CUs
+      long long round_rows = 0;
+      for (int l : order) round_rows += nodes[(size_t)leaf_node[(size_t)l]].seg.count;
+      const bool small_tiles = derive && !part_hist && round_rows <= kSmallTileRoundRows &&
+                               round_rows / part_tile_small + (long long)order.size() + 1 <= max_part_tiles;
+      const int tile = small_tiles ? part_tile_small : part_tile;
       for (int l : order) {
         const int x = leaf_node[(size_t)l];
         Node& X = nodes[(size_t)x];
@@ -2329,7 +2343,7 @@ struct Learner {
           pi.hist_side = -1;
           if (part_hist && !(P.max_depth > 0 && X.depth + 1 >= P.max_depth))
             pi.hist_side = info.o.left_count < info.o.right_count ? 0 : 1;
-          pi.chunk = part_tile;
+          pi.chunk = tile;
           pi.rule = rule_of(info);
           pi.first_task = add_tasks(X.seg.count, pi.chunk, nt);
           pi.ntasks = nt - pi.first_task;
@@ -2413,7 +2427,7 @@ struct Learner {
       if (ni == 0) break;
       stats.plan_ms += since_ms(p0);
       if (derive) {
-        run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, part_hist, keep);
+        run_derived_round(ni, nt, part_rows, hist_bound, order, host_fold, part_hist, small_tiles, keep);
         continue;
       }
       HistPlan hp;
